@@ -1,0 +1,262 @@
+"""Python host mirror of the rotation path of FastTransforms on MI355X.
+
+Thin ctypes bindings over ``libfasttransforms_amd.so`` (the C ABI declared in
+``include/fasttransforms_amd.h``).  The names follow the reference's C API
+(``ft_plan_rotsphere`` -> ``plan_rotsphere``, ``ft_execute_sph_hi2lo`` ->
+``execute_sph_hi2lo`` ...), with the same argument meaning and the same
+column-major layout: an n x M matrix is a float64 numpy array of shape (n, M)
+in Fortran order, or a torch CUDA tensor whose memory is column-major (for
+example ``torch.empty(M, n).T``).
+
+Every computation runs in the HIP kernels.  If the shared library is not
+built, importing this package raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+SPH, TRI, DISK = 0, 1, 2
+HI2LO, LO2HI = 0, 1
+FP64, FP32 = 0, 1
+_KIND = {"sph": SPH, "tri": TRI, "disk": DISK}
+_DIR = {"hi2lo": HI2LO, "lo2hi": LO2HI}
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_here, "libfasttransforms_amd.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError("fasttransforms_amd: %s is not built (run python fasttransforms_amd/build.py)" % LIB_PATH)
+_lib = ctypes.CDLL(LIB_PATH)
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+# Exported symbols of include/fasttransforms_amd.h, with their ctypes signatures.
+SIGNATURES = {
+    "ft_set_num_threads": (None, [_i]),
+    "ft_destroy_rotation_plan": (None, [_vp]),
+    "ft_plan_rotsphere": (_vp, [_i]),
+    "ft_plan_rottriangle": (_vp, [_i, _d, _d, _d]),
+    "ft_plan_rotdisk": (_vp, [_i, _d, _d]),
+    "ft_gpu_last_error": (_i, []),
+    "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
+    "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
+    "ft_gpu_execute_batch": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp]),
+    "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
+}
+for _k in ("sph", "tri", "disk"):
+    for _dn in ("hi2lo", "lo2hi"):
+        SIGNATURES["ft_kernel_%s_%s" % (_k, _dn)] = (None, [_vp, _i, _i, _vp, _i])
+        SIGNATURES["ft_execute_%s_%s" % (_k, _dn)] = (None, [_vp, _vp, _vp, _i])
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+class _PlanHead(ctypes.Structure):
+    # The reference's leading members {double *s; double *c; int n;}.
+    _fields_ = [("s", ctypes.POINTER(ctypes.c_double)), ("c", ctypes.POINTER(ctypes.c_double)), ("n", ctypes.c_int)]
+
+
+class GPUError(RuntimeError):
+    pass
+
+
+def last_error():
+    return _lib.ft_gpu_last_error()
+
+
+def _check():
+    e = _lib.ft_gpu_last_error()
+    if e:
+        raise GPUError("fasttransforms_amd: %s (code %d)" % (_lib.ft_gpu_error_string(e).decode(), e))
+
+
+class RotationPlan:
+    """Owner of an ``ft_rotation_plan *`` (destroyed with the object)."""
+
+    def __init__(self, ptr, kind, n, params):
+        if not ptr:
+            _check()
+            raise GPUError("plan construction failed")
+        self._ptr = ptr
+        self.kind = kind
+        self.n = n
+        self.params = params
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def _head(self):
+        return ctypes.cast(self._ptr, ctypes.POINTER(_PlanHead)).contents
+
+    @property
+    def size(self):
+        n = self.n
+        return n * n if self.kind == DISK else n * (n + 1) // 2
+
+    @property
+    def s(self):
+        """Host FP64 sine table (a copy)."""
+        return np.ctypeslib.as_array(self._head().s, shape=(self.size,)).copy()
+
+    @property
+    def c(self):
+        """Host FP64 cosine table (a copy)."""
+        return np.ctypeslib.as_array(self._head().c, shape=(self.size,)).copy()
+
+    def __del__(self):
+        ptr, self._ptr = getattr(self, "_ptr", None), None
+        if ptr:
+            _lib.ft_destroy_rotation_plan(ptr)
+
+
+def plan_rotsphere(n):
+    return RotationPlan(_lib.ft_plan_rotsphere(int(n)), SPH, int(n), ())
+
+
+def plan_rottriangle(n, alpha, beta, gamma):
+    return RotationPlan(_lib.ft_plan_rottriangle(int(n), float(alpha), float(beta), float(gamma)), TRI, int(n),
+                        (float(alpha), float(beta), float(gamma)))
+
+
+def plan_rotdisk(n, alpha, beta):
+    return RotationPlan(_lib.ft_plan_rotdisk(int(n), float(alpha), float(beta)), DISK, int(n),
+                        (float(alpha), float(beta)))
+
+
+def set_num_threads(n):
+    _lib.ft_set_num_threads(int(n))
+
+
+def set_tuning(L=0, W=0, wg_per_cu=0):
+    rc = _lib.ft_gpu_set_tuning(int(L), int(W), int(wg_per_cu))
+    if rc:
+        raise ValueError("unsupported tuning (L=%r, W=%r, wg_per_cu=%r)" % (L, W, wg_per_cu))
+
+
+def _is_torch(A):
+    return type(A).__module__.startswith("torch")
+
+
+def _pointer(A, n, M, dtype=np.float64):
+    """Address of a column-major n x M buffer (numpy Fortran array or torch tensor)."""
+    if _is_torch(A):
+        import torch
+        want = torch.float64 if dtype == np.float64 else torch.float32
+        if A.dtype != want:
+            raise TypeError("expected a %s tensor" % want)
+        if A.numel() < n * M:
+            raise ValueError("tensor has fewer than n*M elements")
+        if A.dim() == 2:
+            if tuple(A.shape) == (n, M) and A.stride() != (1, n):
+                raise ValueError("an (n, M) tensor must be column-major (stride (1, n))")
+            if tuple(A.shape) == (M, n) and not A.is_contiguous():
+                raise ValueError("an (M, n) tensor must be contiguous")
+        elif not A.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return _vp(A.data_ptr())
+    if not isinstance(A, np.ndarray) or A.dtype != dtype:
+        raise TypeError("expected a %s numpy array" % np.dtype(dtype).name)
+    if A.ndim == 2:
+        if A.shape[0] != n or not A.flags.f_contiguous:
+            raise ValueError("expected a Fortran-ordered (n, M) array")
+    elif not A.flags.c_contiguous:
+        raise ValueError("expected a contiguous buffer")
+    if A.size < n * M:
+        raise ValueError("array has fewer than n*M elements")
+    return A.ctypes.data_as(_vp)
+
+
+def _M_of(A, n, M):
+    if M is not None:
+        return int(M)
+    if A.ndim == 2:
+        return int(A.shape[1]) if int(A.shape[0]) == n else int(A.shape[0])
+    return A.size // n if not _is_torch(A) else A.numel() // n
+
+
+def _execute(kind, direction):
+    name = "ft_execute_%s_%s" % (kind, direction)
+    fn = getattr(_lib, name)
+
+    def run(plan, A, M=None):
+        if plan.kind != _KIND[kind]:
+            raise ValueError("plan is not a %s plan" % kind)
+        M = _M_of(A, plan.n, M)
+        fn(plan.ptr, _pointer(A, plan.n, M), None, M)
+        _check()
+        return A
+
+    run.__name__ = name[3:]
+    run.__doc__ = "%s(plan, A, M=None): in place on the column-major n x M matrix A." % name
+    return run
+
+
+def _kernel(kind, direction):
+    name = "ft_kernel_%s_%s" % (kind, direction)
+    fn = getattr(_lib, name)
+
+    def run(plan, m1, m2, A, S=1):
+        if plan.kind != _KIND[kind]:
+            raise ValueError("plan is not a %s plan" % kind)
+        span = (plan.n - 1) * int(S) + 1
+        if (A.numel() if _is_torch(A) else A.size) < span:
+            raise ValueError("buffer shorter than (n-1)*S+1")
+        fn(plan.ptr, int(m1), int(m2), _pointer(A.reshape(-1), span, 1), int(S))
+        _check()
+        return A
+
+    run.__name__ = name[3:]
+    return run
+
+
+execute_sph_hi2lo = _execute("sph", "hi2lo")
+execute_sph_lo2hi = _execute("sph", "lo2hi")
+execute_tri_hi2lo = _execute("tri", "hi2lo")
+execute_tri_lo2hi = _execute("tri", "lo2hi")
+execute_disk_hi2lo = _execute("disk", "hi2lo")
+execute_disk_lo2hi = _execute("disk", "lo2hi")
+kernel_sph_hi2lo = _kernel("sph", "hi2lo")
+kernel_sph_lo2hi = _kernel("sph", "lo2hi")
+kernel_tri_hi2lo = _kernel("tri", "hi2lo")
+kernel_tri_lo2hi = _kernel("tri", "lo2hi")
+kernel_disk_hi2lo = _kernel("disk", "hi2lo")
+kernel_disk_lo2hi = _kernel("disk", "lo2hi")
+
+
+def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None):
+    """Asynchronous device-resident execution (ft_gpu_execute) on a torch tensor.
+
+    ``stream`` is a torch.cuda.Stream (default: the current stream).
+    """
+    import torch
+    direction = _DIR.get(direction, direction)
+    M = _M_of(A, plan.n, M)
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    ptr = _pointer(A, plan.n, M, np.float64 if precision == FP64 else np.float32)
+    rc = _lib.ft_gpu_execute(plan.ptr, plan.kind, int(direction), int(precision), ptr, M, _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
+
+
+def valid_rows(kind, n, j):
+    """Valid rows of column j in each layout (src/ftutilities.c:67-113)."""
+    kind = _KIND.get(kind, kind)
+    if kind == SPH:
+        return n - (j + 1) // 2
+    if kind == TRI:
+        return n - j
+    return n - (j + 3) // 4
+
+
+def full_M(kind, n):
+    """Columns of the full n-row layout: 2n-1 (sph), n (tri), 4n-3 (disk)."""
+    kind = _KIND.get(kind, kind)
+    return {SPH: 2 * n - 1, TRI: n, DISK: 4 * n - 3}[kind]

@@ -1,0 +1,71 @@
+"""Build libfasttransforms_amd.so in-tree for gfx950.
+
+    python fasttransforms_amd/build.py
+
+(run as a script: importing the package itself needs the built library).
+
+The C host layer is compiled with gcc, the HIP kernels with hipcc (one
+translation unit per geometry, in parallel), and everything is linked into
+``fasttransforms_amd/libfasttransforms_amd.so`` next to this file.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libfasttransforms_amd.so")
+
+ARCH = os.environ.get("FTAMD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+
+C_SOURCES = ["ft_plans.c", "ft_drivers.c"]
+HIP_SOURCES = ["ft_rotations.hip", "ft_k_sph.hip", "ft_k_tri.hip", "ft_k_disk.hip"]
+
+INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wextra"]
+# Packed FP32 VALU forms are disabled on purpose (see the project notes on
+# gfx950); the host pass warns that it does not know the feature.
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError("build step failed:\n%s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    objs = []
+    for src in C_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        objs.append(obj)
+        jobs.append([CC] + C_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        objs.append(obj)
+        jobs.append([HIPCC] + HIP_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, src), "-o", obj])
+    workers = min(len(jobs), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        outs = list(ex.map(_run, jobs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, LIB)
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

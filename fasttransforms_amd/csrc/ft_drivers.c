@@ -1,0 +1,566 @@
+/*
+ * ft_drivers.c -- the reference's executors and single-vector kernels on the
+ * GPU, plus the schedules the pipelined kernel runs.
+ *
+ * src/drivers.c:58-92, 278-312, 378-412 define ft_execute_{sph,tri,disk}_
+ * {hi2lo,lo2hi}: every column j >= 1 of order m (m = (j+1)/2 for sph/disk,
+ * m = j for tri) is taken from order m down to m%2 (sph/disk) or 0 (tri), or
+ * back up.  Here those columns are grouped 64 at a time (one wavefront, one
+ * column per lane) by what sequences they need, and each group's chain of
+ * sequences is cut into pipeline stages and workgroup-sized tasks.  The
+ * reference's warp/permute passes (drivers.c:114-115, 125-126) have no
+ * counterpart: the grouping is only a lane-to-column map.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "ft_internal.h"
+
+/* ------------------------------------------------------------ errors -- */
+
+static __thread int t_last_error = 0;
+
+void ftamd_set_error(int code) {
+    if (code != FTAMD_OK && t_last_error == FTAMD_OK) t_last_error = code;
+}
+
+int ft_gpu_last_error(void) {
+    int e = t_last_error;
+    t_last_error = FTAMD_OK;
+    return e;
+}
+
+const char *ft_gpu_error_string(int code) {
+    switch (code) {
+        case FT_GPU_OK: return "no error";
+        case FT_GPU_EINVAL: return "invalid argument";
+        case FT_GPU_ENOMEM: return "device allocation failed";
+        case FT_GPU_EDEVICE: return "HIP device or runtime error";
+        case FT_GPU_ETIMEOUT: return "kernel wait expired; results are invalid";
+        case FT_GPU_EPTR: return "pointer is not host memory or device memory of the current device";
+        default: return "unknown error";
+    }
+}
+
+/* ------------------------------------------------------------ tuning -- */
+
+static int g_L = 0, g_W = 0, g_wg_per_cu = 0;
+
+int ft_gpu_set_tuning(int L, int W, int wg_per_cu) {
+    if (L != 0 && L != 2 && L != 4 && L != 8) return FT_GPU_EINVAL;
+    if (W != 0 && W != 4 && W != 8 && W != 16) return FT_GPU_EINVAL;
+    if (wg_per_cu < 0 || wg_per_cu > 8) return FT_GPU_EINVAL;
+    g_L = L;
+    g_W = W;
+    g_wg_per_cu = wg_per_cu;
+    return FT_GPU_OK;
+}
+
+int ftamd_tuning_L(int kind, int n) {
+    (void)kind;
+    (void)n;
+    return g_L ? g_L : 4;
+}
+
+int ftamd_tuning_W(void) { return g_W ? g_W : 8; }
+
+int ftamd_tuning_grid(int ntasks) {
+    int cu = 0;
+    if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
+    int per = g_wg_per_cu ? g_wg_per_cu : 1;
+    int grid = cu * per;
+    return grid < ntasks ? grid : ntasks;
+}
+
+/* ---------------------------------------------------------- geometry -- */
+
+static int geo_step(int kind) { return kind == FTAMD_TRI ? 1 : 2; }
+static int geo_top(int kind, int n, int j) {
+    if (kind == FTAMD_SPH) return n - 3 - j;
+    if (kind == FTAMD_TRI) return n - 2 - j;
+    return n - 2 - (j + 1) / 2;
+}
+
+/* --------------------------------------------------------- schedules -- */
+
+/* One column to be transformed. */
+typedef struct {
+    void *A;       /* device matrix */
+    int n, rs;
+    long long off; /* element offset of the column */
+    int jmax;      /* largest sequence j it needs */
+    int jmin;      /* smallest (hi2lo end / lo2hi start) */
+} ft_col;
+
+typedef struct {
+    FtamdGroup g;
+    long long off[64];
+    int jmax[64];
+    int crit;      /* ordering key: longer chains first */
+} ft_hgroup;
+
+/* Group up to 64 consecutive columns of `cols` (already sorted so that
+ * columns sharing sequences are adjacent) with equal jmin and A. */
+static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, int W, ft_hgroup **out,
+                       int *ngroups, int *nprog) {
+    ft_hgroup *gs = calloc((size_t)(ncols / 1 + 1), sizeof(ft_hgroup));
+    if (!gs) return FTAMD_ENOMEM;
+    int step = geo_step(kind), G = 0, prog = 0;
+    int i = 0;
+    while (i < ncols) {
+        int cnt = 0;
+        ft_hgroup *h = &gs[G];
+        int jmaxmax = -1, jmaxmin = 1 << 30;
+        while (i < ncols && cnt < 64 && (cnt == 0 || (cols[i].jmin == cols[i - 1].jmin && cols[i].A == cols[i - 1].A))) {
+            h->off[cnt] = cols[i].off;
+            h->jmax[cnt] = cols[i].jmax;
+            if (cols[i].jmax > jmaxmax) jmaxmax = cols[i].jmax;
+            if (cols[i].jmax < jmaxmin) jmaxmin = cols[i].jmax;
+            cnt++;
+            i++;
+        }
+        const ft_col *c0 = &cols[i - cnt];
+        for (int l = cnt; l < 64; l++) {
+            h->off[l] = -1;
+            h->jmax[l] = -1;
+        }
+        int jmin = c0->jmin;
+        if (jmaxmax < jmin) continue; /* nothing to do for these columns */
+        int Q = (jmaxmax - jmin) / step + 1;
+        h->g.A = c0->A;
+        h->g.n = c0->n;
+        h->g.rs = c0->rs;
+        h->g.j_first = dir == 0 ? jmin + (Q - 1) * step : jmin;
+        h->g.Q = Q;
+        h->g.nstages = (Q + L - 1) / L;
+        h->g.npass = (h->g.nstages + W - 1) / W;
+        h->g.jfull = jmaxmin;
+        h->g.prog_base = prog;
+        h->crit = c0->n + Q * step / 2 + Q;
+        prog += h->g.npass;
+        G++;
+    }
+    *out = gs;
+    *ngroups = G;
+    *nprog = prog;
+    return FTAMD_OK;
+}
+
+/* Rotations one column performs (for the FLOP count). */
+static double column_rotations(int kind, int n, int jmin, int jmax) {
+    double r = 0;
+    for (int j = jmin; j <= jmax; j += geo_step(kind)) {
+        int t = geo_top(kind, n, j);
+        if (t >= 0) r += t + 1;
+    }
+    return r;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+/*
+ * Upload groups, lane maps, the task order and a zeroed workspace.  Tasks run
+ * pass by pass (every group's pass p before any group's pass p+1), longest
+ * chains first within a pass, so a task's producer is always handed out
+ * before it.
+ */
+static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, const void *table, int ntab) {
+    int maxpass = 0;
+    for (int g = 0; g < G; g++)
+        if (gs[g].g.npass > maxpass) maxpass = gs[g].g.npass;
+    int *order = malloc(sizeof(int) * (size_t)(G ? G : 1));
+    if (!order) return FTAMD_ENOMEM;
+    for (int g = 0; g < G; g++) order[g] = g;
+    for (int a = 1; a < G; a++) { /* insertion sort by decreasing crit */
+        int v = order[a], b = a - 1;
+        while (b >= 0 && gs[order[b]].crit < gs[v].crit) {
+            order[b + 1] = order[b];
+            b--;
+        }
+        order[b + 1] = v;
+    }
+    int T = 0;
+    for (int g = 0; g < G; g++) T += gs[g].g.npass;
+    int *tasks = malloc(sizeof(int) * 2 * (size_t)(T ? T : 1));
+    if (!tasks) {
+        free(order);
+        return FTAMD_ENOMEM;
+    }
+    int t = 0;
+    for (int p = 0; p < maxpass; p++)
+        for (int a = 0; a < G; a++) {
+            int g = order[a];
+            if (p < gs[g].g.npass) {
+                tasks[2 * t] = g;
+                tasks[2 * t + 1] = p;
+                t++;
+            }
+        }
+    free(order);
+
+    size_t o_groups = 0;
+    size_t o_off = align256(o_groups + sizeof(FtamdGroup) * (size_t)G);
+    size_t o_jmax = align256(o_off + sizeof(long long) * 64 * (size_t)G);
+    size_t o_tasks = align256(o_jmax + sizeof(int) * 64 * (size_t)G);
+    size_t o_prog = align256(o_tasks + sizeof(int) * 2 * (size_t)T);
+    size_t o_ctl = align256(o_prog + sizeof(int) * (size_t)nprog);
+    size_t total = align256(o_ctl + 4 * sizeof(int));
+    char *host = calloc(1, total);
+    if (!host) {
+        free(tasks);
+        return FTAMD_ENOMEM;
+    }
+    for (int g = 0; g < G; g++) {
+        memcpy(host + o_groups + sizeof(FtamdGroup) * g, &gs[g].g, sizeof(FtamdGroup));
+        memcpy(host + o_off + sizeof(long long) * 64 * g, gs[g].off, sizeof(long long) * 64);
+        memcpy(host + o_jmax + sizeof(int) * 64 * g, gs[g].jmax, sizeof(int) * 64);
+    }
+    memcpy(host + o_tasks, tasks, sizeof(int) * 2 * (size_t)T);
+    for (int i = 0; i < nprog; i++) ((int *)(host + o_prog))[i] = FTAMD_PROG_NONE;
+    free(tasks);
+    void *d = NULL;
+    int rc = ftamd_hip_malloc(&d, total);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, host, total, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    free(host);
+    if (rc != FTAMD_OK) {
+        ftamd_hip_free(d);
+        return rc;
+    }
+    char *db = d;
+    S->d_mem = d;
+    S->ngroups = G;
+    S->ntasks = T;
+    FtamdLaunch *a = &S->launch;
+    a->groups = (const FtamdGroup *)(db + o_groups);
+    a->lane_off = (const long long *)(db + o_off);
+    a->lane_jmax = (const int *)(db + o_jmax);
+    a->tasks = (const int *)(db + o_tasks);
+    a->ntasks = T;
+    a->sc = table;
+    a->ntab = ntab;
+    a->prog = (int *)(db + o_prog);
+    a->nprog = nprog;
+    a->ticket = (unsigned *)(db + o_ctl);
+    a->done = (unsigned *)(db + o_ctl + sizeof(int));
+    a->err = (int *)(db + o_ctl + 2 * sizeof(int));
+    return FTAMD_OK;
+}
+
+/* Columns of one n x M matrix, ordered so that equal-need columns are adjacent:
+ * sph/disk by order parity then increasing order, tri by increasing order. */
+static int matrix_columns(int kind, int n, int M, void *dA, ft_col **out, int *ncols) {
+    ft_col *cols = malloc(sizeof(ft_col) * (size_t)(M > 0 ? M : 1));
+    if (!cols) return FTAMD_ENOMEM;
+    int k = 0;
+    if (kind == FTAMD_TRI) {
+        for (int m = 1; m < M; m++) {
+            ft_col c = {dA, n, 1, (long long)n * m, m - 1, 0};
+            cols[k++] = c;
+        }
+    } else {
+        for (int par = 0; par < 2; par++)
+            for (int j = 3; j < M; j++) {
+                int m = (j + 1) / 2;
+                if (m % 2 != par) continue;
+                ft_col c = {dA, n, 1, (long long)n * j, m - 2, par};
+                cols[k++] = c;
+            }
+    }
+    *out = cols;
+    *ncols = k;
+    return FTAMD_OK;
+}
+
+static double cols_flops(int kind, const ft_col *cols, int ncols) {
+    double r = 0;
+    for (int i = 0; i < ncols; i++) r += column_rotations(kind, cols[i].n, cols[i].jmin, cols[i].jmax);
+    return 6.0 * r;
+}
+
+static int build_from_columns(ft_schedule *S, int kind, int dir, int L, int W, const ft_col *cols, int ncols,
+                              const void *table, int ntab) {
+    ft_hgroup *gs = NULL;
+    int G = 0, nprog = 0;
+    int rc = make_groups(cols, ncols, kind, dir, L, W, &gs, &G, &nprog);
+    if (rc != FTAMD_OK) return rc;
+    rc = finish_schedule(S, gs, G, nprog, table, ntab);
+    free(gs);
+    S->flops = cols_flops(kind, cols, ncols);
+    return rc;
+}
+
+/* Schedule for the whole matrix (cached in the plan; caller holds RP->lock). */
+static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision, int M, void *dA, int *rc) {
+    int L = ftamd_tuning_L(RP->kind, RP->n), W = ftamd_tuning_W();
+    for (ft_schedule *s = RP->scheds; s; s = s->next)
+        if (!s->single && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W) {
+            /* Matrix pointer may change between calls: groups hold it. */
+            return s;
+        }
+    ft_schedule *S = calloc(1, sizeof(ft_schedule));
+    if (!S) {
+        *rc = FTAMD_ENOMEM;
+        return NULL;
+    }
+    S->M = M;
+    S->dir = dir;
+    S->precision = precision;
+    S->L = L;
+    S->W = W;
+    ft_col *cols = NULL;
+    int ncols = 0;
+    *rc = matrix_columns(RP->kind, RP->n, M, dA, &cols, &ncols);
+    if (*rc == FTAMD_OK)
+        *rc = build_from_columns(S, RP->kind, dir, L, W, cols, ncols, ftamd_plan_table(RP, precision), RP->n);
+    free(cols);
+    if (*rc != FTAMD_OK) {
+        free(S);
+        return NULL;
+    }
+    S->next = RP->scheds;
+    RP->scheds = S;
+    return S;
+}
+
+/* Point every group of a cached schedule at matrix dA (same n and layout). */
+static int retarget(ft_schedule *S, void *dA) {
+    if (S->ngroups == 0) return FTAMD_OK;
+    FtamdGroup *h = malloc(sizeof(FtamdGroup) * (size_t)S->ngroups);
+    if (!h) return FTAMD_ENOMEM;
+    int rc = ftamd_hip_d2h(h, S->launch.groups, sizeof(FtamdGroup) * (size_t)S->ngroups, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    if (rc == FTAMD_OK && h[0].A != dA) {
+        for (int g = 0; g < S->ngroups; g++) h[g].A = dA;
+        rc = ftamd_hip_h2d((void *)S->launch.groups, h, sizeof(FtamdGroup) * (size_t)S->ngroups, NULL);
+        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    }
+    free(h);
+    return rc;
+}
+
+static int launch(ft_schedule *S, int precision, int kind, int dir, void *stream) {
+    return ftamd_hip_launch(&S->launch, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S->ntasks), stream);
+}
+
+static int check_kernel_error(ft_schedule *S) {
+    int err = 0;
+    int rc = ftamd_hip_d2h(&err, S->launch.err, sizeof(int), NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    if (rc != FTAMD_OK) return rc;
+    if (err) {
+        int zero = 0;
+        ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), NULL);
+        ftamd_hip_sync(NULL);
+        return FTAMD_ETIMEOUT;
+    }
+    return FTAMD_OK;
+}
+
+/* --------------------------------------------------------- executors -- */
+
+static int valid_M(int kind, int n, int M) {
+    if (M < 0) return 0;
+    if (kind == FTAMD_TRI) return M <= n;
+    if (M == 0) return 1;
+    if (M % 2 == 0) return 0;
+    return kind == FTAMD_SPH ? M <= 2 * n - 1 : M <= 4 * n - 3;
+}
+
+/* Device-resident, asynchronous execution. */
+static int gpu_execute(ft_rotation_plan *RP, int dir, int precision, void *dA, int M, void *stream,
+                       ft_schedule **used) {
+    int rc = FTAMD_OK;
+    pthread_mutex_lock(&RP->lock);
+    rc = ftamd_plan_upload(RP, precision);
+    ft_schedule *S = rc == FTAMD_OK ? matrix_schedule(RP, dir, precision, M, dA, &rc) : NULL;
+    if (S) rc = retarget(S, dA);
+    pthread_mutex_unlock(&RP->lock);
+    if (rc != FTAMD_OK) return rc;
+    if (used) *used = S;
+    return launch(S, precision, RP->kind, dir, stream);
+}
+
+int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
+                   void *stream) {
+    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
+    if (!RP || !dA || RP->kind != kind || (direction != 0 && direction != 1) || (precision != 0 && precision != 1) ||
+        !valid_M(kind, RP->n, M))
+        return FT_GPU_EINVAL;
+    if (M == 0) return FT_GPU_OK;
+    return gpu_execute(RP, direction, precision, dA, M, stream, NULL);
+}
+
+/* The reference entry points: synchronous, host or device A. */
+static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, int M) {
+    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
+    if (!RP || !A || RP->kind != kind || !valid_M(kind, RP->n, M)) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    if (M == 0) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(A, &pk);
+    size_t bytes = (size_t)RP->n * (size_t)M * sizeof(double);
+    ft_schedule *S = NULL;
+    if (pk == FTAMD_PTR_DEVICE) {
+        rc = gpu_execute(RP, dir, 0, A, M, NULL, &S);
+        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    } else {
+        pthread_mutex_lock(&RP->lock);
+        if (RP->stage_bytes < bytes) {
+            ftamd_hip_free(RP->d_stage);
+            RP->d_stage = NULL;
+            RP->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&RP->d_stage, bytes);
+            if (rc == FTAMD_OK) RP->stage_bytes = bytes;
+        }
+        void *d = RP->d_stage;
+        pthread_mutex_unlock(&RP->lock);
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, bytes, NULL);
+        if (rc == FTAMD_OK) rc = gpu_execute(RP, dir, 0, d, M, NULL, &S);
+        if (rc == FTAMD_OK) rc = ftamd_hip_d2h(A, d, bytes, NULL);
+        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    }
+    if (rc == FTAMD_OK && S) rc = check_kernel_error(S);
+    ftamd_set_error(rc);
+}
+
+/* drivers.c:58-92 */
+void ft_execute_sph_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_SPH, 0, A, M);
+}
+void ft_execute_sph_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_SPH, 1, A, M);
+}
+/* drivers.c:278-312 */
+void ft_execute_tri_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_TRI, 0, A, M);
+}
+void ft_execute_tri_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_TRI, 1, A, M);
+}
+/* drivers.c:378-412 */
+void ft_execute_disk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_DISK, 0, A, M);
+}
+void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FTAMD_DISK, 1, A, M);
+}
+
+/* ------------------------------------------------- single vectors -- */
+
+/*
+ * ft_kernel_<kind>_<dir>(RP, m1, m2, A, S) (rotations.c:135-141, 162-168,
+ * 262-268; loops of rotations_default.c:31-71): hi2lo runs j = m2-step,
+ * m2-2*step, ... >= m1; lo2hi runs j = m1, m1+step, ... < m2.  Sequences whose
+ * row range is empty (j beyond the plan) do nothing, as in the reference.
+ */
+static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, int m2, double *A, int S) {
+    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
+    if (!RP || !A || RP->kind != kind || S < 1 || m1 < 0) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int step = geo_step(kind), n = RP->n;
+    int jlo, jhi;
+    if (dir == 0) {
+        jhi = m2 - step;
+        jlo = m1;
+    } else {
+        jlo = m1;
+        jhi = m1 + ((m2 - 1 - m1) / step) * step;
+        if (m2 <= m1) jhi = jlo - 1;
+    }
+    /* keep sequences with rows; j must also index an existing table column */
+    int jcap = kind == FTAMD_DISK ? 2 * n - 2 : n - 1;
+    while (jhi >= jlo && (geo_top(kind, n, jhi) < 0 || jhi > jcap)) jhi -= step;
+    if (jhi < jlo) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(A, &pk);
+    size_t span = ((size_t)(n - 1) * (size_t)S + 1) * sizeof(double);
+    void *d = A;
+    if (pk != FTAMD_PTR_DEVICE) {
+        rc = ftamd_hip_malloc(&d, span);
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, span, NULL);
+    }
+    ft_schedule *Sd = NULL;
+    if (rc == FTAMD_OK) {
+        pthread_mutex_lock(&RP->lock);
+        rc = ftamd_plan_upload(RP, 0);
+        pthread_mutex_unlock(&RP->lock);
+    }
+    if (rc == FTAMD_OK) {
+        Sd = calloc(1, sizeof(ft_schedule));
+        if (!Sd) rc = FTAMD_ENOMEM;
+    }
+    if (rc == FTAMD_OK) {
+        Sd->single = 1;
+        Sd->dir = dir;
+        Sd->L = ftamd_tuning_L(kind, n);
+        Sd->W = ftamd_tuning_W();
+        ft_col c = {d, n, S, 0, jhi, jlo};
+        rc = build_from_columns(Sd, kind, dir, Sd->L, Sd->W, &c, 1, ftamd_plan_table(RP, 0), n);
+    }
+    if (rc == FTAMD_OK) rc = launch(Sd, 0, kind, dir, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    if (rc == FTAMD_OK) rc = check_kernel_error(Sd);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    if (Sd) {
+        ftamd_hip_free(Sd->d_mem);
+        free(Sd);
+    }
+    if (pk != FTAMD_PTR_DEVICE) ftamd_hip_free(d);
+    ftamd_set_error(rc);
+}
+
+void ft_kernel_sph_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_SPH, 0, m1, m2, A, S);
+}
+void ft_kernel_sph_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_SPH, 1, m1, m2, A, S);
+}
+void ft_kernel_tri_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_TRI, 0, m1, m2, A, S);
+}
+void ft_kernel_tri_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_TRI, 1, m1, m2, A, S);
+}
+void ft_kernel_disk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_DISK, 0, m1, m2, A, S);
+}
+void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_DISK, 1, m1, m2, A, S);
+}
+
+/* ------------------------------------------------------------ batch -- */
+
+int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                         void *const *dA, const int *M, int count, void *stream) {
+    (void)plans;
+    (void)kind;
+    (void)direction;
+    (void)precision;
+    (void)dA;
+    (void)M;
+    (void)count;
+    (void)stream;
+    return FT_GPU_EINVAL;
+}

@@ -1,0 +1,53 @@
+/*
+ * ft_internal.h -- private definitions of the C host layer.
+ */
+#ifndef FT_INTERNAL_H
+#define FT_INTERNAL_H
+
+#include <pthread.h>
+
+#include "fasttransforms_amd.h"
+#include "ft_hip_abi.h"
+
+/* A launch-ready schedule: groups, lane maps and the task order on the
+ * device, plus the kernel's ticket/progress workspace. */
+typedef struct ft_schedule {
+    struct ft_schedule *next;
+    /* key */
+    int M, dir, precision, L, W, rs, m1, m2, single;
+    /* device arrays (one allocation) */
+    void *d_mem;
+    FtamdLaunch launch;
+    int ngroups, ntasks;
+    double flops;            /* algorithmic flops (6 per rotation) */
+} ft_schedule;
+
+/* The reference's layout (ftinternal.h:370-374) first, then device state. */
+struct ft_rotation_plan_s {
+    double *s;
+    double *c;
+    int n;
+    /* extension */
+    int kind;
+    double alpha, beta, gamma;
+    long nsc;                /* entries per table */
+    int device;              /* HIP device of the uploads, -1 before upload */
+    void *d_tab[2];          /* interleaved (s,c) with padding: [0] FP64, [1] FP32 */
+    ft_schedule *scheds;
+    void *d_stage;           /* staging buffer for host-pointer calls */
+    size_t stage_bytes;
+    pthread_mutex_t lock;
+};
+
+/* ft_plans.c */
+ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, double gamma);
+int ftamd_plan_upload(ft_rotation_plan *RP, int precision);
+const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
+
+/* ft_drivers.c */
+void ftamd_set_error(int code);
+int ftamd_tuning_L(int kind, int n);
+int ftamd_tuning_W(void);
+int ftamd_tuning_grid(int ntasks);
+
+#endif /* FT_INTERNAL_H */

@@ -1,0 +1,387 @@
+// ft_pipeline.hpp -- the pipelined rotation engine (templates), shared by the
+// per-geometry kernel files.  See ft_rotations.hip for the design.
+
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "ft_hip_abi.h"
+
+namespace ftamd {
+
+constexpr int kLanes = 64;
+
+// ---------------------------------------------------------------- geometry --
+// top(j): largest row k of sequence j (real rows); base(j): first table entry
+// of column j in the plan's packed index (rotations.c:113-114, 170-171) for a
+// table built with n = N.
+template <int KIND> struct Geo;
+template <> struct Geo<FTAMD_SPH> {
+    static constexpr int D = 2, STEP = 2;
+    __device__ static int top(int n, int j) { return n - 3 - j; }
+    __device__ static long long base(int N, int j) { return (long long)j * (2 * N + 1 - j) / 2; }
+};
+template <> struct Geo<FTAMD_TRI> {
+    static constexpr int D = 1, STEP = 1;
+    __device__ static int top(int n, int j) { return n - 2 - j; }
+    __device__ static long long base(int N, int j) { return (long long)j * (2 * N + 1 - j) / 2; }
+};
+template <> struct Geo<FTAMD_DISK> {
+    static constexpr int D = 1, STEP = 2;
+    __device__ static int top(int n, int j) { return n - 2 - (j + 1) / 2; }
+    __device__ static long long base(int N, int j) { return (long long)j * N - (long long)(j / 2) * ((j + 1) / 2); }
+};
+
+// Address-space views: the plan table is read through the constant address
+// space so that wave-uniform loads become scalar (s_load) loads into SGPRs;
+// matrix columns are global memory.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+
+template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c);
+template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// One rotation of the (lower, upper) virtual row pair.
+template <typename T> __device__ __forceinline__ void givens(T &x, T &y, T s, T c) {
+    T nx = fmaT<T>(c, x, s * y);
+    T ny = fmaT<T>(c, y, -(s * x));
+    x = nx;
+    y = ny;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Bounded waits.  ~2^22 polls of >= 64 cycles each is far beyond any
+// legitimate wait; on expiry the caller abandons its task.
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ bool lds_wait_ge(const int *flag, int target, const int *abort_flag) {
+    for (int it = 0; it < kSpinLimit; ++it) {
+        int v = uniform(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (v >= target) return true;
+        if (uniform(__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool global_wait_le(const int *prog, int target, int *known, const int *abort_flag) {
+    if (*known <= target) return true;
+    for (int it = 0; it < kSpinLimit; ++it) {
+        int v = uniform(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (v <= target) { *known = v; return true; }
+        if (uniform(__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------ kernel --
+template <typename T, int KIND, int DIR, int L, int W, int R>
+struct Pipeline {
+    using G = Geo<KIND>;
+    static constexpr int D = G::D;
+    static constexpr int STEP = G::STEP;
+    static constexpr int WS = (D + 1) * L;   // window rows per lane
+
+    struct Shared {
+        T ring[W > 1 ? W - 1 : 1][R][L][kLanes];
+        int emitted[W];
+        int consumed[W];
+        int task;
+        int abort_flag;
+    };
+
+    // Everything one stage wave needs, uniform except the per-lane column data.
+    struct Stage {
+        cptr<T> sc;         // table entry 0 (interleaved s, c)
+        gptr<T> col;        // this lane's column (nullptr: no column)
+        int rs;             // row stride
+        int n, N;           // rows of the transform, n of the table
+        int jmax;           // largest j this lane needs
+        int q0, nq;         // first sequence and count
+        int j0;             // j of sequence q0
+        int K, Bot;         // first tile position, lowest virtual row to emit
+        int ntile;          // tiles
+        int jfull;
+        bool lanes_all;     // every lane active for every sequence of the stage
+        int lo0;            // DIR 1: lowest virtual row of sequence q0
+    };
+
+    __device__ static int seq_j(int j_first, int q) { return DIR == 0 ? j_first - STEP * q : j_first + STEP * q; }
+    __device__ static int real_row(int n, int v) { return DIR == 0 ? v : n - 1 - v; }
+
+    // Table entry of rotation (virtual row v, sequence j).
+    __device__ static long long tab_idx(const Stage &st, int j, int v) {
+        return DIR == 0 ? G::base(st.N, j) + v : G::base(st.N, j) + (st.n - 1 - D - v);
+    }
+
+    template <int PH> __device__ static constexpr int slot(int i) { return (PH * L + i) % WS; }
+
+    // One L x L tile at position k (virtual), phase PH of the register ring.
+    template <int PH, bool CHECKED>
+    __device__ static void rotate_tile(const Stage &st, T (&x)[WS], int k) {
+#pragma unroll
+        for (int lj = 0; lj < L; ++lj) {
+            if (CHECKED && lj >= st.nq) break;
+            const int j = DIR == 0 ? st.j0 - STEP * lj : st.j0 + STEP * lj;
+            // L consecutive table entries of column j (scalar loads).
+            const long long i0 = DIR == 0 ? tab_idx(st, j, k + D * lj) - (L - 1) : tab_idx(st, j, k + D * lj);
+            cptr<T> p = st.sc + 2 * i0;
+            T sv[L], cv[L];
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                sv[i] = p[2 * i];
+                cv[i] = p[2 * i + 1];
+            }
+            int lo = 0, hi = 0;
+            if (CHECKED) {
+                const int top = G::top(st.n, j);
+                lo = DIR == 0 ? 0 : st.n - 1 - D - top;
+                hi = DIR == 0 ? top : st.n - 1 - D;
+            }
+            const bool act = !CHECKED || st.lanes_all || j <= st.jmax;
+#pragma unroll
+            for (int lk = 0; lk < L; ++lk) {
+                const int r = k - lk + D * lj;
+                if (CHECKED && (r < lo || r > hi)) continue;
+                const int i = DIR == 0 ? L - 1 - lk : lk;
+                const int px = slot<PH>(D * L + lk - D * lj);
+                const int py = slot<PH>(D * L + lk - D * lj - D);
+                if (act) givens<T>(x[px], x[py], sv[i], cv[i]);
+            }
+        }
+    }
+
+    // Row v of this lane's column (0 if outside [lo_valid, n-1]).
+    __device__ static T load_row(const Stage &st, int v, int lo_valid) {
+        if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return T(0);
+        return st.col[(long long)real_row(st.n, v) * st.rs];
+    }
+    __device__ static void store_row(const Stage &st, int v, int lo_valid, T val) {
+        if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return;
+        st.col[(long long)real_row(st.n, v) * st.rs] = val;
+    }
+
+    __device__ static void run(const FtamdLaunch &a, Shared &sh) {
+        const int lane = threadIdx.x & (kLanes - 1);
+        const int w = uniform(threadIdx.x / kLanes);
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                sh.task = (int)atomicAdd(a.ticket, 1u);
+                sh.abort_flag = 0;
+            }
+            if (threadIdx.x < W) {
+                sh.emitted[threadIdx.x] = 0;
+                sh.consumed[threadIdx.x] = 0;
+            }
+            __syncthreads();
+            const int task = uniform(sh.task);
+            if (task >= a.ntasks) break;
+            const int g = uniform(a.tasks[2 * task]);
+            const int p = uniform(a.tasks[2 * task + 1]);
+            run_stage(a, sh, g, p, w, lane);
+        }
+        // Last workgroup out resets the ticket and the progress slots so the
+        // next launch (or graph replay) starts clean.
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned d = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (d == gridDim.x - 1) {
+                for (int i = 0; i < a.nprog; ++i)
+                    __hip_atomic_store(a.prog + i, FTAMD_PROG_NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+
+    __device__ static void fail(const FtamdLaunch &a, Shared &sh) {
+        __hip_atomic_store(&sh.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        atomicOr(a.err, FTAMD_ERR_TIMEOUT);
+    }
+
+    __device__ static void run_stage(const FtamdLaunch &a, Shared &sh, int g, int p, int w, int lane) {
+        const FtamdGroup &gd = a.groups[g];
+        const int nstages = uniform(gd.nstages);
+        const int s = p * W + w;
+        if (s >= nstages) return;
+        const int last = uniform(min(W, nstages - p * W) - 1);
+        const bool is_first = w == 0, is_last = w == last;
+
+        Stage st;
+        st.sc = (cptr<T>)(a.sc);
+        st.n = uniform(gd.n);
+        st.N = a.ntab;
+        st.rs = uniform(gd.rs);
+        const long long off = a.lane_off[(long long)g * kLanes + lane];
+        st.col = off < 0 ? nullptr : (gptr<T>)(gd.A) + off;
+        st.jmax = a.lane_jmax[(long long)g * kLanes + lane];
+        const int Q = uniform(gd.Q);
+        const int j_first = uniform(gd.j_first);
+        st.q0 = s * L;
+        st.nq = min(L, Q - st.q0);
+        st.j0 = seq_j(j_first, st.q0);
+        st.jfull = uniform(gd.jfull);
+        {
+            const int jmax_stage = DIR == 0 ? st.j0 : seq_j(j_first, st.q0 + st.nq - 1);
+            st.lanes_all = jmax_stage <= st.jfull;
+        }
+        if (DIR == 0) {
+            st.K = G::top(st.n, st.j0);
+            st.Bot = 0;
+            st.lo0 = 0;
+        } else {
+            st.K = st.n - 1;
+            st.Bot = st.n - 1 - D - G::top(st.n, seq_j(j_first, p * W * L));
+            st.lo0 = st.n - 1 - D - G::top(st.n, st.j0);
+        }
+        {
+            const int E = st.K + (D - 1) * L + 1 - st.Bot;
+            st.ntile = E <= 0 ? 1 : (E + L - 1) / L + 1;
+        }
+        const int emit_off = DIR == 0 ? 0 : D;
+        // Emits the previous stage makes (only needed by consumers).
+        int prev_emits = 0;
+        if (!is_first) {
+            const int qp = st.q0 - L;
+            const int jp = seq_j(j_first, qp);
+            const int Kp = DIR == 0 ? G::top(st.n, jp) : st.n - 1;
+            const int E = Kp + (D - 1) * L + 1 - st.Bot;
+            const int Tp = E <= 0 ? 1 : (E + L - 1) / L + 1;
+            prev_emits = Tp - (DIR == 0 ? 0 : D);
+        }
+        const int lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
+        int *prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
+        int *prog_mine = p + 1 < uniform(gd.npass) ? a.prog + uniform(gd.prog_base) + p : nullptr;
+        int known = FTAMD_PROG_NONE;
+
+        T x[WS];
+#pragma unroll
+        for (int i = 0; i < WS; ++i) x[i] = T(0);
+
+        // Initial carried rows (positions 0 .. DL-1 = virtual rows K+DL .. K+1).
+        // hi2lo: untouched by the earlier stages of this task, read from A.
+        // lo2hi: above the top row, zero.
+        if (DIR == 0) {
+            if (prog_prev && !global_wait_le(prog_prev, st.K + 1, &known, &sh.abort_flag)) { fail(a, sh); return; }
+#pragma unroll
+            for (int i = 0; i < D * L; ++i) x[i] = load_row(st, st.K + D * L - i, lo_valid);
+        }
+
+        int t = 0;
+        for (;;) {
+            if (!tile<0>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
+                         prog_mine, &known)) return;
+            if (++t >= st.ntile) break;
+            if (!tile<1>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
+                         prog_mine, &known)) return;
+            if (++t >= st.ntile) break;
+            if constexpr (D == 2) {
+                if (!tile<2>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
+                             prog_mine, &known)) return;
+                if (++t >= st.ntile) break;
+            }
+        }
+        if (is_last && prog_mine) __hip_atomic_store(prog_mine, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    template <int PH>
+    __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, T (&x)[WS], int t, int w, int lane,
+                                bool is_first, bool is_last, int prev_emits, int emit_off, int lo_valid,
+                                int *prog_prev, int *prog_mine, int *known) {
+        const int k = st.K - t * L;
+        // ---- new rows: positions DL .. DL+L-1 = virtual rows k .. k-L+1
+        if (is_first) {
+            if (prog_prev && !global_wait_le(prog_prev, max(k - L + 1, lo_valid), known, &sh.abort_flag)) {
+                fail(a, sh);
+                return false;
+            }
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = load_row(st, k - i, lo_valid);
+        } else if (t < prev_emits) {
+            if (!lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag)) { fail(a, sh); return false; }
+            const T *src = &sh.ring[w - 1][t % R][0][lane];
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = src[i * kLanes];
+            __hip_atomic_store(&sh.consumed[w - 1], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = T(0);
+        }
+
+        // ---- rotations
+        bool fast = st.nq == L && st.lanes_all;
+        if (DIR == 0) fast = fast && k - (L - 1) >= 0;
+        else fast = fast && k + D * (L - 1) <= st.n - 1 - D && k - (L - 1) >= st.lo0;
+        if (fast) rotate_tile<PH, false>(st, x, k);
+        else rotate_tile<PH, true>(st, x, k);
+
+        // ---- emit: positions 0 .. L-1 = virtual rows k+DL .. k+(D-1)L+1
+        if (t >= emit_off) {
+            const int e = t - emit_off;
+            if (is_last) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) store_row(st, k + D * L - i, lo_valid, x[slot<PH>(i)]);
+                if (prog_mine)
+                    __hip_atomic_store(prog_mine, k + (D - 1) * L + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (e >= R && !lds_wait_ge(&sh.consumed[w], e - R + 1, &sh.abort_flag)) { fail(a, sh); return false; }
+                T *dst = &sh.ring[w][e % R][0][lane];
+#pragma unroll
+                for (int i = 0; i < L; ++i) dst[i * kLanes] = x[slot<PH>(i)];
+                __hip_atomic_store(&sh.emitted[w], e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        return true;
+    }
+};
+
+template <typename T, int KIND, int DIR, int L, int W, int R>
+__global__ void __launch_bounds__(W * kLanes) rot_pipeline(FtamdLaunch a) {
+    using P = Pipeline<T, KIND, DIR, L, W, R>;
+    __shared__ typename P::Shared sh;
+    P::run(a, sh);
+}
+
+using KernelFn = void (*)(FtamdLaunch);
+
+// Kernel for (L, W) if its LDS ring fits the 160 KB of a gfx950 CU.
+template <typename T, int KIND, int DIR, int L, int W>
+KernelFn instance() {
+    constexpr int R = 4;
+    if constexpr (sizeof(typename Pipeline<T, KIND, DIR, L, W, R>::Shared) <= 160 * 1024)
+        return rot_pipeline<T, KIND, DIR, L, W, R>;
+    else
+        return nullptr;
+}
+
+// The (L, W) shapes built for every geometry and direction.
+template <typename T, int KIND, int DIR>
+KernelFn pick_shape(int L, int W) {
+    switch (L * 100 + W) {
+        case 208: return instance<T, KIND, DIR, 2, 8>();
+        case 216: return instance<T, KIND, DIR, 2, 16>();
+        case 404: return instance<T, KIND, DIR, 4, 4>();
+        case 408: return instance<T, KIND, DIR, 4, 8>();
+        case 416: return instance<T, KIND, DIR, 4, 16>();
+        case 804: return instance<T, KIND, DIR, 8, 4>();
+        case 808: return instance<T, KIND, DIR, 8, 8>();
+        default: return nullptr;
+    }
+}
+
+template <int KIND>
+KernelFn pick_kind(int precision, int dir, int L, int W) {
+    if (precision == 0) return dir == 0 ? pick_shape<double, KIND, 0>(L, W) : pick_shape<double, KIND, 1>(L, W);
+    return dir == 0 ? pick_shape<float, KIND, 0>(L, W) : pick_shape<float, KIND, 1>(L, W);
+}
+
+KernelFn pick_sph(int precision, int dir, int L, int W);
+KernelFn pick_tri(int precision, int dir, int L, int W);
+KernelFn pick_disk(int precision, int dir, int L, int W);
+
+}  // namespace ftamd

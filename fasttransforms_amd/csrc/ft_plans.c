@@ -1,0 +1,161 @@
+/*
+ * ft_plans.c -- rotation plans: FP64 tables built on the host, uploaded to
+ * HBM once (FP64, and an FP32 copy rounded from FP64 on first FP32 use).
+ *
+ * The tables are the reference's closed forms (src/rotations.c:113-190) with
+ * the same operation order, so they are bit-identical: integer products for
+ * the sphere, left-to-right double sums for the triangle and the disk, then
+ * one correctly rounded division and one sqrt per entry.  Compiled with
+ * -ffp-contract=off.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ft_internal.h"
+
+static long idx_tri(int n, int l, int m) { return l + (long)m * (2 * n + 1 - m) / 2; }
+static long idx_disk(int n, int l, int m) { return l + (long)m * n - (long)(m / 2) * ((m + 1) / 2); }
+
+static int g_num_threads = 1;
+
+/* fasttransforms.h:295.  Recorded only: no host threads compute here. */
+void ft_set_num_threads(const int n) { g_num_threads = n; }
+
+ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, double gamma) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_rotation_plan *RP = calloc(1, sizeof(ft_rotation_plan));
+    if (!RP) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    RP->n = n;
+    RP->kind = kind;
+    RP->alpha = alpha;
+    RP->beta = beta;
+    RP->gamma = gamma;
+    RP->nsc = kind == FTAMD_DISK ? (long)n * n : (long)n * (n + 1) / 2;
+    RP->s = malloc((size_t)RP->nsc * sizeof(double));
+    RP->c = malloc((size_t)RP->nsc * sizeof(double));
+    RP->device = -1;
+    pthread_mutex_init(&RP->lock, NULL);
+    if (!RP->s || !RP->c) {
+        free(RP->s);
+        free(RP->c);
+        free(RP);
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    return RP;
+}
+
+/* rotations.c:116-133 */
+ft_rotation_plan *ft_plan_rotsphere(const int n) {
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_SPH, n, 0.0, 0.0, 0.0);
+    if (!RP) return NULL;
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)((l + 1) * (l + 2));
+            double numc = (double)((2 * m + 2) * (2 * l + 2 * m + 5));
+            double den = (double)((l + 2 * m + 3) * (l + 2 * m + 4));
+            RP->s[idx_tri(n, l, m)] = sqrt(nums / den);
+            RP->c[idx_tri(n, l, m)] = sqrt(numc / den);
+        }
+    return RP;
+}
+
+/* rotations.c:143-160 */
+ft_rotation_plan *ft_plan_rottriangle(const int n, const double alpha, const double beta, const double gamma) {
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_TRI, n, alpha, beta, gamma);
+    if (!RP) return NULL;
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)(l + 1) * ((l + alpha) + 1);
+            double numc = ((((2 * m) + beta) + gamma) + 2) * (((((2 * l + 2 * m) + alpha) + beta) + gamma) + 4);
+            double den = ((((l + 2 * m) + beta) + gamma) + 3) * (((((l + 2 * m) + alpha) + beta) + gamma) + 3);
+            RP->s[idx_tri(n, l, m)] = sqrt(nums / den);
+            RP->c[idx_tri(n, l, m)] = sqrt(numc / den);
+        }
+    return RP;
+}
+
+/* rotations.c:173-190 */
+ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double beta) {
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_DISK, n, alpha, beta, 0.0);
+    if (!RP) return NULL;
+    for (int m = 0; m < 2 * n - 1; m++)
+        for (int l = 0; l < n - (m + 1) / 2; l++) {
+            double nums = (double)(l + 1) * ((l + beta) + 1);
+            double numc = ((m + alpha) + 1) * ((((2 * l + m) + alpha) + beta) + 3);
+            double den = (((l + m) + alpha) + 2) * ((((l + m) + alpha) + beta) + 2);
+            RP->s[idx_disk(n, l, m)] = -sqrt(nums / den);
+            RP->c[idx_disk(n, l, m)] = sqrt(numc / den);
+        }
+    return RP;
+}
+
+/* rotations.c:6-10, plus the device state. */
+void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
+    if (!RP) return;
+    ft_schedule *sc = RP->scheds;
+    while (sc) {
+        ft_schedule *next = sc->next;
+        ftamd_hip_free(sc->d_mem);
+        free(sc);
+        sc = next;
+    }
+    ftamd_hip_free(RP->d_tab[0]);
+    ftamd_hip_free(RP->d_tab[1]);
+    ftamd_hip_free(RP->d_stage);
+    pthread_mutex_destroy(&RP->lock);
+    free(RP->s);
+    free(RP->c);
+    free(RP);
+}
+
+/*
+ * Upload the interleaved table {s0, c0, s1, c1, ...} with FTAMD_TABLE_PAD zero
+ * entries on each side.  FP32 entries are (float) of the FP64 ones.  Caller
+ * holds RP->lock.
+ */
+int ftamd_plan_upload(ft_rotation_plan *RP, int precision) {
+    if (RP->d_tab[precision]) return FTAMD_OK;
+    int dev = -1, rc = ftamd_hip_device(&dev, NULL);
+    if (rc != FTAMD_OK) return rc;
+    if (RP->device >= 0 && RP->device != dev) return FTAMD_EDEVICE;
+    size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
+    size_t count = 2 * ((size_t)RP->nsc + 2 * FTAMD_TABLE_PAD);
+    void *host = calloc(count, esz);
+    if (!host) return FTAMD_ENOMEM;
+    size_t base = 2 * (size_t)FTAMD_TABLE_PAD;
+    for (long i = 0; i < RP->nsc; i++) {
+        if (precision == 0) {
+            ((double *)host)[base + 2 * i] = RP->s[i];
+            ((double *)host)[base + 2 * i + 1] = RP->c[i];
+        } else {
+            ((float *)host)[base + 2 * i] = (float)RP->s[i];
+            ((float *)host)[base + 2 * i + 1] = (float)RP->c[i];
+        }
+    }
+    void *d = NULL;
+    rc = ftamd_hip_malloc(&d, count * esz);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, host, count * esz, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    free(host);
+    if (rc != FTAMD_OK) {
+        ftamd_hip_free(d);
+        return rc;
+    }
+    RP->d_tab[precision] = d;
+    RP->device = dev;
+    return FTAMD_OK;
+}
+
+/* Device address of table entry 0 (past the leading padding). */
+const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision) {
+    size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
+    return (const char *)RP->d_tab[precision] + 2 * (size_t)FTAMD_TABLE_PAD * esz;
+}

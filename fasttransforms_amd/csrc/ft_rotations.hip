@@ -1,0 +1,127 @@
+// ft_rotations.hip -- gfx950 kernels of the Givens-rotation stage and the thin
+// C-ABI layer the C host code (ft_drivers.c) calls.
+//
+// What is computed (src/rotations/rotations_default.c:31-71): for every
+// column, a chain of rotation sequences.  Sequence j rotates the row pairs
+// (k, k+D) for k running over the sequence's rows, with s = s(k,j), c = c(k,j)
+// from the plan; hi2lo applies sequences in decreasing j with k descending,
+// lo2hi in increasing j with k ascending and the transposed rotation.
+// sph: D = 2, j steps by 2; tri: D = 1, j steps by 1; disk: D = 1, j steps by 2.
+//
+// How it maps onto the GPU (one design for every geometry and direction):
+//
+//  * Lanes are columns.  A "group" is up to 64 columns that need the same
+//    sequences (equal order parity for sph/disk, consecutive orders for tri),
+//    so at any moment every lane applies the same rotation (k, j): s and c are
+//    wave-uniform and come from scalar loads into SGPRs, which v_fma_f64 reads
+//    directly.  Columns of lower order skip the leading sequences by exec mask.
+//  * Waves are pipeline stages.  Stage s applies sequences q = sL .. sL+L-1 of
+//    the group.  It holds a window of (D+1)L rows per lane in registers and
+//    sweeps it down the column L rows per tile, applying the L sequences
+//    along an anti-diagonal (the reference's L x L register tile,
+//    src/rotations/rotations.h:26-45).  Each tile takes L new rows from the previous
+//    stage and hands L finished rows to the next one through an LDS ring.
+//  * lo2hi is run as hi2lo on reversed rows ("virtual" row v = n-1-real row):
+//    the transposed rotation on the reversed pair is exactly the forward
+//    rotation, so one engine covers both directions.
+//  * A task is W consecutive stages (one workgroup, one pass over the rows).
+//    Consecutive tasks of a group are chained through A itself: the last stage
+//    of a task stores its rows in place and publishes how far down it is; the
+//    next task's reads wait for that (agent-scope release/acquire).
+//  * The kernel is persistent: workgroups take tasks from an atomic ticket in a
+//    host-sorted order (long chains first, every producer before its
+//    consumer), so a waiting task's producer is always already running.
+//    Every wait is bounded; on expiry the kernel records FT_GPU_ETIMEOUT and
+//    every wave still reaches its exit.
+//
+// Arithmetic: x' = fma(c, x, s*y), y' = fma(c, y, -(s*x)) -- the fused form of
+// the reference's dispatched SIMD kernels (src/rotations/rotations_AVX512F.c:4-15).
+// This file is compiled with -ffp-contract=off so no other contraction occurs.
+
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "ft_pipeline.hpp"
+
+namespace {
+
+thread_local int g_hip_err = 0;
+
+int map_err(hipError_t e) {
+    if (e == hipSuccess) return FTAMD_OK;
+    g_hip_err = (int)e;
+    return e == hipErrorOutOfMemory ? FTAMD_ENOMEM : FTAMD_EDEVICE;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ====
+
+extern "C" {
+
+int ftamd_hip_device(int *device, int *num_cu) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FTAMD_EDEVICE;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return map_err(e);
+    if (device) *device = dev;
+    if (num_cu) {
+        int cu = 0;
+        e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return map_err(e);
+        *num_cu = cu;
+    }
+    return FTAMD_OK;
+}
+
+int ftamd_hip_pointer_kind(const void *ptr, int *kind) {
+    hipPointerAttribute_t attr;
+    memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *kind = FTAMD_PTR_HOST;
+        return FTAMD_OK;
+    }
+    *kind = (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) ? FTAMD_PTR_DEVICE
+                                                                                     : FTAMD_PTR_HOST;
+    return FTAMD_OK;
+}
+
+int ftamd_hip_malloc(void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    return map_err(hipMalloc(ptr, bytes));
+}
+
+int ftamd_hip_free(void *ptr) { return ptr ? map_err(hipFree(ptr)) : FTAMD_OK; }
+
+int ftamd_hip_memset(void *ptr, int value, size_t bytes) { return map_err(hipMemset(ptr, value, bytes)); }
+
+int ftamd_hip_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    return map_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+}
+
+int ftamd_hip_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    return map_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+}
+
+int ftamd_hip_sync(void *stream) { return map_err(hipStreamSynchronize((hipStream_t)stream)); }
+
+int ftamd_hip_last_runtime_error(void) { return g_hip_err; }
+
+// Launch the pipelined kernel described by *a on `stream`.
+int ftamd_hip_launch(const FtamdLaunch *a, int precision, int kind, int dir, int L, int W, int grid, void *stream) {
+    ftamd::KernelFn fn = kind == FTAMD_SPH ? ftamd::pick_sph(precision, dir, L, W)
+                       : kind == FTAMD_TRI ? ftamd::pick_tri(precision, dir, L, W)
+                       : kind == FTAMD_DISK ? ftamd::pick_disk(precision, dir, L, W) : nullptr;
+    if (!fn) return FTAMD_EINVAL;
+    if (a->ntasks <= 0) return FTAMD_OK;
+    if (grid < 1) grid = 1;
+    if (grid > a->ntasks) grid = a->ntasks;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(W * ftamd::kLanes), 0, (hipStream_t)stream, *a);
+    return map_err(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+"""Test and benchmark utilities of the reference (src/ftutilities.c), in numpy.
+
+* ``fill``: i.i.d. U(-1, 1) entries in each layout's valid triangle, zeros
+  elsewhere -- sphrand / trirand / diskrand (ftutilities.c:75-113) with a
+  seeded, counter-based SplitMix64 stream instead of libc rand(), so inputs
+  are reproducible across hosts and languages: entry k = i + n*j of stream
+  `seed` is the k-th SplitMix64 output, u = (top 53 bits) * 2^-53, value 2u-1.
+* ``ones``: sphones / triones / diskones (ftutilities.c:67-105).
+* ``rel2``, ``relinf``: ft_norm_2arg / ft_norm_1arg and the inf-norm variants
+  (ftutilities_source.c:6-36) as relative errors.
+"""
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def rand_entries(seed, k):
+    """U(-1,1) entries k (uint64 array) of stream `seed`."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (k.astype(np.uint64) + np.uint64(1)) * _GOLDEN
+        z = _mix64(z)
+    return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
+
+
+def valid_rows(kind, n, j):
+    j = np.asarray(j)
+    if kind == "sph":
+        return n - (j + 1) // 2
+    if kind == "tri":
+        return n - j
+    return n - (j + 3) // 4
+
+
+def full_M(kind, n):
+    return {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[kind]
+
+
+def _mask(kind, n, M):
+    rows = valid_rows(kind, n, np.arange(M))
+    return np.arange(n)[:, None] < rows[None, :]
+
+
+def fill(kind, n, M, seed):
+    """Seeded valid-triangle fill; Fortran-ordered (n, M) float64."""
+    k = np.arange(n * M, dtype=np.uint64).reshape(M, n).T  # k = i + n*j
+    A = np.where(_mask(kind, n, M), rand_entries(seed, k), 0.0)
+    return np.asfortranarray(A)
+
+
+def ones(kind, n, M):
+    return np.asfortranarray(_mask(kind, n, M).astype(np.float64))
+
+
+def rel2(A, B):
+    d = np.linalg.norm((np.asarray(A, dtype=np.float64) - np.asarray(B, dtype=np.float64)).ravel())
+    b = np.linalg.norm(np.asarray(B, dtype=np.float64).ravel())
+    return d / b if b else d
+
+
+def relinf(A, B):
+    d = np.max(np.abs(np.asarray(A, dtype=np.float64) - np.asarray(B, dtype=np.float64)))
+    b = np.max(np.abs(np.asarray(B, dtype=np.float64)))
+    return d / b if b else d
+
+
+def rotations(kind, n, M=None):
+    """Number of rotations ft_execute_<kind>_hi2lo applies to an n x M matrix
+    (6 algorithmic flops each)."""
+    if M is None:
+        M = full_M(kind, n)
+    step = 1 if kind == "tri" else 2
+    total = 0
+    for col in range(1, M):
+        if kind == "tri":
+            m, m1 = col, 0
+        else:
+            m = (col + 1) // 2
+            if m < 2:
+                continue
+            m1 = m % 2
+        jmax = m - step
+        for j in range(m1, jmax + 1, step):
+            top = {"sph": n - 3 - j, "tri": n - 2 - j, "disk": n - 2 - (j + 1) // 2}[kind]
+            if top >= 0:
+                total += top + 1
+    return total

@@ -1,0 +1,113 @@
+/*
+ * fasttransforms_amd.h -- C ABI of the MI355X (gfx950) Givens-rotation stage.
+ *
+ * A drop-in for the rotation path of FastTransforms' src/fasttransforms.h:
+ * the declarations below keep the reference's names, argument meaning and
+ * column-major layouts (A[i + n*j]).  Every computation runs on the GPU
+ * through hand-written HIP kernels; A may be a host pointer (staged through
+ * device memory for the call) or a device pointer (updated in place).
+ *
+ * The reference functions return void and validate nothing.  These keep the
+ * void signatures; invalid arguments leave A untouched and record an error
+ * that ft_gpu_last_error() returns (valid arguments never change results).
+ */
+#ifndef FASTTRANSFORMS_AMD_H
+#define FASTTRANSFORMS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fasttransforms.h:298.  The first three members are the reference's
+ * {double *s; double *c; int n;} (ftinternal.h:370-374), so RP->s, RP->c and
+ * RP->n are the host FP64 tables; device copies follow them. */
+typedef struct ft_rotation_plan_s ft_rotation_plan;
+
+/* fasttransforms.h:295 -- kept for link compatibility.  No computation here
+ * uses host threads, so the value is recorded and otherwise ignored. */
+void ft_set_num_threads(const int n);
+
+/* fasttransforms.h:301 (rotations.c:6-10) -- also frees the device copies. */
+void ft_destroy_rotation_plan(ft_rotation_plan *RP);
+
+/* fasttransforms.h:303 (rotations.c:116-133) -- spherical harmonics; tables
+ * bit-identical to the reference's. */
+ft_rotation_plan *ft_plan_rotsphere(const int n);
+
+/* fasttransforms.h:306, 308 (rotations.c:135-141) -- one vector with stride S,
+ * from order m2 down to m1 / from m1 up to m2. */
+void ft_kernel_sph_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+void ft_kernel_sph_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+
+/* fasttransforms.h:310 (rotations.c:143-160) -- Proriol (triangle). */
+ft_rotation_plan *ft_plan_rottriangle(const int n, const double alpha, const double beta, const double gamma);
+
+/* fasttransforms.h:313, 315 (rotations.c:162-168). */
+void ft_kernel_tri_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+void ft_kernel_tri_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+
+/* fasttransforms.h:317 (rotations.c:173-190) -- Zernike (disk). */
+ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double beta);
+
+/* fasttransforms.h:321, 323 (rotations.c:262-268). */
+void ft_kernel_disk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+
+/* fasttransforms.h:349-350 (drivers.c:58-92) -- the north-star entry points.
+ * A is n x M (M odd, M <= 2n-1), updated in place; B is the caller's n x M
+ * scratch, unused here (the GPU path needs no host scratch). */
+void ft_execute_sph_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
+void ft_execute_sph_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:355-356 (drivers.c:278-312) -- A is n x M, M <= n. */
+void ft_execute_tri_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
+void ft_execute_tri_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:358-359 (drivers.c:378-412) -- A is n x M, M odd, M <= 4n-3. */
+void ft_execute_disk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
+void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* ------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference).
+ * ---------------------------------------------------------------------- */
+
+/* Error side channel.  Codes are FT_GPU_*; reading clears the calling
+ * thread's last error. */
+enum {
+    FT_GPU_OK = 0,
+    FT_GPU_EINVAL = 1,     /* bad plan, n, M, m1/m2 or stride */
+    FT_GPU_ENOMEM = 2,     /* device allocation failed */
+    FT_GPU_EDEVICE = 3,    /* no usable HIP device / HIP runtime error */
+    FT_GPU_ETIMEOUT = 4,   /* a kernel's bounded wait expired (results invalid) */
+    FT_GPU_EPTR = 5        /* pointer not host or device memory of the current device */
+};
+int ft_gpu_last_error(void);
+const char *ft_gpu_error_string(int code);
+
+/* Precision of the device path: 0 = FP64 (default), 1 = FP32 storage and
+ * arithmetic with s/c rounded from the FP64 plan. */
+enum { FT_GPU_FP64 = 0, FT_GPU_FP32 = 1 };
+
+/* Device-resident execution on a HIP stream (stream = NULL: the null stream).
+ * kind: 0 sphere, 1 triangle, 2 disk; direction: 0 hi2lo, 1 lo2hi.
+ * dA is n x M column-major in device memory (double* for FP64, float* for
+ * FP32).  Asynchronous: returns after the launch; 0 on success or an FT_GPU_*
+ * code. */
+int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
+                   void *stream);
+
+/* Batched transforms of uneven degree in one launch.  plans[i] may differ in
+ * n (same kind and parameters); dA[i] is plans[i]->n x M[i] on the device.
+ * Every transform runs as one persistent, cost-ordered launch. */
+int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                         void *const *dA, const int *M, int count, void *stream);
+
+/* Tuning knobs of the pipelined kernel: sequences per wave (L), stage waves
+ * per workgroup (W), workgroups per CU.  0 keeps the built-in choice. */
+int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTTRANSFORMS_AMD_H */

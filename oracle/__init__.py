@@ -1,0 +1,217 @@
+"""ctypes access to the CPU checkers (test infrastructure, not product code).
+
+Only the test suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package.  The product library never does.
+
+* ``liboracle.so`` (built from ``ft_oracle.c``): the C restatement of the
+  reference's plans, kernels and executors, its ``__float128`` version and the
+  seeded valid-triangle fills.
+* ``_ref/libftref.so`` (built from the reference's own kernel and permute
+  sources plus ``ref_exec.c``): the reference's dispatched executors.  It is
+  built where the reference tree exists and travels with the repository;
+  ``have_ref()`` says whether it is present.
+
+Matrices are column-major: a numpy array of shape ``(n, M)`` in Fortran order
+(``A[i, j]`` is row i of column j), exactly the reference's ``A[i + n*j]``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+SPH, TRI, DISK = 0, 1, 2
+HI2LO, LO2HI = 0, 1
+KINDS = {"sph": SPH, "tri": TRI, "disk": DISK}
+
+_here = os.path.dirname(os.path.abspath(__file__))
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+_ref = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_here, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        lib = ctypes.CDLL(path)
+        lib.oracle_plan_size.restype = ctypes.c_long
+        lib.oracle_plan_size.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.oracle_plan_sph.argtypes = [ctypes.c_int, _dp, _dp]
+        lib.oracle_plan_tri.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.oracle_plan_disk.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        for name in ("sph", "tri", "disk"):
+            fn = getattr(lib, "oracle_execute_" + name)
+            fn.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
+            for d in ("hi2lo", "lo2hi"):
+                k = getattr(lib, "oracle_kernel_%s_%s" % (name, d))
+                k.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+        lib.oracle_execute_q.restype = ctypes.c_int
+        lib.oracle_execute_q.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        lib.oracle_rand_entry.restype = ctypes.c_double
+        lib.oracle_rand_entry.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        lib.oracle_valid_rows.restype = ctypes.c_int
+        lib.oracle_valid_rows.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.oracle_fill.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _dp]
+        lib.oracle_set_fused.argtypes = [ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _kind(kind):
+    return KINDS[kind] if isinstance(kind, str) else int(kind)
+
+
+def _check_matrix(A, n):
+    if A.dtype != np.float64 or not A.flags.f_contiguous or A.ndim != 2 or A.shape[0] != n:
+        raise ValueError("expected a Fortran-ordered float64 array with n rows")
+
+
+class Plan:
+    """FP64 sine/cosine tables of rotations.c:113-190, as the oracle builds them."""
+
+    def __init__(self, kind, n, alpha=0.0, beta=0.0, gamma=0.0):
+        lib = _load()
+        self.kind, self.n = _kind(kind), int(n)
+        self.alpha, self.beta, self.gamma = float(alpha), float(beta), float(gamma)
+        size = lib.oracle_plan_size(self.kind, self.n)
+        self.s = np.zeros(size)
+        self.c = np.zeros(size)
+        if self.kind == SPH:
+            lib.oracle_plan_sph(self.n, _ptr(self.s), _ptr(self.c))
+        elif self.kind == TRI:
+            lib.oracle_plan_tri(self.n, self.alpha, self.beta, self.gamma, _ptr(self.s), _ptr(self.c))
+        else:
+            lib.oracle_plan_disk(self.n, self.alpha, self.beta, _ptr(self.s), _ptr(self.c))
+
+    def index(self, l, m):
+        n = self.n
+        if self.kind == DISK:
+            return l + m * n - (m // 2) * ((m + 1) // 2)
+        return l + m * (2 * n + 1 - m) // 2
+
+
+def execute(plan, direction, A, M=None):
+    """ft_execute_<kind>_<direction> restated (in place on A)."""
+    lib = _load()
+    _check_matrix(A, plan.n)
+    M = A.shape[1] if M is None else M
+    fn = getattr(lib, "oracle_execute_" + ("sph", "tri", "disk")[plan.kind])
+    fn(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+def kernel(plan, direction, m1, m2, A, S=1):
+    """ft_kernel_<kind>_<direction> restated on a strided 1-D buffer (in place)."""
+    lib = _load()
+    if A.dtype != np.float64 or not A.flags.c_contiguous:
+        raise ValueError("expected a contiguous float64 buffer")
+    name = "oracle_kernel_%s_%s" % (("sph", "tri", "disk")[plan.kind], ("hi2lo", "lo2hi")[direction])
+    getattr(lib, name)(_ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2), _ptr(A), int(S))
+    return A
+
+
+def execute_q(plan, direction, A, hp_plan=False, M=None):
+    """The same executor in __float128 (rounded to FP64 at the end)."""
+    lib = _load()
+    _check_matrix(A, plan.n)
+    M = A.shape[1] if M is None else M
+    rc = lib.oracle_execute_q(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, plan.alpha,
+                              plan.beta, plan.gamma, 1 if hp_plan else 0, _ptr(A), int(M))
+    if rc != 0:
+        raise MemoryError("oracle_execute_q failed")
+    return A
+
+
+def set_fused(fused):
+    """Select the fused (SIMD-path) or unfused (scalar default build) rotation form."""
+    _load().oracle_set_fused(1 if fused else 0)
+
+
+def valid_rows(kind, n, j):
+    return _load().oracle_valid_rows(_kind(kind), int(n), int(j))
+
+
+def fill(kind, n, M, seed):
+    """Seeded U(-1,1) fill of each layout's valid triangle (zeros elsewhere)."""
+    A = np.zeros((n, M), order="F")
+    _load().oracle_fill(_kind(kind), int(n), int(M), ctypes.c_uint64(seed), _ptr(A))
+    return A
+
+
+def rand_entry(seed, k):
+    return _load().oracle_rand_entry(ctypes.c_uint64(seed), ctypes.c_uint64(k))
+
+
+# ---------------------------------------------------------------- reference --
+
+def have_ref():
+    return os.path.exists(os.path.join(_here, "_ref", "libftref.so"))
+
+
+def _load_ref():
+    global _ref
+    if _ref is None:
+        path = os.path.join(_here, "_ref", "libftref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/_ref/libftref.so is not built (run `make -C oracle ref`)")
+        lib = ctypes.CDLL(path)
+        lib.ref_execute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                    ctypes.c_int]
+        lib.ref_kernel_column.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _dp, ctypes.c_int]
+        lib.ref_select_isa.restype = ctypes.c_int
+        lib.ref_set_num_threads.argtypes = [ctypes.c_int]
+        lib.ref_max_threads.restype = ctypes.c_int
+        _ref = lib
+    return _ref
+
+
+REF_ISA_NAMES = {0: "default", 1: "SSE2", 2: "AVX", 3: "AVX_FMA", 4: "AVX512F"}
+
+
+def ref_isa():
+    return _load_ref().ref_select_isa()
+
+
+def ref_set_num_threads(t):
+    _load_ref().ref_set_num_threads(int(t))
+
+
+def ref_execute(plan, direction, A, M=None, isa=-1, B=None):
+    """The reference's ft_execute_* (its compiled kernels, dispatched as drivers.c does)."""
+    lib = _load_ref()
+    _check_matrix(A, plan.n)
+    M = A.shape[1] if M is None else M
+    if B is None:
+        B = np.zeros(plan.n * max(M, 1))
+    lib.ref_execute(plan.kind, int(direction), int(isa), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), _ptr(B),
+                    int(M))
+    return A
+
+
+def ref_kernel(plan, direction, m1, m2, A, S=1):
+    lib = _load_ref()
+    lib.ref_kernel_column(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2),
+                          _ptr(A), int(S))
+    return A
+
+
+def rel2(A, B):
+    """Relative 2-norm ||A-B|| / ||B|| (ftutilities_source.c:6-20)."""
+    d = np.linalg.norm((np.asarray(A) - np.asarray(B)).ravel())
+    b = np.linalg.norm(np.asarray(B).ravel())
+    return d / b if b else d
+
+
+def relinf(A, B):
+    """Relative inf-norm max|A-B| / max|B| (ftutilities_source.c:22-36)."""
+    d = np.max(np.abs(np.asarray(A) - np.asarray(B)))
+    b = np.max(np.abs(np.asarray(B)))
+    return d / b if b else d

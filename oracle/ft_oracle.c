@@ -1,0 +1,326 @@
+/*
+ * ft_oracle.c -- CPU oracle for FastTransforms' Givens-rotation stage.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  This file is the checker.  Only the
+ * test suite (tests/), the smoke check and the CPU-baseline leg of bench.py may
+ * load it; the product library under fasttransforms_amd/ never links or calls it.
+ *
+ * Every routine restates one piece of the reference (citations are into the
+ * FastTransforms source tree):
+ *
+ *   plans        src/rotations.c:113-133 (sphere), :143-160 (triangle),
+ *                :170-190 (disk, negative sines, index sd(l,m)).
+ *   kernels      src/rotations/rotations_default.c:31-71 (loop nests), with the
+ *                rotation written in the fused form the dispatched SIMD path
+ *                computes: fma(c,x,s*y) / fma(c,y,-(s*x)) for hi2lo and
+ *                fma(c,x,-(s*y)) / fma(c,y,s*x) for lo2hi
+ *                (src/rotations/rotations_AVX512F.c:4-15, src/rotations/rotations.h:37-40,82-85).
+ *                The file is compiled with -ffp-contract=off so only these fmas fuse.
+ *   executors    src/drivers.c:94-110 (sphere), :314-326 (triangle), :414-430 (disk):
+ *                orders 0-1 (sphere, disk) and 0 (triangle) are untouched.
+ *   fills        src/ftutilities.c:67-113 (valid triangle of each layout), but
+ *                seeded with a counter-based SplitMix64 instead of libc rand().
+ *   __float128   the same loop nests evaluated in quadruple precision, with the
+ *                sines/cosines either promoted from the FP64 plan or recomputed
+ *                from the closed forms in quadruple precision.
+ *
+ * Pinning (which check fails for which mistake) lives in tests/test_oracle.py:
+ * exact-rational rounding of the plan tables, s^2+c^2=1, orthogonality of the
+ * dense operator at small n, hi2lo o lo2hi = I, Legendre/Jacobi quadrature
+ * identities at small n, the reference's own compiled kernels (oracle/_ref),
+ * and the committed golden fixtures.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <quadmath.h>
+
+#define OR_SPH  0
+#define OR_TRI  1
+#define OR_DISK 2
+
+/* Packed-triangle index of rotations.c:113-114 and disk index of rotations.c:170-171. */
+static inline long idx_tri(int n, int l, int m) { return l + (long)m * (2 * n + 1 - m) / 2; }
+static inline long idx_disk(int n, int l, int m) { return l + (long)m * n - (long)(m / 2) * ((m + 1) / 2); }
+
+long oracle_plan_size(int kind, int n) {
+    return kind == OR_DISK ? (long)n * n : (long)n * (n + 1) / 2;
+}
+
+/* ---------------------------------------------------------------- plans -- */
+
+/* rotations.c:116-133.  The products are evaluated in int, exactly as the
+ * reference does, then converted; one correctly rounded division and sqrt. */
+void oracle_plan_sph(int n, double *s, double *c) {
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)((l + 1) * (l + 2));
+            double numc = (double)((2 * m + 2) * (2 * l + 2 * m + 5));
+            double den = (double)((l + 2 * m + 3) * (l + 2 * m + 4));
+            s[idx_tri(n, l, m)] = sqrt(nums / den);
+            c[idx_tri(n, l, m)] = sqrt(numc / den);
+        }
+}
+
+/* rotations.c:143-160.  C evaluates l+alpha+1 as ((double)l+alpha)+1 and so on;
+ * every sum below keeps the reference's left-to-right association. */
+void oracle_plan_tri(int n, double alpha, double beta, double gamma, double *s, double *c) {
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)(l + 1) * ((l + alpha) + 1);
+            double numc = ((((2 * m) + beta) + gamma) + 2) * (((((2 * l + 2 * m) + alpha) + beta) + gamma) + 4);
+            double den = ((((l + 2 * m) + beta) + gamma) + 3) * (((((l + 2 * m) + alpha) + beta) + gamma) + 3);
+            s[idx_tri(n, l, m)] = sqrt(nums / den);
+            c[idx_tri(n, l, m)] = sqrt(numc / den);
+        }
+}
+
+/* rotations.c:173-190: orders 0 <= m < 2n-1, rows l < n-(m+1)/2, sines negated. */
+void oracle_plan_disk(int n, double alpha, double beta, double *s, double *c) {
+    for (int m = 0; m < 2 * n - 1; m++)
+        for (int l = 0; l < n - (m + 1) / 2; l++) {
+            double nums = (double)(l + 1) * ((l + beta) + 1);
+            double numc = (((m + alpha) + 1)) * (((((2 * l + m) + alpha) + beta) + 3));
+            double den = (((l + m) + alpha) + 2) * ((((l + m) + alpha) + beta) + 2);
+            s[idx_disk(n, l, m)] = -sqrt(nums / den);
+            c[idx_disk(n, l, m)] = sqrt(numc / den);
+        }
+}
+
+/* -------------------------------------------------------------- kernels -- */
+
+/*
+ * Two arithmetic forms of one rotation.  Fused (default): the form the
+ * dispatched SIMD kernels compute with explicit FMA intrinsics.  Unfused: the
+ * scalar default kernel as the reference's own build compiles it (Make.inc
+ * gives rotations_default.c no -m flag, so C*X + S*Y is two roundings of the
+ * products and one of the sum).  oracle_set_fused(0) selects the latter so the
+ * loop nests can be checked bit for bit against the reference's compiled
+ * default executor.
+ */
+static int g_fused = 1;
+void oracle_set_fused(int fused) { g_fused = fused != 0; }
+
+static inline void givens(double S, double C, double *X, double *Y) {
+    double x = X[0], y = Y[0];
+    if (g_fused) {
+        X[0] = fma(C, x, S * y);
+        Y[0] = fma(C, y, -(S * x));
+    } else {
+        X[0] = C * x + S * y;
+        Y[0] = C * y - S * x;
+    }
+}
+
+static inline void givens_t(double S, double C, double *X, double *Y) {
+    double x = X[0], y = Y[0];
+    if (g_fused) {
+        X[0] = fma(C, x, -(S * y));
+        Y[0] = fma(C, y, S * x);
+    } else {
+        X[0] = C * x - S * y;
+        Y[0] = C * y + S * x;
+    }
+}
+
+/* rotations_default.c:31-36 */
+void oracle_kernel_sph_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m2 - 2; j >= m1; j -= 2)
+        for (int k = n - 3 - j; k >= 0; k--)
+            givens(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);
+}
+
+/* rotations_default.c:38-43 */
+void oracle_kernel_sph_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m1; j < m2; j += 2)
+        for (int k = 0; k <= n - 3 - j; k++)
+            givens_t(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);
+}
+
+/* rotations_default.c:45-50 */
+void oracle_kernel_tri_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m2 - 1; j >= m1; j--)
+        for (int k = n - 2 - j; k >= 0; k--)
+            givens(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
+}
+
+/* rotations_default.c:52-57 */
+void oracle_kernel_tri_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m1; j < m2; j++)
+        for (int k = 0; k <= n - 2 - j; k++)
+            givens_t(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
+}
+
+/* rotations_default.c:59-64 */
+void oracle_kernel_disk_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m2 - 2; j >= m1; j -= 2)
+        for (int k = n - 2 - (j + 1) / 2; k >= 0; k--)
+            givens(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
+}
+
+/* rotations_default.c:66-71 */
+void oracle_kernel_disk_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
+    for (int j = m1; j < m2; j += 2)
+        for (int k = 0; k <= n - 2 - (j + 1) / 2; k++)
+            givens_t(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
+}
+
+/* ------------------------------------------------------------ executors -- */
+/* Columns are independent, so the OpenMP loop over orders changes no bits. */
+
+/* drivers.c:94-110: orders 2..M/2, columns 2m-1 and 2m, down to order m%2.
+ * Written as a loop over the in-range columns j = 2m-1+t (order (j+1)/2): for
+ * the odd M the layout defines this is the same set of columns. */
+void oracle_execute_sph(int dir, const double *s, const double *c, int n, double *A, int M) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int j = M - 1; j >= 3; j--) {
+        int m = (j + 1) / 2;
+        double *col = A + (long)n * j;
+        if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
+        else          oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+    }
+}
+
+/* drivers.c:314-326: columns 1..M-1, column m from order m down to order 0. */
+void oracle_execute_tri(int dir, const double *s, const double *c, int n, double *A, int M) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int m = M - 1; m >= 1; m--) {
+        double *col = A + (long)n * m;
+        if (dir == 0) oracle_kernel_tri_hi2lo(s, c, n, 0, m, col, 1);
+        else          oracle_kernel_tri_lo2hi(s, c, n, 0, m, col, 1);
+    }
+}
+
+/* drivers.c:414-430: as the sphere, with the disk kernels. */
+void oracle_execute_disk(int dir, const double *s, const double *c, int n, double *A, int M) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int j = M - 1; j >= 3; j--) {
+        int m = (j + 1) / 2;
+        double *col = A + (long)n * j;
+        if (dir == 0) oracle_kernel_disk_hi2lo(s, c, n, m % 2, m, col, 1);
+        else          oracle_kernel_disk_lo2hi(s, c, n, m % 2, m, col, 1);
+    }
+}
+
+/* --------------------------------------------------------- __float128 -- */
+
+typedef __float128 q_t;
+
+static inline void givens_q(q_t S, q_t C, q_t *X, q_t *Y, int dir) {
+    q_t x = X[0], y = Y[0];
+    if (dir == 0) { X[0] = C * x + S * y; Y[0] = C * y - S * x; }
+    else          { X[0] = C * x - S * y; Y[0] = C * y + S * x; }
+}
+
+/* Quadruple-precision closed forms (same formulas as the FP64 plans). */
+static void plan_q(int kind, int n, double alpha, double beta, double gamma, q_t *s, q_t *c) {
+    q_t a = alpha, b = beta, g = gamma;
+    if (kind == OR_SPH) {
+        for (int m = 0; m < n; m++)
+            for (int l = 0; l < n - m; l++) {
+                q_t den = (q_t)(l + 2 * m + 3) * (q_t)(l + 2 * m + 4);
+                s[idx_tri(n, l, m)] = sqrtq((q_t)(l + 1) * (q_t)(l + 2) / den);
+                c[idx_tri(n, l, m)] = sqrtq((q_t)(2 * m + 2) * (q_t)(2 * l + 2 * m + 5) / den);
+            }
+    } else if (kind == OR_TRI) {
+        for (int m = 0; m < n; m++)
+            for (int l = 0; l < n - m; l++) {
+                q_t den = ((q_t)(l + 2 * m) + b + g + 3) * ((q_t)(l + 2 * m) + a + b + g + 3);
+                s[idx_tri(n, l, m)] = sqrtq((q_t)(l + 1) * ((q_t)l + a + 1) / den);
+                c[idx_tri(n, l, m)] = sqrtq(((q_t)(2 * m) + b + g + 2) * ((q_t)(2 * l + 2 * m) + a + b + g + 4) / den);
+            }
+    } else {
+        for (int m = 0; m < 2 * n - 1; m++)
+            for (int l = 0; l < n - (m + 1) / 2; l++) {
+                q_t den = ((q_t)(l + m) + a + 2) * ((q_t)(l + m) + a + b + 2);
+                s[idx_disk(n, l, m)] = -sqrtq((q_t)(l + 1) * ((q_t)l + b + 1) / den);
+                c[idx_disk(n, l, m)] = sqrtq(((q_t)m + a + 1) * ((q_t)(2 * l + m) + a + b + 3) / den);
+            }
+    }
+}
+
+/* One column in quadruple precision: the loop nests of rotations_default.c:31-71. */
+static void column_q(int kind, int dir, const q_t *s, const q_t *c, int n, int m1, int m2, q_t *A) {
+    if (kind == OR_SPH) {
+        if (dir == 0) { for (int j = m2 - 2; j >= m1; j -= 2) for (int k = n - 3 - j; k >= 0; k--) givens_q(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + k, A + k + 2, 0); }
+        else          { for (int j = m1; j < m2; j += 2) for (int k = 0; k <= n - 3 - j; k++) givens_q(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + k, A + k + 2, 1); }
+    } else if (kind == OR_TRI) {
+        if (dir == 0) { for (int j = m2 - 1; j >= m1; j--) for (int k = n - 2 - j; k >= 0; k--) givens_q(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + k, A + k + 1, 0); }
+        else          { for (int j = m1; j < m2; j++) for (int k = 0; k <= n - 2 - j; k++) givens_q(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + k, A + k + 1, 1); }
+    } else {
+        if (dir == 0) { for (int j = m2 - 2; j >= m1; j -= 2) for (int k = n - 2 - (j + 1) / 2; k >= 0; k--) givens_q(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + k, A + k + 1, 0); }
+        else          { for (int j = m1; j < m2; j += 2) for (int k = 0; k <= n - 2 - (j + 1) / 2; k++) givens_q(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + k, A + k + 1, 1); }
+    }
+}
+
+/*
+ * Whole-matrix executor in quadruple precision.  A (n x M, column-major FP64)
+ * is promoted, transformed and rounded back.  hp_plan = 0 promotes the FP64
+ * tables s, c (isolates the rounding of the rotation chain); hp_plan = 1
+ * recomputes the tables from the closed forms in quadruple precision (the
+ * exact transform, up to 2^-113).
+ */
+int oracle_execute_q(int kind, int dir, const double *s, const double *c, int n,
+                     double alpha, double beta, double gamma, int hp_plan, double *A, int M) {
+    long ns = oracle_plan_size(kind, n);
+    q_t *sq = malloc(ns * sizeof(q_t)), *cq = malloc(ns * sizeof(q_t));
+    if (!sq || !cq) { free(sq); free(cq); return -1; }
+    if (hp_plan) plan_q(kind, n, alpha, beta, gamma, sq, cq);
+    else for (long i = 0; i < ns; i++) { sq[i] = s[i]; cq[i] = c[i]; }
+    int first = kind == OR_TRI ? 1 : 3;
+    int err = 0;
+    #pragma omp parallel
+    {
+        q_t *col = malloc((size_t)n * sizeof(q_t));
+        if (!col) {
+            #pragma omp atomic write
+            err = -1;
+        }
+        #pragma omp for schedule(dynamic, 1)
+        for (int j = M - 1; j >= first; j--) {
+            if (!col) continue;
+            int m = kind == OR_TRI ? j : (j + 1) / 2;
+            int m1 = kind == OR_TRI ? 0 : m % 2;
+            double *a = A + (long)n * j;
+            for (int i = 0; i < n; i++) col[i] = a[i];
+            column_q(kind, dir, sq, cq, n, m1, m, col);
+            for (int i = 0; i < n; i++) a[i] = (double)col[i];
+        }
+        free(col);
+    }
+    free(sq);
+    free(cq);
+    return err;
+}
+
+/* ---------------------------------------------------------------- fills -- */
+
+/* Counter-based SplitMix64: entry k of stream `seed` is the k-th SplitMix64
+ * output, so any traversal order (and numpy vectorisation) gives the same
+ * matrix.  u in [0,1) from the top 53 bits; the fill value is 2u-1. */
+static inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+double oracle_rand_entry(uint64_t seed, uint64_t k) {
+    uint64_t z = mix64(seed + (k + 1) * 0x9E3779B97F4A7C15ULL);
+    return 2.0 * ((double)(z >> 11) * 0x1p-53) - 1.0;
+}
+
+/* Valid rows of column j (ftutilities.c:75-113 with M at its full value). */
+int oracle_valid_rows(int kind, int n, int j) {
+    if (kind == OR_SPH) return n - (j + 1) / 2;
+    if (kind == OR_TRI) return n - j;
+    return n - (j + 3) / 4;
+}
+
+void oracle_fill(int kind, int n, int M, uint64_t seed, double *A) {
+    for (int j = 0; j < M; j++) {
+        int r = oracle_valid_rows(kind, n, j);
+        for (int i = 0; i < n; i++)
+            A[i + (long)n * j] = i < r ? oracle_rand_entry(seed, (uint64_t)i + (uint64_t)n * j) : 0.0;
+    }
+}
