@@ -153,10 +153,10 @@ def _pointer(A, n, M, dtype=np.float64):
         if A.numel() < n * M:
             raise ValueError("tensor has fewer than n*M elements")
         if A.dim() == 2:
-            if tuple(A.shape) == (n, M) and A.stride() != (1, n):
-                raise ValueError("an (n, M) tensor must be column-major (stride (1, n))")
-            if tuple(A.shape) == (M, n) and not A.is_contiguous():
-                raise ValueError("an (M, n) tensor must be contiguous")
+            ok = (tuple(A.shape) == (M, n) and A.is_contiguous()) or \
+                 (tuple(A.shape) == (n, M) and A.stride() == (1, n))
+            if not ok:
+                raise ValueError("expected a contiguous (M, n) tensor or a column-major (n, M) one")
         elif not A.is_contiguous():
             raise ValueError("tensor must be contiguous")
         return _vp(A.data_ptr())

@@ -38,6 +38,7 @@ const char *ft_gpu_error_string(int code) {
         case FT_GPU_EDEVICE: return "HIP device or runtime error";
         case FT_GPU_ETIMEOUT: return "kernel wait expired; results are invalid";
         case FT_GPU_EPTR: return "pointer is not host memory or device memory of the current device";
+        case FT_GPU_EINCOMPLETE: return "a launch ended with work not done; results are invalid";
         default: return "unknown error";
     }
 }
@@ -97,7 +98,11 @@ typedef struct {
     long long off[64];
     int jmax[64];
     int crit;      /* ordering key: longer chains first */
+    double work;   /* rotations of all its columns */
+    int xcd;       /* XCD whose workgroups run every task of the group */
 } ft_hgroup;
+
+static double column_rotations(int kind, int n, int jmin, int jmax);
 
 /* Group up to 64 consecutive columns of `cols` (already sorted so that
  * columns sharing sequences are adjacent) with equal jmin and A. */
@@ -111,7 +116,9 @@ static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, 
         int cnt = 0;
         ft_hgroup *h = &gs[G];
         int jmaxmax = -1, jmaxmin = 1 << 30;
+        h->work = 0;
         while (i < ncols && cnt < 64 && (cnt == 0 || (cols[i].jmin == cols[i - 1].jmin && cols[i].A == cols[i - 1].A))) {
+            h->work += column_rotations(kind, cols[i].n, cols[i].jmin, cols[i].jmax);
             h->off[cnt] = cols[i].off;
             h->jmax[cnt] = cols[i].jmax;
             if (cols[i].jmax > jmaxmax) jmaxmax = cols[i].jmax;
@@ -159,19 +166,43 @@ static double column_rotations(int kind, int n, int jmin, int jmax) {
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 /*
- * Upload groups, lane maps, the task order and a zeroed workspace.  Tasks run
- * pass by pass (every group's pass p before any group's pass p+1), longest
- * chains first within a pass, so a task's producer is always handed out
- * before it.
+ * Upload groups, lane maps, the task order and a zeroed workspace.
+ *
+ * Every group is handed to one XCD (largest work first, onto the least loaded
+ * XCD), because its tasks pass rows to each other through A and only the
+ * workgroups of one XCD share an L2.  Each XCD's list runs pass by pass
+ * (every group's pass p before any group's pass p+1), longest chains first
+ * within a pass, so a task's producer is always handed out before it.
  */
 static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, const void *table, int ntab) {
+    int nxcd = 1;
+    if (ftamd_hip_xcd_count(&nxcd) != FTAMD_OK || nxcd < 1) nxcd = 1;
     int maxpass = 0;
     for (int g = 0; g < G; g++)
         if (gs[g].g.npass > maxpass) maxpass = gs[g].g.npass;
     int *order = malloc(sizeof(int) * (size_t)(G ? G : 1));
     if (!order) return FTAMD_ENOMEM;
+    /* LPT assignment of groups to XCDs */
     for (int g = 0; g < G; g++) order[g] = g;
-    for (int a = 1; a < G; a++) { /* insertion sort by decreasing crit */
+    for (int a = 1; a < G; a++) {
+        int v = order[a], b = a - 1;
+        while (b >= 0 && gs[order[b]].work < gs[v].work) {
+            order[b + 1] = order[b];
+            b--;
+        }
+        order[b + 1] = v;
+    }
+    double load[FTAMD_MAX_XCD] = {0};
+    for (int a = 0; a < G; a++) {
+        int best = 0;
+        for (int x = 1; x < nxcd; x++)
+            if (load[x] < load[best]) best = x;
+        gs[order[a]].xcd = best;
+        load[best] += gs[order[a]].work;
+    }
+    /* within an XCD: pass-major, longest chains first */
+    for (int g = 0; g < G; g++) order[g] = g;
+    for (int a = 1; a < G; a++) {
         int v = order[a], b = a - 1;
         while (b >= 0 && gs[order[b]].crit < gs[v].crit) {
             order[b + 1] = order[b];
@@ -182,29 +213,35 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
     int T = 0;
     for (int g = 0; g < G; g++) T += gs[g].g.npass;
     int *tasks = malloc(sizeof(int) * 2 * (size_t)(T ? T : 1));
+    int xoff[FTAMD_MAX_XCD + 1];
     if (!tasks) {
         free(order);
         return FTAMD_ENOMEM;
     }
     int t = 0;
-    for (int p = 0; p < maxpass; p++)
-        for (int a = 0; a < G; a++) {
-            int g = order[a];
-            if (p < gs[g].g.npass) {
-                tasks[2 * t] = g;
-                tasks[2 * t + 1] = p;
-                t++;
+    for (int x = 0; x < nxcd; x++) {
+        xoff[x] = t;
+        for (int p = 0; p < maxpass; p++)
+            for (int a = 0; a < G; a++) {
+                int g = order[a];
+                if (gs[g].xcd == x && p < gs[g].g.npass) {
+                    tasks[2 * t] = g;
+                    tasks[2 * t + 1] = p;
+                    t++;
+                }
             }
-        }
+    }
+    xoff[nxcd] = t;
     free(order);
 
     size_t o_groups = 0;
     size_t o_off = align256(o_groups + sizeof(FtamdGroup) * (size_t)G);
     size_t o_jmax = align256(o_off + sizeof(long long) * 64 * (size_t)G);
     size_t o_tasks = align256(o_jmax + sizeof(int) * 64 * (size_t)G);
-    size_t o_prog = align256(o_tasks + sizeof(int) * 2 * (size_t)T);
+    size_t o_xoff = align256(o_tasks + sizeof(int) * 2 * (size_t)T);
+    size_t o_prog = align256(o_xoff + sizeof(int) * (FTAMD_MAX_XCD + 1));
     size_t o_ctl = align256(o_prog + sizeof(int) * (size_t)nprog);
-    size_t total = align256(o_ctl + 4 * sizeof(int));
+    size_t total = align256(o_ctl + (FTAMD_MAX_XCD + 4) * sizeof(int));
     char *host = calloc(1, total);
     if (!host) {
         free(tasks);
@@ -216,6 +253,7 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
         memcpy(host + o_jmax + sizeof(int) * 64 * g, gs[g].jmax, sizeof(int) * 64);
     }
     memcpy(host + o_tasks, tasks, sizeof(int) * 2 * (size_t)T);
+    memcpy(host + o_xoff, xoff, sizeof(int) * (size_t)(nxcd + 1));
     for (int i = 0; i < nprog; i++) ((int *)(host + o_prog))[i] = FTAMD_PROG_NONE;
     free(tasks);
     void *d = NULL;
@@ -236,14 +274,17 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
     a->lane_off = (const long long *)(db + o_off);
     a->lane_jmax = (const int *)(db + o_jmax);
     a->tasks = (const int *)(db + o_tasks);
+    a->xcd_off = (const int *)(db + o_xoff);
     a->ntasks = T;
+    a->nxcd = nxcd;
     a->sc = table;
     a->ntab = ntab;
     a->prog = (int *)(db + o_prog);
     a->nprog = nprog;
     a->ticket = (unsigned *)(db + o_ctl);
-    a->done = (unsigned *)(db + o_ctl + sizeof(int));
-    a->err = (int *)(db + o_ctl + 2 * sizeof(int));
+    a->done = (unsigned *)(db + o_ctl + FTAMD_MAX_XCD * sizeof(int));
+    a->completed = (unsigned *)(db + o_ctl + (FTAMD_MAX_XCD + 1) * sizeof(int));
+    a->err = (int *)(db + o_ctl + (FTAMD_MAX_XCD + 2) * sizeof(int));
     return FTAMD_OK;
 }
 
@@ -352,7 +393,7 @@ static int check_kernel_error(ft_schedule *S) {
         int zero = 0;
         ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), NULL);
         ftamd_hip_sync(NULL);
-        return FTAMD_ETIMEOUT;
+        return (err & FTAMD_ERR_TIMEOUT) ? FTAMD_ETIMEOUT : FTAMD_EINCOMPLETE;
     }
     return FTAMD_OK;
 }

@@ -12,9 +12,13 @@ extern "C" {
 #endif
 
 enum { FTAMD_SPH = 0, FTAMD_TRI = 1, FTAMD_DISK = 2 };
-enum { FTAMD_OK = 0, FTAMD_EINVAL = 1, FTAMD_ENOMEM = 2, FTAMD_EDEVICE = 3, FTAMD_ETIMEOUT = 4, FTAMD_EPTR = 5 };
+enum { FTAMD_OK = 0, FTAMD_EINVAL = 1, FTAMD_ENOMEM = 2, FTAMD_EDEVICE = 3, FTAMD_ETIMEOUT = 4, FTAMD_EPTR = 5,
+       FTAMD_EINCOMPLETE = 6 };
 enum { FTAMD_PTR_HOST = 0, FTAMD_PTR_DEVICE = 1 };
-enum { FTAMD_ERR_TIMEOUT = 1 };
+enum { FTAMD_ERR_TIMEOUT = 1, FTAMD_ERR_INCOMPLETE = 2 };
+
+/* Largest number of XCDs (each with its own L2) a device may report. */
+#define FTAMD_MAX_XCD 16
 
 /* "No row is final yet" value of a progress slot (rows >= value are final). */
 #define FTAMD_PROG_NONE 0x7fffffff
@@ -44,18 +48,22 @@ typedef struct {
     const FtamdGroup *groups;
     const long long *lane_off;  /* [groups*64] element offset of the lane's column, -1 if none */
     const int *lane_jmax;       /* [groups*64] largest j the lane's column needs, -1 if none */
-    const int *tasks;           /* [2*ntasks] (group, pass), producers before consumers */
+    const int *tasks;           /* [2*ntasks] (group, pass): per XCD, producers before consumers */
+    const int *xcd_off;         /* [nxcd+1] first task of each XCD's list */
     int ntasks;
+    int nxcd;
     const void *sc;             /* interleaved (s, c) table, entry 0 (padding on both sides) */
     int ntab;                   /* n the table was built for (index function) */
-    unsigned *ticket;           /* zero at launch; reset by the kernel */
+    unsigned *ticket;           /* [nxcd] zero at launch; reset by the kernel */
     unsigned *done;             /* zero at launch; reset by the kernel */
+    unsigned *completed;        /* zero at launch; reset by the kernel */
     int *prog;                  /* FTAMD_PROG_NONE at launch; reset by the kernel */
     int nprog;
     int *err;                   /* FTAMD_ERR_* bits */
 } FtamdLaunch;
 
 int ftamd_hip_device(int *device, int *num_cu);
+int ftamd_hip_xcd_count(int *nxcd);
 int ftamd_hip_pointer_kind(const void *ptr, int *kind);
 int ftamd_hip_malloc(void **ptr, size_t bytes);
 int ftamd_hip_free(void *ptr);

@@ -67,15 +67,41 @@ __device__ __forceinline__ bool lds_wait_ge(const int *flag, int target, const i
     return false;
 }
 
+// Inter-task hand-off through A.  The rows one task passes to the next are
+// stored and loaded with device-scope relaxed atomics (sc1 global stores and
+// loads), which are coherent per location across CUs and L2s without any
+// L2 write-back or invalidation; the producer waits until its row stores are
+// acknowledged (s_waitcnt vmcnt(0)) before publishing its progress with a
+// device-scope atomic store, and the consumer polls that with device-scope
+// atomic loads.  Plain cached accesses with a release/acquire pair would need
+// buffer_wbl2 / buffer_inv of the whole L2 on every tile.  Chained tasks still
+// run on one XCD, which keeps their traffic in one L2 / MALL slice.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
+__device__ __forceinline__ void publish(int *prog, int value) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    wait_stores();
+    __hip_atomic_store(prog, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until rows >= target are final.
 __device__ __forceinline__ bool global_wait_le(const int *prog, int target, int *known, const int *abort_flag) {
-    if (*known <= target) return true;
-    for (int it = 0; it < kSpinLimit; ++it) {
-        int v = uniform(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-        if (v <= target) { *known = v; return true; }
-        if (uniform(__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))) return false;
-        __builtin_amdgcn_s_sleep(2);
+    if (*known > target) {
+        int it = 0;
+        for (;; ++it) {
+            int v = uniform(__hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (v <= target) {
+                *known = v;
+                break;
+            }
+            if (it >= kSpinLimit ||
+                uniform(__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+                return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
-    return false;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return true;
 }
 
 // ------------------------------------------------------------------ kernel --
@@ -158,20 +184,33 @@ struct Pipeline {
     // Row v of this lane's column (0 if outside [lo_valid, n-1]).
     __device__ static T load_row(const Stage &st, int v, int lo_valid) {
         if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return T(0);
-        return st.col[(long long)real_row(st.n, v) * st.rs];
+        return __hip_atomic_load(st.col + (long long)real_row(st.n, v) * st.rs, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ static void store_row(const Stage &st, int v, int lo_valid, T val) {
         if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return;
-        st.col[(long long)real_row(st.n, v) * st.rs] = val;
+        __hip_atomic_store(st.col + (long long)real_row(st.n, v) * st.rs, val, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // XCD (accelerator complex die) this workgroup runs on.  Each XCD has its
+    // own L2; a group's chained tasks are all handed to one XCD so that the
+    // rows they pass through A meet in one L2.
+    __device__ static int xcd_id(int nxcd) {
+        const int id = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));  // HW_REG_XCC_ID[3:0]
+        return nxcd > 1 ? id % nxcd : 0;
     }
 
     __device__ static void run(const FtamdLaunch &a, Shared &sh) {
         const int lane = threadIdx.x & (kLanes - 1);
         const int w = uniform(threadIdx.x / kLanes);
+        const int x = uniform(xcd_id(a.nxcd));
+        const int first = uniform(a.xcd_off[x]);
+        const int count = uniform(a.xcd_off[x + 1]) - first;
         for (;;) {
             __syncthreads();
             if (threadIdx.x == 0) {
-                sh.task = (int)atomicAdd(a.ticket, 1u);
+                sh.task = (int)atomicAdd(a.ticket + x, 1u);
                 sh.abort_flag = 0;
             }
             if (threadIdx.x < W) {
@@ -179,21 +218,28 @@ struct Pipeline {
                 sh.consumed[threadIdx.x] = 0;
             }
             __syncthreads();
-            const int task = uniform(sh.task);
-            if (task >= a.ntasks) break;
+            const int t = uniform(sh.task);
+            if (t >= count) break;
+            const int task = first + t;
             const int g = uniform(a.tasks[2 * task]);
             const int p = uniform(a.tasks[2 * task + 1]);
             run_stage(a, sh, g, p, w, lane);
+            __syncthreads();
+            if (threadIdx.x == 0) atomicAdd(a.completed, 1u);
         }
-        // Last workgroup out resets the ticket and the progress slots so the
-        // next launch (or graph replay) starts clean.
+        // Last workgroup out checks that every task ran and resets the tickets
+        // and progress slots so the next launch (or graph replay) starts clean.
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned d = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (d == gridDim.x - 1) {
+                if (__hip_atomic_load(a.completed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != (unsigned)a.ntasks)
+                    atomicOr(a.err, FTAMD_ERR_INCOMPLETE);
                 for (int i = 0; i < a.nprog; ++i)
                     __hip_atomic_store(a.prog + i, FTAMD_PROG_NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int i = 0; i < a.nxcd; ++i)
+                    __hip_atomic_store(a.ticket + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.completed, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(a.done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -286,7 +332,7 @@ struct Pipeline {
                 if (++t >= st.ntile) break;
             }
         }
-        if (is_last && prog_mine) __hip_atomic_store(prog_mine, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (is_last && prog_mine) publish(prog_mine, -1);
     }
 
     template <int PH>
@@ -326,8 +372,7 @@ struct Pipeline {
             if (is_last) {
 #pragma unroll
                 for (int i = 0; i < L; ++i) store_row(st, k + D * L - i, lo_valid, x[slot<PH>(i)]);
-                if (prog_mine)
-                    __hip_atomic_store(prog_mine, k + (D - 1) * L + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (prog_mine) publish(prog_mine, k + (D - 1) * L + 1);
             } else {
                 if (e >= R && !lds_wait_ge(&sh.consumed[w], e - R + 1, &sh.abort_flag)) { fail(a, sh); return false; }
                 T *dst = &sh.ring[w][e % R][0][lane];

@@ -47,6 +47,19 @@ namespace {
 
 thread_local int g_hip_err = 0;
 
+// Records the XCC_ID of every workgroup (to count the device's XCDs).
+__global__ void xcd_probe(int *out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));
+}
+
+// Diagnostics: full XCC_ID and HW_ID registers of every workgroup.
+__global__ void hwid_probe(int *xcc, int *hw) {
+    if (threadIdx.x == 0) {
+        xcc[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((32 - 1) << 11));
+        hw[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((4) | (0 << 6) | ((32 - 1) << 11));
+    }
+}
+
 int map_err(hipError_t e) {
     if (e == hipSuccess) return FTAMD_OK;
     g_hip_err = (int)e;
@@ -73,6 +86,48 @@ int ftamd_hip_device(int *device, int *num_cu) {
         *num_cu = cu;
     }
     return FTAMD_OK;
+}
+
+// Number of XCDs, from the XCC_IDs a spread of workgroups reports (cached
+// per process; one device).
+int ftamd_hip_xcd_count(int *nxcd) {
+    static int cached = 0;
+    if (cached > 0) {
+        *nxcd = cached;
+        return FTAMD_OK;
+    }
+    const int blocks = 4096;
+    int *d = nullptr;
+    hipError_t e = hipMalloc(&d, blocks * sizeof(int));
+    if (e != hipSuccess) return map_err(e);
+    hipLaunchKernelGGL(xcd_probe, dim3(blocks), dim3(64), 0, 0, d);
+    int *h = (int *)malloc(blocks * sizeof(int));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(h, d, blocks * sizeof(int), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess || !h) {
+        free(h);
+        return map_err(e != hipSuccess ? e : hipErrorOutOfMemory);
+    }
+    int mx = 0;
+    for (int i = 0; i < blocks; ++i) mx = h[i] > mx ? h[i] : mx;
+    free(h);
+    cached = mx + 1 > FTAMD_MAX_XCD ? FTAMD_MAX_XCD : mx + 1;
+    *nxcd = cached;
+    return FTAMD_OK;
+}
+
+// Diagnostics: XCC_ID and HW_ID (raw registers) of `blocks` workgroups.
+int ftamd_hip_probe_ids(int *xcc, int *hw, int blocks) {
+    int *d = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * blocks * sizeof(int));
+    if (e != hipSuccess) return map_err(e);
+    hipLaunchKernelGGL(hwid_probe, dim3(blocks), dim3(64), 0, 0, d, d + blocks);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(xcc, d, blocks * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hw, d + blocks, blocks * sizeof(int), hipMemcpyDeviceToHost);
+    hipFree(d);
+    return map_err(e);
 }
 
 int ftamd_hip_pointer_kind(const void *ptr, int *kind) {

@@ -79,7 +79,8 @@ enum {
     FT_GPU_ENOMEM = 2,     /* device allocation failed */
     FT_GPU_EDEVICE = 3,    /* no usable HIP device / HIP runtime error */
     FT_GPU_ETIMEOUT = 4,   /* a kernel's bounded wait expired (results invalid) */
-    FT_GPU_EPTR = 5        /* pointer not host or device memory of the current device */
+    FT_GPU_EPTR = 5,       /* pointer not host or device memory of the current device */
+    FT_GPU_EINCOMPLETE = 6 /* a launch ended with work not done (results invalid) */
 };
 int ft_gpu_last_error(void);
 const char *ft_gpu_error_string(int code);
