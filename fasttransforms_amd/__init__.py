@@ -246,6 +246,33 @@ def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None):
     return A
 
 
+def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream=None):
+    """ft_gpu_execute_batch: many transforms of one kind and uneven degree in
+    one persistent launch.  ``tensors[i]`` is a CUDA tensor holding the
+    column-major plans[i].n x Ms[i] matrix."""
+    import torch
+    direction = _DIR.get(direction, direction)
+    count = len(plans)
+    if len(tensors) != count:
+        raise ValueError("one tensor per plan")
+    if count == 0:
+        return tensors
+    kind = plans[0].kind
+    if Ms is None:
+        Ms = [_M_of(T, P.n, None) for P, T in zip(plans, tensors)]
+    dtype = np.float64 if precision == FP64 else np.float32
+    ptrs = (_vp * count)(*[_pointer(T, P.n, M, dtype).value for P, T, M in zip(plans, tensors, Ms)])
+    pls = (_vp * count)(*[P.ptr for P in plans])
+    ms = (ctypes.c_int * count)(*[int(M) for M in Ms])
+    if stream is None:
+        stream = torch.cuda.current_stream(tensors[0].device)
+    rc = _lib.ft_gpu_execute_batch(ctypes.cast(pls, _vp), kind, int(direction), int(precision),
+                                   ctypes.cast(ptrs, _vp), ctypes.cast(ms, _vp), count, _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_batch: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return tensors
+
+
 def valid_rows(kind, n, j):
     """Valid rows of column j in each layout (src/ftutilities.c:67-113)."""
     kind = _KIND.get(kind, kind)

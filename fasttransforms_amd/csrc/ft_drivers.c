@@ -380,7 +380,13 @@ static int retarget(ft_schedule *S, void *dA) {
     return rc;
 }
 
+/* Diagnostics: device buffer that instrumented builds fill with a timeline
+ * (8 words per task and stage wave); NULL disables. */
+static void *g_trace = NULL;
+void ftamd_set_trace(void *dev_buffer) { g_trace = dev_buffer; }
+
 static int launch(ft_schedule *S, int precision, int kind, int dir, void *stream) {
+    S->launch.trace = g_trace;
     return ftamd_hip_launch(&S->launch, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S->ntasks), stream);
 }
 
@@ -593,15 +599,109 @@ void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2
 
 /* ------------------------------------------------------------ batch -- */
 
+/*
+ * Many transforms of one kind (and one set of plan parameters) but uneven
+ * degree, in one persistent launch.  The sines and cosines s(l,m), c(l,m) are
+ * closed forms in (l, m) only (rotations.c:122-126, 149-153, 179-183), so the
+ * table of the largest plan serves every transform; only its packed index
+ * uses that plan's n.  Each transform contributes its own groups; all tasks
+ * go into the per-XCD, pass-ordered lists, so small transforms fill in around
+ * the long chains of the large ones.  The schedule of the last distinct
+ * argument list is cached.
+ */
+typedef struct {
+    pthread_mutex_t lock;
+    ft_schedule *sched;
+    int count, kind, dir, precision, L, W;
+    unsigned long long key;
+} ft_batch_cache;
+
+static ft_batch_cache g_batch = {PTHREAD_MUTEX_INITIALIZER, NULL, 0, 0, 0, 0, 0, 0, 0};
+
+static unsigned long long batch_key(const ft_rotation_plan *const *plans, void *const *dA, const int *M, int count) {
+    unsigned long long h = 1469598103934665603ULL;
+    for (int i = 0; i < count; i++) {
+        unsigned long long v[3] = {(unsigned long long)(size_t)plans[i], (unsigned long long)(size_t)dA[i],
+                                   (unsigned long long)(unsigned)M[i]};
+        for (int k = 0; k < 3; k++) {
+            h ^= v[k];
+            h *= 1099511628211ULL;
+        }
+    }
+    return h;
+}
+
 int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                          void *const *dA, const int *M, int count, void *stream) {
-    (void)plans;
-    (void)kind;
-    (void)direction;
-    (void)precision;
-    (void)dA;
-    (void)M;
-    (void)count;
-    (void)stream;
-    return FT_GPU_EINVAL;
+    if (!plans || !dA || !M || count < 0 || (direction != 0 && direction != 1) || (precision != 0 && precision != 1))
+        return FT_GPU_EINVAL;
+    if (count == 0) return FT_GPU_OK;
+    ft_rotation_plan *big = NULL;
+    for (int i = 0; i < count; i++) {
+        const ft_rotation_plan *P = plans[i];
+        if (!P || !dA[i] || P->kind != kind || !valid_M(kind, P->n, M[i])) return FT_GPU_EINVAL;
+        if (P->alpha != plans[0]->alpha || P->beta != plans[0]->beta || P->gamma != plans[0]->gamma)
+            return FT_GPU_EINVAL;
+        if (!big || P->n > big->n) big = (ft_rotation_plan *)P;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) return rc;
+    pthread_mutex_lock(&big->lock);
+    rc = ftamd_plan_upload(big, precision);
+    pthread_mutex_unlock(&big->lock);
+    if (rc != FTAMD_OK) return rc;
+    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W();
+    unsigned long long key = batch_key(plans, dA, M, count);
+    pthread_mutex_lock(&g_batch.lock);
+    ft_schedule *S = g_batch.sched;
+    if (!S || g_batch.key != key || g_batch.count != count || g_batch.kind != kind || g_batch.dir != direction ||
+        g_batch.precision != precision || g_batch.L != L || g_batch.W != W) {
+        if (S) {
+            ftamd_hip_sync(NULL);
+            ftamd_hip_free(S->d_mem);
+            free(S);
+            g_batch.sched = S = NULL;
+        }
+        long total = 0;
+        for (int i = 0; i < count; i++) total += M[i];
+        ft_col *cols = malloc(sizeof(ft_col) * (size_t)(total > 0 ? total : 1));
+        int ncols = 0;
+        rc = cols ? FTAMD_OK : FTAMD_ENOMEM;
+        for (int i = 0; rc == FTAMD_OK && i < count; i++) {
+            ft_col *one = NULL;
+            int k = 0;
+            rc = matrix_columns(kind, plans[i]->n, M[i], dA[i], &one, &k);
+            if (rc == FTAMD_OK) memcpy(cols + ncols, one, sizeof(ft_col) * (size_t)k);
+            ncols += k;
+            free(one);
+        }
+        if (rc == FTAMD_OK) {
+            S = calloc(1, sizeof(ft_schedule));
+            rc = S ? FTAMD_OK : FTAMD_ENOMEM;
+        }
+        if (rc == FTAMD_OK) {
+            S->dir = direction;
+            S->precision = precision;
+            S->L = L;
+            S->W = W;
+            rc = build_from_columns(S, kind, direction, L, W, cols, ncols, ftamd_plan_table(big, precision), big->n);
+        }
+        free(cols);
+        if (rc != FTAMD_OK) {
+            free(S);
+            pthread_mutex_unlock(&g_batch.lock);
+            return rc;
+        }
+        g_batch.sched = S;
+        g_batch.key = key;
+        g_batch.count = count;
+        g_batch.kind = kind;
+        g_batch.dir = direction;
+        g_batch.precision = precision;
+        g_batch.L = L;
+        g_batch.W = W;
+    }
+    rc = launch(S, precision, kind, direction, stream);
+    pthread_mutex_unlock(&g_batch.lock);
+    return rc;
 }

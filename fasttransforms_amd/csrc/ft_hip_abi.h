@@ -60,6 +60,7 @@ typedef struct {
     int *prog;                  /* FTAMD_PROG_NONE at launch; reset by the kernel */
     int nprog;
     int *err;                   /* FTAMD_ERR_* bits */
+    unsigned long long *trace;  /* diagnostic timeline (instrumented builds only), else NULL */
 } FtamdLaunch;
 
 int ftamd_hip_device(int *device, int *num_cu);

@@ -12,6 +12,18 @@ namespace ftamd {
 
 constexpr int kLanes = 64;
 
+// Optional timeline instrumentation (-DFTAMD_TRACE, diagnostic builds only):
+// per stage wave and task, start/end wall clock, tiles, and the shader-clock
+// cycles spent waiting on the LDS ring, on the previous task's progress and on
+// its own row stores.  The product build compiles all of it away.
+#ifdef FTAMD_TRACE
+#define FT_TR_BEGIN() const unsigned long long ft_tr0 = __builtin_amdgcn_s_memtime()
+#define FT_TR_END(slot) sh.tr[w][slot] += __builtin_amdgcn_s_memtime() - ft_tr0
+#else
+#define FT_TR_BEGIN() (void)0
+#define FT_TR_END(slot) (void)0
+#endif
+
 // ---------------------------------------------------------------- geometry --
 // top(j): largest row k of sequence j (real rows); base(j): first table entry
 // of column j in the plan's packed index (rotations.c:113-114, 170-171) for a
@@ -118,6 +130,9 @@ struct Pipeline {
         int consumed[W];
         int task;
         int abort_flag;
+#ifdef FTAMD_TRACE
+        unsigned long long tr[W][4];
+#endif
     };
 
     // Everything one stage wave needs, uniform except the per-lane column data.
@@ -223,7 +238,25 @@ struct Pipeline {
             const int task = first + t;
             const int g = uniform(a.tasks[2 * task]);
             const int p = uniform(a.tasks[2 * task + 1]);
+#ifdef FTAMD_TRACE
+            if (lane == 0) sh.tr[w][0] = sh.tr[w][1] = sh.tr[w][2] = sh.tr[w][3] = 0;
+            const unsigned long long ft_w0 = wall_clock64();
+            const unsigned long long ft_c0 = __builtin_amdgcn_s_memtime();
+#endif
             run_stage(a, sh, g, p, w, lane);
+#ifdef FTAMD_TRACE
+            if (a.trace && lane == 0) {
+                unsigned long long *rec = a.trace + ((long long)task * 16 + w) * 8;
+                rec[0] = ft_w0;
+                rec[1] = wall_clock64();
+                rec[2] = __builtin_amdgcn_s_memtime() - ft_c0;
+                rec[3] = sh.tr[w][0];
+                rec[4] = sh.tr[w][1];
+                rec[5] = sh.tr[w][2];
+                rec[6] = ((unsigned long long)x << 32) | (unsigned)(blockIdx.x);
+                rec[7] = ((unsigned long long)g << 32) | (unsigned)p;
+            }
+#endif
             __syncthreads();
             if (threadIdx.x == 0) atomicAdd(a.completed, 1u);
         }
@@ -342,14 +375,20 @@ struct Pipeline {
         const int k = st.K - t * L;
         // ---- new rows: positions DL .. DL+L-1 = virtual rows k .. k-L+1
         if (is_first) {
-            if (prog_prev && !global_wait_le(prog_prev, max(k - L + 1, lo_valid), known, &sh.abort_flag)) {
+            FT_TR_BEGIN();
+            const bool ok = !prog_prev || global_wait_le(prog_prev, max(k - L + 1, lo_valid), known, &sh.abort_flag);
+            FT_TR_END(1);
+            if (!ok) {
                 fail(a, sh);
                 return false;
             }
 #pragma unroll
             for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = load_row(st, k - i, lo_valid);
         } else if (t < prev_emits) {
-            if (!lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag)) { fail(a, sh); return false; }
+            FT_TR_BEGIN();
+            const bool ok = lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag);
+            FT_TR_END(0);
+            if (!ok) { fail(a, sh); return false; }
             const T *src = &sh.ring[w - 1][t % R][0][lane];
 #pragma unroll
             for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = src[i * kLanes];
@@ -372,9 +411,16 @@ struct Pipeline {
             if (is_last) {
 #pragma unroll
                 for (int i = 0; i < L; ++i) store_row(st, k + D * L - i, lo_valid, x[slot<PH>(i)]);
-                if (prog_mine) publish(prog_mine, k + (D - 1) * L + 1);
+                if (prog_mine) {
+                    FT_TR_BEGIN();
+                    publish(prog_mine, k + (D - 1) * L + 1);
+                    FT_TR_END(2);
+                }
             } else {
-                if (e >= R && !lds_wait_ge(&sh.consumed[w], e - R + 1, &sh.abort_flag)) { fail(a, sh); return false; }
+                FT_TR_BEGIN();
+                const bool ok = e < R || lds_wait_ge(&sh.consumed[w], e - R + 1, &sh.abort_flag);
+                FT_TR_END(0);
+                if (!ok) { fail(a, sh); return false; }
                 T *dst = &sh.ring[w][e % R][0][lane];
 #pragma unroll
                 for (int i = 0; i < L; ++i) dst[i * kLanes] = x[slot<PH>(i)];

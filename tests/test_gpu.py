@@ -145,3 +145,42 @@ def test_async_device_api_and_stream(ft):
     s.synchronize()
     R = O.execute(Q, 0, A.copy(order="F"))
     assert np.array_equal(T.cpu().numpy().T, R)
+
+
+def test_batch_uneven_degrees(ft):
+    """ft_gpu_execute_batch: transforms of log-uniform degree in one launch,
+    each bit-identical to the oracle."""
+    import torch
+    rng = np.random.default_rng(4)
+    ns = np.unique(np.round(np.exp(rng.uniform(np.log(4), np.log(300), 24))).astype(int))
+    for kind, params in (("sph", ()), ("tri", (0.0, 0.0, 0.0)), ("disk", (0.0, 0.0))):
+        for d in (0, 1):
+            _say("batch", kind, d, list(ns))
+            maker = {"sph": ft.plan_rotsphere, "tri": ft.plan_rottriangle, "disk": ft.plan_rotdisk}[kind]
+            plans = [maker(int(n), *params) for n in ns]
+            As = [O.fill(kind, int(n), full_M(kind, int(n)), 100 + int(n)) for n in ns]
+            Ts = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
+            ft.gpu_execute_batch(plans, d, Ts)
+            torch.cuda.synchronize()
+            for n, A, T in zip(ns, As, Ts):
+                R = O.execute(O.Plan(kind, int(n), *params), d, A.copy(order="F"))
+                assert np.array_equal(T.cpu().numpy().T, R), (kind, d, n)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024])
+def test_fp32_path_error(ft, n):
+    """FP32 storage and arithmetic, s/c rounded from the FP64 plan: error
+    against the FP64 oracle (reported) within an FP32 rounding bound."""
+    import torch
+    P = ft.plan_rotsphere(n)
+    Q = O.Plan("sph", n)
+    A = O.fill("sph", n, 2 * n - 1, 77)
+    for d in (0, 1):
+        T = torch.from_numpy(np.ascontiguousarray(A.T)).float().cuda()
+        ft.gpu_execute(P, d, T, precision=ft.FP32)
+        torch.cuda.synchronize()
+        G = T.double().cpu().numpy().T
+        R = O.execute(Q, d, A.copy(order="F"))
+        e2, einf = O.rel2(G, R), O.relinf(G, R)
+        _say("fp32 n=%d dir=%d rel2=%.3e relinf=%.3e" % (n, d, e2, einf))
+        assert e2 < 8 * np.sqrt(n) * 2.0 ** -23
