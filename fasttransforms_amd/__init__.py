@@ -22,6 +22,15 @@ FP64, FP32 = 0, 1
 _KIND = {"sph": SPH, "tri": TRI, "disk": DISK}
 _DIR = {"hi2lo": HI2LO, "lo2hi": LO2HI}
 
+# One HIP runtime per process: PyTorch-ROCm bundles its own libamdhip64
+# (SONAME libamdhip64.so.7) and requests it under another file name, so if this
+# library loaded the system runtime first, torch would load a second copy that
+# finds no GPU.  Importing torch first lets this library bind to torch's copy.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:  # the C ABI works without torch
+    _torch = None
+
 _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, "libfasttransforms_amd.so")
 if not os.path.exists(LIB_PATH):

@@ -108,7 +108,10 @@ static double column_rotations(int kind, int n, int jmin, int jmax);
  * columns sharing sequences are adjacent) with equal jmin and A. */
 static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, int W, ft_hgroup **out,
                        int *ngroups, int *nprog) {
-    ft_hgroup *gs = calloc((size_t)(ncols / 1 + 1), sizeof(ft_hgroup));
+    /* The sphere couples rows k and k+2, so its even and odd rows are two
+     * independent chains: every sphere group is run as two groups (rpar). */
+    const int split = kind == FTAMD_SPH;
+    ft_hgroup *gs = calloc((size_t)(2 * ncols + 2), sizeof(ft_hgroup));
     if (!gs) return FTAMD_ENOMEM;
     int step = geo_step(kind), G = 0, prog = 0;
     int i = 0;
@@ -143,9 +146,19 @@ static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, 
         h->g.npass = (h->g.nstages + W - 1) / W;
         h->g.jfull = jmaxmin;
         h->g.prog_base = prog;
-        h->crit = c0->n + Q * step / 2 + Q;
+        h->crit = (split ? c0->n / 2 : c0->n) + Q;
+        h->g.rpar = split ? 0 : -1;
         prog += h->g.npass;
         G++;
+        if (split) {
+            gs[G] = *h;
+            gs[G].g.rpar = 1;
+            gs[G].g.prog_base = prog;
+            h->work *= 0.5;
+            gs[G].work = h->work;
+            prog += gs[G].g.npass;
+            G++;
+        }
     }
     *out = gs;
     *ngroups = G;

@@ -41,7 +41,8 @@ typedef struct {
     int npass;        /* ceil(nstages / W) */
     int jfull;        /* every column of the group needs every sequence with j <= jfull */
     int prog_base;    /* first progress slot (one per pass) */
-    int pad[2];
+    int rpar;         /* sphere: row parity of this sub-problem (rows rpar, rpar+2, ...); -1 otherwise */
+    int pad;
 } FtamdGroup;
 
 typedef struct {

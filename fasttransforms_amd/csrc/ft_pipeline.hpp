@@ -25,24 +25,47 @@ constexpr int kLanes = 64;
 #endif
 
 // ---------------------------------------------------------------- geometry --
-// top(j): largest row k of sequence j (real rows); base(j): first table entry
-// of column j in the plan's packed index (rotations.c:113-114, 170-171) for a
-// table built with n = N.
+// The engine works on "rows" e = 0 .. ne-1 of a sub-problem in which every
+// rotation couples rows (e, e+1).  For the triangle and the disk that is the
+// column itself (rotations_default.c:45-71 couple k and k+1).  The sphere
+// couples k and k+2 (rotations_default.c:31-43), so its even rows and its odd
+// rows are two independent chains: each sphere column is run as two
+// sub-problems, rpar = 0 (rows 0,2,4,...) and rpar = 1 (rows 1,3,5,...).
+//   ne(n, rpar)        rows of the sub-problem;
+//   top(n, rpar, j)    largest e whose pair (e, e+1) sequence j rotates;
+//   tab(N, rpar, j, e) table entry of that rotation, for a table built with
+//                      n = N (packed index of rotations.c:113-114, 170-171;
+//                      the sphere's device table stores each column's even
+//                      rows first, then its odd rows, so both are contiguous);
+//   real(rpar, e)      row of the column.
 template <int KIND> struct Geo;
 template <> struct Geo<FTAMD_SPH> {
-    static constexpr int D = 2, STEP = 2;
-    __device__ static int top(int n, int j) { return n - 3 - j; }
-    __device__ static long long base(int N, int j) { return (long long)j * (2 * N + 1 - j) / 2; }
+    static constexpr int STEP = 2;
+    __device__ static int ne(int n, int rpar) { return (n - rpar + 1) / 2; }
+    __device__ static int top(int n, int rpar, int j) {
+        const int k = n - 3 - j - rpar;  // largest row k = rpar (mod 2) of sequence j, minus rpar
+        return k < 0 ? -1 : k / 2;
+    }
+    __device__ static long long tab(int N, int rpar, int j, int e) {
+        return (long long)j * (2 * N + 1 - j) / 2 + (rpar ? (N - j + 1) / 2 : 0) + e;
+    }
+    __device__ static int real(int rpar, int e) { return 2 * e + rpar; }
 };
 template <> struct Geo<FTAMD_TRI> {
-    static constexpr int D = 1, STEP = 1;
-    __device__ static int top(int n, int j) { return n - 2 - j; }
-    __device__ static long long base(int N, int j) { return (long long)j * (2 * N + 1 - j) / 2; }
+    static constexpr int STEP = 1;
+    __device__ static int ne(int n, int) { return n; }
+    __device__ static int top(int n, int, int j) { return n - 2 - j; }
+    __device__ static long long tab(int N, int, int j, int e) { return (long long)j * (2 * N + 1 - j) / 2 + e; }
+    __device__ static int real(int, int e) { return e; }
 };
 template <> struct Geo<FTAMD_DISK> {
-    static constexpr int D = 1, STEP = 2;
-    __device__ static int top(int n, int j) { return n - 2 - (j + 1) / 2; }
-    __device__ static long long base(int N, int j) { return (long long)j * N - (long long)(j / 2) * ((j + 1) / 2); }
+    static constexpr int STEP = 2;
+    __device__ static int ne(int n, int) { return n; }
+    __device__ static int top(int n, int, int j) { return n - 2 - (j + 1) / 2; }
+    __device__ static long long tab(int N, int, int j, int e) {
+        return (long long)j * N - (long long)(j / 2) * ((j + 1) / 2) + e;
+    }
+    __device__ static int real(int, int e) { return e; }
 };
 
 // Address-space views: the plan table is read through the constant address
@@ -120,9 +143,8 @@ __device__ __forceinline__ bool global_wait_le(const int *prog, int target, int 
 template <typename T, int KIND, int DIR, int L, int W, int R>
 struct Pipeline {
     using G = Geo<KIND>;
-    static constexpr int D = G::D;
     static constexpr int STEP = G::STEP;
-    static constexpr int WS = (D + 1) * L;   // window rows per lane
+    static constexpr int WS = 2 * L;   // window rows per lane
 
     struct Shared {
         T ring[W > 1 ? W - 1 : 1][R][L][kLanes];
@@ -136,11 +158,16 @@ struct Pipeline {
     };
 
     // Everything one stage wave needs, uniform except the per-lane column data.
+    // Rows are "virtual": v = e for hi2lo, v = ne-1-e for lo2hi (lo2hi runs as
+    // hi2lo on reversed rows: the transposed rotation of the reversed pair is
+    // the forward rotation).
     struct Stage {
         cptr<T> sc;         // table entry 0 (interleaved s, c)
         gptr<T> col;        // this lane's column (nullptr: no column)
         int rs;             // row stride
         int n, N;           // rows of the transform, n of the table
+        int rpar;           // sphere row parity of the sub-problem (-1: none)
+        int ne;             // rows of the sub-problem
         int jmax;           // largest j this lane needs
         int q0, nq;         // first sequence and count
         int j0;             // j of sequence q0
@@ -152,16 +179,18 @@ struct Pipeline {
     };
 
     __device__ static int seq_j(int j_first, int q) { return DIR == 0 ? j_first - STEP * q : j_first + STEP * q; }
-    __device__ static int real_row(int n, int v) { return DIR == 0 ? v : n - 1 - v; }
+    __device__ static int real_row(const Stage &st, int v) { return G::real(st.rpar, DIR == 0 ? v : st.ne - 1 - v); }
+    __device__ static int top_v(const Stage &st, int j) { return G::top(st.n, st.rpar, j); }
 
-    // Table entry of rotation (virtual row v, sequence j).
+    // Table entry of rotation (virtual pair (v, v+1), sequence j).
     __device__ static long long tab_idx(const Stage &st, int j, int v) {
-        return DIR == 0 ? G::base(st.N, j) + v : G::base(st.N, j) + (st.n - 1 - D - v);
+        return G::tab(st.N, st.rpar, j, DIR == 0 ? v : st.ne - 2 - v);
     }
 
     template <int PH> __device__ static constexpr int slot(int i) { return (PH * L + i) % WS; }
 
-    // One L x L tile at position k (virtual), phase PH of the register ring.
+    // One L x L tile at position k (virtual), phase PH of the register ring:
+    // sequence lj rotates the pairs (r, r+1), r = k - lk + lj.
     template <int PH, bool CHECKED>
     __device__ static void rotate_tile(const Stage &st, T (&x)[WS], int k) {
 #pragma unroll
@@ -169,7 +198,7 @@ struct Pipeline {
             if (CHECKED && lj >= st.nq) break;
             const int j = DIR == 0 ? st.j0 - STEP * lj : st.j0 + STEP * lj;
             // L consecutive table entries of column j (scalar loads).
-            const long long i0 = DIR == 0 ? tab_idx(st, j, k + D * lj) - (L - 1) : tab_idx(st, j, k + D * lj);
+            const long long i0 = DIR == 0 ? tab_idx(st, j, k + lj) - (L - 1) : tab_idx(st, j, k + lj);
             cptr<T> p = st.sc + 2 * i0;
             T sv[L], cv[L];
 #pragma unroll
@@ -179,32 +208,32 @@ struct Pipeline {
             }
             int lo = 0, hi = 0;
             if (CHECKED) {
-                const int top = G::top(st.n, j);
-                lo = DIR == 0 ? 0 : st.n - 1 - D - top;
-                hi = DIR == 0 ? top : st.n - 1 - D;
+                const int top = top_v(st, j);
+                lo = DIR == 0 ? 0 : st.ne - 2 - top;
+                hi = DIR == 0 ? top : st.ne - 2;
             }
             const bool act = !CHECKED || st.lanes_all || j <= st.jmax;
 #pragma unroll
             for (int lk = 0; lk < L; ++lk) {
-                const int r = k - lk + D * lj;
+                const int r = k - lk + lj;
                 if (CHECKED && (r < lo || r > hi)) continue;
                 const int i = DIR == 0 ? L - 1 - lk : lk;
-                const int px = slot<PH>(D * L + lk - D * lj);
-                const int py = slot<PH>(D * L + lk - D * lj - D);
+                const int px = slot<PH>(L + lk - lj);
+                const int py = slot<PH>(L + lk - lj - 1);
                 if (act) givens<T>(x[px], x[py], sv[i], cv[i]);
             }
         }
     }
 
-    // Row v of this lane's column (0 if outside [lo_valid, n-1]).
+    // Row v of this lane's column (0 if outside [lo_valid, ne-1]).
     __device__ static T load_row(const Stage &st, int v, int lo_valid) {
-        if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return T(0);
-        return __hip_atomic_load(st.col + (long long)real_row(st.n, v) * st.rs, __ATOMIC_RELAXED,
+        if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return T(0);
+        return __hip_atomic_load(st.col + (long long)real_row(st, v) * st.rs, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ static void store_row(const Stage &st, int v, int lo_valid, T val) {
-        if (v < lo_valid || v > st.n - 1 || st.col == nullptr) return;
-        __hip_atomic_store(st.col + (long long)real_row(st.n, v) * st.rs, val, __ATOMIC_RELAXED,
+        if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return;
+        __hip_atomic_store(st.col + (long long)real_row(st, v) * st.rs, val, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
 
@@ -295,6 +324,8 @@ struct Pipeline {
         st.sc = (cptr<T>)(a.sc);
         st.n = uniform(gd.n);
         st.N = a.ntab;
+        st.rpar = uniform(gd.rpar);
+        st.ne = G::ne(st.n, st.rpar);
         st.rs = uniform(gd.rs);
         const long long off = a.lane_off[(long long)g * kLanes + lane];
         st.col = off < 0 ? nullptr : (gptr<T>)(gd.A) + off;
@@ -310,28 +341,28 @@ struct Pipeline {
             st.lanes_all = jmax_stage <= st.jfull;
         }
         if (DIR == 0) {
-            st.K = G::top(st.n, st.j0);
+            st.K = top_v(st, st.j0);
             st.Bot = 0;
             st.lo0 = 0;
         } else {
-            st.K = st.n - 1;
-            st.Bot = st.n - 1 - D - G::top(st.n, seq_j(j_first, p * W * L));
-            st.lo0 = st.n - 1 - D - G::top(st.n, st.j0);
+            st.K = st.ne - 1;
+            st.Bot = st.ne - 2 - top_v(st, seq_j(j_first, p * W * L));
+            st.lo0 = st.ne - 2 - top_v(st, st.j0);
         }
         {
-            const int E = st.K + (D - 1) * L + 1 - st.Bot;
+            const int E = st.K + 1 - st.Bot;
             st.ntile = E <= 0 ? 1 : (E + L - 1) / L + 1;
         }
-        const int emit_off = DIR == 0 ? 0 : D;
+        const int emit_off = DIR == 0 ? 0 : 1;
         // Emits the previous stage makes (only needed by consumers).
         int prev_emits = 0;
         if (!is_first) {
             const int qp = st.q0 - L;
             const int jp = seq_j(j_first, qp);
-            const int Kp = DIR == 0 ? G::top(st.n, jp) : st.n - 1;
-            const int E = Kp + (D - 1) * L + 1 - st.Bot;
+            const int Kp = DIR == 0 ? top_v(st, jp) : st.ne - 1;
+            const int E = Kp + 1 - st.Bot;
             const int Tp = E <= 0 ? 1 : (E + L - 1) / L + 1;
-            prev_emits = Tp - (DIR == 0 ? 0 : D);
+            prev_emits = Tp - emit_off;
         }
         const int lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
         int *prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
@@ -342,13 +373,13 @@ struct Pipeline {
 #pragma unroll
         for (int i = 0; i < WS; ++i) x[i] = T(0);
 
-        // Initial carried rows (positions 0 .. DL-1 = virtual rows K+DL .. K+1).
+        // Initial carried rows (positions 0 .. L-1 = virtual rows K+L .. K+1).
         // hi2lo: untouched by the earlier stages of this task, read from A.
         // lo2hi: above the top row, zero.
         if (DIR == 0) {
             if (prog_prev && !global_wait_le(prog_prev, st.K + 1, &known, &sh.abort_flag)) { fail(a, sh); return; }
 #pragma unroll
-            for (int i = 0; i < D * L; ++i) x[i] = load_row(st, st.K + D * L - i, lo_valid);
+            for (int i = 0; i < L; ++i) x[i] = load_row(st, st.K + L - i, lo_valid);
         }
 
         int t = 0;
@@ -359,11 +390,6 @@ struct Pipeline {
             if (!tile<1>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
                          prog_mine, &known)) return;
             if (++t >= st.ntile) break;
-            if constexpr (D == 2) {
-                if (!tile<2>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
-                             prog_mine, &known)) return;
-                if (++t >= st.ntile) break;
-            }
         }
         if (is_last && prog_mine) publish(prog_mine, -1);
     }
@@ -373,7 +399,7 @@ struct Pipeline {
                                 bool is_first, bool is_last, int prev_emits, int emit_off, int lo_valid,
                                 int *prog_prev, int *prog_mine, int *known) {
         const int k = st.K - t * L;
-        // ---- new rows: positions DL .. DL+L-1 = virtual rows k .. k-L+1
+        // ---- new rows: positions L .. 2L-1 = virtual rows k .. k-L+1
         if (is_first) {
             FT_TR_BEGIN();
             const bool ok = !prog_prev || global_wait_le(prog_prev, max(k - L + 1, lo_valid), known, &sh.abort_flag);
@@ -383,7 +409,7 @@ struct Pipeline {
                 return false;
             }
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = load_row(st, k - i, lo_valid);
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = load_row(st, k - i, lo_valid);
         } else if (t < prev_emits) {
             FT_TR_BEGIN();
             const bool ok = lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag);
@@ -391,29 +417,29 @@ struct Pipeline {
             if (!ok) { fail(a, sh); return false; }
             const T *src = &sh.ring[w - 1][t % R][0][lane];
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = src[i * kLanes];
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = src[i * kLanes];
             __hip_atomic_store(&sh.consumed[w - 1], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(D * L + i)] = T(0);
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = T(0);
         }
 
         // ---- rotations
         bool fast = st.nq == L && st.lanes_all;
         if (DIR == 0) fast = fast && k - (L - 1) >= 0;
-        else fast = fast && k + D * (L - 1) <= st.n - 1 - D && k - (L - 1) >= st.lo0;
+        else fast = fast && k + (L - 1) <= st.ne - 2 && k - (L - 1) >= st.lo0;
         if (fast) rotate_tile<PH, false>(st, x, k);
         else rotate_tile<PH, true>(st, x, k);
 
-        // ---- emit: positions 0 .. L-1 = virtual rows k+DL .. k+(D-1)L+1
+        // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1
         if (t >= emit_off) {
             const int e = t - emit_off;
             if (is_last) {
 #pragma unroll
-                for (int i = 0; i < L; ++i) store_row(st, k + D * L - i, lo_valid, x[slot<PH>(i)]);
+                for (int i = 0; i < L; ++i) store_row(st, k + L - i, lo_valid, x[slot<PH>(i)]);
                 if (prog_mine) {
                     FT_TR_BEGIN();
-                    publish(prog_mine, k + (D - 1) * L + 1);
+                    publish(prog_mine, k + 1);
                     FT_TR_END(2);
                 }
             } else {

@@ -118,8 +118,10 @@ void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
 
 /*
  * Upload the interleaved table {s0, c0, s1, c1, ...} with FTAMD_TABLE_PAD zero
- * entries on each side.  FP32 entries are (float) of the FP64 ones.  Caller
- * holds RP->lock.
+ * entries on each side.  FP32 entries are (float) of the FP64 ones.  The
+ * sphere's kernels run even and odd rows as separate chains, so each of its
+ * columns is stored with the even rows l = 0, 2, ... first and the odd rows
+ * after them (both then contiguous).  Caller holds RP->lock.
  */
 int ftamd_plan_upload(ft_rotation_plan *RP, int precision) {
     if (RP->d_tab[precision]) return FTAMD_OK;
@@ -131,13 +133,22 @@ int ftamd_plan_upload(ft_rotation_plan *RP, int precision) {
     void *host = calloc(count, esz);
     if (!host) return FTAMD_ENOMEM;
     size_t base = 2 * (size_t)FTAMD_TABLE_PAD;
-    for (long i = 0; i < RP->nsc; i++) {
-        if (precision == 0) {
-            ((double *)host)[base + 2 * i] = RP->s[i];
-            ((double *)host)[base + 2 * i + 1] = RP->c[i];
-        } else {
-            ((float *)host)[base + 2 * i] = (float)RP->s[i];
-            ((float *)host)[base + 2 * i + 1] = (float)RP->c[i];
+    int n = RP->n;
+    int ncol = RP->kind == FTAMD_DISK ? 2 * n - 1 : n;
+    for (int m = 0; m < ncol; m++) {
+        long start = RP->kind == FTAMD_DISK ? idx_disk(n, 0, m) : idx_tri(n, 0, m);
+        long rows = RP->kind == FTAMD_DISK ? n - (m + 1) / 2 : n - m;
+        long nev = (rows + 1) / 2;
+        for (long l = 0; l < rows; l++) {
+            long src = start + l, dst = src;
+            if (RP->kind == FTAMD_SPH) dst = start + (l % 2 == 0 ? l / 2 : nev + l / 2);
+            if (precision == 0) {
+                ((double *)host)[base + 2 * dst] = RP->s[src];
+                ((double *)host)[base + 2 * dst + 1] = RP->c[src];
+            } else {
+                ((float *)host)[base + 2 * dst] = (float)RP->s[src];
+                ((float *)host)[base + 2 * dst + 1] = (float)RP->c[src];
+            }
         }
     }
     void *d = NULL;
