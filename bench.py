@@ -32,6 +32,8 @@ def parse():
     p.add_argument("--n", type=int, default=1024)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--calib-copy", action="store_true",
+                   help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return p.parse_args()
 
 
@@ -96,6 +98,12 @@ def main():
     gflops = flops / (ms * 1e-3) / 1e9
     achieved_tflops = flops / (ms * 1e-3) / 1e12
     compulsory_bytes = 16.0 * n * M + 16.0 * n * (n + 1) / 2
+
+    if args.calib_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_seconds)
     cpu_obj = None

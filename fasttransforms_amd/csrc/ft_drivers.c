@@ -536,8 +536,9 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
     int step = geo_step(kind), n = RP->n;
     int jlo, jhi;
     if (dir == 0) {
+        /* j = m2-step, m2-2*step, ... >= m1: the parity of m2, not of m1 */
         jhi = m2 - step;
-        jlo = m1;
+        jlo = jhi >= m1 ? jhi - ((jhi - m1) / step) * step : m1;
     } else {
         jlo = m1;
         jhi = m1 + ((m2 - 1 - m1) / step) * step;

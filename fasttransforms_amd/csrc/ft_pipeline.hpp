@@ -176,6 +176,17 @@ struct Pipeline {
         int jfull;
         bool lanes_all;     // every lane active for every sequence of the stage
         int lo0;            // DIR 1: lowest virtual row of sequence q0
+        bool coh_load;      // rows of A come from the previous task (device-scope loads)
+        bool coh_store;     // rows of A go to the next task (device-scope stores)
+    };
+
+    // Per-stage control of one sweep.
+    struct Ctl {
+        bool is_first, is_last;
+        int prev_emits, emit_off, lo_valid;
+        int *prog_prev, *prog_mine;
+        int known;          // progress of the previous task seen so far
+        int pend;           // progress to publish once the last tile's stores are acknowledged
     };
 
     __device__ static int seq_j(int j_first, int q) { return DIR == 0 ? j_first - STEP * q : j_first + STEP * q; }
@@ -225,16 +236,21 @@ struct Pipeline {
         }
     }
 
-    // Row v of this lane's column (0 if outside [lo_valid, ne-1]).
+    // Row v of this lane's column (0 if outside [lo_valid, ne-1]).  Rows handed
+    // over by another task use device-scope accesses (see publish()); the
+    // first pass reads what the host wrote and the last pass writes what only
+    // the host reads, with ordinary cached accesses.
     __device__ static T load_row(const Stage &st, int v, int lo_valid) {
         if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return T(0);
-        return __hip_atomic_load(st.col + (long long)real_row(st, v) * st.rs, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
+        gptr<T> ptr = st.col + (long long)real_row(st, v) * st.rs;
+        if (st.coh_load) return __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return *ptr;
     }
     __device__ static void store_row(const Stage &st, int v, int lo_valid, T val) {
         if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return;
-        __hip_atomic_store(st.col + (long long)real_row(st, v) * st.rs, val, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        gptr<T> ptr = st.col + (long long)real_row(st, v) * st.rs;
+        if (st.coh_store) __hip_atomic_store(ptr, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *ptr = val;
     }
 
     // XCD (accelerator complex die) this workgroup runs on.  Each XCD has its
@@ -364,10 +380,18 @@ struct Pipeline {
             const int Tp = E <= 0 ? 1 : (E + L - 1) / L + 1;
             prev_emits = Tp - emit_off;
         }
-        const int lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
-        int *prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
-        int *prog_mine = p + 1 < uniform(gd.npass) ? a.prog + uniform(gd.prog_base) + p : nullptr;
-        int known = FTAMD_PROG_NONE;
+        Ctl c;
+        c.is_first = is_first;
+        c.is_last = is_last;
+        c.prev_emits = prev_emits;
+        c.emit_off = emit_off;
+        c.lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
+        c.prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
+        c.prog_mine = p + 1 < uniform(gd.npass) ? a.prog + uniform(gd.prog_base) + p : nullptr;
+        c.known = FTAMD_PROG_NONE;
+        c.pend = FTAMD_PROG_NONE;
+        st.coh_load = c.prog_prev != nullptr;
+        st.coh_store = c.prog_mine != nullptr;
 
         T x[WS];
 #pragma unroll
@@ -377,40 +401,38 @@ struct Pipeline {
         // hi2lo: untouched by the earlier stages of this task, read from A.
         // lo2hi: above the top row, zero.
         if (DIR == 0) {
-            if (prog_prev && !global_wait_le(prog_prev, st.K + 1, &known, &sh.abort_flag)) { fail(a, sh); return; }
+            if (c.prog_prev && !global_wait_le(c.prog_prev, st.K + 1, &c.known, &sh.abort_flag)) { fail(a, sh); return; }
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[i] = load_row(st, st.K + L - i, lo_valid);
+            for (int i = 0; i < L; ++i) x[i] = load_row(st, st.K + L - i, c.lo_valid);
         }
 
         int t = 0;
         for (;;) {
-            if (!tile<0>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
-                         prog_mine, &known)) return;
+            if (!tile<0>(a, sh, st, c, x, t, w, lane)) return;
             if (++t >= st.ntile) break;
-            if (!tile<1>(a, sh, st, x, t, w, lane, is_first, is_last, prev_emits, emit_off, lo_valid, prog_prev,
-                         prog_mine, &known)) return;
+            if (!tile<1>(a, sh, st, c, x, t, w, lane)) return;
             if (++t >= st.ntile) break;
         }
-        if (is_last && prog_mine) publish(prog_mine, -1);
+        if (is_last && c.prog_mine) publish(c.prog_mine, -1);
     }
 
     template <int PH>
-    __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, T (&x)[WS], int t, int w, int lane,
-                                bool is_first, bool is_last, int prev_emits, int emit_off, int lo_valid,
-                                int *prog_prev, int *prog_mine, int *known) {
+    __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, Ctl &c, T (&x)[WS], int t, int w,
+                                int lane) {
         const int k = st.K - t * L;
         // ---- new rows: positions L .. 2L-1 = virtual rows k .. k-L+1
-        if (is_first) {
+        if (c.is_first) {
             FT_TR_BEGIN();
-            const bool ok = !prog_prev || global_wait_le(prog_prev, max(k - L + 1, lo_valid), known, &sh.abort_flag);
+            const bool ok = !c.prog_prev || global_wait_le(c.prog_prev, max(k - L + 1, c.lo_valid), &c.known,
+                                                            &sh.abort_flag);
             FT_TR_END(1);
             if (!ok) {
                 fail(a, sh);
                 return false;
             }
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = load_row(st, k - i, lo_valid);
-        } else if (t < prev_emits) {
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = load_row(st, k - i, c.lo_valid);
+        } else if (t < c.prev_emits) {
             FT_TR_BEGIN();
             const bool ok = lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag);
             FT_TR_END(0);
@@ -432,14 +454,29 @@ struct Pipeline {
         else rotate_tile<PH, true>(st, x, k);
 
         // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1
-        if (t >= emit_off) {
-            const int e = t - emit_off;
-            if (is_last) {
+        if (t >= c.emit_off) {
+            const int e = t - c.emit_off;
+            if (c.is_last) {
 #pragma unroll
-                for (int i = 0; i < L; ++i) store_row(st, k + L - i, lo_valid, x[slot<PH>(i)]);
-                if (prog_mine) {
+                for (int i = 0; i < L; ++i) store_row(st, k + L - i, c.lo_valid, x[slot<PH>(i)]);
+                if (c.prog_mine) {
+                    // Publish the previous tile's progress once its stores are
+                    // acknowledged, leaving this tile's L stores in flight
+                    // (vmcnt counts only this wave's stores here).  Edge tiles,
+                    // which may store fewer rows, wait for everything.
                     FT_TR_BEGIN();
-                    publish(prog_mine, k + 1);
+                    const bool full = !c.is_first && k + 1 >= c.lo_valid && k + L <= st.ne - 1;
+                    if (full) {
+                        if (c.pend != FTAMD_PROG_NONE) {
+                            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                            __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
+                            __hip_atomic_store(c.prog_mine, c.pend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        c.pend = k + 1;
+                    } else {
+                        publish(c.prog_mine, k + 1);
+                        c.pend = FTAMD_PROG_NONE;
+                    }
                     FT_TR_END(2);
                 }
             } else {
