@@ -63,14 +63,21 @@ int ftamd_tuning_L(int kind, int n) {
     return g_L ? g_L : 4;
 }
 
-int ftamd_tuning_W(void) { return g_W ? g_W : 8; }
+int ftamd_tuning_W(void) { return g_W ? g_W : 16; }
 
-int ftamd_tuning_grid(int ntasks) {
-    int cu = 0;
-    if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
-    int per = g_wg_per_cu ? g_wg_per_cu : 1;
-    int grid = cu * per;
-    return grid < ntasks ? grid : ntasks;
+/* Workgroups are dealt to the XCDs in turn and each takes tasks only from its
+ * own XCD's list, so the grid holds as many workgroups per XCD as the longest
+ * list has tasks: every task gets a workgroup as soon as the hardware has room
+ * (workgroups that find their list empty exit at once).  The tuning knob caps
+ * the grid at wg_per_cu workgroups per CU. */
+int ftamd_tuning_grid(const ft_schedule *S) {
+    int grid = S->launch.nxcd * S->xcd_tasks;
+    if (g_wg_per_cu) {
+        int cu = 0;
+        if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
+        if (grid > cu * g_wg_per_cu) grid = cu * g_wg_per_cu;
+    }
+    return grid;
 }
 
 /* ---------------------------------------------------------- geometry -- */
@@ -95,7 +102,7 @@ typedef struct {
 
 typedef struct {
     FtamdGroup g;
-    long long off[64];
+    int off[64];
     int jmax[64];
     int crit;      /* ordering key: longer chains first */
     double work;   /* rotations of all its columns */
@@ -122,7 +129,7 @@ static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, 
         h->work = 0;
         while (i < ncols && cnt < 64 && (cnt == 0 || (cols[i].jmin == cols[i - 1].jmin && cols[i].A == cols[i - 1].A))) {
             h->work += column_rotations(kind, cols[i].n, cols[i].jmin, cols[i].jmax);
-            h->off[cnt] = cols[i].off;
+            h->off[cnt] = (int)cols[i].off;
             h->jmax[cnt] = cols[i].jmax;
             if (cols[i].jmax > jmaxmax) jmaxmax = cols[i].jmax;
             if (cols[i].jmax < jmaxmin) jmaxmin = cols[i].jmax;
@@ -223,6 +230,19 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
         }
         order[b + 1] = v;
     }
+    /* issue priority by chain length, and the boundary buffers: npass-1 of
+     * [rows of the sub-problem][64 lanes] per group */
+    int maxcrit = 1;
+    for (int g = 0; g < G; g++)
+        if (gs[g].crit > maxcrit) maxcrit = gs[g].crit;
+    long long bnd_elems = 0;
+    for (int g = 0; g < G; g++) {
+        long long c4 = 4LL * gs[g].crit;
+        gs[g].g.prio = c4 > 3LL * maxcrit ? 3 : c4 > 2LL * maxcrit ? 2 : c4 > maxcrit ? 1 : 0;
+        int ne = gs[g].g.rpar >= 0 ? (gs[g].g.n - gs[g].g.rpar + 1) / 2 : gs[g].g.n;
+        gs[g].g.bnd = bnd_elems;
+        bnd_elems += (long long)(gs[g].g.npass - 1) * ne * 64;
+    }
     int T = 0;
     for (int g = 0; g < G; g++) T += gs[g].g.npass;
     int *tasks = malloc(sizeof(int) * 2 * (size_t)(T ? T : 1));
@@ -246,10 +266,14 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
     }
     xoff[nxcd] = t;
     free(order);
+    int xcd_tasks = 0;
+    for (int x = 0; x < nxcd; x++)
+        if (xoff[x + 1] - xoff[x] > xcd_tasks) xcd_tasks = xoff[x + 1] - xoff[x];
+    S->xcd_tasks = xcd_tasks;
 
     size_t o_groups = 0;
     size_t o_off = align256(o_groups + sizeof(FtamdGroup) * (size_t)G);
-    size_t o_jmax = align256(o_off + sizeof(long long) * 64 * (size_t)G);
+    size_t o_jmax = align256(o_off + sizeof(int) * 64 * (size_t)G);
     size_t o_tasks = align256(o_jmax + sizeof(int) * 64 * (size_t)G);
     size_t o_xoff = align256(o_tasks + sizeof(int) * 2 * (size_t)T);
     size_t o_prog = align256(o_xoff + sizeof(int) * (FTAMD_MAX_XCD + 1));
@@ -262,29 +286,34 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
     }
     for (int g = 0; g < G; g++) {
         memcpy(host + o_groups + sizeof(FtamdGroup) * g, &gs[g].g, sizeof(FtamdGroup));
-        memcpy(host + o_off + sizeof(long long) * 64 * g, gs[g].off, sizeof(long long) * 64);
+        memcpy(host + o_off + sizeof(int) * 64 * g, gs[g].off, sizeof(int) * 64);
         memcpy(host + o_jmax + sizeof(int) * 64 * g, gs[g].jmax, sizeof(int) * 64);
     }
     memcpy(host + o_tasks, tasks, sizeof(int) * 2 * (size_t)T);
     memcpy(host + o_xoff, xoff, sizeof(int) * (size_t)(nxcd + 1));
     for (int i = 0; i < nprog; i++) ((int *)(host + o_prog))[i] = FTAMD_PROG_NONE;
     free(tasks);
-    void *d = NULL;
+    void *d = NULL, *dbnd = NULL;
     int rc = ftamd_hip_malloc(&d, total);
+    if (rc == FTAMD_OK && bnd_elems > 0)
+        rc = ftamd_hip_malloc(&dbnd, (size_t)bnd_elems * (S->precision == 0 ? sizeof(double) : sizeof(float)));
     if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, host, total, NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     free(host);
     if (rc != FTAMD_OK) {
         ftamd_hip_free(d);
+        ftamd_hip_free(dbnd);
         return rc;
     }
     char *db = d;
     S->d_mem = d;
+    S->d_bnd = dbnd;
+    S->launch.bnd = dbnd;
     S->ngroups = G;
     S->ntasks = T;
     FtamdLaunch *a = &S->launch;
     a->groups = (const FtamdGroup *)(db + o_groups);
-    a->lane_off = (const long long *)(db + o_off);
+    a->lane_off = (const int *)(db + o_off);
     a->lane_jmax = (const int *)(db + o_jmax);
     a->tasks = (const int *)(db + o_tasks);
     a->xcd_off = (const int *)(db + o_xoff);
@@ -301,9 +330,18 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
     return FTAMD_OK;
 }
 
+void ftamd_schedule_free_device(ft_schedule *S) {
+    if (!S) return;
+    ftamd_hip_free(S->d_mem);
+    ftamd_hip_free(S->d_bnd);
+    S->d_mem = S->d_bnd = NULL;
+}
+
 /* Columns of one n x M matrix, ordered so that equal-need columns are adjacent:
  * sph/disk by order parity then increasing order, tri by increasing order. */
 static int matrix_columns(int kind, int n, int M, void *dA, ft_col **out, int *ncols) {
+    /* the kernels index a matrix with 32-bit element offsets */
+    if ((long long)n * M > 0x7fffffffLL) return FTAMD_EINVAL;
     ft_col *cols = malloc(sizeof(ft_col) * (size_t)(M > 0 ? M : 1));
     if (!cols) return FTAMD_ENOMEM;
     int k = 0;
@@ -400,7 +438,7 @@ void ftamd_set_trace(void *dev_buffer) { g_trace = dev_buffer; }
 
 static int launch(ft_schedule *S, int precision, int kind, int dir, void *stream) {
     S->launch.trace = g_trace;
-    return ftamd_hip_launch(&S->launch, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S->ntasks), stream);
+    return ftamd_hip_launch(&S->launch, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S), stream);
 }
 
 static int check_kernel_error(ft_schedule *S) {
@@ -529,7 +567,7 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
  */
 static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, int m2, double *A, int S) {
     ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
-    if (!RP || !A || RP->kind != kind || S < 1 || m1 < 0) {
+    if (!RP || !A || RP->kind != kind || S < 1 || m1 < 0 || (long long)(RP->n - 1) * S > 0x7fffffffLL) {
         ftamd_set_error(FT_GPU_EINVAL);
         return;
     }
@@ -585,7 +623,7 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     if (Sd) {
-        ftamd_hip_free(Sd->d_mem);
+        ftamd_schedule_free_device(Sd);
         free(Sd);
     }
     if (pk != FTAMD_PTR_DEVICE) ftamd_hip_free(d);
@@ -672,7 +710,7 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
         g_batch.precision != precision || g_batch.L != L || g_batch.W != W) {
         if (S) {
             ftamd_hip_sync(NULL);
-            ftamd_hip_free(S->d_mem);
+            ftamd_schedule_free_device(S);
             free(S);
             g_batch.sched = S = NULL;
         }

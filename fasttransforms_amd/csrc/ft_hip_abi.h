@@ -42,18 +42,20 @@ typedef struct {
     int jfull;        /* every column of the group needs every sequence with j <= jfull */
     int prog_base;    /* first progress slot (one per pass) */
     int rpar;         /* sphere: row parity of this sub-problem (rows rpar, rpar+2, ...); -1 otherwise */
-    int pad;
+    int prio;         /* wave issue priority 0..3 of its tasks (longest chains highest) */
+    long long bnd;    /* element offset of its boundary buffers: npass-1 of [rows][64] */
 } FtamdGroup;
 
 typedef struct {
     const FtamdGroup *groups;
-    const long long *lane_off;  /* [groups*64] element offset of the lane's column, -1 if none */
+    const int *lane_off;        /* [groups*64] element offset of the lane's column in A, -1 if none */
     const int *lane_jmax;       /* [groups*64] largest j the lane's column needs, -1 if none */
     const int *tasks;           /* [2*ntasks] (group, pass): per XCD, producers before consumers */
     const int *xcd_off;         /* [nxcd+1] first task of each XCD's list */
     int ntasks;
     int nxcd;
     const void *sc;             /* interleaved (s, c) table, entry 0 (padding on both sides) */
+    void *bnd;                  /* boundary buffers handing rows from one task to the next */
     int ntab;                   /* n the table was built for (index function) */
     unsigned *ticket;           /* [nxcd] zero at launch; reset by the kernel */
     unsigned *done;             /* zero at launch; reset by the kernel */

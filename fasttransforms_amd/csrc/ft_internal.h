@@ -15,10 +15,13 @@ typedef struct ft_schedule {
     struct ft_schedule *next;
     /* key */
     int M, dir, precision, L, W, rs, m1, m2, single;
-    /* device arrays (one allocation) */
+    /* device arrays (one allocation), and the boundary buffers rows pass
+     * through from one task of a group to the next */
     void *d_mem;
+    void *d_bnd;
     FtamdLaunch launch;
     int ngroups, ntasks;
+    int xcd_tasks;           /* most tasks any XCD's list holds */
     double flops;            /* algorithmic flops (6 per rotation) */
 } ft_schedule;
 
@@ -48,6 +51,7 @@ const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
 void ftamd_set_error(int code);
 int ftamd_tuning_L(int kind, int n);
 int ftamd_tuning_W(void);
-int ftamd_tuning_grid(int ntasks);
+int ftamd_tuning_grid(const ft_schedule *S);
+void ftamd_schedule_free_device(ft_schedule *S);
 
 #endif /* FT_INTERNAL_H */

@@ -14,14 +14,20 @@ constexpr int kLanes = 64;
 
 // Optional timeline instrumentation (-DFTAMD_TRACE, diagnostic builds only):
 // per stage wave and task, start/end wall clock, tiles, and the shader-clock
-// cycles spent waiting on the LDS ring, on the previous task's progress and on
-// its own row stores.  The product build compiles all of it away.
+// cycles spent in each part of a tile.  The product build compiles all of it
+// away.  Slots: 0 LDS-ring input wait, 1 progress wait, 2 row stores and
+// progress publication, 3 input rows arriving, 4 rotations, 5 LDS-ring
+// output, 6 tiles.
 #ifdef FTAMD_TRACE
 #define FT_TR_BEGIN() const unsigned long long ft_tr0 = __builtin_amdgcn_s_memtime()
-#define FT_TR_END(slot) sh.tr[w][slot] += __builtin_amdgcn_s_memtime() - ft_tr0
+#define FT_TR_END(slot) c.tr[slot] += __builtin_amdgcn_s_memtime() - ft_tr0
+#define FT_TR_COUNT(slot) c.tr[slot] += 1
+#define FT_TR_SYNC() __builtin_amdgcn_s_waitcnt(0)
 #else
 #define FT_TR_BEGIN() (void)0
 #define FT_TR_END(slot) (void)0
+#define FT_TR_COUNT(slot) (void)0
+#define FT_TR_SYNC() (void)0
 #endif
 
 // ---------------------------------------------------------------- geometry --
@@ -68,11 +74,11 @@ template <> struct Geo<FTAMD_DISK> {
     __device__ static int real(int, int e) { return e; }
 };
 
-// Address-space views: the plan table is read through the constant address
-// space so that wave-uniform loads become scalar (s_load) loads into SGPRs;
-// matrix columns are global memory.
-template <typename T> using cptr = const __attribute__((address_space(4))) T *;
 template <typename T> using gptr = __attribute__((address_space(1))) T *;
+
+// One table entry: the interleaved (s, c) of one rotation, loaded as one
+// 2-vector (16 bytes in FP64).
+template <typename T> using SCPair = T __attribute__((ext_vector_type(2)));
 
 template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c);
 template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -88,34 +94,54 @@ template <typename T> __device__ __forceinline__ void givens(T &x, T &y, T s, T 
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Lane N of every row of 16 lanes, broadcast to the whole row (DPP
+// row_newbcast).  The sines and cosines of a tile are held one per lane
+// (lane l: entry l mod 16) and broadcast where they are used, so they come
+// from ordinary vector loads -- returned in order, prefetched a tile ahead --
+// rather than from scalar loads, whose scalar-cache misses serialise.
+template <int N, typename T> __device__ __forceinline__ T bcast(T v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xf, 0xf, true);
+}
+// n must fold to a constant (every caller is fully unrolled).
+template <typename T> __device__ __forceinline__ T bcast_n(T v, int n) {
+    switch (n & 15) {
+        case 0: return bcast<0>(v);
+        case 1: return bcast<1>(v);
+        case 2: return bcast<2>(v);
+        case 3: return bcast<3>(v);
+        case 4: return bcast<4>(v);
+        case 5: return bcast<5>(v);
+        case 6: return bcast<6>(v);
+        case 7: return bcast<7>(v);
+        case 8: return bcast<8>(v);
+        case 9: return bcast<9>(v);
+        case 10: return bcast<10>(v);
+        case 11: return bcast<11>(v);
+        case 12: return bcast<12>(v);
+        case 13: return bcast<13>(v);
+        case 14: return bcast<14>(v);
+        default: return bcast<15>(v);
+    }
+}
+
 // Bounded waits.  ~2^22 polls of >= 64 cycles each is far beyond any
 // legitimate wait; on expiry the caller abandons its task.
 constexpr int kSpinLimit = 1 << 22;
 
-__device__ __forceinline__ bool lds_wait_ge(const int *flag, int target, const int *abort_flag) {
-    for (int it = 0; it < kSpinLimit; ++it) {
-        int v = uniform(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-        if (v >= target) return true;
-        if (uniform(__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))) return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return false;
-}
-
-// Inter-task hand-off through A.  The rows one task passes to the next are
-// stored and loaded with device-scope relaxed atomics (sc1 global stores and
-// loads), which are coherent per location across CUs and L2s without any
-// L2 write-back or invalidation; the producer waits until its row stores are
-// acknowledged (s_waitcnt vmcnt(0)) before publishing its progress with a
-// device-scope atomic store, and the consumer polls that with device-scope
-// atomic loads.  Plain cached accesses with a release/acquire pair would need
-// buffer_wbl2 / buffer_inv of the whole L2 on every tile.  Chained tasks still
-// run on one XCD, which keeps their traffic in one L2 / MALL slice.
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+// Inter-task hand-off.  The last stage of a task stores the rows it emits in a
+// row-major boundary buffer ([row][lane]: one row of 64 lanes is 8 whole
+// 64-byte lines) and the first stage of the next task reads them back.  Both
+// use device-scope relaxed atomics (sc1 stores and loads), coherent per
+// location between CUs without any L2 write-back or invalidation; the
+// producer waits until its stores are acknowledged (s_waitcnt vmcnt(0))
+// before it publishes its progress with a device-scope atomic store, and the
+// consumer polls that with device-scope atomic loads.  Chained tasks also run
+// on one XCD, which keeps their traffic in one L2.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
 __device__ __forceinline__ void publish(int *prog, int value) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    wait_stores();
+    wait_vmem();
     __hip_atomic_store(prog, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -139,12 +165,42 @@ __device__ __forceinline__ bool global_wait_le(const int *prog, int target, int 
     return true;
 }
 
+// Issue priority of this wave against the other waves of its SIMD: the
+// stages of the longest chains (the launch's critical path) go first.
+__device__ __forceinline__ void set_priority(int prio) {
+    switch (prio) {
+        case 3: __builtin_amdgcn_s_setprio(3); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
+// LDS accesses of the stage-to-stage ring.  Relaxed atomics so that the
+// compiler neither caches nor reorders them; LDS executes the instructions of
+// one wave in issue order, which is the only ordering the ring relies on.
+template <typename T> __device__ __forceinline__ T lds_ld(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T> __device__ __forceinline__ void lds_st(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ------------------------------------------------------------------ kernel --
 template <typename T, int KIND, int DIR, int L, int W, int R>
 struct Pipeline {
     using G = Geo<KIND>;
     static constexpr int STEP = G::STEP;
-    static constexpr int WS = 2 * L;   // window rows per lane
+    static constexpr int WS = 2 * L;           // window rows per lane
+    static constexpr int NP = L * L;           // rotations (table entries) per tile
+    static constexpr int NS = (NP + 15) / 16;  // lane-distributed registers holding them
+
+    // The table entries of one tile, lane-distributed: register g of lane l
+    // holds entry p = (16 g + l mod 16) mod NP of the tile, p = lj*L + i
+    // (sequence lj, the i-th of its L consecutive entries).
+    struct SC {
+        T s[NS], c[NS];
+    };
 
     struct Shared {
         T ring[W > 1 ? W - 1 : 1][R][L][kLanes];
@@ -153,18 +209,21 @@ struct Pipeline {
         int task;
         int abort_flag;
 #ifdef FTAMD_TRACE
-        unsigned long long tr[W][4];
+        unsigned long long tr[W][8];
 #endif
     };
 
-    // Everything one stage wave needs, uniform except the per-lane column data.
+    // Everything one stage wave needs, uniform except the per-lane data.
     // Rows are "virtual": v = e for hi2lo, v = ne-1-e for lo2hi (lo2hi runs as
     // hi2lo on reversed rows: the transposed rotation of the reversed pair is
     // the forward rotation).
     struct Stage {
-        cptr<T> sc;         // table entry 0 (interleaved s, c)
-        gptr<T> col;        // this lane's column (nullptr: no column)
-        int rs;             // row stride
+        gptr<const SCPair<T>> tab;  // table entry 0
+        gptr<T> A;          // the transform's matrix
+        int coff;           // this lane's column: element offset in A (-1: no column)
+        gptr<T> bin;        // boundary buffer this stage reads (first stage after the first pass), else nullptr
+        gptr<T> bout;       // boundary buffer this stage writes (last stage before the last pass), else nullptr
+        int rs;             // row stride of A
         int n, N;           // rows of the transform, n of the table
         int rpar;           // sphere row parity of the sub-problem (-1: none)
         int ne;             // rows of the sub-problem
@@ -175,9 +234,9 @@ struct Pipeline {
         int ntile;          // tiles
         int jfull;
         bool lanes_all;     // every lane active for every sequence of the stage
-        int lo0;            // DIR 1: lowest virtual row of sequence q0
-        bool coh_load;      // rows of A come from the previous task (device-scope loads)
-        bool coh_store;     // rows of A go to the next task (device-scope stores)
+        int lo0;            // lo2hi: lowest virtual row of sequence q0
+        int lo_next;        // lo2hi: emitted rows below it are final (no later task touches them)
+        int sci[NS];        // this lane's table indices for tile 0
     };
 
     // Per-stage control of one sweep.
@@ -186,76 +245,114 @@ struct Pipeline {
         int prev_emits, emit_off, lo_valid;
         int *prog_prev, *prog_mine;
         int known;          // progress of the previous task seen so far
-        int pend;           // progress to publish once the last tile's stores are acknowledged
+        int consumed_seen;  // tiles the next stage was last seen to have taken
+#ifdef FTAMD_TRACE
+        unsigned long long tr[8];
+        unsigned long long *stamps;  // per tile: input ready, rotated, emitted (first 160 tiles)
+#endif
     };
 
     __device__ static int seq_j(int j_first, int q) { return DIR == 0 ? j_first - STEP * q : j_first + STEP * q; }
     __device__ static int real_row(const Stage &st, int v) { return G::real(st.rpar, DIR == 0 ? v : st.ne - 1 - v); }
     __device__ static int top_v(const Stage &st, int j) { return G::top(st.n, st.rpar, j); }
 
-    // Table entry of rotation (virtual pair (v, v+1), sequence j).
-    __device__ static long long tab_idx(const Stage &st, int j, int v) {
-        return G::tab(st.N, st.rpar, j, DIR == 0 ? v : st.ne - 2 - v);
-    }
-
     template <int PH> __device__ static constexpr int slot(int i) { return (PH * L + i) % WS; }
 
+    // Tile t sits at virtual position k = K - t*L; sequence lj uses rows
+    // k+lj-(L-1) .. k+lj of its table column (hi2lo) or the mirrored rows
+    // (lo2hi), so every lane's index moves by L per tile.
+    __device__ static void init_sc_index(Stage &st, int lane) {
+#pragma unroll
+        for (int g = 0; g < NS; ++g) {
+            const int p = (16 * g + (lane & 15)) % NP;
+            int lj = p / L;
+            const int i = p % L;
+            if (lj >= st.nq) lj = 0;  // no such sequence: any valid entry (never used)
+            const int j = DIR == 0 ? st.j0 - STEP * lj : st.j0 + STEP * lj;
+            st.sci[g] = (int)(DIR == 0 ? G::tab(st.N, st.rpar, j, st.K + lj) - (L - 1) + i
+                                       : G::tab(st.N, st.rpar, j, st.ne - 2 - (st.K + lj)) + i);
+        }
+    }
+    __device__ static void load_sc(const Stage &st, int t, SC &sc) {
+        const int d = DIR == 0 ? -t * L : t * L;
+#pragma unroll
+        for (int g = 0; g < NS; ++g) {
+            const SCPair<T> e = st.tab[st.sci[g] + d];
+            sc.s[g] = e.x;
+            sc.c[g] = e.y;
+        }
+    }
+
     // One L x L tile at position k (virtual), phase PH of the register ring:
-    // sequence lj rotates the pairs (r, r+1), r = k - lk + lj.
+    // sequence lj rotates the pairs (r, r+1), r = k - lk + lj, with entry
+    // i = L-1-lk (hi2lo) or lk (lo2hi) of its L.  The checked form (tiles at
+    // the ends of a sweep, short stages, stages where some columns need fewer
+    // sequences) computes every rotation and keeps the rows of the ones that
+    // do not apply through selects: it stays one basic block that the
+    // compiler interleaves like the plain form, whereas a branch per rotation
+    // would serialise the FP64 latency chain of the tile.
     template <int PH, bool CHECKED>
-    __device__ static void rotate_tile(const Stage &st, T (&x)[WS], int k) {
+    __device__ static void rotate_tile(const Stage &st, T (&x)[WS], int k, const SC &sc) {
 #pragma unroll
         for (int lj = 0; lj < L; ++lj) {
-            if (CHECKED && lj >= st.nq) break;
             const int j = DIR == 0 ? st.j0 - STEP * lj : st.j0 + STEP * lj;
-            // L consecutive table entries of column j (scalar loads).
-            const long long i0 = DIR == 0 ? tab_idx(st, j, k + lj) - (L - 1) : tab_idx(st, j, k + lj);
-            cptr<T> p = st.sc + 2 * i0;
-            T sv[L], cv[L];
-#pragma unroll
-            for (int i = 0; i < L; ++i) {
-                sv[i] = p[2 * i];
-                cv[i] = p[2 * i + 1];
-            }
             int lo = 0, hi = 0;
+            bool act = true;
             if (CHECKED) {
                 const int top = top_v(st, j);
                 lo = DIR == 0 ? 0 : st.ne - 2 - top;
                 hi = DIR == 0 ? top : st.ne - 2;
+                act = lj < st.nq && (st.lanes_all || j <= st.jmax);
             }
-            const bool act = !CHECKED || st.lanes_all || j <= st.jmax;
 #pragma unroll
             for (int lk = 0; lk < L; ++lk) {
                 const int r = k - lk + lj;
-                if (CHECKED && (r < lo || r > hi)) continue;
-                const int i = DIR == 0 ? L - 1 - lk : lk;
+                const int p = lj * L + (DIR == 0 ? L - 1 - lk : lk);
+                // broadcast with every lane active
+                const T s = bcast_n(sc.s[p / 16], p % 16);
+                const T c = bcast_n(sc.c[p / 16], p % 16);
                 const int px = slot<PH>(L + lk - lj);
                 const int py = slot<PH>(L + lk - lj - 1);
-                if (act) givens<T>(x[px], x[py], sv[i], cv[i]);
+                if (CHECKED) {
+                    const bool apply = act && r >= lo && r <= hi;
+                    T nx = x[px], ny = x[py];
+                    givens<T>(nx, ny, s, c);
+                    x[px] = apply ? nx : x[px];
+                    x[py] = apply ? ny : x[py];
+                } else {
+                    givens<T>(x[px], x[py], s, c);
+                }
             }
         }
     }
 
-    // Row v of this lane's column (0 if outside [lo_valid, ne-1]).  Rows handed
-    // over by another task use device-scope accesses (see publish()); the
-    // first pass reads what the host wrote and the last pass writes what only
-    // the host reads, with ordinary cached accesses.
-    __device__ static T load_row(const Stage &st, int v, int lo_valid) {
-        if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return T(0);
-        gptr<T> ptr = st.col + (long long)real_row(st, v) * st.rs;
-        if (st.coh_load) return __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return *ptr;
+    // Row v of this lane's column in A (0 if outside [lo_valid, ne-1]).
+    __device__ static T load_a(const Stage &st, int v, int lo_valid) {
+        if (v < lo_valid || v > st.ne - 1 || st.coff < 0) return T(0);
+        return st.A[st.coff + real_row(st, v) * st.rs];
     }
-    __device__ static void store_row(const Stage &st, int v, int lo_valid, T val) {
-        if (v < lo_valid || v > st.ne - 1 || st.col == nullptr) return;
-        gptr<T> ptr = st.col + (long long)real_row(st, v) * st.rs;
-        if (st.coh_store) __hip_atomic_store(ptr, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else *ptr = val;
+    // Row v handed over by the previous task, or read from A in the first pass.
+    __device__ static T load_in(const Stage &st, int v, int lo_valid, int lane) {
+        if (v < lo_valid || v > st.ne - 1) return T(0);
+        if (st.bin != nullptr)
+            return __hip_atomic_load(st.bin + (long long)v * kLanes + lane, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        if (st.coff < 0) return T(0);
+        return st.A[st.coff + real_row(st, v) * st.rs];
+    }
+    // Emitted row v: to the next task's boundary buffer, and to A where it is
+    // final (the last pass; in lo2hi also the rows no later pass touches).
+    __device__ static void store_out(const Stage &st, int v, int lo_valid, T val, int lane) {
+        if (v < lo_valid || v > st.ne - 1) return;
+        if (st.bout != nullptr)
+            __hip_atomic_store(st.bout + (long long)v * kLanes + lane, val, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if ((st.bout == nullptr || v < st.lo_next) && st.coff >= 0) st.A[st.coff + real_row(st, v) * st.rs] = val;
     }
 
     // XCD (accelerator complex die) this workgroup runs on.  Each XCD has its
     // own L2; a group's chained tasks are all handed to one XCD so that the
-    // rows they pass through A meet in one L2.
+    // rows they pass on meet in one L2.
     __device__ static int xcd_id(int nxcd) {
         const int id = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));  // HW_REG_XCC_ID[3:0]
         return nxcd > 1 ? id % nxcd : 0;
@@ -284,22 +381,23 @@ struct Pipeline {
             const int g = uniform(a.tasks[2 * task]);
             const int p = uniform(a.tasks[2 * task + 1]);
 #ifdef FTAMD_TRACE
-            if (lane == 0) sh.tr[w][0] = sh.tr[w][1] = sh.tr[w][2] = sh.tr[w][3] = 0;
+            if (lane == 0)
+                for (int i = 0; i < 8; ++i) sh.tr[w][i] = 0;
             const unsigned long long ft_w0 = wall_clock64();
             const unsigned long long ft_c0 = __builtin_amdgcn_s_memtime();
 #endif
+            set_priority(uniform(a.groups[g].prio));
             run_stage(a, sh, g, p, w, lane);
+            set_priority(0);
 #ifdef FTAMD_TRACE
             if (a.trace && lane == 0) {
-                unsigned long long *rec = a.trace + ((long long)task * 16 + w) * 8;
+                unsigned long long *rec = a.trace + ((long long)task * 16 + w) * 512;
                 rec[0] = ft_w0;
                 rec[1] = wall_clock64();
                 rec[2] = __builtin_amdgcn_s_memtime() - ft_c0;
-                rec[3] = sh.tr[w][0];
-                rec[4] = sh.tr[w][1];
-                rec[5] = sh.tr[w][2];
-                rec[6] = ((unsigned long long)x << 32) | (unsigned)(blockIdx.x);
-                rec[7] = ((unsigned long long)g << 32) | (unsigned)p;
+                for (int i = 0; i < 8; ++i) rec[3 + i] = sh.tr[w][i];
+                rec[11] = ((unsigned long long)x << 32) | (unsigned)(blockIdx.x);
+                rec[12] = ((unsigned long long)g << 32) | (unsigned)p;
             }
 #endif
             __syncthreads();
@@ -323,6 +421,14 @@ struct Pipeline {
         }
     }
 
+#ifdef FTAMD_TRACE
+    __device__ static int first_task_of(const FtamdLaunch &a, int g, int p) {
+        for (int i = 0; i < a.ntasks; ++i)
+            if (a.tasks[2 * i] == g && a.tasks[2 * i + 1] == p) return i;
+        return -1;
+    }
+#endif
+
     __device__ static void fail(const FtamdLaunch &a, Shared &sh) {
         __hip_atomic_store(&sh.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         atomicOr(a.err, FTAMD_ERR_TIMEOUT);
@@ -335,17 +441,24 @@ struct Pipeline {
         if (s >= nstages) return;
         const int last = uniform(min(W, nstages - p * W) - 1);
         const bool is_first = w == 0, is_last = w == last;
+        const int npass = uniform(gd.npass);
 
         Stage st;
-        st.sc = (cptr<T>)(a.sc);
+        st.tab = (gptr<const SCPair<T>>)(a.sc);
         st.n = uniform(gd.n);
         st.N = a.ntab;
         st.rpar = uniform(gd.rpar);
         st.ne = G::ne(st.n, st.rpar);
         st.rs = uniform(gd.rs);
-        const long long off = a.lane_off[(long long)g * kLanes + lane];
-        st.col = off < 0 ? nullptr : (gptr<T>)(gd.A) + off;
-        st.jmax = a.lane_jmax[(long long)g * kLanes + lane];
+        st.A = (gptr<T>)(gd.A);
+        st.coff = a.lane_off[g * kLanes + lane];
+        {
+            gptr<T> bnd = (gptr<T>)(a.bnd) + gd.bnd;
+            const long long per = (long long)st.ne * kLanes;
+            st.bin = (is_first && p > 0) ? bnd + (long long)(p - 1) * per : nullptr;
+            st.bout = (is_last && p + 1 < npass) ? bnd + (long long)p * per : nullptr;
+        }
+        st.jmax = a.lane_jmax[g * kLanes + lane];
         const int Q = uniform(gd.Q);
         const int j_first = uniform(gd.j_first);
         st.q0 = s * L;
@@ -360,15 +473,18 @@ struct Pipeline {
             st.K = top_v(st, st.j0);
             st.Bot = 0;
             st.lo0 = 0;
+            st.lo_next = 0;
         } else {
             st.K = st.ne - 1;
             st.Bot = st.ne - 2 - top_v(st, seq_j(j_first, p * W * L));
             st.lo0 = st.ne - 2 - top_v(st, st.j0);
+            st.lo_next = p + 1 < npass ? max(st.ne - 2 - top_v(st, seq_j(j_first, (p + 1) * W * L)), 0) : 0;
         }
         {
             const int E = st.K + 1 - st.Bot;
             st.ntile = E <= 0 ? 1 : (E + L - 1) / L + 1;
         }
+        init_sc_index(st, lane);
         const int emit_off = DIR == 0 ? 0 : 1;
         // Emits the previous stage makes (only needed by consumers).
         int prev_emits = 0;
@@ -387,108 +503,214 @@ struct Pipeline {
         c.emit_off = emit_off;
         c.lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
         c.prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
-        c.prog_mine = p + 1 < uniform(gd.npass) ? a.prog + uniform(gd.prog_base) + p : nullptr;
+        c.prog_mine = p + 1 < npass ? a.prog + uniform(gd.prog_base) + p : nullptr;
         c.known = FTAMD_PROG_NONE;
-        c.pend = FTAMD_PROG_NONE;
-        st.coh_load = c.prog_prev != nullptr;
-        st.coh_store = c.prog_mine != nullptr;
+        c.consumed_seen = 0;
+#ifdef FTAMD_TRACE
+        for (int i = 0; i < 8; ++i) c.tr[i] = 0;
+        c.stamps = nullptr;
+        if (a.trace) {
+            const int task = first_task_of(a, g, p);
+            if (task >= 0) c.stamps = a.trace + ((long long)task * 16 + w) * 512 + 16;
+        }
+#endif
 
         T x[WS];
 #pragma unroll
         for (int i = 0; i < WS; ++i) x[i] = T(0);
+        SC sc[2];
+        T rb[L];  // first stage: rows of the next tile, in flight during this one
+#pragma unroll
+        for (int i = 0; i < L; ++i) rb[i] = T(0);
+        load_sc(st, 0, sc[0]);
 
         // Initial carried rows (positions 0 .. L-1 = virtual rows K+L .. K+1).
-        // hi2lo: untouched by the earlier stages of this task, read from A.
+        // hi2lo: untouched by every earlier stage and task, read from A.
         // lo2hi: above the top row, zero.
         if (DIR == 0) {
-            if (c.prog_prev && !global_wait_le(c.prog_prev, st.K + 1, &c.known, &sh.abort_flag)) { fail(a, sh); return; }
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[i] = load_row(st, st.K + L - i, c.lo_valid);
+            for (int i = 0; i < L; ++i) x[i] = load_a(st, st.K + L - i, c.lo_valid);
+        }
+        if (is_first) {
+            // rows of tile 0: positions L .. 2L-1 = virtual rows K .. K-L+1
+            if (c.prog_prev && !global_wait_le(c.prog_prev, max(st.K - L + 1, c.lo_valid), &c.known, &sh.abort_flag)) {
+                fail(a, sh);
+                return;
+            }
+#pragma unroll
+            for (int i = 0; i < L; ++i) rb[i] = load_in(st, st.K - i, c.lo_valid, lane);
         }
 
         int t = 0;
         for (;;) {
-            if (!tile<0>(a, sh, st, c, x, t, w, lane)) return;
+            if (!tile<0>(a, sh, st, c, x, sc, rb, t, w, lane)) return;
             if (++t >= st.ntile) break;
-            if (!tile<1>(a, sh, st, c, x, t, w, lane)) return;
+            if (!tile<1>(a, sh, st, c, x, sc, rb, t, w, lane)) return;
             if (++t >= st.ntile) break;
         }
         if (is_last && c.prog_mine) publish(c.prog_mine, -1);
+#ifdef FTAMD_TRACE
+        for (int i = 0; i < 8; ++i) sh.tr[w][i] = c.tr[i];
+#endif
     }
 
     template <int PH>
-    __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, Ctl &c, T (&x)[WS], int t, int w,
-                                int lane) {
+    __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, Ctl &c, T (&x)[WS], SC (&sc)[2],
+                                T (&rb)[L], int t, int w, int lane) {
         const int k = st.K - t * L;
+        const bool more = t + 1 < st.ntile;
+        // ---- the next tile's table entries, in flight during this tile
+        if (more) load_sc(st, t + 1, sc[1 - PH]);
         // ---- new rows: positions L .. 2L-1 = virtual rows k .. k-L+1
+        // first stage: the next tile's rows are fetched now if the previous
+        // task is known to have finished them, else after this tile's work
+        const int next_target = max(k - 2 * L + 1, c.lo_valid);
+        bool fetch_later = false;
         if (c.is_first) {
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = rb[i];
+            if (more) {
+                if (!c.prog_prev || c.known <= next_target) {
+#pragma unroll
+                    for (int i = 0; i < L; ++i) rb[i] = load_in(st, k - L - i, c.lo_valid, lane);
+                } else {
+                    fetch_later = true;
+                }
+            }
+        } else if (t < c.prev_emits) {
+            // The flag and the rows are read in one batch: the producer writes
+            // its rows before its flag, so rows read after a flag that shows
+            // them are the producer's.
+            // If the first batch finds the flag short, later polls read only
+            // the flag (sleeping between them) and the rows once it is set.
             FT_TR_BEGIN();
-            const bool ok = !c.prog_prev || global_wait_le(c.prog_prev, max(k - L + 1, c.lo_valid), &c.known,
-                                                            &sh.abort_flag);
+            const T *src = &sh.ring[w - 1][t % R][0][lane];
+            T in[L];
+            bool ok = false;
+            {
+                const int f = lds_ld(&sh.emitted[w - 1]);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+                for (int i = 0; i < L; ++i) in[i] = lds_ld(src + i * kLanes);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                ok = uniform(f) >= t + 1;
+            }
+            for (int it = 0; !ok && it < kSpinLimit; ++it) {
+                __builtin_amdgcn_s_sleep(1);
+                if (uniform(lds_ld(&sh.emitted[w - 1])) >= t + 1) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+                    for (int i = 0; i < L; ++i) in[i] = lds_ld(src + i * kLanes);
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    ok = true;
+                    break;
+                }
+                if (uniform(lds_ld(&sh.abort_flag))) break;
+            }
+            FT_TR_END(0);
+            if (!ok) {
+                fail(a, sh);
+                return false;
+            }
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = in[i];
+            lds_st(&sh.consumed[w - 1], t + 1);
+        } else {
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = T(0);
+        }
+
+        {
+            FT_TR_BEGIN();
+            FT_TR_SYNC();
+            FT_TR_END(3);
+        }
+#ifdef FTAMD_TRACE
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t] = __builtin_amdgcn_s_memtime();
+#endif
+        // ---- rotations
+        {
+            FT_TR_BEGIN();
+            bool fast = st.nq == L && st.lanes_all;
+            if (DIR == 0) fast = fast && k - (L - 1) >= 0;
+            else fast = fast && k + (L - 1) <= st.ne - 2 && k - (L - 1) >= st.lo0;
+            if (fast) rotate_tile<PH, false>(st, x, k, sc[PH]);
+            else rotate_tile<PH, true>(st, x, k, sc[PH]);
+            FT_TR_SYNC();
+            FT_TR_END(4);
+            FT_TR_COUNT(6);
+        }
+#ifdef FTAMD_TRACE
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 1] = __builtin_amdgcn_s_memtime();
+#endif
+
+        // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1
+        if (t >= c.emit_off) {
+            const int e = t - c.emit_off;
+            if (c.is_last) {
+                FT_TR_BEGIN();
+#pragma unroll
+                for (int i = 0; i < L; ++i) store_out(st, k + L - i, c.lo_valid, x[slot<PH>(i)], lane);
+                if (c.prog_mine) {
+                    // Progress of the previous tile once its stores are
+                    // acknowledged.  In the common case this tile stored
+                    // exactly L rows, one instruction each, after them (the
+                    // table prefetch came before them), so vmcnt(L) leaves
+                    // only this tile's stores in flight.  Tiles at the ends of
+                    // the sweep, tiles that also store to A and a first stage
+                    // (whose row loads interleave) wait for everything.
+                    const bool plain = !c.is_first && more && t > c.emit_off && k + 1 >= c.lo_valid &&
+                                       k + L <= st.ne - 1 && !(DIR == 1 && k + 1 < st.lo_next);
+                    if (plain) {
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
+                        __hip_atomic_store(c.prog_mine, max(k + L + 1, c.lo_valid), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    } else {
+                        publish(c.prog_mine, max(k + 1, c.lo_valid));
+                    }
+                }
+                FT_TR_END(2);
+            } else {
+                FT_TR_BEGIN();
+                // ring slot free: the consumer has taken tile e-R (the last
+                // value read is kept, so most tiles read nothing)
+                bool ok = e - R + 1 <= c.consumed_seen;
+                for (int it = 0; !ok && it < kSpinLimit; ++it) {
+                    c.consumed_seen = uniform(lds_ld(&sh.consumed[w]));
+                    if (c.consumed_seen >= e - R + 1) {
+                        ok = true;
+                        break;
+                    }
+                    if (uniform(lds_ld(&sh.abort_flag))) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (!ok) {
+                    fail(a, sh);
+                    return false;
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                T *dst = &sh.ring[w][e % R][0][lane];
+#pragma unroll
+                for (int i = 0; i < L; ++i) lds_st(dst + i * kLanes, x[slot<PH>(i)]);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                lds_st(&sh.emitted[w], e + 1);
+                FT_TR_END(5);
+            }
+        }
+#ifdef FTAMD_TRACE
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 2] = __builtin_amdgcn_s_memtime();
+#endif
+        if (fetch_later) {
+            FT_TR_BEGIN();
+            const bool ok = global_wait_le(c.prog_prev, next_target, &c.known, &sh.abort_flag);
             FT_TR_END(1);
             if (!ok) {
                 fail(a, sh);
                 return false;
             }
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = load_row(st, k - i, c.lo_valid);
-        } else if (t < c.prev_emits) {
-            FT_TR_BEGIN();
-            const bool ok = lds_wait_ge(&sh.emitted[w - 1], t + 1, &sh.abort_flag);
-            FT_TR_END(0);
-            if (!ok) { fail(a, sh); return false; }
-            const T *src = &sh.ring[w - 1][t % R][0][lane];
-#pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = src[i * kLanes];
-            __hip_atomic_store(&sh.consumed[w - 1], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-#pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = T(0);
-        }
-
-        // ---- rotations
-        bool fast = st.nq == L && st.lanes_all;
-        if (DIR == 0) fast = fast && k - (L - 1) >= 0;
-        else fast = fast && k + (L - 1) <= st.ne - 2 && k - (L - 1) >= st.lo0;
-        if (fast) rotate_tile<PH, false>(st, x, k);
-        else rotate_tile<PH, true>(st, x, k);
-
-        // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1
-        if (t >= c.emit_off) {
-            const int e = t - c.emit_off;
-            if (c.is_last) {
-#pragma unroll
-                for (int i = 0; i < L; ++i) store_row(st, k + L - i, c.lo_valid, x[slot<PH>(i)]);
-                if (c.prog_mine) {
-                    // Publish the previous tile's progress once its stores are
-                    // acknowledged, leaving this tile's L stores in flight
-                    // (vmcnt counts only this wave's stores here).  Edge tiles,
-                    // which may store fewer rows, wait for everything.
-                    FT_TR_BEGIN();
-                    const bool full = !c.is_first && k + 1 >= c.lo_valid && k + L <= st.ne - 1;
-                    if (full) {
-                        if (c.pend != FTAMD_PROG_NONE) {
-                            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                            __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
-                            __hip_atomic_store(c.prog_mine, c.pend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        }
-                        c.pend = k + 1;
-                    } else {
-                        publish(c.prog_mine, k + 1);
-                        c.pend = FTAMD_PROG_NONE;
-                    }
-                    FT_TR_END(2);
-                }
-            } else {
-                FT_TR_BEGIN();
-                const bool ok = e < R || lds_wait_ge(&sh.consumed[w], e - R + 1, &sh.abort_flag);
-                FT_TR_END(0);
-                if (!ok) { fail(a, sh); return false; }
-                T *dst = &sh.ring[w][e % R][0][lane];
-#pragma unroll
-                for (int i = 0; i < L; ++i) dst[i * kLanes] = x[slot<PH>(i)];
-                __hip_atomic_store(&sh.emitted[w], e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            for (int i = 0; i < L; ++i) rb[i] = load_in(st, k - L - i, c.lo_valid, lane);
         }
         return true;
     }
@@ -503,12 +725,14 @@ __global__ void __launch_bounds__(W * kLanes) rot_pipeline(FtamdLaunch a) {
 
 using KernelFn = void (*)(FtamdLaunch);
 
+// LDS ring depth (tiles a stage may run ahead of the next).
+constexpr int kRing = 4;
+
 // Kernel for (L, W) if its LDS ring fits the 160 KB of a gfx950 CU.
 template <typename T, int KIND, int DIR, int L, int W>
 KernelFn instance() {
-    constexpr int R = 4;
-    if constexpr (sizeof(typename Pipeline<T, KIND, DIR, L, W, R>::Shared) <= 160 * 1024)
-        return rot_pipeline<T, KIND, DIR, L, W, R>;
+    if constexpr (sizeof(typename Pipeline<T, KIND, DIR, L, W, kRing>::Shared) <= 160 * 1024)
+        return rot_pipeline<T, KIND, DIR, L, W, kRing>;
     else
         return nullptr;
 }

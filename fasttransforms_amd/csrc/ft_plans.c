@@ -103,7 +103,7 @@ void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
     ft_schedule *sc = RP->scheds;
     while (sc) {
         ft_schedule *next = sc->next;
-        ftamd_hip_free(sc->d_mem);
+        ftamd_schedule_free_device(sc);
         free(sc);
         sc = next;
     }

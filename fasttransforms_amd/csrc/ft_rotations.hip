@@ -174,7 +174,7 @@ int ftamd_hip_launch(const FtamdLaunch *a, int precision, int kind, int dir, int
     if (!fn) return FTAMD_EINVAL;
     if (a->ntasks <= 0) return FTAMD_OK;
     if (grid < 1) grid = 1;
-    if (grid > a->ntasks) grid = a->ntasks;
+    if (grid > a->nxcd * a->ntasks) grid = a->nxcd * a->ntasks;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(W * ftamd::kLanes), 0, (hipStream_t)stream, *a);
     return map_err(hipGetLastError());
 }
