@@ -52,6 +52,9 @@ SIGNATURES = {
     "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
     "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
     "ft_gpu_execute_batch": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp]),
+    "ft_gpu_plan_batch": (_vp, [_vp, _i, _i, _i, _vp, _vp, _i]),
+    "ft_gpu_execute_batch_plan": (_i, [_vp, _vp]),
+    "ft_gpu_destroy_batch": (None, [_vp]),
     "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
 }
 for _k in ("sph", "tri", "disk"):
@@ -280,6 +283,43 @@ def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream
     if rc:
         raise GPUError("ft_gpu_execute_batch: %s" % _lib.ft_gpu_error_string(rc).decode())
     return tensors
+
+
+class BatchPlan:
+    """ft_gpu_plan_batch: the schedule of one batch, owned by this object, so
+    that its launch can be captured in a HIP graph (valid while the object
+    lives).  ``execute(stream)`` runs ft_gpu_execute_batch_plan."""
+
+    def __init__(self, plans, direction, tensors, Ms=None, precision=FP64):
+        direction = _DIR.get(direction, direction)
+        count = len(plans)
+        if count == 0 or len(tensors) != count:
+            raise ValueError("one tensor per plan, at least one")
+        if Ms is None:
+            Ms = [_M_of(T, P.n, None) for P, T in zip(plans, tensors)]
+        dtype = np.float64 if precision == FP64 else np.float32
+        ptrs = (_vp * count)(*[_pointer(T, P.n, M, dtype).value for P, T, M in zip(plans, tensors, Ms)])
+        pls = (_vp * count)(*[P.ptr for P in plans])
+        ms = (ctypes.c_int * count)(*[int(M) for M in Ms])
+        self._keep = (list(plans), list(tensors))
+        self.ptr = _lib.ft_gpu_plan_batch(ctypes.cast(pls, _vp), plans[0].kind, int(direction), int(precision),
+                                          ctypes.cast(ptrs, _vp), ctypes.cast(ms, _vp), count)
+        if not self.ptr:
+            raise GPUError("ft_gpu_plan_batch: %s" % _lib.ft_gpu_error_string(_lib.ft_gpu_last_error()).decode())
+        self._destroy = _lib.ft_gpu_destroy_batch
+
+    def execute(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        rc = _lib.ft_gpu_execute_batch_plan(self.ptr, _vp(stream.cuda_stream))
+        if rc:
+            raise GPUError("ft_gpu_execute_batch_plan: %s" % _lib.ft_gpu_error_string(rc).decode())
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._destroy(self.ptr)
+            self.ptr = None
 
 
 def valid_rows(kind, n, j):

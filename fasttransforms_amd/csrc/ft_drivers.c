@@ -415,30 +415,18 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     return S;
 }
 
-/* Point every group of a cached schedule at matrix dA (same n and layout). */
-static int retarget(ft_schedule *S, void *dA) {
-    if (S->ngroups == 0) return FTAMD_OK;
-    FtamdGroup *h = malloc(sizeof(FtamdGroup) * (size_t)S->ngroups);
-    if (!h) return FTAMD_ENOMEM;
-    int rc = ftamd_hip_d2h(h, S->launch.groups, sizeof(FtamdGroup) * (size_t)S->ngroups, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    if (rc == FTAMD_OK && h[0].A != dA) {
-        for (int g = 0; g < S->ngroups; g++) h[g].A = dA;
-        rc = ftamd_hip_h2d((void *)S->launch.groups, h, sizeof(FtamdGroup) * (size_t)S->ngroups, NULL);
-        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    }
-    free(h);
-    return rc;
-}
-
 /* Diagnostics: device buffer that instrumented builds fill with a timeline
  * (8 words per task and stage wave); NULL disables. */
 static void *g_trace = NULL;
 void ftamd_set_trace(void *dev_buffer) { g_trace = dev_buffer; }
 
-static int launch(ft_schedule *S, int precision, int kind, int dir, void *stream) {
-    S->launch.trace = g_trace;
-    return ftamd_hip_launch(&S->launch, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S), stream);
+/* dA: the matrix of every group (single-matrix schedules), or NULL to use the
+ * matrices recorded in the groups (batches). */
+static int launch(ft_schedule *S, int precision, int kind, int dir, void *dA, void *stream) {
+    FtamdLaunch a = S->launch;
+    a.trace = g_trace;
+    a.A = dA;
+    return ftamd_hip_launch(&a, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S), stream);
 }
 
 static int check_kernel_error(ft_schedule *S) {
@@ -472,11 +460,10 @@ static int gpu_execute(ft_rotation_plan *RP, int dir, int precision, void *dA, i
     pthread_mutex_lock(&RP->lock);
     rc = ftamd_plan_upload(RP, precision);
     ft_schedule *S = rc == FTAMD_OK ? matrix_schedule(RP, dir, precision, M, dA, &rc) : NULL;
-    if (S) rc = retarget(S, dA);
     pthread_mutex_unlock(&RP->lock);
     if (rc != FTAMD_OK) return rc;
     if (used) *used = S;
-    return launch(S, precision, RP->kind, dir, stream);
+    return launch(S, precision, RP->kind, dir, dA, stream);
 }
 
 int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
@@ -617,7 +604,7 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
         ft_col c = {d, n, S, 0, jhi, jlo};
         rc = build_from_columns(Sd, kind, dir, Sd->L, Sd->W, &c, 1, ftamd_plan_table(RP, 0), n);
     }
-    if (rc == FTAMD_OK) rc = launch(Sd, 0, kind, dir, NULL);
+    if (rc == FTAMD_OK) rc = launch(Sd, 0, kind, dir, NULL, NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     if (rc == FTAMD_OK) rc = check_kernel_error(Sd);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
@@ -683,25 +670,72 @@ static unsigned long long batch_key(const ft_rotation_plan *const *plans, void *
     return h;
 }
 
-int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
-                         void *const *dA, const int *M, int count, void *stream) {
+/* Validates a batch and uploads the largest plan's table; *big gets that plan. */
+static int batch_check(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                       void *const *dA, const int *M, int count, ft_rotation_plan **big) {
     if (!plans || !dA || !M || count < 0 || (direction != 0 && direction != 1) || (precision != 0 && precision != 1))
         return FT_GPU_EINVAL;
-    if (count == 0) return FT_GPU_OK;
-    ft_rotation_plan *big = NULL;
+    *big = NULL;
     for (int i = 0; i < count; i++) {
         const ft_rotation_plan *P = plans[i];
         if (!P || !dA[i] || P->kind != kind || !valid_M(kind, P->n, M[i])) return FT_GPU_EINVAL;
         if (P->alpha != plans[0]->alpha || P->beta != plans[0]->beta || P->gamma != plans[0]->gamma)
             return FT_GPU_EINVAL;
-        if (!big || P->n > big->n) big = (ft_rotation_plan *)P;
+        if (!*big || P->n > (*big)->n) *big = (ft_rotation_plan *)P;
     }
+    if (count == 0) return FT_GPU_OK;
     int rc = ftamd_hip_device(NULL, NULL);
     if (rc != FTAMD_OK) return rc;
-    pthread_mutex_lock(&big->lock);
-    rc = ftamd_plan_upload(big, precision);
-    pthread_mutex_unlock(&big->lock);
-    if (rc != FTAMD_OK) return rc;
+    pthread_mutex_lock(&(*big)->lock);
+    rc = ftamd_plan_upload(*big, precision);
+    pthread_mutex_unlock(&(*big)->lock);
+    return rc;
+}
+
+/* Builds the schedule of a validated batch. */
+static int batch_build(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                       void *const *dA, const int *M, int count, ft_rotation_plan *big, ft_schedule **out) {
+    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W();
+    long total = 0;
+    for (int i = 0; i < count; i++) total += M[i];
+    ft_col *cols = malloc(sizeof(ft_col) * (size_t)(total > 0 ? total : 1));
+    int ncols = 0;
+    int rc = cols ? FTAMD_OK : FTAMD_ENOMEM;
+    for (int i = 0; rc == FTAMD_OK && i < count; i++) {
+        ft_col *one = NULL;
+        int k = 0;
+        rc = matrix_columns(kind, plans[i]->n, M[i], dA[i], &one, &k);
+        if (rc == FTAMD_OK) memcpy(cols + ncols, one, sizeof(ft_col) * (size_t)k);
+        ncols += k;
+        free(one);
+    }
+    ft_schedule *S = NULL;
+    if (rc == FTAMD_OK) {
+        S = calloc(1, sizeof(ft_schedule));
+        rc = S ? FTAMD_OK : FTAMD_ENOMEM;
+    }
+    if (rc == FTAMD_OK) {
+        S->dir = direction;
+        S->precision = precision;
+        S->L = L;
+        S->W = W;
+        rc = build_from_columns(S, kind, direction, L, W, cols, ncols, ftamd_plan_table(big, precision), big->n);
+    }
+    free(cols);
+    if (rc != FTAMD_OK) {
+        if (S) ftamd_schedule_free_device(S);
+        free(S);
+        return rc;
+    }
+    *out = S;
+    return FTAMD_OK;
+}
+
+int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                         void *const *dA, const int *M, int count, void *stream) {
+    ft_rotation_plan *big = NULL;
+    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
+    if (rc != FTAMD_OK || count == 0) return rc;
     int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W();
     unsigned long long key = batch_key(plans, dA, M, count);
     pthread_mutex_lock(&g_batch.lock);
@@ -714,33 +748,8 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
             free(S);
             g_batch.sched = S = NULL;
         }
-        long total = 0;
-        for (int i = 0; i < count; i++) total += M[i];
-        ft_col *cols = malloc(sizeof(ft_col) * (size_t)(total > 0 ? total : 1));
-        int ncols = 0;
-        rc = cols ? FTAMD_OK : FTAMD_ENOMEM;
-        for (int i = 0; rc == FTAMD_OK && i < count; i++) {
-            ft_col *one = NULL;
-            int k = 0;
-            rc = matrix_columns(kind, plans[i]->n, M[i], dA[i], &one, &k);
-            if (rc == FTAMD_OK) memcpy(cols + ncols, one, sizeof(ft_col) * (size_t)k);
-            ncols += k;
-            free(one);
-        }
-        if (rc == FTAMD_OK) {
-            S = calloc(1, sizeof(ft_schedule));
-            rc = S ? FTAMD_OK : FTAMD_ENOMEM;
-        }
-        if (rc == FTAMD_OK) {
-            S->dir = direction;
-            S->precision = precision;
-            S->L = L;
-            S->W = W;
-            rc = build_from_columns(S, kind, direction, L, W, cols, ncols, ftamd_plan_table(big, precision), big->n);
-        }
-        free(cols);
+        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &S);
         if (rc != FTAMD_OK) {
-            free(S);
             pthread_mutex_unlock(&g_batch.lock);
             return rc;
         }
@@ -753,7 +762,50 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
         g_batch.L = L;
         g_batch.W = W;
     }
-    rc = launch(S, precision, kind, direction, stream);
+    rc = launch(S, precision, kind, direction, NULL, stream);
     pthread_mutex_unlock(&g_batch.lock);
     return rc;
+}
+
+/* ------------------------------------------------------- batch plans -- */
+
+struct ft_gpu_batch_s {
+    ft_schedule *S;
+    int kind, dir, precision, count;
+};
+
+ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                                void *const *dA, const int *M, int count) {
+    ft_rotation_plan *big = NULL;
+    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
+    if (rc == FTAMD_OK && count == 0) rc = FTAMD_EINVAL;
+    ft_gpu_batch *B = NULL;
+    if (rc == FTAMD_OK) {
+        B = calloc(1, sizeof(ft_gpu_batch));
+        rc = B ? FTAMD_OK : FTAMD_ENOMEM;
+    }
+    if (rc == FTAMD_OK) rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &B->S);
+    if (rc != FTAMD_OK) {
+        free(B);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    B->kind = kind;
+    B->dir = direction;
+    B->precision = precision;
+    B->count = count;
+    return B;
+}
+
+int ft_gpu_execute_batch_plan(const ft_gpu_batch *B, void *stream) {
+    if (!B) return FT_GPU_EINVAL;
+    return launch(B->S, B->precision, B->kind, B->dir, NULL, stream);
+}
+
+void ft_gpu_destroy_batch(ft_gpu_batch *B) {
+    if (!B) return;
+    ftamd_hip_sync(NULL);
+    ftamd_schedule_free_device(B->S);
+    free(B->S);
+    free(B);
 }

@@ -56,6 +56,8 @@ typedef struct {
     int nxcd;
     const void *sc;             /* interleaved (s, c) table, entry 0 (padding on both sides) */
     void *bnd;                  /* boundary buffers handing rows from one task to the next */
+    void *A;                    /* matrix of every group when non-NULL (single-matrix launches; passed
+                                   by value, so a captured launch keeps the matrix it was given) */
     int ntab;                   /* n the table was built for (index function) */
     unsigned *ticket;           /* [nxcd] zero at launch; reset by the kernel */
     unsigned *done;             /* zero at launch; reset by the kernel */

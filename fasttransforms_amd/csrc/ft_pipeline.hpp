@@ -450,7 +450,7 @@ struct Pipeline {
         st.rpar = uniform(gd.rpar);
         st.ne = G::ne(st.n, st.rpar);
         st.rs = uniform(gd.rs);
-        st.A = (gptr<T>)(gd.A);
+        st.A = (gptr<T>)(a.A ? a.A : gd.A);
         st.coff = a.lane_off[g * kLanes + lane];
         {
             gptr<T> bnd = (gptr<T>)(a.bnd) + gd.bnd;

@@ -99,9 +99,22 @@ int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int prec
 
 /* Batched transforms of uneven degree in one launch.  plans[i] may differ in
  * n (same kind and parameters); dA[i] is plans[i]->n x M[i] on the device.
- * Every transform runs as one persistent, cost-ordered launch. */
+ * Every transform runs as one persistent, cost-ordered launch.  The schedule
+ * of the last argument list is cached; a call with another list replaces it,
+ * so a HIP graph that captured this call is valid only until then (use a
+ * batch plan below for graphs). */
 int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                          void *const *dA, const int *M, int count, void *stream);
+
+/* A batch plan: the schedule of one argument list of ft_gpu_execute_batch,
+ * owned by the caller (for example to launch it inside a captured HIP graph,
+ * which stays valid until ft_gpu_destroy_batch).  NULL on error
+ * (ft_gpu_last_error tells which). */
+typedef struct ft_gpu_batch_s ft_gpu_batch;
+ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                                void *const *dA, const int *M, int count);
+int ft_gpu_execute_batch_plan(const ft_gpu_batch *B, void *stream);
+void ft_gpu_destroy_batch(ft_gpu_batch *B);
 
 /* Tuning knobs of the pipelined kernel: sequences per wave (L), stage waves
  * per workgroup (W), workgroups per CU.  0 keeps the built-in choice. */

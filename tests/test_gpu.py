@@ -184,3 +184,38 @@ def test_fp32_path_error(ft, n):
         e2, einf = O.rel2(G, R), O.relinf(G, R)
         _say("fp32 n=%d dir=%d rel2=%.3e relinf=%.3e" % (n, d, e2, einf))
         assert e2 < 8 * np.sqrt(n) * 2.0 ** -23
+
+
+def test_batch_graph_replays(ft):
+    """A batch plan's launch captured in a HIP graph: every replay (with eager
+    work and synchronisation between replays) equals eager execution."""
+    import torch
+    ns = [40, 77, 130, 64]
+    plans = [ft.plan_rotsphere(n) for n in ns]
+    As = [O.fill("sph", n, 2 * n - 1, 300 + n) for n in ns]
+    G = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
+    E = [t.clone() for t in G]
+    BG = ft.BatchPlan(plans, "hi2lo", G)
+    BE = ft.BatchPlan(plans, "hi2lo", E)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        BG.execute(s)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        _say("graph replay", rep)
+        g.replay()
+        torch.cuda.synchronize()
+        BE.execute()
+        torch.cuda.synchronize()
+        for a, b in zip(G, E):
+            assert torch.equal(a, b), rep
+        _ = (E[0] * 2).sum().item()  # unrelated eager work between replays
+    R = [O.execute(O.Plan("sph", n), 0, A.copy(order="F")) for n, A in zip(ns, As)]
+    for _ in range(2):
+        R = [O.execute(O.Plan("sph", n), 0, r.copy(order="F")) for n, r in zip(ns, R)]
+    for r, t in zip(R, G):
+        assert np.array_equal(t.cpu().numpy().T, r)
+    del g
+    torch.cuda.synchronize()
