@@ -150,7 +150,7 @@ static int make_groups(const ft_col *cols, int ncols, int kind, int dir, int L, 
         h->g.j_first = dir == 0 ? jmin + (Q - 1) * step : jmin;
         h->g.Q = Q;
         h->g.nstages = (Q + L - 1) / L;
-        h->g.npass = (h->g.nstages + W - 1) / W;
+        h->g.npass = (h->g.nstages + (W - 2) - 1) / (W - 2);  /* W-2 stage waves per task */
         h->g.jfull = jmaxmin;
         h->g.prog_base = prog;
         h->crit = (split ? c0->n / 2 : c0->n) + Q;
@@ -241,7 +241,7 @@ static int finish_schedule(ft_schedule *S, ft_hgroup *gs, int G, int nprog, cons
         gs[g].g.prio = c4 > 3LL * maxcrit ? 3 : c4 > 2LL * maxcrit ? 2 : c4 > maxcrit ? 1 : 0;
         int ne = gs[g].g.rpar >= 0 ? (gs[g].g.n - gs[g].g.rpar + 1) / 2 : gs[g].g.n;
         gs[g].g.bnd = bnd_elems;
-        bnd_elems += (long long)(gs[g].g.npass - 1) * ne * 64;
+        bnd_elems += (long long)(gs[g].g.npass - 1) * (ne + 2 * FTAMD_BND_PAD) * 64;
     }
     int T = 0;
     for (int g = 0; g < G; g++) T += gs[g].g.npass;

@@ -23,6 +23,10 @@ enum { FTAMD_ERR_TIMEOUT = 1, FTAMD_ERR_INCOMPLETE = 2 };
 /* "No row is final yet" value of a progress slot (rows >= value are final). */
 #define FTAMD_PROG_NONE 0x7fffffff
 
+/* Spare rows on both sides of every boundary buffer (rows handed from one
+ * task to the next), so that an edge tile's emission stores all of its rows. */
+#define FTAMD_BND_PAD 16
+
 /* Padding (table entries) kept before and after every device table so that
  * edge tiles may load a few entries beyond a column without leaving the
  * allocation. */
@@ -38,12 +42,12 @@ typedef struct {
     int j_first;      /* j of sequence 0 */
     int Q;            /* number of sequences */
     int nstages;      /* ceil(Q / L) */
-    int npass;        /* ceil(nstages / W) */
+    int npass;        /* ceil(nstages / (W-2)): a task's workgroup has W-2 stage waves and 2 I/O waves */
     int jfull;        /* every column of the group needs every sequence with j <= jfull */
     int prog_base;    /* first progress slot (one per pass) */
     int rpar;         /* sphere: row parity of this sub-problem (rows rpar, rpar+2, ...); -1 otherwise */
     int prio;         /* wave issue priority 0..3 of its tasks (longest chains highest) */
-    long long bnd;    /* element offset of its boundary buffers: npass-1 of [rows][64] */
+    long long bnd;    /* element offset of its boundary buffers: npass-1 of [rows + 2*FTAMD_BND_PAD][64] */
 } FtamdGroup;
 
 typedef struct {

@@ -17,8 +17,13 @@ constexpr int kLanes = 64;
 // cycles spent in each part of a tile.  The product build compiles all of it
 // away.  Slots: 0 LDS-ring input wait, 1 progress wait, 2 row stores and
 // progress publication, 3 input rows arriving, 4 rotations, 5 LDS-ring
-// output, 6 tiles.
-#ifdef FTAMD_TRACE
+// output, 6 tiles.  -DFTAMD_STAMPS keeps only the per-tile clock stamps
+// (input taken, rotated, emitted), which perturb the timing least.
+#if defined(FTAMD_STAMPS) && !defined(FTAMD_TRACE)
+#define FTAMD_TRACE
+#define FTAMD_TRACE_LITE
+#endif
+#if defined(FTAMD_TRACE) && !defined(FTAMD_TRACE_LITE)
 #define FT_TR_BEGIN() const unsigned long long ft_tr0 = __builtin_amdgcn_s_memtime()
 #define FT_TR_END(slot) c.tr[slot] += __builtin_amdgcn_s_memtime() - ft_tr0
 #define FT_TR_COUNT(slot) c.tr[slot] += 1
@@ -202,10 +207,18 @@ struct Pipeline {
         T s[NS], c[NS];
     };
 
+    // A task's workgroup: WC compute stages (waves 0 .. WC-1), an importer
+    // (wave WC) that feeds stage 0 the rows it reads from A or from the
+    // previous task, and an exporter (wave WC+1) that takes the last stage's
+    // rows to the next task or to A.  Compute stages talk only through LDS
+    // rings; the global-memory latencies of the hand-off between tasks stay
+    // with the two I/O waves.
+    static constexpr int WC = W - 2;
+
     struct Shared {
-        T ring[W > 1 ? W - 1 : 1][R][L][kLanes];
-        int emitted[W];
-        int consumed[W];
+        T ring[WC + 1][R][L][kLanes];  // ring r: written by stage r-1 (the importer for r = 0)
+        int emitted[WC + 1];
+        int consumed[WC + 1];
         int task;
         int abort_flag;
 #ifdef FTAMD_TRACE
@@ -213,7 +226,7 @@ struct Pipeline {
 #endif
     };
 
-    // Everything one stage wave needs, uniform except the per-lane data.
+    // Everything one stage needs, uniform except the per-lane data.
     // Rows are "virtual": v = e for hi2lo, v = ne-1-e for lo2hi (lo2hi runs as
     // hi2lo on reversed rows: the transposed rotation of the reversed pair is
     // the forward rotation).
@@ -221,8 +234,8 @@ struct Pipeline {
         gptr<const SCPair<T>> tab;  // table entry 0
         gptr<T> A;          // the transform's matrix
         int coff;           // this lane's column: element offset in A (-1: no column)
-        gptr<T> bin;        // boundary buffer this stage reads (first stage after the first pass), else nullptr
-        gptr<T> bout;       // boundary buffer this stage writes (last stage before the last pass), else nullptr
+        gptr<T> bin;        // boundary buffer the task reads (row 0 of it), nullptr in the first pass
+        gptr<T> bout;       // boundary buffer the task writes, nullptr in the last pass
         int rs;             // row stride of A
         int n, N;           // rows of the transform, n of the table
         int rpar;           // sphere row parity of the sub-problem (-1: none)
@@ -236,16 +249,16 @@ struct Pipeline {
         bool lanes_all;     // every lane active for every sequence of the stage
         int lo0;            // lo2hi: lowest virtual row of sequence q0
         int lo_next;        // lo2hi: emitted rows below it are final (no later task touches them)
+        int lo_valid;       // lowest virtual row the task reads or writes
+        int emit_off;       // tiles before the first emission
         int sci[NS];        // this lane's table indices for tile 0
     };
 
     // Per-stage control of one sweep.
     struct Ctl {
-        bool is_first, is_last;
-        int prev_emits, emit_off, lo_valid;
-        int *prog_prev, *prog_mine;
-        int known;          // progress of the previous task seen so far
+        int prev_emits;     // emissions the ring this stage reads will carry
         int consumed_seen;  // tiles the next stage was last seen to have taken
+        int prio;           // issue priority of the stage's other tiles
 #ifdef FTAMD_TRACE
         unsigned long long tr[8];
         unsigned long long *stamps;  // per tile: input ready, rotated, emitted (first 160 tiles)
@@ -326,28 +339,57 @@ struct Pipeline {
         }
     }
 
-    // Row v of this lane's column in A (0 if outside [lo_valid, ne-1]).
-    __device__ static T load_a(const Stage &st, int v, int lo_valid) {
-        if (v < lo_valid || v > st.ne - 1 || st.coff < 0) return T(0);
-        return st.A[st.coff + real_row(st, v) * st.rs];
+    // hi2lo tile that reaches below row 0 (k - (L-1) < 0) in a stage whose
+    // every lane needs all L sequences: only rotations with r = KB - lk + lj
+    // >= 0 apply, and with KB a compile-time constant the others vanish from
+    // the code.  These tiles carry the bottom rows from stage to stage, the
+    // critical path of the launch, so they get straight-line code with no
+    // wasted rotation.
+    template <int PH, int KB>
+    __device__ static void rotate_bottom(T (&x)[WS], const SC &sc) {
+#pragma unroll
+        for (int lj = 0; lj < L; ++lj)
+#pragma unroll
+            for (int lk = 0; lk < L; ++lk) {
+                if (KB - lk + lj < 0) continue;
+                const int p = lj * L + (L - 1 - lk);
+                const T s = bcast_n(sc.s[p / 16], p % 16);
+                const T c = bcast_n(sc.c[p / 16], p % 16);
+                givens<T>(x[slot<PH>(L + lk - lj)], x[slot<PH>(L + lk - lj - 1)], s, c);
+            }
     }
+    // Dispatch k in [-L, L-2] to its specialisation.
+    template <int PH, int KB>
+    __device__ static void bottom_at(int k, T (&x)[WS], const SC &sc) {
+        if constexpr (KB <= L - 2) {
+            if (k == KB) rotate_bottom<PH, KB>(x, sc);
+            else bottom_at<PH, KB + 1>(k, x, sc);
+        }
+    }
+
     // Row v handed over by the previous task, or read from A in the first pass.
-    __device__ static T load_in(const Stage &st, int v, int lo_valid, int lane) {
-        if (v < lo_valid || v > st.ne - 1) return T(0);
+    __device__ static T load_in(const Stage &st, int v, int lane) {
+        if (v < st.lo_valid || v > st.ne - 1) return T(0);
         if (st.bin != nullptr)
             return __hip_atomic_load(st.bin + (long long)v * kLanes + lane, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
         if (st.coff < 0) return T(0);
         return st.A[st.coff + real_row(st, v) * st.rs];
     }
-    // Emitted row v: to the next task's boundary buffer, and to A where it is
-    // final (the last pass; in lo2hi also the rows no later pass touches).
-    __device__ static void store_out(const Stage &st, int v, int lo_valid, T val, int lane) {
-        if (v < lo_valid || v > st.ne - 1) return;
+    // Emitted row v: every one goes to the next task's boundary buffer (which
+    // has FTAMD_BND_PAD spare rows on both sides, so each emission is exactly
+    // L stores), and the final ones to A (the last pass; in lo2hi also the
+    // rows no later pass touches).  Returns whether it wrote A.
+    __device__ static bool store_out(const Stage &st, int v, T val, int lane) {
         if (st.bout != nullptr)
             __hip_atomic_store(st.bout + (long long)v * kLanes + lane, val, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        if ((st.bout == nullptr || v < st.lo_next) && st.coff >= 0) st.A[st.coff + real_row(st, v) * st.rs] = val;
+        if (v < st.lo_valid || v > st.ne - 1) return false;
+        if ((st.bout == nullptr || v < st.lo_next) && st.coff >= 0) {
+            st.A[st.coff + real_row(st, v) * st.rs] = val;
+            return true;
+        }
+        return false;
     }
 
     // XCD (accelerator complex die) this workgroup runs on.  Each XCD has its
@@ -370,7 +412,7 @@ struct Pipeline {
                 sh.task = (int)atomicAdd(a.ticket + x, 1u);
                 sh.abort_flag = 0;
             }
-            if (threadIdx.x < W) {
+            if (threadIdx.x < WC + 1) {
                 sh.emitted[threadIdx.x] = 0;
                 sh.consumed[threadIdx.x] = 0;
             }
@@ -386,8 +428,14 @@ struct Pipeline {
             const unsigned long long ft_w0 = wall_clock64();
             const unsigned long long ft_c0 = __builtin_amdgcn_s_memtime();
 #endif
-            set_priority(uniform(a.groups[g].prio));
-            run_stage(a, sh, g, p, w, lane);
+            if (w < WC) {
+                set_priority(uniform(a.groups[g].prio));
+                run_stage(a, sh, g, p, w, lane);
+            } else {
+                __builtin_amdgcn_s_setprio(3);
+                if (w == WC) run_importer(a, sh, g, p, lane);
+                else run_exporter(a, sh, g, p, lane);
+            }
             set_priority(0);
 #ifdef FTAMD_TRACE
             if (a.trace && lane == 0) {
@@ -433,17 +481,16 @@ struct Pipeline {
         __hip_atomic_store(&sh.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         atomicOr(a.err, FTAMD_ERR_TIMEOUT);
     }
+    __device__ static bool aborted(Shared &sh) { return uniform(lds_ld(&sh.abort_flag)) != 0; }
 
-    __device__ static void run_stage(const FtamdLaunch &a, Shared &sh, int g, int p, int w, int lane) {
+    // Geometry of compute stage c of pass p of group g (false if the group
+    // has no such stage).
+    __device__ static bool setup(const FtamdLaunch &a, int g, int p, int c, int lane, Stage &st) {
         const FtamdGroup &gd = a.groups[g];
         const int nstages = uniform(gd.nstages);
-        const int s = p * W + w;
-        if (s >= nstages) return;
-        const int last = uniform(min(W, nstages - p * W) - 1);
-        const bool is_first = w == 0, is_last = w == last;
+        const int s = p * WC + c;
+        if (s >= nstages) return false;
         const int npass = uniform(gd.npass);
-
-        Stage st;
         st.tab = (gptr<const SCPair<T>>)(a.sc);
         st.n = uniform(gd.n);
         st.N = a.ntab;
@@ -454,9 +501,9 @@ struct Pipeline {
         st.coff = a.lane_off[g * kLanes + lane];
         {
             gptr<T> bnd = (gptr<T>)(a.bnd) + gd.bnd;
-            const long long per = (long long)st.ne * kLanes;
-            st.bin = (is_first && p > 0) ? bnd + (long long)(p - 1) * per : nullptr;
-            st.bout = (is_last && p + 1 < npass) ? bnd + (long long)p * per : nullptr;
+            const long long per = (long long)(st.ne + 2 * FTAMD_BND_PAD) * kLanes;
+            st.bin = p > 0 ? bnd + (long long)(p - 1) * per + FTAMD_BND_PAD * kLanes : nullptr;
+            st.bout = p + 1 < npass ? bnd + (long long)p * per + FTAMD_BND_PAD * kLanes : nullptr;
         }
         st.jmax = a.lane_jmax[g * kLanes + lane];
         const int Q = uniform(gd.Q);
@@ -476,36 +523,161 @@ struct Pipeline {
             st.lo_next = 0;
         } else {
             st.K = st.ne - 1;
-            st.Bot = st.ne - 2 - top_v(st, seq_j(j_first, p * W * L));
+            st.Bot = st.ne - 2 - top_v(st, seq_j(j_first, p * WC * L));
             st.lo0 = st.ne - 2 - top_v(st, st.j0);
-            st.lo_next = p + 1 < npass ? max(st.ne - 2 - top_v(st, seq_j(j_first, (p + 1) * W * L)), 0) : 0;
+            st.lo_next = p + 1 < npass ? max(st.ne - 2 - top_v(st, seq_j(j_first, (p + 1) * WC * L)), 0) : 0;
         }
         {
             const int E = st.K + 1 - st.Bot;
             st.ntile = E <= 0 ? 1 : (E + L - 1) / L + 1;
         }
+        st.lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
+        st.emit_off = DIR == 0 ? 0 : 1;
+        return true;
+    }
+
+    // Last compute stage of pass p.
+    __device__ static int last_stage(const FtamdLaunch &a, int g, int p) {
+        return uniform(min(WC, a.groups[g].nstages - p * WC) - 1);
+    }
+
+    // ------------------------------------------------------------ importer --
+    // Feeds stage 0 the rows of each of its tiles: from A in the first pass,
+    // else from the previous task's boundary buffer once its progress shows
+    // them final.  The next tile's rows are in flight while this tile's wait
+    // for a free ring slot.
+    __device__ static void run_importer(const FtamdLaunch &a, Shared &sh, int g, int p, int lane) {
+        Stage st;
+        if (!setup(a, g, p, 0, lane, st)) return;
+        const int *prog = p > 0 ? a.prog + uniform(a.groups[g].prog_base) + p - 1 : nullptr;
+        int known = FTAMD_PROG_NONE, consumed_seen = 0;
+        T cur[L], nxt[L];
+        if (prog && !global_wait_le(prog, max(st.K - L + 1, st.lo_valid), &known, &sh.abort_flag)) {
+            fail(a, sh);
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < L; ++i) cur[i] = load_in(st, st.K - i, lane);
+        for (int t = 0; t < st.ntile; ++t) {
+            const int k = st.K - t * L;
+            const bool more = t + 1 < st.ntile;
+            const int next_target = max(k - 2 * L + 1, st.lo_valid);
+            bool later = false;
+            if (more) {
+                if (!prog || known <= next_target) {
+#pragma unroll
+                    for (int i = 0; i < L; ++i) nxt[i] = load_in(st, k - L - i, lane);
+                } else {
+                    later = true;
+                }
+            }
+            bool ok = t - R + 1 <= consumed_seen;
+            for (int it = 0; !ok && it < kSpinLimit; ++it) {
+                consumed_seen = uniform(lds_ld(&sh.consumed[0]));
+                if (consumed_seen >= t - R + 1) {
+                    ok = true;
+                    break;
+                }
+                if (aborted(sh)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok) {
+                fail(a, sh);
+                return;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            T *dst = &sh.ring[0][t % R][0][lane];
+#pragma unroll
+            for (int i = 0; i < L; ++i) lds_st(dst + i * kLanes, cur[i]);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            lds_st(&sh.emitted[0], t + 1);
+            if (later) {
+                if (!global_wait_le(prog, next_target, &known, &sh.abort_flag)) {
+                    fail(a, sh);
+                    return;
+                }
+#pragma unroll
+                for (int i = 0; i < L; ++i) nxt[i] = load_in(st, k - L - i, lane);
+            }
+#pragma unroll
+            for (int i = 0; i < L; ++i) cur[i] = nxt[i];
+        }
+    }
+
+    // ------------------------------------------------------------ exporter --
+    // Stores the last stage's emissions (boundary buffer and/or A) and
+    // publishes the progress of the next task's input: the previous
+    // emission's, once its stores are acknowledged, leaving this emission's L
+    // stores in flight (vmcnt(L): the exporter issues no other vector-memory
+    // instructions); an emission that also writes A is waited for whole.
+    __device__ static void run_exporter(const FtamdLaunch &a, Shared &sh, int g, int p, int lane) {
+        const int last = last_stage(a, g, p);
+        Stage st;
+        if (!setup(a, g, p, last, lane, st)) return;
+        const int npass = uniform(a.groups[g].npass);
+        int *prog = p + 1 < npass ? a.prog + uniform(a.groups[g].prog_base) + p : nullptr;
+        const int nemit = st.ntile - st.emit_off;
+        const int r = last + 1;
+        bool prev_plain = false;
+        for (int e = 0; e < nemit; ++e) {
+            const int k = st.K - (e + st.emit_off) * L;
+            const T *src = &sh.ring[r][e % R][0][lane];
+            T in[L];
+            bool ok = false;
+            for (int it = 0; it < kSpinLimit; ++it) {
+                if (uniform(lds_ld(&sh.emitted[r])) >= e + 1) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+                    for (int i = 0; i < L; ++i) in[i] = lds_ld(src + i * kLanes);
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    ok = true;
+                    break;
+                }
+                if (aborted(sh)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok) {
+                fail(a, sh);
+                return;
+            }
+            lds_st(&sh.consumed[r], e + 1);
+            bool wrote_a = false;
+#pragma unroll
+            for (int i = 0; i < L; ++i) wrote_a |= store_out(st, k + L - i, in[i], lane);
+            if (prog) {
+                const bool plain = !(DIR == 1 && k + 1 < st.lo_next);
+                if (plain && prev_plain) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
+                    __hip_atomic_store(prog, max(k + L + 1, st.lo_valid), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    publish(prog, max(k + 1, st.lo_valid));
+                }
+                prev_plain = plain;
+            }
+            (void)wrote_a;
+        }
+        if (prog) publish(prog, -1);
+    }
+
+    // --------------------------------------------------------- compute stage
+    __device__ static void run_stage(const FtamdLaunch &a, Shared &sh, int g, int p, int w, int lane) {
+        Stage st;
+        if (!setup(a, g, p, w, lane, st)) return;
         init_sc_index(st, lane);
-        const int emit_off = DIR == 0 ? 0 : 1;
-        // Emits the previous stage makes (only needed by consumers).
-        int prev_emits = 0;
-        if (!is_first) {
-            const int qp = st.q0 - L;
-            const int jp = seq_j(j_first, qp);
+        Ctl c;
+        if (w == 0) {
+            c.prev_emits = st.ntile;  // the importer feeds every tile
+        } else {
+            const int jp = seq_j(uniform(a.groups[g].j_first), st.q0 - L);
             const int Kp = DIR == 0 ? top_v(st, jp) : st.ne - 1;
             const int E = Kp + 1 - st.Bot;
             const int Tp = E <= 0 ? 1 : (E + L - 1) / L + 1;
-            prev_emits = Tp - emit_off;
+            c.prev_emits = Tp - st.emit_off;
         }
-        Ctl c;
-        c.is_first = is_first;
-        c.is_last = is_last;
-        c.prev_emits = prev_emits;
-        c.emit_off = emit_off;
-        c.lo_valid = DIR == 0 ? 0 : max(st.Bot, 0);
-        c.prog_prev = p > 0 ? a.prog + uniform(gd.prog_base) + p - 1 : nullptr;
-        c.prog_mine = p + 1 < npass ? a.prog + uniform(gd.prog_base) + p : nullptr;
-        c.known = FTAMD_PROG_NONE;
         c.consumed_seen = 0;
+        c.prio = min(uniform(a.groups[g].prio), 2);
 #ifdef FTAMD_TRACE
         for (int i = 0; i < 8; ++i) c.tr[i] = 0;
         c.stamps = nullptr;
@@ -519,9 +691,6 @@ struct Pipeline {
 #pragma unroll
         for (int i = 0; i < WS; ++i) x[i] = T(0);
         SC sc[2];
-        T rb[L];  // first stage: rows of the next tile, in flight during this one
-#pragma unroll
-        for (int i = 0; i < L; ++i) rb[i] = T(0);
         load_sc(st, 0, sc[0]);
 
         // Initial carried rows (positions 0 .. L-1 = virtual rows K+L .. K+1).
@@ -529,26 +698,19 @@ struct Pipeline {
         // lo2hi: above the top row, zero.
         if (DIR == 0) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) x[i] = load_a(st, st.K + L - i, c.lo_valid);
-        }
-        if (is_first) {
-            // rows of tile 0: positions L .. 2L-1 = virtual rows K .. K-L+1
-            if (c.prog_prev && !global_wait_le(c.prog_prev, max(st.K - L + 1, c.lo_valid), &c.known, &sh.abort_flag)) {
-                fail(a, sh);
-                return;
+            for (int i = 0; i < L; ++i) {
+                const int v = st.K + L - i;
+                x[i] = (v > st.ne - 1 || st.coff < 0) ? T(0) : st.A[st.coff + real_row(st, v) * st.rs];
             }
-#pragma unroll
-            for (int i = 0; i < L; ++i) rb[i] = load_in(st, st.K - i, c.lo_valid, lane);
         }
 
         int t = 0;
         for (;;) {
-            if (!tile<0>(a, sh, st, c, x, sc, rb, t, w, lane)) return;
+            if (!tile<0>(a, sh, st, c, x, sc, t, w, lane)) return;
             if (++t >= st.ntile) break;
-            if (!tile<1>(a, sh, st, c, x, sc, rb, t, w, lane)) return;
+            if (!tile<1>(a, sh, st, c, x, sc, t, w, lane)) return;
             if (++t >= st.ntile) break;
         }
-        if (is_last && c.prog_mine) publish(c.prog_mine, -1);
 #ifdef FTAMD_TRACE
         for (int i = 0; i < 8; ++i) sh.tr[w][i] = c.tr[i];
 #endif
@@ -556,39 +718,28 @@ struct Pipeline {
 
     template <int PH>
     __device__ static bool tile(const FtamdLaunch &a, Shared &sh, const Stage &st, Ctl &c, T (&x)[WS], SC (&sc)[2],
-                                T (&rb)[L], int t, int w, int lane) {
+                                int t, int w, int lane) {
         const int k = st.K - t * L;
         const bool more = t + 1 < st.ntile;
+        // The rows every later stage waits for pass through a stage in its
+        // last tiles (hi2lo: the bottom rows) or its first ones (lo2hi: the
+        // top virtual rows); those tiles get the highest issue priority.
+        if (DIR == 0 ? t == st.ntile - 2 || (t == 0 && st.ntile < 2) : t == 0) __builtin_amdgcn_s_setprio(3);
+        if (DIR == 0 ? t == 0 && st.ntile >= 2 : t == 2) set_priority(c.prio);
         // ---- the next tile's table entries, in flight during this tile
         if (more) load_sc(st, t + 1, sc[1 - PH]);
         // ---- new rows: positions L .. 2L-1 = virtual rows k .. k-L+1
-        // first stage: the next tile's rows are fetched now if the previous
-        // task is known to have finished them, else after this tile's work
-        const int next_target = max(k - 2 * L + 1, c.lo_valid);
-        bool fetch_later = false;
-        if (c.is_first) {
-#pragma unroll
-            for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = rb[i];
-            if (more) {
-                if (!c.prog_prev || c.known <= next_target) {
-#pragma unroll
-                    for (int i = 0; i < L; ++i) rb[i] = load_in(st, k - L - i, c.lo_valid, lane);
-                } else {
-                    fetch_later = true;
-                }
-            }
-        } else if (t < c.prev_emits) {
+        if (t < c.prev_emits) {
             // The flag and the rows are read in one batch: the producer writes
             // its rows before its flag, so rows read after a flag that shows
-            // them are the producer's.
-            // If the first batch finds the flag short, later polls read only
-            // the flag (sleeping between them) and the rows once it is set.
+            // them are the producer's.  If the batch finds the flag short,
+            // later polls read only the flag and the rows once it is set.
             FT_TR_BEGIN();
-            const T *src = &sh.ring[w - 1][t % R][0][lane];
+            const T *src = &sh.ring[w][t % R][0][lane];
             T in[L];
             bool ok = false;
             {
-                const int f = lds_ld(&sh.emitted[w - 1]);
+                const int f = lds_ld(&sh.emitted[w]);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
 #pragma unroll
                 for (int i = 0; i < L; ++i) in[i] = lds_ld(src + i * kLanes);
@@ -597,7 +748,7 @@ struct Pipeline {
             }
             for (int it = 0; !ok && it < kSpinLimit; ++it) {
                 __builtin_amdgcn_s_sleep(1);
-                if (uniform(lds_ld(&sh.emitted[w - 1])) >= t + 1) {
+                if (uniform(lds_ld(&sh.emitted[w])) >= t + 1) {
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 #pragma unroll
                     for (int i = 0; i < L; ++i) in[i] = lds_ld(src + i * kLanes);
@@ -605,7 +756,7 @@ struct Pipeline {
                     ok = true;
                     break;
                 }
-                if (uniform(lds_ld(&sh.abort_flag))) break;
+                if (aborted(sh)) break;
             }
             FT_TR_END(0);
             if (!ok) {
@@ -614,7 +765,7 @@ struct Pipeline {
             }
 #pragma unroll
             for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = in[i];
-            lds_st(&sh.consumed[w - 1], t + 1);
+            lds_st(&sh.consumed[w], t + 1);
         } else {
 #pragma unroll
             for (int i = 0; i < L; ++i) x[slot<PH>(L + i)] = T(0);
@@ -631,10 +782,12 @@ struct Pipeline {
         // ---- rotations
         {
             FT_TR_BEGIN();
-            bool fast = st.nq == L && st.lanes_all;
+            const bool full = st.nq == L && st.lanes_all;
+            bool fast = full;
             if (DIR == 0) fast = fast && k - (L - 1) >= 0;
             else fast = fast && k + (L - 1) <= st.ne - 2 && k - (L - 1) >= st.lo0;
             if (fast) rotate_tile<PH, false>(st, x, k, sc[PH]);
+            else if (DIR == 0 && full && k >= -L) bottom_at<PH, -L>(k, x, sc[PH]);
             else rotate_tile<PH, true>(st, x, k, sc[PH]);
             FT_TR_SYNC();
             FT_TR_END(4);
@@ -644,74 +797,38 @@ struct Pipeline {
         if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 1] = __builtin_amdgcn_s_memtime();
 #endif
 
-        // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1
-        if (t >= c.emit_off) {
-            const int e = t - c.emit_off;
-            if (c.is_last) {
-                FT_TR_BEGIN();
-#pragma unroll
-                for (int i = 0; i < L; ++i) store_out(st, k + L - i, c.lo_valid, x[slot<PH>(i)], lane);
-                if (c.prog_mine) {
-                    // Progress of the previous tile once its stores are
-                    // acknowledged.  In the common case this tile stored
-                    // exactly L rows, one instruction each, after them (the
-                    // table prefetch came before them), so vmcnt(L) leaves
-                    // only this tile's stores in flight.  Tiles at the ends of
-                    // the sweep, tiles that also store to A and a first stage
-                    // (whose row loads interleave) wait for everything.
-                    const bool plain = !c.is_first && more && t > c.emit_off && k + 1 >= c.lo_valid &&
-                                       k + L <= st.ne - 1 && !(DIR == 1 && k + 1 < st.lo_next);
-                    if (plain) {
-                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                        __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
-                        __hip_atomic_store(c.prog_mine, max(k + L + 1, c.lo_valid), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                    } else {
-                        publish(c.prog_mine, max(k + 1, c.lo_valid));
-                    }
-                }
-                FT_TR_END(2);
-            } else {
-                FT_TR_BEGIN();
-                // ring slot free: the consumer has taken tile e-R (the last
-                // value read is kept, so most tiles read nothing)
-                bool ok = e - R + 1 <= c.consumed_seen;
-                for (int it = 0; !ok && it < kSpinLimit; ++it) {
-                    c.consumed_seen = uniform(lds_ld(&sh.consumed[w]));
-                    if (c.consumed_seen >= e - R + 1) {
-                        ok = true;
-                        break;
-                    }
-                    if (uniform(lds_ld(&sh.abort_flag))) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                if (!ok) {
-                    fail(a, sh);
-                    return false;
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                T *dst = &sh.ring[w][e % R][0][lane];
-#pragma unroll
-                for (int i = 0; i < L; ++i) lds_st(dst + i * kLanes, x[slot<PH>(i)]);
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                lds_st(&sh.emitted[w], e + 1);
-                FT_TR_END(5);
-            }
-        }
-#ifdef FTAMD_TRACE
-        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 2] = __builtin_amdgcn_s_memtime();
-#endif
-        if (fetch_later) {
+        // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1, to the ring
+        // the next stage (or the exporter) reads
+        if (t >= st.emit_off) {
+            const int e = t - st.emit_off;
             FT_TR_BEGIN();
-            const bool ok = global_wait_le(c.prog_prev, next_target, &c.known, &sh.abort_flag);
-            FT_TR_END(1);
+            // ring slot free: the reader has taken emission e-R (the last
+            // value read is kept, so most tiles read nothing)
+            bool ok = e - R + 1 <= c.consumed_seen;
+            for (int it = 0; !ok && it < kSpinLimit; ++it) {
+                c.consumed_seen = uniform(lds_ld(&sh.consumed[w + 1]));
+                if (c.consumed_seen >= e - R + 1) {
+                    ok = true;
+                    break;
+                }
+                if (aborted(sh)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
             if (!ok) {
                 fail(a, sh);
                 return false;
             }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            T *dst = &sh.ring[w + 1][e % R][0][lane];
 #pragma unroll
-            for (int i = 0; i < L; ++i) rb[i] = load_in(st, k - L - i, c.lo_valid, lane);
+            for (int i = 0; i < L; ++i) lds_st(dst + i * kLanes, x[slot<PH>(i)]);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            lds_st(&sh.emitted[w + 1], e + 1);
+            FT_TR_END(5);
         }
+#ifdef FTAMD_TRACE
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 2] = __builtin_amdgcn_s_memtime();
+#endif
         return true;
     }
 };
