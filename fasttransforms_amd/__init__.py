@@ -56,6 +56,7 @@ SIGNATURES = {
     "ft_gpu_execute_batch_plan": (_i, [_vp, _vp]),
     "ft_gpu_destroy_batch": (None, [_vp]),
     "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
+    "ft_gpu_set_engine": (_i, [_i]),
 }
 for _k in ("sph", "tri", "disk"):
     for _dn in ("hi2lo", "lo2hi"):
@@ -149,6 +150,16 @@ def set_tuning(L=0, W=0, wg_per_cu=0):
     rc = _lib.ft_gpu_set_tuning(int(L), int(W), int(wg_per_cu))
     if rc:
         raise ValueError("unsupported tuning (L=%r, W=%r, wg_per_cu=%r)" % (L, W, wg_per_cu))
+
+
+ENGINE_AUTO, ENGINE_PIPELINE, ENGINE_SYSTOLIC = 0, 1, 2
+
+
+def set_engine(engine=ENGINE_AUTO):
+    """Engine of the whole-matrix executors: ENGINE_AUTO, ENGINE_PIPELINE or ENGINE_SYSTOLIC."""
+    rc = _lib.ft_gpu_set_engine(int(engine))
+    if rc:
+        raise ValueError("unknown engine %r" % (engine,))
 
 
 def _is_torch(A):

@@ -57,6 +57,17 @@ int ft_gpu_set_tuning(int L, int W, int wg_per_cu) {
     return FT_GPU_OK;
 }
 
+static int g_engine = FT_GPU_ENGINE_AUTO;
+
+int ft_gpu_set_engine(int engine) {
+    if (engine != FT_GPU_ENGINE_AUTO && engine != FT_GPU_ENGINE_PIPELINE && engine != FT_GPU_ENGINE_SYSTOLIC)
+        return FT_GPU_EINVAL;
+    g_engine = engine;
+    return FT_GPU_OK;
+}
+
+int ftamd_engine(void) { return g_engine; }
+
 int ftamd_tuning_L(int kind, int n) {
     (void)kind;
     (void)n;
@@ -334,7 +345,8 @@ void ftamd_schedule_free_device(ft_schedule *S) {
     if (!S) return;
     ftamd_hip_free(S->d_mem);
     ftamd_hip_free(S->d_bnd);
-    S->d_mem = S->d_bnd = NULL;
+    ftamd_hip_free(S->d_sys);
+    S->d_mem = S->d_bnd = S->d_sys = NULL;
 }
 
 /* Columns of one n x M matrix, ordered so that equal-need columns are adjacent:
@@ -386,7 +398,7 @@ static int build_from_columns(ft_schedule *S, int kind, int dir, int L, int W, c
 static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision, int M, void *dA, int *rc) {
     int L = ftamd_tuning_L(RP->kind, RP->n), W = ftamd_tuning_W();
     for (ft_schedule *s = RP->scheds; s; s = s->next)
-        if (!s->single && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W) {
+        if (!s->single && !s->sys && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W) {
             /* Matrix pointer may change between calls: groups hold it. */
             return s;
         }
@@ -420,9 +432,75 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
 static void *g_trace = NULL;
 void ftamd_set_trace(void *dev_buffer) { g_trace = dev_buffer; }
 
+/* Band-systolic schedule of the whole matrix (cached in the plan; caller
+ * holds RP->lock).  Returns NULL with *rc == FTAMD_OK where the engine does
+ * not apply. */
+static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, int M, int *rc) {
+    *rc = FTAMD_OK;
+    int Sq = ftamd_sys_choose_S(RP->kind, RP->n, precision);
+    if (Sq == 0 || (long long)RP->n * M > 0x7fffffffLL) return NULL;
+    for (ft_schedule *s = RP->scheds; s; s = s->next)
+        if (s->sys && s->M == M && s->dir == dir && s->precision == precision) return s;
+    if (!RP->sys[dir][precision]) {
+        *rc = ftamd_sys_tables_build(RP, dir, precision, Sq, &RP->sys[dir][precision]);
+        if (*rc != FTAMD_OK) return NULL;
+    }
+    const ft_sys_tables *T = RP->sys[dir][precision];
+    FtamdSysChain *ch = NULL;
+    int nch = 0;
+    *rc = ftamd_sys_chains(RP, T, M, &ch, &nch);
+    if (*rc == FTAMD_EINVAL) {
+        *rc = FTAMD_OK;
+        return NULL;
+    }
+    if (*rc != FTAMD_OK) return NULL;
+    ft_schedule *S = calloc(1, sizeof(ft_schedule));
+    if (!S) {
+        free(ch);
+        *rc = FTAMD_ENOMEM;
+        return NULL;
+    }
+    S->sys = 1;
+    S->M = M;
+    S->dir = dir;
+    S->precision = precision;
+    size_t cbytes = sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1);
+    *rc = ftamd_hip_malloc(&S->d_sys, cbytes + 256);
+    if (*rc == FTAMD_OK && nch > 0) *rc = ftamd_hip_h2d(S->d_sys, ch, cbytes, NULL);
+    if (*rc == FTAMD_OK) *rc = ftamd_hip_memset((char *)S->d_sys + cbytes, 0, 256);
+    if (*rc == FTAMD_OK) *rc = ftamd_hip_sync(NULL);
+    free(ch);
+    if (*rc != FTAMD_OK) {
+        ftamd_schedule_free_device(S);
+        free(S);
+        return NULL;
+    }
+    FtamdSysLaunch *a = &S->sys_launch;
+    a->bands = ftamd_sys_tables_dev_bands(T);
+    a->chains = S->d_sys;
+    a->nchains = nch;
+    a->lda = RP->n;
+    a->tab = ftamd_sys_tables_dev_tab(T);
+    a->tab_bytes = (unsigned)ftamd_sys_tables_bytes(T);
+    a->err = (int *)((char *)S->d_sys + cbytes);
+    S->launch.err = a->err;
+    S->L = ftamd_sys_tables_S(T);
+    S->next = RP->scheds;
+    RP->scheds = S;
+    return S;
+}
+
+
+
 /* dA: the matrix of every group (single-matrix schedules), or NULL to use the
  * matrices recorded in the groups (batches). */
 static int launch(ft_schedule *S, int precision, int kind, int dir, void *dA, void *stream) {
+    if (S->sys) {
+        FtamdSysLaunch b = S->sys_launch;
+        b.A = dA;
+        b.trace = g_trace;
+        return ftamd_hip_sys_launch(&b, precision, S->L, stream);
+    }
     FtamdLaunch a = S->launch;
     a.trace = g_trace;
     a.A = dA;
@@ -459,7 +537,20 @@ static int gpu_execute(ft_rotation_plan *RP, int dir, int precision, void *dA, i
     int rc = FTAMD_OK;
     pthread_mutex_lock(&RP->lock);
     rc = ftamd_plan_upload(RP, precision);
-    ft_schedule *S = rc == FTAMD_OK ? matrix_schedule(RP, dir, precision, M, dA, &rc) : NULL;
+    ft_schedule *S = NULL;
+    if (rc == FTAMD_OK && g_engine != FT_GPU_ENGINE_PIPELINE) {
+        S = sys_schedule(RP, dir, precision, M, &rc);
+        if (rc == FTAMD_OK && !S && g_engine == FT_GPU_ENGINE_SYSTOLIC) rc = FTAMD_EINVAL;
+        /* automatic choice: the band-systolic engine while its workgroups
+         * (one per CU) all run at once; beyond that the pipelined engine's
+         * smaller tasks balance the device better */
+        if (rc == FTAMD_OK && S && g_engine == FT_GPU_ENGINE_AUTO) {
+            int cu = 0;
+            if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
+            if (S->sys_launch.nchains > cu) S = NULL;
+        }
+    }
+    if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, dir, precision, M, dA, &rc);
     pthread_mutex_unlock(&RP->lock);
     if (rc != FTAMD_OK) return rc;
     if (used) *used = S;

@@ -72,6 +72,55 @@ typedef struct {
     unsigned long long *trace;  /* diagnostic timeline (instrumented builds only), else NULL */
 } FtamdLaunch;
 
+/* ---- band-systolic engine (ft_systolic.hpp) ----
+ * Lanes are rotation sequences.  A class is the list of sequences one set of
+ * columns shares (sph: one order parity and one row parity; tri: all; disk:
+ * one order parity), numbered k = 0 .. K-1 in application order.  A band is
+ * FTAMD_SYS_LANES * S consecutive sequences of a class; a chain is the bands
+ * one group of FTAMD_SYS_COLS columns needs, one wave per band. */
+#define FTAMD_SYS_LANES 16   /* lanes (sequence slots) of a band row */
+#define FTAMD_SYS_COLS 8     /* columns of a chain: 4 rows of 16 lanes x 2 per lane */
+#define FTAMD_SYS_WAVES 16   /* most bands of a chain (waves of its workgroup) */
+#define FTAMD_SYS_UNROLL 8   /* steps per loop iteration (a band's steps are a multiple) */
+#define FTAMD_SYS_PREFETCH 16 /* most steps of table entries fetched ahead; tables carry that many spare steps */
+#define FTAMD_SYS_MAX_ROWS_FP64 512   /* most class rows of one chain's stream (FP64 / FP32) */
+#define FTAMD_SYS_MAX_ROWS_FP32 1024
+
+typedef struct {
+    long long tab;    /* entry offset of the band's table: [steps + PREFETCH][LANES][S] (s, c) pairs */
+    int R;            /* the band's row stream is rows 0 .. R of the class */
+    int steps;        /* steps the band runs: R + 2*LANES*S rounded up to UNROLL */
+    int gbase;        /* position of the band's first sequence in the class, in application order */
+    int nseq;         /* sequences of the class in the band (the rest are identities) */
+} FtamdSysBand;
+
+typedef struct {
+    int band0;        /* its first band in the band array */
+    int nb;           /* bands, <= FTAMD_SYS_WAVES */
+    int dir;          /* 0 hi2lo: stream rows R .. 0; 1 lo2hi: rows 0 .. R */
+    int rstep, rpar;  /* matrix row of class row e: rstep*e + rpar */
+    int prio;         /* wave issue priority 0..3 (longest chains highest) */
+    int col[FTAMD_SYS_COLS];  /* [row][c]: column of A, -1 if none */
+    int glo[FTAMD_SYS_COLS];  /* the column applies the sequences at positions glo .. ghi */
+    int ghi[FTAMD_SYS_COLS];
+    long long aoff;   /* element offset of the chain's matrix from the launch's A */
+} FtamdSysChain;
+
+typedef struct {
+    const FtamdSysBand *bands;
+    const FtamdSysChain *chains;
+    int nchains;
+    int lda;          /* column stride of A in elements */
+    const void *tab;  /* band tables */
+    unsigned tab_bytes;  /* their size (< 2^31) */
+    void *A;
+    int *err;         /* FTAMD_ERR_* bits */
+    unsigned long long *trace;  /* diagnostic per-wave timeline (instrumented builds only), else NULL */
+} FtamdSysLaunch;
+
+int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int S, void *stream);
+int ftamd_hip_sys_supported(int S, int precision);
+
 int ftamd_hip_device(int *device, int *num_cu);
 int ftamd_hip_xcd_count(int *nxcd);
 int ftamd_hip_pointer_kind(const void *ptr, int *kind);

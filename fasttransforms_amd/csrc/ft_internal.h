@@ -22,6 +22,10 @@ typedef struct ft_schedule {
     FtamdLaunch launch;
     int ngroups, ntasks;
     int xcd_tasks;           /* most tasks any XCD's list holds */
+    /* band-systolic engine (sys != 0): chains on the device, launch record */
+    int sys;
+    FtamdSysLaunch sys_launch;
+    void *d_sys;
     double flops;            /* algorithmic flops (6 per rotation) */
 } ft_schedule;
 
@@ -37,10 +41,13 @@ struct ft_rotation_plan_s {
     int device;              /* HIP device of the uploads, -1 before upload */
     void *d_tab[2];          /* interleaved (s,c) with padding: [0] FP64, [1] FP32 */
     ft_schedule *scheds;
+    struct ft_sys_tables *sys[2][2];  /* band-systolic tables [direction][precision] */
     void *d_stage;           /* staging buffer for host-pointer calls */
     size_t stage_bytes;
     pthread_mutex_t lock;
 };
+
+typedef struct ft_sys_tables ft_sys_tables;
 
 /* ft_plans.c */
 ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, double gamma);
@@ -53,5 +60,16 @@ int ftamd_tuning_L(int kind, int n);
 int ftamd_tuning_W(void);
 int ftamd_tuning_grid(const ft_schedule *S);
 void ftamd_schedule_free_device(ft_schedule *S);
+int ftamd_engine(void);
+
+/* ft_systolic.c */
+int ftamd_sys_choose_S(int kind, int n, int precision);
+int ftamd_sys_tables_build(const ft_rotation_plan *RP, int dir, int precision, int S, ft_sys_tables **out);
+void ftamd_sys_tables_free(ft_sys_tables *T);
+int ftamd_sys_tables_S(const ft_sys_tables *T);
+const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
+const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
+size_t ftamd_sys_tables_bytes(const ft_sys_tables *T);
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count);
 
 #endif /* FT_INTERNAL_H */

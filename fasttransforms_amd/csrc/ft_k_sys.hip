@@ -1,0 +1,21 @@
+// ft_k_sys.hip -- instantiates the band-systolic engine (ft_systolic.hpp) and
+// its C-ABI launcher.
+#include "ft_systolic.hpp"
+
+// S = 4 in FP64 does not fit the 128 VGPRs a 16-wave workgroup allows
+// without spilling; those plans run on the pipelined engine.
+extern "C" int ftamd_hip_sys_supported(int S, int precision) { return S == 2 || (S == 4 && precision == 1); }
+
+extern "C" int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int S, void *stream) {
+    using Fn = void (*)(FtamdSysLaunch);
+    Fn fn = nullptr;
+    if (!ftamd_hip_sys_supported(S, precision)) return FTAMD_EINVAL;
+    if (precision == 0) fn = ftsys::sys_kernel<double, 2>;
+    else fn = S == 2 ? ftsys::sys_kernel<float, 2> : S == 4 ? ftsys::sys_kernel<float, 4> : nullptr;
+    if (!fn) return FTAMD_EINVAL;
+    if (a->nchains <= 0) return FTAMD_OK;
+    hipLaunchKernelGGL(fn, dim3(a->nchains), dim3(ftsys::kWaves * 64), 0, (hipStream_t)stream, *a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return FTAMD_OK;
+    return e == hipErrorOutOfMemory ? FTAMD_ENOMEM : FTAMD_EDEVICE;
+}

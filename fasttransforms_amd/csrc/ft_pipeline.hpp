@@ -552,6 +552,13 @@ struct Pipeline {
         const int *prog = p > 0 ? a.prog + uniform(a.groups[g].prog_base) + p - 1 : nullptr;
         int known = FTAMD_PROG_NONE, consumed_seen = 0;
         T cur[L], nxt[L];
+#ifdef FTAMD_TRACE
+        unsigned long long *stamps = nullptr;
+        if (a.trace) {
+            const int task = first_task_of(a, g, p);
+            if (task >= 0) stamps = a.trace + ((long long)task * 16 + WC) * 512 + 16;
+        }
+#endif
         if (prog && !global_wait_le(prog, max(st.K - L + 1, st.lo_valid), &known, &sh.abort_flag)) {
             fail(a, sh);
             return;
@@ -591,11 +598,18 @@ struct Pipeline {
             for (int i = 0; i < L; ++i) lds_st(dst + i * kLanes, cur[i]);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             lds_st(&sh.emitted[0], t + 1);
+#ifdef FTAMD_TRACE
+            if (stamps && t < 160 && lane == 0) stamps[3 * t] = wall_clock64();
+            if (stamps && t < 160 && lane == 0) stamps[3 * t + 2] = later ? 1 : 0;
+#endif
             if (later) {
                 if (!global_wait_le(prog, next_target, &known, &sh.abort_flag)) {
                     fail(a, sh);
                     return;
                 }
+#ifdef FTAMD_TRACE
+                if (stamps && t < 160 && lane == 0) stamps[3 * t + 1] = wall_clock64();
+#endif
 #pragma unroll
                 for (int i = 0; i < L; ++i) nxt[i] = load_in(st, k - L - i, lane);
             }
@@ -619,6 +633,13 @@ struct Pipeline {
         const int nemit = st.ntile - st.emit_off;
         const int r = last + 1;
         bool prev_plain = false;
+#ifdef FTAMD_TRACE
+        unsigned long long *stamps = nullptr;
+        if (a.trace) {
+            const int task = first_task_of(a, g, p);
+            if (task >= 0) stamps = a.trace + ((long long)task * 16 + WC + 1) * 512 + 16;
+        }
+#endif
         for (int e = 0; e < nemit; ++e) {
             const int k = st.K - (e + st.emit_off) * L;
             const T *src = &sh.ring[r][e % R][0][lane];
@@ -641,6 +662,9 @@ struct Pipeline {
                 return;
             }
             lds_st(&sh.consumed[r], e + 1);
+#ifdef FTAMD_TRACE
+            if (stamps && e < 160 && lane == 0) stamps[3 * e] = wall_clock64();
+#endif
             bool wrote_a = false;
 #pragma unroll
             for (int i = 0; i < L; ++i) wrote_a |= store_out(st, k + L - i, in[i], lane);
@@ -649,6 +673,9 @@ struct Pipeline {
                 if (plain && prev_plain) {
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     __builtin_amdgcn_s_waitcnt(0x0F70 | L);  // vmcnt(L)
+#ifdef FTAMD_TRACE
+                    if (stamps && e < 160 && lane == 0) stamps[3 * e + 1] = wall_clock64();
+#endif
                     __hip_atomic_store(prog, max(k + L + 1, st.lo_valid), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 } else {
@@ -656,6 +683,9 @@ struct Pipeline {
                 }
                 prev_plain = plain;
             }
+#ifdef FTAMD_TRACE
+            if (stamps && e < 160 && lane == 0) stamps[3 * e + 2] = wall_clock64();
+#endif
             (void)wrote_a;
         }
         if (prog) publish(prog, -1);
@@ -777,7 +807,7 @@ struct Pipeline {
             FT_TR_END(3);
         }
 #ifdef FTAMD_TRACE
-        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t] = __builtin_amdgcn_s_memtime();
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t] = wall_clock64();
 #endif
         // ---- rotations
         {
@@ -794,7 +824,7 @@ struct Pipeline {
             FT_TR_COUNT(6);
         }
 #ifdef FTAMD_TRACE
-        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 1] = __builtin_amdgcn_s_memtime();
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 1] = wall_clock64();
 #endif
 
         // ---- emit: positions 0 .. L-1 = virtual rows k+L .. k+1, to the ring
@@ -827,7 +857,7 @@ struct Pipeline {
             FT_TR_END(5);
         }
 #ifdef FTAMD_TRACE
-        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 2] = __builtin_amdgcn_s_memtime();
+        if (c.stamps && t < 160 && lane == 0) c.stamps[3 * t + 2] = wall_clock64();
 #endif
         return true;
     }

@@ -107,6 +107,8 @@ void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
         free(sc);
         sc = next;
     }
+    for (int d = 0; d < 2; d++)
+        for (int p = 0; p < 2; p++) ftamd_sys_tables_free(RP->sys[d][p]);
     ftamd_hip_free(RP->d_tab[0]);
     ftamd_hip_free(RP->d_tab[1]);
     ftamd_hip_free(RP->d_stage);

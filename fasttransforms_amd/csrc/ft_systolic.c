@@ -1,0 +1,315 @@
+/*
+ * ft_systolic.c -- host side of the band-systolic engine (ft_systolic.hpp):
+ * the classes of sequences of a plan, their bands and step-major tables
+ * (built once per plan, direction and precision), and the chains of one
+ * n x M matrix.
+ *
+ * The columns and sequences are those of the reference's executors
+ * (src/drivers.c:58-110, 314-326, 414-430) and kernels
+ * (src/rotations/rotations_default.c:31-71): a column of order m takes
+ * sequences j = m-2, m-4, .. m%2 (sph, disk) or m-1, .. 0 (tri), rotating
+ * rows (k, k+2) (sph) or (k, k+1) (tri, disk) for k = top .. 0 with the
+ * plan's s(k,j), c(k,j) (packed index rotations.c:113-114, 170-171); lo2hi
+ * takes the same sequences in reverse with the transposed rotation.
+ */
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ft_internal.h"
+
+static long idx_tri(int n, int l, int m) { return l + (long)m * (2 * n + 1 - m) / 2; }
+static long idx_disk(int n, int l, int m) { return l + (long)m * n - (long)(m / 2) * ((m + 1) / 2); }
+
+/* One class: sequences k = 0 .. K-1 of j = jtop - jstep*k (hi2lo order). */
+typedef struct {
+    int jtop, jstep, K;
+    int rpar, rstep;   /* class row e is matrix row rstep*e + rpar */
+    int jpar;          /* order parity of its columns (sph, disk); -1 for tri */
+} sys_class;
+
+static int class_top(int kind, int n, const sys_class *cl, int j) {
+    if (kind == FTAMD_SPH) {
+        int x = n - 3 - j - cl->rpar;
+        return x >= 0 ? x / 2 : -1;
+    }
+    if (kind == FTAMD_TRI) return n - 2 - j;
+    return n - 2 - (j + 1) / 2;
+}
+
+static long class_index(int kind, int n, const sys_class *cl, int e, int j) {
+    int row = cl->rstep * e + cl->rpar;
+    return kind == FTAMD_DISK ? idx_disk(n, row, j) : idx_tri(n, row, j);
+}
+
+static int make_classes(int kind, int n, sys_class *cl) {
+    int nc = 0;
+    if (kind == FTAMD_TRI) {
+        if (n >= 2) {
+            sys_class c = {n - 2, 1, n - 1, 0, 1, -1};
+            cl[nc++] = c;
+        }
+        return nc;
+    }
+    int jmax = kind == FTAMD_SPH ? n - 3 : 2 * n - 4;
+    for (int jpar = 0; jpar < 2; jpar++) {
+        int jtop = jmax - ((jmax - jpar) % 2 + 2) % 2;
+        if (jtop < jpar) continue;
+        int K = (jtop - jpar) / 2 + 1;
+        for (int rpar = 0; rpar < (kind == FTAMD_SPH ? 2 : 1); rpar++) {
+            sys_class c = {jtop, 2, K, rpar, kind == FTAMD_SPH ? 2 : 1, jpar};
+            cl[nc++] = c;
+        }
+    }
+    return nc;
+}
+
+static int round_up(int x, int u) { return (x + u - 1) / u * u; }
+
+/* Tables of every band of every class for one direction and precision. */
+struct ft_sys_tables {
+    int S, dir, precision, ncls;
+    sys_class cls[4];
+    int band0[4], nb[4];     /* bands of class c: band0[c] .. band0[c]+nb[c]-1 */
+    FtamdSysBand *bands;     /* host copy */
+    int nbands;
+    void *d_bands;
+    void *d_tab;
+    size_t tab_bytes;
+};
+
+int ftamd_sys_tables_S(const ft_sys_tables *T) { return T->S; }
+const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T) { return T->d_bands; }
+const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T) { return T->d_tab; }
+size_t ftamd_sys_tables_bytes(const ft_sys_tables *T) { return T->tab_bytes; }
+
+void ftamd_sys_tables_free(ft_sys_tables *T) {
+    if (!T) return;
+    ftamd_hip_free(T->d_bands);
+    ftamd_hip_free(T->d_tab);
+    free(T->bands);
+    free(T);
+}
+
+/* Sequences per band lane for a plan: the smallest S whose chains fit in
+ * FTAMD_SYS_WAVES bands, 0 if none does. */
+int ftamd_sys_choose_S(int kind, int n, int precision) {
+    sys_class cl[4];
+    int nc = make_classes(kind, n, cl);
+    /* every class row of a chain's stream is staged in LDS once */
+    if ((kind == FTAMD_SPH ? (n + 1) / 2 : n) > (precision == 0 ? FTAMD_SYS_MAX_ROWS_FP64 : FTAMD_SYS_MAX_ROWS_FP32))
+        return 0;
+    static const int opts[] = {2, 4};
+    for (int o = 0; o < 2; o++) {
+        int B = FTAMD_SYS_LANES * opts[o], ok = 1;
+        for (int c = 0; c < nc; c++)
+            if ((cl[c].K + B - 1) / B > FTAMD_SYS_WAVES) ok = 0;
+        if (ok && ftamd_hip_sys_supported(opts[o], precision)) return opts[o];
+    }
+    return 0;
+}
+
+/* Bands of every class and their tables, on the host (T->bands, *host). */
+static int tables_host(const ft_rotation_plan *RP, int dir, int precision, int S, ft_sys_tables *T, void **host,
+                       long long *nentries) {
+    const int kind = RP->kind, n = RP->n, B = FTAMD_SYS_LANES * S, U = FTAMD_SYS_UNROLL, PF = FTAMD_SYS_PREFETCH;
+    *host = NULL;
+    T->S = S;
+    T->dir = dir;
+    T->precision = precision;
+    T->ncls = make_classes(kind, n, T->cls);
+    long long entries = 0;
+    for (int c = 0; c < T->ncls; c++) {
+        T->band0[c] = T->nbands;
+        T->nb[c] = (T->cls[c].K + B - 1) / B;
+        T->nbands += T->nb[c];
+    }
+    T->bands = calloc((size_t)(T->nbands > 0 ? T->nbands : 1), sizeof(FtamdSysBand));
+    if (!T->bands) return FTAMD_ENOMEM;
+    for (int c = 0; c < T->ncls; c++) {
+        const sys_class *cl = &T->cls[c];
+        for (int b = 0; b < T->nb[c]; b++) {
+            FtamdSysBand *bd = &T->bands[T->band0[c] + b];
+            int g0 = B * b, nseq = cl->K - g0 < B ? cl->K - g0 : B;
+            /* the stream's top row: one above the longest sequence's top */
+            int klong = dir == 0 ? g0 + nseq - 1 : cl->K - 1 - g0;
+            bd->R = class_top(kind, n, cl, cl->jtop - cl->jstep * klong) + 1;
+            if (bd->R < 0) bd->R = 0;
+            bd->steps = round_up(bd->R + 2 * B, U);
+            bd->gbase = g0;
+            bd->nseq = nseq;
+            bd->tab = entries;
+            entries += (long long)(bd->steps + PF) * FTAMD_SYS_LANES * S;
+        }
+    }
+    size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
+    void *h = malloc((size_t)(entries > 0 ? entries : 1) * 2 * esz);
+    if (!h) return FTAMD_ENOMEM;
+    for (int c = 0; c < T->ncls; c++) {
+        const sys_class *cl = &T->cls[c];
+        for (int b = 0; b < T->nb[c]; b++) {
+            const FtamdSysBand *bd = &T->bands[T->band0[c] + b];
+            for (int t = 0; t < bd->steps + PF; t++)
+                for (int q = 0; q < B; q++) {
+                    /* lane i holds sequences q = S*i .. S*i+S-1: entry (t, i, s) */
+                    long long at = bd->tab + (long long)t * B + q;
+                    double sv = 0.0, cv = 1.0;
+                    if (q < bd->nseq) {
+                        int g = bd->gbase + q;
+                        int k = dir == 0 ? g : cl->K - 1 - g;
+                        int j = cl->jtop - cl->jstep * k;
+                        /* class row whose rotation sequence q does at step t */
+                        int e = dir == 0 ? bd->R + 2 * q - t : t - 2 * q - 1;
+                        if (e >= 0 && e <= class_top(kind, n, cl, j)) {
+                            long ix = class_index(kind, n, cl, e, j);
+                            sv = RP->s[ix];
+                            cv = RP->c[ix];
+                        }
+                    }
+                    if (precision == 0) {
+                        ((double *)h)[2 * at] = sv;
+                        ((double *)h)[2 * at + 1] = cv;
+                    } else {
+                        ((float *)h)[2 * at] = (float)sv;
+                        ((float *)h)[2 * at + 1] = (float)cv;
+                    }
+                }
+        }
+    }
+    *host = h;
+    *nentries = entries;
+    return FTAMD_OK;
+}
+
+int ftamd_sys_tables_build(const ft_rotation_plan *RP, int dir, int precision, int S, ft_sys_tables **out) {
+    *out = NULL;
+    ft_sys_tables *T = calloc(1, sizeof(ft_sys_tables));
+    if (!T) return FTAMD_ENOMEM;
+    void *host = NULL;
+    long long entries = 0;
+    int rc = tables_host(RP, dir, precision, S, T, &host, &entries);
+    size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
+    T->tab_bytes = (size_t)entries * 2 * esz;
+    /* the kernel addresses the tables with 32-bit byte offsets */
+    if (rc == FTAMD_OK && T->tab_bytes >= 0x7fffffffULL) rc = FTAMD_EINVAL;
+    if (rc == FTAMD_OK) rc = ftamd_hip_malloc(&T->d_tab, (size_t)(entries > 0 ? entries : 1) * 2 * esz);
+    if (rc == FTAMD_OK)
+        rc = ftamd_hip_malloc(&T->d_bands, sizeof(FtamdSysBand) * (size_t)(T->nbands > 0 ? T->nbands : 1));
+    if (rc == FTAMD_OK && entries > 0) rc = ftamd_hip_h2d(T->d_tab, host, (size_t)entries * 2 * esz, NULL);
+    if (rc == FTAMD_OK && T->nbands > 0)
+        rc = ftamd_hip_h2d(T->d_bands, T->bands, sizeof(FtamdSysBand) * (size_t)T->nbands, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    free(host);
+    if (rc != FTAMD_OK) {
+        ftamd_sys_tables_free(T);
+        return rc;
+    }
+    *out = T;
+    return FTAMD_OK;
+}
+
+static int cmp_nb_desc(const void *x, const void *y) {
+    const FtamdSysChain *a = x, *b = y;
+    if (a->nb != b->nb) return b->nb - a->nb;
+    return a->band0 - b->band0;
+}
+
+/* Chains of an n x M matrix: per class, the columns in groups of 8 (four
+ * orders of the class's parity x (cos, sin) columns for sph and disk, eight
+ * consecutive orders for tri), each with the bands its columns need. */
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
+    const int kind = RP->kind, B = FTAMD_SYS_LANES * T->S;
+    *out = NULL;
+    *count = 0;
+    int cap = 16, nch = 0;
+    FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)cap);
+    if (!ch) return FTAMD_ENOMEM;
+    for (int c = 0; c < T->ncls; c++) {
+        const sys_class *cl = &T->cls[c];
+        /* orders of the class, highest first */
+        int mhi = kind == FTAMD_TRI ? M - 1 : (M - 1) / 2;
+        if (kind != FTAMD_TRI && mhi % 2 != cl->jpar) mhi--;
+        int mlo = kind == FTAMD_TRI ? 1 : 2;
+        for (int m0 = mhi; m0 >= mlo; m0 -= 8) {
+            FtamdSysChain x;
+            memset(&x, 0, sizeof(x));
+            int kmin = INT_MAX, any = 0;
+            for (int q = 0; q < FTAMD_SYS_COLS; q++) {
+                int m, col;
+                if (kind == FTAMD_TRI) {
+                    m = m0 - q;
+                    col = m;
+                } else {
+                    m = m0 - 2 * (q / 2);
+                    col = 2 * m - 1 + (q % 2);
+                }
+                int jfirst = kind == FTAMD_TRI ? m - 1 : m - 2;
+                int ok = m >= mlo && col < M && jfirst <= cl->jtop && jfirst >= cl->jtop - cl->jstep * (cl->K - 1);
+                x.col[q] = ok ? col : -1;
+                x.glo[q] = INT_MIN;
+                x.ghi[q] = INT_MAX;
+                if (!ok) continue;
+                int ks = (cl->jtop - jfirst) / cl->jstep;
+                if (T->dir == 0) x.glo[q] = ks;
+                else x.ghi[q] = cl->K - 1 - ks;
+                if (ks < kmin) kmin = ks;
+                any = 1;
+            }
+            if (!any) continue;
+            int b_first, b_last;
+            if (T->dir == 0) {
+                b_first = kmin / B;
+                b_last = T->nb[c] - 1;
+            } else {
+                b_first = 0;
+                b_last = (cl->K - 1 - kmin) / B;
+            }
+            x.band0 = T->band0[c] + b_first;
+            x.nb = b_last - b_first + 1;
+            x.dir = T->dir;
+            x.rstep = cl->rstep;
+            x.rpar = cl->rpar;
+            x.prio = x.nb >= 12 ? 3 : x.nb >= 8 ? 2 : x.nb >= 4 ? 1 : 0;
+            x.aoff = 0;
+            if (x.nb > FTAMD_SYS_WAVES) {
+                free(ch);
+                return FTAMD_EINVAL;
+            }
+            if (nch == cap) {
+                cap *= 2;
+                FtamdSysChain *nc = realloc(ch, sizeof(FtamdSysChain) * (size_t)cap);
+                if (!nc) {
+                    free(ch);
+                    return FTAMD_ENOMEM;
+                }
+                ch = nc;
+            }
+            ch[nch++] = x;
+        }
+    }
+    qsort(ch, (size_t)nch, sizeof(FtamdSysChain), cmp_nb_desc);
+    *out = ch;
+    *count = nch;
+    return FTAMD_OK;
+}
+
+/* Diagnostics (no device work): the FP64 bands, table and chains the engine
+ * would run for an n x M matrix, in malloc'd host arrays the caller frees.
+ * Lets the host-side construction be checked on a machine without a GPU. */
+int ftamd_sys_debug_host(const ft_rotation_plan *RP, int dir, int S, int M, FtamdSysBand **bands, int *nbands,
+                         double **tab, long long *nentries, FtamdSysChain **chains, int *nchains) {
+    ft_sys_tables T;
+    memset(&T, 0, sizeof(T));
+    void *host = NULL;
+    int rc = tables_host(RP, dir, 0, S, &T, &host, nentries);
+    if (rc == FTAMD_OK) rc = ftamd_sys_chains(RP, &T, M, chains, nchains);
+    if (rc != FTAMD_OK) {
+        free(T.bands);
+        free(host);
+        return rc;
+    }
+    *bands = T.bands;
+    *nbands = T.nbands;
+    *tab = host;
+    return FTAMD_OK;
+}

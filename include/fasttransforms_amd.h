@@ -120,6 +120,15 @@ void ft_gpu_destroy_batch(ft_gpu_batch *B);
  * per workgroup (W), workgroups per CU.  0 keeps the built-in choice. */
 int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
 
+/* Engine of the whole-matrix executors (no reference counterpart; for tests
+ * and measurements): FT_GPU_ENGINE_AUTO (default) runs the band-systolic
+ * engine wherever a plan's chains fit it and the pipelined engine elsewhere;
+ * FT_GPU_ENGINE_PIPELINE and FT_GPU_ENGINE_SYSTOLIC force one (the latter
+ * fails with FT_GPU_EINVAL where it does not apply).  Single-vector kernels
+ * and batches always run the pipelined engine. */
+enum { FT_GPU_ENGINE_AUTO = 0, FT_GPU_ENGINE_PIPELINE = 1, FT_GPU_ENGINE_SYSTOLIC = 2 };
+int ft_gpu_set_engine(int engine);
+
 #ifdef __cplusplus
 }
 #endif

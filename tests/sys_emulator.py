@@ -1,0 +1,100 @@
+"""Step-by-step CPU emulation of the band-systolic engine (ft_systolic.hpp)
+from the bands, tables and chains the library's host code builds
+(``ftamd_sys_debug_host``).  Test infrastructure: it checks the host-side
+construction (class geometry, band streams, step-major tables, chains,
+column masks) against the oracle without a GPU; the kernel itself is checked
+on the GPU by tests/test_gpu.py.
+"""
+import ctypes
+
+import numpy as np
+
+LANES = 16
+
+
+class Band(ctypes.Structure):
+    _fields_ = [("tab", ctypes.c_longlong), ("R", ctypes.c_int), ("steps", ctypes.c_int),
+                ("gbase", ctypes.c_int), ("nseq", ctypes.c_int)]
+
+
+class Chain(ctypes.Structure):
+    _fields_ = [("band0", ctypes.c_int), ("nb", ctypes.c_int), ("dir", ctypes.c_int), ("rstep", ctypes.c_int),
+                ("rpar", ctypes.c_int), ("prio", ctypes.c_int), ("col", ctypes.c_int * 8),
+                ("glo", ctypes.c_int * 8), ("ghi", ctypes.c_int * 8), ("aoff", ctypes.c_longlong)]
+
+
+def host_build(lib, plan_ptr, direction, S, M):
+    """(bands, table (entries x 2: s, c), chains) as the library builds them."""
+    fn = lib.ftamd_sys_debug_host
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(ctypes.POINTER(Band)), ctypes.POINTER(ctypes.c_int),
+                   ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_longlong),
+                   ctypes.POINTER(ctypes.POINTER(Chain)), ctypes.POINTER(ctypes.c_int)]
+    pb, nb = ctypes.POINTER(Band)(), ctypes.c_int()
+    pt, ne = ctypes.POINTER(ctypes.c_double)(), ctypes.c_longlong()
+    pc, nc = ctypes.POINTER(Chain)(), ctypes.c_int()
+    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, M, ctypes.byref(pb), ctypes.byref(nb), ctypes.byref(pt),
+            ctypes.byref(ne), ctypes.byref(pc), ctypes.byref(nc))
+    if rc != 0:
+        raise RuntimeError("ftamd_sys_debug_host failed: %d" % rc)
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    try:
+        bands = [Band.from_buffer_copy(pb[i]) for i in range(nb.value)]
+        tab = np.ctypeslib.as_array(pt, shape=(max(ne.value, 1) * 2,)).copy().reshape(-1, 2)
+        chains = [Chain.from_buffer_copy(pc[i]) for i in range(nc.value)]
+    finally:
+        for p in (pb, pt, pc):
+            libc.free(ctypes.cast(p, ctypes.c_void_p))
+    return bands, tab, chains
+
+
+def emulate(bands, tab, chains, S, A):
+    """Runs every chain on the n x M Fortran-ordered array A (in place)."""
+    B = LANES * S
+    for ch in chains:
+        cols = np.array(list(ch.col)).reshape(4, 2)
+        valid = cols >= 0
+        rstep, rpar, d = ch.rstep, ch.rpar, ch.dir
+        prev = None
+        for w in range(ch.nb):
+            bd = bands[ch.band0 + w]
+            R, steps = bd.R, bd.steps
+            first, last = w == 0, w == ch.nb - 1
+            Rprev = bands[ch.band0 + w - 1].R if not first else 0
+            Rnext = bands[ch.band0 + w + 1].R if not last else -1
+            npa = R + 1 if first else (R - Rprev if d == 0 else 0)
+            npa_next = 0 if last else (Rnext - R if d == 0 else 0)
+            row_of = (lambda p: R - p) if d == 0 else (lambda p: p)
+            inp = np.zeros((steps + 2, 4, 2))
+            for p in range(min(steps + 2, R + 1)):
+                if p < npa:
+                    e = rstep * row_of(p) + rpar
+                    inp[p][valid] = A[e, cols[valid]]
+                else:
+                    inp[p] = prev[p]
+            q = np.arange(B)
+            g = bd.gbase + q
+            glo = np.array(list(ch.glo)).reshape(4, 2)
+            ghi = np.array(list(ch.ghi)).reshape(4, 2)
+            act = (g[:, None, None] >= glo[None]) & (g[:, None, None] <= ghi[None])
+            Y = np.zeros((B, 4, 2))
+            E = np.zeros((B, 4, 2))
+            nxt = {}
+            for t in range(steps):
+                sc = tab[bd.tab + t * B + q]
+                sn = np.where(act, sc[:, 0, None, None], 0.0)
+                cs = np.where(act, sc[:, 1, None, None], 1.0)
+                X = np.concatenate([inp[t][None], E[:-1]], axis=0)
+                Y, E = cs * X + sn * Y, cs * Y - sn * X
+                po = t - 2 * B + 1
+                if 0 <= po <= R:
+                    P = po + npa_next
+                    if not last and P <= Rnext:
+                        nxt[P] = E[B - 1].copy()
+                    else:
+                        e = rstep * row_of(po) + rpar
+                        A[e, cols[valid]] = E[B - 1][valid]
+            prev = nxt
+    return A

@@ -123,12 +123,28 @@ def test_single_column_kernels(ft, kind, params):
 def test_tuning_shapes_agree(ft, L, W):
     """Every pipeline shape computes the same bits."""
     try:
+        ft.set_engine(ft.ENGINE_PIPELINE)
         ft.set_tuning(L, W, 0)
         for kind, params in CASES:
             for d in (0, 1):
                 _check(ft, kind, params, 150, d, seed=21)
     finally:
         ft.set_tuning(0, 0, 0)
+        ft.set_engine(ft.ENGINE_AUTO)
+
+
+@pytest.mark.parametrize("engine", ["pipeline", "systolic"])
+@pytest.mark.parametrize("kind,params", CASES)
+def test_engines(ft, engine, kind, params):
+    """Each executor engine, forced, against the oracle: sizes that give one
+    band, several bands, column masks across band edges, and partial M."""
+    try:
+        ft.set_engine(ft.ENGINE_PIPELINE if engine == "pipeline" else ft.ENGINE_SYSTOLIC)
+        for n, M in ((40, None), (129, None), (300, None), (300, 77 if kind == "tri" else 155)):
+            for d in (0, 1):
+                _check(ft, kind, params, n, d, seed=31, M=M)
+    finally:
+        ft.set_engine(ft.ENGINE_AUTO)
 
 
 def test_async_device_api_and_stream(ft):
