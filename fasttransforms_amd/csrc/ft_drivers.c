@@ -479,6 +479,7 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     a->bands = ftamd_sys_tables_dev_bands(T);
     a->chains = S->d_sys;
     a->nchains = nch;
+    a->rowmask = RP->kind != FTAMD_TRI;
     a->lda = RP->n;
     a->tab = ftamd_sys_tables_dev_tab(T);
     a->tab_bytes = (unsigned)ftamd_sys_tables_bytes(T);

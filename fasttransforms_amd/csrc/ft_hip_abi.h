@@ -110,6 +110,7 @@ typedef struct {
     const FtamdSysBand *bands;
     const FtamdSysChain *chains;
     int nchains;
+    int rowmask;      /* both columns of every lane row apply the same sequences (sph, disk) */
     int lda;          /* column stride of A in elements */
     const void *tab;  /* band tables */
     unsigned tab_bytes;  /* their size (< 2^31) */

@@ -10,8 +10,12 @@ extern "C" int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int 
     using Fn = void (*)(FtamdSysLaunch);
     Fn fn = nullptr;
     if (!ftamd_hip_sys_supported(S, precision)) return FTAMD_EINVAL;
-    if (precision == 0) fn = ftsys::sys_kernel<double, 2>;
-    else fn = S == 2 ? ftsys::sys_kernel<float, 2> : S == 4 ? ftsys::sys_kernel<float, 4> : nullptr;
+    // sph and disk chains pair the cosine and sine columns of each order
+    // (one mask per lane row); tri chains have one order per column
+    const bool rm = a->rowmask != 0;
+    if (precision == 0) fn = rm ? ftsys::sys_kernel<double, 2, true> : ftsys::sys_kernel<double, 2, false>;
+    else if (S == 2) fn = rm ? ftsys::sys_kernel<float, 2, true> : ftsys::sys_kernel<float, 2, false>;
+    else if (S == 4) fn = rm ? ftsys::sys_kernel<float, 4, true> : ftsys::sys_kernel<float, 4, false>;
     if (!fn) return FTAMD_EINVAL;
     if (a->nchains <= 0) return FTAMD_OK;
     hipLaunchKernelGGL(fn, dim3(a->nchains), dim3(ftsys::kWaves * 64), 0, (hipStream_t)stream, *a);

@@ -77,7 +77,7 @@ template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c);
 template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-template <typename T, int S>
+template <typename T, int S, bool ROWMASK>
 struct Engine {
     static constexpr int B = kLanes * S;  // sequences per band
 
@@ -88,6 +88,7 @@ struct Engine {
     static constexpr int kD = 4 * kSPD;
     static_assert(kSPD >= 1 && kU % kSPD == 0 && kD <= FTAMD_SYS_PREFETCH,
                   "DMA blocks must tile the unroll and the tables' spare steps");
+
 
     static constexpr int kMaxRows = sizeof(T) == 8 ? FTAMD_SYS_MAX_ROWS_FP64 : FTAMD_SYS_MAX_ROWS_FP32;
 
@@ -120,6 +121,10 @@ struct Engine {
         return v[threadIdx.x >> 6];
     }
     __device__ static unsigned long long &tr_out() {
+        static __shared__ unsigned long long v[kWaves];
+        return v[threadIdx.x >> 6];
+    }
+    __device__ static unsigned long long &tr_dma() {
         static __shared__ unsigned long long v[kWaves];
         return v[threadIdx.x >> 6];
     }
@@ -181,7 +186,13 @@ struct Engine {
     // step, when the two blocks after it have been requested and the third
     // not yet.
     __device__ static void wait_block() {
+#ifdef FTAMD_TRACE
+        const unsigned long long c0 = __builtin_amdgcn_s_memtime();
         asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kD / kSPD - 2) : "memory");
+        if ((threadIdx.x & 63) == 0) tr_dma() += __builtin_amdgcn_s_memtime() - c0;
+#else
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kD / kSPD - 2) : "memory");
+#endif
     }
     // Lanes 15, 31, 47, 63 (the last sequence of each row) store the row's
     // two column values at LDS byte address addr.  The loop runs with every
@@ -191,12 +202,12 @@ struct Engine {
         if constexpr (sizeof(T) == 8)
             asm volatile(
                 "s_mov_b32 exec_lo, 0x80008000\n\ts_mov_b32 exec_hi, 0x80008000\n\t"
-                "ds_write_b64 %0, %1\n\tds_write_b64 %0, %2 offset:8\n\ts_mov_b64 exec, -1\n\ts_nop 4"
+                "ds_write_b64 %0, %1\n\tds_write_b64 %0, %2 offset:8\n\ts_mov_b64 exec, -1"
                 ::"v"(addr), "v"(v0), "v"(v1) : "memory");
         else
             asm volatile(
                 "s_mov_b32 exec_lo, 0x80008000\n\ts_mov_b32 exec_hi, 0x80008000\n\t"
-                "ds_write_b32 %0, %1\n\tds_write_b32 %0, %2 offset:4\n\ts_mov_b64 exec, -1\n\ts_nop 4"
+                "ds_write_b32 %0, %1\n\tds_write_b32 %0, %2 offset:4\n\ts_mov_b64 exec, -1"
                 ::"v"(addr), "v"(v0), "v"(v1) : "memory");
     }
     __device__ static unsigned lds_addr(const void *p) { return (unsigned)(uintptr_t)p; }
@@ -210,20 +221,25 @@ struct Engine {
     // One step: rotate every (sequence, column) of the lane.  X0: the fresh
     // rows entering sequence 0 (already shifted in); Y: carried rows; E: rows
     // emitted at the previous step (sequence s reads E[s-1] as its fresh row).
-    template <bool MASKED>
+    // MASK 0: every column applies every sequence of the band; 1: the two
+    // columns of a lane row apply the same sequences (act[s][0]); 2: per
+    // column.  A sequence a column does not apply is the identity (0, 1).
+    template <int MASK>
     __device__ static void rotate(T (&Y)[S][kCols], T (&E)[S][kCols], const T (&X0)[kCols], const SC &sc,
                                   const bool (&act)[S][kCols]) {
 #pragma unroll
         for (int s = S - 1; s >= 0; --s) {
+            const Pair<T> e = sc.pair(s);
+            const T sn_row = MASK == 1 ? (act[s][0] ? e.x : T(0)) : e.x;
+            const T cs_row = MASK == 1 ? (act[s][0] ? e.y : T(1)) : e.y;
 #pragma unroll
             for (int c = 0; c < kCols; ++c) {
                 const T x = s == 0 ? X0[c] : E[s - 1][c];
                 const T y = Y[s][c];
-                const Pair<T> e = sc.pair(s);
-                T sn = e.x, cs = e.y;
-                if (MASKED) {
-                    sn = act[s][c] ? sn : T(0);
-                    cs = act[s][c] ? cs : T(1);
+                T sn = sn_row, cs = cs_row;
+                if (MASK == 2) {
+                    sn = act[s][c] ? e.x : T(0);
+                    cs = act[s][c] ? e.y : T(1);
                 }
                 Y[s][c] = fmaT<T>(cs, x, sn * y);
                 E[s][c] = fmaT<T>(cs, y, -(sn * x));
@@ -231,7 +247,7 @@ struct Engine {
         }
     }
 
-    template <bool MASKED>
+    template <int MASK>
     __device__ static void run_band(const FtamdSysLaunch &a, Shared &sh, const FtamdSysChain &ch, int w, int lane) {
         const int r = lane >> 4, i = lane & 15;
         const int nb = uniform(ch.nb);
@@ -261,13 +277,15 @@ struct Engine {
         // class row of stream position p
         auto row_of = [&](int p) { return dir == 0 ? R - p : p; };
 
+        // with MASK 1 column 0 of a row carries the row's mask (column 1's
+        // when column 0 is absent)
         bool act[S][kCols];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < kCols; ++c) {
                 const int g = gbase + S * i + s;
-                const int k = 2 * r + c;
+                const int k = MASK == 1 && c == 0 && colv[0] < 0 ? 2 * r + 1 : 2 * r + c;
                 act[s][c] = g >= ch.glo[k] && g <= ch.ghi[k];
             }
 
@@ -276,7 +294,7 @@ struct Engine {
         int avail_seen = 0, freed_seen = 0;
         bool ok = true;
 #ifdef FTAMD_TRACE
-        if (lane == 0) tr_in() = tr_out() = 0;
+        if (lane == 0) tr_in() = tr_out() = tr_dma() = 0;
         const unsigned long long tr_start = wall_clock64(), tr_c0 = __builtin_amdgcn_s_memtime();
 #endif
 
@@ -414,6 +432,10 @@ struct Engine {
                 const char *srcb = tab_band + (long long)(t0 + kD) * kStepBytes;  // step t0+kD+K: the DMA of step u, K = u+1-kSPD
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
+                    // keep each step's instructions together: the entries and
+                    // rows read for the next steps then have a whole step of
+                    // issue time to arrive before their first use
+                    __builtin_amdgcn_sched_barrier(0);
                     T X0[kCols];
 #pragma unroll
                     for (int c = 0; c < kCols; ++c) X0[c] = shift_in(inbuf[u % kDin][c], E[S - 1][c]);
@@ -429,7 +451,9 @@ struct Engine {
 #pragma unroll
                         for (int s2 = 0; s2 < S; ++s2) scn.p[s2] = q[s2];
                     }
-                    rotate<MASKED>(Y, E, X0, sc, act);
+                    // the reads above are issued before this step's arithmetic
+                    __builtin_amdgcn_sched_barrier(0);
+                    rotate<MASK>(Y, E, X0, sc, act);
                     if (u % kSPD == kSPD - 1) {
                         // u is a constant once the loop is unrolled
                         switch (u + 1 - kSPD) {
@@ -448,6 +472,7 @@ struct Engine {
                 for (int u2 = 0; u2 < kU; u2 += 2) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
+                        __builtin_amdgcn_sched_barrier(0);
                         const int t = t0 + u2 + h;
                         T X0[kCols];
 #pragma unroll
@@ -456,7 +481,8 @@ struct Engine {
                         const SC sc = scn;
                         if ((t + 1) % kSPD == 0) wait_block();
                         read_sc(sh, w, t + 1, i, scn);
-                        rotate<MASKED>(Y, E, X0, sc, act);
+                        __builtin_amdgcn_sched_barrier(0);
+                        rotate<MASK>(Y, E, X0, sc, act);
                         if ((t + 1) % kSPD == 0) dma_block(sh, w, tab_band, t + 1 - kSPD + kD, lane16, E[0][0]);
                         const int po = t - 2 * B + 1;
                         unsigned so = lds_addr(&sh.dummy[0][0]);
@@ -489,7 +515,7 @@ struct Engine {
             rec[4] = tr_out();
             rec[5] = steps;
             rec[6] = R;
-            rec[7] = ((unsigned long long)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) << 32) | (unsigned)b;
+            rec[7] = tr_dma();
         }
 #endif
         // ---- rows that left the chain here, staged by class row, to A:
@@ -528,22 +554,24 @@ struct Engine {
             case 1: __builtin_amdgcn_s_setprio(1); break;
             default: break;
         }
-        // a band needs the per-column masks if some column of the chain does
-        // not apply some of its sequences
+        // a band needs masks if some column of the chain does not apply some
+        // of its sequences; with ROWMASK one mask per lane row serves both
+        // columns (sph and disk: the two columns of a row are the cosine and
+        // sine columns of one order, so they apply the same sequences)
         const int gbase = uniform(a.bands[uniform(ch.band0) + w].gbase);
         const int glast = gbase + uniform(a.bands[uniform(ch.band0) + w].nseq) - 1;
         bool masked = false;
         for (int k = 0; k < FTAMD_SYS_COLS; ++k)
             masked |= ch.col[k] >= 0 && (ch.glo[k] > gbase || ch.ghi[k] < glast);
-        if (masked) run_band<true>(a, sh, ch, w, lane);
-        else run_band<false>(a, sh, ch, w, lane);
+        if (!masked) run_band<0>(a, sh, ch, w, lane);
+        else run_band<ROWMASK ? 1 : 2>(a, sh, ch, w, lane);
         __builtin_amdgcn_s_setprio(0);
     }
 };
 
-template <typename T, int S>
+template <typename T, int S, bool ROWMASK>
 __global__ void __launch_bounds__(kWaves * 64) sys_kernel(FtamdSysLaunch a) {
-    using E = Engine<T, S>;
+    using E = Engine<T, S, ROWMASK>;
     __shared__ typename E::Shared sh;
     E::run(a, sh);
 }
