@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-batch", action="store_true", help="skip the secondary measurements")
+    p.add_argument("--engine", choices=("auto", "pipeline", "systolic"), default="auto",
+                   help="engine of the headline launches (auto: the library's own choice)")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return p.parse_args()
@@ -157,6 +159,18 @@ def cpu_baseline(n, seconds):
             "isa": O.REF_ISA_NAMES[O.ref_isa()]}
 
 
+def profiled_traffic(engine, n):
+    """HBM bytes per hi2lo launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
+    known size) for this engine and n, from the summary committed under profiles/, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "rot_%s_sph_n%d.md" % (engine, n))
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+    return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -168,7 +182,11 @@ def main():
 
     n = args.n
     M = 2 * n - 1
+    engines = {"auto": ft.ENGINE_AUTO, "pipeline": ft.ENGINE_PIPELINE, "systolic": ft.ENGINE_SYSTOLIC}
+    names = {ft.ENGINE_PIPELINE: "pipeline", ft.ENGINE_SYSTOLIC: "systolic"}
+    ft.set_engine(engines[args.engine])
     ms, flops = single(ft, torch, "sph", n, "hi2lo", ft.FP64, args.steps, args.warmup)
+    engine_used = names.get(ft.last_engine(), "none")
     ms_lo, _ = single(ft, torch, "sph", n, "lo2hi", ft.FP64, max(5, args.steps // 2), 2)
     gflops = flops / (ms * 1e-3) / 1e9
     achieved_tflops = flops / (ms * 1e-3) / 1e12
@@ -176,16 +194,27 @@ def main():
 
     extra = {}
     if not args.no_batch:
+        # both engines on the headline shape and at n/2 (where auto picks the systolic one)
+        for eng in ("pipeline", "systolic"):
+            ft.set_engine(engines[eng])
+            for nn in (n, n // 2):
+                try:
+                    m_e, _ = single(ft, torch, "sph", nn, "hi2lo", ft.FP64, 10, 2)
+                    extra["sph_n%d_hi2lo_ms_%s" % (nn, eng)] = round(m_e, 4)
+                except Exception as e:  # an engine that does not apply to the shape
+                    extra["sph_n%d_hi2lo_ms_%s" % (nn, eng)] = None
+                    print("bench.py: %s engine at n=%d: %s" % (eng, nn, e), file=sys.stderr)
+        ft.set_engine(engines[args.engine])
         ms2, fl2 = single(ft, torch, "sph", 2 * n, "hi2lo", ft.FP64, 5, 2)
         ms32, _ = single(ft, torch, "sph", n, "hi2lo", ft.FP32, 10, 2)
         mst, flt = single(ft, torch, "tri", n, "hi2lo", ft.FP64, 10, 2, (0.0, 0.0, 0.0))
         msd, fld = single(ft, torch, "disk", n, "hi2lo", ft.FP64, 10, 2, (0.0, 0.0))
-        extra = {"sph_n%d_hi2lo_ms" % (2 * n): round(ms2, 4),
+        extra.update({"sph_n%d_hi2lo_ms" % (2 * n): round(ms2, 4),
                  "sph_n%d_hi2lo_gflops" % (2 * n): round(fl2 / (ms2 * 1e-3) / 1e9, 2),
                  "sph_fp32_hi2lo_ms": round(ms32, 4),
                  "tri_hi2lo_ms": round(mst, 4), "tri_hi2lo_gflops": round(flt / (mst * 1e-3) / 1e9, 2),
                  "disk_hi2lo_ms": round(msd, 4), "disk_hi2lo_gflops": round(fld / (msd * 1e-3) / 1e9, 2),
-                 "batch_sph_hi2lo_512": batch(ft, torch)}
+                 "batch_sph_hi2lo_512": batch(ft, torch)})
 
     if args.calib_copy:
         src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -202,9 +231,10 @@ def main():
                              "from its sources) at n=%d, median %.1f ms" % (cpu["runs"], cpu["isa"], n,
                                                                            cpu["median_s"] * 1e3)}
     config = {"workload": "ft_execute_sph_hi2lo, n=%d (%dx%d column-major FP64), device-resident" % (n, n, M),
-              "n": n, "M": M, "precision": "fp64", "lo2hi_ms": round(ms_lo, 5),
+              "n": n, "M": M, "precision": "fp64", "engine": engine_used, "lo2hi_ms": round(ms_lo, 5),
               "lo2hi_gflops": round(flops / (ms_lo * 1e-3) / 1e9, 2)}
     config.update(extra)
+    traffic, traffic_src = profiled_traffic(engine_used, n)
     line = {
         "metric": "sph_hi2lo FP64 GFLOP/s & % FP64 peak at n=1024; batched vs looped transforms/s",
         "value": round(gflops, 2),
@@ -221,7 +251,8 @@ def main():
         "config": config,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "note": "compute-bound (51 flop/B against a 9.8 flop/B ridge) on the FP64 vector ALU; "
                              "MI355X's FP64 peak is 78.6 TFLOP/s for vector and matrix alike",
                      "compulsory_bytes": compulsory_bytes,

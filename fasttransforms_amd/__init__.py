@@ -57,6 +57,7 @@ SIGNATURES = {
     "ft_gpu_destroy_batch": (None, [_vp]),
     "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
     "ft_gpu_set_engine": (_i, [_i]),
+    "ft_gpu_last_engine": (_i, []),
 }
 for _k in ("sph", "tri", "disk"):
     for _dn in ("hi2lo", "lo2hi"):
@@ -160,6 +161,11 @@ def set_engine(engine=ENGINE_AUTO):
     rc = _lib.ft_gpu_set_engine(int(engine))
     if rc:
         raise ValueError("unknown engine %r" % (engine,))
+
+
+def last_engine():
+    """Engine of this thread's last whole-matrix launch (ENGINE_PIPELINE or ENGINE_SYSTOLIC; 0 before any)."""
+    return int(_lib.ft_gpu_last_engine())
 
 
 def _is_torch(A):

@@ -68,6 +68,9 @@ int ft_gpu_set_engine(int engine) {
 
 int ftamd_engine(void) { return g_engine; }
 
+static __thread int t_last_engine = 0;
+int ft_gpu_last_engine(void) { return t_last_engine; }
+
 int ftamd_tuning_L(int kind, int n) {
     (void)kind;
     (void)n;
@@ -542,19 +545,12 @@ static int gpu_execute(ft_rotation_plan *RP, int dir, int precision, void *dA, i
     if (rc == FTAMD_OK && g_engine != FT_GPU_ENGINE_PIPELINE) {
         S = sys_schedule(RP, dir, precision, M, &rc);
         if (rc == FTAMD_OK && !S && g_engine == FT_GPU_ENGINE_SYSTOLIC) rc = FTAMD_EINVAL;
-        /* automatic choice: the band-systolic engine while its workgroups
-         * (one per CU) all run at once; beyond that the pipelined engine's
-         * smaller tasks balance the device better */
-        if (rc == FTAMD_OK && S && g_engine == FT_GPU_ENGINE_AUTO) {
-            int cu = 0;
-            if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
-            if (S->sys_launch.nchains > cu) S = NULL;
-        }
     }
     if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, dir, precision, M, dA, &rc);
     pthread_mutex_unlock(&RP->lock);
     if (rc != FTAMD_OK) return rc;
     if (used) *used = S;
+    t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
     return launch(S, precision, RP->kind, dir, dA, stream);
 }
 

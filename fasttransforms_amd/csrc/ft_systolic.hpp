@@ -98,7 +98,6 @@ struct Engine {
         Pair<T> scr[kWaves][kD][kLanes][S];
         T ring[kWaves][kZ][kRows][kCols];  // ring w: input stream of band (wave) w
         T stage[kMaxRows][kRows][kCols];   // rows leaving the chain, by class row, until stored to A
-        T dummy[kRows][kCols];             // where steps without an output store theirs
         int avail[kWaves];                 // positions of ring w written by wave w-1 (exclusive end)
         int freed[kWaves];                 // positions of ring w wave w has read (exclusive end)
         int abort_flag;
@@ -194,22 +193,9 @@ struct Engine {
         asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kD / kSPD - 2) : "memory");
 #endif
     }
-    // Lanes 15, 31, 47, 63 (the last sequence of each row) store the row's
-    // two column values at LDS byte address addr.  The loop runs with every
-    // lane active, so exec is narrowed and restored around the two stores
-    // without any branch.
-    __device__ static void store_row_ends(unsigned addr, T v0, T v1) {
-        if constexpr (sizeof(T) == 8)
-            asm volatile(
-                "s_mov_b32 exec_lo, 0x80008000\n\ts_mov_b32 exec_hi, 0x80008000\n\t"
-                "ds_write_b64 %0, %1\n\tds_write_b64 %0, %2 offset:8\n\ts_mov_b64 exec, -1"
-                ::"v"(addr), "v"(v0), "v"(v1) : "memory");
-        else
-            asm volatile(
-                "s_mov_b32 exec_lo, 0x80008000\n\ts_mov_b32 exec_hi, 0x80008000\n\t"
-                "ds_write_b32 %0, %1\n\tds_write_b32 %0, %2 offset:4\n\ts_mov_b64 exec, -1"
-                ::"v"(addr), "v"(v0), "v"(v1) : "memory");
-    }
+    // Lane 15 of each row of 16 (the row's last sequence) to every lane of
+    // the row (DPP row_newbcast:15).
+    __device__ static T row_end(T v) { return __builtin_amdgcn_update_dpp(v, v, 0x15f, 0xf, 0xf, false); }
     __device__ static unsigned lds_addr(const void *p) { return (unsigned)(uintptr_t)p; }
 
     __device__ static void read_sc(Shared &sh, int w, int step, int i, SC &sc) {
@@ -358,9 +344,9 @@ struct Engine {
         wait_block();
         read_sc(sh, w, 0, i, scn);
 
-        // Steps run in blocks of kU.  The waits and the choice of where the
-        // block's outputs go are made once per block (uniform), so a step is
-        // only its DPP shift, LDS reads, rotations and at most one store.
+        // Steps run in blocks of kU.  The waits are made once per block, so
+        // a step is only its DPP shifts, LDS reads, rotations and the
+        // broadcast of its output.
         int written = 0;  // ring positions of the next band written (exclusive end)
         for (int t0 = 0; t0 < steps; t0 += kU) {
             if (t0 % kChunk == 0) {
@@ -397,100 +383,71 @@ struct Engine {
                     written = P1 + 1;
                 }
             }
-            // Where step u of the block stores its output: LDS byte address
-            // obase + ((ooff + u) & omask) * ostep (+ the row's place), for a
-            // block whose outputs all go one way; a block that straddles a
-            // boundary runs the general steps below.
-            const unsigned row_bytes = kRows * kCols * sizeof(T);
-            const int po1 = po0 + kU - 1;
-            bool pure = true;
-            unsigned obase = lds_addr(&sh.dummy[0][0]);
-            int ooff = 0, omask = 0, ostep = 0;
-            if (po1 < 0 || po0 > R) {
-            } else if (po0 >= 0 && po1 <= R && po1 <= ring_hi) {
-                obase = lds_addr(&out_ring[0][0][0]);
-                ooff = po0 + npa_next;
-                omask = kZ - 1;
-                ostep = row_bytes;
-            } else if (po0 >= 0 && po1 <= R && po0 > ring_hi) {
-                obase = lds_addr(&sh.stage[row_of(po0)][0][0]);
-                omask = -1;
-                ostep = dir == 0 ? -(int)row_bytes : (int)row_bytes;
-            } else {
-                pure = false;
-            }
-            if (pure) {
-                // the addresses of the block's LDS accesses: ring slots of the
-                // reads (the last kDin steps may wrap to the ring's start),
-                // table slots, DMA targets and sources
-                const T *pin = &in_ring[t0 % kZ][r][0];
-                const T *pin_hi = (t0 % kZ == kZ - kU) ? pin - kZ * kRows * kCols : pin;
-                const int sb = t0 % kD;
-                const Pair<T> *psc = &sh.scr[w][sb][i][0];
-                const Pair<T> *psc_hi = (sb == kD - kU) ? psc - kD * kLanes * S : psc;
-                const unsigned m0b = lds_addr(&sh.scr[w][0][0][0]) + (unsigned)sb * kStepBytes;
-                const char *srcb = tab_band + (long long)(t0 + kD) * kStepBytes;  // step t0+kD+K: the DMA of step u, K = u+1-kSPD
+            // the addresses of the block's LDS accesses: ring slots of the
+            // reads (the last kDin steps may wrap to the ring's start), table
+            // slots, DMA targets and sources
+            const T *pin = &in_ring[t0 % kZ][r][0];
+            const T *pin_hi = (t0 % kZ == kZ - kU) ? pin - kZ * kRows * kCols : pin;
+            const int sb = t0 % kD;
+            const Pair<T> *psc = &sh.scr[w][sb][i][0];
+            const Pair<T> *psc_hi = (sb == kD - kU) ? psc - kD * kLanes * S : psc;
+            const unsigned m0b = lds_addr(&sh.scr[w][0][0][0]) + (unsigned)sb * kStepBytes;
+            const char *srcb = tab_band + (long long)(t0 + kD) * kStepBytes;  // step t0+kD+K: the DMA of step u, K = u+1-kSPD
+            // step u's output (the row-end values of the last sequence, lane
+            // 15 of each row) is broadcast along its row and kept by lane u;
+            // the block's kU outputs are stored together after its last step
+            T outv[kCols];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    // keep each step's instructions together: the entries and
-                    // rows read for the next steps then have a whole step of
-                    // issue time to arrive before their first use
-                    __builtin_amdgcn_sched_barrier(0);
-                    T X0[kCols];
+            for (int c = 0; c < kCols; ++c) outv[c] = T(0);
 #pragma unroll
-                    for (int c = 0; c < kCols; ++c) X0[c] = shift_in(inbuf[u % kDin][c], E[S - 1][c]);
-                    {
-                        const T *q = (u + kDin >= kU ? pin_hi : pin) + (u + kDin) * kRows * kCols;
+            for (int u = 0; u < kU; ++u) {
+                // keep each step's instructions together: the entries and
+                // rows read for the next steps then have a whole step of
+                // issue time to arrive before their first use
+                __builtin_amdgcn_sched_barrier(0);
+                T X0[kCols];
 #pragma unroll
-                        for (int c = 0; c < kCols; ++c) inbuf[u % kDin][c] = lds_ld(q + c);
-                    }
-                    const SC sc = scn;
-                    if ((u + 1) % kSPD == 0) wait_block();
-                    {
-                        const Pair<T> *q = (u + 1 >= kU ? psc_hi : psc) + (u + 1) * kLanes * S;
+                for (int c = 0; c < kCols; ++c) X0[c] = shift_in(inbuf[u % kDin][c], E[S - 1][c]);
+                {
+                    const T *q = (u + kDin >= kU ? pin_hi : pin) + (u + kDin) * kRows * kCols;
 #pragma unroll
-                        for (int s2 = 0; s2 < S; ++s2) scn.p[s2] = q[s2];
-                    }
-                    // the reads above are issued before this step's arithmetic
-                    __builtin_amdgcn_sched_barrier(0);
-                    rotate<MASK>(Y, E, X0, sc, act);
-                    if (u % kSPD == kSPD - 1) {
-                        // u is a constant once the loop is unrolled
-                        switch (u + 1 - kSPD) {
-                            case 0: dma_block_at<0>(m0b, srcb, lane16, E[0][0]); break;
-                            case 2: dma_block_at<2>(m0b, srcb, lane16, E[0][0]); break;
-                            case 4: dma_block_at<4>(m0b, srcb, lane16, E[0][0]); break;
-                            case 6: dma_block_at<6>(m0b, srcb, lane16, E[0][0]); break;
-                            default: break;
-                        }
-                    }
-                    const unsigned so = obase + (unsigned)(((ooff + u) & omask) * ostep);
-                    store_row_ends(so + r16, E[S - 1][0], E[S - 1][1]);
+                    for (int c = 0; c < kCols; ++c) inbuf[u % kDin][c] = lds_ld(q + c);
                 }
-            } else {
-#pragma unroll 1
-                for (int u2 = 0; u2 < kU; u2 += 2) {
+                const SC sc = scn;
+                if ((u + 1) % kSPD == 0) wait_block();
+                {
+                    const Pair<T> *q = (u + 1 >= kU ? psc_hi : psc) + (u + 1) * kLanes * S;
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        __builtin_amdgcn_sched_barrier(0);
-                        const int t = t0 + u2 + h;
-                        T X0[kCols];
-#pragma unroll
-                        for (int c = 0; c < kCols; ++c) X0[c] = shift_in(inbuf[h][c], E[S - 1][c]);
-                        read_in(t + kDin, inbuf[h]);
-                        const SC sc = scn;
-                        if ((t + 1) % kSPD == 0) wait_block();
-                        read_sc(sh, w, t + 1, i, scn);
-                        __builtin_amdgcn_sched_barrier(0);
-                        rotate<MASK>(Y, E, X0, sc, act);
-                        if ((t + 1) % kSPD == 0) dma_block(sh, w, tab_band, t + 1 - kSPD + kD, lane16, E[0][0]);
-                        const int po = t - 2 * B + 1;
-                        unsigned so = lds_addr(&sh.dummy[0][0]);
-                        if (po >= 0 && po <= R)
-                            so = po <= ring_hi ? lds_addr(&out_ring[(po + npa_next) % kZ][0][0])
-                                               : lds_addr(&sh.stage[row_of(po)][0][0]);
-                        store_row_ends(so + r16, E[S - 1][0], E[S - 1][1]);
+                    for (int s2 = 0; s2 < S; ++s2) scn.p[s2] = q[s2];
+                }
+                // the reads above are issued before this step's arithmetic
+                __builtin_amdgcn_sched_barrier(0);
+                rotate<MASK>(Y, E, X0, sc, act);
+                if (u % kSPD == kSPD - 1) {
+                    // u is a constant once the loop is unrolled
+                    switch (u + 1 - kSPD) {
+                        case 0: dma_block_at<0>(m0b, srcb, lane16, E[0][0]); break;
+                        case 2: dma_block_at<2>(m0b, srcb, lane16, E[0][0]); break;
+                        case 4: dma_block_at<4>(m0b, srcb, lane16, E[0][0]); break;
+                        case 6: dma_block_at<6>(m0b, srcb, lane16, E[0][0]); break;
+                        default: break;
                     }
+                }
+#pragma unroll
+                for (int c = 0; c < kCols; ++c) {
+                    const T b = row_end(E[S - 1][c]);
+                    outv[c] = i == u ? b : outv[c];
+                }
+            }
+            // lane i < kU of each row stores output position po0 + i: to the
+            // next band's ring up to ring_hi, else (leaving the chain) to the
+            // stage; positions outside [0, R] carry no row
+            {
+                const int po = po0 + i;
+                if (i < kU && po >= 0 && po <= R) {
+                    T *dst = po <= ring_hi ? &out_ring[(po + npa_next) % kZ][r][0] : &sh.stage[row_of(po)][r][0];
+#pragma unroll
+                    for (int c = 0; c < kCols; ++c) lds_st(dst + c, outv[c]);
                 }
             }
             if (!ok) break;
