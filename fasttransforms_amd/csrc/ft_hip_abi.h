@@ -89,7 +89,7 @@ typedef struct {
 typedef struct {
     long long tab;    /* entry offset of the band's table: [steps + PREFETCH][LANES][S] (s, c) pairs */
     int R;            /* the band's row stream is rows 0 .. R of the class */
-    int steps;        /* steps the band runs: R + 2*LANES*S rounded up to UNROLL */
+    int steps;        /* steps the band runs: R + (S+1)*(LANES-1) + S + 1 rounded up to UNROLL */
     int gbase;        /* position of the band's first sequence in the class, in application order */
     int nseq;         /* sequences of the class in the band (the rest are identities) */
 } FtamdSysBand;

@@ -66,6 +66,12 @@ static int make_classes(int kind, int n, sys_class *cl) {
 
 static int round_up(int x, int u) { return (x + u - 1) / u * u; }
 
+/* Step at which stream position 0 reaches sequence q = S*i + s of a band as
+ * its fresh row: within a lane a sequence takes the row the sequence before
+ * it emits in the same step, from lane to lane a row moves one step later
+ * (DPP of the previous step's registers), so lane i adds S+1 steps. */
+static int sys_delay(int S, int q) { return (S + 1) * (q / S) + q % S; }
+
 /* Tables of every band of every class for one direction and precision. */
 struct ft_sys_tables {
     int S, dir, precision, ncls;
@@ -135,7 +141,8 @@ static int tables_host(const ft_rotation_plan *RP, int dir, int precision, int S
             int klong = dir == 0 ? g0 + nseq - 1 : cl->K - 1 - g0;
             bd->R = class_top(kind, n, cl, cl->jtop - cl->jstep * klong) + 1;
             if (bd->R < 0) bd->R = 0;
-            bd->steps = round_up(bd->R + 2 * B, U);
+            /* the last sequence emits position R at step R + delay + 1 */
+            bd->steps = round_up(bd->R + sys_delay(S, B - 1) + 2, U);
             bd->gbase = g0;
             bd->nseq = nseq;
             bd->tab = entries;
@@ -159,7 +166,8 @@ static int tables_host(const ft_rotation_plan *RP, int dir, int precision, int S
                         int k = dir == 0 ? g : cl->K - 1 - g;
                         int j = cl->jtop - cl->jstep * k;
                         /* class row whose rotation sequence q does at step t */
-                        int e = dir == 0 ? bd->R + 2 * q - t : t - 2 * q - 1;
+                        int d = sys_delay(S, q);
+                        int e = dir == 0 ? bd->R + d - t : t - d - 1;
                         if (e >= 0 && e <= class_top(kind, n, cl, j)) {
                             long ix = class_index(kind, n, cl, e, j);
                             sv = RP->s[ix];

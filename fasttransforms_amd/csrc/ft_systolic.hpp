@@ -16,20 +16,21 @@
 //    of 16 lanes, 2 columns per lane).  Each lane holds, per sequence and
 //    column, the row its sequence carries; every step every sequence rotates
 //    (carried, fresh) and emits the other row to the next sequence: inside a
-//    lane through a register, from lane to lane by a DPP row shift.  Rows
-//    enter the band as a stream at lane 0 of each row (rows R .. 0 for hi2lo,
-//    0 .. R for lo2hi) and leave it, one step per sequence later, at lane 15.
-//    Steps where a sequence has no rotation (before its rows arrive, after
-//    its last row, or for a column that does not apply it) use the identity
-//    (s, c) = (0, 1), which passes the stream through with the same one-step
-//    delay; so the whole band is one uniform loop, and the table of a band is
-//    laid out step-major ([step][lane][slot]) with identities in the gaps.
+//    lane in the same step, from lane to lane by a DPP row shift one step
+//    later.  Rows enter the band as a stream at lane 0 of each row (rows
+//    R .. 0 for hi2lo, 0 .. R for lo2hi) and leave it at lane 15, S+1 steps
+//    per lane later.  Steps where a sequence has no rotation (before its rows
+//    arrive, after its last row, or for a column that does not apply it) use
+//    the identity (s, c) = (0, 1), which passes the stream through with the
+//    same delay; so the whole band is one uniform loop, and the table of a
+//    band is laid out step-major ([step][lane][slot]) with identities in the
+//    gaps.
 //  * s and c are per lane (each lane is a different sequence): no broadcast;
 //    the 4 rows of a wave read the same 16 lanes' entries.
 //  * A chain (one workgroup) runs its bands as waves; band b+1 reads band
 //    b's output stream through an LDS ring as it is produced, so a chain's
 //    bands overlap and the chain's latency is about one band's stream plus
-//    2*LANES*S steps per band.  The first band reads A, the rows that leave
+//    (S+1)*LANES steps per band.  The first band reads A, the rows that leave
 //    the chain for good are stored to A.
 //
 // Arithmetic: the reference's rotation in fused form, x' = fma(c, x, s*y),
@@ -80,6 +81,10 @@ template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float
 template <typename T, int S, bool ROWMASK>
 struct Engine {
     static constexpr int B = kLanes * S;  // sequences per band
+    // a row that enters lane 0 at step p leaves lane 15 at step p + kOutDelay:
+    // S+1 steps per lane (the lane's S sequences pass it on within a step,
+    // the DPP shift to the next lane takes a step)
+    static constexpr int kOutDelay = (S + 1) * (kLanes - 1) + S;
 
     // table bytes of one step (all 16 lanes) and steps per 1 KB LDS-DMA load
     static constexpr int kStepBytes = kLanes * S * 2 * (int)sizeof(T);
@@ -205,8 +210,10 @@ struct Engine {
     }
 
     // One step: rotate every (sequence, column) of the lane.  X0: the fresh
-    // rows entering sequence 0 (already shifted in); Y: carried rows; E: rows
-    // emitted at the previous step (sequence s reads E[s-1] as its fresh row).
+    // rows of the lane's first sequence (the last sequence of the lane before
+    // emitted them at the previous step, or the stream at lane 0); Y: carried
+    // rows; sequence s > 0 takes as its fresh row what sequence s-1 emits in
+    // this step.
     // MASK 0: every column applies every sequence of the band; 1: the two
     // columns of a lane row apply the same sequences (act[s][0]); 2: per
     // column.  A sequence a column does not apply is the identity (0, 1).
@@ -214,7 +221,7 @@ struct Engine {
     __device__ static void rotate(T (&Y)[S][kCols], T (&E)[S][kCols], const T (&X0)[kCols], const SC &sc,
                                   const bool (&act)[S][kCols]) {
 #pragma unroll
-        for (int s = S - 1; s >= 0; --s) {
+        for (int s = 0; s < S; ++s) {
             const Pair<T> e = sc.pair(s);
             const T sn_row = MASK == 1 ? (act[s][0] ? e.x : T(0)) : e.x;
             const T cs_row = MASK == 1 ? (act[s][0] ? e.y : T(1)) : e.y;
@@ -370,7 +377,7 @@ struct Engine {
             // the block's outputs, positions po0 .. po0+kU-1: those in
             // [0, R] go to the next band's ring up to position ring_hi,
             // beyond it (or from the last band) they leave the chain
-            const int po0 = t0 - 2 * B + 1;
+            const int po0 = t0 - kOutDelay;
             const int ring_hi = last ? -1 : min(R, Rnext - npa_next);
             {
                 const int hi = min(po0 + kU - 1, ring_hi);

@@ -86,9 +86,18 @@ def emulate(bands, tab, chains, S, A):
                 sc = tab[bd.tab + t * B + q]
                 sn = np.where(act, sc[:, 0, None, None], 0.0)
                 cs = np.where(act, sc[:, 1, None, None], 1.0)
-                X = np.concatenate([inp[t][None], E[:-1]], axis=0)
-                Y, E = cs * X + sn * Y, cs * Y - sn * X
-                po = t - 2 * B + 1
+                # lane i's first sequence takes what the lane before emitted at
+                # the previous step (lane 0: the stream); sequence s > 0 of a
+                # lane takes what sequence s-1 emits in this step
+                X0 = np.concatenate([inp[t][None], E[S - 1::S][:-1]], axis=0)
+                Yn, En = np.empty_like(Y), np.empty_like(E)
+                for s in range(S):
+                    k = np.arange(LANES) * S + s
+                    x = X0 if s == 0 else En[k - 1]
+                    Yn[k] = cs[k] * x + sn[k] * Y[k]
+                    En[k] = cs[k] * Y[k] - sn[k] * x
+                Y, E = Yn, En
+                po = t - ((S + 1) * (LANES - 1) + S)
                 if 0 <= po <= R:
                     P = po + npa_next
                     if not last and P <= Rnext:

@@ -36,4 +36,5 @@ def test_host_tables_and_chains(ft, kind, params, n, S, M, d):
     assert O.rel2(got, ref) <= 1e-14
     # every chain fits a workgroup and every band's stream fits the staging rows
     assert all(1 <= c.nb <= 16 for c in chains) or not chains
-    assert all(b.R < 1024 and b.steps % 8 == 0 and b.steps >= b.R + 32 * S for b in bands)
+    # the last sequence emits position R at step R + (S+1)*15 + S
+    assert all(b.R < 1024 and b.steps % 8 == 0 and b.steps >= b.R + (S + 1) * 15 + S + 1 for b in bands)
