@@ -100,6 +100,7 @@ typedef struct {
     int dir;          /* 0 hi2lo: stream rows R .. 0; 1 lo2hi: rows 0 .. R */
     int rstep, rpar;  /* matrix row of class row e: rstep*e + rpar */
     int prio;         /* wave issue priority 0..3 (longest chains highest) */
+    int lda;          /* column stride of its matrix in elements */
     int col[FTAMD_SYS_COLS];  /* [row][c]: column of A, -1 if none */
     int glo[FTAMD_SYS_COLS];  /* the column applies the sequences at positions glo .. ghi */
     int ghi[FTAMD_SYS_COLS];
@@ -111,10 +112,8 @@ typedef struct {
     const FtamdSysChain *chains;
     int nchains;
     int rowmask;      /* both columns of every lane row apply the same sequences (sph, disk) */
-    int lda;          /* column stride of A in elements */
-    const void *tab;  /* band tables */
-    unsigned tab_bytes;  /* their size (< 2^31) */
-    void *A;
+    const void *tab;  /* base of the band tables (bands address them by entry offsets from it) */
+    void *A;          /* base of the matrices (chains address them by element offsets from it) */
     int *err;         /* FTAMD_ERR_* bits */
     unsigned long long *trace;  /* diagnostic per-wave timeline (instrumented builds only), else NULL */
 } FtamdSysLaunch;

@@ -24,6 +24,7 @@ typedef struct ft_schedule {
     int xcd_tasks;           /* most tasks any XCD's list holds */
     /* band-systolic engine (sys != 0): chains on the device, launch record */
     int sys;
+    int batch;         /* a batch schedule (grid capped at 2 workgroups per CU by default) */
     FtamdSysLaunch sys_launch;
     void *d_sys;
     double flops;            /* algorithmic flops (6 per rotation) */
@@ -71,5 +72,8 @@ const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
 size_t ftamd_sys_tables_bytes(const ft_sys_tables *T);
 int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count);
+int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
+                    const int *M, int count, FtamdSysBand **bands, int *nbands, FtamdSysChain **chains,
+                    int *nchains, const void **tab_base);
 
 #endif /* FT_INTERNAL_H */

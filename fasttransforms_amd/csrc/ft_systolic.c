@@ -278,6 +278,7 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
             x.rstep = cl->rstep;
             x.rpar = cl->rpar;
             x.prio = x.nb >= 12 ? 3 : x.nb >= 8 ? 2 : x.nb >= 4 ? 1 : 0;
+            x.lda = RP->n;
             x.aoff = 0;
             if (x.nb > FTAMD_SYS_WAVES) {
                 free(ch);
@@ -298,6 +299,80 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
     qsort(ch, (size_t)nch, sizeof(FtamdSysChain), cmp_nb_desc);
     *out = ch;
     *count = nch;
+    return FTAMD_OK;
+}
+
+/* Bands and chains of a batch of transforms of one kind, direction,
+ * precision and S (every plan's tables built): the bands of each distinct
+ * plan once, their table offsets taken from the first plan's tables, the
+ * chains of every transform with its bands, matrix offset from dA[0] and
+ * column stride, longest first.  FTAMD_EINVAL if a chain does not fit. */
+int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
+                    const int *M, int count, FtamdSysBand **bands_out, int *nbands_out, FtamdSysChain **chains_out,
+                    int *nchains_out, const void **tab_base) {
+    *bands_out = NULL;
+    *chains_out = NULL;
+    *nbands_out = *nchains_out = 0;
+    *tab_base = count > 0 ? tabs[0]->d_tab : NULL;
+    if (count <= 0) return FTAMD_OK;
+    const long long pair = tabs[0]->precision == 0 ? 2 * (long long)sizeof(double) : 2 * (long long)sizeof(float);
+    const long long esz = pair / 2;
+    /* distinct tables and the first band of each in the merged array */
+    const ft_sys_tables **seen = malloc(sizeof(*seen) * (size_t)count);
+    int *first = malloc(sizeof(int) * (size_t)count);
+    int nseen = 0, nb = 0, rc = (seen && first) ? FTAMD_OK : FTAMD_ENOMEM;
+    for (int i = 0; rc == FTAMD_OK && i < count; i++) {
+        int k = 0;
+        while (k < nseen && seen[k] != tabs[i]) k++;
+        if (k == nseen) {
+            seen[nseen] = tabs[i];
+            first[nseen++] = nb;
+            nb += tabs[i]->nbands;
+        }
+    }
+    FtamdSysBand *bands = rc == FTAMD_OK ? malloc(sizeof(FtamdSysBand) * (size_t)(nb > 0 ? nb : 1)) : NULL;
+    if (rc == FTAMD_OK && !bands) rc = FTAMD_ENOMEM;
+    for (int k = 0; rc == FTAMD_OK && k < nseen; k++) {
+        /* tables are separate allocations (aligned far beyond one pair) */
+        const long long shift = ((const char *)seen[k]->d_tab - (const char *)*tab_base) / pair;
+        for (int b = 0; b < seen[k]->nbands; b++) {
+            bands[first[k] + b] = seen[k]->bands[b];
+            bands[first[k] + b].tab += shift;
+        }
+    }
+    FtamdSysChain *all = NULL;
+    int nall = 0, cap = 0;
+    for (int i = 0; rc == FTAMD_OK && i < count; i++) {
+        int k = 0;
+        while (seen[k] != tabs[i]) k++;
+        FtamdSysChain *ch = NULL;
+        int nch = 0;
+        rc = ftamd_sys_chains(plans[i], tabs[i], M[i], &ch, &nch);
+        if (rc == FTAMD_OK && nall + nch > cap) {
+            cap = 2 * (nall + nch) + 16;
+            FtamdSysChain *grown = realloc(all, sizeof(FtamdSysChain) * (size_t)cap);
+            if (!grown) rc = FTAMD_ENOMEM;
+            else all = grown;
+        }
+        for (int c = 0; rc == FTAMD_OK && c < nch; c++) {
+            ch[c].band0 += first[k];
+            ch[c].aoff = ((const char *)dA[i] - (const char *)dA[0]) / esz;
+            all[nall++] = ch[c];
+        }
+        free(ch);
+    }
+    free(seen);
+    free(first);
+    if (rc != FTAMD_OK) {
+        free(bands);
+        free(all);
+        return rc;
+    }
+    qsort(all, (size_t)nall, sizeof(FtamdSysChain), cmp_nb_desc);
+    *bands_out = bands;
+    *nbands_out = nb;
+    *chains_out = all;
+    *nchains_out = nall;
     return FTAMD_OK;
 }
 

@@ -93,6 +93,10 @@ struct Engine {
     static constexpr int kD = 4 * kSPD;
     static_assert(kSPD >= 1 && kU % kSPD == 0 && kD <= FTAMD_SYS_PREFETCH,
                   "DMA blocks must tile the unroll and the tables' spare steps");
+    // a block's table slots are at most one wrap of the LDS ring, and its
+    // DMAs use the compile-time offsets 0, 2, 4, 6 (kSPD 2) or 0, 4 (kSPD 4)
+    static_assert(kD >= kU && kD % kU == 0 && (kSPD == 2 || kSPD == 4),
+                  "table staging supports 2 or 4 steps per DMA and a ring of whole blocks");
 
 
     static constexpr int kMaxRows = sizeof(T) == 8 ? FTAMD_SYS_MAX_ROWS_FP64 : FTAMD_SYS_MAX_ROWS_FP32;
@@ -258,7 +262,7 @@ struct Engine {
         // outputs beyond the next band's stream leave the chain
         const int npa_next = last ? 0 : (dir == 0 ? Rnext - R : 0);
         const int rstep = uniform(ch.rstep), rpar = uniform(ch.rpar);
-        const long long lda = a.lda;
+        const long long lda = uniform(ch.lda);
         T *A = (T *)a.A + ch.aoff;
         const int colv[kCols] = {ch.col[2 * r], ch.col[2 * r + 1]};
         const char *tab_band = (const char *)a.tab + a.bands[b].tab * (long long)sizeof(Pair<T>);

@@ -125,13 +125,16 @@ int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
  * engine wherever a plan's chains fit it (at most 512 FP64 / 1024 FP32 rows
  * per chain stream, 16 bands per chain), the pipelined engine elsewhere;
  * FT_GPU_ENGINE_PIPELINE and FT_GPU_ENGINE_SYSTOLIC force one (the latter
- * fails with FT_GPU_EINVAL where it does not apply).  Single-vector kernels
- * and batches always run the pipelined engine. */
+ * fails with FT_GPU_EINVAL where it does not apply).  Batches run the
+ * pipelined engine unless the systolic one is forced (then every transform
+ * must fit it with one band width); single-vector kernels always run the
+ * pipelined engine. */
 enum { FT_GPU_ENGINE_AUTO = 0, FT_GPU_ENGINE_PIPELINE = 1, FT_GPU_ENGINE_SYSTOLIC = 2 };
 int ft_gpu_set_engine(int engine);
 
 /* Engine (FT_GPU_ENGINE_PIPELINE or _SYSTOLIC) of the calling thread's last
- * whole-matrix launch, 0 before any (no reference counterpart; diagnostics). */
+ * whole-matrix or batch launch, 0 before any (no reference counterpart;
+ * diagnostics). */
 int ft_gpu_last_engine(void);
 
 #ifdef __cplusplus

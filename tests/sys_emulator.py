@@ -19,7 +19,7 @@ class Band(ctypes.Structure):
 
 class Chain(ctypes.Structure):
     _fields_ = [("band0", ctypes.c_int), ("nb", ctypes.c_int), ("dir", ctypes.c_int), ("rstep", ctypes.c_int),
-                ("rpar", ctypes.c_int), ("prio", ctypes.c_int), ("col", ctypes.c_int * 8),
+                ("rpar", ctypes.c_int), ("prio", ctypes.c_int), ("lda", ctypes.c_int), ("col", ctypes.c_int * 8),
                 ("glo", ctypes.c_int * 8), ("ghi", ctypes.c_int * 8), ("aoff", ctypes.c_longlong)]
 
 

@@ -147,6 +147,20 @@ def test_engines(ft, engine, kind, params):
         ft.set_engine(ft.ENGINE_AUTO)
 
 
+@pytest.mark.parametrize("engine", ["pipeline", "systolic"])
+def test_engines_at_limits(ft, engine):
+    """Both engines forced at the headline shape (sph n = 1024, 16 bands per
+    chain in the systolic engine) and tri/disk at n = 512, the largest FP64
+    size of the systolic engine (16 bands, 512 staged rows)."""
+    try:
+        ft.set_engine(ft.ENGINE_PIPELINE if engine == "pipeline" else ft.ENGINE_SYSTOLIC)
+        for kind, params, n in (("sph", (), 1024), ("tri", (0.0, 0.0, 0.0), 512), ("disk", (0.0, 0.0), 512)):
+            for d in (0, 1):
+                _check(ft, kind, params, n, d, seed=77)
+    finally:
+        ft.set_engine(ft.ENGINE_AUTO)
+
+
 def test_async_device_api_and_stream(ft):
     import torch
     n = 300
@@ -163,24 +177,39 @@ def test_async_device_api_and_stream(ft):
     assert np.array_equal(T.cpu().numpy().T, R)
 
 
-def test_batch_uneven_degrees(ft):
+@pytest.mark.parametrize("engine", ["auto", "systolic"])
+def test_batch_uneven_degrees(ft, engine):
     """ft_gpu_execute_batch: transforms of log-uniform degree in one launch,
-    each bit-identical to the oracle."""
+    each bit-identical to the oracle; the automatic choice runs batches on the
+    pipelined engine; forced, the band-systolic engine runs them too, and
+    refuses a batch with a transform beyond its sizes (tri n = 600 in FP64)."""
     import torch
     rng = np.random.default_rng(4)
     ns = np.unique(np.round(np.exp(rng.uniform(np.log(4), np.log(300), 24))).astype(int))
-    for kind, params in (("sph", ()), ("tri", (0.0, 0.0, 0.0)), ("disk", (0.0, 0.0))):
-        for d in (0, 1):
-            _say("batch", kind, d, list(ns))
-            maker = {"sph": ft.plan_rotsphere, "tri": ft.plan_rottriangle, "disk": ft.plan_rotdisk}[kind]
-            plans = [maker(int(n), *params) for n in ns]
-            As = [O.fill(kind, int(n), full_M(kind, int(n)), 100 + int(n)) for n in ns]
-            Ts = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
-            ft.gpu_execute_batch(plans, d, Ts)
-            torch.cuda.synchronize()
-            for n, A, T in zip(ns, As, Ts):
-                R = O.execute(O.Plan(kind, int(n), *params), d, A.copy(order="F"))
-                assert np.array_equal(T.cpu().numpy().T, R), (kind, d, n)
+    cases = [("sph", (), ns), ("tri", (0.0, 0.0, 0.0), ns), ("disk", (0.0, 0.0), ns),
+             ("tri", (0.0, 0.0, 0.0), np.append(ns[:6], 600))]
+    try:
+        ft.set_engine(ft.ENGINE_AUTO if engine == "auto" else ft.ENGINE_SYSTOLIC)
+        for kind, params, sizes in cases:
+            for d in (0, 1):
+                _say("batch", engine, kind, d, list(sizes))
+                maker = {"sph": ft.plan_rotsphere, "tri": ft.plan_rottriangle, "disk": ft.plan_rotdisk}[kind]
+                plans = [maker(int(n), *params) for n in sizes]
+                As = [O.fill(kind, int(n), full_M(kind, int(n)), 100 + int(n)) for n in sizes]
+                Ts = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
+                if engine == "systolic" and max(sizes) > 512:
+                    with pytest.raises(ft.GPUError):
+                        ft.gpu_execute_batch(plans, d, Ts)
+                    continue
+                ft.gpu_execute_batch(plans, d, Ts)
+                used = ft.last_engine()
+                torch.cuda.synchronize()
+                assert used == (ft.ENGINE_PIPELINE if engine == "auto" else ft.ENGINE_SYSTOLIC), used
+                for n, A, T in zip(sizes, As, Ts):
+                    R = O.execute(O.Plan(kind, int(n), *params), d, A.copy(order="F"))
+                    assert np.array_equal(T.cpu().numpy().T, R), (kind, d, n)
+    finally:
+        ft.set_engine(ft.ENGINE_AUTO)
 
 
 @pytest.mark.parametrize("n", [256, 512, 1024])
