@@ -58,6 +58,7 @@ SIGNATURES = {
     "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
     "ft_gpu_set_engine": (_i, [_i]),
     "ft_gpu_last_engine": (_i, []),
+    "ft_gpu_set_spin_limit": (_i, [_i]),
 }
 for _k in ("sph", "tri", "disk"):
     for _dn in ("hi2lo", "lo2hi"):
@@ -161,6 +162,12 @@ def set_engine(engine=ENGINE_AUTO):
     rc = _lib.ft_gpu_set_engine(int(engine))
     if rc:
         raise ValueError("unknown engine %r" % (engine,))
+
+
+def set_spin_limit(limit=0):
+    """Polls before a band-systolic wait gives up (tests of the failure path); 0 = default."""
+    if _lib.ft_gpu_set_spin_limit(int(limit)):
+        raise ValueError("spin limit must be >= 0")
 
 
 def last_engine():

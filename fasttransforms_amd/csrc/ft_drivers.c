@@ -58,6 +58,13 @@ int ft_gpu_set_tuning(int L, int W, int wg_per_cu) {
 }
 
 static int g_engine = FT_GPU_ENGINE_AUTO;
+static int g_spin_limit = 0;
+
+int ft_gpu_set_spin_limit(int limit) {
+    if (limit < 0) return FT_GPU_EINVAL;
+    g_spin_limit = limit;
+    return FT_GPU_OK;
+}
 
 int ft_gpu_set_engine(int engine) {
     if (engine != FT_GPU_ENGINE_AUTO && engine != FT_GPU_ENGINE_PIPELINE && engine != FT_GPU_ENGINE_SYSTOLIC)
@@ -504,6 +511,7 @@ static int launch(ft_schedule *S, int precision, int kind, int dir, void *dA, vo
     if (S->sys) {
         FtamdSysLaunch b = S->sys_launch;
         if (dA) b.A = dA;  /* batches keep their own base */
+        b.spin_limit = g_spin_limit ? g_spin_limit : 1 << 22;
         b.trace = g_trace;
         return ftamd_hip_sys_launch(&b, precision, S->L, stream);
     }

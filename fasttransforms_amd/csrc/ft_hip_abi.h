@@ -115,6 +115,7 @@ typedef struct {
     const void *tab;  /* base of the band tables (bands address them by entry offsets from it) */
     void *A;          /* base of the matrices (chains address them by element offsets from it) */
     int *err;         /* FTAMD_ERR_* bits */
+    int spin_limit;   /* polls before a wait between bands gives up (FTAMD_ERR_TIMEOUT) */
     unsigned long long *trace;  /* diagnostic per-wave timeline (instrumented builds only), else NULL */
 } FtamdSysLaunch;
 

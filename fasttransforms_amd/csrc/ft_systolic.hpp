@@ -54,7 +54,6 @@ constexpr int kZ = 32;                 // ring positions between two bands
 constexpr int kU = FTAMD_SYS_UNROLL;   // steps per loop iteration
 constexpr int kDin = 2;                // input positions read ahead
 constexpr int kChunk = 16;             // matrix rows fetched per lane-parallel load
-constexpr int kSpinLimit = 1 << 22;
 
 
 template <typename T> using Pair = T __attribute__((ext_vector_type(2)));
@@ -138,8 +137,9 @@ struct Engine {
     }
 #endif
 
-    // Waits until *ctr >= need (seen caches the last value read).
-    __device__ static bool wait_ge(Shared &sh, int *ctr, int need, int &seen) {
+    // Waits until *ctr >= need (seen caches the last value read), at most
+    // `limit` polls.
+    __device__ static bool wait_ge(Shared &sh, int *ctr, int need, int &seen, int limit) {
 #ifdef FTAMD_TRACE
         struct Acc {
             unsigned long long t0, *dst;
@@ -149,7 +149,7 @@ struct Engine {
         } acc{__builtin_amdgcn_s_memtime(), (ctr >= &sh.avail[0] && ctr < &sh.avail[kWaves]) ? &tr_in() : &tr_out()};
 #endif
         for (int it = 0; seen < need; ++it) {
-            if (it >= kSpinLimit || uniform(lds_ld(&sh.abort_flag))) return false;
+            if (it >= limit || uniform(lds_ld(&sh.abort_flag))) return false;
             __builtin_amdgcn_s_sleep(1);
             seen = uniform(lds_ld(ctr));
         }
@@ -263,6 +263,7 @@ struct Engine {
         const int npa_next = last ? 0 : (dir == 0 ? Rnext - R : 0);
         const int rstep = uniform(ch.rstep), rpar = uniform(ch.rpar);
         const long long lda = uniform(ch.lda);
+        const int spin = uniform(a.spin_limit);  // polls before a wait gives up
         T *A = (T *)a.A + ch.aoff;
         const int colv[kCols] = {ch.col[2 * r], ch.col[2 * r + 1]};
         const char *tab_band = (const char *)a.tab + a.bands[b].tab * (long long)sizeof(Pair<T>);
@@ -326,7 +327,7 @@ struct Engine {
         T inbuf[kDin][kCols];
         auto ensure_in = [&](int p) -> bool {
             bool good = true;
-            if (p >= npa && p <= R && p >= avail_seen) good = wait_ge(sh, &sh.avail[w], p + 1, avail_seen);
+            if (p >= npa && p <= R && p >= avail_seen) good = wait_ge(sh, &sh.avail[w], p + 1, avail_seen, spin);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             return good;
         };
@@ -372,7 +373,7 @@ struct Engine {
             // every input position the block reads ahead is in the ring
             {
                 const int pl = min(t0 + kU - 1 + kDin, R);
-                if (pl >= npa && pl >= avail_seen && !wait_ge(sh, &sh.avail[w], pl + 1, avail_seen)) {
+                if (pl >= npa && pl >= avail_seen && !wait_ge(sh, &sh.avail[w], pl + 1, avail_seen, spin)) {
                     ok = false;
                     break;
                 }
@@ -387,7 +388,7 @@ struct Engine {
                 const int hi = min(po0 + kU - 1, ring_hi);
                 if (hi >= max(po0, 0)) {
                     const int P1 = hi + npa_next;
-                    if (P1 - kZ >= freed_seen && !wait_ge(sh, &sh.freed[w + 1], P1 - kZ + 1, freed_seen)) {
+                    if (P1 - kZ >= freed_seen && !wait_ge(sh, &sh.freed[w + 1], P1 - kZ + 1, freed_seen, spin)) {
                         ok = false;
                         break;
                     }

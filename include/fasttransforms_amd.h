@@ -132,6 +132,11 @@ int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
 enum { FT_GPU_ENGINE_AUTO = 0, FT_GPU_ENGINE_PIPELINE = 1, FT_GPU_ENGINE_SYSTOLIC = 2 };
 int ft_gpu_set_engine(int engine);
 
+/* Polls after which a band-systolic wait between bands gives up and the
+ * launch reports FT_GPU_ETIMEOUT (no reference counterpart; for tests of the
+ * failure path).  0 restores the default (2^22 polls, seconds). */
+int ft_gpu_set_spin_limit(int limit);
+
 /* Engine (FT_GPU_ENGINE_PIPELINE or _SYSTOLIC) of the calling thread's last
  * whole-matrix or batch launch, 0 before any (no reference counterpart;
  * diagnostics). */

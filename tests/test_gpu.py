@@ -161,6 +161,27 @@ def test_engines_at_limits(ft, engine):
         ft.set_engine(ft.ENGINE_AUTO)
 
 
+def test_systolic_wait_timeout_is_reported(ft):
+    """The band-systolic engine's waits are bounded: with a host-set limit of
+    one poll, the waits between bands expire, every wave leaves its loop, and
+    the executor reports FT_GPU_ETIMEOUT; with the limit restored the same
+    plan gives exact results again."""
+    n = 300
+    P, Q = _plans(ft, "sph", n, ())
+    A = O.fill("sph", n, 2 * n - 1, 11)
+    try:
+        ft.set_engine(ft.ENGINE_SYSTOLIC)
+        ft.set_spin_limit(1)
+        _say("systolic timeout path")
+        with pytest.raises(ft.GPUError, match="wait expired"):
+            ft.execute_sph_hi2lo(P, A.copy(order="F"))
+    finally:
+        ft.set_spin_limit(0)
+        ft.set_engine(ft.ENGINE_AUTO)
+    G = ft.execute_sph_hi2lo(P, A.copy(order="F"))
+    assert np.array_equal(G, O.execute(Q, 0, A.copy(order="F")))
+
+
 def test_async_device_api_and_stream(ft):
     import torch
     n = 300
