@@ -252,6 +252,35 @@ def test_fp32_path_error(ft, n):
         assert e2 < 8 * np.sqrt(n) * 2.0 ** -23
 
 
+def test_fp32_wide_bands_engines_agree(ft):
+    """FP32 shapes that the band-systolic engine runs with 4 sequences per
+    lane (more than 16 bands of 2): it and the pipelined engine evaluate the
+    same fused FP32 rotations in the same order, so they agree bit for bit,
+    and both stay within the FP32 bound of the FP64 oracle."""
+    import torch
+    cases = (("sph", (), 2048), ("tri", (0.0, 0.0, 0.0), 1024), ("disk", (0.0, 0.0), 700))
+    try:
+        for kind, params, n in cases:
+            P, Q = _plans(ft, kind, n, params)
+            A = O.fill(kind, n, full_M(kind, n), 88)
+            R = O.execute(Q, 0, A.copy(order="F"))
+            got = {}
+            for name, eng in (("systolic", ft.ENGINE_SYSTOLIC), ("pipeline", ft.ENGINE_PIPELINE)):
+                ft.set_engine(eng)
+                T = torch.from_numpy(np.ascontiguousarray(A.T)).float().cuda()
+                _say("fp32", name, kind, n)
+                ft.gpu_execute(P, 0, T, precision=ft.FP32)
+                assert ft.last_engine() == eng
+                torch.cuda.synchronize()
+                got[name] = T.cpu().numpy().T
+                e2 = O.rel2(got[name].astype(np.float64), R)
+                _say("fp32 %s %s n=%d rel2=%.3e" % (name, kind, n, e2))
+                assert e2 < 8 * np.sqrt(n) * 2.0 ** -23
+            assert np.array_equal(got["systolic"], got["pipeline"]), (kind, n)
+    finally:
+        ft.set_engine(ft.ENGINE_AUTO)
+
+
 def test_batch_graph_replays(ft):
     """A batch plan's launch captured in a HIP graph: every replay (with eager
     work and synchronisation between replays) equals eager execution."""
