@@ -13,6 +13,7 @@
  * takes the same sequences in reverse with the transposed rotation.
  */
 #include <limits.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -87,7 +88,6 @@ struct ft_sys_tables {
 int ftamd_sys_tables_S(const ft_sys_tables *T) { return T->S; }
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T) { return T->d_bands; }
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T) { return T->d_tab; }
-size_t ftamd_sys_tables_bytes(const ft_sys_tables *T) { return T->tab_bytes; }
 
 void ftamd_sys_tables_free(ft_sys_tables *T) {
     if (!T) return;
@@ -334,7 +334,7 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
     if (rc == FTAMD_OK && !bands) rc = FTAMD_ENOMEM;
     for (int k = 0; rc == FTAMD_OK && k < nseen; k++) {
         /* tables are separate allocations (aligned far beyond one pair) */
-        const long long shift = ((const char *)seen[k]->d_tab - (const char *)*tab_base) / pair;
+        const long long shift = ((long long)(uintptr_t)seen[k]->d_tab - (long long)(uintptr_t)*tab_base) / pair;
         for (int b = 0; b < seen[k]->nbands; b++) {
             bands[first[k] + b] = seen[k]->bands[b];
             bands[first[k] + b].tab += shift;
@@ -356,7 +356,7 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
         }
         for (int c = 0; rc == FTAMD_OK && c < nch; c++) {
             ch[c].band0 += first[k];
-            ch[c].aoff = ((const char *)dA[i] - (const char *)dA[0]) / esz;
+            ch[c].aoff = ((long long)(uintptr_t)dA[i] - (long long)(uintptr_t)dA[0]) / esz;
             all[nall++] = ch[c];
         }
         free(ch);
