@@ -51,6 +51,7 @@ SIGNATURES = {
     "ft_gpu_last_error": (_i, []),
     "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
     "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
+    "ft_gpu_synchronize": (_i, [_vp]),
     "ft_gpu_execute_batch": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp]),
     "ft_gpu_plan_batch": (_vp, [_vp, _i, _i, _i, _vp, _vp, _i]),
     "ft_gpu_execute_batch_plan": (_i, [_vp, _vp]),
@@ -280,6 +281,18 @@ def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None):
     if rc:
         raise GPUError("ft_gpu_execute: %s" % _lib.ft_gpu_error_string(rc).decode())
     return A
+
+
+def synchronize(stream=None):
+    """ft_gpu_synchronize: wait for ``stream`` (a torch.cuda.Stream, default the
+    current one) and raise GPUError if an asynchronous launch made on it ended
+    with invalid results (a bounded wait expired, or work was left undone)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    rc = _lib.ft_gpu_synchronize(_vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_synchronize: %s (code %d)" % (_lib.ft_gpu_error_string(rc).decode(), rc))
 
 
 def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream=None):

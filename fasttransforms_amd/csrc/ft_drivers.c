@@ -45,15 +45,18 @@ const char *ft_gpu_error_string(int code) {
 
 /* ------------------------------------------------------------ tuning -- */
 
+/* Process-wide knobs, read and written atomically (a call reads each once). */
 static int g_L = 0, g_W = 0, g_wg_per_cu = 0;
+#define KNOB_GET(v) __atomic_load_n(&(v), __ATOMIC_RELAXED)
+#define KNOB_SET(v, x) __atomic_store_n(&(v), (x), __ATOMIC_RELAXED)
 
 int ft_gpu_set_tuning(int L, int W, int wg_per_cu) {
     if (L != 0 && L != 2 && L != 4 && L != 8) return FT_GPU_EINVAL;
     if (W != 0 && W != 4 && W != 8 && W != 16) return FT_GPU_EINVAL;
     if (wg_per_cu < 0 || wg_per_cu > 8) return FT_GPU_EINVAL;
-    g_L = L;
-    g_W = W;
-    g_wg_per_cu = wg_per_cu;
+    KNOB_SET(g_L, L);
+    KNOB_SET(g_W, W);
+    KNOB_SET(g_wg_per_cu, wg_per_cu);
     return FT_GPU_OK;
 }
 
@@ -62,18 +65,18 @@ static int g_spin_limit = 0;
 
 int ft_gpu_set_spin_limit(int limit) {
     if (limit < 0) return FT_GPU_EINVAL;
-    g_spin_limit = limit;
+    KNOB_SET(g_spin_limit, limit);
     return FT_GPU_OK;
 }
 
 int ft_gpu_set_engine(int engine) {
     if (engine != FT_GPU_ENGINE_AUTO && engine != FT_GPU_ENGINE_PIPELINE && engine != FT_GPU_ENGINE_SYSTOLIC)
         return FT_GPU_EINVAL;
-    g_engine = engine;
+    KNOB_SET(g_engine, engine);
     return FT_GPU_OK;
 }
 
-int ftamd_engine(void) { return g_engine; }
+int ftamd_engine(void) { return KNOB_GET(g_engine); }
 
 static __thread int t_last_engine = 0;
 int ft_gpu_last_engine(void) { return t_last_engine; }
@@ -81,10 +84,14 @@ int ft_gpu_last_engine(void) { return t_last_engine; }
 int ftamd_tuning_L(int kind, int n) {
     (void)kind;
     (void)n;
-    return g_L ? g_L : 4;
+    int L = KNOB_GET(g_L);
+    return L ? L : 4;
 }
 
-int ftamd_tuning_W(void) { return g_W ? g_W : 16; }
+int ftamd_tuning_W(void) {
+    int W = KNOB_GET(g_W);
+    return W ? W : 16;
+}
 
 /* Workgroups are dealt to the XCDs in turn and each takes tasks only from its
  * own XCD's list, so the grid holds as many workgroups per XCD as the longest
@@ -96,7 +103,8 @@ int ftamd_tuning_grid(const ft_schedule *S) {
     /* batches default to 2 per CU: their many short tasks keep that many busy,
      * and more workgroups only contend (512 sph transforms of degree
      * 32..1018: 9.9 ms capped, 10.4 ms uncapped) */
-    int per_cu = g_wg_per_cu ? g_wg_per_cu : (S->batch ? 2 : 0);
+    int knob = KNOB_GET(g_wg_per_cu);
+    int per_cu = knob ? knob : (S->batch ? 2 : 0);
     if (per_cu) {
         int cu = 0;
         if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
@@ -363,6 +371,89 @@ void ftamd_schedule_free_device(ft_schedule *S) {
     S->d_mem = S->d_bnd = S->d_sys = NULL;
 }
 
+/* ------------------------------------------------- launch error words --
+ * Every schedule has a device error word its kernels set (FTAMD_ERR_*).  The
+ * synchronous entry points clear it before and read it after their launch.
+ * Asynchronous launches (ft_gpu_execute, batches) register the schedule here
+ * with its stream; ft_gpu_synchronize(stream) waits for the stream and reads
+ * and clears the words of the schedules launched on it. */
+static pthread_mutex_t g_pend_lock = PTHREAD_MUTEX_INITIALIZER;
+static ft_schedule **g_pend = NULL;
+static int g_npend = 0, g_cap_pend = 0;
+
+static void pending_add(ft_schedule *S, void *stream) {
+    pthread_mutex_lock(&g_pend_lock);
+    if (!S->pending) {
+        if (g_npend == g_cap_pend) {
+            int cap = g_cap_pend ? 2 * g_cap_pend : 16;
+            ft_schedule **grown = realloc(g_pend, sizeof(*grown) * (size_t)cap);
+            if (!grown) {
+                pthread_mutex_unlock(&g_pend_lock);
+                return; /* the error word is still read by a later synchronous call */
+            }
+            g_pend = grown;
+            g_cap_pend = cap;
+        }
+        g_pend[g_npend++] = S;
+        S->pending = 1;
+    }
+    S->pending_stream = stream;
+    pthread_mutex_unlock(&g_pend_lock);
+}
+
+static void pending_remove_locked(ft_schedule *S) {
+    for (int i = 0; i < g_npend; i++)
+        if (g_pend[i] == S) {
+            g_pend[i] = g_pend[--g_npend];
+            break;
+        }
+    S->pending = 0;
+}
+
+/* Frees a schedule (its device memory and the struct). */
+void ftamd_schedule_release(ft_schedule *S) {
+    if (!S) return;
+    pthread_mutex_lock(&g_pend_lock);
+    if (S->pending) pending_remove_locked(S);
+    pthread_mutex_unlock(&g_pend_lock);
+    ftamd_schedule_free_device(S);
+    free(S);
+}
+
+/* Reads and clears one schedule's error word (stream-ordered). */
+static int read_error_word(ft_schedule *S, void *stream) {
+    int err = 0;
+    int rc = ftamd_hip_d2h(&err, S->launch.err, sizeof(int), stream);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(stream);
+    if (rc != FTAMD_OK) return rc;
+    if (err) {
+        static const int zero = 0;
+        rc = ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), stream);
+        if (rc == FTAMD_OK) rc = ftamd_hip_sync(stream);
+        if (rc != FTAMD_OK) return rc;
+        return (err & FTAMD_ERR_TIMEOUT) ? FTAMD_ETIMEOUT : FTAMD_EINCOMPLETE;
+    }
+    return FTAMD_OK;
+}
+
+int ft_gpu_synchronize(void *stream) {
+    int rc = ftamd_hip_sync(stream);
+    if (rc != FTAMD_OK) return rc;
+    pthread_mutex_lock(&g_pend_lock);
+    for (int i = 0; i < g_npend;) {
+        ft_schedule *S = g_pend[i];
+        if (S->pending_stream != stream) {
+            i++;
+            continue;
+        }
+        int r = read_error_word(S, stream);
+        if (rc == FTAMD_OK) rc = r;
+        pending_remove_locked(S); /* swaps the last entry into slot i */
+    }
+    pthread_mutex_unlock(&g_pend_lock);
+    return rc;
+}
+
 /* Columns of one n x M matrix, ordered so that equal-need columns are adjacent:
  * sph/disk by order parity then increasing order, tri by increasing order. */
 static int matrix_columns(int kind, int n, int M, void *dA, ft_col **out, int *ncols) {
@@ -441,11 +532,6 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     return S;
 }
 
-/* Diagnostics: device buffer that instrumented builds fill with a timeline
- * (8 words per task and stage wave); NULL disables. */
-static void *g_trace = NULL;
-void ftamd_set_trace(void *dev_buffer) { g_trace = dev_buffer; }
-
 /* Band-systolic schedule of the whole matrix (cached in the plan; caller
  * holds RP->lock).  Returns NULL with *rc == FTAMD_OK where the engine does
  * not apply. */
@@ -511,28 +597,30 @@ static int launch(ft_schedule *S, int precision, int kind, int dir, void *dA, vo
     if (S->sys) {
         FtamdSysLaunch b = S->sys_launch;
         if (dA) b.A = dA;  /* batches keep their own base */
-        b.spin_limit = g_spin_limit ? g_spin_limit : 1 << 22;
-        b.trace = g_trace;
+        int spin = KNOB_GET(g_spin_limit);
+        b.spin_limit = spin ? spin : 1 << 22;
+        b.trace = NULL;
         return ftamd_hip_sys_launch(&b, precision, S->L, stream);
     }
     FtamdLaunch a = S->launch;
-    a.trace = g_trace;
+    a.trace = NULL;
     a.A = dA;
     return ftamd_hip_launch(&a, precision, kind, dir, S->L, S->W, ftamd_tuning_grid(S), stream);
 }
 
-static int check_kernel_error(ft_schedule *S) {
-    int err = 0;
-    int rc = ftamd_hip_d2h(&err, S->launch.err, sizeof(int), NULL);
+/* Synchronous launches: clear the schedule's error word first (a word an
+ * unchecked asynchronous launch left set is not this call's), launch, wait,
+ * read it. */
+static int launch_checked(ft_schedule *S, int precision, int kind, int dir, void *dA) {
+    static const int zero = 0;
+    pthread_mutex_lock(&g_pend_lock);
+    if (S->pending) pending_remove_locked(S);
+    pthread_mutex_unlock(&g_pend_lock);
+    int rc = ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), NULL);
+    if (rc == FTAMD_OK) rc = launch(S, precision, kind, dir, dA, NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    if (rc != FTAMD_OK) return rc;
-    if (err) {
-        int zero = 0;
-        ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), NULL);
-        ftamd_hip_sync(NULL);
-        return (err & FTAMD_ERR_TIMEOUT) ? FTAMD_ETIMEOUT : FTAMD_EINCOMPLETE;
-    }
-    return FTAMD_OK;
+    if (rc == FTAMD_OK) rc = read_error_word(S, NULL);
+    return rc;
 }
 
 /* --------------------------------------------------------- executors -- */
@@ -545,23 +633,21 @@ static int valid_M(int kind, int n, int M) {
     return kind == FTAMD_SPH ? M <= 2 * n - 1 : M <= 4 * n - 3;
 }
 
-/* Device-resident, asynchronous execution. */
-static int gpu_execute(ft_rotation_plan *RP, int dir, int precision, void *dA, int M, void *stream,
-                       ft_schedule **used) {
-    int rc = FTAMD_OK;
+/* Schedule for one launch on RP (caller holds no lock). */
+static int pick_schedule(ft_rotation_plan *RP, int dir, int precision, void *dA, int M, ft_schedule **out) {
+    int rc = FTAMD_OK, engine = KNOB_GET(g_engine);
     pthread_mutex_lock(&RP->lock);
     rc = ftamd_plan_upload(RP, precision);
     ft_schedule *S = NULL;
-    if (rc == FTAMD_OK && g_engine != FT_GPU_ENGINE_PIPELINE) {
+    if (rc == FTAMD_OK && engine != FT_GPU_ENGINE_PIPELINE) {
         S = sys_schedule(RP, dir, precision, M, &rc);
-        if (rc == FTAMD_OK && !S && g_engine == FT_GPU_ENGINE_SYSTOLIC) rc = FTAMD_EINVAL;
+        if (rc == FTAMD_OK && !S && engine == FT_GPU_ENGINE_SYSTOLIC) rc = FTAMD_EINVAL;
     }
     if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, dir, precision, M, dA, &rc);
     pthread_mutex_unlock(&RP->lock);
-    if (rc != FTAMD_OK) return rc;
-    if (used) *used = S;
-    t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
-    return launch(S, precision, RP->kind, dir, dA, stream);
+    *out = S;
+    if (rc == FTAMD_OK) t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
+    return rc;
 }
 
 int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
@@ -571,10 +657,30 @@ int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int pre
         !valid_M(kind, RP->n, M))
         return FT_GPU_EINVAL;
     if (M == 0) return FT_GPU_OK;
-    return gpu_execute(RP, direction, precision, dA, M, stream, NULL);
+    ft_schedule *S = NULL;
+    int rc = pick_schedule(RP, direction, precision, dA, M, &S);
+    if (rc != FTAMD_OK) return rc;
+    rc = launch(S, precision, RP->kind, direction, dA, stream);
+    if (rc == FTAMD_OK) pending_add(S, stream);
+    return rc;
 }
 
-/* The reference entry points: synchronous, host or device A. */
+/* Grows the plan's staging buffer to `bytes` (caller holds RP->exec_lock). */
+static int stage_reserve(ft_rotation_plan *RP, size_t bytes) {
+    if (RP->stage_bytes >= bytes) return FTAMD_OK;
+    ftamd_hip_free(RP->d_stage);
+    RP->d_stage = NULL;
+    RP->stage_bytes = 0;
+    int rc = ftamd_hip_malloc(&RP->d_stage, bytes);
+    if (rc == FTAMD_OK) RP->stage_bytes = bytes;
+    return rc;
+}
+
+/* The reference entry points: synchronous, host or device A.  The reference
+ * only reads the plan, so concurrent calls on one plan with distinct A are
+ * safe there (drivers.c:120-124); here every call holds the plan's exec_lock
+ * from staging to the last copy back, so calls on one plan run one after
+ * another and never share the staging buffer or a schedule's workspace. */
 static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, int M) {
     ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
     if (!RP || !A || RP->kind != kind || !valid_M(kind, RP->n, M)) {
@@ -591,26 +697,18 @@ static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, i
     ftamd_hip_pointer_kind(A, &pk);
     size_t bytes = (size_t)RP->n * (size_t)M * sizeof(double);
     ft_schedule *S = NULL;
-    if (pk == FTAMD_PTR_DEVICE) {
-        rc = gpu_execute(RP, dir, 0, A, M, NULL, &S);
-        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    } else {
-        pthread_mutex_lock(&RP->lock);
-        if (RP->stage_bytes < bytes) {
-            ftamd_hip_free(RP->d_stage);
-            RP->d_stage = NULL;
-            RP->stage_bytes = 0;
-            rc = ftamd_hip_malloc(&RP->d_stage, bytes);
-            if (rc == FTAMD_OK) RP->stage_bytes = bytes;
-        }
-        void *d = RP->d_stage;
-        pthread_mutex_unlock(&RP->lock);
+    pthread_mutex_lock(&RP->exec_lock);
+    void *d = A;
+    if (pk != FTAMD_PTR_DEVICE) {
+        rc = stage_reserve(RP, bytes);
+        d = RP->d_stage;
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, bytes, NULL);
-        if (rc == FTAMD_OK) rc = gpu_execute(RP, dir, 0, d, M, NULL, &S);
-        if (rc == FTAMD_OK) rc = ftamd_hip_d2h(A, d, bytes, NULL);
-        if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     }
-    if (rc == FTAMD_OK && S) rc = check_kernel_error(S);
+    if (rc == FTAMD_OK) rc = pick_schedule(RP, dir, 0, d, M, &S);
+    if (rc == FTAMD_OK) rc = launch_checked(S, 0, RP->kind, dir, d);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, bytes, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&RP->exec_lock);
     ftamd_set_error(rc);
 }
 
@@ -643,6 +741,72 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
 }
 
 /* ------------------------------------------------- single vectors -- */
+
+/* Single-vector schedules a plan keeps (the reference's callers loop over
+ * orders with the same few (m1, m2, S)); the oldest beyond this are freed. */
+#define FTAMD_SINGLE_CACHE 32
+
+/* The cached single-vector schedule of (dir, jlo, jhi, S) on RP, built on
+ * first use; the column is a launch argument.  Caller holds RP->exec_lock. */
+static int single_schedule(ft_rotation_plan *RP, int kind, int dir, int jlo, int jhi, int S, ft_schedule **out) {
+    int L = ftamd_tuning_L(kind, RP->n), W = ftamd_tuning_W();
+    int rc = FTAMD_OK;
+    pthread_mutex_lock(&RP->lock);
+    ft_schedule *prev = NULL, *hit = NULL;
+    int count = 0;
+    for (ft_schedule *s = RP->scheds; s; prev = s, s = s->next) {
+        if (!s->single) continue;
+        if (!hit && s->dir == dir && s->m1 == jlo && s->m2 == jhi && s->rs == S && s->L == L && s->W == W) {
+            hit = s;
+            /* move to the front: the list's tail holds the least recently used */
+            if (prev) {
+                prev->next = s->next;
+                s->next = RP->scheds;
+                RP->scheds = s;
+            }
+            break;
+        }
+    }
+    if (!hit) {
+        rc = ftamd_plan_upload(RP, 0);
+        ft_schedule *Sd = rc == FTAMD_OK ? calloc(1, sizeof(ft_schedule)) : NULL;
+        if (rc == FTAMD_OK && !Sd) rc = FTAMD_ENOMEM;
+        if (rc == FTAMD_OK) {
+            Sd->single = 1;
+            Sd->dir = dir;
+            Sd->m1 = jlo;
+            Sd->m2 = jhi;
+            Sd->rs = S;
+            Sd->L = L;
+            Sd->W = W;
+            ft_col c = {NULL, RP->n, S, 0, jhi, jlo};
+            rc = build_from_columns(Sd, kind, dir, L, W, &c, 1, ftamd_plan_table(RP, 0), RP->n);
+            if (rc != FTAMD_OK) {
+                ftamd_schedule_free_device(Sd);
+                free(Sd);
+            } else {
+                Sd->next = RP->scheds;
+                RP->scheds = Sd;
+                hit = Sd;
+            }
+        }
+        /* evict the least recently used single-vector schedules beyond the cap */
+        ft_schedule **link = &RP->scheds;
+        count = 0;
+        while (*link) {
+            ft_schedule *s = *link;
+            if (s->single && ++count > FTAMD_SINGLE_CACHE) {
+                *link = s->next;
+                ftamd_schedule_release(s);
+                continue;
+            }
+            link = &s->next;
+        }
+    }
+    pthread_mutex_unlock(&RP->lock);
+    *out = hit;
+    return rc;
+}
 
 /*
  * ft_kernel_<kind>_<dir>(RP, m1, m2, A, S) (rotations.c:135-141, 162-168,
@@ -679,39 +843,19 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
     int pk = FTAMD_PTR_HOST;
     ftamd_hip_pointer_kind(A, &pk);
     size_t span = ((size_t)(n - 1) * (size_t)S + 1) * sizeof(double);
+    pthread_mutex_lock(&RP->exec_lock);
     void *d = A;
     if (pk != FTAMD_PTR_DEVICE) {
-        rc = ftamd_hip_malloc(&d, span);
+        rc = stage_reserve(RP, span);
+        d = RP->d_stage;
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, span, NULL);
     }
     ft_schedule *Sd = NULL;
-    if (rc == FTAMD_OK) {
-        pthread_mutex_lock(&RP->lock);
-        rc = ftamd_plan_upload(RP, 0);
-        pthread_mutex_unlock(&RP->lock);
-    }
-    if (rc == FTAMD_OK) {
-        Sd = calloc(1, sizeof(ft_schedule));
-        if (!Sd) rc = FTAMD_ENOMEM;
-    }
-    if (rc == FTAMD_OK) {
-        Sd->single = 1;
-        Sd->dir = dir;
-        Sd->L = ftamd_tuning_L(kind, n);
-        Sd->W = ftamd_tuning_W();
-        ft_col c = {d, n, S, 0, jhi, jlo};
-        rc = build_from_columns(Sd, kind, dir, Sd->L, Sd->W, &c, 1, ftamd_plan_table(RP, 0), n);
-    }
-    if (rc == FTAMD_OK) rc = launch(Sd, 0, kind, dir, NULL, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    if (rc == FTAMD_OK) rc = check_kernel_error(Sd);
+    if (rc == FTAMD_OK) rc = single_schedule(RP, kind, dir, jlo, jhi, S, &Sd);
+    if (rc == FTAMD_OK) rc = launch_checked(Sd, 0, kind, dir, d);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
-    if (Sd) {
-        ftamd_schedule_free_device(Sd);
-        free(Sd);
-    }
-    if (pk != FTAMD_PTR_DEVICE) ftamd_hip_free(d);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&RP->exec_lock);
     ftamd_set_error(rc);
 }
 
@@ -745,31 +889,98 @@ void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2
  * table of the largest plan serves every transform; only its packed index
  * uses that plan's n.  Each transform contributes its own groups; all tasks
  * go into the per-XCD, pass-ordered lists, so small transforms fill in around
- * the long chains of the large ones.  The schedule of the last distinct
- * argument list is cached.
+ * the long chains of the large ones.
+ *
+ * The schedule of the last argument list is cached.  It is keyed on the whole
+ * list, compared entry by entry: every plan pointer with the plan's serial
+ * number (a plan created at a freed plan's address gets another serial) and
+ * n, every matrix pointer and M, plus kind, direction, precision, engine and
+ * tuning.  Destroying a plan the cached schedule uses drops the schedule.
  */
 typedef struct {
     pthread_mutex_t lock;
     ft_schedule *sched;
     int count, kind, dir, precision, L, W, engine;
-    unsigned long long key;
+    const ft_rotation_plan **plans;
+    unsigned long long *serial;
+    int *n;
+    void **dA;
+    int *M;
 } ft_batch_cache;
 
-static ft_batch_cache g_batch = {PTHREAD_MUTEX_INITIALIZER, NULL, 0, 0, 0, 0, 0, 0, 0, 0};
+static ft_batch_cache g_batch = {PTHREAD_MUTEX_INITIALIZER, NULL, 0, 0, 0, 0, 0, 0, 0, NULL, NULL, NULL, NULL, NULL};
 
-static unsigned long long batch_key(const ft_rotation_plan *const *plans, void *const *dA, const int *M, int count) {
-    unsigned long long h = 1469598103934665603ULL;
-    for (int i = 0; i < count; i++) {
-        unsigned long long v[3] = {(unsigned long long)(size_t)plans[i], (unsigned long long)(size_t)dA[i],
-                                   (unsigned long long)(unsigned)M[i]};
-        for (int k = 0; k < 3; k++) {
-            h ^= v[k];
-            h *= 1099511628211ULL;
-        }
+static void batch_cache_clear_locked(void) {
+    if (g_batch.sched) {
+        ftamd_hip_sync(NULL);
+        ftamd_schedule_release(g_batch.sched);
     }
-    return h;
+    g_batch.sched = NULL;
+    free(g_batch.plans);
+    free(g_batch.serial);
+    free(g_batch.n);
+    free(g_batch.dA);
+    free(g_batch.M);
+    g_batch.plans = NULL;
+    g_batch.serial = NULL;
+    g_batch.n = NULL;
+    g_batch.dA = NULL;
+    g_batch.M = NULL;
+    g_batch.count = 0;
 }
 
+void ftamd_batch_forget_plan(const ft_rotation_plan *RP) {
+    pthread_mutex_lock(&g_batch.lock);
+    for (int i = 0; g_batch.sched && i < g_batch.count; i++)
+        if (g_batch.plans[i] == RP && g_batch.serial[i] == RP->serial) {
+            batch_cache_clear_locked();
+            break;
+        }
+    pthread_mutex_unlock(&g_batch.lock);
+}
+
+static int batch_cache_hit(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+                           void *const *dA, const int *M, int count, int L, int W, int engine) {
+    if (!g_batch.sched || g_batch.count != count || g_batch.kind != kind || g_batch.dir != direction ||
+        g_batch.precision != precision || g_batch.L != L || g_batch.W != W || g_batch.engine != engine)
+        return 0;
+    for (int i = 0; i < count; i++)
+        if (g_batch.plans[i] != plans[i] || g_batch.serial[i] != plans[i]->serial || g_batch.n[i] != plans[i]->n ||
+            g_batch.dA[i] != dA[i] || g_batch.M[i] != M[i])
+            return 0;
+    return 1;
+}
+
+static int batch_cache_store(ft_schedule *S, const ft_rotation_plan *const *plans, int kind, int direction,
+                             int precision, void *const *dA, const int *M, int count, int L, int W, int engine) {
+    size_t c = (size_t)count;
+    g_batch.plans = malloc(sizeof(*g_batch.plans) * c);
+    g_batch.serial = malloc(sizeof(*g_batch.serial) * c);
+    g_batch.n = malloc(sizeof(int) * c);
+    g_batch.dA = malloc(sizeof(void *) * c);
+    g_batch.M = malloc(sizeof(int) * c);
+    if (!g_batch.plans || !g_batch.serial || !g_batch.n || !g_batch.dA || !g_batch.M) {
+        g_batch.sched = S;
+        batch_cache_clear_locked();
+        return FTAMD_ENOMEM;
+    }
+    for (int i = 0; i < count; i++) {
+        g_batch.plans[i] = plans[i];
+        g_batch.serial[i] = plans[i]->serial;
+        g_batch.n[i] = plans[i]->n;
+        g_batch.dA[i] = dA[i];
+        g_batch.M[i] = M[i];
+    }
+    g_batch.sched = S;
+    g_batch.count = count;
+    g_batch.kind = kind;
+    g_batch.dir = direction;
+    g_batch.precision = precision;
+    g_batch.L = L;
+    g_batch.W = W;
+    g_batch.engine = engine;
+    return FTAMD_OK;
+}
 /* Validates a batch and uploads the largest plan's table; *big gets that plan. */
 static int batch_check(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                        void *const *dA, const int *M, int count, ft_rotation_plan **big) {
@@ -861,7 +1072,7 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
  * 20.2 ms against 10.4 ms for 512 sph transforms of degree 32..1018). */
 static int batch_build(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                        void *const *dA, const int *M, int count, ft_rotation_plan *big, ft_schedule **out) {
-    if (g_engine == FT_GPU_ENGINE_SYSTOLIC) {
+    if (KNOB_GET(g_engine) == FT_GPU_ENGINE_SYSTOLIC) {
         int rs = batch_build_sys(plans, kind, direction, precision, dA, M, count, out);
         if (rs != FTAMD_OK) return rs;
         return *out ? FTAMD_OK : FTAMD_EINVAL;
@@ -908,36 +1119,22 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
     ft_rotation_plan *big = NULL;
     int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
     if (rc != FTAMD_OK || count == 0) return rc;
-    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W();
-    unsigned long long key = batch_key(plans, dA, M, count);
+    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W(), engine = KNOB_GET(g_engine);
     pthread_mutex_lock(&g_batch.lock);
     ft_schedule *S = g_batch.sched;
-    if (!S || g_batch.key != key || g_batch.count != count || g_batch.kind != kind || g_batch.dir != direction ||
-        g_batch.precision != precision || g_batch.L != L || g_batch.W != W || g_batch.engine != g_engine) {
-        if (S) {
-            ftamd_hip_sync(NULL);
-            ftamd_schedule_free_device(S);
-            free(S);
-            g_batch.sched = S = NULL;
-        }
+    if (!batch_cache_hit(plans, kind, direction, precision, dA, M, count, L, W, engine)) {
+        batch_cache_clear_locked();
+        S = NULL;
         rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &S);
+        if (rc == FTAMD_OK) rc = batch_cache_store(S, plans, kind, direction, precision, dA, M, count, L, W, engine);
         if (rc != FTAMD_OK) {
             pthread_mutex_unlock(&g_batch.lock);
             return rc;
         }
-        g_batch.sched = S;
-        g_batch.key = key;
-        g_batch.count = count;
-        g_batch.kind = kind;
-        g_batch.dir = direction;
-        g_batch.precision = precision;
-        g_batch.L = L;
-        g_batch.W = W;
-        g_batch.engine = g_engine;
     }
-    if (S->sys) t_last_engine = FT_GPU_ENGINE_SYSTOLIC;
-    else t_last_engine = FT_GPU_ENGINE_PIPELINE;
+    t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
     rc = launch(S, precision, kind, direction, NULL, stream);
+    if (rc == FTAMD_OK) pending_add(S, stream);
     pthread_mutex_unlock(&g_batch.lock);
     return rc;
 }
@@ -974,13 +1171,14 @@ ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, 
 
 int ft_gpu_execute_batch_plan(const ft_gpu_batch *B, void *stream) {
     if (!B) return FT_GPU_EINVAL;
-    return launch(B->S, B->precision, B->kind, B->dir, NULL, stream);
+    int rc = launch(B->S, B->precision, B->kind, B->dir, NULL, stream);
+    if (rc == FTAMD_OK) pending_add(B->S, stream);
+    return rc;
 }
 
 void ft_gpu_destroy_batch(ft_gpu_batch *B) {
     if (!B) return;
     ftamd_hip_sync(NULL);
-    ftamd_schedule_free_device(B->S);
-    free(B->S);
+    ftamd_schedule_release(B->S);
     free(B);
 }

@@ -13,8 +13,11 @@
  * device, plus the kernel's ticket/progress workspace. */
 typedef struct ft_schedule {
     struct ft_schedule *next;
-    /* key */
+    /* key (single-vector schedules: m1, m2 = first and last sequence j, rs = stride) */
     int M, dir, precision, L, W, rs, m1, m2, single;
+    /* launched asynchronously since its error word was last read (ft_gpu_synchronize) */
+    int pending;
+    void *pending_stream;
     /* device arrays (one allocation), and the boundary buffers rows pass
      * through from one task of a group to the next */
     void *d_mem;
@@ -43,9 +46,11 @@ struct ft_rotation_plan_s {
     void *d_tab[2];          /* interleaved (s,c) with padding: [0] FP64, [1] FP32 */
     ft_schedule *scheds;
     struct ft_sys_tables *sys[2][2];  /* band-systolic tables [direction][precision] */
-    void *d_stage;           /* staging buffer for host-pointer calls */
+    void *d_stage;           /* staging buffer of host-pointer calls (used under exec_lock) */
     size_t stage_bytes;
-    pthread_mutex_t lock;
+    unsigned long long serial; /* distinct for every plan the process creates */
+    pthread_mutex_t lock;      /* device state: uploads, schedule lists */
+    pthread_mutex_t exec_lock; /* held for the whole of a synchronous executor or kernel call */
 };
 
 typedef struct ft_sys_tables ft_sys_tables;
@@ -61,6 +66,8 @@ int ftamd_tuning_L(int kind, int n);
 int ftamd_tuning_W(void);
 int ftamd_tuning_grid(const ft_schedule *S);
 void ftamd_schedule_free_device(ft_schedule *S);
+void ftamd_schedule_release(ft_schedule *S);
+void ftamd_batch_forget_plan(const ft_rotation_plan *RP);
 int ftamd_engine(void);
 
 /* ft_systolic.c */

@@ -41,7 +41,10 @@ ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, d
     RP->s = malloc((size_t)RP->nsc * sizeof(double));
     RP->c = malloc((size_t)RP->nsc * sizeof(double));
     RP->device = -1;
+    static unsigned long long next_serial = 0;
+    RP->serial = __atomic_add_fetch(&next_serial, 1, __ATOMIC_RELAXED);
     pthread_mutex_init(&RP->lock, NULL);
+    pthread_mutex_init(&RP->exec_lock, NULL);
     if (!RP->s || !RP->c) {
         free(RP->s);
         free(RP->c);
@@ -100,11 +103,13 @@ ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double 
 /* rotations.c:6-10, plus the device state. */
 void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
     if (!RP) return;
+    /* a cached batch schedule that uses this plan's tables goes with it */
+    ftamd_batch_forget_plan(RP);
+    if (RP->device >= 0) ftamd_hip_sync(NULL);
     ft_schedule *sc = RP->scheds;
     while (sc) {
         ft_schedule *next = sc->next;
-        ftamd_schedule_free_device(sc);
-        free(sc);
+        ftamd_schedule_release(sc);
         sc = next;
     }
     for (int d = 0; d < 2; d++)
@@ -113,6 +118,7 @@ void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
     ftamd_hip_free(RP->d_tab[1]);
     ftamd_hip_free(RP->d_stage);
     pthread_mutex_destroy(&RP->lock);
+    pthread_mutex_destroy(&RP->exec_lock);
     free(RP->s);
     free(RP->c);
     free(RP);

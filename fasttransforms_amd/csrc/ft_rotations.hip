@@ -8,31 +8,34 @@
 // lo2hi in increasing j with k ascending and the transposed rotation.
 // sph: D = 2, j steps by 2; tri: D = 1, j steps by 1; disk: D = 1, j steps by 2.
 //
-// How it maps onto the GPU (one design for every geometry and direction):
+// This file holds the pipelined engine's launcher and the HIP runtime calls
+// of the C ABI; the engines themselves are templates in ft_pipeline.hpp
+// (lanes are columns, waves are pipeline stages of L sequences) and
+// ft_systolic.hpp (lanes are sequences, rows stream through a band of lanes).
+// In the pipelined engine:
 //
 //  * Lanes are columns.  A "group" is up to 64 columns that need the same
 //    sequences (equal order parity for sph/disk, consecutive orders for tri),
-//    so at any moment every lane applies the same rotation (k, j): s and c are
-//    wave-uniform and come from scalar loads into SGPRs, which v_fma_f64 reads
-//    directly.  Columns of lower order skip the leading sequences by exec mask.
+//    so every lane applies the same rotation (k, j) at a time.  A tile's
+//    (s, c) pairs are fetched by ordinary vector loads, one pair per lane, and
+//    broadcast to the wave where used with DPP row_newbcast.  Columns of lower
+//    order skip the leading sequences by per-lane masks.
 //  * Waves are pipeline stages.  Stage s applies sequences q = sL .. sL+L-1 of
-//    the group.  It holds a window of (D+1)L rows per lane in registers and
-//    sweeps it down the column L rows per tile, applying the L sequences
-//    along an anti-diagonal (the reference's L x L register tile,
-//    src/rotations/rotations.h:26-45).  Each tile takes L new rows from the previous
-//    stage and hands L finished rows to the next one through an LDS ring.
-//  * lo2hi is run as hi2lo on reversed rows ("virtual" row v = n-1-real row):
-//    the transposed rotation on the reversed pair is exactly the forward
-//    rotation, so one engine covers both directions.
-//  * A task is W consecutive stages (one workgroup, one pass over the rows).
-//    Consecutive tasks of a group are chained through A itself: the last stage
-//    of a task stores its rows in place and publishes how far down it is; the
-//    next task's reads wait for that (agent-scope release/acquire).
-//  * The kernel is persistent: workgroups take tasks from an atomic ticket in a
-//    host-sorted order (long chains first, every producer before its
-//    consumer), so a waiting task's producer is always already running.
-//    Every wait is bounded; on expiry the kernel records FT_GPU_ETIMEOUT and
-//    every wave still reaches its exit.
+//    the group over a window of rows held in registers, sweeping down the
+//    column L rows per tile along an anti-diagonal (the reference's L x L
+//    register tile, src/rotations/rotations.h:26-45), and hands L finished
+//    rows per tile to the next stage through an LDS ring.
+//  * lo2hi is run as hi2lo on reversed rows: the transposed rotation of the
+//    reversed pair is the forward rotation, so one engine covers both.
+//  * A task is one workgroup's pass over the rows (W-2 stage waves plus an
+//    importer and an exporter wave).  Consecutive tasks of a group hand rows
+//    over through row-major boundary buffers in global memory, accessed with
+//    device-scope relaxed atomics, and publish progress counters; every task
+//    of a group runs on one XCD (one L2).  The kernel is persistent:
+//    workgroups take tasks from per-XCD ticket counters in a host-sorted
+//    order (producers before consumers, long chains first).  Every wait is
+//    bounded; on expiry the kernel records FT_GPU_ETIMEOUT and every wave
+//    still reaches its exit.
 //
 // Arithmetic: x' = fma(c, x, s*y), y' = fma(c, y, -(s*x)) -- the fused form of
 // the reference's dispatched SIMD kernels (src/rotations/rotations_AVX512F.c:4-15).
@@ -50,14 +53,6 @@ thread_local int g_hip_err = 0;
 // Records the XCC_ID of every workgroup (to count the device's XCDs).
 __global__ void xcd_probe(int *out) {
     if (threadIdx.x == 0) out[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));
-}
-
-// Diagnostics: full XCC_ID and HW_ID registers of every workgroup.
-__global__ void hwid_probe(int *xcc, int *hw) {
-    if (threadIdx.x == 0) {
-        xcc[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((32 - 1) << 11));
-        hw[blockIdx.x] = (int)__builtin_amdgcn_s_getreg((4) | (0 << 6) | ((32 - 1) << 11));
-    }
 }
 
 int map_err(hipError_t e) {
@@ -115,19 +110,6 @@ int ftamd_hip_xcd_count(int *nxcd) {
     cached = mx + 1 > FTAMD_MAX_XCD ? FTAMD_MAX_XCD : mx + 1;
     *nxcd = cached;
     return FTAMD_OK;
-}
-
-// Diagnostics: XCC_ID and HW_ID (raw registers) of `blocks` workgroups.
-int ftamd_hip_probe_ids(int *xcc, int *hw, int blocks) {
-    int *d = nullptr;
-    hipError_t e = hipMalloc(&d, 2 * blocks * sizeof(int));
-    if (e != hipSuccess) return map_err(e);
-    hipLaunchKernelGGL(hwid_probe, dim3(blocks), dim3(64), 0, 0, d, d + blocks);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(xcc, d, blocks * sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(hw, d + blocks, blocks * sizeof(int), hipMemcpyDeviceToHost);
-    hipFree(d);
-    return map_err(e);
 }
 
 int ftamd_hip_pointer_kind(const void *ptr, int *kind) {

@@ -93,9 +93,21 @@ enum { FT_GPU_FP64 = 0, FT_GPU_FP32 = 1 };
  * kind: 0 sphere, 1 triangle, 2 disk; direction: 0 hi2lo, 1 lo2hi.
  * dA is n x M column-major in device memory (double* for FP64, float* for
  * FP32).  Asynchronous: returns after the launch; 0 on success or an FT_GPU_*
- * code. */
+ * code.  Launches with the same (plan, M, direction, precision) share a
+ * schedule and its device workspace: order them on one stream (or wait for
+ * one before the next).  The first call for such a combination builds and
+ * uploads the schedule (hipMalloc, synchronous copies), so make one call
+ * outside any HIP-graph capture before capturing. */
 int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
                    void *stream);
+
+/* Waits for `stream` and reads the device error words of every asynchronous
+ * launch (ft_gpu_execute, ft_gpu_execute_batch, ft_gpu_execute_batch_plan)
+ * made on it since the last such check, clearing them: FT_GPU_OK, or
+ * FT_GPU_ETIMEOUT / FT_GPU_EINCOMPLETE if one of those launches ended with
+ * its results invalid (or a HIP error code).  No reference counterpart: the
+ * reference has no asynchronous calls. */
+int ft_gpu_synchronize(void *stream);
 
 /* Batched transforms of uneven degree in one launch.  plans[i] may differ in
  * n (same kind and parameters); dA[i] is plans[i]->n x M[i] on the device.
@@ -109,7 +121,8 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
 /* A batch plan: the schedule of one argument list of ft_gpu_execute_batch,
  * owned by the caller (for example to launch it inside a captured HIP graph,
  * which stays valid until ft_gpu_destroy_batch).  NULL on error
- * (ft_gpu_last_error tells which). */
+ * (ft_gpu_last_error tells which).  It uses the plans' device tables: destroy
+ * it before any of its plans. */
 typedef struct ft_gpu_batch_s ft_gpu_batch;
 ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                                 void *const *dA, const int *M, int count);

@@ -50,6 +50,14 @@ def _load():
         lib.oracle_execute_q.restype = ctypes.c_int
         lib.oracle_execute_q.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        for name in ("sph", "tri", "disk"):
+            for d in ("hi2lo", "lo2hi"):
+                k = getattr(lib, "oracle_kernel_%s_%s_f32" % (name, d))
+                k.argtypes = [_fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int]
+        lib.oracle_execute_f32.restype = ctypes.c_int
+        lib.oracle_execute_f32.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         lib.oracle_rand_entry.restype = ctypes.c_double
         lib.oracle_rand_entry.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         lib.oracle_valid_rows.restype = ctypes.c_int
@@ -114,6 +122,35 @@ def kernel(plan, direction, m1, m2, A, S=1):
         raise ValueError("expected a contiguous float64 buffer")
     name = "oracle_kernel_%s_%s" % (("sph", "tri", "disk")[plan.kind], ("hi2lo", "lo2hi")[direction])
     getattr(lib, name)(_ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2), _ptr(A), int(S))
+    return A
+
+
+def execute_f32(plan, direction, A, M=None):
+    """The FP32 variant (float storage, s and c rounded once from the FP64
+    plan, fused float rotations), in place on a Fortran-ordered float32 array."""
+    lib = _load()
+    if A.dtype != np.float32 or not A.flags.f_contiguous or A.ndim != 2 or A.shape[0] != plan.n:
+        raise ValueError("expected a Fortran-ordered float32 array with n rows")
+    M = A.shape[1] if M is None else M
+    rc = lib.oracle_execute_f32(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n,
+                                A.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(M))
+    if rc != 0:
+        raise MemoryError("oracle_execute_f32 failed")
+    return A
+
+
+def kernel_f32(plan, direction, m1, m2, A, S=1):
+    """The FP32 single-vector loop nest on a contiguous float32 buffer (in
+    place), with s and c rounded once from the FP64 plan."""
+    lib = _load()
+    if A.dtype != np.float32 or not A.flags.c_contiguous:
+        raise ValueError("expected a contiguous float32 buffer")
+    sf = plan.s.astype(np.float32)
+    cf = plan.c.astype(np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    name = "oracle_kernel_%s_%s_f32" % (("sph", "tri", "disk")[plan.kind], ("hi2lo", "lo2hi")[direction])
+    getattr(lib, name)(sf.ctypes.data_as(fp), cf.ctypes.data_as(fp), plan.n, int(m1), int(m2),
+                       A.ctypes.data_as(fp), int(S))
     return A
 
 

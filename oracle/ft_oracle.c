@@ -20,6 +20,9 @@
  *                orders 0-1 (sphere, disk) and 0 (triangle) are untouched.
  *   fills        src/ftutilities.c:67-113 (valid triangle of each layout), but
  *                seeded with a counter-based SplitMix64 instead of libc rand().
+ *   FP32         the same loop nests in float (s, c rounded once from the FP64
+ *                plan, fmaf): the device path's FP32 variant, which has no
+ *                reference counterpart.
  *   __float128   the same loop nests evaluated in quadruple precision, with the
  *                sines/cosines either promoted from the FP64 plan or recomputed
  *                from the closed forms in quadruple precision.
@@ -124,83 +127,115 @@ static inline void givens_t(double S, double C, double *X, double *Y) {
     }
 }
 
-/* rotations_default.c:31-36 */
-void oracle_kernel_sph_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m2 - 2; j >= m1; j -= 2)
-        for (int k = n - 3 - j; k >= 0; k--)
-            givens(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);
+/* The FP32 variant of the device path (no reference counterpart: the
+ * reference computes in FP64 only): float storage, s and c rounded once from
+ * the FP64 plan, and the same fused rotation in float. */
+static inline void givens_f32(float S, float C, float *X, float *Y) {
+    float x = X[0], y = Y[0];
+    X[0] = fmaf(C, x, S * y);
+    Y[0] = fmaf(C, y, -(S * x));
 }
 
-/* rotations_default.c:38-43 */
-void oracle_kernel_sph_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m1; j < m2; j += 2)
-        for (int k = 0; k <= n - 3 - j; k++)
-            givens_t(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);
+static inline void givens_t_f32(float S, float C, float *X, float *Y) {
+    float x = X[0], y = Y[0];
+    X[0] = fmaf(C, x, -(S * y));
+    Y[0] = fmaf(C, y, S * x);
 }
 
-/* rotations_default.c:45-50 */
-void oracle_kernel_tri_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m2 - 1; j >= m1; j--)
-        for (int k = n - 2 - j; k >= 0; k--)
-            givens(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
-}
-
-/* rotations_default.c:52-57 */
-void oracle_kernel_tri_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m1; j < m2; j++)
-        for (int k = 0; k <= n - 2 - j; k++)
-            givens_t(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
-}
-
-/* rotations_default.c:59-64 */
-void oracle_kernel_disk_hi2lo(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m2 - 2; j >= m1; j -= 2)
-        for (int k = n - 2 - (j + 1) / 2; k >= 0; k--)
-            givens(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
-}
-
-/* rotations_default.c:66-71 */
-void oracle_kernel_disk_lo2hi(const double *s, const double *c, int n, int m1, int m2, double *A, int S) {
-    for (int j = m1; j < m2; j += 2)
-        for (int k = 0; k <= n - 2 - (j + 1) / 2; k++)
-            givens_t(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);
-}
-
-/* ------------------------------------------------------------ executors -- */
-/* Columns are independent, so the OpenMP loop over orders changes no bits. */
-
-/* drivers.c:94-110: orders 2..M/2, columns 2m-1 and 2m, down to order m%2.
- * Written as a loop over the in-range columns j = 2m-1+t (order (j+1)/2): for
- * the odd M the layout defines this is the same set of columns. */
-void oracle_execute_sph(int dir, const double *s, const double *c, int n, double *A, int M) {
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (int j = M - 1; j >= 3; j--) {
-        int m = (j + 1) / 2;
-        double *col = A + (long)n * j;
-        if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
-        else          oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+/*
+ * The loop nests and executors, written once and instantiated for FP64 (the
+ * restatement of the reference) and for FP32 (suffix _f32).
+ *
+ * Kernels: rotations_default.c:31-36 (sph hi2lo), :38-43 (sph lo2hi),
+ * :45-50 (tri hi2lo), :52-57 (tri lo2hi), :59-64 (disk hi2lo), :66-71
+ * (disk lo2hi).
+ *
+ * Executors (columns are independent, so the OpenMP loop changes no bits):
+ * drivers.c:94-110, sph: orders 2..M/2, columns 2m-1 and 2m, down to order
+ * m%2 -- written as a loop over the in-range columns j = 2m-1+t (order
+ * (j+1)/2), which for the odd M the layout defines is the same set of
+ * columns; drivers.c:314-326, tri: columns 1..M-1, column m from order m
+ * down to order 0; drivers.c:414-430, disk: as the sphere, with the disk
+ * kernels.
+ */
+#define ORACLE_LOOPS(SUF, T, ROT, ROT_T)                                                                          \
+    void oracle_kernel_sph_hi2lo##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {              \
+        for (int j = m2 - 2; j >= m1; j -= 2)                                                                     \
+            for (int k = n - 3 - j; k >= 0; k--)                                                                  \
+                ROT(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);            \
+    }                                                                                                             \
+    void oracle_kernel_sph_lo2hi##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {              \
+        for (int j = m1; j < m2; j += 2)                                                                          \
+            for (int k = 0; k <= n - 3 - j; k++)                                                                  \
+                ROT_T(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 2) * S);          \
+    }                                                                                                             \
+    void oracle_kernel_tri_hi2lo##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {              \
+        for (int j = m2 - 1; j >= m1; j--)                                                                        \
+            for (int k = n - 2 - j; k >= 0; k--)                                                                  \
+                ROT(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);            \
+    }                                                                                                             \
+    void oracle_kernel_tri_lo2hi##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {              \
+        for (int j = m1; j < m2; j++)                                                                             \
+            for (int k = 0; k <= n - 2 - j; k++)                                                                  \
+                ROT_T(s[idx_tri(n, k, j)], c[idx_tri(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);          \
+    }                                                                                                             \
+    void oracle_kernel_disk_hi2lo##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {             \
+        for (int j = m2 - 2; j >= m1; j -= 2)                                                                     \
+            for (int k = n - 2 - (j + 1) / 2; k >= 0; k--)                                                        \
+                ROT(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);          \
+    }                                                                                                             \
+    void oracle_kernel_disk_lo2hi##SUF(const T *s, const T *c, int n, int m1, int m2, T *A, int S) {             \
+        for (int j = m1; j < m2; j += 2)                                                                          \
+            for (int k = 0; k <= n - 2 - (j + 1) / 2; k++)                                                        \
+                ROT_T(s[idx_disk(n, k, j)], c[idx_disk(n, k, j)], A + (long)k * S, A + (long)(k + 1) * S);        \
+    }                                                                                                             \
+    void oracle_execute_sph##SUF(int dir, const T *s, const T *c, int n, T *A, int M) {                         \
+        _Pragma("omp parallel for schedule(dynamic, 1)") for (int j = M - 1; j >= 3; j--) {                       \
+            int m = (j + 1) / 2;                                                                                  \
+            T *col = A + (long)n * j;                                                                             \
+            if (dir == 0) oracle_kernel_sph_hi2lo##SUF(s, c, n, m % 2, m, col, 1);                                \
+            else oracle_kernel_sph_lo2hi##SUF(s, c, n, m % 2, m, col, 1);                                         \
+        }                                                                                                         \
+    }                                                                                                             \
+    void oracle_execute_tri##SUF(int dir, const T *s, const T *c, int n, T *A, int M) {                         \
+        _Pragma("omp parallel for schedule(dynamic, 1)") for (int m = M - 1; m >= 1; m--) {                       \
+            T *col = A + (long)n * m;                                                                             \
+            if (dir == 0) oracle_kernel_tri_hi2lo##SUF(s, c, n, 0, m, col, 1);                                    \
+            else oracle_kernel_tri_lo2hi##SUF(s, c, n, 0, m, col, 1);                                             \
+        }                                                                                                         \
+    }                                                                                                             \
+    void oracle_execute_disk##SUF(int dir, const T *s, const T *c, int n, T *A, int M) {                        \
+        _Pragma("omp parallel for schedule(dynamic, 1)") for (int j = M - 1; j >= 3; j--) {                       \
+            int m = (j + 1) / 2;                                                                                  \
+            T *col = A + (long)n * j;                                                                             \
+            if (dir == 0) oracle_kernel_disk_hi2lo##SUF(s, c, n, m % 2, m, col, 1);                               \
+            else oracle_kernel_disk_lo2hi##SUF(s, c, n, m % 2, m, col, 1);                                        \
+        }                                                                                                         \
     }
-}
 
-/* drivers.c:314-326: columns 1..M-1, column m from order m down to order 0. */
-void oracle_execute_tri(int dir, const double *s, const double *c, int n, double *A, int M) {
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (int m = M - 1; m >= 1; m--) {
-        double *col = A + (long)n * m;
-        if (dir == 0) oracle_kernel_tri_hi2lo(s, c, n, 0, m, col, 1);
-        else          oracle_kernel_tri_lo2hi(s, c, n, 0, m, col, 1);
-    }
-}
+ORACLE_LOOPS(, double, givens, givens_t)
+ORACLE_LOOPS(_f32, float, givens_f32, givens_t_f32)
 
-/* drivers.c:414-430: as the sphere, with the disk kernels. */
-void oracle_execute_disk(int dir, const double *s, const double *c, int n, double *A, int M) {
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (int j = M - 1; j >= 3; j--) {
-        int m = (j + 1) / 2;
-        double *col = A + (long)n * j;
-        if (dir == 0) oracle_kernel_disk_hi2lo(s, c, n, m % 2, m, col, 1);
-        else          oracle_kernel_disk_lo2hi(s, c, n, m % 2, m, col, 1);
+/* FP32 executor on an FP64 plan: s and c rounded to float once, then the
+ * FP32 loop nest (kind 0 sph, 1 tri, 2 disk).  0, or -1 if out of memory. */
+int oracle_execute_f32(int kind, int dir, const double *s, const double *c, int n, float *A, int M) {
+    long ns = oracle_plan_size(kind, n);
+    float *sf = malloc((size_t)ns * sizeof(float)), *cf = malloc((size_t)ns * sizeof(float));
+    if (!sf || !cf) {
+        free(sf);
+        free(cf);
+        return -1;
     }
+    for (long i = 0; i < ns; i++) {
+        sf[i] = (float)s[i];
+        cf[i] = (float)c[i];
+    }
+    if (kind == OR_SPH) oracle_execute_sph_f32(dir, sf, cf, n, A, M);
+    else if (kind == OR_TRI) oracle_execute_tri_f32(dir, sf, cf, n, A, M);
+    else oracle_execute_disk_f32(dir, sf, cf, n, A, M);
+    free(sf);
+    free(cf);
+    return 0;
 }
 
 /* --------------------------------------------------------- __float128 -- */

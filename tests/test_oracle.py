@@ -262,3 +262,104 @@ def test_fill_stream_matches_numpy_restatement(ft):
     for kind, n in (("sph", 9), ("tri", 11), ("disk", 6)):
         M = full_M(kind, n)
         assert np.array_equal(O.fill(kind, n, M, 1234), utilities.fill(kind, n, M, 1234))
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,params,n", [("sph", (), 1024), ("sph", (), 2048), ("tri", (0.0, 0.0, 0.0), 1024),
+                                           ("tri", (-0.5, -0.5, -0.5), 1024), ("disk", (0.0, 0.0), 1024)])
+def test_fused_oracle_vs_reference_dispatch_at_scale(kind, params, n):
+    """The north-star sizes: the fused oracle (which the GPU matches bit for
+    bit) against the reference's dispatched executor, both directions, within
+    the 1e-14 relative 2-norm bar (measured here ~1e-15)."""
+    M = full_M(kind, n)
+    P = O.Plan(kind, n, *params)
+    A = O.fill(kind, n, M, 1000 + n)
+    for d in (O.HI2LO, O.LO2HI):
+        X = O.execute(P, d, A.copy(order="F"))
+        R = O.ref_execute(P, d, A.copy(order="F"))
+        err = O.rel2(X, R)
+        print("%s%s n=%d dir=%d rel2 vs reference dispatch %.3e" % (kind, params, n, d, err))
+        assert err <= 1e-14
+
+
+# ------------------------------------------------------------------ FP32 ----
+
+def _f32_exact(q):
+    """Correct rounding (to nearest, ties to even) of a Fraction to binary32,
+    returned as a Fraction (normal range only: every value here is)."""
+    if q == 0:
+        return Fraction(0)
+    sign = -1 if q < 0 else 1
+    a = abs(q)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** e > a:
+        e -= 1
+    elif Fraction(2) ** (e + 1) <= a:
+        e += 1
+    scale = Fraction(2) ** (23 - e)
+    m = a * scale
+    mi = m.numerator // m.denominator
+    r = m - mi
+    if r > Fraction(1, 2) or (r == Fraction(1, 2) and mi % 2 == 1):
+        mi += 1
+    return sign * Fraction(mi) / scale
+
+
+@pytest.mark.parametrize("kind,params", [("sph", ()), ("tri", (0.0, 0.0, 0.0)), ("disk", (0.0, 0.0))])
+def test_fp32_rotation_arithmetic_exact(kind, params):
+    """One FP32 rotation of the oracle's float loop nest against exact
+    rational arithmetic with correct binary32 rounding: s and c rounded once
+    from the FP64 plan, the product s*y rounded, then c*x + (s*y) rounded once
+    (the fused form), and the transposed rotation for lo2hi.  An unfused
+    evaluation, a product kept in double, or unrounded s/c would differ in
+    some of these cases."""
+    n = 3 if kind == "sph" else 2
+    m2 = 2 if kind != "tri" else 1
+    P = O.Plan(kind, 6, *params)  # a table with several distinct entries
+    rng = np.random.default_rng(5)
+    lo_row, hi_row = 0, (2 if kind == "sph" else 1)
+    mism = 0
+    for trial in range(400):
+        # take a random table entry by building a plan view whose only rotation is (0, j)
+        x0, y0 = (np.float32(v) for v in rng.uniform(-1, 1, 2))
+        Q = O.Plan(kind, n, *params)
+        k = int(rng.integers(0, len(P.s)))
+        Q.s[:] = P.s[k]
+        Q.c[:] = P.c[k]
+        S, C = _f32_exact(Fraction(float(P.s[k]))), _f32_exact(Fraction(float(P.c[k])))
+        x, y = Fraction(float(x0)), Fraction(float(y0))
+        for d in (O.HI2LO, O.LO2HI):
+            buf = np.zeros(n, dtype=np.float32)
+            buf[lo_row], buf[hi_row] = x0, y0
+            O.kernel_f32(Q, d, 0, m2, buf)
+            if d == O.HI2LO:
+                ex = _f32_exact(C * x + _f32_exact(S * y))
+                ey = _f32_exact(C * y - _f32_exact(S * x))
+            else:
+                ex = _f32_exact(C * x - _f32_exact(S * y))
+                ey = _f32_exact(C * y + _f32_exact(S * x))
+            got = (Fraction(float(buf[lo_row])), Fraction(float(buf[hi_row])))
+            mism += got != (ex, ey)
+    assert mism == 0
+
+
+@pytest.mark.parametrize("kind,params", PLANS)
+def test_fp32_oracle_close_to_fp64(kind, params):
+    """The FP32 loop nests are the FP64 oracle's (one macro text, two
+    instantiations): the FP32 executor stays within an FP32 rounding bound of
+    the FP64 one in both directions, really computes in float (it is not the
+    FP64 result rounded), and its round trip returns the input within the
+    FP32 analogue of the reference's bound."""
+    for n in (33, 200):
+        M = full_M(kind, n)
+        P = O.Plan(kind, n, *params)
+        A = O.fill(kind, n, M, 61)
+        for d in (O.HI2LO, O.LO2HI):
+            R = O.execute(P, d, A.copy(order="F"))
+            F = O.execute_f32(P, d, A.astype(np.float32, order="F"))
+            err = O.rel2(F.astype(np.float64), R)
+            assert 2.0 ** -26 < err < 8 * np.sqrt(n) * 2.0 ** -23, (n, d, err)
+            assert not np.array_equal(F, R.astype(np.float32))
+        F = O.execute_f32(P, O.HI2LO, A.astype(np.float32, order="F"))
+        F = O.execute_f32(P, O.LO2HI, F)
+        assert O.rel2(F.astype(np.float64), A.astype(np.float32).astype(np.float64)) < 4 * np.sqrt(n) * 2.0 ** -23
