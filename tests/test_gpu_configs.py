@@ -228,8 +228,8 @@ def test_batch_cache_survives_plan_recreation(ft):
     for rnd, ns in enumerate(([60, 90, 40], [61, 90, 41], [60, 90, 40])):
         plans = [ft.plan_rotsphere(n) for n in ns]
         M = 79  # valid for every n here, so only the plans tell the rounds apart
-        As = [O.fill("sph", n, 2 * n - 1, 40 + n) for n in ns]
-        T = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
+        As = [O.fill("sph", n, 2 * n - 1, 40 + n)[:, :M].copy(order="F") for n in ns]
+        T = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]  # (M, n): row j = column j
         _say("batch cache round", rnd, ns)
         ft.gpu_execute_batch(plans, 0, T, Ms=[M] * len(ns))
         ft.synchronize()
