@@ -19,6 +19,9 @@ import numpy as np
 SPH, TRI, DISK = 0, 1, 2
 HI2LO, LO2HI = 0, 1
 FP64, FP32 = 0, 1
+# error codes of the C ABI (FT_GPU_* in include/fasttransforms_amd.h)
+(FT_GPU_OK, FT_GPU_EINVAL, FT_GPU_ENOMEM, FT_GPU_EDEVICE, FT_GPU_ETIMEOUT, FT_GPU_EPTR,
+ FT_GPU_EINCOMPLETE) = range(7)
 _KIND = {"sph": SPH, "tri": TRI, "disk": DISK}
 _DIR = {"hi2lo": HI2LO, "lo2hi": LO2HI}
 
@@ -52,6 +55,7 @@ SIGNATURES = {
     "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
     "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
     "ft_gpu_synchronize": (_i, [_vp]),
+    "ft_gpu_execute_noalloc": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
     "ft_gpu_execute_batch": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp]),
     "ft_gpu_plan_batch": (_vp, [_vp, _i, _i, _i, _vp, _vp, _i]),
     "ft_gpu_execute_batch_plan": (_i, [_vp, _vp]),

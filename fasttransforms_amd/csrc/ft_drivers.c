@@ -420,21 +420,31 @@ void ftamd_schedule_release(ft_schedule *S) {
     free(S);
 }
 
-/* Reads and clears one schedule's error word (stream-ordered). */
-static int read_error_word(ft_schedule *S, void *stream) {
+/* Reads and clears a device error word (stream-ordered). */
+static int read_error_at(int *err_dev, void *stream) {
     int err = 0;
-    int rc = ftamd_hip_d2h(&err, S->launch.err, sizeof(int), stream);
+    int rc = ftamd_hip_d2h(&err, err_dev, sizeof(int), stream);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(stream);
     if (rc != FTAMD_OK) return rc;
     if (err) {
         static const int zero = 0;
-        rc = ftamd_hip_h2d(S->launch.err, &zero, sizeof(int), stream);
+        rc = ftamd_hip_h2d(err_dev, &zero, sizeof(int), stream);
         if (rc == FTAMD_OK) rc = ftamd_hip_sync(stream);
         if (rc != FTAMD_OK) return rc;
         return (err & FTAMD_ERR_TIMEOUT) ? FTAMD_ETIMEOUT : FTAMD_EINCOMPLETE;
     }
     return FTAMD_OK;
 }
+
+static int read_error_word(ft_schedule *S, void *stream) { return read_error_at(S->launch.err, stream); }
+
+/* Launches of the allocation-free entry point not yet checked, per device
+ * (their error word is the device scratch's first word). */
+static struct {
+    int *err;
+    int pending;
+    void *stream;
+} g_na_pend[FTAMD_MAX_DEVICES];
 
 int ft_gpu_synchronize(void *stream) {
     int rc = ftamd_hip_sync(stream);
@@ -449,6 +459,13 @@ int ft_gpu_synchronize(void *stream) {
         int r = read_error_word(S, stream);
         if (rc == FTAMD_OK) rc = r;
         pending_remove_locked(S); /* swaps the last entry into slot i */
+    }
+    int dev = -1;
+    if (ftamd_hip_device(&dev, NULL) == FTAMD_OK && dev >= 0 && dev < FTAMD_MAX_DEVICES && g_na_pend[dev].pending &&
+        g_na_pend[dev].stream == stream) {
+        int r = read_error_at(g_na_pend[dev].err, stream);
+        if (rc == FTAMD_OK) rc = r;
+        g_na_pend[dev].pending = 0;
     }
     pthread_mutex_unlock(&g_pend_lock);
     return rc;
@@ -662,6 +679,75 @@ int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int pre
     if (rc != FTAMD_OK) return rc;
     rc = launch(S, precision, RP->kind, direction, dA, stream);
     if (rc == FTAMD_OK) pending_add(S, stream);
+    return rc;
+}
+
+/*
+ * The allocation-free variant of ft_gpu_execute (no reference counterpart).
+ * It runs the band-systolic engine with its work list and error word in the
+ * library's fixed device scratch array (FTAMD_NOALLOC_SCRATCH_BYTES per
+ * device, ft_noalloc.hip) instead of a schedule the library allocates: the
+ * chain records are built on the host into a fixed host buffer and copied
+ * into the scratch on `stream`, then the kernel runs.  It calls no hipMalloc
+ * or hipFree.  The plan's band tables for this direction and precision must
+ * already be resident (any earlier executor call on the plan with the same
+ * direction and precision leaves them so); shapes the engine does not run, or
+ * whose chain records exceed the scratch, are rejected with FT_GPU_EINVAL.
+ * Every call on one device shares its scratch: calls that overlap on a device
+ * (other streams, other threads) race.
+ */
+static pthread_mutex_t g_na_lock = PTHREAD_MUTEX_INITIALIZER;
+static FtamdSysChain g_na_chains[(FTAMD_NOALLOC_SCRATCH_BYTES - 256) / sizeof(FtamdSysChain)];
+
+int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
+                           void *stream) {
+    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
+    if (!RP || !dA || RP->kind != kind || (direction != 0 && direction != 1) || (precision != 0 && precision != 1) ||
+        !valid_M(kind, RP->n, M))
+        return FT_GPU_EINVAL;
+    if (M == 0) return FT_GPU_OK;
+    if ((long long)RP->n * M > 0x7fffffffLL) return FT_GPU_EINVAL;
+    int dev = -1, rc = ftamd_hip_device(&dev, NULL);
+    if (rc != FTAMD_OK) return rc;
+    if (dev < 0 || dev >= FTAMD_MAX_DEVICES) return FT_GPU_EDEVICE;
+    pthread_mutex_lock(&RP->lock);
+    const ft_sys_tables *T = RP->device == dev ? RP->sys[direction][precision] : NULL;
+    pthread_mutex_unlock(&RP->lock);
+    if (!T) return FT_GPU_EINVAL; /* tables not resident on this device */
+    void *scr = NULL;
+    size_t scr_bytes = 0;
+    rc = ftamd_hip_scratch(&scr, &scr_bytes);
+    if (rc != FTAMD_OK) return rc;
+    pthread_mutex_lock(&g_na_lock);
+    int cap = (int)((scr_bytes - 256) / sizeof(FtamdSysChain)), nch = 0;
+    if (cap > (int)(sizeof g_na_chains / sizeof g_na_chains[0])) cap = (int)(sizeof g_na_chains / sizeof g_na_chains[0]);
+    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, cap, &nch);
+    if (rc != FTAMD_OK) rc = FTAMD_EINVAL; /* a chain or the chain list does not fit */
+    if (rc == FTAMD_OK && nch > 0)
+        rc = ftamd_hip_h2d((char *)scr + 256, g_na_chains, sizeof(FtamdSysChain) * (size_t)nch, stream);
+    if (rc == FTAMD_OK && nch > 0) {
+        FtamdSysLaunch b;
+        memset(&b, 0, sizeof(b));
+        b.bands = ftamd_sys_tables_dev_bands(T);
+        b.chains = (const FtamdSysChain *)((char *)scr + 256);
+        b.nchains = nch;
+        b.rowmask = kind != FTAMD_TRI;
+        b.tab = ftamd_sys_tables_dev_tab(T);
+        b.A = dA;
+        b.err = (int *)scr;
+        int spin = KNOB_GET(g_spin_limit);
+        b.spin_limit = spin ? spin : 1 << 22;
+        rc = ftamd_hip_sys_launch(&b, precision, ftamd_sys_tables_S(T), stream);
+    }
+    pthread_mutex_unlock(&g_na_lock);
+    if (rc == FTAMD_OK) {
+        t_last_engine = FT_GPU_ENGINE_SYSTOLIC;
+        pthread_mutex_lock(&g_pend_lock);
+        g_na_pend[dev].err = (int *)scr;
+        g_na_pend[dev].pending = 1;
+        g_na_pend[dev].stream = stream;
+        pthread_mutex_unlock(&g_pend_lock);
+    }
     return rc;
 }
 

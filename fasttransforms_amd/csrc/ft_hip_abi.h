@@ -120,6 +120,13 @@ typedef struct {
 } FtamdSysLaunch;
 
 int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int S, void *stream);
+
+/* The allocation-free entry point's device scratch: one fixed array per
+ * device, FTAMD_NOALLOC_SCRATCH_BYTES long, whose first 256 bytes are the
+ * error word's area and the rest the chain records of the launch. */
+#define FTAMD_NOALLOC_SCRATCH_BYTES (256 * 1024)
+#define FTAMD_MAX_DEVICES 64
+int ftamd_hip_scratch(void **ptr, size_t *bytes);
 int ftamd_hip_sys_supported(int S, int precision);
 
 int ftamd_hip_device(int *device, int *num_cu);

@@ -78,6 +78,8 @@ int ftamd_sys_tables_S(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
 int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count);
+int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
+                          int *count);
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
                     const int *M, int count, FtamdSysBand **bands, int *nbands, FtamdSysChain **chains,
                     int *nchains, const void **tab_base);

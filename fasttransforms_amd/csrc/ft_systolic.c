@@ -222,16 +222,34 @@ static int cmp_nb_desc(const void *x, const void *y) {
     return a->band0 - b->band0;
 }
 
+/* Longest chains first, in place and without allocating (Shell sort). */
+static void sort_chains(FtamdSysChain *c, int n) {
+    static const int gaps[] = {701, 301, 132, 57, 23, 10, 4, 1};
+    for (int g = 0; g < 8; g++) {
+        int gap = gaps[g];
+        for (int i = gap; i < n; i++) {
+            FtamdSysChain v = c[i];
+            int j = i;
+            while (j >= gap && cmp_nb_desc(&c[j - gap], &v) > 0) {
+                c[j] = c[j - gap];
+                j -= gap;
+            }
+            c[j] = v;
+        }
+    }
+}
+
 /* Chains of an n x M matrix: per class, the columns in groups of 8 (four
  * orders of the class's parity x (cos, sin) columns for sph and disk, eight
- * consecutive orders for tri), each with the bands its columns need. */
-int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
+ * consecutive orders for tri), each with the bands its columns need, written
+ * to buf (at most cap records, longest first) without allocating.
+ * FTAMD_EINVAL if a chain needs more bands than a workgroup has waves,
+ * FTAMD_ENOMEM if more than cap chains are needed (*count then holds the
+ * number needed). */
+int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
+                          int *count) {
     const int kind = RP->kind, B = FTAMD_SYS_LANES * T->S;
-    *out = NULL;
-    *count = 0;
-    int cap = 16, nch = 0;
-    FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)cap);
-    if (!ch) return FTAMD_ENOMEM;
+    int nch = 0;
     for (int c = 0; c < T->ncls; c++) {
         const sys_class *cl = &T->cls[c];
         /* orders of the class, highest first */
@@ -280,25 +298,33 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
             x.prio = x.nb >= 12 ? 3 : x.nb >= 8 ? 2 : x.nb >= 4 ? 1 : 0;
             x.lda = RP->n;
             x.aoff = 0;
-            if (x.nb > FTAMD_SYS_WAVES) {
-                free(ch);
-                return FTAMD_EINVAL;
-            }
-            if (nch == cap) {
-                cap *= 2;
-                FtamdSysChain *nc = realloc(ch, sizeof(FtamdSysChain) * (size_t)cap);
-                if (!nc) {
-                    free(ch);
-                    return FTAMD_ENOMEM;
-                }
-                ch = nc;
-            }
-            ch[nch++] = x;
+            if (x.nb > FTAMD_SYS_WAVES) return FTAMD_EINVAL;
+            if (nch < cap) buf[nch] = x;
+            nch++;
         }
     }
-    qsort(ch, (size_t)nch, sizeof(FtamdSysChain), cmp_nb_desc);
-    *out = ch;
     *count = nch;
+    if (nch > cap) return FTAMD_ENOMEM;
+    sort_chains(buf, nch);
+    return FTAMD_OK;
+}
+
+/* The same in a malloc'd array the caller frees. */
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
+    *out = NULL;
+    *count = 0;
+    int need = 0;
+    int rc = ftamd_sys_chains_fill(RP, T, M, NULL, 0, &need);
+    if (rc == FTAMD_EINVAL) return rc;
+    FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)(need > 0 ? need : 1));
+    if (!ch) return FTAMD_ENOMEM;
+    rc = ftamd_sys_chains_fill(RP, T, M, ch, need, count);
+    if (rc != FTAMD_OK) {
+        free(ch);
+        *count = 0;
+        return rc;
+    }
+    *out = ch;
     return FTAMD_OK;
 }
 

@@ -101,6 +101,25 @@ enum { FT_GPU_FP64 = 0, FT_GPU_FP32 = 1 };
 int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
                    void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP-graph-capture discipline, tight serving loops,
+ * harnesses that forbid hipMalloc): ft_gpu_execute on the band-systolic
+ * engine with its work list and error word in a fixed device scratch array of
+ * the library (256 KiB per device, held until the process ends) instead of a
+ * schedule it allocates; it calls no hipMalloc or hipFree.  The plan's
+ * band-systolic tables for this direction and precision must already be
+ * resident on the current device: one earlier ft_gpu_execute (or reference
+ * executor) call on the plan with the same direction and precision that ran
+ * on that engine leaves them so.  Returns FT_GPU_EINVAL where they are not,
+ * where the engine does not apply (beyond sph n = 1024, tri and disk n = 512
+ * in FP64; twice that in FP32), or where the work list would exceed the
+ * scratch.  Asynchronous like ft_gpu_execute (errors through
+ * ft_gpu_synchronize).  All calls on one device share its scratch, which
+ * keeps the last call's contents: calls that overlap on one device (several
+ * streams or threads) race. */
+int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
+                           void *stream);
+
 /* Waits for `stream` and reads the device error words of every asynchronous
  * launch (ft_gpu_execute, ft_gpu_execute_batch, ft_gpu_execute_batch_plan)
  * made on it since the last such check, clearing them: FT_GPU_OK, or
