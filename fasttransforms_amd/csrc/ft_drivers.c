@@ -549,6 +549,61 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     return S;
 }
 
+/* Persistent workgroups of a band-systolic launch: one per CU (the kernel's
+ * LDS allows one per CU), fewer if there are fewer chains. */
+static int sys_workgroups(int nch) {
+    int cu = 0;
+    if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
+    return nch < cu ? nch : cu;
+}
+
+/* Deals the chains to workgroups and uploads [chains][256 B error word]
+ * [extra][items][wg_items] into S->d_sys (extra: nextra bytes the caller
+ * places there, for example a batch's merged bands).  Fills the launch's
+ * chain, item and error pointers; *extra_dev gets the extra bytes' address. */
+static int sys_upload(ft_schedule *S, const FtamdSysChain *ch, int nch, const FtamdSysBand *hbands, const void *extra,
+                      size_t nextra, void **extra_dev) {
+    int nwg = sys_workgroups(nch > 0 ? nch : 1);
+    FtamdSysItem *items = malloc(sizeof(FtamdSysItem) * (size_t)(nch > 0 ? nch : 1));
+    int *wg_items = malloc(sizeof(int) * (size_t)(nwg + 1));
+    long *ws_load = malloc(sizeof(long) * (size_t)nwg * (1 + FTAMD_SYS_WAVES));
+    int *ws_chain = malloc(sizeof(int) * 2 * (size_t)(nch > 0 ? nch : 1));
+    int rc = (items && wg_items && ws_load && ws_chain) ? FTAMD_OK : FTAMD_ENOMEM;
+    if (rc == FTAMD_OK) rc = ftamd_sys_assign(ch, nch, hbands, nwg, items, wg_items, ws_load, ws_chain);
+    size_t cbytes = align256(sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1));
+    size_t xbytes = align256(nextra);
+    size_t ibytes = align256(sizeof(FtamdSysItem) * (size_t)(nch > 0 ? nch : 1));
+    size_t wbytes = sizeof(int) * (size_t)(nwg + 1);
+    char *d = NULL;
+    if (rc == FTAMD_OK) rc = ftamd_hip_malloc((void **)&d, cbytes + 256 + xbytes + ibytes + wbytes);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d, ch, sizeof(FtamdSysChain) * (size_t)nch, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_memset(d + cbytes, 0, 256);
+    if (rc == FTAMD_OK && nextra > 0) rc = ftamd_hip_h2d(d + cbytes + 256, extra, nextra, NULL);
+    if (rc == FTAMD_OK && nch > 0)
+        rc = ftamd_hip_h2d(d + cbytes + 256 + xbytes, items, sizeof(FtamdSysItem) * (size_t)nch, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d + cbytes + 256 + xbytes + ibytes, wg_items, wbytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    free(items);
+    free(wg_items);
+    free(ws_load);
+    free(ws_chain);
+    if (rc != FTAMD_OK) {
+        ftamd_hip_free(d);
+        return rc;
+    }
+    S->d_sys = d;
+    FtamdSysLaunch *a = &S->sys_launch;
+    a->chains = (const FtamdSysChain *)d;
+    a->nchains = nch;
+    a->err = (int *)(d + cbytes);
+    a->items = (const FtamdSysItem *)(d + cbytes + 256 + xbytes);
+    a->wg_items = (const int *)(d + cbytes + 256 + xbytes + ibytes);
+    a->nwg = nwg;
+    S->launch.err = a->err;
+    if (extra_dev) *extra_dev = d + cbytes + 256;
+    return FTAMD_OK;
+}
+
 /* Band-systolic schedule of the whole matrix (cached in the plan; caller
  * holds RP->lock).  Returns NULL with *rc == FTAMD_OK where the engine does
  * not apply. */
@@ -581,25 +636,16 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     S->M = M;
     S->dir = dir;
     S->precision = precision;
-    size_t cbytes = sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1);
-    *rc = ftamd_hip_malloc(&S->d_sys, cbytes + 256);
-    if (*rc == FTAMD_OK && nch > 0) *rc = ftamd_hip_h2d(S->d_sys, ch, cbytes, NULL);
-    if (*rc == FTAMD_OK) *rc = ftamd_hip_memset((char *)S->d_sys + cbytes, 0, 256);
-    if (*rc == FTAMD_OK) *rc = ftamd_hip_sync(NULL);
+    *rc = sys_upload(S, ch, nch, ftamd_sys_tables_host_bands(T), NULL, 0, NULL);
     free(ch);
     if (*rc != FTAMD_OK) {
-        ftamd_schedule_free_device(S);
         free(S);
         return NULL;
     }
     FtamdSysLaunch *a = &S->sys_launch;
     a->bands = ftamd_sys_tables_dev_bands(T);
-    a->chains = S->d_sys;
-    a->nchains = nch;
     a->rowmask = RP->kind != FTAMD_TRI;
     a->tab = ftamd_sys_tables_dev_tab(T);
-    a->err = (int *)((char *)S->d_sys + cbytes);
-    S->launch.err = a->err;
     S->L = ftamd_sys_tables_S(T);
     S->next = RP->scheds;
     RP->scheds = S;
@@ -697,7 +743,14 @@ int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int pre
  * (other streams, other threads) race.
  */
 static pthread_mutex_t g_na_lock = PTHREAD_MUTEX_INITIALIZER;
-static FtamdSysChain g_na_chains[(FTAMD_NOALLOC_SCRATCH_BYTES - 256) / sizeof(FtamdSysChain)];
+#define NA_MAX_WG 1024
+#define NA_MAX_CHAINS \
+    ((FTAMD_NOALLOC_SCRATCH_BYTES - 256 - sizeof(int) * (NA_MAX_WG + 1)) / (sizeof(FtamdSysChain) + sizeof(FtamdSysItem)))
+static FtamdSysChain g_na_chains[NA_MAX_CHAINS];
+static FtamdSysItem g_na_items[NA_MAX_CHAINS];
+static int g_na_wg_items[NA_MAX_WG + 1];
+static long g_na_ws_load[NA_MAX_WG * (1 + FTAMD_SYS_WAVES)];
+static int g_na_ws_chain[2 * NA_MAX_CHAINS];
 
 int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
                            void *stream) {
@@ -719,18 +772,30 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
     rc = ftamd_hip_scratch(&scr, &scr_bytes);
     if (rc != FTAMD_OK) return rc;
     pthread_mutex_lock(&g_na_lock);
-    int cap = (int)((scr_bytes - 256) / sizeof(FtamdSysChain)), nch = 0;
-    if (cap > (int)(sizeof g_na_chains / sizeof g_na_chains[0])) cap = (int)(sizeof g_na_chains / sizeof g_na_chains[0]);
-    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, cap, &nch);
-    if (rc != FTAMD_OK) rc = FTAMD_EINVAL; /* a chain or the chain list does not fit */
-    if (rc == FTAMD_OK && nch > 0)
-        rc = ftamd_hip_h2d((char *)scr + 256, g_na_chains, sizeof(FtamdSysChain) * (size_t)nch, stream);
+    /* scratch: [256 B error word][chains][items][wg_items] */
+    int nch = 0;
+    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, (int)NA_MAX_CHAINS, &nch);
+    int nwg = sys_workgroups(nch > 0 ? nch : 1);
+    if (nwg > NA_MAX_WG) nwg = NA_MAX_WG;
+    if (rc == FTAMD_OK)
+        rc = ftamd_sys_assign(g_na_chains, nch, ftamd_sys_tables_host_bands(T), nwg, g_na_items, g_na_wg_items,
+                              g_na_ws_load, g_na_ws_chain);
+    if (rc != FTAMD_OK) rc = FTAMD_EINVAL; /* a chain or the work list does not fit */
+    size_t cbytes = sizeof(FtamdSysChain) * (size_t)nch, ibytes = sizeof(FtamdSysItem) * (size_t)nch;
+    size_t wbytes = sizeof(int) * (size_t)(nwg + 1);
+    if (rc == FTAMD_OK && 256 + cbytes + ibytes + wbytes > scr_bytes) rc = FTAMD_EINVAL;
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256, g_na_chains, cbytes, stream);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256 + cbytes, g_na_items, ibytes, stream);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256 + cbytes + ibytes, g_na_wg_items, wbytes, stream);
     if (rc == FTAMD_OK && nch > 0) {
         FtamdSysLaunch b;
         memset(&b, 0, sizeof(b));
         b.bands = ftamd_sys_tables_dev_bands(T);
         b.chains = (const FtamdSysChain *)((char *)scr + 256);
         b.nchains = nch;
+        b.items = (const FtamdSysItem *)((char *)scr + 256 + cbytes);
+        b.wg_items = (const int *)((char *)scr + 256 + cbytes + ibytes);
+        b.nwg = nwg;
         b.rowmask = kind != FTAMD_TRI;
         b.tab = ftamd_sys_tables_dev_tab(T);
         b.A = dA;
@@ -1119,19 +1184,18 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     if (rc == FTAMD_OK) rc = ftamd_sys_batch(plans, tabs, dA, M, count, &bands, &nb, &ch, &nch, &tab_base);
     free(tabs);
     if (rc == FTAMD_EINVAL) return FTAMD_OK; /* a chain needs more bands than a workgroup has waves */
-    ft_schedule *S = rc == FTAMD_OK ? calloc(1, sizeof(ft_schedule)) : NULL;
-    if (rc == FTAMD_OK && !S) rc = FTAMD_ENOMEM;
-    size_t cbytes = sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1);
-    size_t bbytes = sizeof(FtamdSysBand) * (size_t)(nb > 0 ? nb : 1);
-    if (rc == FTAMD_OK) rc = ftamd_hip_malloc(&S->d_sys, cbytes + 256 + bbytes);
-    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(S->d_sys, ch, cbytes, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_memset((char *)S->d_sys + cbytes, 0, 256);
-    if (rc == FTAMD_OK && nb > 0) rc = ftamd_hip_h2d((char *)S->d_sys + cbytes + 256, bands, bbytes, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    if (rc != FTAMD_OK) {
+        free(bands);
+        free(ch);
+        return rc;
+    }
+    ft_schedule *S = calloc(1, sizeof(ft_schedule));
+    if (!S) rc = FTAMD_ENOMEM;
+    void *dbands = NULL;
+    if (rc == FTAMD_OK) rc = sys_upload(S, ch, nch, bands, bands, sizeof(FtamdSysBand) * (size_t)nb, &dbands);
     free(bands);
     free(ch);
     if (rc != FTAMD_OK) {
-        if (S) ftamd_schedule_free_device(S);
         free(S);
         return rc;
     }
@@ -1140,14 +1204,10 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     S->precision = precision;
     S->L = S0;
     FtamdSysLaunch *a = &S->sys_launch;
-    a->bands = (const FtamdSysBand *)((char *)S->d_sys + cbytes + 256);
-    a->chains = S->d_sys;
-    a->nchains = nch;
+    a->bands = (const FtamdSysBand *)dbands;
     a->rowmask = kind != FTAMD_TRI;
     a->tab = tab_base;
     a->A = dA[0];
-    a->err = (int *)((char *)S->d_sys + cbytes);
-    S->launch.err = a->err;
     *out = S;
     return FTAMD_OK;
 }

@@ -83,8 +83,6 @@ typedef struct {
 #define FTAMD_SYS_WAVES 16   /* most bands of a chain (waves of its workgroup) */
 #define FTAMD_SYS_UNROLL 8   /* steps per loop iteration (a band's steps are a multiple) */
 #define FTAMD_SYS_PREFETCH 16 /* most steps of table entries fetched ahead; tables carry that many spare steps */
-#define FTAMD_SYS_MAX_ROWS_FP64 512   /* most class rows of one chain's stream (FP64 / FP32) */
-#define FTAMD_SYS_MAX_ROWS_FP32 1024
 
 typedef struct {
     long long tab;    /* entry offset of the band's table: [steps + PREFETCH][LANES][S] (s, c) pairs */
@@ -107,10 +105,21 @@ typedef struct {
     long long aoff;   /* element offset of the chain's matrix from the launch's A */
 } FtamdSysChain;
 
+/* One chain of a workgroup's list: its bands run on waves wave0 .. wave0+nb-1.
+ * A workgroup streams the chains of its list one after another through its
+ * waves (FTAMD_SYS_WAVES of them). */
+typedef struct {
+    int chain;        /* index in the chain array */
+    int wave0;        /* wave of its first band */
+} FtamdSysItem;
+
 typedef struct {
     const FtamdSysBand *bands;
     const FtamdSysChain *chains;
     int nchains;
+    const FtamdSysItem *items;  /* every workgroup's chain list, one after another */
+    const int *wg_items;        /* [nwg+1]: workgroup w runs items wg_items[w] .. wg_items[w+1]-1 */
+    int nwg;                    /* workgroups (persistent: about one per CU) */
     int rowmask;      /* both columns of every lane row apply the same sequences (sph, disk) */
     const void *tab;  /* base of the band tables (bands address them by entry offsets from it) */
     void *A;          /* base of the matrices (chains address them by element offsets from it) */

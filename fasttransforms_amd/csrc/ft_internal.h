@@ -78,6 +78,9 @@ int ftamd_sys_tables_S(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
 int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count);
+int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nwg, FtamdSysItem *items,
+                     int *wg_items, long *ws_load, int *ws_chain);
+const FtamdSysBand *ftamd_sys_tables_host_bands(const ft_sys_tables *T);
 int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
                           int *count);
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,

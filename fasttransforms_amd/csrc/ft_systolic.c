@@ -88,6 +88,7 @@ struct ft_sys_tables {
 int ftamd_sys_tables_S(const ft_sys_tables *T) { return T->S; }
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T) { return T->d_bands; }
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T) { return T->d_tab; }
+const FtamdSysBand *ftamd_sys_tables_host_bands(const ft_sys_tables *T) { return T->bands; }
 
 void ftamd_sys_tables_free(ft_sys_tables *T) {
     if (!T) return;
@@ -102,9 +103,6 @@ void ftamd_sys_tables_free(ft_sys_tables *T) {
 int ftamd_sys_choose_S(int kind, int n, int precision) {
     sys_class cl[4];
     int nc = make_classes(kind, n, cl);
-    /* every class row of a chain's stream is staged in LDS once */
-    if ((kind == FTAMD_SPH ? (n + 1) / 2 : n) > (precision == 0 ? FTAMD_SYS_MAX_ROWS_FP64 : FTAMD_SYS_MAX_ROWS_FP32))
-        return 0;
     static const int opts[] = {2, 4};
     for (int o = 0; o < 2; o++) {
         int B = FTAMD_SYS_LANES * opts[o], ok = 1;
@@ -309,6 +307,56 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
     return FTAMD_OK;
 }
 
+/* Deals chains (sorted longest first) to nwg persistent workgroups and
+ * places each on a run of its workgroup's waves.  Chain by chain: the
+ * workgroup with the least work so far (work = steps of all bands, the
+ * longest-processing-time rule), then the wave offset that keeps that
+ * workgroup's busiest wave least busy (its waves stream their bands of
+ * successive chains one after another).  Writes items[nch] grouped by
+ * workgroup (each group in dealing order) and wg_items[nwg+1]; ws holds
+ * nwg*(1+FTAMD_SYS_WAVES) longs and 2*nch ints of workspace.  No allocation. */
+int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nwg, FtamdSysItem *items,
+                     int *wg_items, long *ws_load, int *ws_chain) {
+    long *wg_load = ws_load, *wave_load = ws_load + nwg;
+    int *chain_wg = ws_chain, *chain_w0 = ws_chain + nch;
+    for (int w = 0; w < nwg * (1 + FTAMD_SYS_WAVES); w++) ws_load[w] = 0;
+    for (int c = 0; c < nch; c++) {
+        const int nb = ch[c].nb;
+        const FtamdSysBand *bd = bands + ch[c].band0;
+        long work = 0;
+        for (int k = 0; k < nb; k++) work += bd[k].steps;
+        int best = 0;
+        for (int w = 1; w < nwg; w++)
+            if (wg_load[w] < wg_load[best]) best = w;
+        long *wl = wave_load + (long)best * FTAMD_SYS_WAVES;
+        int o_best = FTAMD_SYS_WAVES - nb;
+        long m_best = -1;
+        for (int o = FTAMD_SYS_WAVES - nb; o >= 0; o--) {
+            long m = 0;
+            for (int k = 0; k < nb; k++)
+                if (wl[o + k] + bd[k].steps > m) m = wl[o + k] + bd[k].steps;
+            if (m_best < 0 || m < m_best) {
+                m_best = m;
+                o_best = o;
+            }
+        }
+        for (int k = 0; k < nb; k++) wl[o_best + k] += bd[k].steps;
+        wg_load[best] += work;
+        chain_wg[c] = best;
+        chain_w0[c] = o_best;
+    }
+    for (int w = 0; w <= nwg; w++) wg_items[w] = 0;
+    for (int c = 0; c < nch; c++) wg_items[chain_wg[c] + 1]++;
+    for (int w = 0; w < nwg; w++) wg_items[w + 1] += wg_items[w];
+    for (int w = 0; w < nwg; w++) ws_load[w] = wg_items[w]; /* next free item of each workgroup */
+    for (int c = 0; c < nch; c++) {
+        long at = ws_load[chain_wg[c]]++;
+        items[at].chain = c;
+        items[at].wave0 = chain_w0[c];
+    }
+    return FTAMD_OK;
+}
+
 /* The same in a malloc'd array the caller frees. */
 int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
     *out = NULL;
@@ -394,7 +442,7 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
         free(all);
         return rc;
     }
-    qsort(all, (size_t)nall, sizeof(FtamdSysChain), cmp_nb_desc);
+    sort_chains(all, nall);
     *bands_out = bands;
     *nbands_out = nb;
     *chains_out = all;

@@ -27,11 +27,13 @@
 //    gaps.
 //  * s and c are per lane (each lane is a different sequence): no broadcast;
 //    the 4 rows of a wave read the same 16 lanes' entries.
-//  * A chain (one workgroup) runs its bands as waves; band b+1 reads band
+//  * A chain runs its bands as waves of one workgroup; band b+1 reads band
 //    b's output stream through an LDS ring as it is produced, so a chain's
 //    bands overlap and the chain's latency is about one band's stream plus
-//    (S+1)*LANES steps per band.  The first band reads A, the rows that leave
-//    the chain for good are stored to A.
+//    (S+1)*LANES steps per band.  A workgroup runs several chains side by
+//    side when their bands fit its 16 waves (the host packs them).  The first
+//    band reads A; rows that leave the chain are stored to A once per 8-step
+//    block by the band that emits them.
 //
 // Arithmetic: the reference's rotation in fused form, x' = fma(c, x, s*y),
 // y' = fma(c, y, -(s*x)) (src/rotations/rotations_AVX512F.c:4-15).  With
@@ -98,16 +100,13 @@ struct Engine {
                   "table staging supports 2 or 4 steps per DMA and a ring of whole blocks");
 
 
-    static constexpr int kMaxRows = sizeof(T) == 8 ? FTAMD_SYS_MAX_ROWS_FP64 : FTAMD_SYS_MAX_ROWS_FP32;
-
     struct Shared {
         // table entries of the next kD steps of each wave, written by LDS DMA
         // (first in the block: the DMA's LDS address is taken from M0)
         Pair<T> scr[kWaves][kD][kLanes][S];
-        T ring[kWaves][kZ][kRows][kCols];  // ring w: input stream of band (wave) w
-        T stage[kMaxRows][kRows][kCols];   // rows leaving the chain, by class row, until stored to A
-        int avail[kWaves];                 // positions of ring w written by wave w-1 (exclusive end)
-        int freed[kWaves];                 // positions of ring w wave w has read (exclusive end)
+        T ring[kWaves][kZ][kRows][kCols];  // ring g: input stream of the band wave g runs
+        int avail[kWaves];                 // positions of ring g written by wave g-1 (exclusive end)
+        int freed[kWaves];                 // positions of ring g wave g has read (exclusive end)
         int abort_flag;
     };
 
@@ -150,7 +149,10 @@ struct Engine {
 #endif
         for (int it = 0; seen < need; ++it) {
             if (it >= limit || uniform(lds_ld(&sh.abort_flag))) return false;
-            __builtin_amdgcn_s_sleep(1);
+            // far behind (a band waiting for the chain's fill): poll rarely,
+            // so waiting waves leave the issue slots to the working ones
+            if (need - seen > 2 * kU) __builtin_amdgcn_s_sleep(16);
+            else __builtin_amdgcn_s_sleep(1);
             seen = uniform(lds_ld(ctr));
         }
         return true;
@@ -244,8 +246,14 @@ struct Engine {
         }
     }
 
+    // w: the band's index in its chain; g: the wave (and ring) index in the
+    // workgroup.  Ring positions are counted across the chains a workgroup
+    // streams: this band's input stream starts at position in_base of ring g,
+    // its output (the next band's input) at out_base of ring g+1, both
+    // multiples of kZ.  Returns false if a wait expired (abort raised).
     template <int MASK>
-    __device__ static void run_band(const FtamdSysLaunch &a, Shared &sh, const FtamdSysChain &ch, int w, int lane) {
+    __device__ static bool run_band(const FtamdSysLaunch &a, Shared &sh, const FtamdSysChain &ch, int w, int g,
+                                    int lane, int in_base, int out_base) {
         const int r = lane >> 4, i = lane & 15;
         const int nb = uniform(ch.nb);
         const int b = uniform(ch.band0) + w;
@@ -268,7 +276,6 @@ struct Engine {
         const int colv[kCols] = {ch.col[2 * r], ch.col[2 * r + 1]};
         const char *tab_band = (const char *)a.tab + a.bands[b].tab * (long long)sizeof(Pair<T>);
         const unsigned lane16 = (unsigned)lane * 16u;
-        const unsigned r16 = (unsigned)r * (kCols * sizeof(T));  // row r's place in a ring or stage entry
 
         // matrix element of class row e of column c of this lane's row
         auto aptr = [&](int c, int e) -> T * { return A + (long long)colv[c] * lda + (long long)rstep * e + rpar; };
@@ -282,14 +289,14 @@ struct Engine {
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < kCols; ++c) {
-                const int g = gbase + S * i + s;
+                const int q = gbase + S * i + s;
                 const int k = MASK == 1 && c == 0 && colv[0] < 0 ? 2 * r + 1 : 2 * r + c;
-                act[s][c] = g >= ch.glo[k] && g <= ch.ghi[k];
+                act[s][c] = q >= ch.glo[k] && q <= ch.ghi[k];
             }
 
-        T(*in_ring)[kRows][kCols] = sh.ring[w];
-        T(*out_ring)[kRows][kCols] = last ? sh.ring[w] : sh.ring[w + 1];
-        int avail_seen = 0, freed_seen = 0;
+        T(*in_ring)[kRows][kCols] = sh.ring[g];
+        T(*out_ring)[kRows][kCols] = last ? sh.ring[g] : sh.ring[g + 1];
+        int avail_seen = 0, freed_seen = 0;  // last counter values read (global positions)
         bool ok = true;
 #ifdef FTAMD_TRACE
         if (lane == 0) tr_in() = tr_out() = tr_dma() = 0;
@@ -297,9 +304,8 @@ struct Engine {
 #endif
 
         // ---- stream positions from A: lane i of each row fetches position
-        // p0 + i of a chunk of 16.  (Loads only: this wave issues
-        // no global store before its loop ends, so its waits on the table
-        // loads count exactly.)
+        // p0 + i of a chunk of 16.  (Loads and stores of A only make the
+        // waits on the table loads stricter: see the output stores.)
         auto fetch = [&](int p0, T (&v)[kCols]) {
             const int p = p0 + i;
 #pragma unroll
@@ -327,7 +333,8 @@ struct Engine {
         T inbuf[kDin][kCols];
         auto ensure_in = [&](int p) -> bool {
             bool good = true;
-            if (p >= npa && p <= R && p >= avail_seen) good = wait_ge(sh, &sh.avail[w], p + 1, avail_seen, spin);
+            if (p >= npa && p <= R && in_base + p >= avail_seen)
+                good = wait_ge(sh, &sh.avail[g], in_base + p + 1, avail_seen, spin);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             return good;
         };
@@ -343,7 +350,7 @@ struct Engine {
 
         // ---- table prefetch: steps 0 .. kD-1
 #pragma unroll
-        for (int d = 0; d < kD; d += kSPD) dma_block(sh, w, tab_band, d, lane16, T(0));
+        for (int d = 0; d < kD; d += kSPD) dma_block(sh, g, tab_band, d, lane16, T(0));
 
         T Y[S][kCols], E[S][kCols];
 #pragma unroll
@@ -354,12 +361,12 @@ struct Engine {
         // entries of step 0 (read one step ahead from here on)
         SC scn;
         wait_block();
-        read_sc(sh, w, 0, i, scn);
+        read_sc(sh, g, 0, i, scn);
 
         // Steps run in blocks of kU.  The waits are made once per block, so
         // a step is only its DPP shifts, LDS reads, rotations and the
         // broadcast of its output.
-        int written = 0;  // ring positions of the next band written (exclusive end)
+        int written = out_base;  // ring positions of the next band written (exclusive end, global)
         for (int t0 = 0; t0 < steps; t0 += kU) {
             if (t0 % kChunk == 0) {
                 // positions t0+16 .. t0+31 to the ring, into the slots of
@@ -373,7 +380,8 @@ struct Engine {
             // every input position the block reads ahead is in the ring
             {
                 const int pl = min(t0 + kU - 1 + kDin, R);
-                if (pl >= npa && pl >= avail_seen && !wait_ge(sh, &sh.avail[w], pl + 1, avail_seen, spin)) {
+                if (pl >= npa && in_base + pl >= avail_seen &&
+                    !wait_ge(sh, &sh.avail[g], in_base + pl + 1, avail_seen, spin)) {
                     ok = false;
                     break;
                 }
@@ -387,8 +395,8 @@ struct Engine {
             {
                 const int hi = min(po0 + kU - 1, ring_hi);
                 if (hi >= max(po0, 0)) {
-                    const int P1 = hi + npa_next;
-                    if (P1 - kZ >= freed_seen && !wait_ge(sh, &sh.freed[w + 1], P1 - kZ + 1, freed_seen, spin)) {
+                    const int P1 = out_base + hi + npa_next;
+                    if (P1 - kZ >= freed_seen && !wait_ge(sh, &sh.freed[g + 1], P1 - kZ + 1, freed_seen, spin)) {
                         ok = false;
                         break;
                     }
@@ -401,9 +409,9 @@ struct Engine {
             const T *pin = &in_ring[t0 % kZ][r][0];
             const T *pin_hi = (t0 % kZ == kZ - kU) ? pin - kZ * kRows * kCols : pin;
             const int sb = t0 % kD;
-            const Pair<T> *psc = &sh.scr[w][sb][i][0];
+            const Pair<T> *psc = &sh.scr[g][sb][i][0];
             const Pair<T> *psc_hi = (sb == kD - kU) ? psc - kD * kLanes * S : psc;
-            const unsigned m0b = lds_addr(&sh.scr[w][0][0][0]) + (unsigned)sb * kStepBytes;
+            const unsigned m0b = lds_addr(&sh.scr[g][0][0][0]) + (unsigned)sb * kStepBytes;
             const char *srcb = tab_band + (long long)(t0 + kD) * kStepBytes;  // step t0+kD+K: the DMA of step u, K = u+1-kSPD
             // step u's output (the row-end values of the last sequence, lane
             // 15 of each row) is broadcast along its row and kept by lane u;
@@ -452,56 +460,62 @@ struct Engine {
                 }
             }
             // lane i < kU of each row stores output position po0 + i: to the
-            // next band's ring up to ring_hi, else (leaving the chain) to the
-            // stage; positions outside [0, R] carry no row
+            // next band's ring up to ring_hi, else (leaving the chain) to its
+            // row of A; positions outside [0, R] carry no row.  (The global
+            // stores only lengthen the table waits that follow them: those
+            // wait for the DMAs issued before them, and loads return in order.)
             {
                 const int po = po0 + i;
                 if (i < kU && po >= 0 && po <= R) {
-                    T *dst = po <= ring_hi ? &out_ring[(po + npa_next) % kZ][r][0] : &sh.stage[row_of(po)][r][0];
+                    if (po <= ring_hi) {
+                        T *dst = &out_ring[(po + npa_next) % kZ][r][0];
 #pragma unroll
-                    for (int c = 0; c < kCols; ++c) lds_st(dst + c, outv[c]);
+                        for (int c = 0; c < kCols; ++c) lds_st(dst + c, outv[c]);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < kCols; ++c)
+                            if (colv[c] >= 0) *aptr(c, row_of(po)) = outv[c];
+                    }
                 }
             }
             if (!ok) break;
             // publish what the block wrote and read
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (!last && lane == 0) lds_st(&sh.avail[w + 1], written);
-            if (lane == 0) lds_st(&sh.freed[w], t0 + kU);
+            if (!last && lane == 0) lds_st(&sh.avail[g + 1], written);
+            // positions read, never beyond this chain's part of the ring
+            if (lane == 0) lds_st(&sh.freed[g], in_base + min(t0 + kU, span(R)));
         }
 
         if (!ok) {
             fail(sh);
             atomicOr(a.err, FTAMD_ERR_TIMEOUT);
-            return;
+            return false;
         }
 #ifdef FTAMD_TRACE
         if (a.trace && lane == 0) {
-            unsigned long long *rec = a.trace + ((long long)blockIdx.x * kWaves + w) * 8;
-            rec[0] = tr_start;
+            // accumulated over the bands the wave runs
+            unsigned long long *rec = a.trace + ((long long)blockIdx.x * kWaves + g) * 8;
+            if (rec[0] == 0) rec[0] = tr_start;
             rec[1] = wall_clock64();
-            rec[2] = __builtin_amdgcn_s_memtime() - tr_c0;
-            rec[3] = tr_in();
-            rec[4] = tr_out();
-            rec[5] = steps;
-            rec[6] = R;
-            rec[7] = tr_dma();
+            rec[2] += __builtin_amdgcn_s_memtime() - tr_c0;
+            rec[3] += tr_in();
+            rec[4] += tr_out();
+            rec[5] += steps;
+            rec[6] += R;
+            rec[7] += tr_dma();
         }
 #endif
-        // ---- rows that left the chain here, staged by class row, to A:
-        // all rows 0 .. R from the last band; none from the other hi2lo bands
-        // (every row they emit goes on to the next band); rows Rnext+1 .. R
-        // from the other lo2hi bands
-        const int lo = last ? 0 : (dir == 0 ? R + 1 : Rnext + 1);
-        for (int e0 = lo; e0 <= R; e0 += kLanes) {
-            const int e = e0 + i;
-            if (e <= R) {
-#pragma unroll
-                for (int c = 0; c < kCols; ++c)
-                    if (colv[c] >= 0) *aptr(c, e) = lds_ld(&sh.stage[e][r][c]);
-            }
-        }
+        return true;
     }
 
+    // Ring positions one band's stream takes: its R + 1 positions rounded up
+    // to whole rings, so that every stream starts at ring slot 0.
+    __device__ static int span(int R) { return (R + kZ) / kZ * kZ; }
+
+    // A workgroup streams its chains (items) one after another through its
+    // waves: item k runs its chain's bands on waves wave0 .. wave0+nb-1, and
+    // every wave takes the items in list order, so a wave starts its band of
+    // the next chain as soon as it has finished its band of the previous one.
     __device__ static void run(const FtamdSysLaunch &a, Shared &sh) {
         {
             T *rp = &sh.ring[0][0][0][0];
@@ -513,27 +527,38 @@ struct Engine {
             if (threadIdx.x == 0) sh.abort_flag = 0;
         }
         __syncthreads();
-        const FtamdSysChain &ch = a.chains[blockIdx.x];
         const int lane = threadIdx.x & 63;
-        const int w = uniform(threadIdx.x >> 6);
-        if (w >= uniform(ch.nb)) return;
-        switch (uniform(ch.prio)) {
-            case 3: __builtin_amdgcn_s_setprio(3); break;
-            case 2: __builtin_amdgcn_s_setprio(2); break;
-            case 1: __builtin_amdgcn_s_setprio(1); break;
-            default: break;
+        const int g = uniform(threadIdx.x >> 6);
+        const int it0 = uniform(a.wg_items[blockIdx.x]), it1 = uniform(a.wg_items[blockIdx.x + 1]);
+        int in_base = 0, out_base = 0;  // where the current chain's streams start in rings g and g+1
+        for (int it = it0; it < it1; ++it) {
+            const FtamdSysChain &ch = a.chains[uniform(a.items[it].chain)];
+            const int nb = uniform(ch.nb), b0 = uniform(ch.band0);
+            const int w = g - uniform(a.items[it].wave0);  // this wave's band of the chain, if any
+            if (w >= 0 && w < nb) {
+                switch (uniform(ch.prio)) {
+                    case 3: __builtin_amdgcn_s_setprio(3); break;
+                    case 2: __builtin_amdgcn_s_setprio(2); break;
+                    case 1: __builtin_amdgcn_s_setprio(1); break;
+                    default: __builtin_amdgcn_s_setprio(0); break;
+                }
+                // a band needs masks if some column of the chain does not
+                // apply some of its sequences; with ROWMASK one mask per lane
+                // row serves both columns (sph and disk: the two columns of a
+                // row are the cosine and sine columns of one order, so they
+                // apply the same sequences)
+                const int gbase = uniform(a.bands[b0 + w].gbase);
+                const int glast = gbase + uniform(a.bands[b0 + w].nseq) - 1;
+                bool masked = false;
+                for (int k = 0; k < FTAMD_SYS_COLS; ++k)
+                    masked |= ch.col[k] >= 0 && (ch.glo[k] > gbase || ch.ghi[k] < glast);
+                const bool ok = !masked ? run_band<0>(a, sh, ch, w, g, lane, in_base, out_base)
+                                        : run_band<ROWMASK ? 1 : 2>(a, sh, ch, w, g, lane, in_base, out_base);
+                if (!ok) break;
+                in_base += span(uniform(a.bands[b0 + w].R));
+            }
+            if (w + 1 >= 0 && w + 1 < nb) out_base += span(uniform(a.bands[b0 + w + 1].R));
         }
-        // a band needs masks if some column of the chain does not apply some
-        // of its sequences; with ROWMASK one mask per lane row serves both
-        // columns (sph and disk: the two columns of a row are the cosine and
-        // sine columns of one order, so they apply the same sequences)
-        const int gbase = uniform(a.bands[uniform(ch.band0) + w].gbase);
-        const int glast = gbase + uniform(a.bands[uniform(ch.band0) + w].nseq) - 1;
-        bool masked = false;
-        for (int k = 0; k < FTAMD_SYS_COLS; ++k)
-            masked |= ch.col[k] >= 0 && (ch.glo[k] > gbase || ch.ghi[k] < glast);
-        if (!masked) run_band<0>(a, sh, ch, w, lane);
-        else run_band<ROWMASK ? 1 : 2>(a, sh, ch, w, lane);
         __builtin_amdgcn_s_setprio(0);
     }
 };
