@@ -62,6 +62,7 @@ SIGNATURES = {
     "ft_gpu_destroy_batch": (None, [_vp]),
     "ft_gpu_set_tuning": (_i, [_i, _i, _i]),
     "ft_gpu_set_engine": (_i, [_i]),
+    "ft_gpu_set_systolic": (_i, [_i]),
     "ft_gpu_last_engine": (_i, []),
     "ft_gpu_set_spin_limit": (_i, [_i]),
 }
@@ -167,6 +168,12 @@ def set_engine(engine=ENGINE_AUTO):
     rc = _lib.ft_gpu_set_engine(int(engine))
     if rc:
         raise ValueError("unknown engine %r" % (engine,))
+
+
+def set_systolic(maxpart=0):
+    """Most bands of a band-systolic chain part (1..16; longer chains run in parts); 0 = built-in choice."""
+    if _lib.ft_gpu_set_systolic(int(maxpart)):
+        raise ValueError("maxpart must be in 0..16")
 
 
 def set_spin_limit(limit=0):

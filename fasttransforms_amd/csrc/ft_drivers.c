@@ -78,6 +78,24 @@ int ft_gpu_set_engine(int engine) {
 
 int ftamd_engine(void) { return KNOB_GET(g_engine); }
 
+/* Bands per chain part of the band-systolic engine (longer chains run in
+ * parts on several workgroups of one XCD). */
+static int g_sys_maxpart = 0;
+
+int ft_gpu_set_systolic(int maxpart) {
+    if (maxpart < 0 || maxpart > FTAMD_SYS_WAVES) return FT_GPU_EINVAL;
+    KNOB_SET(g_sys_maxpart, maxpart);
+    return FT_GPU_OK;
+}
+
+int ftamd_sys_maxpart(int kind, int n, int precision) {
+    (void)kind;
+    (void)n;
+    (void)precision;
+    int k = KNOB_GET(g_sys_maxpart);
+    return k ? k : 8;
+}
+
 static __thread int t_last_engine = 0;
 int ft_gpu_last_engine(void) { return t_last_engine; }
 
@@ -549,42 +567,99 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     return S;
 }
 
-/* Persistent workgroups of a band-systolic launch: one per CU (the kernel's
- * LDS allows one per CU), fewer if there are fewer chains. */
-static int sys_workgroups(int nch) {
-    int cu = 0;
+/* Slots of a band-systolic launch of nch chain parts: nxcd XCDs of *nsx
+ * slots each, about one slot per CU (the kernel's LDS allows one workgroup
+ * per CU), fewer if there are fewer parts, and at least as many per XCD as
+ * the chain with the most parts has.  Without split chains the slots form
+ * one list (nxcd 1) that workgroups on every XCD take in order. */
+static void sys_slots(const FtamdSysChain *ch, int nch, int *nxcd, int *nsx) {
+    int x = 1, cu = 0;
+    if (ftamd_hip_xcd_count(&x) != FTAMD_OK || x < 1) x = 1;
+    if (x > FTAMD_MAX_XCD) x = FTAMD_MAX_XCD;
     if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
-    return nch < cu ? nch : cu;
+    int per = cu / x > 0 ? cu / x : 1;
+    int longest = 1, run = 1;
+    for (int c = 1; c < nch; c++) {
+        run = (ch[c].split_in >= 0 && ch[c].split_in == ch[c - 1].split_out) ? run + 1 : 1;
+        if (run > longest) longest = run;
+    }
+    if (longest == 1) {
+        /* no part waits on another: one list of slots, whatever XCD runs them */
+        *nxcd = 1;
+        *nsx = nch < cu ? (nch > 0 ? nch : 1) : cu;
+        return;
+    }
+    int need = (nch + x - 1) / x;
+    int s = need < per ? need : per;
+    if (s < longest) s = longest;
+    *nxcd = x;
+    *nsx = s > 0 ? s : 1;
 }
 
-/* Deals the chains to workgroups and uploads [chains][256 B error word]
- * [extra][items][wg_items] into S->d_sys (extra: nextra bytes the caller
- * places there, for example a batch's merged bands).  Fills the launch's
- * chain, item and error pointers; *extra_dev gets the extra bytes' address. */
-static int sys_upload(ft_schedule *S, const FtamdSysChain *ch, int nch, const FtamdSysBand *hbands, const void *extra,
-                      size_t nextra, void **extra_dev) {
-    int nwg = sys_workgroups(nch > 0 ? nch : 1);
+/* Fills the launch's work-list fields for the arrays at device address d
+ * laid out as sys_layout describes. */
+typedef struct {
+    size_t prog, chains, extra, items, slot_items, xcd_slot, total;
+} sys_layout;
+
+/* [256 B control words][progress counters][chains][extra][items][slot_items]
+ * [xcd_slot], each part 256-byte aligned. */
+static sys_layout sys_layout_of(int nch, int nprog, size_t nextra, int nslots, int nxcd) {
+    sys_layout L;
+    L.prog = FTAMD_SYS_CTL_BYTES;
+    L.chains = L.prog + align256(sizeof(int) * (size_t)(nprog > 0 ? nprog : 1));
+    L.extra = L.chains + align256(sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1));
+    L.items = L.extra + align256(nextra);
+    L.slot_items = L.items + align256(sizeof(FtamdSysItem) * (size_t)(nch > 0 ? nch : 1));
+    L.xcd_slot = L.slot_items + align256(sizeof(int) * (size_t)(nslots + 1));
+    L.total = L.xcd_slot + sizeof(int) * (size_t)(nxcd + 1);
+    return L;
+}
+
+static void sys_fill_launch(FtamdSysLaunch *a, char *d, const sys_layout *L, int nch, int nprog, int nslots,
+                            int nxcd, int nsx) {
+    a->ctl = (int *)d;
+    a->prog = (int *)(d + L->prog);
+    a->nprog = nprog;
+    a->chains = (const FtamdSysChain *)(d + L->chains);
+    a->nchains = nch;
+    a->items = (const FtamdSysItem *)(d + L->items);
+    a->slot_items = (const int *)(d + L->slot_items);
+    a->xcd_slot = (const int *)(d + L->xcd_slot);
+    a->nslots = nslots;
+    a->nxcd = nxcd;
+    a->grid = nxcd * nsx;
+}
+
+/* Deals the chain parts to slots and uploads the work list (sys_layout;
+ * extra: nextra bytes the caller places there, for example a batch's merged
+ * bands) into S->d_sys.  Fills the launch's work-list pointers; *extra_dev
+ * gets the extra bytes' address. */
+static int sys_upload(ft_schedule *S, const FtamdSysChain *ch, int nch, int nprog, const FtamdSysBand *hbands,
+                      const void *extra, size_t nextra, void **extra_dev) {
+    int nxcd = 1, nsx = 1;
+    sys_slots(ch, nch, &nxcd, &nsx);
+    const int nslots = nxcd * nsx;
+    const sys_layout L = sys_layout_of(nch, nprog, nextra, nslots, nxcd);
     FtamdSysItem *items = malloc(sizeof(FtamdSysItem) * (size_t)(nch > 0 ? nch : 1));
-    int *wg_items = malloc(sizeof(int) * (size_t)(nwg + 1));
-    long *ws_load = malloc(sizeof(long) * (size_t)nwg * (1 + FTAMD_SYS_WAVES));
+    int *slot_items = malloc(sizeof(int) * (size_t)(nslots + 1));
+    int xcd_slot[FTAMD_MAX_XCD + 1];
+    long *ws_load = malloc(sizeof(long) * ((size_t)nslots * (1 + FTAMD_SYS_WAVES) + (size_t)nxcd));
     int *ws_chain = malloc(sizeof(int) * 2 * (size_t)(nch > 0 ? nch : 1));
-    int rc = (items && wg_items && ws_load && ws_chain) ? FTAMD_OK : FTAMD_ENOMEM;
-    if (rc == FTAMD_OK) rc = ftamd_sys_assign(ch, nch, hbands, nwg, items, wg_items, ws_load, ws_chain);
-    size_t cbytes = align256(sizeof(FtamdSysChain) * (size_t)(nch > 0 ? nch : 1));
-    size_t xbytes = align256(nextra);
-    size_t ibytes = align256(sizeof(FtamdSysItem) * (size_t)(nch > 0 ? nch : 1));
-    size_t wbytes = sizeof(int) * (size_t)(nwg + 1);
+    int rc = (items && slot_items && ws_load && ws_chain) ? FTAMD_OK : FTAMD_ENOMEM;
+    if (rc == FTAMD_OK)
+        rc = ftamd_sys_assign(ch, nch, hbands, nxcd, nsx, items, slot_items, xcd_slot, ws_load, ws_chain);
     char *d = NULL;
-    if (rc == FTAMD_OK) rc = ftamd_hip_malloc((void **)&d, cbytes + 256 + xbytes + ibytes + wbytes);
-    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d, ch, sizeof(FtamdSysChain) * (size_t)nch, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_memset(d + cbytes, 0, 256);
-    if (rc == FTAMD_OK && nextra > 0) rc = ftamd_hip_h2d(d + cbytes + 256, extra, nextra, NULL);
-    if (rc == FTAMD_OK && nch > 0)
-        rc = ftamd_hip_h2d(d + cbytes + 256 + xbytes, items, sizeof(FtamdSysItem) * (size_t)nch, NULL);
-    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d + cbytes + 256 + xbytes + ibytes, wg_items, wbytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_malloc((void **)&d, L.total);
+    if (rc == FTAMD_OK) rc = ftamd_hip_memset(d, 0, L.chains); /* control words and progress counters */
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d + L.chains, ch, sizeof(FtamdSysChain) * (size_t)nch, NULL);
+    if (rc == FTAMD_OK && nextra > 0) rc = ftamd_hip_h2d(d + L.extra, extra, nextra, NULL);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d + L.items, items, sizeof(FtamdSysItem) * (size_t)nch, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d + L.slot_items, slot_items, sizeof(int) * (size_t)(nslots + 1), NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d + L.xcd_slot, xcd_slot, sizeof(int) * (size_t)(nxcd + 1), NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     free(items);
-    free(wg_items);
+    free(slot_items);
     free(ws_load);
     free(ws_chain);
     if (rc != FTAMD_OK) {
@@ -592,15 +667,9 @@ static int sys_upload(ft_schedule *S, const FtamdSysChain *ch, int nch, const Ft
         return rc;
     }
     S->d_sys = d;
-    FtamdSysLaunch *a = &S->sys_launch;
-    a->chains = (const FtamdSysChain *)d;
-    a->nchains = nch;
-    a->err = (int *)(d + cbytes);
-    a->items = (const FtamdSysItem *)(d + cbytes + 256 + xbytes);
-    a->wg_items = (const int *)(d + cbytes + 256 + xbytes + ibytes);
-    a->nwg = nwg;
-    S->launch.err = a->err;
-    if (extra_dev) *extra_dev = d + cbytes + 256;
+    sys_fill_launch(&S->sys_launch, d, &L, nch, nprog, nslots, nxcd, nsx);
+    S->launch.err = S->sys_launch.ctl + FTAMD_SYS_CTL_ERR;
+    if (extra_dev) *extra_dev = d + L.extra;
     return FTAMD_OK;
 }
 
@@ -611,20 +680,17 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     *rc = FTAMD_OK;
     int Sq = ftamd_sys_choose_S(RP->kind, RP->n, precision);
     if (Sq == 0 || (long long)RP->n * M > 0x7fffffffLL) return NULL;
+    const int maxpart = ftamd_sys_maxpart(RP->kind, RP->n, precision);
     for (ft_schedule *s = RP->scheds; s; s = s->next)
-        if (s->sys && s->M == M && s->dir == dir && s->precision == precision) return s;
+        if (s->sys && s->M == M && s->dir == dir && s->precision == precision && s->maxpart == maxpart) return s;
     if (!RP->sys[dir][precision]) {
         *rc = ftamd_sys_tables_build(RP, dir, precision, Sq, &RP->sys[dir][precision]);
         if (*rc != FTAMD_OK) return NULL;
     }
     const ft_sys_tables *T = RP->sys[dir][precision];
     FtamdSysChain *ch = NULL;
-    int nch = 0;
-    *rc = ftamd_sys_chains(RP, T, M, &ch, &nch);
-    if (*rc == FTAMD_EINVAL) {
-        *rc = FTAMD_OK;
-        return NULL;
-    }
+    int nch = 0, nprog = 0;
+    *rc = ftamd_sys_chains(RP, T, M, maxpart, &ch, &nch, &nprog);
     if (*rc != FTAMD_OK) return NULL;
     ft_schedule *S = calloc(1, sizeof(ft_schedule));
     if (!S) {
@@ -636,7 +702,8 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     S->M = M;
     S->dir = dir;
     S->precision = precision;
-    *rc = sys_upload(S, ch, nch, ftamd_sys_tables_host_bands(T), NULL, 0, NULL);
+    S->maxpart = maxpart;
+    *rc = sys_upload(S, ch, nch, nprog, ftamd_sys_tables_host_bands(T), NULL, 0, NULL);
     free(ch);
     if (*rc != FTAMD_OK) {
         free(S);
@@ -743,14 +810,17 @@ int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int pre
  * (other streams, other threads) race.
  */
 static pthread_mutex_t g_na_lock = PTHREAD_MUTEX_INITIALIZER;
-#define NA_MAX_WG 1024
-#define NA_MAX_CHAINS \
-    ((FTAMD_NOALLOC_SCRATCH_BYTES - 256 - sizeof(int) * (NA_MAX_WG + 1)) / (sizeof(FtamdSysChain) + sizeof(FtamdSysItem)))
+#define NA_MAX_SLOTS 1024
+#define NA_MAX_CHAINS                                                                                            \
+    ((FTAMD_NOALLOC_SCRATCH_BYTES - FTAMD_SYS_CTL_BYTES - 6 * 256 - sizeof(int) * (NA_MAX_SLOTS + FTAMD_MAX_XCD + 2)) / \
+     (sizeof(FtamdSysChain) + sizeof(FtamdSysItem) + sizeof(int)))
 static FtamdSysChain g_na_chains[NA_MAX_CHAINS];
 static FtamdSysItem g_na_items[NA_MAX_CHAINS];
-static int g_na_wg_items[NA_MAX_WG + 1];
-static long g_na_ws_load[NA_MAX_WG * (1 + FTAMD_SYS_WAVES)];
+static int g_na_slot_items[NA_MAX_SLOTS + 1];
+static int g_na_xcd_slot[FTAMD_MAX_XCD + 1];
+static long g_na_ws_load[NA_MAX_SLOTS * (1 + FTAMD_SYS_WAVES) + FTAMD_MAX_XCD];
 static int g_na_ws_chain[2 * NA_MAX_CHAINS];
+static const int g_na_zeros[NA_MAX_CHAINS];
 
 int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
                            void *stream) {
@@ -772,34 +842,40 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
     rc = ftamd_hip_scratch(&scr, &scr_bytes);
     if (rc != FTAMD_OK) return rc;
     pthread_mutex_lock(&g_na_lock);
-    /* scratch: [256 B error word][chains][items][wg_items] */
-    int nch = 0;
-    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, (int)NA_MAX_CHAINS, &nch);
-    int nwg = sys_workgroups(nch > 0 ? nch : 1);
-    if (nwg > NA_MAX_WG) nwg = NA_MAX_WG;
+    /* the scratch holds the work list as a schedule does (sys_layout) */
+    int nfull = 0, nch = 0, nprog = 0, nxcd = 1, nsx = 1;
+    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, (int)NA_MAX_CHAINS, &nfull);
     if (rc == FTAMD_OK)
-        rc = ftamd_sys_assign(g_na_chains, nch, ftamd_sys_tables_host_bands(T), nwg, g_na_items, g_na_wg_items,
-                              g_na_ws_load, g_na_ws_chain);
-    if (rc != FTAMD_OK) rc = FTAMD_EINVAL; /* a chain or the work list does not fit */
-    size_t cbytes = sizeof(FtamdSysChain) * (size_t)nch, ibytes = sizeof(FtamdSysItem) * (size_t)nch;
-    size_t wbytes = sizeof(int) * (size_t)(nwg + 1);
-    if (rc == FTAMD_OK && 256 + cbytes + ibytes + wbytes > scr_bytes) rc = FTAMD_EINVAL;
-    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256, g_na_chains, cbytes, stream);
-    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256 + cbytes, g_na_items, ibytes, stream);
-    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d((char *)scr + 256 + cbytes + ibytes, g_na_wg_items, wbytes, stream);
+        rc = ftamd_sys_split(g_na_chains, nfull, (int)NA_MAX_CHAINS, ftamd_sys_maxpart(kind, RP->n, precision), &nch,
+                             &nprog);
+    if (rc == FTAMD_OK) {
+        sys_slots(g_na_chains, nch, &nxcd, &nsx);
+        if (nxcd * nsx > NA_MAX_SLOTS) rc = FTAMD_EINVAL;
+    }
+    if (rc == FTAMD_OK)
+        rc = ftamd_sys_assign(g_na_chains, nch, ftamd_sys_tables_host_bands(T), nxcd, nsx, g_na_items,
+                              g_na_slot_items, g_na_xcd_slot, g_na_ws_load, g_na_ws_chain);
+    if (rc != FTAMD_OK) rc = FTAMD_EINVAL; /* the work list does not fit */
+    const int nslots = nxcd * nsx;
+    const sys_layout L = sys_layout_of(nch, nprog, 0, nslots, nxcd);
+    char *d = (char *)scr;
+    if (rc == FTAMD_OK && L.total > scr_bytes) rc = FTAMD_EINVAL;
+    /* progress counters start at zero (earlier calls may have left other
+     * records there); the control words are zero after every launch */
+    if (rc == FTAMD_OK && nprog > 0) rc = ftamd_hip_h2d(d + L.prog, g_na_zeros, sizeof(int) * (size_t)nprog, stream);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d + L.chains, g_na_chains, sizeof(FtamdSysChain) * (size_t)nch, stream);
+    if (rc == FTAMD_OK && nch > 0) rc = ftamd_hip_h2d(d + L.items, g_na_items, sizeof(FtamdSysItem) * (size_t)nch, stream);
+    if (rc == FTAMD_OK)
+        rc = ftamd_hip_h2d(d + L.slot_items, g_na_slot_items, sizeof(int) * (size_t)(nslots + 1), stream);
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d + L.xcd_slot, g_na_xcd_slot, sizeof(int) * (size_t)(nxcd + 1), stream);
     if (rc == FTAMD_OK && nch > 0) {
         FtamdSysLaunch b;
         memset(&b, 0, sizeof(b));
+        sys_fill_launch(&b, d, &L, nch, nprog, nslots, nxcd, nsx);
         b.bands = ftamd_sys_tables_dev_bands(T);
-        b.chains = (const FtamdSysChain *)((char *)scr + 256);
-        b.nchains = nch;
-        b.items = (const FtamdSysItem *)((char *)scr + 256 + cbytes);
-        b.wg_items = (const int *)((char *)scr + 256 + cbytes + ibytes);
-        b.nwg = nwg;
         b.rowmask = kind != FTAMD_TRI;
         b.tab = ftamd_sys_tables_dev_tab(T);
         b.A = dA;
-        b.err = (int *)scr;
         int spin = KNOB_GET(g_spin_limit);
         b.spin_limit = spin ? spin : 1 << 22;
         rc = ftamd_hip_sys_launch(&b, precision, ftamd_sys_tables_S(T), stream);
@@ -808,7 +884,7 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
     if (rc == FTAMD_OK) {
         t_last_engine = FT_GPU_ENGINE_SYSTOLIC;
         pthread_mutex_lock(&g_pend_lock);
-        g_na_pend[dev].err = (int *)scr;
+        g_na_pend[dev].err = (int *)scr + FTAMD_SYS_CTL_ERR;
         g_na_pend[dev].pending = 1;
         g_na_pend[dev].stream = stream;
         pthread_mutex_unlock(&g_pend_lock);
@@ -1046,12 +1122,12 @@ void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2
  * list, compared entry by entry: every plan pointer with the plan's serial
  * number (a plan created at a freed plan's address gets another serial) and
  * n, every matrix pointer and M, plus kind, direction, precision, engine and
- * tuning.  Destroying a plan the cached schedule uses drops the schedule.
+ * tuning (L, W and the band-systolic part size).  Destroying a plan the cached schedule uses drops the schedule.
  */
 typedef struct {
     pthread_mutex_t lock;
     ft_schedule *sched;
-    int count, kind, dir, precision, L, W, engine;
+    int count, kind, dir, precision, L, W, engine, part;
     const ft_rotation_plan **plans;
     unsigned long long *serial;
     int *n;
@@ -1059,7 +1135,7 @@ typedef struct {
     int *M;
 } ft_batch_cache;
 
-static ft_batch_cache g_batch = {PTHREAD_MUTEX_INITIALIZER, NULL, 0, 0, 0, 0, 0, 0, 0, NULL, NULL, NULL, NULL, NULL};
+static ft_batch_cache g_batch = {PTHREAD_MUTEX_INITIALIZER, NULL, 0, 0, 0, 0, 0, 0, 0, 0, NULL, NULL, NULL, NULL, NULL};
 
 static void batch_cache_clear_locked(void) {
     if (g_batch.sched) {
@@ -1091,9 +1167,10 @@ void ftamd_batch_forget_plan(const ft_rotation_plan *RP) {
 }
 
 static int batch_cache_hit(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
-                           void *const *dA, const int *M, int count, int L, int W, int engine) {
+                           void *const *dA, const int *M, int count, int L, int W, int engine, int part) {
     if (!g_batch.sched || g_batch.count != count || g_batch.kind != kind || g_batch.dir != direction ||
-        g_batch.precision != precision || g_batch.L != L || g_batch.W != W || g_batch.engine != engine)
+        g_batch.precision != precision || g_batch.L != L || g_batch.W != W || g_batch.engine != engine ||
+        g_batch.part != part)
         return 0;
     for (int i = 0; i < count; i++)
         if (g_batch.plans[i] != plans[i] || g_batch.serial[i] != plans[i]->serial || g_batch.n[i] != plans[i]->n ||
@@ -1103,7 +1180,8 @@ static int batch_cache_hit(const ft_rotation_plan *const *plans, int kind, int d
 }
 
 static int batch_cache_store(ft_schedule *S, const ft_rotation_plan *const *plans, int kind, int direction,
-                             int precision, void *const *dA, const int *M, int count, int L, int W, int engine) {
+                             int precision, void *const *dA, const int *M, int count, int L, int W, int engine,
+                             int part) {
     size_t c = (size_t)count;
     g_batch.plans = malloc(sizeof(*g_batch.plans) * c);
     g_batch.serial = malloc(sizeof(*g_batch.serial) * c);
@@ -1129,6 +1207,7 @@ static int batch_cache_store(ft_schedule *S, const ft_rotation_plan *const *plan
     g_batch.precision = precision;
     g_batch.L = L;
     g_batch.W = W;
+    g_batch.part = part;
     g_batch.engine = engine;
     return FTAMD_OK;
 }
@@ -1179,11 +1258,15 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     }
     FtamdSysBand *bands = NULL;
     FtamdSysChain *ch = NULL;
-    int nb = 0, nch = 0;
+    int nb = 0, nch = 0, nprog = 0;
     const void *tab_base = NULL;
-    if (rc == FTAMD_OK) rc = ftamd_sys_batch(plans, tabs, dA, M, count, &bands, &nb, &ch, &nch, &tab_base);
+    int nmax = 0;
+    for (int i = 0; i < count; i++)
+        if (plans[i]->n > nmax) nmax = plans[i]->n;
+    const int maxpart = ftamd_sys_maxpart(kind, nmax, precision);
+    if (rc == FTAMD_OK)
+        rc = ftamd_sys_batch(plans, tabs, dA, M, count, maxpart, &bands, &nb, &ch, &nch, &nprog, &tab_base);
     free(tabs);
-    if (rc == FTAMD_EINVAL) return FTAMD_OK; /* a chain needs more bands than a workgroup has waves */
     if (rc != FTAMD_OK) {
         free(bands);
         free(ch);
@@ -1192,7 +1275,8 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     ft_schedule *S = calloc(1, sizeof(ft_schedule));
     if (!S) rc = FTAMD_ENOMEM;
     void *dbands = NULL;
-    if (rc == FTAMD_OK) rc = sys_upload(S, ch, nch, bands, bands, sizeof(FtamdSysBand) * (size_t)nb, &dbands);
+    if (rc == FTAMD_OK)
+        rc = sys_upload(S, ch, nch, nprog, bands, bands, sizeof(FtamdSysBand) * (size_t)nb, &dbands);
     free(bands);
     free(ch);
     if (rc != FTAMD_OK) {
@@ -1202,6 +1286,7 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     S->sys = 1;
     S->dir = direction;
     S->precision = precision;
+    S->maxpart = maxpart;
     S->L = S0;
     FtamdSysLaunch *a = &S->sys_launch;
     a->bands = (const FtamdSysBand *)dbands;
@@ -1266,13 +1351,15 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
     int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
     if (rc != FTAMD_OK || count == 0) return rc;
     int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W(), engine = KNOB_GET(g_engine);
+    int part = ftamd_sys_maxpart(kind, big->n, precision);
     pthread_mutex_lock(&g_batch.lock);
     ft_schedule *S = g_batch.sched;
-    if (!batch_cache_hit(plans, kind, direction, precision, dA, M, count, L, W, engine)) {
+    if (!batch_cache_hit(plans, kind, direction, precision, dA, M, count, L, W, engine, part)) {
         batch_cache_clear_locked();
         S = NULL;
         rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &S);
-        if (rc == FTAMD_OK) rc = batch_cache_store(S, plans, kind, direction, precision, dA, M, count, L, W, engine);
+        if (rc == FTAMD_OK)
+            rc = batch_cache_store(S, plans, kind, direction, precision, dA, M, count, L, W, engine, part);
         if (rc != FTAMD_OK) {
             pthread_mutex_unlock(&g_batch.lock);
             return rc;

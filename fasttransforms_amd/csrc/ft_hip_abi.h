@@ -77,10 +77,13 @@ typedef struct {
  * columns shares (sph: one order parity and one row parity; tri: all; disk:
  * one order parity), numbered k = 0 .. K-1 in application order.  A band is
  * FTAMD_SYS_LANES * S consecutive sequences of a class; a chain is the bands
- * one group of FTAMD_SYS_COLS columns needs, one wave per band. */
+ * one group of FTAMD_SYS_COLS columns needs.  A chain runs in parts of at
+ * most FTAMD_SYS_WAVES bands, one wave per band; consecutive parts of a
+ * chain run in different workgroups of one XCD and hand the rows over
+ * through the matrix itself (a progress counter per hand-off). */
 #define FTAMD_SYS_LANES 16   /* lanes (sequence slots) of a band row */
 #define FTAMD_SYS_COLS 8     /* columns of a chain: 4 rows of 16 lanes x 2 per lane */
-#define FTAMD_SYS_WAVES 16   /* most bands of a chain (waves of its workgroup) */
+#define FTAMD_SYS_WAVES 16   /* most bands of a chain part (waves of its workgroup) */
 #define FTAMD_SYS_UNROLL 8   /* steps per loop iteration (a band's steps are a multiple) */
 #define FTAMD_SYS_PREFETCH 16 /* most steps of table entries fetched ahead; tables carry that many spare steps */
 
@@ -99,32 +102,50 @@ typedef struct {
     int rstep, rpar;  /* matrix row of class row e: rstep*e + rpar */
     int prio;         /* wave issue priority 0..3 (longest chains highest) */
     int lda;          /* column stride of its matrix in elements */
+    int split_in;     /* progress counter of the part before it (its first band reads that part's
+                         rows from A as they are published), -1 for a chain's first part */
+    int split_out;    /* progress counter its last band publishes (it stores every row it emits
+                         to A), -1 for a chain's last part */
     int col[FTAMD_SYS_COLS];  /* [row][c]: column of A, -1 if none */
     int glo[FTAMD_SYS_COLS];  /* the column applies the sequences at positions glo .. ghi */
     int ghi[FTAMD_SYS_COLS];
     long long aoff;   /* element offset of the chain's matrix from the launch's A */
 } FtamdSysChain;
 
-/* One chain of a workgroup's list: its bands run on waves wave0 .. wave0+nb-1.
- * A workgroup streams the chains of its list one after another through its
- * waves (FTAMD_SYS_WAVES of them). */
+/* One chain part of a slot's list: its bands run on waves wave0 .. wave0+nb-1.
+ * A slot's workgroup streams the parts of its list one after another through
+ * its waves (FTAMD_SYS_WAVES of them). */
 typedef struct {
     int chain;        /* index in the chain array */
     int wave0;        /* wave of its first band */
 } FtamdSysItem;
 
+/* Control words of a launch, in one 256-byte area (zero at upload; the
+ * kernel's last workgroup resets all but the error word): */
+#define FTAMD_SYS_CTL_BYTES 256
+#define FTAMD_SYS_CTL_ERR 0        /* int: FTAMD_ERR_* bits */
+#define FTAMD_SYS_CTL_DONE 1       /* unsigned: workgroups finished */
+#define FTAMD_SYS_CTL_COMPLETED 2  /* unsigned: slots run */
+#define FTAMD_SYS_CTL_TICKET 16    /* unsigned[FTAMD_MAX_XCD]: next slot of each XCD */
+
 typedef struct {
     const FtamdSysBand *bands;
     const FtamdSysChain *chains;
     int nchains;
-    const FtamdSysItem *items;  /* every workgroup's chain list, one after another */
-    const int *wg_items;        /* [nwg+1]: workgroup w runs items wg_items[w] .. wg_items[w+1]-1 */
-    int nwg;                    /* workgroups (persistent: about one per CU) */
+    const FtamdSysItem *items;  /* every slot's list of chain parts, one after another */
+    const int *slot_items;      /* [nslots+1]: slot s runs items slot_items[s] .. slot_items[s+1]-1 */
+    const int *xcd_slot;        /* [nxcd+1]: the slots of XCD x are xcd_slot[x] .. xcd_slot[x+1]-1; a
+                                   part that waits on another runs in a later slot of the same XCD */
+    int nslots;
+    int nxcd;
+    int grid;                   /* workgroups: nxcd * (most slots of one XCD) */
     int rowmask;      /* both columns of every lane row apply the same sequences (sph, disk) */
     const void *tab;  /* base of the band tables (bands address them by entry offsets from it) */
     void *A;          /* base of the matrices (chains address them by element offsets from it) */
-    int *err;         /* FTAMD_ERR_* bits */
-    int spin_limit;   /* polls before a wait between bands gives up (FTAMD_ERR_TIMEOUT) */
+    int *ctl;         /* control words (FTAMD_SYS_CTL_*) */
+    int *prog;        /* [nprog] rows published by each split part, zero at launch; reset by the kernel */
+    int nprog;
+    int spin_limit;   /* polls before a wait gives up (FTAMD_ERR_TIMEOUT) */
     unsigned long long *trace;  /* diagnostic per-wave timeline (instrumented builds only), else NULL */
 } FtamdSysLaunch;
 

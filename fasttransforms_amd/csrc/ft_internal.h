@@ -28,6 +28,7 @@ typedef struct ft_schedule {
     /* band-systolic engine (sys != 0): chains on the device, launch record */
     int sys;
     int batch;         /* a batch schedule (grid capped at 2 workgroups per CU by default) */
+    int maxpart;       /* bands per chain part it was built with (part of its key) */
     FtamdSysLaunch sys_launch;
     void *d_sys;
     double flops;            /* algorithmic flops (6 per rotation) */
@@ -69,6 +70,7 @@ void ftamd_schedule_free_device(ft_schedule *S);
 void ftamd_schedule_release(ft_schedule *S);
 void ftamd_batch_forget_plan(const ft_rotation_plan *RP);
 int ftamd_engine(void);
+int ftamd_sys_maxpart(int kind, int n, int precision);
 
 /* ft_systolic.c */
 int ftamd_sys_choose_S(int kind, int n, int precision);
@@ -77,14 +79,16 @@ void ftamd_sys_tables_free(ft_sys_tables *T);
 int ftamd_sys_tables_S(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
-int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count);
-int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nwg, FtamdSysItem *items,
-                     int *wg_items, long *ws_load, int *ws_chain);
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int maxpart, FtamdSysChain **out,
+                     int *count, int *nprog);
+int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nxcd, int nsx,
+                     FtamdSysItem *items, int *slot_items, int *xcd_slot, long *ws_load, int *ws_chain);
 const FtamdSysBand *ftamd_sys_tables_host_bands(const ft_sys_tables *T);
 int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
                           int *count);
+int ftamd_sys_split(FtamdSysChain *buf, int nfull, int cap, int maxpart, int *count, int *nprog);
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
-                    const int *M, int count, FtamdSysBand **bands, int *nbands, FtamdSysChain **chains,
-                    int *nchains, const void **tab_base);
+                    const int *M, int count, int maxpart, FtamdSysBand **bands, int *nbands, FtamdSysChain **chains,
+                    int *nchains, int *nprog, const void **tab_base);
 
 #endif /* FT_INTERNAL_H */

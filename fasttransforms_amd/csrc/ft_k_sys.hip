@@ -17,8 +17,11 @@ extern "C" int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int 
     else if (S == 2) fn = rm ? ftsys::sys_kernel<float, 2, true> : ftsys::sys_kernel<float, 2, false>;
     else if (S == 4) fn = rm ? ftsys::sys_kernel<float, 4, true> : ftsys::sys_kernel<float, 4, false>;
     if (!fn) return FTAMD_EINVAL;
-    if (a->nchains <= 0 || a->nwg <= 0) return FTAMD_OK;
-    hipLaunchKernelGGL(fn, dim3(a->nwg), dim3(ftsys::kWaves * 64), 0, (hipStream_t)stream, *a);
+    if (a->nchains <= 0) return FTAMD_OK;
+    // the kernel indexes xcd_slot[0 .. nxcd] and one ticket per XCD
+    if (a->nxcd < 1 || a->nxcd > FTAMD_MAX_XCD || a->grid < 1 || a->nslots < 1 || !a->ctl || (a->nprog > 0 && !a->prog))
+        return FTAMD_EINVAL;
+    hipLaunchKernelGGL(fn, dim3(a->grid), dim3(ftsys::kWaves * 64), 0, (hipStream_t)stream, *a);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) return FTAMD_OK;
     return e == hipErrorOutOfMemory ? FTAMD_ENOMEM : FTAMD_EDEVICE;

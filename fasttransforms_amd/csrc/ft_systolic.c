@@ -98,18 +98,13 @@ void ftamd_sys_tables_free(ft_sys_tables *T) {
     free(T);
 }
 
-/* Sequences per band lane for a plan: the smallest S whose chains fit in
- * FTAMD_SYS_WAVES bands, 0 if none does. */
+/* Sequences per band lane of the plan's tables: 2, or 4 where the build has
+ * no S = 2 engine for the precision (0 if neither). */
 int ftamd_sys_choose_S(int kind, int n, int precision) {
-    sys_class cl[4];
-    int nc = make_classes(kind, n, cl);
-    static const int opts[] = {2, 4};
-    for (int o = 0; o < 2; o++) {
-        int B = FTAMD_SYS_LANES * opts[o], ok = 1;
-        for (int c = 0; c < nc; c++)
-            if ((cl[c].K + B - 1) / B > FTAMD_SYS_WAVES) ok = 0;
-        if (ok && ftamd_hip_sys_supported(opts[o], precision)) return opts[o];
-    }
+    (void)kind;
+    (void)n;
+    if (ftamd_hip_sys_supported(2, precision)) return 2;
+    if (ftamd_hip_sys_supported(4, precision)) return 4;
     return 0;
 }
 
@@ -237,13 +232,12 @@ static void sort_chains(FtamdSysChain *c, int n) {
     }
 }
 
-/* Chains of an n x M matrix: per class, the columns in groups of 8 (four
- * orders of the class's parity x (cos, sin) columns for sph and disk, eight
- * consecutive orders for tri), each with the bands its columns need, written
- * to buf (at most cap records, longest first) without allocating.
- * FTAMD_EINVAL if a chain needs more bands than a workgroup has waves,
- * FTAMD_ENOMEM if more than cap chains are needed (*count then holds the
- * number needed). */
+/* Whole chains of an n x M matrix: per class, the columns in groups of 8
+ * (four orders of the class's parity x (cos, sin) columns for sph and disk,
+ * eight consecutive orders for tri), each with the bands its columns need,
+ * written to buf (at most cap records, longest first) without allocating.
+ * The records are not split yet (ftamd_sys_split).  FTAMD_ENOMEM if more
+ * than cap chains are needed (*count then holds the number needed). */
 int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
                           int *count) {
     const int kind = RP->kind, B = FTAMD_SYS_LANES * T->S;
@@ -293,10 +287,9 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
             x.dir = T->dir;
             x.rstep = cl->rstep;
             x.rpar = cl->rpar;
-            x.prio = x.nb >= 12 ? 3 : x.nb >= 8 ? 2 : x.nb >= 4 ? 1 : 0;
             x.lda = RP->n;
             x.aoff = 0;
-            if (x.nb > FTAMD_SYS_WAVES) return FTAMD_EINVAL;
+            x.split_in = x.split_out = -1;
             if (nch < cap) buf[nch] = x;
             nch++;
         }
@@ -307,66 +300,136 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
     return FTAMD_OK;
 }
 
-/* Deals chains (sorted longest first) to nwg persistent workgroups and
- * places each on a run of its workgroup's waves.  Chain by chain: the
- * workgroup with the least work so far (work = steps of all bands, the
- * longest-processing-time rule), then the wave offset that keeps that
- * workgroup's busiest wave least busy (its waves stream their bands of
- * successive chains one after another).  Writes items[nch] grouped by
- * workgroup (each group in dealing order) and wg_items[nwg+1]; ws holds
- * nwg*(1+FTAMD_SYS_WAVES) longs and 2*nch ints of workspace.  No allocation. */
-int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nwg, FtamdSysItem *items,
-                     int *wg_items, long *ws_load, int *ws_chain) {
-    long *wg_load = ws_load, *wave_load = ws_load + nwg;
-    int *chain_wg = ws_chain, *chain_w0 = ws_chain + nch;
-    for (int w = 0; w < nwg * (1 + FTAMD_SYS_WAVES); w++) ws_load[w] = 0;
-    for (int c = 0; c < nch; c++) {
-        const int nb = ch[c].nb;
-        const FtamdSysBand *bd = bands + ch[c].band0;
-        long work = 0;
-        for (int k = 0; k < nb; k++) work += bd[k].steps;
-        int best = 0;
-        for (int w = 1; w < nwg; w++)
-            if (wg_load[w] < wg_load[best]) best = w;
-        long *wl = wave_load + (long)best * FTAMD_SYS_WAVES;
-        int o_best = FTAMD_SYS_WAVES - nb;
-        long m_best = -1;
-        for (int o = FTAMD_SYS_WAVES - nb; o >= 0; o--) {
-            long m = 0;
-            for (int k = 0; k < nb; k++)
-                if (wl[o + k] + bd[k].steps > m) m = wl[o + k] + bd[k].steps;
-            if (m_best < 0 || m < m_best) {
-                m_best = m;
-                o_best = o;
-            }
-        }
-        for (int k = 0; k < nb; k++) wl[o_best + k] += bd[k].steps;
-        wg_load[best] += work;
-        chain_wg[c] = best;
-        chain_w0[c] = o_best;
+/* Parts of a chain of nb bands with parts of at most maxpart bands. */
+static int split_parts(int nb, int maxpart) { return (nb + maxpart - 1) / maxpart; }
+
+/* Splits whole chains (buf[0 .. nfull-1], longest first) in place into
+ * parts of at most maxpart (<= FTAMD_SYS_WAVES) bands, as equal as
+ * possible: the parts of a chain stay together, in chain order, linked by
+ * progress counters 0 .. *nprog-1.  Sets each part's issue priority from its
+ * whole chain's length.  FTAMD_ENOMEM if the parts need more than cap records
+ * (*count then holds the number needed). */
+int ftamd_sys_split(FtamdSysChain *buf, int nfull, int cap, int maxpart, int *count, int *nprog) {
+    if (maxpart < 1) maxpart = 1;
+    if (maxpart > FTAMD_SYS_WAVES) maxpart = FTAMD_SYS_WAVES;
+    int total = 0, links = 0;
+    for (int c = 0; c < nfull; c++) {
+        int np = split_parts(buf[c].nb, maxpart);
+        total += np;
+        links += np - 1;
     }
-    for (int w = 0; w <= nwg; w++) wg_items[w] = 0;
-    for (int c = 0; c < nch; c++) wg_items[chain_wg[c] + 1]++;
-    for (int w = 0; w < nwg; w++) wg_items[w + 1] += wg_items[w];
-    for (int w = 0; w < nwg; w++) ws_load[w] = wg_items[w]; /* next free item of each workgroup */
-    for (int c = 0; c < nch; c++) {
-        long at = ws_load[chain_wg[c]]++;
-        items[at].chain = c;
-        items[at].wave0 = chain_w0[c];
+    *count = total;
+    *nprog = links;
+    if (total > cap) return FTAMD_ENOMEM;
+    /* from the back, so that no record is overwritten before it is read */
+    int at = total, link = links;
+    for (int c = nfull - 1; c >= 0; c--) {
+        const FtamdSysChain x = buf[c];
+        const int np = split_parts(x.nb, maxpart);
+        const int prio = x.nb >= 12 ? 3 : x.nb >= 8 ? 2 : x.nb >= 4 ? 1 : 0;
+        at -= np;
+        link -= np - 1;
+        int b = 0;
+        for (int k = 0; k < np; k++) {
+            FtamdSysChain y = x;
+            int nbk = x.nb / np + (k < x.nb % np ? 1 : 0);
+            y.band0 = x.band0 + b;
+            y.nb = nbk;
+            y.prio = prio;
+            y.split_in = k > 0 ? link + k - 1 : -1;
+            y.split_out = k < np - 1 ? link + k : -1;
+            buf[at + k] = y;
+            b += nbk;
+        }
     }
     return FTAMD_OK;
 }
 
-/* The same in a malloc'd array the caller frees. */
-int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
+/* Deals chain parts to slots: nxcd XCDs of nsx slots each (slot s of XCD x
+ * is x*nsx + s; the kernel runs an XCD's slots in order, each in a
+ * workgroup of that XCD).  Chain by chain (longest first): the XCD with the
+ * least work so far, then part by part the slot with the least work among
+ * those after the previous part's (a part waits only on parts in earlier
+ * slots of its XCD), then the wave offset that keeps that slot's busiest wave
+ * least busy (its waves stream their bands of successive parts one after
+ * another).  Work = steps of all bands (the longest-processing-time rule).
+ * Writes items[nch] grouped by slot, slot_items[nxcd*nsx+1] and
+ * xcd_slot[nxcd+1]; ws holds nxcd*nsx*(1+FTAMD_SYS_WAVES) + nxcd longs and
+ * 2*nch ints of workspace.  FTAMD_EINVAL if a chain has more parts than an
+ * XCD has slots.  No allocation. */
+int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nxcd, int nsx,
+                     FtamdSysItem *items, int *slot_items, int *xcd_slot, long *ws_load, int *ws_chain) {
+    const int nslots = nxcd * nsx;
+    long *slot_load = ws_load, *wave_load = ws_load + nslots, *xcd_load = wave_load + (long)nslots * FTAMD_SYS_WAVES;
+    int *chain_slot = ws_chain, *chain_w0 = ws_chain + nch;
+    for (long w = 0; w < (long)nslots * (1 + FTAMD_SYS_WAVES) + nxcd; w++) ws_load[w] = 0;
+    for (int c = 0; c < nch;) {
+        /* the parts of one chain: c .. e-1 */
+        int e = c + 1;
+        while (e < nch && ch[e].split_in >= 0 && ch[e].split_in == ch[e - 1].split_out) e++;
+        const int np = e - c;
+        if (np > nsx) return FTAMD_EINVAL;
+        long chain_work = 0;
+        for (int p = c; p < e; p++)
+            for (int k = 0; k < ch[p].nb; k++) chain_work += bands[ch[p].band0 + k].steps;
+        int x = 0;
+        for (int y = 1; y < nxcd; y++)
+            if (xcd_load[y] < xcd_load[x]) x = y;
+        xcd_load[x] += chain_work;
+        int lo = 0;
+        for (int p = c; p < e; p++) {
+            const int nb = ch[p].nb;
+            const FtamdSysBand *bd = bands + ch[p].band0;
+            long work = 0;
+            for (int k = 0; k < nb; k++) work += bd[k].steps;
+            /* leave a later slot for each part still to come */
+            const int hi = nsx - (e - 1 - p);
+            int best = lo;
+            for (int s = lo + 1; s < hi; s++)
+                if (slot_load[x * nsx + s] < slot_load[x * nsx + best]) best = s;
+            const int slot = x * nsx + best;
+            long *wl = wave_load + (long)slot * FTAMD_SYS_WAVES;
+            int o_best = FTAMD_SYS_WAVES - nb;
+            long m_best = -1;
+            for (int o = FTAMD_SYS_WAVES - nb; o >= 0; o--) {
+                long m = 0;
+                for (int k = 0; k < nb; k++)
+                    if (wl[o + k] + bd[k].steps > m) m = wl[o + k] + bd[k].steps;
+                if (m_best < 0 || m < m_best) {
+                    m_best = m;
+                    o_best = o;
+                }
+            }
+            for (int k = 0; k < nb; k++) wl[o_best + k] += bd[k].steps;
+            slot_load[slot] += work;
+            chain_slot[p] = slot;
+            chain_w0[p] = o_best;
+            lo = best + 1;
+        }
+        c = e;
+    }
+    for (int s = 0; s <= nslots; s++) slot_items[s] = 0;
+    for (int c = 0; c < nch; c++) slot_items[chain_slot[c] + 1]++;
+    for (int s = 0; s < nslots; s++) slot_items[s + 1] += slot_items[s];
+    for (int s = 0; s < nslots; s++) ws_load[s] = slot_items[s]; /* next free item of each slot */
+    for (int c = 0; c < nch; c++) {
+        long at = ws_load[chain_slot[c]]++;
+        items[at].chain = c;
+        items[at].wave0 = chain_w0[c];
+    }
+    for (int y = 0; y <= nxcd; y++) xcd_slot[y] = y * nsx;
+    return FTAMD_OK;
+}
+
+/* Whole chains in a malloc'd array the caller frees. */
+static int chains_full(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
     *out = NULL;
     *count = 0;
     int need = 0;
-    int rc = ftamd_sys_chains_fill(RP, T, M, NULL, 0, &need);
-    if (rc == FTAMD_EINVAL) return rc;
+    ftamd_sys_chains_fill(RP, T, M, NULL, 0, &need);
     FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)(need > 0 ? need : 1));
     if (!ch) return FTAMD_ENOMEM;
-    rc = ftamd_sys_chains_fill(RP, T, M, ch, need, count);
+    int rc = ftamd_sys_chains_fill(RP, T, M, ch, need, count);
     if (rc != FTAMD_OK) {
         free(ch);
         *count = 0;
@@ -376,17 +439,48 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
     return FTAMD_OK;
 }
 
+/* Splits the malloc'd whole chains *ch (n of them) into parts, growing the
+ * array as needed. */
+static int split_alloc(FtamdSysChain **ch, int n, int maxpart, int *count, int *nprog) {
+    int need = 0;
+    if (ftamd_sys_split(*ch, n, 0, maxpart, &need, nprog) == FTAMD_OK && need == 0) {
+        *count = 0;
+        return FTAMD_OK;
+    }
+    FtamdSysChain *grown = realloc(*ch, sizeof(FtamdSysChain) * (size_t)(need > n ? need : n));
+    if (!grown) return FTAMD_ENOMEM;
+    *ch = grown;
+    return ftamd_sys_split(*ch, n, need, maxpart, count, nprog);
+}
+
+/* The chain parts of an n x M matrix (longest chains first, parts of at most
+ * maxpart bands) in a malloc'd array the caller frees; *nprog progress
+ * counters link them. */
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int maxpart, FtamdSysChain **out,
+                     int *count, int *nprog) {
+    int n = 0;
+    *nprog = 0;
+    int rc = chains_full(RP, T, M, out, &n);
+    if (rc == FTAMD_OK) rc = split_alloc(out, n, maxpart, count, nprog);
+    if (rc != FTAMD_OK) {
+        free(*out);
+        *out = NULL;
+        *count = 0;
+    }
+    return rc;
+}
+
 /* Bands and chains of a batch of transforms of one kind, direction,
  * precision and S (every plan's tables built): the bands of each distinct
  * plan once, their table offsets taken from the first plan's tables, the
  * chains of every transform with its bands, matrix offset from dA[0] and
  * column stride, longest first.  FTAMD_EINVAL if a chain does not fit. */
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
-                    const int *M, int count, FtamdSysBand **bands_out, int *nbands_out, FtamdSysChain **chains_out,
-                    int *nchains_out, const void **tab_base) {
+                    const int *M, int count, int maxpart, FtamdSysBand **bands_out, int *nbands_out,
+                    FtamdSysChain **chains_out, int *nchains_out, int *nprog_out, const void **tab_base) {
     *bands_out = NULL;
     *chains_out = NULL;
-    *nbands_out = *nchains_out = 0;
+    *nbands_out = *nchains_out = *nprog_out = 0;
     *tab_base = count > 0 ? tabs[0]->d_tab : NULL;
     if (count <= 0) return FTAMD_OK;
     const long long pair = tabs[0]->precision == 0 ? 2 * (long long)sizeof(double) : 2 * (long long)sizeof(float);
@@ -421,7 +515,7 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
         while (seen[k] != tabs[i]) k++;
         FtamdSysChain *ch = NULL;
         int nch = 0;
-        rc = ftamd_sys_chains(plans[i], tabs[i], M[i], &ch, &nch);
+        rc = chains_full(plans[i], tabs[i], M[i], &ch, &nch);
         if (rc == FTAMD_OK && nall + nch > cap) {
             cap = 2 * (nall + nch) + 16;
             FtamdSysChain *grown = realloc(all, sizeof(FtamdSysChain) * (size_t)cap);
@@ -443,23 +537,32 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
         return rc;
     }
     sort_chains(all, nall);
+    int nparts = 0, nprog = 0;
+    rc = all ? split_alloc(&all, nall, maxpart, &nparts, &nprog) : FTAMD_OK;
+    if (rc != FTAMD_OK) {
+        free(bands);
+        free(all);
+        return rc;
+    }
     *bands_out = bands;
     *nbands_out = nb;
     *chains_out = all;
-    *nchains_out = nall;
+    *nchains_out = nparts;
+    *nprog_out = nprog;
     return FTAMD_OK;
 }
 
-/* Diagnostics (no device work): the FP64 bands, table and chains the engine
- * would run for an n x M matrix, in malloc'd host arrays the caller frees.
+/* Diagnostics (no device work): the FP64 bands, table and chain parts (of
+ * at most maxpart bands) the engine would run for an n x M matrix, in malloc'd host arrays the caller frees.
  * Lets the host-side construction be checked on a machine without a GPU. */
-int ftamd_sys_debug_host(const ft_rotation_plan *RP, int dir, int S, int M, FtamdSysBand **bands, int *nbands,
-                         double **tab, long long *nentries, FtamdSysChain **chains, int *nchains) {
+int ftamd_sys_debug_host(const ft_rotation_plan *RP, int dir, int S, int M, int maxpart, FtamdSysBand **bands,
+                         int *nbands, double **tab, long long *nentries, FtamdSysChain **chains, int *nchains) {
     ft_sys_tables T;
     memset(&T, 0, sizeof(T));
     void *host = NULL;
     int rc = tables_host(RP, dir, 0, S, &T, &host, nentries);
-    if (rc == FTAMD_OK) rc = ftamd_sys_chains(RP, &T, M, chains, nchains);
+    int nprog = 0;
+    if (rc == FTAMD_OK) rc = ftamd_sys_chains(RP, &T, M, maxpart, chains, nchains, &nprog);
     if (rc != FTAMD_OK) {
         free(T.bands);
         free(host);

@@ -30,10 +30,20 @@
 //  * A chain runs its bands as waves of one workgroup; band b+1 reads band
 //    b's output stream through an LDS ring as it is produced, so a chain's
 //    bands overlap and the chain's latency is about one band's stream plus
-//    (S+1)*LANES steps per band.  A workgroup runs several chains side by
-//    side when their bands fit its 16 waves (the host packs them).  The first
-//    band reads A; rows that leave the chain are stored to A once per 8-step
-//    block by the band that emits them.
+//    (S+1)*LANES steps per band.  The first band reads A; rows that leave the
+//    chain are stored to A once per 8-step block by the band that emits them.
+//  * A long chain runs in parts of at most a set number of bands, each part
+//    on its own workgroup: the last band of a part stores every row it emits
+//    to A and publishes a progress counter, and the first band of the next
+//    part reads those rows back from A as they are published.  This spreads a
+//    long chain's waves over several CUs and lifts the bound of 16 bands (one
+//    workgroup's waves) per chain.
+//  * Workgroups are persistent (about one per CU).  The host deals the parts
+//    to slots, per XCD, and a workgroup takes its XCD's slots in order; a
+//    slot's parts stream through the workgroup's waves one after another.
+//    The parts of one chain run in increasing slots of one XCD, so a waiting
+//    part only ever waits on a slot that a running workgroup holds or has
+//    finished, and the rows they hand over stay in that XCD's L2.
 //
 // Arithmetic: the reference's rotation in fused form, x' = fma(c, x, s*y),
 // y' = fma(c, y, -(s*x)) (src/rotations/rotations_AVX512F.c:4-15).  With
@@ -108,6 +118,7 @@ struct Engine {
         int avail[kWaves];                 // positions of ring g written by wave g-1 (exclusive end)
         int freed[kWaves];                 // positions of ring g wave g has read (exclusive end)
         int abort_flag;
+        int slot;                          // the workgroup's current slot (ticket)
     };
 
     // The table entries of one lane for one step: S (s, c) pairs.
@@ -206,7 +217,8 @@ struct Engine {
     }
     // Lane 15 of each row of 16 (the row's last sequence) to every lane of
     // the row (DPP row_newbcast:15).
-    __device__ static T row_end(T v) { return __builtin_amdgcn_update_dpp(v, v, 0x15f, 0xf, 0xf, false); }
+    // (Every lane has a source, so the old value is irrelevant: one DPP move.)
+    __device__ static T row_end(T v) { return __builtin_amdgcn_update_dpp(T(0), v, 0x15f, 0xf, 0xf, true); }
     __device__ static unsigned lds_addr(const void *p) { return (unsigned)(uintptr_t)p; }
 
     __device__ static void read_sc(Shared &sh, int w, int step, int i, SC &sc) {
@@ -215,30 +227,54 @@ struct Engine {
         for (int s = 0; s < S; ++s) sc.p[s] = p[s];
     }
 
+    // Progress counters of split chains.  The producing band stores its rows
+    // to A with device-scope stores and, once they are complete, publishes
+    // how many it has stored with a device-scope store; the consuming band
+    // polls the counter with a scalar load that bypasses the scalar cache
+    // (GLC), so that polling never waits on the vector-memory counter the
+    // table DMAs run on, and then reads the rows with device-scope loads.
+    // Both parts run on one XCD, so they meet in one L2.
+    __device__ static int poll_prog(const int *p) {
+        int v;
+        asm volatile("s_load_dword %0, %1, 0x0 glc\n\ts_waitcnt lgkmcnt(0)" : "=s"(v) : "s"(p) : "memory");
+        return v;
+    }
+    __device__ static bool wait_prog(Shared &sh, const int *p, int need, int &seen, int limit) {
+        for (int it = 0; seen < need; ++it) {
+            if (it >= limit || uniform(lds_ld(&sh.abort_flag))) return false;
+            if (it > 0) __builtin_amdgcn_s_sleep(2);
+            seen = uniform(poll_prog(p));
+        }
+        return true;
+    }
+
     // One step: rotate every (sequence, column) of the lane.  X0: the fresh
     // rows of the lane's first sequence (the last sequence of the lane before
     // emitted them at the previous step, or the stream at lane 0); Y: carried
     // rows; sequence s > 0 takes as its fresh row what sequence s-1 emits in
     // this step.
     // MASK 0: every column applies every sequence of the band; 1: the two
-    // columns of a lane row apply the same sequences (act[s][0]); 2: per
-    // column.  A sequence a column does not apply is the identity (0, 1).
+    // columns of a lane row apply the same sequences (mask [s][0]); 2: per
+    // column.  A sequence a column does not apply is the identity (0, 1):
+    // with m = 1 where the column applies the sequence and 0 where not, the
+    // applied pair is (s*m, fma(c, m, 1-m)), exact for both values of m and
+    // two FP64 operations where a select of two doubles takes four.
     template <int MASK>
     __device__ static void rotate(T (&Y)[S][kCols], T (&E)[S][kCols], const T (&X0)[kCols], const SC &sc,
-                                  const bool (&act)[S][kCols]) {
+                                  const T (&m)[S][kCols], const T (&m1)[S][kCols]) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const Pair<T> e = sc.pair(s);
-            const T sn_row = MASK == 1 ? (act[s][0] ? e.x : T(0)) : e.x;
-            const T cs_row = MASK == 1 ? (act[s][0] ? e.y : T(1)) : e.y;
+            const T sn_row = MASK == 1 ? e.x * m[s][0] : e.x;
+            const T cs_row = MASK == 1 ? fmaT<T>(e.y, m[s][0], m1[s][0]) : e.y;
 #pragma unroll
             for (int c = 0; c < kCols; ++c) {
                 const T x = s == 0 ? X0[c] : E[s - 1][c];
                 const T y = Y[s][c];
                 T sn = sn_row, cs = cs_row;
                 if (MASK == 2) {
-                    sn = act[s][c] ? e.x : T(0);
-                    cs = act[s][c] ? e.y : T(1);
+                    sn = e.x * m[s][c];
+                    cs = fmaT<T>(e.y, m[s][c], m1[s][c]);
                 }
                 Y[s][c] = fmaT<T>(cs, x, sn * y);
                 E[s][c] = fmaT<T>(cs, y, -(sn * x));
@@ -250,8 +286,10 @@ struct Engine {
     // workgroup.  Ring positions are counted across the chains a workgroup
     // streams: this band's input stream starts at position in_base of ring g,
     // its output (the next band's input) at out_base of ring g+1, both
-    // multiples of kZ.  Returns false if a wait expired (abort raised).
-    template <int MASK>
+    // multiples of kZ.  LINK: the band hands rows to or takes them from
+    // another part of its chain (through A).  Returns false if a wait expired
+    // (abort raised).
+    template <int MASK, bool LINK>
     __device__ static bool run_band(const FtamdSysLaunch &a, Shared &sh, const FtamdSysChain &ch, int w, int g,
                                     int lane, int in_base, int out_base) {
         const int r = lane >> 4, i = lane & 15;
@@ -266,6 +304,14 @@ struct Engine {
         const int Rnext = last ? -1 : uniform(a.bands[b + 1].R);
         // stream positions 0 .. npa-1 come from A, the rest from the previous band
         const int npa = first ? R + 1 : (dir == 0 ? R - Rprev : 0);
+        // a part after the first of its chain: its positions from npa_own on
+        // are the previous part's output position p - npa_own, published on
+        // counter split_in; the last band of a part before the last stores
+        // every row it emits and publishes them on counter split_out
+        const int split_in = LINK && first ? uniform(ch.split_in) : -1;
+        const int split_out = LINK && last ? uniform(ch.split_out) : -1;
+        const int npa_own = split_in >= 0 && dir == 0 ? R - uniform(a.bands[b - 1].R) : 0;
+        int prog_seen = 0, published = 0, pub = 0;
         // the next band's position of our output position p is p + npa_next;
         // outputs beyond the next band's stream leave the chain
         const int npa_next = last ? 0 : (dir == 0 ? Rnext - R : 0);
@@ -284,14 +330,16 @@ struct Engine {
 
         // with MASK 1 column 0 of a row carries the row's mask (column 1's
         // when column 0 is absent)
-        bool act[S][kCols];
+        T m[S][kCols], m1[S][kCols];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < kCols; ++c) {
                 const int q = gbase + S * i + s;
                 const int k = MASK == 1 && c == 0 && colv[0] < 0 ? 2 * r + 1 : 2 * r + c;
-                act[s][c] = q >= ch.glo[k] && q <= ch.ghi[k];
+                const bool on = q >= ch.glo[k] && q <= ch.ghi[k];
+                m[s][c] = on ? T(1) : T(0);
+                m1[s][c] = on ? T(0) : T(1);
             }
 
         T(*in_ring)[kRows][kCols] = sh.ring[g];
@@ -306,11 +354,22 @@ struct Engine {
         // ---- stream positions from A: lane i of each row fetches position
         // p0 + i of a chunk of 16.  (Loads and stores of A only make the
         // waits on the table loads stricter: see the output stores.)
-        auto fetch = [&](int p0, T (&v)[kCols]) {
+        auto fetch = [&](int p0, T (&v)[kCols]) -> bool {
             const int p = p0 + i;
+            if (split_in >= 0) {
+                const int need = min(p0 + kChunk - 1, R) - npa_own + 1;
+                if (need > 0 && !wait_prog(sh, a.prog + split_in, need, prog_seen, spin)) return false;
+#pragma unroll
+                for (int c = 0; c < kCols; ++c)
+                    v[c] = (p <= R && colv[c] >= 0)
+                               ? __hip_atomic_load(aptr(c, row_of(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : T(0);
+                return true;
+            }
 #pragma unroll
             for (int c = 0; c < kCols; ++c)
                 v[c] = (p < npa && p <= R && colv[c] >= 0) ? *aptr(c, row_of(p)) : T(0);
+            return true;
         };
         auto put = [&](int p0, const T (&v)[kCols]) {
             const int p = p0 + i;
@@ -324,9 +383,9 @@ struct Engine {
         T chunk[kCols];
         {
             T v[kCols];
-            fetch(0, v);
+            ok = fetch(0, v);
             put(0, v);
-            fetch(kChunk, chunk);
+            ok = ok && fetch(kChunk, chunk);
         }
 
         // ---- input prefetch: positions 0 .. kDin-1
@@ -374,7 +433,10 @@ struct Engine {
                 const int p0 = t0 + kChunk;
                 if (p0 < npa) {
                     put(p0, chunk);
-                    fetch(p0 + kChunk, chunk);
+                    if (!fetch(p0 + kChunk, chunk)) {
+                        ok = false;
+                        break;
+                    }
                 }
             }
             // every input position the block reads ahead is in the ring
@@ -442,7 +504,7 @@ struct Engine {
                 }
                 // the reads above are issued before this step's arithmetic
                 __builtin_amdgcn_sched_barrier(0);
-                rotate<MASK>(Y, E, X0, sc, act);
+                rotate<MASK>(Y, E, X0, sc, m, m1);
                 if (u % kSPD == kSPD - 1) {
                     // u is a constant once the loop is unrolled
                     switch (u + 1 - kSPD) {
@@ -459,11 +521,20 @@ struct Engine {
                     outv[c] = i == u ? b : outv[c];
                 }
             }
+            // a split part's last band publishes the rows the previous block
+            // stored: they are complete once at most this block's DMAs are
+            // outstanding (vector memory operations complete in issue order)
+            if (split_out >= 0 && pub > published) {
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kU / kSPD) : "memory");
+                if (lane == 0) __hip_atomic_store(a.prog + split_out, pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                published = pub;
+            }
             // lane i < kU of each row stores output position po0 + i: to the
-            // next band's ring up to ring_hi, else (leaving the chain) to its
-            // row of A; positions outside [0, R] carry no row.  (The global
-            // stores only lengthen the table waits that follow them: those
-            // wait for the DMAs issued before them, and loads return in order.)
+            // next band's ring up to ring_hi, else (leaving the chain or the
+            // part) to its row of A; positions outside [0, R] carry no row.
+            // (The global stores only lengthen the table waits that follow
+            // them: those wait for the DMAs issued before them, and loads
+            // return in order.)
             {
                 const int po = po0 + i;
                 if (i < kU && po >= 0 && po <= R) {
@@ -471,6 +542,12 @@ struct Engine {
                         T *dst = &out_ring[(po + npa_next) % kZ][r][0];
 #pragma unroll
                         for (int c = 0; c < kCols; ++c) lds_st(dst + c, outv[c]);
+                    } else if (split_out >= 0) {
+#pragma unroll
+                        for (int c = 0; c < kCols; ++c)
+                            if (colv[c] >= 0)
+                                __hip_atomic_store(aptr(c, row_of(po)), outv[c], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
                     } else {
 #pragma unroll
                         for (int c = 0; c < kCols; ++c)
@@ -478,6 +555,7 @@ struct Engine {
                     }
                 }
             }
+            pub = max(0, min(po0 + kU, R + 1));
             if (!ok) break;
             // publish what the block wrote and read
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -488,8 +566,13 @@ struct Engine {
 
         if (!ok) {
             fail(sh);
-            atomicOr(a.err, FTAMD_ERR_TIMEOUT);
+            atomicOr(a.ctl + FTAMD_SYS_CTL_ERR, FTAMD_ERR_TIMEOUT);
             return false;
+        }
+        if (split_out >= 0) {
+            // every row stored and complete
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (lane == 0) __hip_atomic_store(a.prog + split_out, R + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 #ifdef FTAMD_TRACE
         if (a.trace && lane == 0) {
@@ -512,10 +595,20 @@ struct Engine {
     // to whole rings, so that every stream starts at ring slot 0.
     __device__ static int span(int R) { return (R + kZ) / kZ * kZ; }
 
-    // A workgroup streams its chains (items) one after another through its
-    // waves: item k runs its chain's bands on waves wave0 .. wave0+nb-1, and
-    // every wave takes the items in list order, so a wave starts its band of
-    // the next chain as soon as it has finished its band of the previous one.
+    // XCD (accelerator complex die, with its own L2) this workgroup runs on.
+    __device__ static int xcd_id(int nxcd) {
+        const int id = (int)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));  // HW_REG_XCC_ID[3:0]
+        return nxcd > 1 ? id % nxcd : 0;
+    }
+
+    // A workgroup takes the slots of its XCD in order (tickets), and streams
+    // a slot's chain parts (items) one after another through its waves: item
+    // k runs its part's bands on waves wave0 .. wave0+nb-1, and every wave
+    // takes the items in list order, so a wave starts its band of the next
+    // part as soon as it has finished its band of the previous one.  A part
+    // that waits on another part waits on an earlier slot of the same XCD,
+    // which a running workgroup holds or has finished: the launch completes
+    // however many workgroups the hardware runs at once.
     __device__ static void run(const FtamdSysLaunch &a, Shared &sh) {
         {
             T *rp = &sh.ring[0][0][0][0];
@@ -526,40 +619,79 @@ struct Engine {
             }
             if (threadIdx.x == 0) sh.abort_flag = 0;
         }
-        __syncthreads();
         const int lane = threadIdx.x & 63;
         const int g = uniform(threadIdx.x >> 6);
-        const int it0 = uniform(a.wg_items[blockIdx.x]), it1 = uniform(a.wg_items[blockIdx.x + 1]);
-        int in_base = 0, out_base = 0;  // where the current chain's streams start in rings g and g+1
-        for (int it = it0; it < it1; ++it) {
-            const FtamdSysChain &ch = a.chains[uniform(a.items[it].chain)];
-            const int nb = uniform(ch.nb), b0 = uniform(ch.band0);
-            const int w = g - uniform(a.items[it].wave0);  // this wave's band of the chain, if any
-            if (w >= 0 && w < nb) {
-                switch (uniform(ch.prio)) {
-                    case 3: __builtin_amdgcn_s_setprio(3); break;
-                    case 2: __builtin_amdgcn_s_setprio(2); break;
-                    case 1: __builtin_amdgcn_s_setprio(1); break;
-                    default: __builtin_amdgcn_s_setprio(0); break;
+        const int x = uniform(xcd_id(a.nxcd));
+        const int s0 = uniform(a.xcd_slot[x]), ns = uniform(a.xcd_slot[x + 1]) - s0;
+        unsigned *ticket = (unsigned *)(a.ctl + FTAMD_SYS_CTL_TICKET) + x;
+        int in_base = 0, out_base = 0;  // where the current part's streams start in rings g and g+1
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) sh.slot = (int)atomicAdd(ticket, 1u);
+            __syncthreads();
+            const int t = uniform(lds_ld(&sh.slot));
+            if (t >= ns || uniform(lds_ld(&sh.abort_flag))) break;
+            const int slot = s0 + t;
+            const int it0 = uniform(a.slot_items[slot]), it1 = uniform(a.slot_items[slot + 1]);
+            for (int it = it0; it < it1; ++it) {
+                const FtamdSysChain &ch = a.chains[uniform(a.items[it].chain)];
+                const int nb = uniform(ch.nb), b0 = uniform(ch.band0);
+                const int w = g - uniform(a.items[it].wave0);  // this wave's band of the part, if any
+                if (w >= 0 && w < nb) {
+                    switch (uniform(ch.prio)) {
+                        case 3: __builtin_amdgcn_s_setprio(3); break;
+                        case 2: __builtin_amdgcn_s_setprio(2); break;
+                        case 1: __builtin_amdgcn_s_setprio(1); break;
+                        default: __builtin_amdgcn_s_setprio(0); break;
+                    }
+                    // a band needs masks if some column of the chain does
+                    // not apply some of its sequences; with ROWMASK one mask
+                    // per lane row serves both columns (sph and disk: the two
+                    // columns of a row are the cosine and sine columns of one
+                    // order, so they apply the same sequences)
+                    const int gbase = uniform(a.bands[b0 + w].gbase);
+                    const int glast = gbase + uniform(a.bands[b0 + w].nseq) - 1;
+                    bool masked = false;
+                    for (int k = 0; k < FTAMD_SYS_COLS; ++k)
+                        masked |= ch.col[k] >= 0 && (ch.glo[k] > gbase || ch.ghi[k] < glast);
+                    // bands that hand rows to or take rows from another
+                    // part of the chain run their own instantiation, so the
+                    // others carry none of that code
+                    const bool link = (w == 0 && uniform(ch.split_in) >= 0) || (w == nb - 1 && uniform(ch.split_out) >= 0);
+                    constexpr int MK = ROWMASK ? 1 : 2;
+                    const bool ok = link ? (!masked ? run_band<0, true>(a, sh, ch, w, g, lane, in_base, out_base)
+                                                    : run_band<MK, true>(a, sh, ch, w, g, lane, in_base, out_base))
+                                         : (!masked ? run_band<0, false>(a, sh, ch, w, g, lane, in_base, out_base)
+                                                    : run_band<MK, false>(a, sh, ch, w, g, lane, in_base, out_base));
+                    if (!ok) break;
+                    in_base += span(uniform(a.bands[b0 + w].R));
                 }
-                // a band needs masks if some column of the chain does not
-                // apply some of its sequences; with ROWMASK one mask per lane
-                // row serves both columns (sph and disk: the two columns of a
-                // row are the cosine and sine columns of one order, so they
-                // apply the same sequences)
-                const int gbase = uniform(a.bands[b0 + w].gbase);
-                const int glast = gbase + uniform(a.bands[b0 + w].nseq) - 1;
-                bool masked = false;
-                for (int k = 0; k < FTAMD_SYS_COLS; ++k)
-                    masked |= ch.col[k] >= 0 && (ch.glo[k] > gbase || ch.ghi[k] < glast);
-                const bool ok = !masked ? run_band<0>(a, sh, ch, w, g, lane, in_base, out_base)
-                                        : run_band<ROWMASK ? 1 : 2>(a, sh, ch, w, g, lane, in_base, out_base);
-                if (!ok) break;
-                in_base += span(uniform(a.bands[b0 + w].R));
+                if (w + 1 >= 0 && w + 1 < nb) out_base += span(uniform(a.bands[b0 + w + 1].R));
             }
-            if (w + 1 >= 0 && w + 1 < nb) out_base += span(uniform(a.bands[b0 + w + 1].R));
+            __builtin_amdgcn_s_setprio(0);
+            __syncthreads();
+            if (threadIdx.x == 0 && !lds_ld(&sh.abort_flag)) atomicAdd((unsigned *)(a.ctl + FTAMD_SYS_CTL_COMPLETED), 1u);
         }
-        __builtin_amdgcn_s_setprio(0);
+        // The last workgroup out checks that every slot ran and resets the
+        // tickets, counters and progress words for the next launch (or graph
+        // replay).
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned *done = (unsigned *)(a.ctl + FTAMD_SYS_CTL_DONE);
+            unsigned *completed = (unsigned *)(a.ctl + FTAMD_SYS_CTL_COMPLETED);
+            const unsigned d = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (d == gridDim.x - 1) {
+                if (__hip_atomic_load(completed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != (unsigned)a.nslots)
+                    atomicOr(a.ctl + FTAMD_SYS_CTL_ERR, FTAMD_ERR_INCOMPLETE);
+                for (int k = 0; k < a.nprog; ++k)
+                    __hip_atomic_store(a.prog + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int k = 0; k < a.nxcd; ++k)
+                    __hip_atomic_store((unsigned *)(a.ctl + FTAMD_SYS_CTL_TICKET) + k, 0u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(completed, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
     }
 };
 

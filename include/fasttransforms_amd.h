@@ -154,15 +154,19 @@ int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
 
 /* Engine of the whole-matrix executors (no reference counterpart; for tests
  * and measurements): FT_GPU_ENGINE_AUTO (default) runs the band-systolic
- * engine wherever a plan's chains fit it (at most 512 FP64 / 1024 FP32 rows
- * per chain stream, 16 bands per chain), the pipelined engine elsewhere;
- * FT_GPU_ENGINE_PIPELINE and FT_GPU_ENGINE_SYSTOLIC force one (the latter
- * fails with FT_GPU_EINVAL where it does not apply).  Batches run the
- * pipelined engine unless the systolic one is forced (then every transform
- * must fit it with one band width); single-vector kernels always run the
- * pipelined engine. */
+ * engine wherever the build has it for the precision, the pipelined engine
+ * elsewhere; FT_GPU_ENGINE_PIPELINE and FT_GPU_ENGINE_SYSTOLIC force one (the
+ * latter fails with FT_GPU_EINVAL where it does not apply).  Batches run the
+ * pipelined engine unless the systolic one is forced; single-vector kernels
+ * always run the pipelined engine. */
 enum { FT_GPU_ENGINE_AUTO = 0, FT_GPU_ENGINE_PIPELINE = 1, FT_GPU_ENGINE_SYSTOLIC = 2 };
 int ft_gpu_set_engine(int engine);
+
+/* Band-systolic tuning knob (no reference counterpart): most bands of a
+ * chain part (1..16).  A chain of more bands runs in parts on several
+ * workgroups of one XCD.  0 keeps the built-in choice.  Schedules are cached
+ * per value; results are bitwise the same for every value. */
+int ft_gpu_set_systolic(int maxpart);
 
 /* Polls after which a band-systolic wait between bands gives up and the
  * launch reports FT_GPU_ETIMEOUT (no reference counterpart; for tests of the

@@ -19,22 +19,24 @@ class Band(ctypes.Structure):
 
 class Chain(ctypes.Structure):
     _fields_ = [("band0", ctypes.c_int), ("nb", ctypes.c_int), ("dir", ctypes.c_int), ("rstep", ctypes.c_int),
-                ("rpar", ctypes.c_int), ("prio", ctypes.c_int), ("lda", ctypes.c_int), ("col", ctypes.c_int * 8),
+                ("rpar", ctypes.c_int), ("prio", ctypes.c_int), ("lda", ctypes.c_int), ("split_in", ctypes.c_int),
+                ("split_out", ctypes.c_int), ("col", ctypes.c_int * 8),
                 ("glo", ctypes.c_int * 8), ("ghi", ctypes.c_int * 8), ("aoff", ctypes.c_longlong)]
 
 
-def host_build(lib, plan_ptr, direction, S, M):
-    """(bands, table (entries x 2: s, c), chains) as the library builds them."""
+def host_build(lib, plan_ptr, direction, S, M, maxpart=16):
+    """(bands, table (entries x 2: s, c), chain parts of at most maxpart bands)
+    as the library builds them."""
     fn = lib.ftamd_sys_debug_host
     fn.restype = ctypes.c_int
-    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                    ctypes.POINTER(ctypes.POINTER(Band)), ctypes.POINTER(ctypes.c_int),
                    ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_longlong),
                    ctypes.POINTER(ctypes.POINTER(Chain)), ctypes.POINTER(ctypes.c_int)]
     pb, nb = ctypes.POINTER(Band)(), ctypes.c_int()
     pt, ne = ctypes.POINTER(ctypes.c_double)(), ctypes.c_longlong()
     pc, nc = ctypes.POINTER(Chain)(), ctypes.c_int()
-    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, M, ctypes.byref(pb), ctypes.byref(nb), ctypes.byref(pt),
+    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, M, maxpart, ctypes.byref(pb), ctypes.byref(nb), ctypes.byref(pt),
             ctypes.byref(ne), ctypes.byref(pc), ctypes.byref(nc))
     if rc != 0:
         raise RuntimeError("ftamd_sys_debug_host failed: %d" % rc)
@@ -51,7 +53,11 @@ def host_build(lib, plan_ptr, direction, S, M):
 
 
 def emulate(bands, tab, chains, S, A):
-    """Runs every chain on the n x M Fortran-ordered array A (in place)."""
+    """Runs every chain part on the n x M Fortran-ordered array A (in place),
+    in list order (the parts of a chain are listed in chain order).  A part
+    after the first of its chain (split_in >= 0) reads its whole stream from
+    A, where the part before it (split_out >= 0) stored every row it emitted:
+    what the kernel hands over through A and a progress counter."""
     B = LANES * S
     for ch in chains:
         cols = np.array(list(ch.col)).reshape(4, 2)
@@ -107,3 +113,30 @@ def emulate(bands, tab, chains, S, A):
                         A[e, cols[valid]] = E[B - 1][valid]
             prev = nxt
     return A
+
+
+class Item(ctypes.Structure):
+    _fields_ = [("chain", ctypes.c_int), ("wave0", ctypes.c_int)]
+
+
+def assign(lib, bands, chains, nxcd, nsx):
+    """(items, slot_items, xcd_slot) as the library deals the chain parts to
+    nxcd XCDs of nsx slots each (ftamd_sys_assign)."""
+    fn = lib.ftamd_sys_assign
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    nch, nslots = len(chains), nxcd * nsx
+    ch = (Chain * max(nch, 1))(*chains)
+    bd = (Band * max(len(bands), 1))(*bands)
+    items = (Item * max(nch, 1))()
+    slot_items = (ctypes.c_int * (nslots + 1))()
+    xcd_slot = (ctypes.c_int * (nxcd + 1))()
+    ws_load = (ctypes.c_long * (nslots * 17 + nxcd))()
+    ws_chain = (ctypes.c_int * max(2 * nch, 1))()
+    rc = fn(ctypes.addressof(ch), nch, ctypes.addressof(bd), nxcd, nsx, ctypes.addressof(items),
+            ctypes.addressof(slot_items), ctypes.addressof(xcd_slot), ctypes.addressof(ws_load),
+            ctypes.addressof(ws_chain))
+    if rc != 0:
+        raise RuntimeError("ftamd_sys_assign failed: %d" % rc)
+    return [(it.chain, it.wave0) for it in items[:nch]], list(slot_items), list(xcd_slot)

@@ -150,14 +150,32 @@ def test_engines(ft, engine, kind, params):
 @pytest.mark.parametrize("engine", ["pipeline", "systolic"])
 def test_engines_at_limits(ft, engine):
     """Both engines forced at the headline shape (sph n = 1024, 16 bands per
-    chain in the systolic engine) and tri/disk at n = 512, the largest FP64
-    size of the systolic engine (16 bands, 512 staged rows)."""
+    chain in the systolic engine) and tri/disk at n = 1024 (32 bands per
+    chain: the systolic engine runs them in parts on several workgroups)."""
     try:
         ft.set_engine(ft.ENGINE_PIPELINE if engine == "pipeline" else ft.ENGINE_SYSTOLIC)
-        for kind, params, n in (("sph", (), 1024), ("tri", (0.0, 0.0, 0.0), 512), ("disk", (0.0, 0.0), 512)):
+        for kind, params, n in (("sph", (), 1024), ("tri", (0.0, 0.0, 0.0), 1024), ("disk", (0.0, 0.0), 1024)):
             for d in (0, 1):
                 _check(ft, kind, params, n, d, seed=77)
     finally:
+        ft.set_engine(ft.ENGINE_AUTO)
+
+
+@pytest.mark.parametrize("maxpart", [1, 3, 16])
+def test_systolic_chain_parts(ft, maxpart):
+    """Chains split into parts of at most `maxpart` bands (one band per part:
+    every band hands its rows to the next through A and a progress counter,
+    across workgroups) give the oracle's bits, both directions, every
+    geometry, full and partial M."""
+    try:
+        ft.set_engine(ft.ENGINE_SYSTOLIC)
+        ft.set_systolic(maxpart)
+        for kind, params in CASES:
+            for n, M in ((129, None), (300, 77 if kind == "tri" else 155), (300, None)):
+                for d in (0, 1):
+                    _check(ft, kind, params, n, d, seed=41 + maxpart, M=M)
+    finally:
+        ft.set_systolic(0)
         ft.set_engine(ft.ENGINE_AUTO)
 
 
@@ -202,8 +220,8 @@ def test_async_device_api_and_stream(ft):
 def test_batch_uneven_degrees(ft, engine):
     """ft_gpu_execute_batch: transforms of log-uniform degree in one launch,
     each bit-identical to the oracle; the automatic choice runs batches on the
-    pipelined engine; forced, the band-systolic engine runs them too, and
-    refuses a batch with a transform beyond its sizes (tri n = 600 in FP64)."""
+    pipelined engine; forced, the band-systolic engine runs them too (tri
+    n = 600 in FP64 with chains split over several workgroups)."""
     import torch
     rng = np.random.default_rng(4)
     ns = np.unique(np.round(np.exp(rng.uniform(np.log(4), np.log(300), 24))).astype(int))
@@ -218,10 +236,6 @@ def test_batch_uneven_degrees(ft, engine):
                 plans = [maker(int(n), *params) for n in sizes]
                 As = [O.fill(kind, int(n), full_M(kind, int(n)), 100 + int(n)) for n in sizes]
                 Ts = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
-                if engine == "systolic" and max(sizes) > 512:
-                    with pytest.raises(ft.GPUError):
-                        ft.gpu_execute_batch(plans, d, Ts)
-                    continue
                 ft.gpu_execute_batch(plans, d, Ts)
                 used = ft.last_engine()
                 torch.cuda.synchronize()

@@ -63,9 +63,9 @@ def test_noalloc_rejects_what_it_cannot_run(ft):
     P = ft.plan_rotsphere(90)
     T = torch.zeros(2 * 90 - 1, 90, dtype=torch.float64, device="cuda")
     assert _noalloc(ft, P, 0, T, ft.FP64, stream) == ft.FT_GPU_EINVAL
-    # a shape beyond the band-systolic engine (FP64 sph n = 2048 runs on the pipelined one)
-    Q = ft.plan_rotsphere(2048)
-    U = torch.zeros(2 * 2048 - 1, 2048, dtype=torch.float64, device="cuda")
+    # a work list beyond the scratch (sph n = 4096: over 5000 chain parts)
+    Q = ft.plan_rotsphere(4096)
+    U = torch.zeros(2 * 4096 - 1, 4096, dtype=torch.float64, device="cuda")
     ft.gpu_execute(Q, 0, U)
     ft.synchronize(stream)
     assert _noalloc(ft, Q, 0, U, ft.FP64, stream) == ft.FT_GPU_EINVAL
