@@ -87,8 +87,13 @@ typedef struct {
 #define FTAMD_SYS_UNROLL 8   /* steps per loop iteration (a band's steps are a multiple) */
 #define FTAMD_SYS_PREFETCH 16 /* most steps of table entries fetched ahead; tables carry that many spare steps */
 
+/* Within a step of a band table the (s, c) pairs of the LANES x S sequences
+ * are slot-major ([S][LANES]) in FP64 and lane-major ([LANES][S]) in FP32
+ * (lane i's slot s is sequence S*i + s of the band). */
+#define FTAMD_SYS_SLOT_MAJOR(precision) ((precision) == 0)
+
 typedef struct {
-    long long tab;    /* entry offset of the band's table: [steps + PREFETCH][LANES][S] (s, c) pairs */
+    long long tab;    /* entry offset of the band's table: [steps + PREFETCH] steps of LANES*S (s, c) pairs */
     int R;            /* the band's row stream is rows 0 .. R of the class */
     int steps;        /* steps the band runs: R + (S+1)*(LANES-1) + S + 1 rounded up to UNROLL */
     int gbase;        /* position of the band's first sequence in the class, in application order */

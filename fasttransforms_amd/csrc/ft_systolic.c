@@ -151,8 +151,11 @@ static int tables_host(const ft_rotation_plan *RP, int dir, int precision, int S
             const FtamdSysBand *bd = &T->bands[T->band0[c] + b];
             for (int t = 0; t < bd->steps + PF; t++)
                 for (int q = 0; q < B; q++) {
-                    /* lane i holds sequences q = S*i .. S*i+S-1: entry (t, i, s) */
-                    long long at = bd->tab + (long long)t * B + q;
+                    /* lane i holds sequences q = S*i .. S*i+S-1: entry (t, i, s),
+                     * stored slot-major in FP64 */
+                    int i_lane = q / S, s_slot = q % S;
+                    long long at = bd->tab + (long long)t * B +
+                                   (FTAMD_SYS_SLOT_MAJOR(precision) ? s_slot * FTAMD_SYS_LANES + i_lane : q);
                     double sv = 0.0, cv = 1.0;
                     if (q < bd->nseq) {
                         int g = bd->gbase + q;

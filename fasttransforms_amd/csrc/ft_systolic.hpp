@@ -113,7 +113,12 @@ struct Engine {
     struct Shared {
         // table entries of the next kD steps of each wave, written by LDS DMA
         // (first in the block: the DMA's LDS address is taken from M0)
-        Pair<T> scr[kWaves][kD][kLanes][S];
+        // step-major; within a step slot-major for FP64 ([slot][lane]: a
+        // lane's 16-byte pair next to the next lane's, so a 128-bit read of
+        // 8 lanes spans all banks once) and lane-major for FP32 ([lane][slot]:
+        // a lane's S pairs in one 128-bit read); the tables are laid out
+        // the same way (FTAMD_SYS_SLOT_MAJOR)
+        Pair<T> scr[kWaves][kD][kLanes * S];
         T ring[kWaves][kZ][kRows][kCols];  // ring g: input stream of the band wave g runs
         int avail[kWaves];                 // positions of ring g written by wave g-1 (exclusive end)
         int freed[kWaves];                 // positions of ring g wave g has read (exclusive end)
@@ -183,7 +188,7 @@ struct Engine {
     // those reads have returned.
     __device__ static void dma_block(Shared &sh, int w, const char *tab_band, int step, unsigned lane16, T after) {
         const int slot = step % kD;
-        const unsigned lds = (unsigned)(uintptr_t)&sh.scr[w][slot][0][0];
+        const unsigned lds = (unsigned)(uintptr_t)&sh.scr[w][slot][0];
         const char *src = tab_band + (long long)uniform(step) * kStepBytes;
         // M0 (the DMA's LDS address) needs one wait state before the DMA
         // reads it; nothing else in this kernel uses M0.
@@ -221,10 +226,15 @@ struct Engine {
     __device__ static T row_end(T v) { return __builtin_amdgcn_update_dpp(T(0), v, 0x15f, 0xf, 0xf, true); }
     __device__ static unsigned lds_addr(const void *p) { return (unsigned)(uintptr_t)p; }
 
+    // position of (lane i, slot s) in a step's entries, and of lane i's first
+    static constexpr bool kSlotMajor = sizeof(T) == 8;
+    static constexpr int kSlotStride = kSlotMajor ? kLanes : 1;
+    __device__ static int lane_entry(int i) { return kSlotMajor ? i : i * S; }
+
     __device__ static void read_sc(Shared &sh, int w, int step, int i, SC &sc) {
-        const Pair<T> *p = &sh.scr[w][step % kD][i][0];
+        const Pair<T> *p = &sh.scr[w][step % kD][lane_entry(i)];
 #pragma unroll
-        for (int s = 0; s < S; ++s) sc.p[s] = p[s];
+        for (int s = 0; s < S; ++s) sc.p[s] = p[s * kSlotStride];
     }
 
     // Progress counters of split chains.  The producing band stores its rows
@@ -471,9 +481,9 @@ struct Engine {
             const T *pin = &in_ring[t0 % kZ][r][0];
             const T *pin_hi = (t0 % kZ == kZ - kU) ? pin - kZ * kRows * kCols : pin;
             const int sb = t0 % kD;
-            const Pair<T> *psc = &sh.scr[g][sb][i][0];
+            const Pair<T> *psc = &sh.scr[g][sb][lane_entry(i)];
             const Pair<T> *psc_hi = (sb == kD - kU) ? psc - kD * kLanes * S : psc;
-            const unsigned m0b = lds_addr(&sh.scr[g][0][0][0]) + (unsigned)sb * kStepBytes;
+            const unsigned m0b = lds_addr(&sh.scr[g][0][0]) + (unsigned)sb * kStepBytes;
             const char *srcb = tab_band + (long long)(t0 + kD) * kStepBytes;  // step t0+kD+K: the DMA of step u, K = u+1-kSPD
             // step u's output (the row-end values of the last sequence, lane
             // 15 of each row) is broadcast along its row and kept by lane u;
@@ -500,7 +510,7 @@ struct Engine {
                 {
                     const Pair<T> *q = (u + 1 >= kU ? psc_hi : psc) + (u + 1) * kLanes * S;
 #pragma unroll
-                    for (int s2 = 0; s2 < S; ++s2) scn.p[s2] = q[s2];
+                    for (int s2 = 0; s2 < S; ++s2) scn.p[s2] = q[s2 * kSlotStride];
                 }
                 // the reads above are issued before this step's arithmetic
                 __builtin_amdgcn_sched_barrier(0);

@@ -89,7 +89,8 @@ def emulate(bands, tab, chains, S, A):
             E = np.zeros((B, 4, 2))
             nxt = {}
             for t in range(steps):
-                sc = tab[bd.tab + t * B + q]
+                # FP64 tables are slot-major within a step: sequence S*i + s at s*16 + i
+                sc = tab[bd.tab + t * B + (q % S) * LANES + q // S]
                 sn = np.where(act, sc[:, 0, None, None], 0.0)
                 cs = np.where(act, sc[:, 1, None, None], 1.0)
                 # lane i's first sequence takes what the lane before emitted at
