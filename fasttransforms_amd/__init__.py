@@ -65,6 +65,9 @@ SIGNATURES = {
     "ft_gpu_set_systolic": (_i, [_i]),
     "ft_gpu_last_engine": (_i, []),
     "ft_gpu_set_spin_limit": (_i, [_i]),
+    "ft_gpu_set_tuning_table": (_i, [_vp, _i]),
+    "ft_gpu_tuning_table": (_i, [_vp, _i]),
+    "ft_gpu_tuned_choice": (_i, [_i, _i, _i, _i, _i, _vp]),
 }
 for _k in ("sph", "tri", "disk"):
     for _dn in ("hi2lo", "lo2hi"):
@@ -174,6 +177,44 @@ def set_systolic(maxpart=0):
     """Most bands of a band-systolic chain part (1..16; longer chains run in parts); 0 = built-in choice."""
     if _lib.ft_gpu_set_systolic(int(maxpart)):
         raise ValueError("maxpart must be in 0..16")
+
+
+TUNE_COLS = 11
+TUNE_FIELDS = ("kind", "precision", "direction", "batch", "nlo", "nhi", "engine", "maxpart", "L", "W", "wg_per_cu")
+
+
+def set_tuning_table(rows):
+    """Installs an autotuning table: a list of TUNE_COLS-int rows (TUNE_FIELDS),
+    or "compiled" for the table built into the library, or None for none."""
+    if rows is None or rows == "compiled":
+        rc = _lib.ft_gpu_set_tuning_table(None, -1 if rows is None else 0)
+    else:
+        flat = [int(v) for r in rows for v in r]
+        if any(len(r) != TUNE_COLS for r in rows):
+            raise ValueError("tuning rows have %d fields" % TUNE_COLS)
+        buf = (ctypes.c_int * max(1, len(flat)))(*flat)
+        rc = _lib.ft_gpu_set_tuning_table(ctypes.cast(buf, _vp), len(rows)) if rows else \
+            _lib.ft_gpu_set_tuning_table(None, -1)
+    if rc:
+        raise ValueError("invalid tuning table")
+
+
+def tuning_table():
+    """The active autotuning table as a list of TUNE_COLS-int rows."""
+    n = int(_lib.ft_gpu_tuning_table(None, 0))
+    buf = (ctypes.c_int * max(1, n * TUNE_COLS))()
+    n = int(_lib.ft_gpu_tuning_table(ctypes.cast(buf, _vp), n))
+    return [list(buf[k * TUNE_COLS:(k + 1) * TUNE_COLS]) for k in range(n)]
+
+
+def tuned_choice(kind, precision, direction, batch, n):
+    """(engine, maxpart, L, W, wg_per_cu) a launch of that shape uses now."""
+    kind = {"sph": SPH, "tri": TRI, "disk": DISK}.get(kind, kind)
+    direction = {"hi2lo": 0, "lo2hi": 1}.get(direction, direction)
+    out = (ctypes.c_int * 5)()
+    if _lib.ft_gpu_tuned_choice(int(kind), int(precision), int(direction), int(batch), int(n), ctypes.cast(out, _vp)):
+        raise ValueError("invalid shape")
+    return tuple(out)
 
 
 def set_spin_limit(limit=0):

@@ -88,41 +88,50 @@ int ft_gpu_set_systolic(int maxpart) {
     return FT_GPU_OK;
 }
 
-int ftamd_sys_maxpart(int kind, int n, int precision) {
-    (void)kind;
-    (void)n;
-    (void)precision;
-    int k = KNOB_GET(g_sys_maxpart);
-    return k ? k : 8;
-}
-
 static __thread int t_last_engine = 0;
 int ft_gpu_last_engine(void) { return t_last_engine; }
 
-int ftamd_tuning_L(int kind, int n) {
-    (void)kind;
-    (void)n;
-    int L = KNOB_GET(g_L);
-    return L ? L : 4;
+void ftamd_choose(int kind, int precision, int dir, int batch, int n, ftamd_choice *c) {
+    int te = 0, tm = 0, tL = 0, tW = 0, tg = 0;
+    const int found = ftamd_tuned_lookup(kind, precision, dir, batch, n, &te, &tm, &tL, &tW, &tg);
+    const int ke = KNOB_GET(g_engine), km = KNOB_GET(g_sys_maxpart), kL = KNOB_GET(g_L), kW = KNOB_GET(g_W),
+              kg = KNOB_GET(g_wg_per_cu);
+    /* built-in: the band-systolic engine for single transforms, the pipelined
+     * one for batches (many short chains leave the systolic one's CUs idle),
+     * whole chains of up to 16 bands, L 4, W 16, batches at 2 workgroups per
+     * CU (512 sph transforms of degree 32..1018: 9.9 ms capped, 10.4 ms not) */
+    c->forced = ke != FT_GPU_ENGINE_AUTO;
+    c->engine = c->forced ? ke : found ? te : (batch ? FT_GPU_ENGINE_PIPELINE : FT_GPU_ENGINE_SYSTOLIC);
+    c->maxpart = km ? km : (found && tm) ? tm : FTAMD_SYS_WAVES;
+    c->L = kL ? kL : (found && tL) ? tL : 4;
+    c->W = kW ? kW : (found && tW) ? tW : 16;
+    c->wg_per_cu = kg ? kg : (found && tg) ? tg : (batch ? 2 : 0);
 }
 
-int ftamd_tuning_W(void) {
-    int W = KNOB_GET(g_W);
-    return W ? W : 16;
+int ft_gpu_tuned_choice(int kind, int precision, int direction, int batch, int n, int *choice) {
+    if (kind < 0 || kind > 2 || (precision != 0 && precision != 1) || (direction != 0 && direction != 1) ||
+        (batch != 0 && batch != 1) || n < 1 || !choice)
+        return FT_GPU_EINVAL;
+    ftamd_choice c;
+    ftamd_choose(kind, precision, direction, batch, n, &c);
+    choice[0] = c.engine;
+    choice[1] = c.maxpart;
+    choice[2] = c.L;
+    choice[3] = c.W;
+    choice[4] = c.wg_per_cu;
+    return FT_GPU_OK;
 }
 
 /* Workgroups are dealt to the XCDs in turn and each takes tasks only from its
  * own XCD's list, so the grid holds as many workgroups per XCD as the longest
  * list has tasks: every task gets a workgroup as soon as the hardware has room
- * (workgroups that find their list empty exit at once).  The tuning knob caps
- * the grid at wg_per_cu workgroups per CU. */
+ * (workgroups that find their list empty exit at once).  A schedule's
+ * wg_per_cu (the knob at launch time if set) caps the grid at that many
+ * workgroups per CU. */
 int ftamd_tuning_grid(const ft_schedule *S) {
     int grid = S->launch.nxcd * S->xcd_tasks;
-    /* batches default to 2 per CU: their many short tasks keep that many busy,
-     * and more workgroups only contend (512 sph transforms of degree
-     * 32..1018: 9.9 ms capped, 10.4 ms uncapped) */
     int knob = KNOB_GET(g_wg_per_cu);
-    int per_cu = knob ? knob : (S->batch ? 2 : 0);
+    int per_cu = knob ? knob : S->wg_per_cu;
     if (per_cu) {
         int cu = 0;
         if (ftamd_hip_device(NULL, &cu) != FTAMD_OK || cu <= 0) cu = 256;
@@ -536,9 +545,12 @@ static int build_from_columns(ft_schedule *S, int kind, int dir, int L, int W, c
 
 /* Schedule for the whole matrix (cached in the plan; caller holds RP->lock). */
 static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision, int M, void *dA, int *rc) {
-    int L = ftamd_tuning_L(RP->kind, RP->n), W = ftamd_tuning_W();
+    ftamd_choice ch;
+    ftamd_choose(RP->kind, precision, dir, 0, RP->n, &ch);
+    const int L = ch.L, W = ch.W;
     for (ft_schedule *s = RP->scheds; s; s = s->next)
-        if (!s->single && !s->sys && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W) {
+        if (!s->single && !s->sys && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W &&
+            s->wg_per_cu == ch.wg_per_cu) {
             /* Matrix pointer may change between calls: groups hold it. */
             return s;
         }
@@ -552,6 +564,7 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     S->precision = precision;
     S->L = L;
     S->W = W;
+    S->wg_per_cu = ch.wg_per_cu;
     ft_col *cols = NULL;
     int ncols = 0;
     *rc = matrix_columns(RP->kind, RP->n, M, dA, &cols, &ncols);
@@ -680,7 +693,9 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     *rc = FTAMD_OK;
     int Sq = ftamd_sys_choose_S(RP->kind, RP->n, precision);
     if (Sq == 0 || (long long)RP->n * M > 0x7fffffffLL) return NULL;
-    const int maxpart = ftamd_sys_maxpart(RP->kind, RP->n, precision);
+    ftamd_choice choice;
+    ftamd_choose(RP->kind, precision, dir, 0, RP->n, &choice);
+    const int maxpart = choice.maxpart;
     for (ft_schedule *s = RP->scheds; s; s = s->next)
         if (s->sys && s->M == M && s->dir == dir && s->precision == precision && s->maxpart == maxpart) return s;
     if (!RP->sys[dir][precision]) {
@@ -765,13 +780,15 @@ static int valid_M(int kind, int n, int M) {
 
 /* Schedule for one launch on RP (caller holds no lock). */
 static int pick_schedule(ft_rotation_plan *RP, int dir, int precision, void *dA, int M, ft_schedule **out) {
-    int rc = FTAMD_OK, engine = KNOB_GET(g_engine);
+    int rc = FTAMD_OK;
+    ftamd_choice ch;
+    ftamd_choose(RP->kind, precision, dir, 0, RP->n, &ch);
     pthread_mutex_lock(&RP->lock);
     rc = ftamd_plan_upload(RP, precision);
     ft_schedule *S = NULL;
-    if (rc == FTAMD_OK && engine != FT_GPU_ENGINE_PIPELINE) {
+    if (rc == FTAMD_OK && ch.engine == FT_GPU_ENGINE_SYSTOLIC) {
         S = sys_schedule(RP, dir, precision, M, &rc);
-        if (rc == FTAMD_OK && !S && engine == FT_GPU_ENGINE_SYSTOLIC) rc = FTAMD_EINVAL;
+        if (rc == FTAMD_OK && !S && ch.forced) rc = FTAMD_EINVAL;
     }
     if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, dir, precision, M, dA, &rc);
     pthread_mutex_unlock(&RP->lock);
@@ -846,8 +863,11 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
     int nfull = 0, nch = 0, nprog = 0, nxcd = 1, nsx = 1;
     rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, (int)NA_MAX_CHAINS, &nfull);
     if (rc == FTAMD_OK)
-        rc = ftamd_sys_split(g_na_chains, nfull, (int)NA_MAX_CHAINS, ftamd_sys_maxpart(kind, RP->n, precision), &nch,
-                             &nprog);
+    {
+        ftamd_choice ch;
+        ftamd_choose(kind, precision, direction, 0, RP->n, &ch);
+        rc = ftamd_sys_split(g_na_chains, nfull, (int)NA_MAX_CHAINS, ch.maxpart, &nch, &nprog);
+    }
     if (rc == FTAMD_OK) {
         sys_slots(g_na_chains, nch, &nxcd, &nsx);
         if (nxcd * nsx > NA_MAX_SLOTS) rc = FTAMD_EINVAL;
@@ -976,7 +996,9 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
 /* The cached single-vector schedule of (dir, jlo, jhi, S) on RP, built on
  * first use; the column is a launch argument.  Caller holds RP->exec_lock. */
 static int single_schedule(ft_rotation_plan *RP, int kind, int dir, int jlo, int jhi, int S, ft_schedule **out) {
-    int L = ftamd_tuning_L(kind, RP->n), W = ftamd_tuning_W();
+    ftamd_choice ch;
+    ftamd_choose(kind, 0, dir, 0, RP->n, &ch);
+    const int L = ch.L, W = ch.W;
     int rc = FTAMD_OK;
     pthread_mutex_lock(&RP->lock);
     ft_schedule *prev = NULL, *hit = NULL;
@@ -1237,7 +1259,7 @@ static int batch_check(const ft_rotation_plan *const *plans, int kind, int direc
  * some transform does not fit the engine or the transforms would need
  * different band widths. */
 static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
-                           void *const *dA, const int *M, int count, ft_schedule **out) {
+                           void *const *dA, const int *M, int count, int maxpart, ft_schedule **out) {
     *out = NULL;
     int S0 = 0;
     for (int i = 0; i < count; i++) {
@@ -1260,10 +1282,6 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     FtamdSysChain *ch = NULL;
     int nb = 0, nch = 0, nprog = 0;
     const void *tab_base = NULL;
-    int nmax = 0;
-    for (int i = 0; i < count; i++)
-        if (plans[i]->n > nmax) nmax = plans[i]->n;
-    const int maxpart = ftamd_sys_maxpart(kind, nmax, precision);
     if (rc == FTAMD_OK)
         rc = ftamd_sys_batch(plans, tabs, dA, M, count, maxpart, &bands, &nb, &ch, &nch, &nprog, &tab_base);
     free(tabs);
@@ -1302,13 +1320,16 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
  * short chains of a batch of small transforms leave most of each CU idle:
  * 20.2 ms against 10.4 ms for 512 sph transforms of degree 32..1018). */
 static int batch_build(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
-                       void *const *dA, const int *M, int count, ft_rotation_plan *big, ft_schedule **out) {
-    if (KNOB_GET(g_engine) == FT_GPU_ENGINE_SYSTOLIC) {
-        int rs = batch_build_sys(plans, kind, direction, precision, dA, M, count, out);
+                       void *const *dA, const int *M, int count, ft_rotation_plan *big, const ftamd_choice *ch,
+                       ft_schedule **out) {
+    if (ch->engine == FT_GPU_ENGINE_SYSTOLIC) {
+        int rs = batch_build_sys(plans, kind, direction, precision, dA, M, count, ch->maxpart, out);
         if (rs != FTAMD_OK) return rs;
-        return *out ? FTAMD_OK : FTAMD_EINVAL;
+        if (*out) return FTAMD_OK;
+        if (ch->forced) return FTAMD_EINVAL;
+        /* preferred but not applicable (no systolic kernel for the precision) */
     }
-    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W();
+    const int L = ch->L, W = ch->W;
     long total = 0;
     for (int i = 0; i < count; i++) total += M[i];
     ft_col *cols = malloc(sizeof(ft_col) * (size_t)(total > 0 ? total : 1));
@@ -1332,6 +1353,7 @@ static int batch_build(const ft_rotation_plan *const *plans, int kind, int direc
         S->precision = precision;
         S->L = L;
         S->W = W;
+        S->wg_per_cu = ch->wg_per_cu;
         S->batch = 1;
         rc = build_from_columns(S, kind, direction, L, W, cols, ncols, ftamd_plan_table(big, precision), big->n);
     }
@@ -1350,14 +1372,16 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
     ft_rotation_plan *big = NULL;
     int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
     if (rc != FTAMD_OK || count == 0) return rc;
-    int L = ftamd_tuning_L(kind, big->n), W = ftamd_tuning_W(), engine = KNOB_GET(g_engine);
-    int part = ftamd_sys_maxpart(kind, big->n, precision);
+    ftamd_choice ch;
+    ftamd_choose(kind, precision, direction, 1, big->n, &ch);
+    /* the cache key holds every choice the schedule was built with */
+    const int L = ch.L, W = ch.W, engine = ch.engine * 2 + ch.forced, part = ch.maxpart * 16 + ch.wg_per_cu;
     pthread_mutex_lock(&g_batch.lock);
     ft_schedule *S = g_batch.sched;
     if (!batch_cache_hit(plans, kind, direction, precision, dA, M, count, L, W, engine, part)) {
         batch_cache_clear_locked();
         S = NULL;
-        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &S);
+        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &ch, &S);
         if (rc == FTAMD_OK)
             rc = batch_cache_store(S, plans, kind, direction, precision, dA, M, count, L, W, engine, part);
         if (rc != FTAMD_OK) {
@@ -1389,7 +1413,11 @@ ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, 
         B = calloc(1, sizeof(ft_gpu_batch));
         rc = B ? FTAMD_OK : FTAMD_ENOMEM;
     }
-    if (rc == FTAMD_OK) rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &B->S);
+    if (rc == FTAMD_OK) {
+        ftamd_choice ch;
+        ftamd_choose(kind, precision, direction, 1, big->n, &ch);
+        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &ch, &B->S);
+    }
     if (rc != FTAMD_OK) {
         free(B);
         ftamd_set_error(rc);

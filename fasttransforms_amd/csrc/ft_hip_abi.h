@@ -166,6 +166,7 @@ int ftamd_hip_sys_supported(int S, int precision);
 
 int ftamd_hip_device(int *device, int *num_cu);
 int ftamd_hip_xcd_count(int *nxcd);
+int ftamd_hip_arch(char *name, int cap);
 int ftamd_hip_pointer_kind(const void *ptr, int *kind);
 int ftamd_hip_malloc(void **ptr, size_t bytes);
 int ftamd_hip_free(void *ptr);

@@ -29,6 +29,7 @@ typedef struct ft_schedule {
     int sys;
     int batch;         /* a batch schedule (grid capped at 2 workgroups per CU by default) */
     int maxpart;       /* bands per chain part it was built with (part of its key) */
+    int wg_per_cu;     /* pipelined: workgroups per CU it was built for, 0: no cap */
     FtamdSysLaunch sys_launch;
     void *d_sys;
     double flops;            /* algorithmic flops (6 per rotation) */
@@ -63,14 +64,26 @@ const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
 
 /* ft_drivers.c */
 void ftamd_set_error(int code);
-int ftamd_tuning_L(int kind, int n);
-int ftamd_tuning_W(void);
 int ftamd_tuning_grid(const ft_schedule *S);
 void ftamd_schedule_free_device(ft_schedule *S);
 void ftamd_schedule_release(ft_schedule *S);
 void ftamd_batch_forget_plan(const ft_rotation_plan *RP);
 int ftamd_engine(void);
-int ftamd_sys_maxpart(int kind, int n, int precision);
+
+/* The engine and shape a launch uses: the knobs (ft_gpu_set_*) where set,
+ * else the autotuning table's row, else the built-in choice. */
+typedef struct {
+    int engine;     /* FT_GPU_ENGINE_PIPELINE or FT_GPU_ENGINE_SYSTOLIC (preferred; forced if the knob is) */
+    int forced;     /* the engine knob is set */
+    int maxpart;    /* band-systolic bands per chain part */
+    int L, W;       /* pipelined sequences per wave, stage waves per workgroup */
+    int wg_per_cu;  /* pipelined workgroups per CU, 0: no cap */
+} ftamd_choice;
+void ftamd_choose(int kind, int precision, int dir, int batch, int n, ftamd_choice *c);
+
+/* ft_tuning.c */
+int ftamd_tuned_lookup(int kind, int precision, int dir, int batch, int n, int *engine, int *maxpart, int *L, int *W,
+                       int *wg_per_cu);
 
 /* ft_systolic.c */
 int ftamd_sys_choose_S(int kind, int n, int precision);

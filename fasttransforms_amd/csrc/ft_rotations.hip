@@ -85,6 +85,25 @@ int ftamd_hip_device(int *device, int *num_cu) {
 
 // Number of XCDs, from the XCC_IDs a spread of workgroups reports (cached
 // per process; one device).
+// Architecture name of the current device (e.g. "gfx950"), cached per device.
+int ftamd_hip_arch(char *name, int cap) {
+    static char cache[FTAMD_MAX_DEVICES][64];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return map_err(e);
+    if (dev < 0 || dev >= FTAMD_MAX_DEVICES || cap < 1) return FTAMD_EINVAL;
+    if (!cache[dev][0]) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, dev);
+        if (e != hipSuccess) return map_err(e);
+        strncpy(cache[dev], prop.gcnArchName, 63);
+        cache[dev][63] = 0;
+    }
+    strncpy(name, cache[dev], (size_t)cap - 1);
+    name[cap - 1] = 0;
+    return FTAMD_OK;
+}
+
 int ftamd_hip_xcd_count(int *nxcd) {
     static int cached = 0;
     if (cached > 0) {

@@ -149,8 +149,29 @@ int ft_gpu_execute_batch_plan(const ft_gpu_batch *B, void *stream);
 void ft_gpu_destroy_batch(ft_gpu_batch *B);
 
 /* Tuning knobs of the pipelined kernel: sequences per wave (L), stage waves
- * per workgroup (W), workgroups per CU.  0 keeps the built-in choice. */
+ * per workgroup (W), workgroups per CU.  0 keeps the tuned (or built-in)
+ * choice. */
 int ft_gpu_set_tuning(int L, int W, int wg_per_cu);
+
+/* Autotuning table (no reference counterpart; the reference picks its SIMD
+ * kernel by CPU feature alone, src/drivers.c:58-110).  One row of
+ * FT_GPU_TUNE_COLS ints per (kind, precision, direction, batch, degree
+ * range): kind, precision, direction, batch (0 single transform, 1 batch),
+ * nlo, nhi, engine (FT_GPU_ENGINE_PIPELINE or _SYSTOLIC), band-systolic
+ * bands per chain part (0: built-in), pipelined L and W (0: built-in),
+ * pipelined workgroups per CU (0: built-in).  The library carries the table
+ * fasttransforms_amd/tune.py measured on gfx950; knobs set with
+ * ft_gpu_set_engine, ft_gpu_set_systolic and ft_gpu_set_tuning override it.
+ * ft_gpu_set_tuning_table installs `count` rows (count 0: the compiled
+ * table again, -1: no table, the built-in choices); FT_GPU_EINVAL for an
+ * invalid row.  ft_gpu_tuning_table copies up to `cap` rows of the active
+ * table and returns how many it has.  ft_gpu_tuned_choice gives the engine,
+ * bands per part, L, W and workgroups per CU a launch of that shape uses
+ * now (knobs, then table, then built-in). */
+#define FT_GPU_TUNE_COLS 11
+int ft_gpu_set_tuning_table(const int *rows, int count);
+int ft_gpu_tuning_table(int *rows, int cap);
+int ft_gpu_tuned_choice(int kind, int precision, int direction, int batch, int n, int *choice);
 
 /* Engine of the whole-matrix executors (no reference counterpart; for tests
  * and measurements): FT_GPU_ENGINE_AUTO (default) runs the band-systolic

@@ -220,7 +220,7 @@ def test_async_device_api_and_stream(ft):
 def test_batch_uneven_degrees(ft, engine):
     """ft_gpu_execute_batch: transforms of log-uniform degree in one launch,
     each bit-identical to the oracle; the automatic choice runs batches on the
-    pipelined engine; forced, the band-systolic engine runs them too (tri
+    engine the tuning table names; forced, the band-systolic engine runs them (tri
     n = 600 in FP64 with chains split over several workgroups)."""
     import torch
     rng = np.random.default_rng(4)
@@ -239,7 +239,8 @@ def test_batch_uneven_degrees(ft, engine):
                 ft.gpu_execute_batch(plans, d, Ts)
                 used = ft.last_engine()
                 torch.cuda.synchronize()
-                assert used == (ft.ENGINE_PIPELINE if engine == "auto" else ft.ENGINE_SYSTOLIC), used
+                want = ft.tuned_choice(kind, 0, d, 1, int(max(sizes)))[0] if engine == "auto" else ft.ENGINE_SYSTOLIC
+                assert used == want, (used, want)
                 for n, A, T in zip(sizes, As, Ts):
                     R = O.execute(O.Plan(kind, int(n), *params), d, A.copy(order="F"))
                     assert np.array_equal(T.cpu().numpy().T, R), (kind, d, n)
