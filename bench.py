@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the secondary measurements")
     p.add_argument("--engine", choices=("auto", "pipeline", "systolic"), default="auto",
                    help="engine of the headline launches (auto: the library's own choice)")
+    p.add_argument("--case", help="time one case only, KIND:N:DIR[:fp32][:batch] (for profiling runs), "
+                                  "e.g. tri:1024:hi2lo or sph:1024:hi2lo:batch (the 512-transform batch)")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return p.parse_args()
@@ -358,6 +360,55 @@ def profiled_traffic(engine, n):
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
+TUNING_CASES = (("sph", 1024, "hi2lo", ()), ("sph", 1024, "lo2hi", ()), ("sph", 2048, "hi2lo", ()),
+                ("sph", 2048, "lo2hi", ()), ("tri", 1024, "lo2hi", (0.0, 0.0, 0.0)),
+                ("disk", 1024, "lo2hi", (0.0, 0.0)))
+
+
+def tuning_report(ft, torch):
+    """The tuned table's choice against the built-in one (band-systolic
+    engine with whole chains of up to 16 bands; pipelined L 4, W 16), per
+    case: both times and the tuned choice."""
+    out = {}
+    names = {ft.ENGINE_PIPELINE: "pipeline", ft.ENGINE_SYSTOLIC: "systolic"}
+    for kind, nn, d, params in TUNING_CASES:
+        row = {}
+        for label, table in (("tuned", "compiled"), ("builtin", None)):
+            ft.set_tuning_table(table)
+            e, mp, L, W, wg = ft.tuned_choice(kind, 0, d, 0, nn)
+            m_, _ = single(ft, torch, kind, nn, d, ft.FP64, 10, 2, params)
+            row[label + "_ms"] = round(m_, 4)
+            row[label + "_choice"] = {"engine": names[e], "maxpart": mp} if e == ft.ENGINE_SYSTOLIC else \
+                {"engine": names[e], "L": L, "W": W}
+        ft.set_tuning_table("compiled")
+        row["speedup"] = round(row["builtin_ms"] / row["tuned_ms"], 3)
+        out["%s_n%d_%s" % (kind, nn, d)] = row
+    return out
+
+
+def calib_copy(torch):
+    """A 64 MiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the HBM counters."""
+    src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def run_case(ft, torch, spec, steps, warmup):
+    """--case: one configuration only, for profiling runs."""
+    parts = spec.split(":")
+    kind, nn, d = parts[0], int(parts[1]), parts[2]
+    precision = ft.FP32 if "fp32" in parts[3:] else ft.FP64
+    params = {"sph": (), "tri": (0.0, 0.0, 0.0), "disk": (0.0, 0.0)}[kind]
+    if "batch" in parts[3:]:
+        res = batch(ft, torch, steps=steps)
+        print(json.dumps({"case": spec, "batched_ms": res["batched_ms"]}), flush=True)
+        return
+    m_, fl = single(ft, torch, kind, nn, d, precision, steps, warmup, params)
+    print(json.dumps({"case": spec, "ms": round(m_, 5), "gflops": round(fl / (m_ * 1e-3) / 1e9, 2),
+                      "engine": ft.last_engine()}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -366,6 +417,12 @@ def main():
         sys.exit(2)
     import torch
     import fasttransforms_amd as ft
+
+    if args.case:
+        run_case(ft, torch, args.case, args.steps, args.warmup)
+        if args.calib_copy:
+            calib_copy(torch)
+        return
 
     n = args.n
     M = 2 * n - 1
@@ -416,13 +473,11 @@ def main():
             "batch64_tri": batch64(ft, torch, "tri", (0.0, 0.0, 0.0)),
             "batch64_tri_m05": batch64(ft, torch, "tri", (-0.5, -0.5, -0.5)),
             "batch64_disk": batch64(ft, torch, "disk", (0.0, 0.0)),
+            "tuning": tuning_report(ft, torch),
         })
 
     if args.calib_copy:
-        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        calib_copy(torch)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_seconds)
     cpu_obj = None
