@@ -16,13 +16,17 @@ import os
 
 import numpy as np
 
-SPH, TRI, DISK = 0, 1, 2
+# executor kinds (FT_GPU_SPH .. FT_GPU_RECTDISK); a plan's kind is its
+# constructor's: SPH, TRI, DISK or RECTDISK (sphv runs on a SPH plan)
+SPH, TRI, DISK, SPHV, RECTDISK = 0, 1, 2, 3, 4
 HI2LO, LO2HI = 0, 1
 FP64, FP32 = 0, 1
 # error codes of the C ABI (FT_GPU_* in include/fasttransforms_amd.h)
 (FT_GPU_OK, FT_GPU_EINVAL, FT_GPU_ENOMEM, FT_GPU_EDEVICE, FT_GPU_ETIMEOUT, FT_GPU_EPTR,
  FT_GPU_EINCOMPLETE) = range(7)
-_KIND = {"sph": SPH, "tri": TRI, "disk": DISK}
+_KIND = {"sph": SPH, "tri": TRI, "disk": DISK, "sphv": SPHV, "rectdisk": RECTDISK}
+# table geometry of each kind: sphv and rectdisk use the sphere's
+_GEOMETRY = {SPH: SPH, TRI: TRI, DISK: DISK, SPHV: SPH, RECTDISK: SPH}
 _DIR = {"hi2lo": HI2LO, "lo2hi": LO2HI}
 
 # One HIP runtime per process: PyTorch-ROCm bundles its own libamdhip64
@@ -51,6 +55,7 @@ SIGNATURES = {
     "ft_plan_rotsphere": (_vp, [_i]),
     "ft_plan_rottriangle": (_vp, [_i, _d, _d, _d]),
     "ft_plan_rotdisk": (_vp, [_i, _d, _d]),
+    "ft_plan_rotrectdisk": (_vp, [_i, _d]),
     "ft_gpu_last_error": (_i, []),
     "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
     "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
@@ -69,9 +74,10 @@ SIGNATURES = {
     "ft_gpu_tuning_table": (_i, [_vp, _i]),
     "ft_gpu_tuned_choice": (_i, [_i, _i, _i, _i, _i, _vp]),
 }
-for _k in ("sph", "tri", "disk"):
-    for _dn in ("hi2lo", "lo2hi"):
+for _dn in ("hi2lo", "lo2hi"):
+    for _k in ("sph", "tri", "disk", "rectdisk"):
         SIGNATURES["ft_kernel_%s_%s" % (_k, _dn)] = (None, [_vp, _i, _i, _vp, _i])
+    for _k in ("sph", "tri", "disk", "sphv", "rectdisk"):
         SIGNATURES["ft_execute_%s_%s" % (_k, _dn)] = (None, [_vp, _vp, _vp, _i])
 
 for _name, (_res, _args) in SIGNATURES.items():
@@ -148,6 +154,10 @@ def plan_rottriangle(n, alpha, beta, gamma):
                         (float(alpha), float(beta), float(gamma)))
 
 
+def plan_rotrectdisk(n, beta):
+    return RotationPlan(_lib.ft_plan_rotrectdisk(int(n), float(beta)), RECTDISK, int(n), (float(beta),))
+
+
 def plan_rotdisk(n, alpha, beta):
     return RotationPlan(_lib.ft_plan_rotdisk(int(n), float(alpha), float(beta)), DISK, int(n),
                         (float(alpha), float(beta)))
@@ -209,7 +219,7 @@ def tuning_table():
 
 def tuned_choice(kind, precision, direction, batch, n):
     """(engine, maxpart, L, W, wg_per_cu) a launch of that shape uses now."""
-    kind = {"sph": SPH, "tri": TRI, "disk": DISK}.get(kind, kind)
+    kind = _KIND.get(kind, kind)
     direction = {"hi2lo": 0, "lo2hi": 1}.get(direction, direction)
     out = (ctypes.c_int * 5)()
     if _lib.ft_gpu_tuned_choice(int(kind), int(precision), int(direction), int(batch), int(n), ctypes.cast(out, _vp)):
@@ -274,8 +284,8 @@ def _execute(kind, direction):
     fn = getattr(_lib, name)
 
     def run(plan, A, M=None):
-        if plan.kind != _KIND[kind]:
-            raise ValueError("plan is not a %s plan" % kind)
+        if _GEOMETRY[plan.kind] != _GEOMETRY[_KIND[kind]]:
+            raise ValueError("plan's tables are not in the %s layout" % kind)
         M = _M_of(A, plan.n, M)
         fn(plan.ptr, _pointer(A, plan.n, M), None, M)
         _check()
@@ -291,8 +301,8 @@ def _kernel(kind, direction):
     fn = getattr(_lib, name)
 
     def run(plan, m1, m2, A, S=1):
-        if plan.kind != _KIND[kind]:
-            raise ValueError("plan is not a %s plan" % kind)
+        if _GEOMETRY[plan.kind] != _GEOMETRY[_KIND[kind]]:
+            raise ValueError("plan's tables are not in the %s layout" % kind)
         span = (plan.n - 1) * int(S) + 1
         if (A.numel() if _is_torch(A) else A.size) < span:
             raise ValueError("buffer shorter than (n-1)*S+1")
@@ -310,26 +320,34 @@ execute_tri_hi2lo = _execute("tri", "hi2lo")
 execute_tri_lo2hi = _execute("tri", "lo2hi")
 execute_disk_hi2lo = _execute("disk", "hi2lo")
 execute_disk_lo2hi = _execute("disk", "lo2hi")
+execute_sphv_hi2lo = _execute("sphv", "hi2lo")
+execute_sphv_lo2hi = _execute("sphv", "lo2hi")
+execute_rectdisk_hi2lo = _execute("rectdisk", "hi2lo")
+execute_rectdisk_lo2hi = _execute("rectdisk", "lo2hi")
 kernel_sph_hi2lo = _kernel("sph", "hi2lo")
 kernel_sph_lo2hi = _kernel("sph", "lo2hi")
 kernel_tri_hi2lo = _kernel("tri", "hi2lo")
 kernel_tri_lo2hi = _kernel("tri", "lo2hi")
 kernel_disk_hi2lo = _kernel("disk", "hi2lo")
 kernel_disk_lo2hi = _kernel("disk", "lo2hi")
+kernel_rectdisk_hi2lo = _kernel("rectdisk", "hi2lo")
+kernel_rectdisk_lo2hi = _kernel("rectdisk", "lo2hi")
 
 
-def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None):
+def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None, kind=None):
     """Asynchronous device-resident execution (ft_gpu_execute) on a torch tensor.
 
-    ``stream`` is a torch.cuda.Stream (default: the current stream).
+    ``stream`` is a torch.cuda.Stream (default: the current stream); ``kind``
+    the executor (default: the plan's own; "sphv" runs on a sphere plan).
     """
     import torch
     direction = _DIR.get(direction, direction)
+    kind = plan.kind if kind is None else _KIND.get(kind, kind)
     M = _M_of(A, plan.n, M)
     if stream is None:
         stream = torch.cuda.current_stream(A.device)
     ptr = _pointer(A, plan.n, M, np.float64 if precision == FP64 else np.float32)
-    rc = _lib.ft_gpu_execute(plan.ptr, plan.kind, int(direction), int(precision), ptr, M, _vp(stream.cuda_stream))
+    rc = _lib.ft_gpu_execute(plan.ptr, kind, int(direction), int(precision), ptr, M, _vp(stream.cuda_stream))
     if rc:
         raise GPUError("ft_gpu_execute: %s" % _lib.ft_gpu_error_string(rc).decode())
     return A
@@ -347,10 +365,11 @@ def synchronize(stream=None):
         raise GPUError("ft_gpu_synchronize: %s (code %d)" % (_lib.ft_gpu_error_string(rc).decode(), rc))
 
 
-def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream=None):
+def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream=None, kind=None):
     """ft_gpu_execute_batch: many transforms of one kind and uneven degree in
     one persistent launch.  ``tensors[i]`` is a CUDA tensor holding the
-    column-major plans[i].n x Ms[i] matrix."""
+    column-major plans[i].n x Ms[i] matrix; ``kind`` the executor (default:
+    the plans' own)."""
     import torch
     direction = _DIR.get(direction, direction)
     count = len(plans)
@@ -358,7 +377,7 @@ def gpu_execute_batch(plans, direction, tensors, Ms=None, precision=FP64, stream
         raise ValueError("one tensor per plan")
     if count == 0:
         return tensors
-    kind = plans[0].kind
+    kind = plans[0].kind if kind is None else _KIND.get(kind, kind)
     if Ms is None:
         Ms = [_M_of(T, P.n, None) for P, T in zip(plans, tensors)]
     dtype = np.float64 if precision == FP64 else np.float32
@@ -379,8 +398,9 @@ class BatchPlan:
     that its launch can be captured in a HIP graph (valid while the object
     lives).  ``execute(stream)`` runs ft_gpu_execute_batch_plan."""
 
-    def __init__(self, plans, direction, tensors, Ms=None, precision=FP64):
+    def __init__(self, plans, direction, tensors, Ms=None, precision=FP64, kind=None):
         direction = _DIR.get(direction, direction)
+        kind = plans[0].kind if kind is None and plans else _KIND.get(kind, kind)
         count = len(plans)
         if count == 0 or len(tensors) != count:
             raise ValueError("one tensor per plan, at least one")
@@ -391,7 +411,7 @@ class BatchPlan:
         pls = (_vp * count)(*[P.ptr for P in plans])
         ms = (ctypes.c_int * count)(*[int(M) for M in Ms])
         self._keep = (list(plans), list(tensors))
-        self.ptr = _lib.ft_gpu_plan_batch(ctypes.cast(pls, _vp), plans[0].kind, int(direction), int(precision),
+        self.ptr = _lib.ft_gpu_plan_batch(ctypes.cast(pls, _vp), kind, int(direction), int(precision),
                                           ctypes.cast(ptrs, _vp), ctypes.cast(ms, _vp), count)
         if not self.ptr:
             raise GPUError("ft_gpu_plan_batch: %s" % _lib.ft_gpu_error_string(_lib.ft_gpu_last_error()).decode())
@@ -412,16 +432,17 @@ class BatchPlan:
 
 
 def valid_rows(kind, n, j):
-    """Valid rows of column j in each layout (src/ftutilities.c:67-113)."""
+    """Valid rows of column j in each full layout (src/ftutilities.c:67-129;
+    sphv uses the sphere's, rectdisk with M = n keeps rows i < n - j)."""
     kind = _KIND.get(kind, kind)
-    if kind == SPH:
+    if kind in (SPH, SPHV):
         return n - (j + 1) // 2
-    if kind == TRI:
+    if kind in (TRI, RECTDISK):
         return n - j
     return n - (j + 3) // 4
 
 
 def full_M(kind, n):
-    """Columns of the full n-row layout: 2n-1 (sph), n (tri), 4n-3 (disk)."""
+    """Columns of the full n-row layout: 2n-1 (sph, sphv), n (tri, rectdisk), 4n-3 (disk)."""
     kind = _KIND.get(kind, kind)
-    return {SPH: 2 * n - 1, TRI: n, DISK: 4 * n - 3}[kind]
+    return {SPH: 2 * n - 1, SPHV: 2 * n - 1, TRI: n, RECTDISK: n, DISK: 4 * n - 3}[kind]

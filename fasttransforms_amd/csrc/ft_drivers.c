@@ -108,12 +108,17 @@ void ftamd_choose(int kind, int precision, int dir, int batch, int n, ftamd_choi
     c->wg_per_cu = kg ? kg : (found && tg) ? tg : (batch ? 2 : 0);
 }
 
+static int kind_geometry(int kind, int *layout);
+
+/* Choices are made per table geometry: sphv and rectdisk launches use the
+ * sphere's. */
 int ft_gpu_tuned_choice(int kind, int precision, int direction, int batch, int n, int *choice) {
-    if (kind < 0 || kind > 2 || (precision != 0 && precision != 1) || (direction != 0 && direction != 1) ||
+    int layout, geo = kind_geometry(kind, &layout);
+    if (geo < 0 || (precision != 0 && precision != 1) || (direction != 0 && direction != 1) ||
         (batch != 0 && batch != 1) || n < 1 || !choice)
         return FT_GPU_EINVAL;
     ftamd_choice c;
-    ftamd_choose(kind, precision, direction, batch, n, &c);
+    ftamd_choose(geo, precision, direction, batch, n, &c);
     choice[0] = c.engine;
     choice[1] = c.maxpart;
     choice[2] = c.L;
@@ -499,8 +504,11 @@ int ft_gpu_synchronize(void *stream) {
 }
 
 /* Columns of one n x M matrix, ordered so that equal-need columns are adjacent:
- * sph/disk by order parity then increasing order, tri by increasing order. */
-static int matrix_columns(int kind, int n, int M, void *dA, ft_col **out, int *ncols) {
+ * sph, sphv, rectdisk and disk by order parity then increasing order, tri by
+ * increasing order.  Column j of order m: sph and disk j = 2m-1, 2m
+ * (drivers.c:94-110, 414-430), sphv j = 2m+1, 2m+2 (drivers.c:204-220),
+ * rectdisk and tri j = m (drivers.c:496-508, 314-326). */
+static int matrix_columns(int kind, int layout, int n, int M, void *dA, ft_col **out, int *ncols) {
     /* the kernels index a matrix with 32-bit element offsets */
     if ((long long)n * M > 0x7fffffffLL) return FTAMD_EINVAL;
     ft_col *cols = malloc(sizeof(ft_col) * (size_t)(M > 0 ? M : 1));
@@ -513,9 +521,11 @@ static int matrix_columns(int kind, int n, int M, void *dA, ft_col **out, int *n
         }
     } else {
         for (int par = 0; par < 2; par++)
-            for (int j = 3; j < M; j++) {
-                int m = (j + 1) / 2;
-                if (m % 2 != par) continue;
+            for (int j = 2; j < M; j++) {
+                int m = layout == FTAMD_LAYOUT_RECTDISK ? j : layout == FTAMD_LAYOUT_SPHV ? (j - 1) / 2 : (j + 1) / 2;
+                if (m < 2 || m % 2 != par) continue;
+                /* sphv: orders m <= M/2-1 only (the last column of an even M is not one of theirs) */
+                if (layout == FTAMD_LAYOUT_SPHV && m > M / 2 - 1) continue;
                 ft_col c = {dA, n, 1, (long long)n * j, m - 2, par};
                 cols[k++] = c;
             }
@@ -544,13 +554,14 @@ static int build_from_columns(ft_schedule *S, int kind, int dir, int L, int W, c
 }
 
 /* Schedule for the whole matrix (cached in the plan; caller holds RP->lock). */
-static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision, int M, void *dA, int *rc) {
+static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int layout, int dir, int precision, int M, void *dA,
+                                    int *rc) {
     ftamd_choice ch;
     ftamd_choose(RP->kind, precision, dir, 0, RP->n, &ch);
     const int L = ch.L, W = ch.W;
     for (ft_schedule *s = RP->scheds; s; s = s->next)
-        if (!s->single && !s->sys && s->M == M && s->dir == dir && s->precision == precision && s->L == L && s->W == W &&
-            s->wg_per_cu == ch.wg_per_cu) {
+        if (!s->single && !s->sys && s->layout == layout && s->M == M && s->dir == dir && s->precision == precision &&
+            s->L == L && s->W == W && s->wg_per_cu == ch.wg_per_cu) {
             /* Matrix pointer may change between calls: groups hold it. */
             return s;
         }
@@ -560,6 +571,7 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
         return NULL;
     }
     S->M = M;
+    S->layout = layout;
     S->dir = dir;
     S->precision = precision;
     S->L = L;
@@ -567,7 +579,7 @@ static ft_schedule *matrix_schedule(ft_rotation_plan *RP, int dir, int precision
     S->wg_per_cu = ch.wg_per_cu;
     ft_col *cols = NULL;
     int ncols = 0;
-    *rc = matrix_columns(RP->kind, RP->n, M, dA, &cols, &ncols);
+    *rc = matrix_columns(RP->kind, layout, RP->n, M, dA, &cols, &ncols);
     if (*rc == FTAMD_OK)
         *rc = build_from_columns(S, RP->kind, dir, L, W, cols, ncols, ftamd_plan_table(RP, precision), RP->n);
     free(cols);
@@ -686,10 +698,14 @@ static int sys_upload(ft_schedule *S, const FtamdSysChain *ch, int nch, int npro
     return FTAMD_OK;
 }
 
+/* Whether both columns of every chain row share one order (sph, sphv and
+ * disk pair an order's two columns; tri and rectdisk have one per order). */
+static int sys_rowmask(int kind, int layout) { return kind != FTAMD_TRI && layout != FTAMD_LAYOUT_RECTDISK; }
+
 /* Band-systolic schedule of the whole matrix (cached in the plan; caller
  * holds RP->lock).  Returns NULL with *rc == FTAMD_OK where the engine does
  * not apply. */
-static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, int M, int *rc) {
+static ft_schedule *sys_schedule(ft_rotation_plan *RP, int layout, int dir, int precision, int M, int *rc) {
     *rc = FTAMD_OK;
     int Sq = ftamd_sys_choose_S(RP->kind, RP->n, precision);
     if (Sq == 0 || (long long)RP->n * M > 0x7fffffffLL) return NULL;
@@ -697,7 +713,9 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     ftamd_choose(RP->kind, precision, dir, 0, RP->n, &choice);
     const int maxpart = choice.maxpart;
     for (ft_schedule *s = RP->scheds; s; s = s->next)
-        if (s->sys && s->M == M && s->dir == dir && s->precision == precision && s->maxpart == maxpart) return s;
+        if (s->sys && s->layout == layout && s->M == M && s->dir == dir && s->precision == precision &&
+            s->maxpart == maxpart)
+            return s;
     if (!RP->sys[dir][precision]) {
         *rc = ftamd_sys_tables_build(RP, dir, precision, Sq, &RP->sys[dir][precision]);
         if (*rc != FTAMD_OK) return NULL;
@@ -705,7 +723,7 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     const ft_sys_tables *T = RP->sys[dir][precision];
     FtamdSysChain *ch = NULL;
     int nch = 0, nprog = 0;
-    *rc = ftamd_sys_chains(RP, T, M, maxpart, &ch, &nch, &nprog);
+    *rc = ftamd_sys_chains(RP, T, M, layout, maxpart, &ch, &nch, &nprog);
     if (*rc != FTAMD_OK) return NULL;
     ft_schedule *S = calloc(1, sizeof(ft_schedule));
     if (!S) {
@@ -715,6 +733,7 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     }
     S->sys = 1;
     S->M = M;
+    S->layout = layout;
     S->dir = dir;
     S->precision = precision;
     S->maxpart = maxpart;
@@ -726,7 +745,7 @@ static ft_schedule *sys_schedule(ft_rotation_plan *RP, int dir, int precision, i
     }
     FtamdSysLaunch *a = &S->sys_launch;
     a->bands = ftamd_sys_tables_dev_bands(T);
-    a->rowmask = RP->kind != FTAMD_TRI;
+    a->rowmask = sys_rowmask(RP->kind, layout);
     a->tab = ftamd_sys_tables_dev_tab(T);
     S->L = ftamd_sys_tables_S(T);
     S->next = RP->scheds;
@@ -770,16 +789,46 @@ static int launch_checked(ft_schedule *S, int precision, int kind, int dir, void
 
 /* --------------------------------------------------------- executors -- */
 
+/* Table geometry (FTAMD_SPH, _TRI, _DISK) of a public kind (FT_GPU_SPH ..
+ * FT_GPU_RECTDISK) and its column layout; -1 for an unknown kind.  sphv and
+ * rectdisk run the sphere's kernels (drivers.c:204-220, rotations_default.c:73-85). */
+static int kind_geometry(int kind, int *layout) {
+    *layout = FTAMD_LAYOUT_STD;
+    switch (kind) {
+    case FT_GPU_SPH: return FTAMD_SPH;
+    case FT_GPU_TRI: return FTAMD_TRI;
+    case FT_GPU_DISK: return FTAMD_DISK;
+    case FT_GPU_SPHV: *layout = FTAMD_LAYOUT_SPHV; return FTAMD_SPH;
+    case FT_GPU_RECTDISK: *layout = FTAMD_LAYOUT_RECTDISK; return FTAMD_SPH;
+    }
+    return -1;
+}
+
+/* Matrix widths each executor accepts: every column it touches exists and
+ * every sequence it applies is in the plan (n x n tables; disk n x 2n-1). */
 static int valid_M(int kind, int n, int M) {
     if (M < 0) return 0;
-    if (kind == FTAMD_TRI) return M <= n;
-    if (M == 0) return 1;
-    if (M % 2 == 0) return 0;
-    return kind == FTAMD_SPH ? M <= 2 * n - 1 : M <= 4 * n - 3;
+    switch (kind) {
+    case FT_GPU_TRI:
+    case FT_GPU_RECTDISK: return M <= n;
+    case FT_GPU_SPH: return M == 0 || (M % 2 == 1 && M <= 2 * n - 1);
+    case FT_GPU_SPHV: return M == 0 || (M % 2 == 1 && M <= 2 * n + 1);
+    case FT_GPU_DISK: return M == 0 || (M % 2 == 1 && M <= 4 * n - 3);
+    }
+    return 0;
+}
+
+/* A plan of the kind's table geometry and a valid width (the reference does
+ * not check; a plan of another geometry would be read out of bounds).
+ * Sets *geo and *layout. */
+static int valid_call(const ft_rotation_plan *RP, int kind, int M, int *geo, int *layout) {
+    *geo = kind_geometry(kind, layout);
+    return RP && *geo >= 0 && RP->kind == *geo && valid_M(kind, RP->n, M);
 }
 
 /* Schedule for one launch on RP (caller holds no lock). */
-static int pick_schedule(ft_rotation_plan *RP, int dir, int precision, void *dA, int M, ft_schedule **out) {
+static int pick_schedule(ft_rotation_plan *RP, int layout, int dir, int precision, void *dA, int M,
+                         ft_schedule **out) {
     int rc = FTAMD_OK;
     ftamd_choice ch;
     ftamd_choose(RP->kind, precision, dir, 0, RP->n, &ch);
@@ -787,10 +836,10 @@ static int pick_schedule(ft_rotation_plan *RP, int dir, int precision, void *dA,
     rc = ftamd_plan_upload(RP, precision);
     ft_schedule *S = NULL;
     if (rc == FTAMD_OK && ch.engine == FT_GPU_ENGINE_SYSTOLIC) {
-        S = sys_schedule(RP, dir, precision, M, &rc);
+        S = sys_schedule(RP, layout, dir, precision, M, &rc);
         if (rc == FTAMD_OK && !S && ch.forced) rc = FTAMD_EINVAL;
     }
-    if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, dir, precision, M, dA, &rc);
+    if (rc == FTAMD_OK && !S) S = matrix_schedule(RP, layout, dir, precision, M, dA, &rc);
     pthread_mutex_unlock(&RP->lock);
     *out = S;
     if (rc == FTAMD_OK) t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
@@ -800,12 +849,13 @@ static int pick_schedule(ft_rotation_plan *RP, int dir, int precision, void *dA,
 int ft_gpu_execute(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
                    void *stream) {
     ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
-    if (!RP || !dA || RP->kind != kind || (direction != 0 && direction != 1) || (precision != 0 && precision != 1) ||
-        !valid_M(kind, RP->n, M))
+    int geo, layout;
+    if (!valid_call(RP, kind, M, &geo, &layout) || !dA || (direction != 0 && direction != 1) ||
+        (precision != 0 && precision != 1))
         return FT_GPU_EINVAL;
     if (M == 0) return FT_GPU_OK;
     ft_schedule *S = NULL;
-    int rc = pick_schedule(RP, direction, precision, dA, M, &S);
+    int rc = pick_schedule(RP, layout, direction, precision, dA, M, &S);
     if (rc != FTAMD_OK) return rc;
     rc = launch(S, precision, RP->kind, direction, dA, stream);
     if (rc == FTAMD_OK) pending_add(S, stream);
@@ -842,8 +892,9 @@ static const int g_na_zeros[NA_MAX_CHAINS];
 int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction, int precision, void *dA, int M,
                            void *stream) {
     ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
-    if (!RP || !dA || RP->kind != kind || (direction != 0 && direction != 1) || (precision != 0 && precision != 1) ||
-        !valid_M(kind, RP->n, M))
+    int geo, layout;
+    if (!valid_call(RP, kind, M, &geo, &layout) || !dA || (direction != 0 && direction != 1) ||
+        (precision != 0 && precision != 1))
         return FT_GPU_EINVAL;
     if (M == 0) return FT_GPU_OK;
     if ((long long)RP->n * M > 0x7fffffffLL) return FT_GPU_EINVAL;
@@ -861,11 +912,10 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
     pthread_mutex_lock(&g_na_lock);
     /* the scratch holds the work list as a schedule does (sys_layout) */
     int nfull = 0, nch = 0, nprog = 0, nxcd = 1, nsx = 1;
-    rc = ftamd_sys_chains_fill(RP, T, M, g_na_chains, (int)NA_MAX_CHAINS, &nfull);
-    if (rc == FTAMD_OK)
-    {
+    rc = ftamd_sys_chains_fill(RP, T, M, layout, g_na_chains, (int)NA_MAX_CHAINS, &nfull);
+    if (rc == FTAMD_OK) {
         ftamd_choice ch;
-        ftamd_choose(kind, precision, direction, 0, RP->n, &ch);
+        ftamd_choose(geo, precision, direction, 0, RP->n, &ch);
         rc = ftamd_sys_split(g_na_chains, nfull, (int)NA_MAX_CHAINS, ch.maxpart, &nch, &nprog);
     }
     if (rc == FTAMD_OK) {
@@ -893,7 +943,7 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP_, int kind, int direction,
         memset(&b, 0, sizeof(b));
         sys_fill_launch(&b, d, &L, nch, nprog, nslots, nxcd, nsx);
         b.bands = ftamd_sys_tables_dev_bands(T);
-        b.rowmask = kind != FTAMD_TRI;
+        b.rowmask = sys_rowmask(geo, layout);
         b.tab = ftamd_sys_tables_dev_tab(T);
         b.A = dA;
         int spin = KNOB_GET(g_spin_limit);
@@ -930,7 +980,8 @@ static int stage_reserve(ft_rotation_plan *RP, size_t bytes) {
  * another and never share the staging buffer or a schedule's workspace. */
 static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, int M) {
     ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
-    if (!RP || !A || RP->kind != kind || !valid_M(kind, RP->n, M)) {
+    int geo, layout;
+    if (!valid_call(RP, kind, M, &geo, &layout) || !A) {
         ftamd_set_error(FT_GPU_EINVAL);
         return;
     }
@@ -951,7 +1002,7 @@ static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, i
         d = RP->d_stage;
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, bytes, NULL);
     }
-    if (rc == FTAMD_OK) rc = pick_schedule(RP, dir, 0, d, M, &S);
+    if (rc == FTAMD_OK) rc = pick_schedule(RP, layout, dir, 0, d, M, &S);
     if (rc == FTAMD_OK) rc = launch_checked(S, 0, RP->kind, dir, d);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, bytes, NULL);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
@@ -962,29 +1013,47 @@ static void execute(const ft_rotation_plan *RP_, int kind, int dir, double *A, i
 /* drivers.c:58-92 */
 void ft_execute_sph_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_SPH, 0, A, M);
+    execute(RP, FT_GPU_SPH, 0, A, M);
 }
 void ft_execute_sph_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_SPH, 1, A, M);
+    execute(RP, FT_GPU_SPH, 1, A, M);
+}
+/* drivers.c:168-202 */
+void ft_execute_sphv_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FT_GPU_SPHV, 0, A, M);
+}
+void ft_execute_sphv_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FT_GPU_SPHV, 1, A, M);
 }
 /* drivers.c:278-312 */
 void ft_execute_tri_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_TRI, 0, A, M);
+    execute(RP, FT_GPU_TRI, 0, A, M);
 }
 void ft_execute_tri_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_TRI, 1, A, M);
+    execute(RP, FT_GPU_TRI, 1, A, M);
 }
 /* drivers.c:378-412 */
 void ft_execute_disk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_DISK, 0, A, M);
+    execute(RP, FT_GPU_DISK, 0, A, M);
 }
 void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
     (void)B;
-    execute(RP, FTAMD_DISK, 1, A, M);
+    execute(RP, FT_GPU_DISK, 1, A, M);
+}
+/* drivers.c:488-494 */
+void ft_execute_rectdisk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FT_GPU_RECTDISK, 0, A, M);
+}
+void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M) {
+    (void)B;
+    execute(RP, FT_GPU_RECTDISK, 1, A, M);
 }
 
 /* ------------------------------------------------- single vectors -- */
@@ -1126,6 +1195,14 @@ void ft_kernel_disk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2
 void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
     kernel_one(RP, FTAMD_DISK, 1, m1, m2, A, S);
 }
+/* rotations.c:289-295: the sphere's kernels on a rectangle-disk plan
+ * (rotations_default.c:73-85) */
+void ft_kernel_rectdisk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_SPH, 0, m1, m2, A, S);
+}
+void ft_kernel_rectdisk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
+    kernel_one(RP, FTAMD_SPH, 1, m1, m2, A, S);
+}
 
 /* ------------------------------------------------------------ batch -- */
 
@@ -1234,15 +1311,21 @@ static int batch_cache_store(ft_schedule *S, const ft_rotation_plan *const *plan
     return FTAMD_OK;
 }
 /* Validates a batch and uploads the largest plan's table; *big gets that plan. */
+/* Validates a batch: plans of one family and one set of parameters (their
+ * tables are one closed form), each of the kind's geometry with a valid
+ * width.  Sets *geo and *layout of the kind and *big, the largest plan. */
 static int batch_check(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
-                       void *const *dA, const int *M, int count, ft_rotation_plan **big) {
+                       void *const *dA, const int *M, int count, int *geo, int *layout, ft_rotation_plan **big) {
     if (!plans || !dA || !M || count < 0 || (direction != 0 && direction != 1) || (precision != 0 && precision != 1))
         return FT_GPU_EINVAL;
     *big = NULL;
+    *geo = kind_geometry(kind, layout);
+    if (*geo < 0) return FT_GPU_EINVAL;
     for (int i = 0; i < count; i++) {
         const ft_rotation_plan *P = plans[i];
-        if (!P || !dA[i] || P->kind != kind || !valid_M(kind, P->n, M[i])) return FT_GPU_EINVAL;
-        if (P->alpha != plans[0]->alpha || P->beta != plans[0]->beta || P->gamma != plans[0]->gamma)
+        if (!valid_call(P, kind, M[i], geo, layout) || !dA[i]) return FT_GPU_EINVAL;
+        if (P->family != plans[0]->family || P->alpha != plans[0]->alpha || P->beta != plans[0]->beta ||
+            P->gamma != plans[0]->gamma)
             return FT_GPU_EINVAL;
         if (!*big || P->n > (*big)->n) *big = (ft_rotation_plan *)P;
     }
@@ -1258,7 +1341,7 @@ static int batch_check(const ft_rotation_plan *const *plans, int kind, int direc
 /* Band-systolic schedule of a validated batch, or NULL (rc FTAMD_OK) where
  * some transform does not fit the engine or the transforms would need
  * different band widths. */
-static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int layout, int direction, int precision,
                            void *const *dA, const int *M, int count, int maxpart, ft_schedule **out) {
     *out = NULL;
     int S0 = 0;
@@ -1283,7 +1366,7 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
     int nb = 0, nch = 0, nprog = 0;
     const void *tab_base = NULL;
     if (rc == FTAMD_OK)
-        rc = ftamd_sys_batch(plans, tabs, dA, M, count, maxpart, &bands, &nb, &ch, &nch, &nprog, &tab_base);
+        rc = ftamd_sys_batch(plans, tabs, dA, M, count, layout, maxpart, &bands, &nb, &ch, &nch, &nprog, &tab_base);
     free(tabs);
     if (rc != FTAMD_OK) {
         free(bands);
@@ -1302,13 +1385,14 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
         return rc;
     }
     S->sys = 1;
+    S->layout = layout;
     S->dir = direction;
     S->precision = precision;
     S->maxpart = maxpart;
     S->L = S0;
     FtamdSysLaunch *a = &S->sys_launch;
     a->bands = (const FtamdSysBand *)dbands;
-    a->rowmask = kind != FTAMD_TRI;
+    a->rowmask = sys_rowmask(kind, layout);
     a->tab = tab_base;
     a->A = dA[0];
     *out = S;
@@ -1319,11 +1403,11 @@ static int batch_build_sys(const ft_rotation_plan *const *plans, int kind, int d
  * band-systolic one when forced (it keeps one chain per CU, and the many
  * short chains of a batch of small transforms leave most of each CU idle:
  * 20.2 ms against 10.4 ms for 512 sph transforms of degree 32..1018). */
-static int batch_build(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
+static int batch_build(const ft_rotation_plan *const *plans, int kind, int layout, int direction, int precision,
                        void *const *dA, const int *M, int count, ft_rotation_plan *big, const ftamd_choice *ch,
                        ft_schedule **out) {
     if (ch->engine == FT_GPU_ENGINE_SYSTOLIC) {
-        int rs = batch_build_sys(plans, kind, direction, precision, dA, M, count, ch->maxpart, out);
+        int rs = batch_build_sys(plans, kind, layout, direction, precision, dA, M, count, ch->maxpart, out);
         if (rs != FTAMD_OK) return rs;
         if (*out) return FTAMD_OK;
         if (ch->forced) return FTAMD_EINVAL;
@@ -1338,7 +1422,7 @@ static int batch_build(const ft_rotation_plan *const *plans, int kind, int direc
     for (int i = 0; rc == FTAMD_OK && i < count; i++) {
         ft_col *one = NULL;
         int k = 0;
-        rc = matrix_columns(kind, plans[i]->n, M[i], dA[i], &one, &k);
+        rc = matrix_columns(kind, layout, plans[i]->n, M[i], dA[i], &one, &k);
         if (rc == FTAMD_OK) memcpy(cols + ncols, one, sizeof(ft_col) * (size_t)k);
         ncols += k;
         free(one);
@@ -1349,6 +1433,7 @@ static int batch_build(const ft_rotation_plan *const *plans, int kind, int direc
         rc = S ? FTAMD_OK : FTAMD_ENOMEM;
     }
     if (rc == FTAMD_OK) {
+        S->layout = layout;
         S->dir = direction;
         S->precision = precision;
         S->L = L;
@@ -1370,10 +1455,11 @@ static int batch_build(const ft_rotation_plan *const *plans, int kind, int direc
 int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                          void *const *dA, const int *M, int count, void *stream) {
     ft_rotation_plan *big = NULL;
-    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
+    int geo, layout;
+    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &geo, &layout, &big);
     if (rc != FTAMD_OK || count == 0) return rc;
     ftamd_choice ch;
-    ftamd_choose(kind, precision, direction, 1, big->n, &ch);
+    ftamd_choose(geo, precision, direction, 1, big->n, &ch);
     /* the cache key holds every choice the schedule was built with */
     const int L = ch.L, W = ch.W, engine = ch.engine * 2 + ch.forced, part = ch.maxpart * 16 + ch.wg_per_cu;
     pthread_mutex_lock(&g_batch.lock);
@@ -1381,7 +1467,7 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
     if (!batch_cache_hit(plans, kind, direction, precision, dA, M, count, L, W, engine, part)) {
         batch_cache_clear_locked();
         S = NULL;
-        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &ch, &S);
+        rc = batch_build(plans, geo, layout, direction, precision, dA, M, count, big, &ch, &S);
         if (rc == FTAMD_OK)
             rc = batch_cache_store(S, plans, kind, direction, precision, dA, M, count, L, W, engine, part);
         if (rc != FTAMD_OK) {
@@ -1390,7 +1476,7 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
         }
     }
     t_last_engine = S->sys ? FT_GPU_ENGINE_SYSTOLIC : FT_GPU_ENGINE_PIPELINE;
-    rc = launch(S, precision, kind, direction, NULL, stream);
+    rc = launch(S, precision, geo, direction, NULL, stream);
     if (rc == FTAMD_OK) pending_add(S, stream);
     pthread_mutex_unlock(&g_batch.lock);
     return rc;
@@ -1400,13 +1486,14 @@ int ft_gpu_execute_batch(const ft_rotation_plan *const *plans, int kind, int dir
 
 struct ft_gpu_batch_s {
     ft_schedule *S;
-    int kind, dir, precision, count;
+    int geo, dir, precision, count; /* geo: the table geometry of its kind */
 };
 
 ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, int direction, int precision,
                                 void *const *dA, const int *M, int count) {
     ft_rotation_plan *big = NULL;
-    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &big);
+    int geo = -1, layout = 0;
+    int rc = batch_check(plans, kind, direction, precision, dA, M, count, &geo, &layout, &big);
     if (rc == FTAMD_OK && count == 0) rc = FTAMD_EINVAL;
     ft_gpu_batch *B = NULL;
     if (rc == FTAMD_OK) {
@@ -1415,15 +1502,15 @@ ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, 
     }
     if (rc == FTAMD_OK) {
         ftamd_choice ch;
-        ftamd_choose(kind, precision, direction, 1, big->n, &ch);
-        rc = batch_build(plans, kind, direction, precision, dA, M, count, big, &ch, &B->S);
+        ftamd_choose(geo, precision, direction, 1, big->n, &ch);
+        rc = batch_build(plans, geo, layout, direction, precision, dA, M, count, big, &ch, &B->S);
     }
     if (rc != FTAMD_OK) {
         free(B);
         ftamd_set_error(rc);
         return NULL;
     }
-    B->kind = kind;
+    B->geo = geo;
     B->dir = direction;
     B->precision = precision;
     B->count = count;
@@ -1432,7 +1519,7 @@ ft_gpu_batch *ft_gpu_plan_batch(const ft_rotation_plan *const *plans, int kind, 
 
 int ft_gpu_execute_batch_plan(const ft_gpu_batch *B, void *stream) {
     if (!B) return FT_GPU_EINVAL;
-    int rc = launch(B->S, B->precision, B->kind, B->dir, NULL, stream);
+    int rc = launch(B->S, B->precision, B->geo, B->dir, NULL, stream);
     if (rc == FTAMD_OK) pending_add(B->S, stream);
     return rc;
 }

@@ -15,6 +15,7 @@ typedef struct ft_schedule {
     struct ft_schedule *next;
     /* key (single-vector schedules: m1, m2 = first and last sequence j, rs = stride) */
     int M, dir, precision, L, W, rs, m1, m2, single;
+    int layout;              /* FTAMD_LAYOUT_* of its columns (part of the key) */
     /* launched asynchronously since its error word was last read (ft_gpu_synchronize) */
     int pending;
     void *pending_stream;
@@ -41,7 +42,8 @@ struct ft_rotation_plan_s {
     double *c;
     int n;
     /* extension */
-    int kind;
+    int kind;                /* geometry of the tables: FTAMD_SPH, _TRI or _DISK */
+    int family;              /* its constructor: FT_GPU_SPH, _TRI, _DISK or _RECTDISK (batches need one) */
     double alpha, beta, gamma;
     long nsc;                /* entries per table */
     int device;              /* HIP device of the uploads, -1 before upload */
@@ -58,7 +60,7 @@ struct ft_rotation_plan_s {
 typedef struct ft_sys_tables ft_sys_tables;
 
 /* ft_plans.c */
-ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, double gamma);
+ft_rotation_plan *ftamd_plan_alloc(int kind, int family, int n, double alpha, double beta, double gamma);
 int ftamd_plan_upload(ft_rotation_plan *RP, int precision);
 const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
 
@@ -92,16 +94,17 @@ void ftamd_sys_tables_free(ft_sys_tables *T);
 int ftamd_sys_tables_S(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_bands(const ft_sys_tables *T);
 const void *ftamd_sys_tables_dev_tab(const ft_sys_tables *T);
-int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int maxpart, FtamdSysChain **out,
-                     int *count, int *nprog);
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, int maxpart,
+                     FtamdSysChain **out, int *count, int *nprog);
 int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands, int nxcd, int nsx,
                      FtamdSysItem *items, int *slot_items, int *xcd_slot, long *ws_load, int *ws_chain);
 const FtamdSysBand *ftamd_sys_tables_host_bands(const ft_sys_tables *T);
-int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
-                          int *count);
+int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, FtamdSysChain *buf,
+                          int cap, int *count);
 int ftamd_sys_split(FtamdSysChain *buf, int nfull, int cap, int maxpart, int *count, int *nprog);
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
-                    const int *M, int count, int maxpart, FtamdSysBand **bands, int *nbands, FtamdSysChain **chains,
+                    const int *M, int count, int layout, int maxpart, FtamdSysBand **bands, int *nbands,
+                    FtamdSysChain **chains,
                     int *nchains, int *nprog, const void **tab_base);
 
 #endif /* FT_INTERNAL_H */

@@ -22,7 +22,7 @@ static int g_num_threads = 1;
 /* fasttransforms.h:295.  Recorded only: no host threads compute here. */
 void ft_set_num_threads(const int n) { g_num_threads = n; }
 
-ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, double gamma) {
+ft_rotation_plan *ftamd_plan_alloc(int kind, int family, int n, double alpha, double beta, double gamma) {
     if (n < 1) {
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
@@ -34,6 +34,7 @@ ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, d
     }
     RP->n = n;
     RP->kind = kind;
+    RP->family = family;
     RP->alpha = alpha;
     RP->beta = beta;
     RP->gamma = gamma;
@@ -57,7 +58,7 @@ ft_rotation_plan *ftamd_plan_alloc(int kind, int n, double alpha, double beta, d
 
 /* rotations.c:116-133 */
 ft_rotation_plan *ft_plan_rotsphere(const int n) {
-    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_SPH, n, 0.0, 0.0, 0.0);
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_SPH, FT_GPU_SPH, n, 0.0, 0.0, 0.0);
     if (!RP) return NULL;
     for (int m = 0; m < n; m++)
         for (int l = 0; l < n - m; l++) {
@@ -72,7 +73,7 @@ ft_rotation_plan *ft_plan_rotsphere(const int n) {
 
 /* rotations.c:143-160 */
 ft_rotation_plan *ft_plan_rottriangle(const int n, const double alpha, const double beta, const double gamma) {
-    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_TRI, n, alpha, beta, gamma);
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_TRI, FT_GPU_TRI, n, alpha, beta, gamma);
     if (!RP) return NULL;
     for (int m = 0; m < n; m++)
         for (int l = 0; l < n - m; l++) {
@@ -87,7 +88,7 @@ ft_rotation_plan *ft_plan_rottriangle(const int n, const double alpha, const dou
 
 /* rotations.c:173-190 */
 ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double beta) {
-    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_DISK, n, alpha, beta, 0.0);
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_DISK, FT_GPU_DISK, n, alpha, beta, 0.0);
     if (!RP) return NULL;
     for (int m = 0; m < 2 * n - 1; m++)
         for (int l = 0; l < n - (m + 1) / 2; l++) {
@@ -96,6 +97,23 @@ ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double 
             double den = (((l + m) + alpha) + 2) * ((((l + m) + alpha) + beta) + 2);
             RP->s[idx_disk(n, l, m)] = -sqrt(nums / den);
             RP->c[idx_disk(n, l, m)] = sqrt(numc / den);
+        }
+    return RP;
+}
+
+/* rotations.c:270-287.  The sphere's table layout and kernels; each term is
+ * summed left to right in double as the reference writes it, the numerator
+ * of s an exact integer product. */
+ft_rotation_plan *ft_plan_rotrectdisk(const int n, const double beta) {
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_SPH, FT_GPU_RECTDISK, n, 0.0, beta, 0.0);
+    if (!RP) return NULL;
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)((l + 1) * (l + 2));
+            double numc = (((2 * m) + 2 * beta) + 3) * ((((2 * l + 2 * m) + 2 * beta)) + 6);
+            double den = (((l + 2 * m) + 2 * beta) + 4) * (((l + 2 * m) + 2 * beta) + 5);
+            RP->s[idx_tri(n, l, m)] = sqrt(nums / den);
+            RP->c[idx_tri(n, l, m)] = sqrt(numc / den);
         }
     return RP;
 }

@@ -236,22 +236,30 @@ static void sort_chains(FtamdSysChain *c, int n) {
 }
 
 /* Whole chains of an n x M matrix: per class, the columns in groups of 8
- * (four orders of the class's parity x (cos, sin) columns for sph and disk,
- * eight consecutive orders for tri), each with the bands its columns need,
- * written to buf (at most cap records, longest first) without allocating.
- * The records are not split yet (ftamd_sys_split).  FTAMD_ENOMEM if more
- * than cap chains are needed (*count then holds the number needed). */
-int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain *buf, int cap,
-                          int *count) {
+ * (four orders of the class's parity x their two columns for sph, sphv and
+ * disk, eight orders of the parity for the rectangle-disk, eight consecutive
+ * orders for tri), each with the bands its columns need, written to buf (at
+ * most cap records, longest first) without allocating.  `layout` is the
+ * column layout (FTAMD_LAYOUT_*) of a sphere-geometry matrix.  The records
+ * are not split yet (ftamd_sys_split).  FTAMD_ENOMEM if more than cap chains
+ * are needed (*count then holds the number needed). */
+int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, FtamdSysChain *buf,
+                          int cap, int *count) {
     const int kind = RP->kind, B = FTAMD_SYS_LANES * T->S;
+    /* orders per chain step and the highest order of the layout
+     * (drivers.c:94-110, 204-220, 314-326, 414-430, 496-508) */
+    const int one_col = kind == FTAMD_TRI || layout == FTAMD_LAYOUT_RECTDISK;
+    const int mtop = kind == FTAMD_TRI || layout == FTAMD_LAYOUT_RECTDISK ? M - 1
+                     : layout == FTAMD_LAYOUT_SPHV                        ? M / 2 - 1
+                                                                          : (M - 1) / 2;
     int nch = 0;
     for (int c = 0; c < T->ncls; c++) {
         const sys_class *cl = &T->cls[c];
         /* orders of the class, highest first */
-        int mhi = kind == FTAMD_TRI ? M - 1 : (M - 1) / 2;
+        int mhi = mtop;
         if (kind != FTAMD_TRI && mhi % 2 != cl->jpar) mhi--;
         int mlo = kind == FTAMD_TRI ? 1 : 2;
-        for (int m0 = mhi; m0 >= mlo; m0 -= 8) {
+        for (int m0 = mhi; m0 >= mlo; m0 -= kind == FTAMD_TRI ? 8 : 8 * (one_col ? 2 : 1)) {
             FtamdSysChain x;
             memset(&x, 0, sizeof(x));
             int kmin = INT_MAX, any = 0;
@@ -260,9 +268,12 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
                 if (kind == FTAMD_TRI) {
                     m = m0 - q;
                     col = m;
+                } else if (one_col) {
+                    m = m0 - 2 * q;
+                    col = m;
                 } else {
                     m = m0 - 2 * (q / 2);
-                    col = 2 * m - 1 + (q % 2);
+                    col = (layout == FTAMD_LAYOUT_SPHV ? 2 * m + 1 : 2 * m - 1) + (q % 2);
                 }
                 int jfirst = kind == FTAMD_TRI ? m - 1 : m - 2;
                 int ok = m >= mlo && col < M && jfirst <= cl->jtop && jfirst >= cl->jtop - cl->jstep * (cl->K - 1);
@@ -425,14 +436,15 @@ int ftamd_sys_assign(const FtamdSysChain *ch, int nch, const FtamdSysBand *bands
 }
 
 /* Whole chains in a malloc'd array the caller frees. */
-static int chains_full(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, FtamdSysChain **out, int *count) {
+static int chains_full(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, FtamdSysChain **out,
+                       int *count) {
     *out = NULL;
     *count = 0;
     int need = 0;
-    ftamd_sys_chains_fill(RP, T, M, NULL, 0, &need);
+    ftamd_sys_chains_fill(RP, T, M, layout, NULL, 0, &need);
     FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)(need > 0 ? need : 1));
     if (!ch) return FTAMD_ENOMEM;
-    int rc = ftamd_sys_chains_fill(RP, T, M, ch, need, count);
+    int rc = ftamd_sys_chains_fill(RP, T, M, layout, ch, need, count);
     if (rc != FTAMD_OK) {
         free(ch);
         *count = 0;
@@ -459,11 +471,11 @@ static int split_alloc(FtamdSysChain **ch, int n, int maxpart, int *count, int *
 /* The chain parts of an n x M matrix (longest chains first, parts of at most
  * maxpart bands) in a malloc'd array the caller frees; *nprog progress
  * counters link them. */
-int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int maxpart, FtamdSysChain **out,
-                     int *count, int *nprog) {
+int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, int maxpart,
+                     FtamdSysChain **out, int *count, int *nprog) {
     int n = 0;
     *nprog = 0;
-    int rc = chains_full(RP, T, M, out, &n);
+    int rc = chains_full(RP, T, M, layout, out, &n);
     if (rc == FTAMD_OK) rc = split_alloc(out, n, maxpart, count, nprog);
     if (rc != FTAMD_OK) {
         free(*out);
@@ -479,7 +491,7 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
  * chains of every transform with its bands, matrix offset from dA[0] and
  * column stride, longest first.  FTAMD_EINVAL if a chain does not fit. */
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
-                    const int *M, int count, int maxpart, FtamdSysBand **bands_out, int *nbands_out,
+                    const int *M, int count, int layout, int maxpart, FtamdSysBand **bands_out, int *nbands_out,
                     FtamdSysChain **chains_out, int *nchains_out, int *nprog_out, const void **tab_base) {
     *bands_out = NULL;
     *chains_out = NULL;
@@ -518,7 +530,7 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
         while (seen[k] != tabs[i]) k++;
         FtamdSysChain *ch = NULL;
         int nch = 0;
-        rc = chains_full(plans[i], tabs[i], M[i], &ch, &nch);
+        rc = chains_full(plans[i], tabs[i], M[i], layout, &ch, &nch);
         if (rc == FTAMD_OK && nall + nch > cap) {
             cap = 2 * (nall + nch) + 16;
             FtamdSysChain *grown = realloc(all, sizeof(FtamdSysChain) * (size_t)cap);
@@ -558,14 +570,15 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
 /* Diagnostics (no device work): the FP64 bands, table and chain parts (of
  * at most maxpart bands) the engine would run for an n x M matrix, in malloc'd host arrays the caller frees.
  * Lets the host-side construction be checked on a machine without a GPU. */
-int ftamd_sys_debug_host(const ft_rotation_plan *RP, int dir, int S, int M, int maxpart, FtamdSysBand **bands,
-                         int *nbands, double **tab, long long *nentries, FtamdSysChain **chains, int *nchains) {
+int ftamd_sys_debug_host(const ft_rotation_plan *RP, int dir, int S, int M, int layout, int maxpart,
+                         FtamdSysBand **bands, int *nbands, double **tab, long long *nentries, FtamdSysChain **chains,
+                         int *nchains) {
     ft_sys_tables T;
     memset(&T, 0, sizeof(T));
     void *host = NULL;
     int rc = tables_host(RP, dir, 0, S, &T, &host, nentries);
     int nprog = 0;
-    if (rc == FTAMD_OK) rc = ftamd_sys_chains(RP, &T, M, maxpart, chains, nchains, &nprog);
+    if (rc == FTAMD_OK) rc = ftamd_sys_chains(RP, &T, M, layout, maxpart, chains, nchains, &nprog);
     if (rc != FTAMD_OK) {
         free(T.bands);
         free(host);

@@ -53,11 +53,27 @@ ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double 
 void ft_kernel_disk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 
-/* fasttransforms.h:349-350 (drivers.c:58-92) -- the north-star entry points.
+/* fasttransforms.h:325 (rotations.c:270-287) -- Dunkl-Xu (rectangle and
+ * disk) with parameter beta; tables bit-identical to the reference's, in the
+ * sphere's layout (n(n+1)/2 entries, s(l, m) at l + m(2n+1-m)/2). */
+ft_rotation_plan *ft_plan_rotrectdisk(const int n, const double beta);
+
+/* fasttransforms.h:328, 330 (rotations.c:289-295) -- the sphere's kernel
+ * (rows k and k+2, k <= n-3-j) on a rectangle-disk plan. */
+void ft_kernel_rectdisk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+void ft_kernel_rectdisk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
+
+/* fasttransforms.h:349-350 (drivers.c:58-92) -- spherical harmonics.
  * A is n x M (M odd, M <= 2n-1), updated in place; B is the caller's n x M
  * scratch, unused here (the GPU path needs no host scratch). */
 void ft_execute_sph_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
 void ft_execute_sph_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:352-353 (drivers.c:168-220) -- spherical vector fields
+ * on a sphere plan: order m (2 <= m <= M/2-1) in columns 2m+1 and 2m+2.  A is
+ * n x M, M odd, M <= 2n+1. */
+void ft_execute_sphv_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
+void ft_execute_sphv_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
 
 /* fasttransforms.h:355-356 (drivers.c:278-312) -- A is n x M, M <= n. */
 void ft_execute_tri_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
@@ -66,6 +82,12 @@ void ft_execute_tri_lo2hi(const ft_rotation_plan *RP, double *A, double *B, cons
 /* fasttransforms.h:358-359 (drivers.c:378-412) -- A is n x M, M odd, M <= 4n-3. */
 void ft_execute_disk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
 void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:361-362 (drivers.c:488-508) -- rectangle and disk on a
+ * rectangle-disk plan: order m (2 <= m <= M-1) in column m.  A is n x M,
+ * M <= n. */
+void ft_execute_rectdisk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
+void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
 
 /* ------------------------------------------------------------------------
  * Extensions (no counterpart in the reference).
@@ -89,8 +111,14 @@ const char *ft_gpu_error_string(int code);
  * arithmetic with s/c rounded from the FP64 plan. */
 enum { FT_GPU_FP64 = 0, FT_GPU_FP32 = 1 };
 
+/* Executor kinds of the device-resident calls: the whole-matrix executors
+ * ft_execute_{sph,tri,disk,sphv,rectdisk}_*.  A sphv call takes a sphere
+ * plan, a rectdisk call a rectangle-disk plan (any plan of the sphere's table
+ * layout runs, as in the reference). */
+enum { FT_GPU_SPH = 0, FT_GPU_TRI = 1, FT_GPU_DISK = 2, FT_GPU_SPHV = 3, FT_GPU_RECTDISK = 4 };
+
 /* Device-resident execution on a HIP stream (stream = NULL: the null stream).
- * kind: 0 sphere, 1 triangle, 2 disk; direction: 0 hi2lo, 1 lo2hi.
+ * kind: FT_GPU_SPH .. FT_GPU_RECTDISK; direction: 0 hi2lo, 1 lo2hi.
  * dA is n x M column-major in device memory (double* for FP64, float* for
  * FP32).  Asynchronous: returns after the launch; 0 on success or an FT_GPU_*
  * code.  Launches with the same (plan, M, direction, precision) share a
@@ -129,7 +157,8 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, 
 int ft_gpu_synchronize(void *stream);
 
 /* Batched transforms of uneven degree in one launch.  plans[i] may differ in
- * n (same kind and parameters); dA[i] is plans[i]->n x M[i] on the device.
+ * n (plans of one constructor and one set of parameters); dA[i] is
+ * plans[i]->n x M[i] on the device.
  * Every transform runs as one persistent, cost-ordered launch.  The schedule
  * of the last argument list is cached; a call with another list replaces it,
  * so a HIP graph that captured this call is valid only until then (use a

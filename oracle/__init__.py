@@ -19,9 +19,9 @@ import os
 
 import numpy as np
 
-SPH, TRI, DISK = 0, 1, 2
+SPH, TRI, DISK, RECTDISK = 0, 1, 2, 4
 HI2LO, LO2HI = 0, 1
-KINDS = {"sph": SPH, "tri": TRI, "disk": DISK}
+KINDS = {"sph": SPH, "tri": TRI, "disk": DISK, "rectdisk": RECTDISK}
 
 _here = os.path.dirname(os.path.abspath(__file__))
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -41,6 +41,9 @@ def _load():
         lib.oracle_plan_sph.argtypes = [ctypes.c_int, _dp, _dp]
         lib.oracle_plan_tri.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.oracle_plan_disk.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.oracle_plan_rectdisk.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        for name in ("sphv", "rectdisk"):
+            getattr(lib, "oracle_execute_" + name).argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         for name in ("sph", "tri", "disk"):
             fn = getattr(lib, "oracle_execute_" + name)
             fn.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
@@ -82,19 +85,25 @@ def _check_matrix(A, n):
 
 
 class Plan:
-    """FP64 sine/cosine tables of rotations.c:113-190, as the oracle builds them."""
+    """FP64 sine/cosine tables of rotations.c:113-190 and 270-287, as the
+    oracle builds them.  Plan("rectdisk", n, beta) takes the reference's one
+    parameter (ft_plan_rotrectdisk(n, beta))."""
 
     def __init__(self, kind, n, alpha=0.0, beta=0.0, gamma=0.0):
         lib = _load()
         self.kind, self.n = _kind(kind), int(n)
+        if self.kind == RECTDISK:
+            alpha, beta = 0.0, alpha
         self.alpha, self.beta, self.gamma = float(alpha), float(beta), float(gamma)
-        size = lib.oracle_plan_size(self.kind, self.n)
+        size = lib.oracle_plan_size(TRI if self.kind == RECTDISK else self.kind, self.n)
         self.s = np.zeros(size)
         self.c = np.zeros(size)
         if self.kind == SPH:
             lib.oracle_plan_sph(self.n, _ptr(self.s), _ptr(self.c))
         elif self.kind == TRI:
             lib.oracle_plan_tri(self.n, self.alpha, self.beta, self.gamma, _ptr(self.s), _ptr(self.c))
+        elif self.kind == RECTDISK:
+            lib.oracle_plan_rectdisk(self.n, self.beta, _ptr(self.s), _ptr(self.c))
         else:
             lib.oracle_plan_disk(self.n, self.alpha, self.beta, _ptr(self.s), _ptr(self.c))
 
@@ -105,13 +114,29 @@ class Plan:
         return l + m * (2 * n + 1 - m) // 2
 
 
+def _geometry(plan):
+    """The loop nests of the plan's kind (a rectangle-disk plan runs the sphere's)."""
+    return SPH if plan.kind == RECTDISK else plan.kind
+
+
 def execute(plan, direction, A, M=None):
     """ft_execute_<kind>_<direction> restated (in place on A)."""
     lib = _load()
     _check_matrix(A, plan.n)
     M = A.shape[1] if M is None else M
-    fn = getattr(lib, "oracle_execute_" + ("sph", "tri", "disk")[plan.kind])
+    fn = getattr(lib, "oracle_execute_" + {SPH: "sph", TRI: "tri", DISK: "disk", RECTDISK: "rectdisk"}[plan.kind])
     fn(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+def execute_sphv(plan, direction, A, M=None):
+    """ft_execute_sphv_<direction> restated (a sphere plan; in place on A)."""
+    lib = _load()
+    _check_matrix(A, plan.n)
+    if plan.kind != SPH:
+        raise ValueError("sphv runs on a sphere plan")
+    M = A.shape[1] if M is None else M
+    lib.oracle_execute_sphv(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
     return A
 
 
@@ -120,7 +145,7 @@ def kernel(plan, direction, m1, m2, A, S=1):
     lib = _load()
     if A.dtype != np.float64 or not A.flags.c_contiguous:
         raise ValueError("expected a contiguous float64 buffer")
-    name = "oracle_kernel_%s_%s" % (("sph", "tri", "disk")[plan.kind], ("hi2lo", "lo2hi")[direction])
+    name = "oracle_kernel_%s_%s" % (("sph", "tri", "disk")[_geometry(plan)], ("hi2lo", "lo2hi")[direction])
     getattr(lib, name)(_ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2), _ptr(A), int(S))
     return A
 
@@ -148,7 +173,7 @@ def kernel_f32(plan, direction, m1, m2, A, S=1):
     sf = plan.s.astype(np.float32)
     cf = plan.c.astype(np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
-    name = "oracle_kernel_%s_%s_f32" % (("sph", "tri", "disk")[plan.kind], ("hi2lo", "lo2hi")[direction])
+    name = "oracle_kernel_%s_%s_f32" % (("sph", "tri", "disk")[_geometry(plan)], ("hi2lo", "lo2hi")[direction])
     getattr(lib, name)(sf.ctypes.data_as(fp), cf.ctypes.data_as(fp), plan.n, int(m1), int(m2),
                        A.ctypes.data_as(fp), int(S))
     return A
@@ -203,6 +228,8 @@ def _load_ref():
                                     ctypes.c_int]
         lib.ref_kernel_column.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, _dp, ctypes.c_int]
+        for name in ("sphv", "rectdisk"):
+            getattr(lib, "ref_execute_" + name).argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         lib.ref_select_isa.restype = ctypes.c_int
         lib.ref_set_num_threads.argtypes = [ctypes.c_int]
         lib.ref_max_threads.restype = ctypes.c_int
@@ -230,6 +257,24 @@ def ref_execute(plan, direction, A, M=None, isa=-1, B=None):
         B = np.zeros(plan.n * max(M, 1))
     lib.ref_execute(plan.kind, int(direction), int(isa), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), _ptr(B),
                     int(M))
+    return A
+
+
+def ref_execute_sphv(plan, direction, A, M=None):
+    """The reference's scalar ft_execute_sphv_* (its compiled sphere kernels)."""
+    lib = _load_ref()
+    _check_matrix(A, plan.n)
+    M = A.shape[1] if M is None else M
+    lib.ref_execute_sphv(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+def ref_execute_rectdisk(plan, direction, A, M=None):
+    """The reference's ft_execute_rectdisk_* (its compiled rectangle-disk kernels)."""
+    lib = _load_ref()
+    _check_matrix(A, plan.n)
+    M = A.shape[1] if M is None else M
+    lib.ref_execute_rectdisk(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
     return A
 
 

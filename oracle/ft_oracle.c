@@ -9,7 +9,8 @@
  * FastTransforms source tree):
  *
  *   plans        src/rotations.c:113-133 (sphere), :143-160 (triangle),
- *                :170-190 (disk, negative sines, index sd(l,m)).
+ *                :170-190 (disk, negative sines, index sd(l,m)),
+ *                :270-287 (Dunkl-Xu rectangle-disk, triangle index).
  *   kernels      src/rotations/rotations_default.c:31-71 (loop nests), with the
  *                rotation written in the fused form the dispatched SIMD path
  *                computes: fma(c,x,s*y) / fma(c,y,-(s*x)) for hi2lo and
@@ -17,7 +18,11 @@
  *                (src/rotations/rotations_AVX512F.c:4-15, src/rotations/rotations.h:37-40,82-85).
  *                The file is compiled with -ffp-contract=off so only these fmas fuse.
  *   executors    src/drivers.c:94-110 (sphere), :314-326 (triangle), :414-430 (disk):
- *                orders 0-1 (sphere, disk) and 0 (triangle) are untouched.
+ *                orders 0-1 (sphere, disk) and 0 (triangle) are untouched;
+ *                :204-220 (vector sphere: the sphere kernels on columns 2m+1,
+ *                2m+2 of orders 2..M/2-1) and :496-508 (rectangle-disk: its
+ *                kernels, rotations_default.c:73-85, which are the sphere's
+ *                loop nests, on column m of orders 2..M-1).
  *   fills        src/ftutilities.c:67-113 (valid triangle of each layout), but
  *                seeded with a counter-based SplitMix64 instead of libc rand().
  *   FP32         the same loop nests in float (s, c rounded once from the FP64
@@ -88,6 +93,20 @@ void oracle_plan_disk(int n, double alpha, double beta, double *s, double *c) {
             double den = (((l + m) + alpha) + 2) * ((((l + m) + alpha) + beta) + 2);
             s[idx_disk(n, l, m)] = -sqrt(nums / den);
             c[idx_disk(n, l, m)] = sqrt(numc / den);
+        }
+}
+
+/* rotations.c:270-287: s = sqrt(nums/den), c = sqrt(numc/den) with the
+ * reference's operation order (nums an int product converted once; numc and
+ * den in double). */
+void oracle_plan_rectdisk(int n, double beta, double *s, double *c) {
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (l + 1) * (l + 2);
+            double numc = (2 * m + 2 * beta + 3) * (2 * l + 2 * m + 2 * beta + 6);
+            double den = (l + 2 * m + 2 * beta + 4) * (l + 2 * m + 2 * beta + 5);
+            s[idx_tri(n, l, m)] = sqrt(nums / den);
+            c[idx_tri(n, l, m)] = sqrt(numc / den);
         }
 }
 
@@ -215,6 +234,28 @@ static inline void givens_t_f32(float S, float C, float *X, float *Y) {
 
 ORACLE_LOOPS(, double, givens, givens_t)
 ORACLE_LOOPS(_f32, float, givens_f32, givens_t_f32)
+
+/* drivers.c:204-220: the sphere kernels on columns 2m+1 and 2m+2 of orders
+ * m = 2 .. M/2-1 (vector spherical harmonics; the sphere's plan). */
+void oracle_execute_sphv(int dir, const double *s, const double *c, int n, double *A, int M) {
+    _Pragma("omp parallel for schedule(dynamic, 1)") for (int m = M / 2 - 1; m >= 2; m--)
+        for (int t = 1; t <= 2; t++) {
+            double *col = A + (long)n * (2 * m + t);
+            if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
+            else oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+        }
+}
+
+/* drivers.c:496-508 with rotations_default.c:73-85 (the sphere's loop nests:
+ * rows (k, k+2), k = n-3-j .. 0, triangle index) on column m of orders
+ * m = 2 .. M-1 (the rectangle-disk plan). */
+void oracle_execute_rectdisk(int dir, const double *s, const double *c, int n, double *A, int M) {
+    _Pragma("omp parallel for schedule(dynamic, 1)") for (int m = M - 1; m >= 2; m--) {
+        double *col = A + (long)n * m;
+        if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
+        else oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+    }
+}
 
 /* FP32 executor on an FP64 plan: s and c rounded to float once, then the
  * FP32 loop nest (kind 0 sph, 1 tri, 2 disk).  0, or -1 if out of memory. */

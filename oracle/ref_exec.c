@@ -131,5 +131,27 @@ void ref_kernel_column(int kind, int dir, double *s, double *c, int n, int m1, i
     pick(kind, dir, REF_DEFAULT)(&RP, m1, m2, A, S);
 }
 
+/* ft_execute_sphv_{hi2lo,lo2hi}'s scalar path (drivers.c:204-220) and
+ * ft_execute_rectdisk_{hi2lo,lo2hi} (drivers.c:488-508, scalar only) around
+ * the reference's compiled kernels (kernel_sph_*_default,
+ * kernel_rectdisk_*_default). */
+void ref_execute_sphv(int dir, double *s, double *c, int n, double *A, int M) {
+    ft_rotation_plan RP = {s, c, n};
+    ref_kernel k = pick(0, dir, REF_DEFAULT);
+    #pragma omp parallel for schedule(static, 1)
+    for (int m = 2; m <= M / 2 - 1; m++) {
+        k(&RP, m % 2, m, A + (long)n * (2 * m + 1), 1);
+        k(&RP, m % 2, m, A + (long)n * (2 * m + 2), 1);
+    }
+}
+
+void ref_execute_rectdisk(int dir, double *s, double *c, int n, double *A, int M) {
+    ft_rotation_plan RP = {s, c, n};
+    ref_kernel k = dir == 0 ? kernel_rectdisk_hi2lo_default : kernel_rectdisk_lo2hi_default;
+    #pragma omp parallel for schedule(static, 1)
+    for (int m = 2; m < M; m++)
+        k(&RP, m % 2, m, A + (long)n * m, 1);
+}
+
 void ref_set_num_threads(int t) { omp_set_num_threads(t); }
 int ref_max_threads(void) { return omp_get_max_threads(); }

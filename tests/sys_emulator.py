@@ -24,19 +24,24 @@ class Chain(ctypes.Structure):
                 ("glo", ctypes.c_int * 8), ("ghi", ctypes.c_int * 8), ("aoff", ctypes.c_longlong)]
 
 
-def host_build(lib, plan_ptr, direction, S, M, maxpart=16):
+# column layouts of a sphere-geometry matrix (FTAMD_LAYOUT_* in ft_hip_abi.h)
+LAYOUT = {"std": 0, "sphv": 1, "rectdisk": 2}
+
+
+def host_build(lib, plan_ptr, direction, S, M, maxpart=16, layout="std"):
     """(bands, table (entries x 2: s, c), chain parts of at most maxpart bands)
-    as the library builds them."""
+    as the library builds them for the column layout (std: the plan's own;
+    sphv or rectdisk on a sphere-geometry plan)."""
     fn = lib.ftamd_sys_debug_host
     fn.restype = ctypes.c_int
-    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                    ctypes.POINTER(ctypes.POINTER(Band)), ctypes.POINTER(ctypes.c_int),
                    ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_longlong),
                    ctypes.POINTER(ctypes.POINTER(Chain)), ctypes.POINTER(ctypes.c_int)]
     pb, nb = ctypes.POINTER(Band)(), ctypes.c_int()
     pt, ne = ctypes.POINTER(ctypes.c_double)(), ctypes.c_longlong()
     pc, nc = ctypes.POINTER(Chain)(), ctypes.c_int()
-    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, M, maxpart, ctypes.byref(pb), ctypes.byref(nb), ctypes.byref(pt),
+    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, M, LAYOUT[layout], maxpart, ctypes.byref(pb), ctypes.byref(nb), ctypes.byref(pt),
             ctypes.byref(ne), ctypes.byref(pc), ctypes.byref(nc))
     if rc != 0:
         raise RuntimeError("ftamd_sys_debug_host failed: %d" % rc)

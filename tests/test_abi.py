@@ -20,12 +20,14 @@ def declared_functions():
 
 def test_header_declares_reference_entry_points():
     names = set(declared_functions())
-    for kind in ("sph", "tri", "disk"):
+    for kind in ("sph", "tri", "disk", "rectdisk"):
+        for d in ("hi2lo", "lo2hi"):
+            assert "ft_kernel_%s_%s" % (kind, d) in names
+    for kind in ("sph", "sphv", "tri", "disk", "rectdisk"):
         for d in ("hi2lo", "lo2hi"):
             assert "ft_execute_%s_%s" % (kind, d) in names
-            assert "ft_kernel_%s_%s" % (kind, d) in names
-    for fn in ("ft_plan_rotsphere", "ft_plan_rottriangle", "ft_plan_rotdisk", "ft_destroy_rotation_plan",
-               "ft_set_num_threads"):
+    for fn in ("ft_plan_rotsphere", "ft_plan_rottriangle", "ft_plan_rotdisk", "ft_plan_rotrectdisk",
+               "ft_destroy_rotation_plan", "ft_set_num_threads"):
         assert fn in names
 
 

@@ -78,7 +78,7 @@ def test_partial_M(ft, kind, params):
 
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_sph_full_sizes(ft, n):
-    """The north-star sizes: n = 1024 (1024 x 2047) and 2048 (2048 x 4095)."""
+    """The headline sizes: n = 1024 (1024 x 2047) and 2048 (2048 x 4095)."""
     for d in (0, 1):
         _check(ft, "sph", (), n, d, seed=1000 + n)
 

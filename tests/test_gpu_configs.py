@@ -4,7 +4,7 @@
   within the reference's bounds (test_drivers.c:52-57) and close to the
   reference's own dispatched executor (oracle/_ref).
 * C2: n = 1024 and 2048 (and tri/disk at 1024) against the reference's own
-  dispatched executor directly, the north-star bar of 1e-14 relative 2-norm.
+  dispatched executor directly, within 1e-14 relative 2-norm.
 * C3: the FP32 variant bit-identical to the oracle's FP32 loop nests at
   n = 512, 1024, 2048, both directions, with its error against the FP64 and
   the __float128 evaluations reported per degree.
@@ -80,7 +80,7 @@ def test_c1_sph_256_roundtrip(ft):
                                            ("tri", (-0.5, -0.5, -0.5), 1024), ("disk", (0.0, 0.0), 1024)])
 def test_c2_against_reference_dispatch(ft, kind, params, n):
     """The GPU output against the reference's own dispatched executor on the
-    same seeded input: relative 2-norm <= 1e-14 (north-star bar)."""
+    same seeded input: relative 2-norm <= 1e-14."""
     import torch
     M = full_M(kind, n)
     P = _maker(ft, kind)(n, *params)

@@ -268,7 +268,7 @@ def test_fill_stream_matches_numpy_restatement(ft):
 @pytest.mark.parametrize("kind,params,n", [("sph", (), 1024), ("sph", (), 2048), ("tri", (0.0, 0.0, 0.0), 1024),
                                            ("tri", (-0.5, -0.5, -0.5), 1024), ("disk", (0.0, 0.0), 1024)])
 def test_fused_oracle_vs_reference_dispatch_at_scale(kind, params, n):
-    """The north-star sizes: the fused oracle (which the GPU matches bit for
+    """The headline sizes: the fused oracle (which the GPU matches bit for
     bit) against the reference's dispatched executor, both directions, within
     the 1e-14 relative 2-norm bar (measured here ~1e-15)."""
     M = full_M(kind, n)

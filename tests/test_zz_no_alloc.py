@@ -18,8 +18,8 @@ def _say(*args):
     sys.stdout.flush()
 
 
-def _noalloc(ft, P, direction, T, precision, stream):
-    return ft._lib.ft_gpu_execute_noalloc(P.ptr, P.kind, int(direction), int(precision),
+def _noalloc(ft, P, direction, T, precision, stream, kind=None):
+    return ft._lib.ft_gpu_execute_noalloc(P.ptr, P.kind if kind is None else kind, int(direction), int(precision),
                                           ctypes.c_void_p(T.data_ptr()), int(T.shape[0]),
                                           ctypes.c_void_p(stream.cuda_stream))
 
@@ -75,3 +75,27 @@ def test_noalloc_rejects_what_it_cannot_run(ft):
     assert ft._lib.ft_gpu_execute_noalloc(P.ptr, ft.SPH, 0, 0, ctypes.c_void_p(T.data_ptr()), 180,
                                           ctypes.c_void_p(stream.cuda_stream)) == ft.FT_GPU_EINVAL
     ft.synchronize(stream)
+
+
+def test_noalloc_vector_and_rectangle_layouts(ft):
+    """The sphv and rectdisk executor kinds through the allocation-free entry
+    point: the same bits as ft_gpu_execute, two calls in a row, the second
+    smaller."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for kind, n in ((ft.SPHV, 300), (ft.SPHV, 140), (ft.RECTDISK, 400), (ft.RECTDISK, 90)):
+        P = ft.plan_rotsphere(n) if kind == ft.SPHV else ft.plan_rotrectdisk(n, 0.5)
+        M = 2 * n - 1 if kind == ft.SPHV else n
+        A = np.random.default_rng(n).uniform(-1.0, 1.0, (M, n))
+        X = torch.from_numpy(A).cuda()
+        Y = X.clone()
+        ft.set_engine(ft.ENGINE_SYSTOLIC)
+        try:
+            ft.gpu_execute(P, 1, X, kind=kind, stream=stream)
+            ft.synchronize(stream)
+        finally:
+            ft.set_engine(ft.ENGINE_AUTO)
+        _say("noalloc kind=%d n=%d" % (kind, n))
+        assert _noalloc(ft, P, 1, Y, ft.FP64, stream, kind=kind) == 0
+        ft.synchronize(stream)
+        assert torch.equal(X, Y), (kind, n)
