@@ -113,6 +113,8 @@ class RotationPlan:
             _check()
             raise GPUError("plan construction failed")
         self._ptr = ptr
+        # kept on the object: module globals may be gone when it is collected at exit
+        self._destroy = _lib.ft_destroy_rotation_plan
         self.kind = kind
         self.n = n
         self.params = params
@@ -142,7 +144,7 @@ class RotationPlan:
     def __del__(self):
         ptr, self._ptr = getattr(self, "_ptr", None), None
         if ptr:
-            _lib.ft_destroy_rotation_plan(ptr)
+            self._destroy(ptr)
 
 
 def plan_rotsphere(n):

@@ -62,7 +62,10 @@ constexpr int kLanes = FTAMD_SYS_LANES;
 constexpr int kRows = 4;
 constexpr int kCols = 2;
 constexpr int kWaves = FTAMD_SYS_WAVES;
-constexpr int kZ = 32;                 // ring positions between two bands
+// ring positions between two bands (a multiple of kU, at least 2*kChunk):
+// 64 lets a band run up to 64 rows ahead of the next (64 KB of LDS per
+// workgroup; 3-9 % faster than 32 at n = 1024, 80 no better)
+constexpr int kZ = 64;
 constexpr int kU = FTAMD_SYS_UNROLL;   // steps per loop iteration
 constexpr int kDin = 2;                // input positions read ahead
 constexpr int kChunk = 16;             // matrix rows fetched per lane-parallel load
@@ -102,6 +105,7 @@ struct Engine {
     static constexpr int kSPD = 1024 / kStepBytes;
     // steps of table entries staged ahead in LDS (4 DMA blocks)
     static constexpr int kD = 4 * kSPD;
+    static_assert(kZ % kU == 0 && kZ >= 2 * kChunk, "ring of whole blocks, two fetch chunks deep");
     static_assert(kSPD >= 1 && kU % kSPD == 0 && kD <= FTAMD_SYS_PREFETCH,
                   "DMA blocks must tile the unroll and the tables' spare steps");
     // a block's table slots are at most one wrap of the LDS ring, and its
@@ -439,7 +443,8 @@ struct Engine {
         for (int t0 = 0; t0 < steps; t0 += kU) {
             if (t0 % kChunk == 0) {
                 // positions t0+16 .. t0+31 to the ring, into the slots of
-                // positions t0-16 .. t0-1 (all read); fetch the next 16
+                // positions t0+16-kZ .. t0+31-kZ (all read: kZ >= 32);
+                // fetch the next 16
                 const int p0 = t0 + kChunk;
                 if (p0 < npa) {
                     put(p0, chunk);
