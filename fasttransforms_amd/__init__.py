@@ -56,6 +56,7 @@ SIGNATURES = {
     "ft_plan_rottriangle": (_vp, [_i, _d, _d, _d]),
     "ft_plan_rotdisk": (_vp, [_i, _d, _d]),
     "ft_plan_rotrectdisk": (_vp, [_i, _d]),
+    "ft_plan_rotannulus": (_vp, [_i, _d, _d, _d, _d]),
     "ft_gpu_last_error": (_i, []),
     "ft_gpu_error_string": (ctypes.c_char_p, [_i]),
     "ft_gpu_execute": (_i, [_vp, _i, _i, _i, _vp, _i, _vp]),
@@ -116,6 +117,7 @@ class RotationPlan:
         # kept on the object: module globals may be gone when it is collected at exit
         self._destroy = _lib.ft_destroy_rotation_plan
         self.kind = kind
+        self.family = {SPH: "sph", TRI: "tri", DISK: "disk", RECTDISK: "rectdisk"}.get(kind)
         self.n = n
         self.params = params
 
@@ -158,6 +160,14 @@ def plan_rottriangle(n, alpha, beta, gamma):
 
 def plan_rotrectdisk(n, beta):
     return RotationPlan(_lib.ft_plan_rotrectdisk(int(n), float(beta)), RECTDISK, int(n), (float(beta),))
+
+
+def plan_rotannulus(n, alpha, beta, gamma, rho):
+    """ft_plan_rotannulus: the annulus of inner radius rho (gamma = 0); a disk-layout plan."""
+    P = RotationPlan(_lib.ft_plan_rotannulus(int(n), float(alpha), float(beta), float(gamma), float(rho)), DISK,
+                     int(n), (float(alpha), float(beta), float(gamma), float(rho)))
+    P.family = "annulus"
+    return P
 
 
 def plan_rotdisk(n, alpha, beta):

@@ -1325,7 +1325,7 @@ static int batch_check(const ft_rotation_plan *const *plans, int kind, int direc
         const ft_rotation_plan *P = plans[i];
         if (!valid_call(P, kind, M[i], geo, layout) || !dA[i]) return FT_GPU_EINVAL;
         if (P->family != plans[0]->family || P->alpha != plans[0]->alpha || P->beta != plans[0]->beta ||
-            P->gamma != plans[0]->gamma)
+            P->gamma != plans[0]->gamma || P->rho != plans[0]->rho)
             return FT_GPU_EINVAL;
         if (!*big || P->n > (*big)->n) *big = (ft_rotation_plan *)P;
     }

@@ -43,8 +43,9 @@ struct ft_rotation_plan_s {
     int n;
     /* extension */
     int kind;                /* geometry of the tables: FTAMD_SPH, _TRI or _DISK */
-    int family;              /* its constructor: FT_GPU_SPH, _TRI, _DISK or _RECTDISK (batches need one) */
-    double alpha, beta, gamma;
+    int family;              /* its constructor: FT_GPU_SPH, _TRI, _DISK, _RECTDISK or FTAMD_FAMILY_ANNULUS
+                                (a batch takes plans of one family and one parameter set) */
+    double alpha, beta, gamma, rho;
     long nsc;                /* entries per table */
     int device;              /* HIP device of the uploads, -1 before upload */
     void *d_tab[2];          /* interleaved (s,c) with padding: [0] FP64, [1] FP32 */
@@ -58,6 +59,9 @@ struct ft_rotation_plan_s {
 };
 
 typedef struct ft_sys_tables ft_sys_tables;
+
+/* Family of ft_plan_rotannulus plans (disk tables and kernels). */
+#define FTAMD_FAMILY_ANNULUS 5
 
 /* ft_plans.c */
 ft_rotation_plan *ftamd_plan_alloc(int kind, int family, int n, double alpha, double beta, double gamma);

@@ -118,6 +118,142 @@ ft_rotation_plan *ft_plan_rotrectdisk(const int n, const double beta) {
     return RP;
 }
 
+/* ------------------------------------------------------------ annulus -- */
+
+/* The normalized Jacobi operator x P^{(a,b)} on N coefficients as the
+ * reference's banded constructor evaluates it (banded_source.c:1797-1818,
+ * norm 1): diagonal d[j], superdiagonal up[j] = (j, j+1), subdiagonal
+ * lo[j] = (j+1, j).  The two off-diagonals are equal in exact arithmetic but
+ * evaluated by different expressions; the QR steps read the first, the
+ * Cholesky factor both. */
+static void jacobi_operator(int N, double a, double b, double *d, double *up, double *lo) {
+    for (int j = 0; j < N; j++) {
+        double v;
+        if (j == 1)
+            v = 2 * sqrt(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)));
+        else
+            v = 2 * sqrt((j * (j + a) * (j + b) * (j + a + b)) /
+                         ((2 * j + a + b - 1) * (2 * j + a + b) * (2 * j + a + b) * (2 * j + a + b + 1)));
+        if (j > 0) up[j - 1] = v;
+        if (j == 0) {
+            d[0] = (b - a) / (a + b + 2);
+            if (N > 1) lo[0] = 2 * sqrt(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)));
+        } else {
+            d[j] = ((b - a) * (a + b)) / ((2 * j + a + b) * (2 * j + a + b + 2));
+            if (j + 1 < N)
+                lo[j] = 2 * sqrt(((j + 1) * (j + a + 1) * (j + b + 1) * (j + a + b + 1)) /
+                                 ((2 * j + a + b + 1) * (2 * j + a + b + 2) * (2 * j + a + b + 2) * (2 * j + a + b + 3)));
+        }
+    }
+}
+
+/* Givens rotation zeroing y against x (banded_source.c:1415-1425). */
+static void givens(double x, double y, double *c, double *s, double *r) {
+    *r = hypot(x, y);
+    if (*r <= 0x1p-970) { /* floatmin / eps */
+        *c = 1;
+        *s = 0;
+    } else {
+        *c = x / *r;
+        *s = y / *r;
+    }
+}
+
+/* QR factorization by Givens rotations of the symmetric tridiagonal (a, b)
+ * of size N >= 2 (banded_source.c:1429-1456): the N-1 rotations (c, s) and
+ * the diagonal and first superdiagonal of R. */
+static void tridiag_qr(int N, const double *a, const double *b, double *c, double *s, double *rd, double *ru) {
+    double r, aip1 = a[0], bip1r = b[0];
+    for (int i = 0; i < N - 2; i++) {
+        givens(aip1, -b[i], c + i, s + i, &r);
+        rd[i] = r;
+        ru[i] = c[i] * bip1r - s[i] * a[i + 1];
+        aip1 = c[i] * a[i + 1] + s[i] * bip1r;
+        bip1r = c[i] * b[i + 1];
+    }
+    givens(aip1, -b[N - 2], c + N - 2, s + N - 2, &r);
+    rd[N - 2] = r;
+    ru[N - 2] = c[N - 2] * bip1r - s[N - 2] * a[N - 1];
+    rd[N - 1] = c[N - 2] * a[N - 1] + s[N - 2] * bip1r;
+}
+
+/* The leading N-1 x N-1 block of R X R^{-1} for the symmetric tridiagonal X
+ * = (a, b) of size N and the upper triangular R (diagonal rd, superdiagonal
+ * ru), in place: a gets N-1 entries, b N-2 (banded_source.c:1609-1617,
+ * 1641-1649). */
+static void jacobi_similarity(int N, double *a, double *b, const double *rd, const double *ru) {
+    double bprev = 0;
+    for (int i = 0; i < N - 1; i++) {
+        const double bq = i < N - 2 ? rd[i + 1] / rd[i] * b[i] : 0;
+        if (i == 0) a[0] = a[0] + ru[0] / rd[0] * b[0];
+        else a[i] = (rd[i] * a[i] + ru[i] * b[i] - ru[i - 1] * bprev) / rd[i];
+        if (i < N - 2) b[i] = bq;
+        bprev = bq;
+    }
+}
+
+/* rotations.c:194-260 (gamma = 0): the rotations of the annulus with inner
+ * radius rho, in the disk's layout and kernels.  With t = 1 + 2 rho^2 / (1 -
+ * rho^2) and J the Jacobi operator of P^{(beta, alpha)} on n+1 coefficients,
+ * the even orders m = 0, 2, ... take the Givens rotations of the QR
+ * factorization of J + tI, then of R (J + tI) R^{-1} (one size smaller) and
+ * so on; the odd orders start from U (J + tI) U^{-1} with U the Cholesky
+ * factor of J + tI (banded_source.c:484-499).  Host FP64 in the reference's
+ * operation order, with no contraction. */
+ft_rotation_plan *ft_plan_rotannulus(const int n, const double alpha, const double beta, const double gamma,
+                                     const double rho) {
+    if (n < 1 || gamma != 0.0 || !(rho >= 0.0 && rho < 1.0)) {
+        ftamd_set_error(FT_GPU_EINVAL); /* the reference implements gamma = 0 only (rotations.c:252-254) */
+        return NULL;
+    }
+    ft_rotation_plan *RP = ftamd_plan_alloc(FTAMD_DISK, FTAMD_FAMILY_ANNULUS, n, alpha, beta, 0.0);
+    if (!RP) return NULL;
+    RP->rho = rho;
+    const int N = n + 1;
+    double *w = malloc(sizeof(double) * (size_t)(9 * N));
+    if (!w) {
+        ft_destroy_rotation_plan(RP);
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    double *d = w, *up = w + N, *lo = w + 2 * N, *a = w + 3 * N, *b = w + 4 * N, *c = w + 5 * N, *s = w + 6 * N,
+           *rd = w + 7 * N, *ru = w + 8 * N;
+    const double t = 1.0 + 2.0 * rho * rho / (1.0 - rho * rho);
+    jacobi_operator(N, beta, alpha, d, up, lo);
+    for (int par = 0; par < 2; par++) {
+        int size = N;
+        for (int i = 0; i < N; i++) a[i] = d[i] + t;
+        for (int i = 0; i < N - 1; i++) b[i] = up[i];
+        if (par == 1) {
+            /* U: Cholesky factor of J + tI, rows j: U(j,j) = sqrt(p_j),
+             * U(j,j+1) = up[j] / sqrt(p_j), p_{j+1} = (t + d[j+1]) - lo[j]/p_j * up[j] */
+            double p = t + d[0];
+            for (int j = 0; j < N; j++) {
+                const double mult = j + 1 < N ? lo[j] / p : 0;
+                rd[j] = sqrt(p);
+                if (j + 1 < N) {
+                    const double pn = (t + d[j + 1]) - mult * up[j];
+                    ru[j] = up[j] / rd[j];
+                    p = pn;
+                }
+            }
+            jacobi_similarity(size, a, b, rd, ru);
+            size--;
+        }
+        for (int m = par; m < 2 * n - 1; m += 2) {
+            tridiag_qr(size, a, b, c, s, rd, ru);
+            for (int l = 0; l < n - (m + 1) / 2; l++) {
+                RP->s[idx_disk(n, l, m)] = s[l];
+                RP->c[idx_disk(n, l, m)] = c[l];
+            }
+            jacobi_similarity(size, a, b, rd, ru);
+            size--;
+        }
+    }
+    free(w);
+    return RP;
+}
+
 /* rotations.c:6-10, plus the device state. */
 void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
     if (!RP) return;

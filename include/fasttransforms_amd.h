@@ -53,6 +53,15 @@ ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double 
 void ft_kernel_disk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 void ft_kernel_disk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 
+/* fasttransforms.h:318 (rotations.c:194-260) -- annulus with inner radius
+ * rho (0 <= rho < 1): the Givens rotations of successive QR factorizations
+ * of the shifted Jacobi operator of P^{(beta, alpha)}, built on the host in
+ * FP64, in the disk's layout; run with the disk kernels and executors.  The
+ * reference implements gamma = 0 only (and warns otherwise); here any other
+ * gamma, or rho outside [0, 1), returns NULL with FT_GPU_EINVAL. */
+ft_rotation_plan *ft_plan_rotannulus(const int n, const double alpha, const double beta, const double gamma,
+                                     const double rho);
+
 /* fasttransforms.h:325 (rotations.c:270-287) -- Dunkl-Xu (rectangle and
  * disk) with parameter beta; tables bit-identical to the reference's, in the
  * sphere's layout (n(n+1)/2 entries, s(l, m) at l + m(2n+1-m)/2). */

@@ -19,9 +19,9 @@ import os
 
 import numpy as np
 
-SPH, TRI, DISK, RECTDISK = 0, 1, 2, 4
+SPH, TRI, DISK, RECTDISK, ANNULUS = 0, 1, 2, 4, 5
 HI2LO, LO2HI = 0, 1
-KINDS = {"sph": SPH, "tri": TRI, "disk": DISK, "rectdisk": RECTDISK}
+KINDS = {"sph": SPH, "tri": TRI, "disk": DISK, "rectdisk": RECTDISK, "annulus": ANNULUS}
 
 _here = os.path.dirname(os.path.abspath(__file__))
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -42,6 +42,8 @@ def _load():
         lib.oracle_plan_tri.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.oracle_plan_disk.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.oracle_plan_rectdisk.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        lib.oracle_plan_annulus.restype = ctypes.c_int
+        lib.oracle_plan_annulus.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         for name in ("sphv", "rectdisk"):
             getattr(lib, "oracle_execute_" + name).argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         for name in ("sph", "tri", "disk"):
@@ -85,17 +87,19 @@ def _check_matrix(A, n):
 
 
 class Plan:
-    """FP64 sine/cosine tables of rotations.c:113-190 and 270-287, as the
-    oracle builds them.  Plan("rectdisk", n, beta) takes the reference's one
-    parameter (ft_plan_rotrectdisk(n, beta))."""
+    """FP64 sine/cosine tables of rotations.c:113-190, 194-260 and 270-287, as
+    the oracle builds them.  Plan("rectdisk", n, beta) takes the reference's
+    one parameter (ft_plan_rotrectdisk(n, beta)); Plan("annulus", n, alpha,
+    beta, rho=r) the annulus of inner radius r with gamma = 0
+    (ft_plan_rotannulus(n, alpha, beta, 0, r))."""
 
-    def __init__(self, kind, n, alpha=0.0, beta=0.0, gamma=0.0):
+    def __init__(self, kind, n, alpha=0.0, beta=0.0, gamma=0.0, rho=0.0):
         lib = _load()
         self.kind, self.n = _kind(kind), int(n)
         if self.kind == RECTDISK:
             alpha, beta = 0.0, alpha
-        self.alpha, self.beta, self.gamma = float(alpha), float(beta), float(gamma)
-        size = lib.oracle_plan_size(TRI if self.kind == RECTDISK else self.kind, self.n)
+        self.alpha, self.beta, self.gamma, self.rho = float(alpha), float(beta), float(gamma), float(rho)
+        size = lib.oracle_plan_size(_geometry(self) if self.kind != RECTDISK else TRI, self.n)
         self.s = np.zeros(size)
         self.c = np.zeros(size)
         if self.kind == SPH:
@@ -104,19 +108,25 @@ class Plan:
             lib.oracle_plan_tri(self.n, self.alpha, self.beta, self.gamma, _ptr(self.s), _ptr(self.c))
         elif self.kind == RECTDISK:
             lib.oracle_plan_rectdisk(self.n, self.beta, _ptr(self.s), _ptr(self.c))
+        elif self.kind == ANNULUS:
+            if self.gamma != 0.0:
+                raise ValueError("the reference's annulus plan has gamma = 0 only")
+            if lib.oracle_plan_annulus(self.n, self.alpha, self.beta, self.rho, _ptr(self.s), _ptr(self.c)) != 0:
+                raise MemoryError("oracle_plan_annulus failed")
         else:
             lib.oracle_plan_disk(self.n, self.alpha, self.beta, _ptr(self.s), _ptr(self.c))
 
     def index(self, l, m):
         n = self.n
-        if self.kind == DISK:
+        if _geometry(self) == DISK:
             return l + m * n - (m // 2) * ((m + 1) // 2)
         return l + m * (2 * n + 1 - m) // 2
 
 
 def _geometry(plan):
-    """The loop nests of the plan's kind (a rectangle-disk plan runs the sphere's)."""
-    return SPH if plan.kind == RECTDISK else plan.kind
+    """The loop nests of the plan's kind (a rectangle-disk plan runs the
+    sphere's, an annulus plan the disk's)."""
+    return {RECTDISK: SPH, ANNULUS: DISK}.get(plan.kind, plan.kind)
 
 
 def execute(plan, direction, A, M=None):
@@ -124,7 +134,8 @@ def execute(plan, direction, A, M=None):
     lib = _load()
     _check_matrix(A, plan.n)
     M = A.shape[1] if M is None else M
-    fn = getattr(lib, "oracle_execute_" + {SPH: "sph", TRI: "tri", DISK: "disk", RECTDISK: "rectdisk"}[plan.kind])
+    fn = getattr(lib, "oracle_execute_" + {SPH: "sph", TRI: "tri", DISK: "disk", RECTDISK: "rectdisk",
+                                           ANNULUS: "disk"}[plan.kind])
     fn(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
     return A
 
@@ -157,7 +168,7 @@ def execute_f32(plan, direction, A, M=None):
     if A.dtype != np.float32 or not A.flags.f_contiguous or A.ndim != 2 or A.shape[0] != plan.n:
         raise ValueError("expected a Fortran-ordered float32 array with n rows")
     M = A.shape[1] if M is None else M
-    rc = lib.oracle_execute_f32(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n,
+    rc = lib.oracle_execute_f32(_geometry(plan), int(direction), _ptr(plan.s), _ptr(plan.c), plan.n,
                                 A.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(M))
     if rc != 0:
         raise MemoryError("oracle_execute_f32 failed")
@@ -183,6 +194,8 @@ def execute_q(plan, direction, A, hp_plan=False, M=None):
     """The same executor in __float128 (rounded to FP64 at the end)."""
     lib = _load()
     _check_matrix(A, plan.n)
+    if plan.kind not in (SPH, TRI, DISK):
+        raise ValueError("the quadruple-precision executor covers sphere, triangle and disk plans")
     M = A.shape[1] if M is None else M
     rc = lib.oracle_execute_q(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, plan.alpha,
                               plan.beta, plan.gamma, 1 if hp_plan else 0, _ptr(A), int(M))
@@ -255,7 +268,7 @@ def ref_execute(plan, direction, A, M=None, isa=-1, B=None):
     M = A.shape[1] if M is None else M
     if B is None:
         B = np.zeros(plan.n * max(M, 1))
-    lib.ref_execute(plan.kind, int(direction), int(isa), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), _ptr(B),
+    lib.ref_execute(_geometry(plan), int(direction), int(isa), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), _ptr(B),
                     int(M))
     return A
 
@@ -280,7 +293,7 @@ def ref_execute_rectdisk(plan, direction, A, M=None):
 
 def ref_kernel(plan, direction, m1, m2, A, S=1):
     lib = _load_ref()
-    lib.ref_kernel_column(plan.kind, int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2),
+    lib.ref_kernel_column(_geometry(plan), int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, int(m1), int(m2),
                           _ptr(A), int(S))
     return A
 

@@ -10,7 +10,9 @@
  *
  *   plans        src/rotations.c:113-133 (sphere), :143-160 (triangle),
  *                :170-190 (disk, negative sines, index sd(l,m)),
- *                :270-287 (Dunkl-Xu rectangle-disk, triangle index).
+ *                :270-287 (Dunkl-Xu rectangle-disk, triangle index),
+ *                :194-260 (annulus, gamma = 0: Givens rotations of QR
+ *                factorizations of shifted Jacobi operators, disk index).
  *   kernels      src/rotations/rotations_default.c:31-71 (loop nests), with the
  *                rotation written in the fused form the dispatched SIMD path
  *                computes: fma(c,x,s*y) / fma(c,y,-(s*x)) for hi2lo and
@@ -94,6 +96,107 @@ void oracle_plan_disk(int n, double alpha, double beta, double *s, double *c) {
             s[idx_disk(n, l, m)] = -sqrt(nums / den);
             c[idx_disk(n, l, m)] = sqrt(numc / den);
         }
+}
+
+/* rotations.c:194-260 with gamma = 0 (the annulus, inner radius rho): the
+ * Givens rotations (c = x/r, s = y/r, r = hypot(x, y), banded_source.c:1415-1425)
+ * of the QR factorization of the symmetric tridiagonal K (banded_source.c:
+ * 1429-1456), order by order: even orders from K = J + tI, odd orders from
+ * K = U (J + tI) U^{-1} with U^T U = J + tI (banded Cholesky,
+ * banded_source.c:484-499); after each order K <- the leading block of
+ * R K R^{-1} (banded_source.c:1609-1617).  J is the normalized Jacobi
+ * operator of P^{(beta, alpha)} on n+1 coefficients (banded_source.c:
+ * 1797-1818: its superdiagonal feeds K, its subdiagonal the Cholesky
+ * multipliers), t = 1 + 2 rho^2 / (1 - rho^2).  Entries in the disk's index.
+ * Returns -1 if out of memory. */
+static void or_jacobi(int N, double al, double be, double *diag, double *sup, double *sub) {
+    for (int j = 0; j < N; j++) {
+        if (j == 0) {
+            diag[0] = (be - al) / (al + be + 2);
+        } else {
+            diag[j] = ((be - al) * (al + be)) / ((2 * j + al + be) * (2 * j + al + be + 2));
+            sup[j - 1] = j == 1 ? 2 * sqrt(((al + 1) * (be + 1)) / ((al + be + 2) * (al + be + 2) * (al + be + 3)))
+                                : 2 * sqrt((j * (j + al) * (j + be) * (j + al + be)) /
+                                           ((2 * j + al + be - 1) * (2 * j + al + be) * (2 * j + al + be) *
+                                            (2 * j + al + be + 1)));
+        }
+        if (j + 1 < N)
+            sub[j] = j == 0 ? 2 * sqrt(((al + 1) * (be + 1)) / ((al + be + 2) * (al + be + 2) * (al + be + 3)))
+                            : 2 * sqrt(((j + 1) * (j + al + 1) * (j + be + 1) * (j + al + be + 1)) /
+                                       ((2 * j + al + be + 1) * (2 * j + al + be + 2) * (2 * j + al + be + 2) *
+                                        (2 * j + al + be + 3)));
+    }
+}
+
+/* K (diagonal ka[0..N-1], off-diagonal kb[0..N-2]) -> leading N-1 block of
+ * R K R^{-1}, R with diagonal r0 and superdiagonal r1, into (qa, qb). */
+static void or_similarity(int N, const double *ka, const double *kb, const double *r0, const double *r1, double *qa,
+                          double *qb) {
+    for (int i = 0; i + 2 < N; i++) qb[i] = r0[i + 1] / r0[i] * kb[i];
+    if (N >= 2) qa[0] = ka[0] + r1[0] / r0[0] * kb[0];
+    for (int i = 1; i + 1 < N; i++) qa[i] = (r0[i] * ka[i] + r1[i] * kb[i] - r1[i - 1] * qb[i - 1]) / r0[i];
+}
+
+int oracle_plan_annulus(int n, double alpha, double beta, double rho, double *s, double *c) {
+    const int N = n + 1;
+    double *buf = calloc((size_t)(12 * N), sizeof(double));
+    if (!buf) return -1;
+    double *diag = buf, *sup = buf + N, *sub = buf + 2 * N, *ka = buf + 3 * N, *kb = buf + 4 * N, *qa = buf + 5 * N,
+           *qb = buf + 6 * N, *r0 = buf + 7 * N, *r1 = buf + 8 * N, *gc = buf + 9 * N, *gs = buf + 10 * N;
+    const double t = 1.0 + 2.0 * rho * rho / (1.0 - rho * rho);
+    or_jacobi(N, beta, alpha, diag, sup, sub);
+    for (int odd = 0; odd <= 1; odd++) {
+        int K = N;
+        for (int i = 0; i < N; i++) ka[i] = diag[i] + t;
+        for (int i = 0; i + 1 < N; i++) kb[i] = sup[i];
+        if (odd) {
+            double piv = t + diag[0];
+            for (int j = 0; j < N; j++) {
+                r0[j] = sqrt(piv);
+                if (j + 1 < N) {
+                    double mult = sub[j] / piv;
+                    piv = (t + diag[j + 1]) - mult * sup[j];
+                    r1[j] = sup[j] / r0[j];
+                }
+            }
+            or_similarity(K, ka, kb, r0, r1, qa, qb);
+            K--;
+            memcpy(ka, qa, sizeof(double) * (size_t)K);
+            memcpy(kb, qb, sizeof(double) * (size_t)(K > 1 ? K - 1 : 0));
+        }
+        for (int m = odd; m < 2 * n - 1; m += 2) {
+            /* Givens QR of K, top row first */
+            double x = ka[0], z = kb[0];
+            for (int i = 0; i + 1 < K; i++) {
+                double y = -kb[i], r = hypot(x, y);
+                if (r <= 0x1p-970) {
+                    gc[i] = 1;
+                    gs[i] = 0;
+                } else {
+                    gc[i] = x / r;
+                    gs[i] = y / r;
+                }
+                r0[i] = r;
+                r1[i] = gc[i] * z - gs[i] * ka[i + 1];
+                if (i + 2 < K) {
+                    x = gc[i] * ka[i + 1] + gs[i] * z;
+                    z = gc[i] * kb[i + 1];
+                } else {
+                    r0[i + 1] = gc[i] * ka[i + 1] + gs[i] * z;
+                }
+            }
+            for (int l = 0; l < n - (m + 1) / 2; l++) {
+                s[idx_disk(n, l, m)] = gs[l];
+                c[idx_disk(n, l, m)] = gc[l];
+            }
+            or_similarity(K, ka, kb, r0, r1, qa, qb);
+            K--;
+            memcpy(ka, qa, sizeof(double) * (size_t)K);
+            memcpy(kb, qb, sizeof(double) * (size_t)(K > 1 ? K - 1 : 0));
+        }
+    }
+    free(buf);
+    return 0;
 }
 
 /* rotations.c:270-287: s = sqrt(nums/den), c = sqrt(numc/den) with the

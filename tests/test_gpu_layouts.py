@@ -1,5 +1,5 @@
-"""GPU parity of the vector-field (sphv) and rectangle-disk (rectdisk)
-executors against the pinned oracle.
+"""GPU parity of the vector-field (sphv), rectangle-disk (rectdisk) and
+annulus executors against the pinned oracle.
 
 Both run the sphere's loop nests on other columns (drivers.c:204-220,
 496-508): sphv pairs the columns 2m+1, 2m+2 of order m on a sphere plan,
@@ -7,7 +7,8 @@ rectdisk applies order m to column m on a rectangle-disk plan.  The GPU
 evaluates the same fused rotations in the same dependency order as the
 fused oracle, so results are asserted bit for bit (which implies the
 relative 2-norm bar of 1e-14), on both engines, both directions, full and
-partial widths, single and batched launches, FP64 and FP32.
+partial widths, single and batched launches, FP64 and FP32.  Annulus plans
+(QR-built on the host) run the disk executors (drivers.c:1073-1121).
 """
 import sys
 
@@ -202,3 +203,45 @@ def test_roundtrip_rectdisk_1024(ft):
     X = T.cpu().numpy().T
     assert O.rel2(X, A) < 4 * np.sqrt(n) * 2.0 ** -52
     assert O.relinf(X, A) < 2 * n * 2.0 ** -52
+
+
+@pytest.mark.parametrize("engine", ["auto", "pipeline", "systolic"])
+def test_annulus_through_disk_kernels(ft, engine):
+    """An annulus plan (ft_plan_rotannulus, QR-built on the host) runs the disk
+    executors (drivers.c:1073-1121): bit for bit against the oracle's disk
+    loop nests on the same tables, both directions, full and partial widths."""
+    eng = {"auto": ft.ENGINE_AUTO, "pipeline": ft.ENGINE_PIPELINE, "systolic": ft.ENGINE_SYSTOLIC}[engine]
+    try:
+        ft.set_engine(eng)
+        for n, M, rho in ((5, None, 0.3), (64, None, np.sqrt(1 / 3)), (300, None, 0.6), (300, 457, 0.2),
+                          (1024, None, 0.5)):
+            P = ft.plan_rotannulus(n, 0.0, 0.0, 0.0, rho)
+            Q = O.Plan("annulus", n, 0.0, 0.0, rho=rho)
+            M = 4 * n - 3 if M is None else M
+            A = O.fill("disk", n, 4 * n - 3, n)[:, :M].copy(order="F")
+            for d in (0, 1):
+                _say("annulus", engine, n, M, d)
+                G = (ft.execute_disk_hi2lo if d == 0 else ft.execute_disk_lo2hi)(P, A.copy(order="F"), M)
+                assert np.array_equal(G, O.execute(Q, d, A.copy(order="F"), M)), (n, M, d)
+    finally:
+        ft.set_engine(ft.ENGINE_AUTO)
+
+
+def test_annulus_batch(ft):
+    """A batch of annulus plans of uneven degree (one inner radius) in one
+    launch, each bit for bit; plans of another radius do not batch with them."""
+    import torch
+    ns = [7, 40, 129, 300]
+    plans = [ft.plan_rotannulus(n, 0.0, 0.0, 0.0, 0.5) for n in ns]
+    As = [O.fill("disk", n, 4 * n - 3, 70 + n) for n in ns]
+    for d in (0, 1):
+        Ts = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in As]
+        ft.gpu_execute_batch(plans, d, Ts)
+        ft.synchronize()
+        for n, A, T in zip(ns, As, Ts):
+            R = O.execute(O.Plan("annulus", n, 0.0, 0.0, rho=0.5), d, A.copy(order="F"))
+            assert np.array_equal(T.cpu().numpy().T, R), (n, d)
+    other = plans[:2] + [ft.plan_rotannulus(40, 0.0, 0.0, 0.0, 0.25)]
+    Ts = [torch.zeros(4 * P.n - 3, P.n, dtype=torch.float64, device="cuda") for P in other]
+    with pytest.raises(ft.GPUError):
+        ft.gpu_execute_batch(other, 0, Ts)
