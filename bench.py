@@ -16,7 +16,8 @@ non-zero too.
 Besides the headline the JSON line carries, under "config": lo2hi at n = 1024;
 the n = 256 round trip; hi2lo at n = 2048; FP32 at n = 512, 1024, 2048 with
 its error against the FP64 result; tri (0,0,0), tri (-1/2,-1/2,-1/2) and disk
-(0,0) at n = 1024; the batch of 512 sph transforms of log-uniform degree in
+(0,0) at n = 1024; the vector-field (sphv), rectangle-disk (beta 0) and
+annulus (rho = sqrt(1/3)) executors at n = 1024; the batch of 512 sph transforms of log-uniform degree in
 [32, 1024] (one launch, a captured HIP graph, and a host loop over the
 single-transform entry point); and batches of 64 tri/disk transforms with n
 uniform on [128, 1024].  It also carries the roofline of the rotation kernel
@@ -54,7 +55,8 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the secondary measurements")
     p.add_argument("--engine", choices=("auto", "pipeline", "systolic"), default="auto",
                    help="engine of the headline launches (auto: the library's own choice)")
-    p.add_argument("--case", help="time one case only, KIND:N:DIR[:fp32][:batch] (for profiling runs), "
+    p.add_argument("--case", help="time one case only, KIND:N:DIR[:fp32][:batch] (for profiling runs; KIND one of "
+                                  "sph, sphv, tri, disk, rectdisk, annulus), "
                                   "e.g. tri:1024:hi2lo or sph:1024:hi2lo:batch (the 512-transform batch)")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -82,7 +84,15 @@ def events(ft, torch, stream, fn, steps, warmup):
 
 
 def _plan(ft, kind, n, params=()):
-    return {"sph": ft.plan_rotsphere, "tri": ft.plan_rottriangle, "disk": ft.plan_rotdisk}[kind](n, *params)
+    return {"sph": ft.plan_rotsphere, "sphv": ft.plan_rotsphere, "tri": ft.plan_rottriangle, "disk": ft.plan_rotdisk,
+            "rectdisk": ft.plan_rotrectdisk, "annulus": ft.plan_rotannulus}[kind](n, *params)
+
+
+# parameters of the secondary kinds' plans: rectangle-disk beta = 0; annulus
+# alpha = beta = gamma = 0 with the reference test's inner radius sqrt(1/3)
+# (test_drivers.c:978)
+PARAMS = {"sph": (), "sphv": (), "tri": (0.0, 0.0, 0.0), "disk": (0.0, 0.0), "rectdisk": (0.0,),
+          "annulus": (0.0, 0.0, 0.0, float(np.sqrt(1.0 / 3.0)))}
 
 
 def single(ft, torch, kind, n, direction, precision, steps, warmup, params=(), seed=2024, keep=False):
@@ -92,8 +102,9 @@ def single(ft, torch, kind, n, direction, precision, steps, warmup, params=(), s
     dtype = torch.float64 if precision == ft.FP64 else torch.float32
     T = torch.from_numpy(np.ascontiguousarray(fill(kind, n, M, seed).T)).to(dtype).cuda()  # (M, n): row j = column j
     stream = torch.cuda.current_stream()
-    ms = events(ft, torch, stream, lambda: ft.gpu_execute(P, direction, T, precision=precision, stream=stream),
-                steps, warmup)
+    ex = "sphv" if kind == "sphv" else None  # the other kinds run their plan's own executor
+    ms = events(ft, torch, stream,
+                lambda: ft.gpu_execute(P, direction, T, precision=precision, stream=stream, kind=ex), steps, warmup)
     out = (ms, 6.0 * rotations(kind, n, M))
     return out + ((P, T),) if keep else out
 
@@ -399,7 +410,7 @@ def run_case(ft, torch, spec, steps, warmup):
     parts = spec.split(":")
     kind, nn, d = parts[0], int(parts[1]), parts[2]
     precision = ft.FP32 if "fp32" in parts[3:] else ft.FP64
-    params = {"sph": (), "tri": (0.0, 0.0, 0.0), "disk": (0.0, 0.0)}[kind]
+    params = PARAMS[kind]
     if "batch" in parts[3:]:
         res = batch(ft, torch, steps=steps)
         print(json.dumps({"case": spec, "batched_ms": res["batched_ms"]}), flush=True)
@@ -460,6 +471,10 @@ def main():
         mst, flt = single(ft, torch, "tri", n, "hi2lo", ft.FP64, 10, 2, (0.0, 0.0, 0.0))
         msth, _ = single(ft, torch, "tri", n, "hi2lo", ft.FP64, 10, 2, (-0.5, -0.5, -0.5))
         msd, fld = single(ft, torch, "disk", n, "hi2lo", ft.FP64, 10, 2, (0.0, 0.0))
+        for kind in ("sphv", "rectdisk", "annulus"):
+            m_k, fl_k = single(ft, torch, kind, n, "hi2lo", ft.FP64, 10, 2, PARAMS[kind])
+            extra["%s_hi2lo_ms" % kind] = round(m_k, 4)
+            extra["%s_hi2lo_gflops" % kind] = round(fl_k / (m_k * 1e-3) / 1e9, 2)
         extra.update({
             "sph_n256_roundtrip": roundtrip_256(ft, torch),
             "sph_n%d_hi2lo_ms" % (2 * n): round(ms2, 4),

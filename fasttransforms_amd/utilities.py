@@ -30,8 +30,15 @@ def rand_entries(seed, k):
     return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
 
 
+# the layout of each executor kind's matrices: sphv is the sphere's, the
+# rectangle-disk (full width M = n) keeps rows i < n - j as the triangle does
+# (rectdiskrand, ftutilities.c:115-129), the annulus is the disk's
+_LAYOUT = {"sph": "sph", "sphv": "sph", "tri": "tri", "rectdisk": "tri", "disk": "disk", "annulus": "disk"}
+
+
 def valid_rows(kind, n, j):
     j = np.asarray(j)
+    kind = _LAYOUT[kind]
     if kind == "sph":
         return n - (j + 1) // 2
     if kind == "tri":
@@ -40,7 +47,7 @@ def valid_rows(kind, n, j):
 
 
 def full_M(kind, n):
-    return {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[kind]
+    return {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[_LAYOUT[kind]]
 
 
 def _mask(kind, n, M):
@@ -73,22 +80,25 @@ def relinf(A, B):
 
 def rotations(kind, n, M=None):
     """Number of rotations ft_execute_<kind>_hi2lo applies to an n x M matrix
-    (6 algorithmic flops each)."""
+    (6 algorithmic flops each).  Column col holds order m: tri and rectdisk
+    m = col, sph and disk (annulus) m = (col+1)//2, sphv m = (col-1)//2 up to
+    M//2 - 1; orders below 2 (tri: 1) apply nothing."""
     if M is None:
         M = full_M(kind, n)
-    step = 1 if kind == "tri" else 2
+    geo = {"sphv": "sph", "rectdisk": "sph", "annulus": "disk"}.get(kind, kind)
+    step = 1 if geo == "tri" else 2
     total = 0
     for col in range(1, M):
         if kind == "tri":
             m, m1 = col, 0
         else:
-            m = (col + 1) // 2
-            if m < 2:
+            m = col if kind == "rectdisk" else (col - 1) // 2 if kind == "sphv" else (col + 1) // 2
+            if m < 2 or (kind == "sphv" and m > M // 2 - 1):
                 continue
             m1 = m % 2
         jmax = m - step
         for j in range(m1, jmax + 1, step):
-            top = {"sph": n - 3 - j, "tri": n - 2 - j, "disk": n - 2 - (j + 1) // 2}[kind]
+            top = {"sph": n - 3 - j, "tri": n - 2 - j, "disk": n - 2 - (j + 1) // 2}[geo]
             if top >= 0:
                 total += top + 1
     return total

@@ -215,3 +215,20 @@ def test_invalid_widths_and_plans_rejected(ft):
         ft.FT_GPU_EINVAL
     assert ft._lib.ft_gpu_execute(R.ptr, 5, 0, 0, A.ctypes.data_as(ctypes.c_void_p), 8, None) == ft.FT_GPU_EINVAL
     assert ft.last_error() == 0
+
+
+def test_rotation_counts(ft):
+    """utilities.rotations for the new kinds against a direct count of the
+    reference's loop nests (drivers.c:204-220, 496-508; rotations_default.c:31-36)."""
+    from fasttransforms_amd.utilities import rotations
+
+    def count(n, pairs):
+        return sum(max(n - 2 - j, 0) for _, m in pairs for j in range(m % 2, m - 1, 2))
+
+    for n in (3, 10, 64, 101):
+        M = 2 * n - 1
+        assert rotations("sphv", n, M) == count(n, _sphv_pairs(M))
+        assert rotations("sphv", n, 37 if n > 20 else 5) == count(n, _sphv_pairs(37 if n > 20 else 5))
+        assert rotations("rectdisk", n, n) == count(n, _rectdisk_pairs(n))
+        assert rotations("annulus", n) == rotations("disk", n)
+    assert rotations("rectdisk", 1024) * 2 == rotations("sph", 1024)
