@@ -17,7 +17,8 @@ Besides the headline the JSON line carries, under "config": lo2hi at n = 1024;
 the n = 256 round trip; hi2lo at n = 2048; FP32 at n = 512, 1024, 2048 with
 its error against the FP64 result; tri (0,0,0), tri (-1/2,-1/2,-1/2) and disk
 (0,0) at n = 1024; the vector-field (sphv), rectangle-disk (beta 0) and
-annulus (rho = sqrt(1/3)) executors at n = 1024; the batch of 512 sph transforms of log-uniform degree in
+annulus (rho = sqrt(1/3)) executors at n = 1024; sph2fourier and
+fourier2sph at n = 1024 beside the reference's published times; the batch of 512 sph transforms of log-uniform degree in
 [32, 1024] (one launch, a captured HIP graph, and a host loop over the
 single-transform entry point); and batches of 64 tri/disk transforms with n
 uniform on [128, 1024].  It also carries the roofline of the rotation kernel
@@ -405,9 +406,41 @@ def calib_copy(torch):
     torch.cuda.synchronize()
 
 
+# the reference's published sph2fourier forward / backward times (README.md:53-54,
+# 18-thread Xeon W-2191B): degree 1023 is n = 1024
+PUBLISHED_SPH2FOURIER_S = {1024: (0.026590, 0.027123), 2048: (0.141962, 0.145149), 512: (0.006277, 0.006357)}
+
+
+def harmonic(ft, torch, n, transform, trans, steps, warmup):
+    """ms per ft_gpu_execute_harmonic (device-resident n x (2n-1) FP64 matrix)."""
+    from fasttransforms_amd.utilities import fill
+    H = ft.plan_sph2fourier(n)
+    T = torch.from_numpy(np.ascontiguousarray(fill("sph", n, 2 * n - 1, 2024).T)).cuda()
+    stream = torch.cuda.current_stream()
+    return events(ft, torch, stream, lambda: ft.gpu_execute_harmonic(transform, trans, H, T, stream=stream),
+                  steps, warmup)
+
+
+def harmonic_rows(ft, torch, n):
+    """sph2fourier / fourier2sph at n against the reference's published times."""
+    out = {}
+    for transform in ("sph2fourier", "fourier2sph"):
+        ms = harmonic(ft, torch, n, transform, "N", 10, 2)
+        out["%s_ms" % transform] = round(ms, 4)
+        pub = PUBLISHED_SPH2FOURIER_S.get(n)
+        if pub:
+            out["%s_published_ms" % transform] = pub[0 if transform == "sph2fourier" else 1] * 1e3
+            out["%s_vs_published" % transform] = round(out["%s_published_ms" % transform] / ms, 1)
+    return out
+
+
 def run_case(ft, torch, spec, steps, warmup):
     """--case: one configuration only, for profiling runs."""
     parts = spec.split(":")
+    if parts[0] in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv"):
+        ms = harmonic(ft, torch, int(parts[1]), parts[0], parts[2] if len(parts) > 2 else "N", steps, warmup)
+        print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
+        return
     kind, nn, d = parts[0], int(parts[1]), parts[2]
     precision = ft.FP32 if "fp32" in parts[3:] else ft.FP64
     params = PARAMS[kind]
@@ -484,6 +517,7 @@ def main():
             "tri_hi2lo_ms": round(mst, 4), "tri_hi2lo_gflops": round(flt / (mst * 1e-3) / 1e9, 2),
             "tri_m05_hi2lo_ms": round(msth, 4), "tri_m05_hi2lo_gflops": round(flt / (msth * 1e-3) / 1e9, 2),
             "disk_hi2lo_ms": round(msd, 4), "disk_hi2lo_gflops": round(fld / (msd * 1e-3) / 1e9, 2),
+            "harmonic_n%d" % n: harmonic_rows(ft, torch, n),
             "batch_sph_hi2lo_512": batch(ft, torch),
             "batch64_tri": batch64(ft, torch, "tri", (0.0, 0.0, 0.0)),
             "batch64_tri_m05": batch64(ft, torch, "tri", (-0.5, -0.5, -0.5)),

@@ -74,7 +74,12 @@ SIGNATURES = {
     "ft_gpu_set_tuning_table": (_i, [_vp, _i]),
     "ft_gpu_tuning_table": (_i, [_vp, _i]),
     "ft_gpu_tuned_choice": (_i, [_i, _i, _i, _i, _i, _vp]),
+    "ft_plan_sph2fourier": (_vp, [_i]),
+    "ft_destroy_harmonic_plan": (None, [_vp]),
+    "ft_gpu_execute_harmonic": (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp]),
 }
+for _t in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv"):
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 for _dn in ("hi2lo", "lo2hi"):
     for _k in ("sph", "tri", "disk", "rectdisk"):
         SIGNATURES["ft_kernel_%s_%s" % (_k, _dn)] = (None, [_vp, _i, _i, _vp, _i])
@@ -243,6 +248,92 @@ def set_spin_limit(limit=0):
     """Polls before a band-systolic wait gives up (tests of the failure path); 0 = default."""
     if _lib.ft_gpu_set_spin_limit(int(limit)):
         raise ValueError("spin limit must be >= 0")
+
+
+class _HarmonicHead(ctypes.Structure):
+    # the reference's ft_harmonic_plan members (fasttransforms.h:371-385), then the device state
+    _fields_ = [("RP", ctypes.POINTER(ctypes.c_void_p)), ("MP", ctypes.c_void_p),
+                ("B", ctypes.POINTER(ctypes.c_double)), ("P", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+                ("Pinv", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))), ("alpha", ctypes.c_double),
+                ("beta", ctypes.c_double), ("gamma", ctypes.c_double), ("delta", ctypes.c_double),
+                ("rho", ctypes.c_double), ("NRP", ctypes.c_int), ("NMP", ctypes.c_int), ("NP", ctypes.c_int),
+                ("dev", ctypes.c_void_p)]
+
+
+SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV = 0, 1, 2, 3
+_HARMONIC = {"sph2fourier": SPH2FOURIER, "fourier2sph": FOURIER2SPH, "sphv2fourier": SPHV2FOURIER,
+             "fourier2sphv": FOURIER2SPHV}
+
+
+class HarmonicPlan:
+    """Owner of an ``ft_harmonic_plan *`` (ft_plan_sph2fourier)."""
+
+    def __init__(self, ptr, n):
+        if not ptr:
+            _check()
+            raise GPUError("harmonic plan construction failed")
+        self._ptr = ptr
+        self._destroy = _lib.ft_destroy_harmonic_plan
+        self.n = n
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def _head(self):
+        return ctypes.cast(self._ptr, ctypes.POINTER(_HarmonicHead)).contents
+
+    def P(self, k):
+        """Host connection matrix P[k] (a copy, n x n)."""
+        return np.ctypeslib.as_array(self._head().P[k], shape=(self.n * self.n,)).copy().reshape(self.n, self.n, order="F")
+
+    def Pinv(self, k):
+        return np.ctypeslib.as_array(self._head().Pinv[k], shape=(self.n * self.n,)).copy().reshape(self.n, self.n,
+                                                                                                    order="F")
+
+    def __del__(self):
+        ptr, self._ptr = getattr(self, "_ptr", None), None
+        if ptr:
+            self._destroy(ptr)
+
+
+def plan_sph2fourier(n):
+    return HarmonicPlan(_lib.ft_plan_sph2fourier(int(n)), int(n))
+
+
+def _harmonic(name):
+    fn = getattr(_lib, "ft_execute_" + name)
+
+    def run(trans, plan, A, N=None, M=None):
+        N = plan.n if N is None else int(N)
+        M = _M_of(A, N, M)
+        fn(trans.encode() if isinstance(trans, str) else trans, plan.ptr, _pointer(A, N, M), int(N), M)
+        _check()
+        return A
+
+    run.__name__ = "execute_" + name
+    run.__doc__ = "ft_execute_%s(TRANS, plan, A, N, M): in place on the column-major N x M matrix A." % name
+    return run
+
+
+execute_sph2fourier = _harmonic("sph2fourier")
+execute_fourier2sph = _harmonic("fourier2sph")
+execute_sphv2fourier = _harmonic("sphv2fourier")
+execute_fourier2sphv = _harmonic("fourier2sphv")
+
+
+def gpu_execute_harmonic(transform, trans, plan, A, M=None, stream=None):
+    """ft_gpu_execute_harmonic on a device tensor (asynchronous; errors through synchronize())."""
+    import torch
+    transform = _HARMONIC.get(transform, transform)
+    M = _M_of(A, plan.n, M)
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    rc = _lib.ft_gpu_execute_harmonic(int(transform), trans.encode() if isinstance(trans, str) else trans, plan.ptr,
+                                      _pointer(A, plan.n, M), plan.n, M, _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_harmonic: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
 
 
 def last_engine():

@@ -806,7 +806,8 @@ static int kind_geometry(int kind, int *layout) {
 
 /* Matrix widths each executor accepts: every column it touches exists and
  * every sequence it applies is in the plan (n x n tables; disk n x 2n-1). */
-static int valid_M(int kind, int n, int M) {
+static int valid_M(int kind, int n, int M) { return ftamd_valid_M(kind, n, M); }
+int ftamd_valid_M(int kind, int n, int M) {
     if (M < 0) return 0;
     switch (kind) {
     case FT_GPU_TRI:

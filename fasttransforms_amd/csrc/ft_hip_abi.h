@@ -161,6 +161,25 @@ typedef struct {
 
 int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int S, void *stream);
 
+/* Strided triangular products of the harmonic transforms (drivers.c:833-864):
+ * for each class c < ncls, the columns j = c, c + cstride, ... < M of the
+ * n x M column-major X are multiplied by op(U[c]) (U[c] upper triangular,
+ * n x n column-major; op = identity, or the transpose when trans) into the
+ * same columns of Y (n x M, another buffer); U[c] == NULL leaves class c out.
+ * Output tiles of FTAMD_TRMM_ROWS x FTAMD_TRMM_COLS; ctiles[c] column tiles
+ * per class. */
+#define FTAMD_TRMM_ROWS 64
+#define FTAMD_TRMM_COLS 32
+typedef struct {
+    const double *U[4];
+    const double *X;
+    double *Y;
+    int n, M, cstride, ncls, trans;
+    int ctiles[4];
+} FtamdTrmmLaunch;
+int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream);
+int ftamd_hip_d2d(void *dst, const void *src, size_t bytes, void *stream);
+
 /* The allocation-free entry point's device scratch: one fixed array per
  * device, FTAMD_NOALLOC_SCRATCH_BYTES long, whose first 256 bytes are the
  * error word's area and the rest the chain records of the launch. */
@@ -179,6 +198,7 @@ int ftamd_hip_memset(void *ptr, int value, size_t bytes);
 int ftamd_hip_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int ftamd_hip_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int ftamd_hip_sync(void *stream);
+int ftamd_hip_device_sync(void);
 int ftamd_hip_last_runtime_error(void);
 int ftamd_hip_launch(const FtamdLaunch *a, int precision, int kind, int dir, int L, int W, int grid, void *stream);
 

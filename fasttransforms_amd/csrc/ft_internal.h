@@ -68,7 +68,14 @@ ft_rotation_plan *ftamd_plan_alloc(int kind, int family, int n, double alpha, do
 int ftamd_plan_upload(ft_rotation_plan *RP, int precision);
 const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
 
+/* ft_connection.c: connection matrices (transforms.c:50-93), malloc'd n x n
+ * column-major upper triangular, NULL if out of memory */
+double *ftamd_plan_leg2cheb(int normleg, int normcheb, int n);
+double *ftamd_plan_cheb2leg(int normcheb, int normleg, int n);
+double *ftamd_plan_ultra2ultra(int norm1, int norm2, int n, double lambda, double mu);
+
 /* ft_drivers.c */
+int ftamd_valid_M(int kind, int n, int M); /* widths a whole-matrix executor of public kind `kind` accepts */
 void ftamd_set_error(int code);
 int ftamd_tuning_grid(const ft_schedule *S);
 void ftamd_schedule_free_device(ft_schedule *S);

@@ -163,6 +163,12 @@ int ftamd_hip_d2h(void *dst, const void *src, size_t bytes, void *stream) {
     return map_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
 }
 
+int ftamd_hip_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    return map_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+}
+
+int ftamd_hip_device_sync(void) { return map_err(hipDeviceSynchronize()); }
+
 int ftamd_hip_sync(void *stream) { return map_err(hipStreamSynchronize((hipStream_t)stream)); }
 
 int ftamd_hip_last_runtime_error(void) { return g_hip_err; }

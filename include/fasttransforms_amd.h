@@ -98,6 +98,53 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
 void ft_execute_rectdisk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
 void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
 
+/* fasttransforms.h:371-385 -- the harmonic plan: rotation plans, the 1D
+ * connection matrices P and their inverses Pinv (host FP64, n x n
+ * column-major, upper triangular) and parameters; the reference's members
+ * first, then this library's device state.  ft_modified_plan is opaque here
+ * (no plan of this library builds one; MP is NULL, NMP 0), and B, the
+ * reference's host scratch, is NULL: the device path needs none. */
+typedef struct ft_modified_plan_s ft_modified_plan;
+typedef struct {
+    ft_rotation_plan **RP;
+    ft_modified_plan **MP;
+    double *B;
+    double **P;
+    double **Pinv;
+    double alpha;
+    double beta;
+    double gamma;
+    double delta;
+    double rho;
+    int NRP;
+    int NMP;
+    int NP;
+    struct ftamd_harmonic_dev *dev; /* extension: device copies of P and Pinv, work buffers */
+} ft_harmonic_plan;
+
+/* fasttransforms.h:388 (drivers.c:793-814) -- also frees the device state. */
+void ft_destroy_harmonic_plan(ft_harmonic_plan *P);
+
+/* fasttransforms.h:391 (drivers.c:816-831) -- spherical harmonics <->
+ * bivariate Fourier: the sphere's rotation plan, P = {normalized Legendre ->
+ * Chebyshev, normalized ultraspherical(3/2) -> ultraspherical(1)} and their
+ * inverses (transforms.c:50-93, built on the host in long double as the
+ * reference does). */
+ft_harmonic_plan *ft_plan_sph2fourier(const int n);
+
+/* fasttransforms.h:394-399 (drivers.c:833-899) -- A is N x M (N the plan's
+ * n, M odd, M <= 2N-1; M <= 2N+1 for the vector-field pair), updated in
+ * place, host or device memory.  TRANS 'N' applies the transform, 'T' its
+ * transpose.  sph2fourier 'N' is the sphere's hi2lo rotations then, on the
+ * columns j = 0, 3 mod 4, P[0] and on j = 1, 2 mod 4, P[1] (upper triangular
+ * products); fourier2sph inverts it with Pinv and lo2hi.  The vector-field
+ * pair swaps the two matrices' column classes and runs the sphv
+ * rotations. */
+void ft_execute_sph2fourier(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_fourier2sph(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_sphv2fourier(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_fourier2sphv(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+
 /* ------------------------------------------------------------------------
  * Extensions (no counterpart in the reference).
  * ---------------------------------------------------------------------- */
@@ -156,6 +203,16 @@ int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int prec
  * streams or threads) race. */
 int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
                            void *stream);
+
+/* The harmonic transforms on a device-resident A (dA, N x M FP64) on a HIP
+ * stream, asynchronously: transform FT_GPU_SPH2FOURIER, _FOURIER2SPH,
+ * _SPHV2FOURIER or _FOURIER2SPHV, TRANS 'N' or 'T', as the executors above.
+ * Errors of the launches come back through ft_gpu_synchronize.  Calls on
+ * one harmonic plan share its device work buffer: order them on one
+ * stream. */
+enum { FT_GPU_SPH2FOURIER = 0, FT_GPU_FOURIER2SPH = 1, FT_GPU_SPHV2FOURIER = 2, FT_GPU_FOURIER2SPHV = 3 };
+int ft_gpu_execute_harmonic(int transform, char TRANS, const ft_harmonic_plan *P, void *dA, int N, int M,
+                            void *stream);
 
 /* Waits for `stream` and reads the device error words of every asynchronous
  * launch (ft_gpu_execute, ft_gpu_execute_batch, ft_gpu_execute_batch_plan)

@@ -42,6 +42,8 @@ def _load():
         lib.oracle_plan_tri.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.oracle_plan_disk.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.oracle_plan_rectdisk.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        lib.oracle_connection.restype = ctypes.c_int
+        lib.oracle_connection.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_double, _dp]
         lib.oracle_plan_annulus.restype = ctypes.c_int
         lib.oracle_plan_annulus.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         for name in ("sphv", "rectdisk"):
@@ -201,6 +203,49 @@ def execute_q(plan, direction, A, hp_plan=False, M=None):
                               plan.beta, plan.gamma, 1 if hp_plan else 0, _ptr(A), int(M))
     if rc != 0:
         raise MemoryError("oracle_execute_q failed")
+    return A
+
+
+CONNECTIONS = {"leg2cheb": 0, "cheb2leg": 1, "ultra2ultra": 2}
+
+
+def connection(kind, flag1, flag2, n, lam=0.0, mu=0.0):
+    """n x n connection matrix of transforms.c:50-93 (upper triangular):
+    connection("leg2cheb", normleg, normcheb, n), ("cheb2leg", normcheb,
+    normleg, n), ("ultra2ultra", norm1, norm2, n, lambda, mu)."""
+    V = np.zeros((n, n), order="F")
+    if _load().oracle_connection(CONNECTIONS[kind], int(flag1), int(flag2), int(n), float(lam), float(mu),
+                                 _ptr(V)) != 0:
+        raise MemoryError("oracle_connection failed")
+    return V
+
+
+def harmonic(transform, trans, plan, P, Pinv, A, M=None):
+    """The harmonic transforms of drivers.c:833-899 on the n x M matrix A (in
+    place): transform "sph2fourier", "fourier2sph", "sphv2fourier" or
+    "fourier2sphv", trans "N" or "T"; plan the sphere's rotation Plan, P and
+    Pinv the two connection matrices and their inverses.  The rotations are
+    the oracle's; each class's upper-triangular product is evaluated in long
+    double and rounded once."""
+    M = A.shape[1] if M is None else M
+    vec = transform.startswith("sphv") or transform.endswith("sphv")
+    inverse = transform.startswith("fourier")
+    H = Pinv if inverse else P
+    rot_first = (trans == "T") if inverse else (trans == "N")
+    run = execute_sphv if vec else execute
+    if rot_first:
+        run(plan, HI2LO, A, M)
+    for c in range(4):
+        cols = np.arange(c, M, 4)
+        if cols.size == 0:
+            continue
+        even = c in (0, 3)
+        U = np.triu(H[0] if even != vec else H[1]).astype(np.longdouble)
+        if trans == "T":
+            U = U.T
+        A[:, cols] = (U @ A[:, cols].astype(np.longdouble)).astype(np.float64)
+    if not rot_first:
+        run(plan, LO2HI, A, M)
     return A
 
 

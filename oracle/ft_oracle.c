@@ -13,6 +13,9 @@
  *                :270-287 (Dunkl-Xu rectangle-disk, triangle index),
  *                :194-260 (annulus, gamma = 0: Givens rotations of QR
  *                factorizations of shifted Jacobi operators, disk index).
+ *   connection   src/transforms.c:50-93 (Legendre <-> Chebyshev,
+ *                ultraspherical lambda -> mu): eigenvectors of triangular
+ *                banded pencils by back substitution in long double.
  *   kernels      src/rotations/rotations_default.c:31-71 (loop nests), with the
  *                rotation written in the fused form the dispatched SIMD path
  *                computes: fma(c,x,s*y) / fma(c,y,-(s*x)) for hi2lo and
@@ -196,6 +199,107 @@ int oracle_plan_annulus(int n, double alpha, double beta, double rho, double *s,
         }
     }
     free(buf);
+    return 0;
+}
+
+/* Connection matrices (transforms.c:50-93): column j of the n x n upper
+ * triangular V (column-major) is the eigenvector of the pencil A v = lambda
+ * B v, A and B upper triangular with nonzeros on the diagonal and the second
+ * superdiagonal only (a0/b0: diagonal, a2/b2: entry (i-2, i) at index i),
+ * normalized to the diagonal entry V(j, j) = D[j] (the ratio of leading
+ * coefficients), entries above by back substitution with the reference's
+ * cancellation test (banded_source.c:779-798); long double, rounded once.
+ * The pencils and diagonals: legendre_to_chebyshev banded_source.c:2024-2060,
+ * 2316-2338; chebyshev_to_legendre :2063-2101, 2340-2384;
+ * ultraspherical_to_ultraspherical :2102-2131, 2386-2430. */
+static void or_pencil(int n, const long double *a0, const long double *a2, const long double *b0,
+                      const long double *b2, long double *V) {
+    const long double eps = 0x1p-64L;
+    for (int j = 1; j < n; j++) {
+        long double lam = a0[j] / b0[j];
+        for (int i = j - 1; i >= 0; i--) {
+            /* entries (i, i+1) of A and B are zero; (i, i+2) are a2[i+2], b2[i+2] */
+            long double t = (0 - lam * 0) * V[i + 1 + (size_t)j * n], kt = (fabsl(0.0L) + fabsl(lam * 0)) * fabsl(V[i + 1 + (size_t)j * n]);
+            if (i + 2 < n) {
+                t += (a2[i + 2] - lam * b2[i + 2]) * V[i + 2 + (size_t)j * n];
+                kt += (fabsl(a2[i + 2]) + fabsl(lam * b2[i + 2])) * fabsl(V[i + 2 + (size_t)j * n]);
+            }
+            long double d = lam * b0[i] - a0[i], kd = fabsl(lam * b0[i]) + fabsl(a0[i]);
+            V[i + (size_t)j * n] = (fabsl(d) < 4 * kd * eps || fabsl(t) < 4 * kt * eps) ? 0.0L : t / d;
+        }
+    }
+}
+
+enum { OR_LEG2CHEB = 0, OR_CHEB2LEG = 1, OR_ULTRA2ULTRA = 2 };
+
+/* kind OR_LEG2CHEB (f1 = normleg, f2 = normcheb), OR_CHEB2LEG (f1 =
+ * normcheb, f2 = normleg) or OR_ULTRA2ULTRA (f1 = norm1, f2 = norm2,
+ * lambda, mu) into V (n*n doubles).  0, or -1 if out of memory. */
+int oracle_connection(int kind, int f1, int f2, int n, double lambda_, double mu_, double *V) {
+    long double *w = calloc((size_t)(5 * n) + (size_t)n * n, sizeof(long double));
+    if (!w) return -1;
+    long double *a0 = w, *a2 = w + n, *b0 = w + 2 * n, *b2 = w + 3 * n, *D = w + 4 * n, *Vl = w + 5 * n;
+    const long double L = lambda_, U = mu_;
+    for (int i = 0; i < n; i++) {
+        long double x = i;
+        if (kind == OR_LEG2CHEB) {
+            a0[i] = i == 0 ? 0 : i == 1 ? 2 : i * (i + 1.0L);
+            a2[i] = i >= 2 ? -i * (i - 1.0L) : 0;
+            b0[i] = i == 0 ? (f2 ? sqrtl(2) : 2) : 1;
+            b2[i] = i >= 2 ? -1 : 0;
+            if (f1) D[i] = i == 0 ? (f2 ? sqrtl(0.5) * tgammal(0.5) : sqrtl(0.5))
+                           : i == 1 ? (f2 ? sqrtl(1.5) * D[0] : sqrtl(1.5))
+                                    : sqrtl((2 * i + 1) * (2 * i - 1.0L)) * D[i - 1] / (2 * i);
+            else D[i] = i == 0 ? (f2 ? tgammal(0.5) : 1) : i == 1 ? (f2 ? D[0] / sqrtl(2) : 1) : (2 * i - 1) * D[i - 1] / (2 * i);
+        } else if (kind == OR_CHEB2LEG) {
+            if (f2) {
+                a0[i] = i == 0 ? 0 : i == 1 ? sqrtl(2.0L / 5) : i * sqrtl(((i + 1.0L) * (i + 2)) / ((2 * i + 1.0L) * (2 * i + 3))) * i;
+                a2[i] = i >= 2 ? -(i + 1) * sqrtl(((i - 1.0L) * i) / ((2 * i - 1.0L) * (2 * i + 1))) * (i + 1) : 0;
+                b0[i] = i == 0 ? sqrtl(2.0L / 3) : i == 1 ? sqrtl(2.0L / 5) : sqrtl(((i + 1.0L) * (i + 2)) / ((2 * i + 1.0L) * (2 * i + 3)));
+                b2[i] = i >= 2 ? -sqrtl(((i - 1.0L) * i) / ((2 * i - 1.0L) * (2 * i + 1))) : 0;
+            } else {
+                a0[i] = i == 0 ? 0 : i == 1 ? 1.0L / 3 : i / (2 * i + 1.0L) * i;
+                a2[i] = i >= 2 ? -(i + 1) / (2 * i + 1.0L) * (i + 1) : 0;
+                b0[i] = i == 0 ? 1 : i == 1 ? 1.0L / 3 : 1 / (2 * i + 1.0L);
+                b2[i] = i >= 2 ? -1 / (2 * i + 1.0L) : 0;
+            }
+            if (f1 && f2) D[i] = i == 0 ? sqrtl(2) / tgammal(0.5) : i == 1 ? D[0] / sqrtl(1.5) : (2 * i) / sqrtl((2 * i + 1) * (2 * i - 1.0L)) * D[i - 1];
+            else if (f1) D[i] = i == 0 ? 1 / tgammal(0.5) : i == 1 ? sqrtl(2) * D[0] : (2 * i) * D[i - 1] / (2 * i - 1);
+            else if (f2) D[i] = i == 0 ? sqrtl(2) : i == 1 ? 1 / sqrtl(1.5) : (2 * i) / sqrtl((2 * i + 1) * (2 * i - 1.0L)) * D[i - 1];
+            else D[i] = i <= 1 ? 1 : (2 * i) * D[i - 1] / (2 * i - 1);
+        } else {
+            if (f2) {
+                a0[i] = i == 0 ? 0 : i == 1 ? (1 + 2 * L) * copysignl(sqrtl((2 * U + 1) / (2 * U + 4)), U)
+                                 : i * (i + 2 * L) * copysignl(sqrtl(((i + 2 * U) * (i + 2 * U + 1)) / (4 * (i + U) * (i + U + 1))), U);
+                a2[i] = i >= 2 ? -(i + 2 * U) * (i + 2 * (U - L)) * copysignl(sqrtl((i * (i - 1.0L)) / (4 * (i + U) * (i + U - 1))), U) : 0;
+                b0[i] = i == 0 ? sqrtl((2 * U + 1) / (2 * U + 2)) : i == 1 ? copysignl(sqrtl((2 * U + 1) / (2 * U + 4)), U)
+                                 : copysignl(sqrtl(((i + 2 * U) * (i + 2 * U + 1)) / (4 * (i + U) * (i + U + 1))), U);
+                b2[i] = i >= 2 ? -copysignl(sqrtl((i * (i - 1.0L)) / (4 * (i + U) * (i + U - 1))), U) : 0;
+            } else {
+                a0[i] = i == 0 ? 0 : i == 1 ? (1 + 2 * L) * U / (1 + U) : i * (i + 2 * L) * U / (i + U);
+                a2[i] = i >= 2 ? -(i + 2 * U) * (i + 2 * (U - L)) * U / (i + U) : 0;
+                b0[i] = i == 0 ? 1 : i == 1 ? U / (1 + U) : U / (i + U);
+                b2[i] = i >= 2 ? -U / (i + U) : 0;
+            }
+            if (i == 0) {
+                long double d1 = sqrtl(tgammal(L + 1) / (tgammal(0.5) * tgammal(L + 0.5)));
+                long double d2 = sqrtl(tgammal(0.5) * tgammal(U + 0.5) / tgammal(U + 1));
+                D[0] = f1 && f2 ? d1 * d2 : f1 ? d1 : f2 ? d2 : 1;
+            } else {
+                long double s2 = sqrtl((i - 1 + U) / i * (i - 1 + 2 * U) / (i + U));
+                long double s1 = sqrtl(i / (i - 1 + L) * (i + L) / (i - 1 + 2 * L));
+                D[i] = f1 && f2 ? s2 * s1 * (i - 1 + L) / (i - 1 + U) * D[i - 1]
+                       : f1     ? s1 * (i - 1 + L) / (i - 1 + U) * D[i - 1]
+                       : f2     ? s2 * (i - 1 + L) / (i - 1 + U) * D[i - 1]
+                                : (i - 1 + L) / (i - 1 + U) * D[i - 1];
+            }
+        }
+        (void)x;
+        Vl[i + (size_t)i * n] = D[i];
+    }
+    or_pencil(n, a0, a2, b0, b2, Vl);
+    for (size_t k = 0; k < (size_t)n * n; k++) V[k] = (double)Vl[k];
+    free(w);
     return 0;
 }
 

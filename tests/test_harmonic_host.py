@@ -1,0 +1,162 @@
+"""The connection matrices of the harmonic transforms (transforms.c:50-93) and
+the harmonic plan, on the CPU.
+
+The oracle's matrices (oracle/ft_oracle.c) are pinned against exact rational
+conversions that do not use the reference's method: the three-term
+recurrences of the Legendre and Gegenbauer polynomials carried out in the
+Chebyshev (T) or Gegenbauer C^mu basis with Python fractions, then scaled by
+the textbook normalizations (orthonormal Legendre sqrt(l + 1/2); orthonormal
+Chebyshev 1/sqrt(pi), sqrt(2/pi); orthonormal Gegenbauer 1/sqrt(h_l),
+h_l = pi 2^(1-2 lambda) Gamma(l + 2 lambda) / (l! (l + lambda) Gamma(lambda)^2)).
+Inverse pairs multiply to the identity.  The library's plan matrices equal
+the oracle's bit for bit, and the composite oracle inverts itself.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+EPS = 2.0 ** -52
+
+
+def _legendre_in_chebyshev(n):
+    """C[k, l]: P_l = sum_k C[k, l] T_k, exactly (x T_0 = T_1, x T_k = (T_{k-1} + T_{k+1}) / 2)."""
+    def times_x(v):
+        out = [Fraction(0)] * (len(v) + 1)
+        for k, a in enumerate(v):
+            if a == 0:
+                continue
+            if k == 0:
+                out[1] += a
+            else:
+                out[k - 1] += a / 2
+                out[k + 1] += a / 2
+        return out
+    P = [[Fraction(1)], [Fraction(0), Fraction(1)]]
+    for l in range(1, n - 1):
+        xp = times_x(P[l])
+        prev = P[l - 1] + [Fraction(0)] * (len(xp) - len(P[l - 1]))
+        P.append([((2 * l + 1) * a - l * b) / (l + 1) for a, b in zip(xp, prev)])
+    C = np.zeros((n, n))
+    for l in range(n):
+        for k, a in enumerate(P[l][:n]):
+            C[k, l] = float(a)
+    return C
+
+
+def _gegenbauer_in_gegenbauer(n, lam, mu):
+    """C[k, l]: C^lam_l = sum_k C[k, l] C^mu_k, exactly (lam, mu rational > 0):
+    (l+1) C_{l+1} = 2 (l + lam) x C_l - (l + 2 lam - 1) C_{l-1}, and
+    x C^mu_k = ((k + 1) C^mu_{k+1} + (k + 2 mu - 1) C^mu_{k-1}) / (2 (k + mu))."""
+    lam, mu = Fraction(lam), Fraction(mu)
+
+    def times_x(v):
+        out = [Fraction(0)] * (len(v) + 1)
+        for k, a in enumerate(v):
+            if a == 0:
+                continue
+            out[k + 1] += a * (k + 1) / (2 * (k + mu))
+            if k > 0:
+                out[k - 1] += a * (k + 2 * mu - 1) / (2 * (k + mu))
+        return out
+    G = [[Fraction(1)], times_x([2 * lam])]
+    G[1] = [a for a in G[1]]
+    for l in range(1, n - 1):
+        xg = times_x(G[l])
+        prev = G[l - 1] + [Fraction(0)] * (len(xg) - len(G[l - 1]))
+        G.append([(2 * (l + lam) * a - (l + 2 * lam - 1) * b) / (l + 1) for a, b in zip(xg, prev)])
+    C = np.zeros((n, n))
+    for l in range(n):
+        for k, a in enumerate(G[l][:n]):
+            C[k, l] = float(a)
+    return C
+
+
+def _gegenbauer_norm(l, lam):
+    """sqrt(h_l) of C^lam_l under the weight (1 - x^2)^(lam - 1/2)."""
+    lg = math.lgamma(l + 2 * lam) - math.lgamma(l + 1) - 2 * math.lgamma(lam)
+    return math.sqrt(math.pi * 2.0 ** (1 - 2 * lam) / (l + lam) * math.exp(lg))
+
+
+def _close(V, W):
+    return np.max(np.abs(V - W)) <= 8 * EPS * max(1.0, np.max(np.abs(W)))
+
+
+@pytest.mark.parametrize("normleg,normcheb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_legendre_to_chebyshev_exact(normleg, normcheb):
+    n = 40
+    C = _legendre_in_chebyshev(n)
+    dl = np.array([math.sqrt(l + 0.5) if normleg else 1.0 for l in range(n)])
+    dc = np.array([(1 / math.sqrt(math.pi) if k == 0 else math.sqrt(2 / math.pi)) if normcheb else 1.0
+                   for k in range(n)])
+    want = C * dl[None, :] / dc[:, None]
+    V = O.connection("leg2cheb", normleg, normcheb, n)
+    assert _close(V, want)
+    W = O.connection("cheb2leg", normcheb, normleg, n)
+    assert np.max(np.abs(W @ V - np.eye(n))) < 64 * EPS
+
+
+@pytest.mark.parametrize("lam,mu", [(1.5, 1.0), (1.0, 1.5), (0.5, 1.0), (2.0, 0.5), (2.5, 1.5)])
+@pytest.mark.parametrize("norm1,norm2", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_ultraspherical_exact(lam, mu, norm1, norm2):
+    n = 30
+    C = _gegenbauer_in_gegenbauer(n, lam, mu)
+    d1 = np.array([1 / _gegenbauer_norm(l, lam) if norm1 else 1.0 for l in range(n)])
+    d2 = np.array([1 / _gegenbauer_norm(k, mu) if norm2 else 1.0 for k in range(n)])
+    want = C * d1[None, :] / d2[:, None]
+    V = O.connection("ultra2ultra", norm1, norm2, n, lam, mu)
+    assert np.max(np.abs(V - want)) <= 1e-13 * max(1.0, np.max(np.abs(want)))
+    W = O.connection("ultra2ultra", norm2, norm1, n, mu, lam)
+    assert np.max(np.abs(W @ V - np.eye(n))) < 1e-13
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 256])
+def test_product_connection_matrices_bitwise_equal_oracle(ft, n):
+    H = ft.plan_sph2fourier(n)
+    assert np.array_equal(H.P(0), O.connection("leg2cheb", 1, 0, n))
+    assert np.array_equal(H.P(1), O.connection("ultra2ultra", 1, 0, n, 1.5, 1.0))
+    assert np.array_equal(H.Pinv(0), O.connection("cheb2leg", 0, 1, n))
+    assert np.array_equal(H.Pinv(1), O.connection("ultra2ultra", 0, 1, n, 1.0, 1.5))
+    head = H._head()
+    assert head.NRP == 1 and head.NP == 2 and head.NMP == 0 and not head.B
+
+
+def _sph_plan(n):
+    return O.Plan("sph", n), O.connection("leg2cheb", 1, 0, n), O.connection("ultra2ultra", 1, 0, n, 1.5, 1.0), \
+        O.connection("cheb2leg", 0, 1, n), O.connection("ultra2ultra", 0, 1, n, 1.0, 1.5)
+
+
+@pytest.mark.parametrize("vec", [False, True])
+def test_harmonic_oracle_round_trip_and_transpose(vec):
+    """fourier2sph inverts sph2fourier (README.md:55-56 reports ~1e-15 for the
+    reference); 'T' is the transpose of 'N': <y, F x> = <F^T y, x>."""
+    n = 64
+    Q, P0, P1, I0, I1 = _sph_plan(n)
+    M = 2 * n - 1
+    fwd, inv = ("sphv2fourier", "fourier2sphv") if vec else ("sph2fourier", "fourier2sph")
+    A = O.fill("sph", n, M, 5)
+    X = O.harmonic(inv, "N", Q, (P0, P1), (I0, I1), O.harmonic(fwd, "N", Q, (P0, P1), (I0, I1), A.copy(order="F")))
+    assert O.rel2(X, A) < 4 * np.sqrt(n) * EPS
+    rng = np.random.default_rng(1)
+    x = np.asfortranarray(rng.standard_normal((n, M)))
+    y = np.asfortranarray(rng.standard_normal((n, M)))
+    Fx = O.harmonic(fwd, "N", Q, (P0, P1), (I0, I1), x.copy(order="F"))
+    Fty = O.harmonic(fwd, "T", Q, (P0, P1), (I0, I1), y.copy(order="F"))
+    assert abs(np.sum(y * Fx) - np.sum(Fty * x)) < 1e-12 * np.sqrt(np.sum(y * y) * np.sum(x * x)) * n
+
+
+def test_harmonic_plan_rejects(ft):
+    with pytest.raises(ft.GPUError):
+        ft.plan_sph2fourier(0)
+    H = ft.plan_sph2fourier(8)
+    A = np.zeros((8, 15), order="F")
+    with pytest.raises(ft.GPUError):
+        ft.execute_sph2fourier("X", H, A)        # TRANS
+    with pytest.raises(ft.GPUError):
+        ft.execute_sph2fourier("N", H, np.zeros((8, 16), order="F"))  # even M
+    with pytest.raises(ft.GPUError):
+        ft.execute_sph2fourier("N", H, np.zeros((9, 15), order="F"), N=9)  # N != plan's n
+    assert ft.last_error() == 0
