@@ -17,8 +17,9 @@ Besides the headline the JSON line carries, under "config": lo2hi at n = 1024;
 the n = 256 round trip; hi2lo at n = 2048; FP32 at n = 512, 1024, 2048 with
 its error against the FP64 result; tri (0,0,0), tri (-1/2,-1/2,-1/2) and disk
 (0,0) at n = 1024; the vector-field (sphv), rectangle-disk (beta 0) and
-annulus (rho = sqrt(1/3)) executors at n = 1024; sph2fourier and
-fourier2sph at n = 1024 beside the reference's published times; the batch of 512 sph transforms of log-uniform degree in
+annulus (rho = sqrt(1/3)) executors at n = 1024; the harmonic transforms
+(sph2fourier, tri2cheb, disk2cxf and their inverses) at n = 1024 beside the
+reference's published times; the batch of 512 sph transforms of log-uniform degree in
 [32, 1024] (one launch, a captured HIP graph, and a host loop over the
 single-transform entry point); and batches of 64 tri/disk transforms with n
 uniform on [128, 1024].  It also carries the roofline of the rotation kernel
@@ -406,38 +407,52 @@ def calib_copy(torch):
     torch.cuda.synchronize()
 
 
-# the reference's published sph2fourier forward / backward times (README.md:53-54,
-# 18-thread Xeon W-2191B): degree 1023 is n = 1024
-PUBLISHED_SPH2FOURIER_S = {1024: (0.026590, 0.027123), 2048: (0.141962, 0.145149), 512: (0.006277, 0.006357)}
+# the reference's published forward / backward times of the harmonic transforms
+# (README.md:51-76, 18-thread Xeon W-2191B, (alpha, beta, gamma) = 0): the
+# column "degree 1023" is n = 1024
+PUBLISHED_S = {
+    "sph2fourier": {512: 0.006277, 1024: 0.026590, 2048: 0.141962},
+    "fourier2sph": {512: 0.006357, 1024: 0.027123, 2048: 0.145149},
+    "tri2cheb": {1024: 0.019123, 2048: 0.113299},
+    "cheb2tri": {1024: 0.019529, 2048: 0.114718},
+    "disk2cxf": {1024: 0.064321, 2048: 0.369272},
+    "cxf2disk": {1024: 0.064544, 2048: 0.372493},
+}
 
 
 def harmonic(ft, torch, n, transform, trans, steps, warmup):
-    """ms per ft_gpu_execute_harmonic (device-resident n x (2n-1) FP64 matrix)."""
+    """ms per ft_gpu_execute_harmonic on a device-resident FP64 matrix of the transform's full width."""
     from fasttransforms_amd.utilities import fill
-    H = ft.plan_sph2fourier(n)
-    T = torch.from_numpy(np.ascontiguousarray(fill("sph", n, 2 * n - 1, 2024).T)).cuda()
+    if transform in ("tri2cheb", "cheb2tri"):
+        H, kind = ft.plan_tri2cheb(n, 0.0, 0.0, 0.0), "tri"
+    elif transform in ("disk2cxf", "cxf2disk"):
+        H, kind = ft.plan_disk2cxf(n, 0.0, 0.0), "disk"
+    else:
+        H, kind = ft.plan_sph2fourier(n), "sph"
+    M = {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[kind]
+    T = torch.from_numpy(np.ascontiguousarray(fill(kind, n, M, 2024).T)).cuda()
     stream = torch.cuda.current_stream()
     return events(ft, torch, stream, lambda: ft.gpu_execute_harmonic(transform, trans, H, T, stream=stream),
                   steps, warmup)
 
 
 def harmonic_rows(ft, torch, n):
-    """sph2fourier / fourier2sph at n against the reference's published times."""
+    """The harmonic transforms at n against the reference's published times."""
     out = {}
-    for transform in ("sph2fourier", "fourier2sph"):
+    for transform in ("sph2fourier", "fourier2sph", "tri2cheb", "cheb2tri", "disk2cxf", "cxf2disk"):
         ms = harmonic(ft, torch, n, transform, "N", 10, 2)
         out["%s_ms" % transform] = round(ms, 4)
-        pub = PUBLISHED_SPH2FOURIER_S.get(n)
+        pub = PUBLISHED_S[transform].get(n)
         if pub:
-            out["%s_published_ms" % transform] = pub[0 if transform == "sph2fourier" else 1] * 1e3
-            out["%s_vs_published" % transform] = round(out["%s_published_ms" % transform] / ms, 1)
+            out["%s_published_ms" % transform] = round(pub * 1e3, 3)
+            out["%s_vs_published" % transform] = round(pub * 1e3 / ms, 1)
     return out
 
 
 def run_case(ft, torch, spec, steps, warmup):
     """--case: one configuration only, for profiling runs."""
     parts = spec.split(":")
-    if parts[0] in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv"):
+    if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv"):
         ms = harmonic(ft, torch, int(parts[1]), parts[0], parts[2] if len(parts) > 2 else "N", steps, warmup)
         print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
         return

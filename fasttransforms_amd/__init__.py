@@ -75,10 +75,13 @@ SIGNATURES = {
     "ft_gpu_tuning_table": (_i, [_vp, _i]),
     "ft_gpu_tuned_choice": (_i, [_i, _i, _i, _i, _i, _vp]),
     "ft_plan_sph2fourier": (_vp, [_i]),
+    "ft_plan_tri2cheb": (_vp, [_i, _d, _d, _d]),
+    "ft_plan_disk2cxf": (_vp, [_i, _d, _d]),
     "ft_destroy_harmonic_plan": (None, [_vp]),
     "ft_gpu_execute_harmonic": (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp]),
 }
-for _t in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv"):
+for _t in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv", "tri2cheb", "cheb2tri", "disk2cxf",
+           "cxf2disk"):
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 for _dn in ("hi2lo", "lo2hi"):
     for _k in ("sph", "tri", "disk", "rectdisk"):
@@ -260,9 +263,10 @@ class _HarmonicHead(ctypes.Structure):
                 ("dev", ctypes.c_void_p)]
 
 
-SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV = 0, 1, 2, 3
+SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV, DISK2CXF, CXF2DISK, TRI2CHEB, CHEB2TRI = range(8)
 _HARMONIC = {"sph2fourier": SPH2FOURIER, "fourier2sph": FOURIER2SPH, "sphv2fourier": SPHV2FOURIER,
-             "fourier2sphv": FOURIER2SPHV}
+             "fourier2sphv": FOURIER2SPHV, "disk2cxf": DISK2CXF, "cxf2disk": CXF2DISK, "tri2cheb": TRI2CHEB,
+             "cheb2tri": CHEB2TRI}
 
 
 class HarmonicPlan:
@@ -301,6 +305,14 @@ def plan_sph2fourier(n):
     return HarmonicPlan(_lib.ft_plan_sph2fourier(int(n)), int(n))
 
 
+def plan_tri2cheb(n, alpha, beta, gamma):
+    return HarmonicPlan(_lib.ft_plan_tri2cheb(int(n), float(alpha), float(beta), float(gamma)), int(n))
+
+
+def plan_disk2cxf(n, alpha, beta):
+    return HarmonicPlan(_lib.ft_plan_disk2cxf(int(n), float(alpha), float(beta)), int(n))
+
+
 def _harmonic(name):
     fn = getattr(_lib, "ft_execute_" + name)
 
@@ -320,6 +332,10 @@ execute_sph2fourier = _harmonic("sph2fourier")
 execute_fourier2sph = _harmonic("fourier2sph")
 execute_sphv2fourier = _harmonic("sphv2fourier")
 execute_fourier2sphv = _harmonic("fourier2sphv")
+execute_tri2cheb = _harmonic("tri2cheb")
+execute_cheb2tri = _harmonic("cheb2tri")
+execute_disk2cxf = _harmonic("disk2cxf")
+execute_cxf2disk = _harmonic("cxf2disk")
 
 
 def gpu_execute_harmonic(transform, trans, plan, A, M=None, stream=None):

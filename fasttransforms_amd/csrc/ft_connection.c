@@ -16,6 +16,14 @@
 
 #include "ft_internal.h"
 
+/* pi-related constants in long double (as <math.h> gives them under _GNU_SOURCE) */
+#ifndef M_PIl
+#define M_PIl 3.141592653589793238462643383279502884L
+#define M_PI_2l 1.570796326794896619231321691639751442L
+#define M_1_PIl 0.318309886183790671537767526745028724L
+#define M_2_PIl 0.636619772367581343075535053490057448L
+#endif
+
 /* A triangular-banded matrix of bandwidth 2 in long double: entry (i, j),
  * 0 <= j - i <= 2, at d[i + 2 (j + 1)] (banded_source.c:261-275). */
 typedef struct {
@@ -229,4 +237,156 @@ double *ftamd_plan_ultra2ultra(int norm1, int norm2, int n, double lambda_, doub
         }
     }
     return finish(&A, &B, V, n);
+}
+
+/* --------------------------------------------------------------- Jacobi -- */
+
+/* Entries of the reference's banded Jacobi operators (banded_source.c:
+ * 1774-1905), long double, normalized (norm) or not.  Derivative D^order of
+ * P^{(a,b)}: entry (j-order, j).  Multiplication x P^{(a,b)}: (j+d, j) for
+ * d = -1, 0, 1.  Raising P^{(a,b)} -> P^{(a+1,b+1)}: (j-2+e, j) for e = 0, 1,
+ * 2.  Lowering (1-x^2) P^{(a+1,b+1)} -> P^{(a,b)}: (j+e, j), e = 0, 1, 2. */
+static long double jac_derivative(int norm, int order, int j, long double a, long double b) {
+    long double v = 1;
+    for (int k = 0; k < order; k++) v *= norm ? (j - k) * (j + a + b + k + 1) : (j + a + b + k + 1) / 2;
+    return norm ? sqrtl(v) : v;
+}
+
+static long double jac_multiplication(int norm, int d, int j, long double a, long double b) {
+    if (norm) {
+        if (d == -1)
+            return j == 1 ? 2 * sqrtl(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)))
+                          : 2 * sqrtl((j * (j + a) * (j + b) * (j + a + b)) /
+                                      ((2 * j + a + b - 1) * (2 * j + a + b) * (2 * j + a + b) * (2 * j + a + b + 1)));
+        if (d == 0) return j == 0 ? (b - a) / (a + b + 2) : ((b - a) * (a + b)) / ((2 * j + a + b) * (2 * j + a + b + 2));
+        return j == 0 ? 2 * sqrtl(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)))
+                      : 2 * sqrtl(((j + 1) * (j + a + 1) * (j + b + 1) * (j + a + b + 1)) /
+                                  ((2 * j + a + b + 1) * (2 * j + a + b + 2) * (2 * j + a + b + 2) * (2 * j + a + b + 3)));
+    }
+    if (d == -1) return (2 * (j + a) * (j + b)) / ((2 * j + a + b) * (2 * j + a + b + 1));
+    if (d == 0) return j == 0 ? (b - a) / (a + b + 2) : ((b - a) * (a + b)) / ((2 * j + a + b) * (2 * j + a + b + 2));
+    return j == 0 ? 2 / (a + b + 2) : (2 * (j + 1) * (j + a + b + 1)) / ((2 * j + a + b + 1) * (2 * j + a + b + 2));
+}
+
+static long double jac_raising(int norm, int e, int j, long double a, long double b) {
+    if (norm) {
+        if (e == 0)
+            return -2 * sqrtl(((j - 1) * j * (j + a) * (j + b)) /
+                              ((2 * j + a + b - 1) * (2 * j + a + b) * (2 * j + a + b) * (2 * j + a + b + 1)));
+        if (e == 1) return 2 * (a - b) * sqrtl(j * (j + a + b + 1)) / ((2 * j + a + b) * (2 * j + a + b + 2));
+        return j == 0 ? 2 * sqrtl(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 3)))
+                      : 2 * sqrtl(((j + a + 1) * (j + b + 1) * (j + a + b + 1) * (j + a + b + 2)) /
+                                  ((2 * j + a + b + 1) * (2 * j + a + b + 2) * (2 * j + a + b + 2) * (2 * j + a + b + 3)));
+    }
+    if (e == 0) return -((j + a) * (j + b)) / ((2 * j + a + b) * (2 * j + a + b + 1));
+    if (e == 1) return ((a - b) * (j + a + b + 1)) / ((2 * j + a + b) * (2 * j + a + b + 2));
+    return j == 0 ? 1 : ((j + a + b + 1) * (j + a + b + 2)) / ((2 * j + a + b + 1) * (2 * j + a + b + 2));
+}
+
+static long double jac_lowering(int norm, int e, int j, long double a, long double b) {
+    if (norm) {
+        if (e == 0)
+            return j == 0 ? 2 * sqrtl(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 3)))
+                          : 2 * sqrtl(((j + a + 1) * (j + b + 1) * (j + a + b + 1) * (j + a + b + 2)) /
+                                      ((2 * j + a + b + 1) * (2 * j + a + b + 2) * (2 * j + a + b + 2) * (2 * j + a + b + 3)));
+        if (e == 1) return 2 * (a - b) * sqrtl((j + 1) * (j + a + b + 2)) / ((2 * j + a + b + 2) * (2 * j + a + b + 4));
+        return -2 * sqrtl(((j + 1) * (j + 2) * (j + a + 2) * (j + b + 2)) /
+                          ((2 * j + a + b + 3) * (2 * j + a + b + 4) * (2 * j + a + b + 4) * (2 * j + a + b + 5)));
+    }
+    if (e == 0) return (4 * (j + a + 1) * (j + b + 1)) / ((2 * j + a + b + 2) * (2 * j + a + b + 3));
+    if (e == 1) return (4 * (a - b) * (j + 1)) / ((2 * j + a + b + 2) * (2 * j + a + b + 4));
+    return -(4 * (j + 1) * (j + 2)) / ((2 * j + a + b + 3) * (2 * j + a + b + 4));
+}
+
+/* transforms.c:95-108 with banded_source.c:2133-2172 (diagonal) and
+ * 2432-2455: A = -L1 D2 + (alpha-beta) D1 + (alpha+beta+2) M1 D1 in the
+ * P^{(gamma+1, delta+1)} basis (D1, D2 derivatives of P^{(gamma,delta)}, L1
+ * lowering and M1 multiplication at (gamma+1, delta+1)), B the raising
+ * operator of P^{(gamma, delta)}; every product has a single term, so the
+ * entries are formed in the reference's order. */
+double *ftamd_plan_jac2jac(int norm1, int norm2, int n, double alpha_, double beta_, double gamma_, double delta_) {
+    const long double al = alpha_, be = beta_, ga = gamma_, de = delta_;
+    tb2 A, B;
+    tb_alloc(&A, n);
+    tb_alloc(&B, n);
+    long double *V = calloc((size_t)n * (size_t)(n > 0 ? n : 1), sizeof(long double));
+    if (A.d && B.d) {
+        for (int j = 0; j < n; j++)
+            for (int i = j - 2 < 0 ? 0 : j - 2; i <= j; i++) {
+                long double ab = 0;
+                if (j >= 2) ab = jac_lowering(norm2, i - (j - 2), j - 2, ga + 1, de + 1) * jac_derivative(norm2, 2, j, ga, de);
+                long double v = -1 * ab + 0 * 0.0L;
+                v = 1 * v + (al - be) * (i == j - 1 ? jac_derivative(norm2, 1, j, ga, de) : 0.0L);
+                ab = 0;
+                if (j >= 1) ab = jac_multiplication(norm2, i - (j - 1), j - 1, ga + 1, de + 1) * jac_derivative(norm2, 1, j, ga, de);
+                v = (al + be + 2) * ab + 1 * v;
+                tb_set(&A, v, i, j);
+                tb_set(&B, jac_raising(norm2, i - (j - 2), j, ga, de), i, j);
+            }
+    }
+    if (V) {
+        const size_t inc = (size_t)n + 1;
+        for (int i = 0; i < n; i++) {
+            long double d;
+            if (norm1 && norm2) {
+                if (i == 0)
+                    d = sqrtl(tgammal(al + be + 2) / (powl(2, al + be + 1) * tgammal(al + 1) * tgammal(be + 1)) *
+                              (powl(2, ga + de + 1) * tgammal(ga + 1) * tgammal(de + 1)) / tgammal(ga + de + 2));
+                else if (i == 1)
+                    d = (al + be + 2) / (ga + de + 2) * sqrtl((al + be + 3) / ((al + 1) * (be + 1)) * ((ga + 1) * (de + 1)) / (ga + de + 3)) * V[0];
+                else
+                    d = (2 * i + al + be) / (2 * i + ga + de) *
+                        sqrtl(((2 * i + al + be - 1) * (2 * i + al + be + 1)) / ((i + al) * (i + be) * (i + al + be)) *
+                              ((i + ga) * (i + de) * (i + ga + de)) / ((2 * i + ga + de - 1) * (2 * i + ga + de + 1))) *
+                        V[(i - 1) * inc];
+            } else if (norm1) {
+                if (i == 0) d = sqrtl(tgammal(al + be + 2) / (powl(2, al + be + 1) * tgammal(al + 1) * tgammal(be + 1)));
+                else if (i == 1) d = (al + be + 2) / (ga + de + 2) * sqrtl((al + be + 3) / ((al + 1) * (be + 1))) * V[0];
+                else
+                    d = sqrtl(((2 * i + al + be - 1) * (2 * i + al + be) * (2 * i + al + be) * (2 * i + al + be + 1)) /
+                              (i * (i + al) * (i + be) * (i + al + be))) *
+                        (i * (i + ga + de)) / ((2 * i + ga + de - 1) * (2 * i + ga + de)) * V[(i - 1) * inc];
+            } else if (norm2) {
+                if (i == 0) d = sqrtl((powl(2, ga + de + 1) * tgammal(ga + 1) * tgammal(de + 1)) / tgammal(ga + de + 2));
+                else if (i == 1) d = (al + be + 2) / (ga + de + 2) * sqrtl(((ga + 1) * (de + 1)) / (ga + de + 3)) * V[0];
+                else
+                    d = ((2 * i + al + be - 1) * (2 * i + al + be)) / (i * (i + al + be)) *
+                        sqrtl((i * (i + ga) * (i + de) * (i + ga + de)) /
+                              ((2 * i + ga + de - 1) * (2 * i + ga + de) * (2 * i + ga + de) * (2 * i + ga + de + 1))) *
+                        V[(i - 1) * inc];
+            } else {
+                if (i == 0) d = 1;
+                else if (i == 1) d = (al + be + 2) / (ga + de + 2);
+                else
+                    d = ((2 * i + al + be - 1) * (2 * i + al + be) * (i + ga + de)) /
+                        ((i + al + be) * (2 * i + ga + de - 1) * (2 * i + ga + de)) * V[(i - 1) * inc];
+            }
+            V[i * inc] = d;
+        }
+    }
+    return finish(&A, &B, V, n);
+}
+
+/* transforms.c:157-169: rows scaled by sqrt(1/pi), sqrt(2/pi) (rounded to
+ * double first) for the unnormalized Chebyshev target. */
+double *ftamd_plan_jac2cheb(int normjac, int normcheb, int n, double alpha, double beta) {
+    double *V = ftamd_plan_jac2jac(normjac, 1, n, alpha, beta, -0.5, -0.5);
+    if (V && !normcheb)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i <= j; i++) {
+                const double sclrow = i ? (double)sqrtl(M_2_PIl) : (double)sqrtl(M_1_PIl);
+                V[i + (size_t)j * n] *= sclrow;
+            }
+    return V;
+}
+
+/* transforms.c:171-183: columns scaled by sqrt(pi), sqrt(pi/2). */
+double *ftamd_plan_cheb2jac(int normcheb, int normjac, int n, double alpha, double beta) {
+    double *V = ftamd_plan_jac2jac(1, normjac, n, -0.5, -0.5, alpha, beta);
+    if (V && !normcheb)
+        for (int j = 0; j < n; j++) {
+            const double sclcol = j ? (double)sqrtl(M_PI_2l) : (double)sqrtl(M_PIl);
+            for (int i = 0; i <= j; i++) V[i + (size_t)j * n] *= sclcol;
+        }
+    return V;
 }

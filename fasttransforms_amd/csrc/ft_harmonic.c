@@ -1,17 +1,20 @@
 /*
- * ft_harmonic.c -- the spherical-harmonic <-> bivariate Fourier transforms
- * (drivers.c:793-899): the rotation stage on the GPU followed (or preceded)
- * by the strided upper-triangular products with the 1D connection matrices,
- * which run as one matrix-core launch (ft_trmm.hip) instead of four BLAS
- * calls.
+ * ft_harmonic.c -- the harmonic transforms (drivers.c:793-1021): the rotation
+ * stage on the GPU followed (or preceded) by the upper-triangular products
+ * with the 1D connection matrices, which run as matrix-core launches
+ * (ft_trmm.hip) instead of BLAS calls.
  *
- * TRANS 'N' of sph2fourier is hi2lo then the products with P; 'T' is the
- * transposed products then lo2hi; fourier2sph uses Pinv with lo2hi after
- * ('N') or hi2lo before ('T').  Column classes j mod 4: the sphere's orders
- * of even parity sit in j = 0, 3 and take P[0], the odd ones in j = 1, 2 and
- * take P[1]; the vector-field layout (columns 2m+1, 2m+2) swaps them
- * (drivers.c:867-899).
+ * sph2fourier / disk2cxf: TRANS 'N' is hi2lo then the products with P on the
+ * column classes j mod 4 (the orders of even parity sit in j = 0, 3 and take
+ * P[0], the odd ones in j = 1, 2 and take P[1]; the vector-field layout,
+ * columns 2m+1 and 2m+2, swaps them, drivers.c:867-899); 'T' is the
+ * transposed products then lo2hi; the inverses (fourier2sph, cxf2disk) use
+ * Pinv with lo2hi after ('N') or hi2lo before ('T').  tri2cheb: hi2lo, P[0]
+ * from the left on every column, P[1]^T from the right on every row, the 2D
+ * Chebyshev normalization; cheb2tri runs the inverses in reverse order
+ * (drivers.c:921-957).
  */
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -53,14 +56,11 @@ void ft_destroy_harmonic_plan(ft_harmonic_plan *P) {
     free(P);
 }
 
-/* drivers.c:816-831 */
-ft_harmonic_plan *ft_plan_sph2fourier(const int n) {
-    if (n < 1) {
-        ftamd_set_error(FT_GPU_EINVAL);
-        return NULL;
-    }
+/* A plan with one rotation plan and two matrix pairs, members allocated
+ * (NULL if out of memory). */
+static ft_harmonic_plan *plan_alloc(void) {
     ft_harmonic_plan *P = calloc(1, sizeof(ft_harmonic_plan));
-    if (!P) goto oom;
+    if (!P) return NULL;
     P->dev = calloc(1, sizeof(struct ftamd_harmonic_dev));
     if (P->dev) {
         pthread_mutex_init(&P->dev->lock, NULL);
@@ -69,20 +69,89 @@ ft_harmonic_plan *ft_plan_sph2fourier(const int n) {
     P->RP = calloc(1, sizeof(ft_rotation_plan *));
     P->P = calloc(2, sizeof(double *));
     P->Pinv = calloc(2, sizeof(double *));
-    if (!P->dev || !P->RP || !P->P || !P->Pinv) goto oom;
+    if (!P->dev || !P->RP || !P->P || !P->Pinv) {
+        ft_destroy_harmonic_plan(P);
+        return NULL;
+    }
     P->NRP = 1;
     P->NP = 2;
+    return P;
+}
+
+/* The plan if every member was built, else NULL (destroyed; error set). */
+static ft_harmonic_plan *plan_done(ft_harmonic_plan *P) {
+    if (P && P->RP[0] && P->P[0] && P->P[1] && P->Pinv[0] && P->Pinv[1]) return P;
+    ft_destroy_harmonic_plan(P);
+    ftamd_set_error(FT_GPU_ENOMEM);
+    return NULL;
+}
+
+/* drivers.c:816-831 */
+ft_harmonic_plan *ft_plan_sph2fourier(const int n) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_harmonic_plan *P = plan_alloc();
+    if (!P) return plan_done(NULL);
     P->RP[0] = ft_plan_rotsphere(n);
     P->P[0] = ftamd_plan_leg2cheb(1, 0, n);
     P->P[1] = ftamd_plan_ultra2ultra(1, 0, n, 1.5, 1.0);
     P->Pinv[0] = ftamd_plan_cheb2leg(0, 1, n);
     P->Pinv[1] = ftamd_plan_ultra2ultra(0, 1, n, 1.0, 1.5);
-    if (!P->RP[0] || !P->P[0] || !P->P[1] || !P->Pinv[0] || !P->Pinv[1]) goto oom;
-    return P;
-oom:
-    ft_destroy_harmonic_plan(P);
-    ftamd_set_error(FT_GPU_ENOMEM);
-    return NULL;
+    return plan_done(P);
+}
+
+/* drivers.c:901-919 */
+ft_harmonic_plan *ft_plan_tri2cheb(const int n, const double alpha, const double beta, const double gamma) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_harmonic_plan *P = plan_alloc();
+    if (!P) return plan_done(NULL);
+    P->RP[0] = ft_plan_rottriangle(n, alpha, beta, gamma);
+    P->P[0] = ftamd_plan_jac2jac(1, 1, n, beta + gamma + 1.0, alpha, -0.5, -0.5);
+    P->P[1] = ftamd_plan_jac2jac(1, 1, n, gamma, beta, -0.5, -0.5);
+    P->Pinv[0] = ftamd_plan_jac2jac(1, 1, n, -0.5, -0.5, beta + gamma + 1.0, alpha);
+    P->Pinv[1] = ftamd_plan_jac2jac(1, 1, n, -0.5, -0.5, gamma, beta);
+    P->alpha = alpha;
+    P->beta = beta;
+    P->gamma = gamma;
+    return plan_done(P);
+}
+
+/* drivers.c:959-985: P[0] rows of the Chebyshev side and P[1] scaled by
+ * 2^((alpha+beta+2)/2) (times sqrt(2/pi) for P[1]), the inverses by the
+ * reciprocals, on and above the diagonal. */
+ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double beta) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_harmonic_plan *P = plan_alloc();
+    if (!P) return plan_done(NULL);
+    P->RP[0] = ft_plan_rotdisk(n, alpha, beta);
+    P->P[0] = ftamd_plan_jac2cheb(1, 0, n, beta, alpha);
+    P->P[1] = ftamd_plan_jac2jac(1, 1, n, beta, alpha + 1.0, -0.5, 0.5);
+    P->Pinv[0] = ftamd_plan_cheb2jac(0, 1, n, beta, alpha);
+    P->Pinv[1] = ftamd_plan_jac2jac(1, 1, n, -0.5, 0.5, beta, alpha + 1.0);
+    if (P->P[0] && P->P[1] && P->Pinv[0] && P->Pinv[1]) {
+        const double cst0 = pow(2.0, 0.5 * (alpha + beta + 2.0));
+        const double cst1 = cst0 * 0.797884560802865355; /* sqrt(2/pi), ftinternal.h:58 */
+        const double cst0inv = pow(2.0, -0.5 * (alpha + beta + 2.0));
+        const double cst1inv = cst0inv * 1.253314137315500251; /* sqrt(pi/2), ftinternal.h:57 */
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i <= j; i++) {
+                P->P[0][i + (size_t)j * n] *= cst0;
+                P->P[1][i + (size_t)j * n] *= cst1;
+                P->Pinv[0][i + (size_t)j * n] *= cst0inv;
+                P->Pinv[1][i + (size_t)j * n] *= cst1inv;
+            }
+    }
+    P->alpha = alpha;
+    P->beta = beta;
+    return plan_done(P);
 }
 
 /* Device copies of the connection matrices (caller holds the lock). */
@@ -115,11 +184,108 @@ static int reserve(void **buf, size_t *have, size_t bytes) {
     return rc;
 }
 
+/* The rotation executor kind of each transform. */
+static int rotation_kind(int transform) {
+    switch (transform) {
+    case FT_GPU_SPH2FOURIER:
+    case FT_GPU_FOURIER2SPH: return FT_GPU_SPH;
+    case FT_GPU_SPHV2FOURIER:
+    case FT_GPU_FOURIER2SPHV: return FT_GPU_SPHV;
+    case FT_GPU_DISK2CXF:
+    case FT_GPU_CXF2DISK: return FT_GPU_DISK;
+    case FT_GPU_TRI2CHEB:
+    case FT_GPU_CHEB2TRI: return FT_GPU_TRI;
+    }
+    return -1;
+}
+
 static int valid(int transform, char TRANS, const ft_harmonic_plan *P, int N, int M) {
-    if (!P || !P->dev || !P->RP || !P->RP[0] || transform < 0 || transform > 3 || (TRANS != 'N' && TRANS != 'T'))
-        return 0;
-    const int vec = transform >= FT_GPU_SPHV2FOURIER;
-    return N == P->RP[0]->n && M >= 1 && ftamd_valid_M(vec ? FT_GPU_SPHV : FT_GPU_SPH, N, M);
+    const int rk = rotation_kind(transform);
+    if (!P || !P->dev || !P->RP || !P->RP[0] || rk < 0 || (TRANS != 'N' && TRANS != 'T')) return 0;
+    /* the plan's rotation tables must be the transform's geometry */
+    const int geo = rk == FT_GPU_TRI ? FTAMD_TRI : rk == FT_GPU_DISK ? FTAMD_DISK : FTAMD_SPH;
+    return P->RP[0]->kind == geo && N == P->RP[0]->n && M >= 1 && ftamd_valid_M(rk, N, M);
+}
+
+/* One product into the work buffer and back into dA. */
+static int product(struct ftamd_harmonic_dev *d, FtamdTrmmLaunch *t, void *dA, size_t bytes, void *stream) {
+    t->X = dA;
+    t->Y = d->dY;
+    int rc = ftamd_hip_trmm_launch(t, stream);
+    if (rc == FTAMD_OK) rc = ftamd_hip_d2d(dA, d->dY, bytes, stream);
+    return rc;
+}
+
+/* The products of the column-class transforms (sph, sphv, disk). */
+static int class_products(struct ftamd_harmonic_dev *d, double *const *H, int vec, int trans, int n, int M, void *dA,
+                          size_t bytes, void *stream) {
+    FtamdTrmmLaunch t;
+    memset(&t, 0, sizeof(t));
+    for (int c = 0; c < 4; c++) {
+        const int even = (c == 0 || c == 3); /* columns of even-parity orders (sph and disk layouts) */
+        t.U[c] = H[even != vec ? 0 : 1];
+        const int count = M > c ? (M - c + 3) / 4 : 0;
+        t.ctiles[c] = (count + FTAMD_TRMM_COLS - 1) / FTAMD_TRMM_COLS;
+    }
+    t.n = n;
+    t.M = M;
+    t.cstride = 4;
+    t.ncls = 4;
+    t.trans = trans;
+    return product(d, &t, dA, bytes, stream);
+}
+
+/* One triangle product on every column (left) or every row (right). */
+static int tri_product(struct ftamd_harmonic_dev *d, const double *U, int right, int trans, int n, int M, void *dA,
+                       size_t bytes, void *stream) {
+    FtamdTrmmLaunch t;
+    memset(&t, 0, sizeof(t));
+    t.U[0] = U;
+    t.ctiles[0] = (M + FTAMD_TRMM_COLS - 1) / FTAMD_TRMM_COLS;
+    t.n = n;
+    t.M = M;
+    t.cstride = 1;
+    t.ncls = 1;
+    t.trans = trans;
+    t.right = right;
+    return product(d, &t, dA, bytes, stream);
+}
+
+/* The triangle's transforms (drivers.c:921-957): tri2cheb 'N' is hi2lo, P[0]
+ * from the left unless (alpha, beta + gamma) = (-1/2, -3/2), P[1]^T from the
+ * right unless beta = gamma = -1/2, then the Chebyshev normalization;
+ * cheb2tri inverts it. */
+static int run_tri(ft_harmonic_plan *P, int inverse, char TRANS, void *dA, int M, size_t bytes, void *stream) {
+    struct ftamd_harmonic_dev *d = P->dev;
+    const int n = P->RP[0]->n;
+    const int use0 = (P->beta + P->gamma != -1.5) || (P->alpha != -0.5);
+    const int use1 = (P->gamma != -0.5) || (P->beta != -0.5);
+    const double s1 = inverse ? 1.41421356237309504880 : 0.70710678118654752440; /* M_SQRT2, M_SQRT1_2 */
+    const double s2 = inverse ? 1.57079632679489661923 : 0.63661977236758134308; /* M_PI_2, M_2_PI */
+    double *const *H = inverse ? d->dPinv : d->dP;
+    int rc = FTAMD_OK;
+    if (!inverse && TRANS == 'N') {
+        rc = ft_gpu_execute(P->RP[0], FT_GPU_TRI, 0, FT_GPU_FP64, dA, M, stream);
+        if (rc == FTAMD_OK && use0) rc = tri_product(d, H[0], 0, 0, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK && use1) rc = tri_product(d, H[1], 1, 1, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ftamd_hip_cheb_scale(dA, n, M, s1, s2, stream);
+    } else if (!inverse) {
+        rc = ftamd_hip_cheb_scale(dA, n, M, s1, s2, stream);
+        if (rc == FTAMD_OK && use1) rc = tri_product(d, H[1], 1, 0, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK && use0) rc = tri_product(d, H[0], 0, 1, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ft_gpu_execute(P->RP[0], FT_GPU_TRI, 1, FT_GPU_FP64, dA, M, stream);
+    } else if (TRANS == 'N') {
+        rc = ftamd_hip_cheb_scale(dA, n, M, s1, s2, stream);
+        if (rc == FTAMD_OK && use1) rc = tri_product(d, H[1], 1, 1, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK && use0) rc = tri_product(d, H[0], 0, 0, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ft_gpu_execute(P->RP[0], FT_GPU_TRI, 1, FT_GPU_FP64, dA, M, stream);
+    } else {
+        rc = ft_gpu_execute(P->RP[0], FT_GPU_TRI, 0, FT_GPU_FP64, dA, M, stream);
+        if (rc == FTAMD_OK && use0) rc = tri_product(d, H[0], 0, 1, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK && use1) rc = tri_product(d, H[1], 1, 0, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ftamd_hip_cheb_scale(dA, n, M, s1, s2, stream);
+    }
+    return rc;
 }
 
 /* The transform on device matrix dA, asynchronously on `stream` (caller
@@ -127,33 +293,18 @@ static int valid(int transform, char TRANS, const ft_harmonic_plan *P, int N, in
 static int run(ft_harmonic_plan *P, int transform, char TRANS, void *dA, int M, void *stream) {
     struct ftamd_harmonic_dev *d = P->dev;
     const int n = P->RP[0]->n;
-    const int vec = transform >= FT_GPU_SPHV2FOURIER, inverse = transform & 1;
-    const int rot_kind = vec ? FT_GPU_SPHV : FT_GPU_SPH;
-    /* drivers.c:833-899: rotations first for sph2fourier 'N' and fourier2sph 'T' */
-    const int rot_first = inverse ? TRANS == 'T' : TRANS == 'N';
-    const int rot_dir = rot_first ? 0 : 1;
-    double *const *H = inverse ? d->dPinv : d->dP;
-    FtamdTrmmLaunch t;
-    memset(&t, 0, sizeof(t));
-    for (int c = 0; c < 4; c++) {
-        const int even = (c == 0 || c == 3); /* columns of even-parity orders (sph layout) */
-        t.U[c] = H[even != vec ? 0 : 1];
-        const int count = M > c ? (M - c + 3) / 4 : 0;
-        t.ctiles[c] = (count + FTAMD_TRMM_COLS - 1) / FTAMD_TRMM_COLS;
-    }
-    t.X = dA;
-    t.n = n;
-    t.M = M;
-    t.cstride = 4;
-    t.ncls = 4;
-    t.trans = TRANS == 'T';
+    const int rk = rotation_kind(transform), inverse = transform & 1;
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)M;
     int rc = reserve(&d->dY, &d->y_bytes, bytes);
-    t.Y = d->dY;
-    if (rc == FTAMD_OK && rot_first) rc = ft_gpu_execute(P->RP[0], rot_kind, rot_dir, FT_GPU_FP64, dA, M, stream);
-    if (rc == FTAMD_OK) rc = ftamd_hip_trmm_launch(&t, stream);
-    if (rc == FTAMD_OK) rc = ftamd_hip_d2d(dA, d->dY, bytes, stream);
-    if (rc == FTAMD_OK && !rot_first) rc = ft_gpu_execute(P->RP[0], rot_kind, rot_dir, FT_GPU_FP64, dA, M, stream);
+    if (rc != FTAMD_OK) return rc;
+    if (rk == FT_GPU_TRI) return run_tri(P, inverse, TRANS, dA, M, bytes, stream);
+    /* drivers.c:833-899, 987-1021: rotations first for the forward 'N' and the inverse 'T' */
+    const int rot_first = inverse ? TRANS == 'T' : TRANS == 'N';
+    const int rot_dir = rot_first ? 0 : 1;
+    if (rot_first) rc = ft_gpu_execute(P->RP[0], rk, rot_dir, FT_GPU_FP64, dA, M, stream);
+    if (rc == FTAMD_OK)
+        rc = class_products(d, inverse ? d->dPinv : d->dP, rk == FT_GPU_SPHV, TRANS == 'T', n, M, dA, bytes, stream);
+    if (rc == FTAMD_OK && !rot_first) rc = ft_gpu_execute(P->RP[0], rk, rot_dir, FT_GPU_FP64, dA, M, stream);
     return rc;
 }
 
@@ -216,4 +367,20 @@ void ft_execute_sphv2fourier(const char TRANS, const ft_harmonic_plan *P, double
 /* drivers.c:884-899 */
 void ft_execute_fourier2sphv(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
     harmonic(FT_GPU_FOURIER2SPHV, TRANS, P, A, N, M);
+}
+/* drivers.c:921-938 */
+void ft_execute_tri2cheb(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_TRI2CHEB, TRANS, P, A, N, M);
+}
+/* drivers.c:940-957 */
+void ft_execute_cheb2tri(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_CHEB2TRI, TRANS, P, A, N, M);
+}
+/* drivers.c:987-1003 */
+void ft_execute_disk2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_DISK2CXF, TRANS, P, A, N, M);
+}
+/* drivers.c:1005-1021 */
+void ft_execute_cxf2disk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_CXF2DISK, TRANS, P, A, N, M);
 }

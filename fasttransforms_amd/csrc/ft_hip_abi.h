@@ -161,11 +161,13 @@ typedef struct {
 
 int ftamd_hip_sys_launch(const FtamdSysLaunch *a, int precision, int S, void *stream);
 
-/* Strided triangular products of the harmonic transforms (drivers.c:833-864):
- * for each class c < ncls, the columns j = c, c + cstride, ... < M of the
- * n x M column-major X are multiplied by op(U[c]) (U[c] upper triangular,
- * n x n column-major; op = identity, or the transpose when trans) into the
- * same columns of Y (n x M, another buffer); U[c] == NULL leaves class c out.
+/* Triangular products of the harmonic transforms (drivers.c:833-1021), out
+ * of place from X (n x M, column-major) into Y (same shape, another buffer).
+ * Left (right == 0): for each class c < ncls, the columns j = c,
+ * c + cstride, ... < M are multiplied by op(U[c]) (n x n upper triangular,
+ * leading dimension n; op = identity, or the transpose when trans);
+ * U[c] == NULL leaves class c out.  Right (right == 1, ncls 1, cstride 1):
+ * Y = X op(U[0]) with U[0]'s leading M x M block (leading dimension n).
  * Output tiles of FTAMD_TRMM_ROWS x FTAMD_TRMM_COLS; ctiles[c] column tiles
  * per class. */
 #define FTAMD_TRMM_ROWS 64
@@ -174,10 +176,13 @@ typedef struct {
     const double *U[4];
     const double *X;
     double *Y;
-    int n, M, cstride, ncls, trans;
+    int n, M, cstride, ncls, trans, right;
     int ctiles[4];
 } FtamdTrmmLaunch;
 int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream);
+/* A[i + j n] *= (j == 0 ? s1 : 1) * (i == 0 ? s1 : 1) * s2, in that order,
+ * for the n x M matrix A (the 2D Chebyshev normalization, drivers.c:6-24). */
+int ftamd_hip_cheb_scale(double *A, int n, int M, double s1, double s2, void *stream);
 int ftamd_hip_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
 /* The allocation-free entry point's device scratch: one fixed array per

@@ -73,6 +73,9 @@ const void *ftamd_plan_table(const ft_rotation_plan *RP, int precision);
 double *ftamd_plan_leg2cheb(int normleg, int normcheb, int n);
 double *ftamd_plan_cheb2leg(int normcheb, int normleg, int n);
 double *ftamd_plan_ultra2ultra(int norm1, int norm2, int n, double lambda, double mu);
+double *ftamd_plan_jac2jac(int norm1, int norm2, int n, double alpha, double beta, double gamma, double delta);
+double *ftamd_plan_jac2cheb(int normjac, int normcheb, int n, double alpha, double beta);
+double *ftamd_plan_cheb2jac(int normcheb, int normjac, int n, double alpha, double beta);
 
 /* ft_drivers.c */
 int ftamd_valid_M(int kind, int n, int M); /* widths a whole-matrix executor of public kind `kind` accepts */

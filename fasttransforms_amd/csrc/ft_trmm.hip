@@ -1,18 +1,19 @@
-// ft_trmm.hip -- the strided upper-triangular products of the harmonic
-// transforms (drivers.c:833-864: four cblas_dtrmm calls per transform, one
-// per class of columns j = c mod 4) as one launch on the FP64 matrix cores.
+// ft_trmm.hip -- the upper-triangular products of the harmonic transforms
+// (drivers.c:833-1021: cblas_dtrmm from the left on the column classes
+// j = c mod 4, or on every column, and from the right on every row) as one
+// launch each on the FP64 matrix cores.
 //
-// Y[:, j] = op(U_c) X[:, j] for every column j of every class c, out of
-// place.  A workgroup (4 waves) computes a 64 x 32 tile of Y: wave w the rows
-// 16w .. 16w+15 as two 16 x 16 blocks, accumulated by
-// v_mfma_f64_16x16x4_f64 over k in stages of 16 staged through LDS (op(U)
-// 64 x 16, X 16 x 32), the next stage's global loads issued before the
-// current stage's products.  Lane layout of the 16x16x4 FP64 MFMA: A
-// operand lane i = A[i % 16][i / 16], B operand lane i = B[i / 16][i % 16],
-// result register j of lane i = D[i / 16 + 4 j][i % 16].  Only the
-// triangle's k range is visited: k >= first row of the tile for U, k <= last
-// row for U^T.  The products sum in a different order from the reference's
-// BLAS, so results agree to rounding, not bit for bit.
+// Left: Y[:, j] = op(U_c) X[:, j] for every column j of every class c.
+// Right: Y = X op(U) with U's leading M x M block.  Out of place.  A
+// workgroup (4 waves) computes a 64 x 32 tile of Y: wave w the rows 16w ..
+// 16w+15 as two 16 x 16 blocks, accumulated by v_mfma_f64_16x16x4_f64 over k
+// in stages of 16 staged through LDS (the A operand 64 x 16, the B operand
+// 16 x 32), the next stage's global loads issued before the current stage's
+// products.  Lane layout of the 16x16x4 FP64 MFMA: A operand lane i =
+// A[i % 16][i / 16], B operand lane i = B[i / 16][i % 16], result register
+// j of lane i = D[i / 16 + 4 j][i % 16].  Only the triangle's k range is
+// visited.  The products sum in another order than the reference's BLAS, so
+// results agree to rounding, not bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "ft_hip_abi.h"
@@ -25,12 +26,13 @@ constexpr int TR = FTAMD_TRMM_ROWS;  // output rows per workgroup
 constexpr int TC = FTAMD_TRMM_COLS;  // output columns per workgroup
 constexpr int TK = 16;               // k per stage
 constexpr int PAD = 2;               // LDS row padding (doubles)
-constexpr int NU = TR * TK / 256;    // op(U) elements each thread stages
-constexpr int NX = TK * TC / 256;    // X elements each thread stages
+constexpr int NA = TR * TK / 256;    // A-operand elements each thread stages
+constexpr int NB = TK * TC / 256;    // B-operand elements each thread stages
 
+template <bool RIGHT>
 __global__ void __launch_bounds__(256) trmm_kernel(FtamdTrmmLaunch a) {
-    __shared__ double sU[TK][TR + PAD];  // sU[k][r] = op(U)[r0 + r][k0 + k]
-    __shared__ double sX[TK][TC + PAD];  // sX[k][q] = X[k0 + k][column q of the tile]
+    __shared__ double sA[TK][TR + PAD];  // sA[k][r] = A operand (r0 + r, k0 + k)
+    __shared__ double sB[TK][TC + PAD];  // sB[k][q] = B operand (k0 + k, column q of the tile)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = a.n;
     // class and column tile of this workgroup
@@ -38,69 +40,101 @@ __global__ void __launch_bounds__(256) trmm_kernel(FtamdTrmmLaunch a) {
     while (c + 1 < a.ncls && t >= a.ctiles[c]) t -= a.ctiles[c++];
     const double *U = a.U[c];
     if (U == nullptr || t >= a.ctiles[c]) return;
-    const int count = (a.M - c + a.cstride - 1) / a.cstride;  // columns of class c
+    const int count = (a.M - c + a.cstride - 1) / a.cstride;  // output columns of the class
     const int q0 = t * TC;                                    // first class-local column of the tile
     const int r0 = blockIdx.y * TR;
     const bool trans = a.trans != 0;
-    const int kbeg = trans ? 0 : r0;
-    const int kend = trans ? min(n, r0 + TR) : n;
+    // k ranges of the triangle: left, rows of op(U) from r0 (U) or up to the
+    // tile's last row (U^T); right, columns of op(U) up to the tile's last
+    // column (U) or from q0 (U^T)
+    const int K = RIGHT ? a.M : n;
+    int kbeg, kend;
+    if (!RIGHT) {
+        kbeg = trans ? 0 : r0;
+        kend = trans ? min(n, r0 + TR) : n;
+    } else {
+        kbeg = trans ? q0 : 0;
+        kend = trans ? K : min(K, q0 + TC);
+    }
+    const long long xcol = a.cstride;  // X column of class-local column q: c + q * cstride
 
-    // staging: element e of op(U)'s 64 x 16 block and of X's 16 x 32 block
-    auto load_u = [&](int k0, double (&v)[NU]) {
+    // element e of the A operand's 64 x 16 block: consecutive threads along
+    // the contiguous direction of the source
+    auto a_rk = [&](int e, int &r, int &k) {
+        const bool along_k = !RIGHT && trans;  // U^T: a column of U is a row of the operand
+        r = along_k ? e / TK : e % TR;
+        k = along_k ? e % TK : e / TR;
+    };
+    auto load_a = [&](int k0, double (&v)[NA]) {
 #pragma unroll
-        for (int s = 0; s < NU; ++s) {
-            const int e = tid + 256 * s;
-            // U: consecutive threads take consecutive rows r (a column of U);
-            // U^T: consecutive k (a column of U, read as a row of U^T)
-            const int r = trans ? e / TK : e % TR, k = trans ? e % TK : e / TR;
+        for (int s = 0; s < NA; ++s) {
+            int r, k;
+            a_rk(tid + 256 * s, r, k);
             const int gr = r0 + r, gk = k0 + k;
-            v[s] = (gr < n && gk < n) ? (trans ? U[gk + (long long)gr * n] : U[gr + (long long)gk * n]) : 0.0;
+            double x = 0.0;
+            if (gr < n && gk < K) {
+                if (RIGHT) x = a.X[gr + (long long)gk * n];                  // X[gr][gk]
+                else x = trans ? U[gk + (long long)gr * n] : U[gr + (long long)gk * n];  // op(U)[gr][gk]
+            }
+            v[s] = x;
         }
     };
-    auto load_x = [&](int k0, double (&v)[NX]) {
+    // element e of the B operand's 16 x 32 block
+    auto b_kq = [&](int e, int &k, int &q) {
+        const bool along_q = RIGHT && trans;  // U^T from the right: op(U)[k][q] = U[q][k], contiguous in q
+        k = along_q ? e / TC : e % TK;
+        q = along_q ? e % TC : e / TK;
+    };
+    auto load_b = [&](int k0, double (&v)[NB]) {
 #pragma unroll
-        for (int s = 0; s < NX; ++s) {
-            const int e = tid + 256 * s;
-            const int k = e % TK, q = e / TK;
+        for (int s = 0; s < NB; ++s) {
+            int k, q;
+            b_kq(tid + 256 * s, k, q);
             const int gk = k0 + k, gq = q0 + q;
-            v[s] = (gk < n && gq < count) ? a.X[gk + (long long)(c + gq * a.cstride) * n] : 0.0;
+            double x = 0.0;
+            if (gk < K && gq < count) {
+                if (RIGHT) x = trans ? U[gq + (long long)gk * n] : U[gk + (long long)gq * n];  // op(U)[gk][gq]
+                else x = a.X[gk + (c + gq * xcol) * n];                                        // X[gk][column]
+            }
+            v[s] = x;
         }
     };
-    auto store = [&](const double (&u)[NU], const double (&x)[NX]) {
+    auto store = [&](const double (&va)[NA], const double (&vb)[NB]) {
 #pragma unroll
-        for (int s = 0; s < NU; ++s) {
-            const int e = tid + 256 * s;
-            const int r = trans ? e / TK : e % TR, k = trans ? e % TK : e / TR;
-            sU[k][r] = u[s];
+        for (int s = 0; s < NA; ++s) {
+            int r, k;
+            a_rk(tid + 256 * s, r, k);
+            sA[k][r] = va[s];
         }
 #pragma unroll
-        for (int s = 0; s < NX; ++s) {
-            const int e = tid + 256 * s;
-            sX[e % TK][e / TK] = x[s];
+        for (int s = 0; s < NB; ++s) {
+            int k, q;
+            b_kq(tid + 256 * s, k, q);
+            sB[k][q] = vb[s];
         }
     };
 
     v4d acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-    double u[NU], x[NX];
+    double va[NA], vb[NB];
     const int k_first = kbeg / TK * TK;
     if (k_first < kend) {
-        load_u(k_first, u);
-        load_x(k_first, x);
+        load_a(k_first, va);
+        load_b(k_first, vb);
     }
     for (int k0 = k_first; k0 < kend; k0 += TK) {
-        store(u, x);
+        store(va, vb);
         __syncthreads();
         // the next stage's loads are in flight during this stage's products
         if (k0 + TK < kend) {
-            load_u(k0 + TK, u);
-            load_x(k0 + TK, x);
+            load_a(k0 + TK, va);
+            load_b(k0 + TK, vb);
         }
 #pragma unroll
         for (int ks = 0; ks < TK; ks += 4) {
             const int kk = ks + (lane >> 4), ii = lane & 15;
-            const double av = sU[kk][16 * w + ii];
-            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sX[kk][ii], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sX[kk][16 + ii], acc1, 0, 0, 0);
+            const double av = sA[kk][16 * w + ii];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sB[kk][ii], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sB[kk][16 + ii], acc1, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -109,8 +143,20 @@ __global__ void __launch_bounds__(256) trmm_kernel(FtamdTrmmLaunch a) {
         const int r = r0 + 16 * w + (lane >> 4) + 4 * j;
         if (r >= n) continue;
         const int qa = q0 + (lane & 15), qb = qa + 16;
-        if (qa < count) a.Y[r + (long long)(c + qa * a.cstride) * n] = acc0[j];
-        if (qb < count) a.Y[r + (long long)(c + qb * a.cstride) * n] = acc1[j];
+        if (qa < count) a.Y[r + (c + qa * xcol) * n] = acc0[j];
+        if (qb < count) a.Y[r + (c + qb * xcol) * n] = acc1[j];
+    }
+}
+
+__global__ void cheb_scale_kernel(double *A, int n, int M, double s1, double s2) {
+    const long long total = (long long)n * M;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % n), j = (int)(e / n);
+        double v = A[e];
+        if (j == 0) v *= s1;
+        if (i == 0) v *= s1;
+        A[e] = v * s2;
     }
 }
 
@@ -118,17 +164,28 @@ __global__ void __launch_bounds__(256) trmm_kernel(FtamdTrmmLaunch a) {
 
 extern "C" int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream) {
     if (a->n < 1 || a->M < 0 || a->cstride < 1 || a->ncls < 1 || a->ncls > 4 || !a->X || !a->Y) return FTAMD_EINVAL;
+    if (a->right && (a->ncls != 1 || a->cstride != 1 || a->M > a->n)) return FTAMD_EINVAL;
     int tiles = 0;
     for (int c = 0; c < a->ncls; ++c) {
         const int count = a->M > c ? (a->M - c + a->cstride - 1) / a->cstride : 0;
         if (a->ctiles[c] != (count + FTAMD_TRMM_COLS - 1) / FTAMD_TRMM_COLS) return FTAMD_EINVAL;
         tiles += a->ctiles[c];
     }
-    // element offsets of Y and X fit the kernel's 64-bit index math; the
-    // grid's second dimension covers the rows
     const int rows = (a->n + FTAMD_TRMM_ROWS - 1) / FTAMD_TRMM_ROWS;
     if (tiles == 0) return FTAMD_OK;
-    hipLaunchKernelGGL(fttrmm::trmm_kernel, dim3(tiles, rows), dim3(256), 0, (hipStream_t)stream, *a);
+    if (a->right) hipLaunchKernelGGL(fttrmm::trmm_kernel<true>, dim3(tiles, rows), dim3(256), 0, (hipStream_t)stream, *a);
+    else hipLaunchKernelGGL(fttrmm::trmm_kernel<false>, dim3(tiles, rows), dim3(256), 0, (hipStream_t)stream, *a);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FTAMD_OK : (e == hipErrorOutOfMemory ? FTAMD_ENOMEM : FTAMD_EDEVICE);
+}
+
+extern "C" int ftamd_hip_cheb_scale(double *A, int n, int M, double s1, double s2, void *stream) {
+    if (n < 1 || M < 0 || !A) return FTAMD_EINVAL;
+    if (M == 0) return FTAMD_OK;
+    const long long total = (long long)n * M;
+    const long long want = (total + 255) / 256;
+    const int blocks = (int)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(fttrmm::cheb_scale_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, A, n, M, s1, s2);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }

@@ -145,6 +145,26 @@ void ft_execute_fourier2sph(const char TRANS, const ft_harmonic_plan *P, double 
 void ft_execute_sphv2fourier(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 void ft_execute_fourier2sphv(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 
+/* fasttransforms.h:402-407 (drivers.c:901-957) -- Proriol <-> bivariate
+ * Chebyshev: the triangle's rotation plan, P = {P^{(beta+gamma+1, alpha)},
+ * P^{(gamma, beta)} -> Chebyshev} (normalized Jacobi connections) and their
+ * inverses.  A is N x M (M <= N); tri2cheb 'N' is hi2lo, P[0] from the left
+ * on every column, P[1]^T from the right on every row (each skipped where it
+ * is the identity, as the reference does) and the 2D Chebyshev
+ * normalization. */
+ft_harmonic_plan *ft_plan_tri2cheb(const int n, const double alpha, const double beta, const double gamma);
+void ft_execute_tri2cheb(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_cheb2tri(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+
+/* fasttransforms.h:410-415 (drivers.c:959-1021) -- generalized Zernike <->
+ * Chebyshev x Fourier: the disk's rotation plan, P[0] = Jacobi(beta, alpha)
+ * -> Chebyshev and P[1] = Jacobi(beta, alpha+1) -> Jacobi(-1/2, 1/2), scaled
+ * as the reference does, and their inverses; the column classes as for
+ * sph2fourier.  A is N x M (M odd, M <= 4N-3). */
+ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double beta);
+void ft_execute_disk2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_cxf2disk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+
 /* ------------------------------------------------------------------------
  * Extensions (no counterpart in the reference).
  * ---------------------------------------------------------------------- */
@@ -205,12 +225,21 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, 
                            void *stream);
 
 /* The harmonic transforms on a device-resident A (dA, N x M FP64) on a HIP
- * stream, asynchronously: transform FT_GPU_SPH2FOURIER, _FOURIER2SPH,
- * _SPHV2FOURIER or _FOURIER2SPHV, TRANS 'N' or 'T', as the executors above.
+ * stream, asynchronously: transform FT_GPU_SPH2FOURIER .. FT_GPU_CHEB2TRI,
+ * TRANS 'N' or 'T', as the executors above.
  * Errors of the launches come back through ft_gpu_synchronize.  Calls on
  * one harmonic plan share its device work buffer: order them on one
  * stream. */
-enum { FT_GPU_SPH2FOURIER = 0, FT_GPU_FOURIER2SPH = 1, FT_GPU_SPHV2FOURIER = 2, FT_GPU_FOURIER2SPHV = 3 };
+enum {
+    FT_GPU_SPH2FOURIER = 0,
+    FT_GPU_FOURIER2SPH = 1,
+    FT_GPU_SPHV2FOURIER = 2,
+    FT_GPU_FOURIER2SPHV = 3,
+    FT_GPU_DISK2CXF = 4,
+    FT_GPU_CXF2DISK = 5,
+    FT_GPU_TRI2CHEB = 6,
+    FT_GPU_CHEB2TRI = 7
+};
 int ft_gpu_execute_harmonic(int transform, char TRANS, const ft_harmonic_plan *P, void *dA, int N, int M,
                             void *stream);
 

@@ -220,16 +220,64 @@ def connection(kind, flag1, flag2, n, lam=0.0, mu=0.0):
     return V
 
 
-def harmonic(transform, trans, plan, P, Pinv, A, M=None):
-    """The harmonic transforms of drivers.c:833-899 on the n x M matrix A (in
-    place): transform "sph2fourier", "fourier2sph", "sphv2fourier" or
-    "fourier2sphv", trans "N" or "T"; plan the sphere's rotation Plan, P and
-    Pinv the two connection matrices and their inverses.  The rotations are
-    the oracle's; each class's upper-triangular product is evaluated in long
-    double and rounded once."""
+def _tri_harmonic(transform, trans, plan, H, A, M, params):
+    """tri2cheb / cheb2tri (drivers.c:921-957): left and right triangular
+    products in long double, the 2D Chebyshev normalization (drivers.c:6-24)."""
+    alpha, beta, gamma = params
+    use0 = (beta + gamma != -1.5) or (alpha != -0.5)
+    use1 = (gamma != -0.5) or (beta != -0.5)
+    inverse = transform == "cheb2tri"
+
+    def left(U, t):
+        U = np.triu(U).astype(np.longdouble)
+        A[:, :M] = ((U.T if t else U) @ A[:, :M].astype(np.longdouble)).astype(np.float64)
+
+    def right(U, t):
+        U = np.triu(U[:M, :M]).astype(np.longdouble)
+        A[:, :M] = (A[:, :M].astype(np.longdouble) @ (U.T if t else U)).astype(np.float64)
+
+    def normalize():
+        s1, s2 = (np.sqrt(2.0), np.pi / 2) if inverse else (np.sqrt(0.5), 2 / np.pi)
+        A[:, 0] *= s1
+        A[0, :M] *= s1
+        A[:, :M] *= s2
+
+    if not inverse and trans == "N":
+        execute(plan, HI2LO, A, M)
+        if use0: left(H[0], False)
+        if use1: right(H[1], True)
+        normalize()
+    elif not inverse:
+        normalize()
+        if use1: right(H[1], False)
+        if use0: left(H[0], True)
+        execute(plan, LO2HI, A, M)
+    elif trans == "N":
+        normalize()
+        if use1: right(H[1], True)
+        if use0: left(H[0], False)
+        execute(plan, LO2HI, A, M)
+    else:
+        execute(plan, HI2LO, A, M)
+        if use0: left(H[0], True)
+        if use1: right(H[1], False)
+        normalize()
+    return A
+
+
+def harmonic(transform, trans, plan, P, Pinv, A, M=None, params=None):
+    """The harmonic transforms of drivers.c:833-1021 on the n x M matrix A (in
+    place): transform "sph2fourier", "fourier2sph", "sphv2fourier",
+    "fourier2sphv" (plan: the sphere's rotation Plan), "disk2cxf", "cxf2disk"
+    (the disk's) or "tri2cheb", "cheb2tri" (the triangle's, params its
+    (alpha, beta, gamma)); trans "N" or "T"; P and Pinv the two connection
+    matrices and their inverses.  The rotations are the oracle's; each
+    triangular product is evaluated in long double and rounded once."""
     M = A.shape[1] if M is None else M
+    if transform in ("tri2cheb", "cheb2tri"):
+        return _tri_harmonic(transform, trans, plan, Pinv if transform == "cheb2tri" else P, A, M, params)
     vec = transform.startswith("sphv") or transform.endswith("sphv")
-    inverse = transform.startswith("fourier")
+    inverse = transform.startswith("fourier") or transform == "cxf2disk"
     H = Pinv if inverse else P
     rot_first = (trans == "T") if inverse else (trans == "N")
     run = execute_sphv if vec else execute

@@ -74,3 +74,49 @@ def test_round_trip_1024_device_resident(ft):
     e2, einf = O.rel2(X, A), O.relinf(X, A)
     _say("round trip n=1024 rel2 %.2e relinf %.2e" % (e2, einf))
     assert e2 < 4 * np.sqrt(n) * 2.0 ** -52 and einf < 2 * n * 2.0 ** -52
+
+
+TRI_DISK = [("tri2cheb", (0.0, 0.0, 0.0)), ("cheb2tri", (0.0, 0.0, 0.0)), ("tri2cheb", (-0.5, -0.5, -0.5)),
+            ("cheb2tri", (0.5, 1.0, -0.25)), ("disk2cxf", (0.0, 0.0)), ("cxf2disk", (0.0, 0.0)),
+            ("disk2cxf", (0.5, 1.5)), ("cxf2disk", (-0.5, 0.25))]
+
+
+def _tri_disk(ft, transform, params, n):
+    tri = transform in ("tri2cheb", "cheb2tri")
+    H = ft.plan_tri2cheb(n, *params) if tri else ft.plan_disk2cxf(n, *params)
+    Q = O.Plan("tri" if tri else "disk", n, *params)
+    return H, Q, (H.P(0), H.P(1)), (H.Pinv(0), H.Pinv(1)), (n if tri else 4 * n - 3)
+
+
+@pytest.mark.parametrize("transform,params", TRI_DISK)
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_tri_disk_transforms(ft, transform, params, trans):
+    """tri2cheb / cheb2tri (left and right products, the 2D Chebyshev
+    normalization) and disk2cxf / cxf2disk against the oracle composite with
+    the plan's own connection matrices (pinned on the CPU)."""
+    for n in (1, 2, 5, 64, 129, 300):
+        H, Q, P, Pinv, full = _tri_disk(ft, transform, params, n)
+        tri = transform in ("tri2cheb", "cheb2tri")
+        for M in sorted({m for m in (1, 3, 37, full) if m <= full and (tri or m % 2 == 1)}):
+            A = np.asfortranarray(np.random.default_rng(n + M).uniform(-1, 1, (n, M)))
+            _say("harmonic", transform, params, trans, "n=%d M=%d" % (n, M))
+            G = getattr(ft, "execute_" + transform)(trans, H, A.copy(order="F"))
+            R = O.harmonic(transform, trans, Q, P, Pinv, A.copy(order="F"), params=params)
+            assert O.rel2(G, R) <= 1e-14, (transform, params, trans, n, M, O.rel2(G, R))
+
+
+@pytest.mark.parametrize("transform,params", [("tri2cheb", (0.0, 0.0, 0.0)), ("disk2cxf", (0.0, 0.0))])
+def test_tri_disk_at_1024_and_round_trip(ft, transform, params):
+    n = 1024
+    H, Q, P, Pinv, M = _tri_disk(ft, transform, params, n)
+    inv = {"tri2cheb": "cheb2tri", "disk2cxf": "cxf2disk"}[transform]
+    A = O.fill("tri" if transform == "tri2cheb" else "disk", n, M, 17)
+    G = getattr(ft, "execute_" + transform)("N", H, A.copy(order="F"))
+    R = O.harmonic(transform, "N", Q, P, Pinv, A.copy(order="F"), params=params)
+    err = O.rel2(G, R)
+    X = getattr(ft, "execute_" + inv)("N", H, G.copy(order="F"))
+    _say("%s n=1024 rel2 vs oracle %.2e, round trip rel2 %.2e" % (transform, err, O.rel2(X, A)))
+    assert err <= 1e-14
+    # the reference's published round-trip errors at degree 1023 are 3.6e-14
+    # (tri2cheb) and 2.6e-15 (disk2cxf) (README.md:64, 75)
+    assert O.rel2(X, A) < (1e-13 if transform == "tri2cheb" else 1e-14)

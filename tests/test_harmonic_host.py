@@ -9,8 +9,11 @@ the textbook normalizations (orthonormal Legendre sqrt(l + 1/2); orthonormal
 Chebyshev 1/sqrt(pi), sqrt(2/pi); orthonormal Gegenbauer 1/sqrt(h_l),
 h_l = pi 2^(1-2 lambda) Gamma(l + 2 lambda) / (l! (l + lambda) Gamma(lambda)^2)).
 Inverse pairs multiply to the identity.  The library's plan matrices equal
-the oracle's bit for bit, and the composite oracle inverts itself.
+the oracle's bit for bit, and the composite oracle inverts itself.  The
+Jacobi connections of tri2cheb and disk2cxf (the library's host code) are
+pinned against scipy's Jacobi polynomials evaluated at Chebyshev points.
 """
+import ctypes
 import math
 from fractions import Fraction
 
@@ -160,3 +163,99 @@ def test_harmonic_plan_rejects(ft):
     with pytest.raises(ft.GPUError):
         ft.execute_sph2fourier("N", H, np.zeros((9, 15), order="F"), N=9)  # N != plan's n
     assert ft.last_error() == 0
+
+
+# ----------------------------------------------------------------- Jacobi ----
+
+def _jacobi_norm(l, a, b):
+    """sqrt(h_l) of P_l^{(a,b)} under (1-x)^a (1+x)^b."""
+    if l == 0:
+        lg = (a + b + 1) * math.log(2) + math.lgamma(a + 1) + math.lgamma(b + 1) - math.lgamma(a + b + 2)
+    else:
+        lg = (a + b + 1) * math.log(2) + math.lgamma(l + a + 1) + math.lgamma(l + b + 1) - \
+            math.log(2 * l + a + b + 1) - math.lgamma(l + a + b + 1) - math.lgamma(l + 1)
+    return math.exp(0.5 * lg)
+
+
+def _jacobi_connection_by_evaluation(n, a, b, c, d, norm1, norm2):
+    """V with P_l^{(a,b)} = sum_k V[k, l] P_k^{(c,d)} (normalized where asked),
+    from scipy's Jacobi polynomials at n Chebyshev points (a square solve)."""
+    from scipy.special import eval_jacobi
+    x = np.cos(np.pi * (np.arange(n) + 0.5) / n)
+    src = np.array([eval_jacobi(l, a, b, x) / (_jacobi_norm(l, a, b) if norm1 else 1) for l in range(n)]).T
+    dst = np.array([eval_jacobi(k, c, d, x) / (_jacobi_norm(k, c, d) if norm2 else 1) for k in range(n)]).T
+    return np.linalg.solve(dst, src)
+
+
+def _lib_matrix(ft, name, n, *args):
+    fn = getattr(ft._lib, name)
+    fn.restype = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.c_int] * (len(args) - sum(isinstance(v, float) for v in args)) + \
+        [ctypes.c_double] * sum(isinstance(v, float) for v in args)
+    p = fn(*args)
+    V = np.ctypeslib.as_array(p, shape=(n * n,)).copy().reshape(n, n, order="F")
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(ctypes.cast(p, ctypes.c_void_p))
+    return V
+
+
+@pytest.mark.parametrize("a,b,c,d", [(0.0, 0.0, -0.5, -0.5), (1.5, 0.0, -0.5, -0.5), (0.5, -0.25, -0.5, 0.5),
+                                     (-0.5, -0.5, 1.0, 0.0), (-0.5, 0.5, 0.0, 1.0), (2.0, 1.0, 0.5, 0.25)])
+@pytest.mark.parametrize("norm1,norm2", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_jacobi_connection_against_evaluation(ft, a, b, c, d, norm1, norm2):
+    """The library's Jacobi -> Jacobi matrix (transforms.c:95-108) against
+    scipy's Jacobi polynomials at n Chebyshev points; it is upper triangular."""
+    n = 24
+    V = _lib_matrix(ft, "ftamd_plan_jac2jac", n, norm1, norm2, n, a, b, c, d)
+    W = _jacobi_connection_by_evaluation(n, a, b, c, d, norm1, norm2)
+    assert np.array_equal(V, np.triu(V))
+    assert np.max(np.abs(V - W)) <= 1e-11 * max(1.0, np.max(np.abs(W)))
+    Vi = _lib_matrix(ft, "ftamd_plan_jac2jac", n, norm2, norm1, n, c, d, a, b)
+    assert np.max(np.abs(Vi @ V - np.eye(n))) < 1e-12
+
+
+def test_jacobi_chebyshev_scalings(ft):
+    """transforms.c:157-183: the unnormalized Chebyshev side scales rows (or
+    columns) of the normalized connection by sqrt(1/pi), sqrt(2/pi)."""
+    n = 20
+    V = _lib_matrix(ft, "ftamd_plan_jac2jac", n, 1, 1, n, 0.5, -0.25, -0.5, -0.5)
+    C = _lib_matrix(ft, "ftamd_plan_jac2cheb", n, 1, 0, n, 0.5, -0.25)
+    row = np.array([math.sqrt(1 / math.pi)] + [math.sqrt(2 / math.pi)] * (n - 1))
+    assert np.max(np.abs(C - V * row[:, None])) <= 2 * EPS * np.max(np.abs(C))  # the constants' last bit
+    Vi = _lib_matrix(ft, "ftamd_plan_jac2jac", n, 1, 1, n, -0.5, -0.5, 0.5, -0.25)
+    Ci = _lib_matrix(ft, "ftamd_plan_cheb2jac", n, 0, 1, n, 0.5, -0.25)
+    col = np.array([math.sqrt(math.pi)] + [math.sqrt(math.pi / 2)] * (n - 1))
+    assert np.max(np.abs(Ci - Vi * col[None, :])) <= 2 * EPS * np.max(np.abs(Ci))
+    assert np.max(np.abs(Ci @ C - np.eye(n))) < 1e-12
+
+
+@pytest.mark.parametrize("which,params", [("tri", (0.0, 0.0, 0.0)), ("tri", (-0.5, -0.5, -0.5)), ("tri", (0.5, 1.0, -0.25)),
+                                          ("disk", (0.0, 0.0)), ("disk", (0.5, 1.5))])
+def test_tri_disk_plans_and_oracle_round_trip(ft, which, params):
+    """The plans hold the reference's connection matrices (drivers.c:901-985);
+    the composite oracle inverts itself and 'T' is the transpose of 'N'."""
+    n = 40
+    if which == "tri":
+        H = ft.plan_tri2cheb(n, *params)
+        a, b, g = params
+        assert np.array_equal(H.P(0), _lib_matrix(ft, "ftamd_plan_jac2jac", n, 1, 1, n, b + g + 1.0, a, -0.5, -0.5))
+        assert np.array_equal(H.Pinv(1), _lib_matrix(ft, "ftamd_plan_jac2jac", n, 1, 1, n, -0.5, -0.5, g, b))
+        Q, M, fwd, inv = O.Plan("tri", n, *params), n, "tri2cheb", "cheb2tri"
+    else:
+        H = ft.plan_disk2cxf(n, *params)
+        a, b = params
+        cst0 = 2.0 ** (0.5 * (a + b + 2.0))
+        assert np.array_equal(H.P(0), np.triu(_lib_matrix(ft, "ftamd_plan_jac2cheb", n, 1, 0, n, b, a)) * cst0)
+        Q, M, fwd, inv = O.Plan("disk", n, *params), 4 * n - 3, "disk2cxf", "cxf2disk"
+    P, Pinv = (H.P(0), H.P(1)), (H.Pinv(0), H.Pinv(1))
+    A = O.fill(which, n, M, 3)
+    X = O.harmonic(inv, "N", Q, P, Pinv, O.harmonic(fwd, "N", Q, P, Pinv, A.copy(order="F"), params=params),
+                   params=params)
+    assert O.rel2(X, A) < 1e-13
+    rng = np.random.default_rng(2)
+    x = np.asfortranarray(rng.standard_normal((n, M)))
+    y = np.asfortranarray(rng.standard_normal((n, M)))
+    Fx = O.harmonic(fwd, "N", Q, P, Pinv, x.copy(order="F"), params=params)
+    Fty = O.harmonic(fwd, "T", Q, P, Pinv, y.copy(order="F"), params=params)
+    assert abs(np.sum(y * Fx) - np.sum(Fty * x)) < 1e-12 * np.sqrt(np.sum(y * y) * np.sum(x * x)) * n
