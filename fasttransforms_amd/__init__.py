@@ -77,11 +77,12 @@ SIGNATURES = {
     "ft_plan_sph2fourier": (_vp, [_i]),
     "ft_plan_tri2cheb": (_vp, [_i, _d, _d, _d]),
     "ft_plan_disk2cxf": (_vp, [_i, _d, _d]),
+    "ft_plan_rectdisk2cheb": (_vp, [_i, _d]),
     "ft_destroy_harmonic_plan": (None, [_vp]),
     "ft_gpu_execute_harmonic": (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp]),
 }
 for _t in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv", "tri2cheb", "cheb2tri", "disk2cxf",
-           "cxf2disk"):
+           "cxf2disk", "rectdisk2cheb", "cheb2rectdisk"):
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 for _dn in ("hi2lo", "lo2hi"):
     for _k in ("sph", "tri", "disk", "rectdisk"):
@@ -263,10 +264,11 @@ class _HarmonicHead(ctypes.Structure):
                 ("dev", ctypes.c_void_p)]
 
 
-SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV, DISK2CXF, CXF2DISK, TRI2CHEB, CHEB2TRI = range(8)
+(SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV, DISK2CXF, CXF2DISK, TRI2CHEB, CHEB2TRI, RECTDISK2CHEB,
+ CHEB2RECTDISK) = range(10)
 _HARMONIC = {"sph2fourier": SPH2FOURIER, "fourier2sph": FOURIER2SPH, "sphv2fourier": SPHV2FOURIER,
              "fourier2sphv": FOURIER2SPHV, "disk2cxf": DISK2CXF, "cxf2disk": CXF2DISK, "tri2cheb": TRI2CHEB,
-             "cheb2tri": CHEB2TRI}
+             "cheb2tri": CHEB2TRI, "rectdisk2cheb": RECTDISK2CHEB, "cheb2rectdisk": CHEB2RECTDISK}
 
 
 class HarmonicPlan:
@@ -313,6 +315,10 @@ def plan_disk2cxf(n, alpha, beta):
     return HarmonicPlan(_lib.ft_plan_disk2cxf(int(n), float(alpha), float(beta)), int(n))
 
 
+def plan_rectdisk2cheb(n, beta):
+    return HarmonicPlan(_lib.ft_plan_rectdisk2cheb(int(n), float(beta)), int(n))
+
+
 def _harmonic(name):
     fn = getattr(_lib, "ft_execute_" + name)
 
@@ -336,6 +342,8 @@ execute_tri2cheb = _harmonic("tri2cheb")
 execute_cheb2tri = _harmonic("cheb2tri")
 execute_disk2cxf = _harmonic("disk2cxf")
 execute_cxf2disk = _harmonic("cxf2disk")
+execute_rectdisk2cheb = _harmonic("rectdisk2cheb")
+execute_cheb2rectdisk = _harmonic("cheb2rectdisk")
 
 
 def gpu_execute_harmonic(transform, trans, plan, A, M=None, stream=None):

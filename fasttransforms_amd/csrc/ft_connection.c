@@ -390,3 +390,54 @@ double *ftamd_plan_cheb2jac(int normcheb, int normjac, int n, double alpha, doub
         }
     return V;
 }
+
+/* transforms.c:185-197 (with :141-155): ultraspherical(lambda) -> Chebyshev
+ * through Jacobi(lambda-1/2, lambda-1/2) -> Jacobi(-1/2, -1/2); for the
+ * unnormalized Chebyshev side the entries of equal parity (the others are
+ * zero) scaled by the rows' sqrt(1/pi), sqrt(2/pi); for the unnormalized
+ * ultraspherical side the columns by the ratio of the two normalizations. */
+double *ftamd_plan_ultra2cheb(int normultra, int normcheb, int n, double lambda) {
+    double *V = ftamd_plan_jac2jac(normultra, 1, n, lambda - 0.5, lambda - 0.5, -0.5, -0.5);
+    if (!V) return NULL;
+    if (!normultra) {
+        double sclcol = 1;
+        for (int j = 0; j < n; j++) {
+            if (j > 0) sclcol = (2 * lambda + j - 1) / (lambda + j - 0.5) * sclcol;
+            for (int i = 0; i <= j; i++) V[i + (size_t)j * n] *= sclcol;
+        }
+    }
+    if (!normcheb)
+        for (int j = 0; j < n; j++)
+            for (int i = j; i >= 0; i -= 2) {
+                const double sclrow = i ? (double)sqrtl(M_2_PIl) : (double)sqrtl(M_1_PIl);
+                V[i + (size_t)j * n] *= sclrow;
+            }
+    return V;
+}
+
+/* transforms.c:199-211 (with :125-139): the inverse direction. */
+double *ftamd_plan_cheb2ultra(int normcheb, int normultra, int n, double lambda) {
+    double *V = ftamd_plan_jac2jac(1, normultra, n, -0.5, -0.5, lambda - 0.5, lambda - 0.5);
+    if (!V) return NULL;
+    if (!normultra) {
+        double sclrow = 1;
+        double *row = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        if (!row) {
+            free(V);
+            return NULL;
+        }
+        for (int i = 0; i < n; i++) {
+            if (i > 0) sclrow = (lambda + i - 0.5) / (2 * lambda + i - 1) * sclrow;
+            row[i] = sclrow;
+        }
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i <= j; i++) V[i + (size_t)j * n] *= row[i];
+        free(row);
+    }
+    if (!normcheb)
+        for (int j = 0; j < n; j++) {
+            const double sclcol = j ? (double)sqrtl(M_PI_2l) : (double)sqrtl(M_PIl);
+            for (int i = j; i >= 0; i -= 2) V[i + (size_t)j * n] *= sclcol;
+        }
+    return V;
+}

@@ -24,7 +24,7 @@
 struct ftamd_harmonic_dev {
     pthread_mutex_t lock; /* held for a whole call: guards the buffers below */
     int device;           /* HIP device of the uploads, -1 before */
-    double *dP[2], *dPinv[2];
+    double *dP[3], *dPinv[3];
     void *dY;             /* product output, n x M */
     size_t y_bytes;
     void *dA;             /* staging of host matrices */
@@ -36,7 +36,7 @@ void ft_destroy_harmonic_plan(ft_harmonic_plan *P) {
     if (P->dev) {
         struct ftamd_harmonic_dev *d = P->dev;
         if (d->device >= 0) ftamd_hip_sync(NULL);
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < 3; k++) {
             ftamd_hip_free(d->dP[k]);
             ftamd_hip_free(d->dPinv[k]);
         }
@@ -56,9 +56,9 @@ void ft_destroy_harmonic_plan(ft_harmonic_plan *P) {
     free(P);
 }
 
-/* A plan with one rotation plan and two matrix pairs, members allocated
+/* A plan with one rotation plan and np matrix pairs, members allocated
  * (NULL if out of memory). */
-static ft_harmonic_plan *plan_alloc(void) {
+static ft_harmonic_plan *plan_alloc(int np) {
     ft_harmonic_plan *P = calloc(1, sizeof(ft_harmonic_plan));
     if (!P) return NULL;
     P->dev = calloc(1, sizeof(struct ftamd_harmonic_dev));
@@ -67,20 +67,22 @@ static ft_harmonic_plan *plan_alloc(void) {
         P->dev->device = -1;
     }
     P->RP = calloc(1, sizeof(ft_rotation_plan *));
-    P->P = calloc(2, sizeof(double *));
-    P->Pinv = calloc(2, sizeof(double *));
+    P->P = calloc((size_t)np, sizeof(double *));
+    P->Pinv = calloc((size_t)np, sizeof(double *));
     if (!P->dev || !P->RP || !P->P || !P->Pinv) {
         ft_destroy_harmonic_plan(P);
         return NULL;
     }
     P->NRP = 1;
-    P->NP = 2;
+    P->NP = np;
     return P;
 }
 
 /* The plan if every member was built, else NULL (destroyed; error set). */
 static ft_harmonic_plan *plan_done(ft_harmonic_plan *P) {
-    if (P && P->RP[0] && P->P[0] && P->P[1] && P->Pinv[0] && P->Pinv[1]) return P;
+    int ok = P && P->RP[0];
+    for (int k = 0; ok && k < P->NP; k++) ok = P->P[k] && P->Pinv[k];
+    if (ok) return P;
     ft_destroy_harmonic_plan(P);
     ftamd_set_error(FT_GPU_ENOMEM);
     return NULL;
@@ -92,7 +94,7 @@ ft_harmonic_plan *ft_plan_sph2fourier(const int n) {
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
     }
-    ft_harmonic_plan *P = plan_alloc();
+    ft_harmonic_plan *P = plan_alloc(2);
     if (!P) return plan_done(NULL);
     P->RP[0] = ft_plan_rotsphere(n);
     P->P[0] = ftamd_plan_leg2cheb(1, 0, n);
@@ -108,7 +110,7 @@ ft_harmonic_plan *ft_plan_tri2cheb(const int n, const double alpha, const double
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
     }
-    ft_harmonic_plan *P = plan_alloc();
+    ft_harmonic_plan *P = plan_alloc(2);
     if (!P) return plan_done(NULL);
     P->RP[0] = ft_plan_rottriangle(n, alpha, beta, gamma);
     P->P[0] = ftamd_plan_jac2jac(1, 1, n, beta + gamma + 1.0, alpha, -0.5, -0.5);
@@ -121,6 +123,25 @@ ft_harmonic_plan *ft_plan_tri2cheb(const int n, const double alpha, const double
     return plan_done(P);
 }
 
+/* drivers.c:1123-1141 */
+ft_harmonic_plan *ft_plan_rectdisk2cheb(const int n, const double beta) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_harmonic_plan *P = plan_alloc(3);
+    if (!P) return plan_done(NULL);
+    P->RP[0] = ft_plan_rotrectdisk(n, beta);
+    P->P[0] = ftamd_plan_ultra2cheb(1, 0, n, beta + 1.0);
+    P->P[1] = ftamd_plan_ultra2ultra(1, 0, n, beta + 2.0, 1.0);
+    P->P[2] = ftamd_plan_ultra2cheb(1, 0, n, beta + 0.5);
+    P->Pinv[0] = ftamd_plan_cheb2ultra(0, 1, n, beta + 1.0);
+    P->Pinv[1] = ftamd_plan_ultra2ultra(0, 1, n, 1.0, beta + 2.0);
+    P->Pinv[2] = ftamd_plan_cheb2ultra(0, 1, n, beta + 0.5);
+    P->beta = beta;
+    return plan_done(P);
+}
+
 /* drivers.c:959-985: P[0] rows of the Chebyshev side and P[1] scaled by
  * 2^((alpha+beta+2)/2) (times sqrt(2/pi) for P[1]), the inverses by the
  * reciprocals, on and above the diagonal. */
@@ -129,7 +150,7 @@ ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
     }
-    ft_harmonic_plan *P = plan_alloc();
+    ft_harmonic_plan *P = plan_alloc(2);
     if (!P) return plan_done(NULL);
     P->RP[0] = ft_plan_rotdisk(n, alpha, beta);
     P->P[0] = ftamd_plan_jac2cheb(1, 0, n, beta, alpha);
@@ -162,7 +183,7 @@ static int upload(ft_harmonic_plan *P) {
     if (d->device >= 0) return d->device == dev ? FTAMD_OK : FTAMD_EINVAL;
     const int n = P->RP[0]->n;
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
-    for (int k = 0; rc == FTAMD_OK && k < 2; k++) {
+    for (int k = 0; rc == FTAMD_OK && k < P->NP; k++) {
         rc = ftamd_hip_malloc((void **)&d->dP[k], bytes);
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d->dP[k], P->P[k], bytes, NULL);
         if (rc == FTAMD_OK) rc = ftamd_hip_malloc((void **)&d->dPinv[k], bytes);
@@ -195,6 +216,8 @@ static int rotation_kind(int transform) {
     case FT_GPU_CXF2DISK: return FT_GPU_DISK;
     case FT_GPU_TRI2CHEB:
     case FT_GPU_CHEB2TRI: return FT_GPU_TRI;
+    case FT_GPU_RECTDISK2CHEB:
+    case FT_GPU_CHEB2RECTDISK: return FT_GPU_RECTDISK;
     }
     return -1;
 }
@@ -204,7 +227,8 @@ static int valid(int transform, char TRANS, const ft_harmonic_plan *P, int N, in
     if (!P || !P->dev || !P->RP || !P->RP[0] || rk < 0 || (TRANS != 'N' && TRANS != 'T')) return 0;
     /* the plan's rotation tables must be the transform's geometry */
     const int geo = rk == FT_GPU_TRI ? FTAMD_TRI : rk == FT_GPU_DISK ? FTAMD_DISK : FTAMD_SPH;
-    return P->RP[0]->kind == geo && N == P->RP[0]->n && M >= 1 && ftamd_valid_M(rk, N, M);
+    const int np = rk == FT_GPU_RECTDISK ? 3 : 2; /* matrix pairs the transform uses */
+    return P->RP[0]->kind == geo && P->NP >= np && N == P->RP[0]->n && M >= 1 && ftamd_valid_M(rk, N, M);
 }
 
 /* One product into the work buffer and back into dA. */
@@ -288,6 +312,51 @@ static int run_tri(ft_harmonic_plan *P, int inverse, char TRANS, void *dA, int M
     return rc;
 }
 
+/* The rectangle-disk's transforms (drivers.c:1143-1169): rectdisk2cheb 'N'
+ * is hi2lo, P[0] from the left on the even columns and P[1] on the odd ones
+ * (column classes j mod 2), then P[2]^T from the right on every row;
+ * cheb2rectdisk inverts it with Pinv in reverse order. */
+static int run_rectdisk(ft_harmonic_plan *P, int inverse, char TRANS, void *dA, int M, size_t bytes, void *stream) {
+    struct ftamd_harmonic_dev *d = P->dev;
+    const int n = P->RP[0]->n;
+    double *const *H = inverse ? d->dPinv : d->dP;
+    FtamdTrmmLaunch t;
+    memset(&t, 0, sizeof(t));
+    t.n = n;
+    t.M = M;
+    t.cstride = 2;
+    t.ncls = 2;
+    for (int c = 0; c < 2; c++) {
+        t.U[c] = H[c];
+        const int count = M > c ? (M - c + 1) / 2 : 0;
+        t.ctiles[c] = (count + FTAMD_TRMM_COLS - 1) / FTAMD_TRMM_COLS;
+    }
+    const int fwd = !inverse ? TRANS == 'N' : TRANS == 'T'; /* rotations before the products */
+    int rc = FTAMD_OK;
+    if (fwd && !inverse) {
+        rc = ft_gpu_execute(P->RP[0], FT_GPU_RECTDISK, 0, FT_GPU_FP64, dA, M, stream);
+        t.trans = 0;
+        if (rc == FTAMD_OK) rc = product(d, &t, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = tri_product(d, H[2], 1, 1, n, M, dA, bytes, stream);
+    } else if (!inverse) {
+        rc = tri_product(d, H[2], 1, 0, n, M, dA, bytes, stream);
+        t.trans = 1;
+        if (rc == FTAMD_OK) rc = product(d, &t, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ft_gpu_execute(P->RP[0], FT_GPU_RECTDISK, 1, FT_GPU_FP64, dA, M, stream);
+    } else if (TRANS == 'N') {
+        rc = tri_product(d, H[2], 1, 1, n, M, dA, bytes, stream);
+        t.trans = 0;
+        if (rc == FTAMD_OK) rc = product(d, &t, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ft_gpu_execute(P->RP[0], FT_GPU_RECTDISK, 1, FT_GPU_FP64, dA, M, stream);
+    } else {
+        rc = ft_gpu_execute(P->RP[0], FT_GPU_RECTDISK, 0, FT_GPU_FP64, dA, M, stream);
+        t.trans = 1;
+        if (rc == FTAMD_OK) rc = product(d, &t, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = tri_product(d, H[2], 1, 0, n, M, dA, bytes, stream);
+    }
+    return rc;
+}
+
 /* The transform on device matrix dA, asynchronously on `stream` (caller
  * holds the lock and has uploaded the matrices). */
 static int run(ft_harmonic_plan *P, int transform, char TRANS, void *dA, int M, void *stream) {
@@ -298,6 +367,7 @@ static int run(ft_harmonic_plan *P, int transform, char TRANS, void *dA, int M, 
     int rc = reserve(&d->dY, &d->y_bytes, bytes);
     if (rc != FTAMD_OK) return rc;
     if (rk == FT_GPU_TRI) return run_tri(P, inverse, TRANS, dA, M, bytes, stream);
+    if (rk == FT_GPU_RECTDISK) return run_rectdisk(P, inverse, TRANS, dA, M, bytes, stream);
     /* drivers.c:833-899, 987-1021: rotations first for the forward 'N' and the inverse 'T' */
     const int rot_first = inverse ? TRANS == 'T' : TRANS == 'N';
     const int rot_dir = rot_first ? 0 : 1;
@@ -375,6 +445,14 @@ void ft_execute_tri2cheb(const char TRANS, const ft_harmonic_plan *P, double *A,
 /* drivers.c:940-957 */
 void ft_execute_cheb2tri(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
     harmonic(FT_GPU_CHEB2TRI, TRANS, P, A, N, M);
+}
+/* drivers.c:1143-1156 */
+void ft_execute_rectdisk2cheb(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_RECTDISK2CHEB, TRANS, P, A, N, M);
+}
+/* drivers.c:1158-1171 */
+void ft_execute_cheb2rectdisk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_CHEB2RECTDISK, TRANS, P, A, N, M);
 }
 /* drivers.c:987-1003 */
 void ft_execute_disk2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {

@@ -76,6 +76,8 @@ double *ftamd_plan_ultra2ultra(int norm1, int norm2, int n, double lambda, doubl
 double *ftamd_plan_jac2jac(int norm1, int norm2, int n, double alpha, double beta, double gamma, double delta);
 double *ftamd_plan_jac2cheb(int normjac, int normcheb, int n, double alpha, double beta);
 double *ftamd_plan_cheb2jac(int normcheb, int normjac, int n, double alpha, double beta);
+double *ftamd_plan_ultra2cheb(int normultra, int normcheb, int n, double lambda);
+double *ftamd_plan_cheb2ultra(int normcheb, int normultra, int n, double lambda);
 
 /* ft_drivers.c */
 int ftamd_valid_M(int kind, int n, int M); /* widths a whole-matrix executor of public kind `kind` accepts */

@@ -156,6 +156,16 @@ ft_harmonic_plan *ft_plan_tri2cheb(const int n, const double alpha, const double
 void ft_execute_tri2cheb(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 void ft_execute_cheb2tri(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 
+/* fasttransforms.h:426-431 (drivers.c:1123-1171) -- Dunkl-Xu (rectangle and
+ * disk) <-> bivariate Chebyshev: the rectangle-disk rotation plan, P =
+ * {ultraspherical(beta+1) -> Chebyshev, ultraspherical(beta+2) ->
+ * ultraspherical(1), ultraspherical(beta+1/2) -> Chebyshev} and their
+ * inverses.  A is N x M (M <= N); rectdisk2cheb 'N' is hi2lo, P[0] and P[1]
+ * from the left on the even and odd columns, P[2]^T from the right. */
+ft_harmonic_plan *ft_plan_rectdisk2cheb(const int n, const double beta);
+void ft_execute_rectdisk2cheb(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_cheb2rectdisk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+
 /* fasttransforms.h:410-415 (drivers.c:959-1021) -- generalized Zernike <->
  * Chebyshev x Fourier: the disk's rotation plan, P[0] = Jacobi(beta, alpha)
  * -> Chebyshev and P[1] = Jacobi(beta, alpha+1) -> Jacobi(-1/2, 1/2), scaled
@@ -225,7 +235,7 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, 
                            void *stream);
 
 /* The harmonic transforms on a device-resident A (dA, N x M FP64) on a HIP
- * stream, asynchronously: transform FT_GPU_SPH2FOURIER .. FT_GPU_CHEB2TRI,
+ * stream, asynchronously: transform FT_GPU_SPH2FOURIER .. FT_GPU_CHEB2RECTDISK,
  * TRANS 'N' or 'T', as the executors above.
  * Errors of the launches come back through ft_gpu_synchronize.  Calls on
  * one harmonic plan share its device work buffer: order them on one
@@ -238,7 +248,9 @@ enum {
     FT_GPU_DISK2CXF = 4,
     FT_GPU_CXF2DISK = 5,
     FT_GPU_TRI2CHEB = 6,
-    FT_GPU_CHEB2TRI = 7
+    FT_GPU_CHEB2TRI = 7,
+    FT_GPU_RECTDISK2CHEB = 8,
+    FT_GPU_CHEB2RECTDISK = 9
 };
 int ft_gpu_execute_harmonic(int transform, char TRANS, const ft_harmonic_plan *P, void *dA, int N, int M,
                             void *stream);

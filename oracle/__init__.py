@@ -265,17 +265,56 @@ def _tri_harmonic(transform, trans, plan, H, A, M, params):
     return A
 
 
+def _rectdisk_harmonic(transform, trans, plan, H, A, M):
+    """rectdisk2cheb / cheb2rectdisk (drivers.c:1143-1169): P[0], P[1] from
+    the left on the even and odd columns, P[2] from the right; long double
+    products."""
+    inverse = transform == "cheb2rectdisk"
+
+    def left(t):
+        for c in (0, 1):
+            cols = np.arange(c, M, 2)
+            if cols.size:
+                U = np.triu(H[c]).astype(np.longdouble)
+                A[:, cols] = ((U.T if t else U) @ A[:, cols].astype(np.longdouble)).astype(np.float64)
+
+    def right(t):
+        U = np.triu(H[2][:M, :M]).astype(np.longdouble)
+        A[:, :M] = (A[:, :M].astype(np.longdouble) @ (U.T if t else U)).astype(np.float64)
+
+    if not inverse and trans == "N":
+        execute(plan, HI2LO, A, M)
+        left(False)
+        right(True)
+    elif not inverse:
+        right(False)
+        left(True)
+        execute(plan, LO2HI, A, M)
+    elif trans == "N":
+        right(True)
+        left(False)
+        execute(plan, LO2HI, A, M)
+    else:
+        execute(plan, HI2LO, A, M)
+        left(True)
+        right(False)
+    return A
+
+
 def harmonic(transform, trans, plan, P, Pinv, A, M=None, params=None):
     """The harmonic transforms of drivers.c:833-1021 on the n x M matrix A (in
     place): transform "sph2fourier", "fourier2sph", "sphv2fourier",
     "fourier2sphv" (plan: the sphere's rotation Plan), "disk2cxf", "cxf2disk"
-    (the disk's) or "tri2cheb", "cheb2tri" (the triangle's, params its
-    (alpha, beta, gamma)); trans "N" or "T"; P and Pinv the two connection
-    matrices and their inverses.  The rotations are the oracle's; each
+    (the disk's), "tri2cheb", "cheb2tri" (the triangle's, params its
+    (alpha, beta, gamma)) or "rectdisk2cheb", "cheb2rectdisk" (the
+    rectangle-disk's; three matrices); trans "N" or "T"; P and Pinv the
+    connection matrices and their inverses.  The rotations are the oracle's; each
     triangular product is evaluated in long double and rounded once."""
     M = A.shape[1] if M is None else M
     if transform in ("tri2cheb", "cheb2tri"):
         return _tri_harmonic(transform, trans, plan, Pinv if transform == "cheb2tri" else P, A, M, params)
+    if transform in ("rectdisk2cheb", "cheb2rectdisk"):
+        return _rectdisk_harmonic(transform, trans, plan, Pinv if transform == "cheb2rectdisk" else P, A, M)
     vec = transform.startswith("sphv") or transform.endswith("sphv")
     inverse = transform.startswith("fourier") or transform == "cxf2disk"
     H = Pinv if inverse else P

@@ -120,3 +120,23 @@ def test_tri_disk_at_1024_and_round_trip(ft, transform, params):
     # the reference's published round-trip errors at degree 1023 are 3.6e-14
     # (tri2cheb) and 2.6e-15 (disk2cxf) (README.md:64, 75)
     assert O.rel2(X, A) < (1e-13 if transform == "tri2cheb" else 1e-14)
+
+
+@pytest.mark.parametrize("transform", ["rectdisk2cheb", "cheb2rectdisk"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("beta", [0.0, -0.25])
+def test_rectdisk_transforms(ft, transform, trans, beta):
+    """rectdisk2cheb / cheb2rectdisk (left products on the even and odd
+    columns, the right product) against the oracle composite."""
+    for n in (1, 2, 5, 64, 129, 300, 1024):
+        H = ft.plan_rectdisk2cheb(n, beta)
+        head = H._head()
+        P = [H.P(k) for k in range(3)]
+        Pinv = [np.ctypeslib.as_array(head.Pinv[k], shape=(n * n,)).copy().reshape(n, n, order="F") for k in range(3)]
+        Q = O.Plan("rectdisk", n, beta)
+        for M in sorted({m for m in (1, 2, 37, n) if m <= n}):
+            A = np.asfortranarray(np.random.default_rng(n + 3 * M).uniform(-1, 1, (n, M)))
+            _say("harmonic", transform, beta, trans, "n=%d M=%d" % (n, M))
+            G = getattr(ft, "execute_" + transform)(trans, H, A.copy(order="F"))
+            R = O.harmonic(transform, trans, Q, P, Pinv, A.copy(order="F"))
+            assert O.rel2(G, R) <= 1e-14, (transform, trans, beta, n, M, O.rel2(G, R))

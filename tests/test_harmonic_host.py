@@ -259,3 +259,33 @@ def test_tri_disk_plans_and_oracle_round_trip(ft, which, params):
     Fx = O.harmonic(fwd, "N", Q, P, Pinv, x.copy(order="F"), params=params)
     Fty = O.harmonic(fwd, "T", Q, P, Pinv, y.copy(order="F"), params=params)
     assert abs(np.sum(y * Fx) - np.sum(Fty * x)) < 1e-12 * np.sqrt(np.sum(y * y) * np.sum(x * x)) * n
+
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.5, -0.25])
+def test_rectdisk2cheb_plan_and_oracle_round_trip(ft, beta):
+    """ft_plan_rectdisk2cheb's matrices (drivers.c:1123-1141): P[0] takes the
+    normalized ultraspherical(beta+1) polynomials (Jacobi(beta+1/2, beta+1/2))
+    to Chebyshev T (checked against scipy's polynomials at Chebyshev points);
+    inverse pairs multiply to the identity; the composite oracle inverts
+    itself."""
+    n = 30
+    H = ft.plan_rectdisk2cheb(n, beta)
+    head = H._head()
+    assert head.NP == 3
+    from scipy.special import eval_chebyt, eval_jacobi
+    x = np.cos(np.pi * (np.arange(n) + 0.5) / n)
+    lam = beta + 1.0
+    src = np.array([eval_jacobi(l, lam - 0.5, lam - 0.5, x) / _jacobi_norm(l, lam - 0.5, lam - 0.5)
+                    for l in range(n)]).T
+    dst = np.array([eval_chebyt(k, x) for k in range(n)]).T
+    W = np.linalg.solve(dst, src)  # normalized ultraspherical(lam) in the Chebyshev T basis
+    assert np.max(np.abs(H.P(0) - W)) <= 1e-11 * np.max(np.abs(W))
+    Pm = [H.P(k) for k in range(3)]
+    Pi = [np.ctypeslib.as_array(head.Pinv[k], shape=(n * n,)).copy().reshape(n, n, order="F") for k in range(3)]
+    for k in range(3):
+        assert np.max(np.abs(Pi[k] @ Pm[k] - np.eye(n))) < 1e-12
+    Q = O.Plan("rectdisk", n, beta)
+    A = O.fill("tri", n, n, 9)
+    X = O.harmonic("cheb2rectdisk", "N", Q, Pm, Pi, O.harmonic("rectdisk2cheb", "N", Q, Pm, Pi, A.copy(order="F")))
+    assert O.rel2(X, A) < 1e-13
