@@ -427,6 +427,8 @@ def harmonic(ft, torch, n, transform, trans, steps, warmup):
         H, kind = ft.plan_tri2cheb(n, 0.0, 0.0, 0.0), "tri"
     elif transform in ("disk2cxf", "cxf2disk"):
         H, kind = ft.plan_disk2cxf(n, 0.0, 0.0), "disk"
+    elif transform in ("rectdisk2cheb", "cheb2rectdisk"):
+        H, kind = ft.plan_rectdisk2cheb(n, 0.0), "tri"
     else:
         H, kind = ft.plan_sph2fourier(n), "sph"
     M = {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[kind]
@@ -452,7 +454,7 @@ def harmonic_rows(ft, torch, n):
 def run_case(ft, torch, spec, steps, warmup):
     """--case: one configuration only, for profiling runs."""
     parts = spec.split(":")
-    if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv"):
+    if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv", "rectdisk2cheb", "cheb2rectdisk"):
         ms = harmonic(ft, torch, int(parts[1]), parts[0], parts[2] if len(parts) > 2 else "N", steps, warmup)
         print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
         return
