@@ -58,7 +58,7 @@ def parse():
     p.add_argument("--engine", choices=("auto", "pipeline", "systolic"), default="auto",
                    help="engine of the headline launches (auto: the library's own choice)")
     p.add_argument("--case", help="time one case only, KIND:N:DIR[:fp32][:batch] (for profiling runs; KIND one of "
-                                  "sph, sphv, tri, disk, rectdisk, annulus), "
+                                  "sph, sphv, tri, disk, rectdisk, annulus, or tet on the N x N x N array), "
                                   "e.g. tri:1024:hi2lo or sph:1024:hi2lo:batch (the 512-transform batch)")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -109,6 +109,29 @@ def single(ft, torch, kind, n, direction, precision, steps, warmup, params=(), s
                 lambda: ft.gpu_execute(P, direction, T, precision=precision, stream=stream, kind=ex), steps, warmup)
     out = (ms, 6.0 * rotations(kind, n, M))
     return out + ((P, T),) if keep else out
+
+
+def tet(ft, torch, N, direction, steps, warmup):
+    """ft_gpu_execute_tet on the N x N x N array (the reference's timing shape,
+    test_drivers.c:1376-1380, its parameters alpha = beta = gamma = delta = 0)."""
+    from fasttransforms_amd.utilities import tet_rotations
+    L = M = N
+    P1, P2 = ft.plan_rottriangle(N, 0.0, 0.0, 1.0), ft.plan_rottriangle(N, 0.0, 0.0, 0.0)
+    g = torch.Generator(device="cpu").manual_seed(N)
+    T = (2 * torch.rand(N * L * M, dtype=torch.float64, generator=g) - 1).cuda()
+    stream = torch.cuda.current_stream()
+    ms = events(ft, torch, stream, lambda: ft.gpu_execute_tet(P1, P2, direction, T, L, M, stream=stream), steps, warmup)
+    return ms, 6.0 * tet_rotations(N, L, M)
+
+
+def tet_rows(ft, torch, sizes=(256, 512)):
+    out = {}
+    for N in sizes:
+        for d in ("hi2lo", "lo2hi"):
+            ms, fl = tet(ft, torch, N, d, 5, 2)
+            out["tet_n%d_%s_ms" % (N, d)] = round(ms, 4)
+            out["tet_n%d_%s_gflops" % (N, d)] = round(fl / (ms * 1e-3) / 1e9, 2)
+    return out
 
 
 def gpu_vs_reference(ft, torch, n):
@@ -459,6 +482,10 @@ def run_case(ft, torch, spec, steps, warmup):
         print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
         return
     kind, nn, d = parts[0], int(parts[1]), parts[2]
+    if kind == "tet":
+        m_, fl = tet(ft, torch, nn, d, steps, warmup)
+        print(json.dumps({"case": spec, "ms": round(m_, 5), "gflops": round(fl / (m_ * 1e-3) / 1e9, 2)}), flush=True)
+        return
     precision = ft.FP32 if "fp32" in parts[3:] else ft.FP64
     params = PARAMS[kind]
     if "batch" in parts[3:]:
@@ -535,6 +562,7 @@ def main():
             "tri_m05_hi2lo_ms": round(msth, 4), "tri_m05_hi2lo_gflops": round(flt / (msth * 1e-3) / 1e9, 2),
             "disk_hi2lo_ms": round(msd, 4), "disk_hi2lo_gflops": round(fld / (msd * 1e-3) / 1e9, 2),
             "harmonic_n%d" % n: harmonic_rows(ft, torch, n),
+            "tet": tet_rows(ft, torch),
             "batch_sph_hi2lo_512": batch(ft, torch),
             "batch64_tri": batch64(ft, torch, "tri", (0.0, 0.0, 0.0)),
             "batch64_tri_m05": batch64(ft, torch, "tri", (-0.5, -0.5, -0.5)),

@@ -90,6 +90,11 @@ for _dn in ("hi2lo", "lo2hi"):
     for _k in ("sph", "tri", "disk", "sphv", "rectdisk"):
         SIGNATURES["ft_execute_%s_%s" % (_k, _dn)] = (None, [_vp, _vp, _vp, _i])
 
+for _dn in ("hi2lo", "lo2hi"):
+    SIGNATURES["ft_execute_tet_" + _dn] = (None, [_vp, _vp, _vp, _i, _i])
+    SIGNATURES["ft_kernel_tet_" + _dn] = (None, [_vp, _i, _i, _vp])
+SIGNATURES["ft_gpu_execute_tet"] = (_i, [_vp, _vp, _i, _vp, _i, _i, _vp])
+
 for _name, (_res, _args) in SIGNATURES.items():
     _fn = getattr(_lib, _name)
     _fn.restype = _res
@@ -459,6 +464,70 @@ kernel_disk_hi2lo = _kernel("disk", "hi2lo")
 kernel_disk_lo2hi = _kernel("disk", "lo2hi")
 kernel_rectdisk_hi2lo = _kernel("rectdisk", "hi2lo")
 kernel_rectdisk_lo2hi = _kernel("rectdisk", "lo2hi")
+
+
+def _tet_pointer(plan1, plan2, A, L, M):
+    if plan1.kind != TRI or plan2.kind != TRI:
+        raise ValueError("the tetrahedron takes triangle plans")
+    n = plan1.n
+    if _is_torch(A):
+        if not A.is_contiguous() or A.numel() < n * L * M:
+            raise ValueError("expected a contiguous tensor of n*L*M entries")
+        import torch
+        if A.dtype != torch.float64:
+            raise TypeError("expected a torch.float64 tensor")
+        return _vp(A.data_ptr())
+    if not isinstance(A, np.ndarray) or A.dtype != np.float64:
+        raise TypeError("expected a float64 numpy array")
+    if not A.flags.f_contiguous or A.size < n * L * M:
+        raise ValueError("expected a Fortran-ordered array of n*L*M entries")
+    return A.ctypes.data_as(_vp)
+
+
+def _execute_tet(direction):
+    fn = getattr(_lib, "ft_execute_tet_" + direction)
+
+    def run(plan1, plan2, A, L, M):
+        fn(plan1.ptr, plan2.ptr, _tet_pointer(plan1, plan2, A, int(L), int(M)), int(L), int(M))
+        _check()
+        return A
+
+    run.__name__ = "execute_tet_" + direction
+    run.__doc__ = ("execute_tet_%s(plan1, plan2, A, L, M): in place on the n x L x M array A (Fortran order; "
+                   "slab m is A[:, :, m])." % direction)
+    return run
+
+
+def _kernel_tet(direction):
+    fn = getattr(_lib, "ft_kernel_tet_" + direction)
+
+    def run(plan, L, m, A):
+        fn(plan.ptr, int(L), int(m), _tet_pointer(plan, plan, A, int(L), 1))
+        _check()
+        return A
+
+    run.__name__ = "kernel_tet_" + direction
+    return run
+
+
+execute_tet_hi2lo = _execute_tet("hi2lo")
+execute_tet_lo2hi = _execute_tet("lo2hi")
+kernel_tet_hi2lo = _kernel_tet("hi2lo")
+kernel_tet_lo2hi = _kernel_tet("lo2hi")
+
+
+def gpu_execute_tet(plan1, plan2, direction, A, L, M, stream=None):
+    """Asynchronous ft_gpu_execute_tet on a contiguous torch float64 tensor of
+    n*L*M entries (the n x L x M array in Fortran order)."""
+    import torch
+    direction = _DIR.get(direction, direction)
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    ptr = _tet_pointer(plan1, plan2, A, int(L), int(M))
+    rc = _lib.ft_gpu_execute_tet(plan1.ptr, plan2.ptr, int(direction), ptr, int(L), int(M), _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_tet: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
 
 
 def gpu_execute(plan, direction, A, M=None, precision=FP64, stream=None, kind=None):

@@ -1057,6 +1057,226 @@ void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B,
     execute(RP, FT_GPU_RECTDISK, 1, A, M);
 }
 
+/* ------------------------------------------------------ tetrahedron -- */
+
+/* The block rotations of slabs of L < n columns read the triangle's (s, c)
+ * for l + j <= L - 2 only, which are those of the triangle plan of size L
+ * with RP's parameters (the entries do not depend on the size,
+ * rotations.c:143-160): that plan, kept in RP's list and built on first use
+ * (caller holds RP->lock). */
+static ft_rotation_plan *tet_block_plan(ft_rotation_plan *RP, int L, int *rc) {
+    *rc = FTAMD_OK;
+    if (L == RP->n) return RP;
+    for (ft_rotation_plan *p = RP->subs; p; p = p->sub_next)
+        if (p->n == L) return p;
+    ft_rotation_plan *p = ft_plan_rottriangle(L, RP->alpha, RP->beta, RP->gamma);
+    if (!p) {
+        *rc = FTAMD_ENOMEM;
+        return NULL;
+    }
+    p->sub_next = RP->subs;
+    RP->subs = p;
+    return p;
+}
+
+/* The band-systolic schedule of a tetrahedron stage (ftamd_sys_chains_tet_fill)
+ * on `count` slabs of N rows and L columns from order m0, on P, the plan whose
+ * tables it runs: cached in P if `cache`, else the caller's to release
+ * (caller holds no lock). */
+static int tet_schedule(ft_rotation_plan *P, int part, int N, int L, int m0, int count, int dir, int cache,
+                        ft_schedule **out) {
+    *out = NULL;
+    const int layout = part == 0 ? FTAMD_LAYOUT_TET : FTAMD_LAYOUT_TETBLOCK;
+    const int Sq = ftamd_sys_choose_S(FTAMD_TRI, P->n, 0);
+    if (Sq == 0) return FTAMD_EINVAL; /* a build without the band-systolic engine */
+    ftamd_choice choice;
+    ftamd_choose(FTAMD_TRI, 0, dir, 0, P->n, &choice);
+    const int maxpart = choice.maxpart;
+    pthread_mutex_lock(&P->lock);
+    int rc = ftamd_plan_upload(P, 0);
+    ft_schedule *S = NULL;
+    for (ft_schedule *s = rc == FTAMD_OK && cache ? P->scheds : NULL; s; s = s->next)
+        if (s->sys && s->layout == layout && s->M == count && s->dir == dir && s->precision == 0 &&
+            s->maxpart == maxpart && s->slab == L && s->order0 == m0 && s->rs == N) {
+            S = s;
+            break;
+        }
+    if (rc == FTAMD_OK && !S && !P->sys[dir][0]) rc = ftamd_sys_tables_build(P, dir, 0, Sq, &P->sys[dir][0]);
+    if (rc == FTAMD_OK && !S) {
+        const ft_sys_tables *T = P->sys[dir][0];
+        FtamdSysChain *ch = NULL;
+        int nch = 0, nprog = 0;
+        rc = ftamd_sys_chains_tet(P, T, part, N, L, m0, count, maxpart, &ch, &nch, &nprog);
+        if (rc == FTAMD_OK) {
+            S = calloc(1, sizeof(ft_schedule));
+            if (!S) rc = FTAMD_ENOMEM;
+        }
+        if (rc == FTAMD_OK) {
+            S->sys = 1;
+            S->M = count;
+            S->layout = layout;
+            S->dir = dir;
+            S->precision = 0;
+            S->maxpart = maxpart;
+            S->slab = L;
+            S->order0 = m0;
+            S->rs = N;
+            rc = sys_upload(S, ch, nch, nprog, ftamd_sys_tables_host_bands(T), NULL, 0, NULL);
+            if (rc != FTAMD_OK) {
+                free(S);
+                S = NULL;
+            }
+        }
+        free(ch);
+        if (rc == FTAMD_OK) {
+            FtamdSysLaunch *a = &S->sys_launch;
+            a->bands = ftamd_sys_tables_dev_bands(T);
+            a->rowmask = 0;
+            a->tab = ftamd_sys_tables_dev_tab(T);
+            S->L = ftamd_sys_tables_S(T);
+            if (cache) {
+                S->next = P->scheds;
+                P->scheds = S;
+            }
+        }
+    }
+    pthread_mutex_unlock(&P->lock);
+    *out = S;
+    return rc;
+}
+
+/* A tetrahedron call: triangle plans of one size N (the reference reads RP1's
+ * size as the slab's rows and RP2's as their stride, drivers.c:510-533,
+ * rotations.c:297-321), 0 <= L <= N (a slab of more columns would read the
+ * block rotations beyond RP2's table), M >= 0. */
+static int tet_valid(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, int L, int M) {
+    return RP1 && RP2 && RP1->kind == FTAMD_TRI && RP2->kind == FTAMD_TRI && RP1->family == FT_GPU_TRI &&
+           RP2->family == FT_GPU_TRI && RP1->n == RP2->n && L >= 0 && L <= RP1->n && M >= 0 &&
+           (long long)L * M <= 0x7fffffffLL;
+}
+
+/* The stages' schedules of a tetrahedron call (S[0] the slab columns on RP1,
+ * S[1] the slab rows on RP2's block plan; part_mask selects them), m0 and
+ * count slabs, cached or the caller's to release. */
+static int tet_schedules(ft_rotation_plan *RP1, ft_rotation_plan *RP2, int part_mask, int L, int m0, int count,
+                         int dir, int cache, ft_schedule **S) {
+    const int N = RP2->n;
+    int rc = FTAMD_OK;
+    S[0] = S[1] = NULL;
+    if (part_mask & 1) rc = tet_schedule(RP1, 0, N, L, m0, count, dir, cache, &S[0]);
+    if (rc == FTAMD_OK && (part_mask & 2)) {
+        pthread_mutex_lock(&RP2->lock);
+        ft_rotation_plan *B = tet_block_plan(RP2, L, &rc);
+        pthread_mutex_unlock(&RP2->lock);
+        if (rc == FTAMD_OK) rc = tet_schedule(B, 1, N, L, m0, count, dir, cache, &S[1]);
+    }
+    return rc;
+}
+
+/* Launch order of a tetrahedron call: hi2lo runs the slab columns' rotations,
+ * then the block rotations; lo2hi the reverse (drivers.c:510-533). */
+static int tet_launch(ft_schedule *const *S, int dir, void *dA, void *stream, int checked) {
+    int rc = FTAMD_OK;
+    for (int i = 0; i < 2 && rc == FTAMD_OK; i++) {
+        ft_schedule *x = S[dir == 0 ? i : 1 - i];
+        if (!x) continue;
+        if (checked) {
+            rc = launch_checked(x, 0, FTAMD_TRI, dir, dA);
+        } else {
+            rc = launch(x, 0, FTAMD_TRI, dir, dA, stream);
+            if (rc == FTAMD_OK) pending_add(x, stream);
+        }
+    }
+    return rc;
+}
+
+int ft_gpu_execute_tet(const ft_rotation_plan *RP1_, const ft_rotation_plan *RP2_, int direction, double *dA, int L,
+                       int M, void *stream) {
+    ft_rotation_plan *RP1 = (ft_rotation_plan *)RP1_, *RP2 = (ft_rotation_plan *)RP2_;
+    if (!tet_valid(RP1, RP2, L, M) || !dA || (direction != 0 && direction != 1)) return FT_GPU_EINVAL;
+    if (L < 2 || M == 0) return FT_GPU_OK;
+    ft_schedule *S[2];
+    int rc = tet_schedules(RP1, RP2, 3, L, 0, M, direction, 1, S);
+    if (rc == FTAMD_OK) rc = tet_launch(S, direction, dA, stream, 0);
+    if (rc == FTAMD_OK) t_last_engine = FT_GPU_ENGINE_SYSTOLIC;
+    return rc;
+}
+
+/* Both plans' exec locks, in a fixed order (by serial) so that two calls
+ * with the plans swapped cannot deadlock. */
+static void lock_pair(ft_rotation_plan *a, ft_rotation_plan *b, int lock) {
+    ft_rotation_plan *first = a, *second = b;
+    if (a == b) second = NULL;
+    else if (b->serial < a->serial) first = b, second = a;
+    if (lock) {
+        pthread_mutex_lock(&first->exec_lock);
+        if (second) pthread_mutex_lock(&second->exec_lock);
+    } else {
+        if (second) pthread_mutex_unlock(&second->exec_lock);
+        pthread_mutex_unlock(&first->exec_lock);
+    }
+}
+
+/* The synchronous tetrahedron calls, host or device A of `slabs` slabs of
+ * N x L (staged through RP1's buffer): both stages (part_mask 3, schedules
+ * cached) for the executors, the block rotations of one slab (part_mask 2,
+ * a schedule per call: callers loop over the orders) for the kernels. */
+static void execute_tet(const ft_rotation_plan *RP1_, const ft_rotation_plan *RP2_, int part_mask, int dir, double *A,
+                        int L, int m0, int slabs) {
+    ft_rotation_plan *RP1 = (ft_rotation_plan *)RP1_, *RP2 = (ft_rotation_plan *)RP2_;
+    if (!tet_valid(RP1, RP2, L, m0 + slabs) || !A || m0 < 0) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    if (L < 2 || slabs == 0) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(A, &pk);
+    size_t bytes = (size_t)RP1->n * (size_t)L * (size_t)slabs * sizeof(double);
+    lock_pair(RP1, RP2, 1);
+    void *d = A;
+    if (pk != FTAMD_PTR_DEVICE) {
+        rc = stage_reserve(RP1, bytes);
+        d = RP1->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, bytes, NULL);
+    }
+    ft_schedule *S[2] = {NULL, NULL};
+    const int cache = part_mask == 3;
+    if (rc == FTAMD_OK) rc = tet_schedules(RP1, RP2, part_mask, L, m0, slabs, dir, cache, S);
+    if (rc == FTAMD_OK) rc = tet_launch(S, dir, d, NULL, 1);
+    if (!cache) {
+        ftamd_schedule_release(S[0]);
+        ftamd_schedule_release(S[1]);
+    }
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, bytes, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
+    lock_pair(RP1, RP2, 0);
+    ftamd_set_error(rc);
+}
+
+/* drivers.c:510-533 */
+void ft_execute_tet_hi2lo(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, double *A, const int L,
+                          const int M) {
+    execute_tet(RP1, RP2, 3, 0, A, L, 0, M);
+}
+void ft_execute_tet_lo2hi(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, double *A, const int L,
+                          const int M) {
+    execute_tet(RP1, RP2, 3, 1, A, L, 0, M);
+}
+
+/* rotations.c:297-321: the block rotations of the N x L slab A (N = RP->n)
+ * of order m. */
+void ft_kernel_tet_hi2lo(const ft_rotation_plan *RP, const int L, const int m, double *A) {
+    execute_tet(RP, RP, 2, 0, A, L, m, 1);
+}
+void ft_kernel_tet_lo2hi(const ft_rotation_plan *RP, const int L, const int m, double *A) {
+    execute_tet(RP, RP, 2, 1, A, L, m, 1);
+}
+
 /* ------------------------------------------------- single vectors -- */
 
 /* Single-vector schedules a plan keeps (the reference's callers loop over

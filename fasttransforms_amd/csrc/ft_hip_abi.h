@@ -17,6 +17,8 @@ enum { FTAMD_SPH = 0, FTAMD_TRI = 1, FTAMD_DISK = 2 };
  * (orders 2..M/2-1, columns 2m+1 and 2m+2), the rectangle-disk (orders
  * 2..M-1, column m).  Triangle and disk matrices have one layout each. */
 enum { FTAMD_LAYOUT_STD = 0, FTAMD_LAYOUT_SPHV = 1, FTAMD_LAYOUT_RECTDISK = 2 };
+/* The tetrahedron's two stages on triangle plans: the slab columns, the slab rows. */
+enum { FTAMD_LAYOUT_TET = 3, FTAMD_LAYOUT_TETBLOCK = 4 };
 enum { FTAMD_OK = 0, FTAMD_EINVAL = 1, FTAMD_ENOMEM = 2, FTAMD_EDEVICE = 3, FTAMD_ETIMEOUT = 4, FTAMD_EPTR = 5,
        FTAMD_EINCOMPLETE = 6 };
 enum { FTAMD_PTR_HOST = 0, FTAMD_PTR_DEVICE = 1 };

@@ -16,6 +16,8 @@ typedef struct ft_schedule {
     /* key (single-vector schedules: m1, m2 = first and last sequence j, rs = stride) */
     int M, dir, precision, L, W, rs, m1, m2, single;
     int layout;              /* FTAMD_LAYOUT_* of its columns (part of the key) */
+    int slab, order0;        /* tetrahedron stages: columns per slab and the first slab's order
+                                (with M slabs and rs = rows per slab, part of the key) */
     /* launched asynchronously since its error word was last read (ft_gpu_synchronize) */
     int pending;
     void *pending_stream;
@@ -54,6 +56,9 @@ struct ft_rotation_plan_s {
     void *d_stage;           /* staging buffer of host-pointer calls (used under exec_lock) */
     size_t stage_bytes;
     unsigned long long serial; /* distinct for every plan the process creates */
+    /* triangle plans of smaller sizes with this plan's parameters (the
+     * tetrahedron's block rotations on slabs of fewer columns), freed with it */
+    struct ft_rotation_plan_s *subs, *sub_next;
     pthread_mutex_t lock;      /* device state: uploads, schedule lists */
     pthread_mutex_t exec_lock; /* held for the whole of a synchronous executor or kernel call */
 };
@@ -118,6 +123,13 @@ const FtamdSysBand *ftamd_sys_tables_host_bands(const ft_sys_tables *T);
 int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, FtamdSysChain *buf,
                           int cap, int *count);
 int ftamd_sys_split(FtamdSysChain *buf, int nfull, int cap, int maxpart, int *count, int *nprog);
+int ftamd_sys_chains_tet_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0,
+                              int count, FtamdSysChain *buf, int cap, int *nout);
+int ftamd_sys_chains_tet(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0, int count,
+                         int maxpart, FtamdSysChain **out, int *nout, int *nprog);
+int ftamd_sys_debug_tet(const ft_rotation_plan *RP, int dir, int S, int part, int N, int L, int m0, int count,
+                        int maxpart, FtamdSysBand **bands, int *nbands, double **tab, long long *nentries,
+                        FtamdSysChain **chains, int *nchains);
 int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *tabs, void *const *dA,
                     const int *M, int count, int layout, int maxpart, FtamdSysBand **bands, int *nbands,
                     FtamdSysChain **chains,

@@ -257,6 +257,11 @@ ft_rotation_plan *ft_plan_rotannulus(const int n, const double alpha, const doub
 /* rotations.c:6-10, plus the device state. */
 void ft_destroy_rotation_plan(ft_rotation_plan *RP) {
     if (!RP) return;
+    while (RP->subs) {
+        ft_rotation_plan *next = RP->subs->sub_next;
+        ft_destroy_rotation_plan(RP->subs);
+        RP->subs = next;
+    }
     /* a cached batch schedule that uses this plan's tables goes with it */
     ftamd_batch_forget_plan(RP);
     if (RP->device >= 0) ftamd_hip_sync(NULL);

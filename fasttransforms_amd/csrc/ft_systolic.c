@@ -235,6 +235,50 @@ static void sort_chains(FtamdSysChain *c, int n) {
     }
 }
 
+/* The chain record of class c over up to FTAMD_SYS_COLS columns: column
+ * col[q] of stride lda from aoff (-1: none) applies the class's sequences from
+ * j = jfirst[q] down (hi2lo order; lo2hi up to it), the class's rows at
+ * stride rmul * rstep.  0 if no column starts at a sequence of the class. */
+static int chain_record(const ft_sys_tables *T, int c, const int *col, const int *jfirst, int lda, int rmul,
+                        long long aoff, FtamdSysChain *out) {
+    const int B = FTAMD_SYS_LANES * T->S;
+    const sys_class *cl = &T->cls[c];
+    FtamdSysChain x;
+    memset(&x, 0, sizeof(x));
+    int kmin = INT_MAX, any = 0;
+    for (int q = 0; q < FTAMD_SYS_COLS; q++) {
+        const int ok = col[q] >= 0 && jfirst[q] <= cl->jtop && jfirst[q] >= cl->jtop - cl->jstep * (cl->K - 1);
+        x.col[q] = ok ? col[q] : -1;
+        x.glo[q] = INT_MIN;
+        x.ghi[q] = INT_MAX;
+        if (!ok) continue;
+        int ks = (cl->jtop - jfirst[q]) / cl->jstep;
+        if (T->dir == 0) x.glo[q] = ks;
+        else x.ghi[q] = cl->K - 1 - ks;
+        if (ks < kmin) kmin = ks;
+        any = 1;
+    }
+    if (!any) return 0;
+    int b_first, b_last;
+    if (T->dir == 0) {
+        b_first = kmin / B;
+        b_last = T->nb[c] - 1;
+    } else {
+        b_first = 0;
+        b_last = (cl->K - 1 - kmin) / B;
+    }
+    x.band0 = T->band0[c] + b_first;
+    x.nb = b_last - b_first + 1;
+    x.dir = T->dir;
+    x.rstep = rmul * cl->rstep;
+    x.rpar = cl->rpar;
+    x.lda = lda;
+    x.aoff = aoff;
+    x.split_in = x.split_out = -1;
+    *out = x;
+    return 1;
+}
+
 /* Whole chains of an n x M matrix: per class, the columns in groups of 8
  * (four orders of the class's parity x their two columns for sph, sphv and
  * disk, eight orders of the parity for the rectangle-disk, eight consecutive
@@ -245,7 +289,7 @@ static void sort_chains(FtamdSysChain *c, int n) {
  * are needed (*count then holds the number needed). */
 int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, int layout, FtamdSysChain *buf,
                           int cap, int *count) {
-    const int kind = RP->kind, B = FTAMD_SYS_LANES * T->S;
+    const int kind = RP->kind;
     /* orders per chain step and the highest order of the layout
      * (drivers.c:94-110, 204-220, 314-326, 414-430, 496-508) */
     const int one_col = kind == FTAMD_TRI || layout == FTAMD_LAYOUT_RECTDISK;
@@ -260,9 +304,7 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
         if (kind != FTAMD_TRI && mhi % 2 != cl->jpar) mhi--;
         int mlo = kind == FTAMD_TRI ? 1 : 2;
         for (int m0 = mhi; m0 >= mlo; m0 -= kind == FTAMD_TRI ? 8 : 8 * (one_col ? 2 : 1)) {
-            FtamdSysChain x;
-            memset(&x, 0, sizeof(x));
-            int kmin = INT_MAX, any = 0;
+            int cols[FTAMD_SYS_COLS], jfirst[FTAMD_SYS_COLS];
             for (int q = 0; q < FTAMD_SYS_COLS; q++) {
                 int m, col;
                 if (kind == FTAMD_TRI) {
@@ -275,40 +317,69 @@ int ftamd_sys_chains_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, in
                     m = m0 - 2 * (q / 2);
                     col = (layout == FTAMD_LAYOUT_SPHV ? 2 * m + 1 : 2 * m - 1) + (q % 2);
                 }
-                int jfirst = kind == FTAMD_TRI ? m - 1 : m - 2;
-                int ok = m >= mlo && col < M && jfirst <= cl->jtop && jfirst >= cl->jtop - cl->jstep * (cl->K - 1);
-                x.col[q] = ok ? col : -1;
-                x.glo[q] = INT_MIN;
-                x.ghi[q] = INT_MAX;
-                if (!ok) continue;
-                int ks = (cl->jtop - jfirst) / cl->jstep;
-                if (T->dir == 0) x.glo[q] = ks;
-                else x.ghi[q] = cl->K - 1 - ks;
-                if (ks < kmin) kmin = ks;
-                any = 1;
+                jfirst[q] = kind == FTAMD_TRI ? m - 1 : m - 2;
+                cols[q] = m >= mlo && col < M ? col : -1;
             }
-            if (!any) continue;
-            int b_first, b_last;
-            if (T->dir == 0) {
-                b_first = kmin / B;
-                b_last = T->nb[c] - 1;
-            } else {
-                b_first = 0;
-                b_last = (cl->K - 1 - kmin) / B;
-            }
-            x.band0 = T->band0[c] + b_first;
-            x.nb = b_last - b_first + 1;
-            x.dir = T->dir;
-            x.rstep = cl->rstep;
-            x.rpar = cl->rpar;
-            x.lda = RP->n;
-            x.aoff = 0;
-            x.split_in = x.split_out = -1;
+            FtamdSysChain x;
+            if (!chain_record(T, c, cols, jfirst, RP->n, 1, 0, &x)) continue;
             if (nch < cap) buf[nch] = x;
             nch++;
         }
     }
     *count = nch;
+    if (nch > cap) return FTAMD_ENOMEM;
+    sort_chains(buf, nch);
+    return FTAMD_OK;
+}
+
+/* Whole chains of one of the tetrahedron's two rotation stages (drivers.c:
+ * 510-533) on `count` consecutive slabs of an N x L x M array, slab s (order
+ * m = m0 + s) being the N x L block at element N*L*s, into buf as
+ * ftamd_sys_chains_fill does:
+ *   part 0, the triangle's rotations on the slab columns (RP: the triangle
+ *   plan of size N): column l of slab m is a triangle column of order l + m
+ *   (rotations.c:163 on A + N*(l + L*m)); chains of eight consecutive columns
+ *   of one slab;
+ *   part 1, the block rotations (rotations.c:297-321; RP: the triangle plan
+ *   of size L, whose (s, c) are those of the size-N plan for every (l, j)
+ *   with l + j <= L - 2): row k of slab m is a vector of L entries at stride
+ *   N rotated as a triangle column of order m; chains of eight consecutive
+ *   rows. */
+int ftamd_sys_chains_tet_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0,
+                              int count, FtamdSysChain *buf, int cap, int *nout) {
+    int nch = 0;
+    for (int c = 0; c < T->ncls; c++) {
+        const int jtop = T->cls[c].jtop;
+        for (int sl = 0; sl < count; sl++) {
+            const int m = m0 + sl;
+            const long long aoff = (long long)N * L * sl;
+            int cols[FTAMD_SYS_COLS], jfirst[FTAMD_SYS_COLS];
+            FtamdSysChain x;
+            if (part == 0) {
+                for (int l0 = L - m - 1; l0 >= 0; l0 -= FTAMD_SYS_COLS) {
+                    for (int q = 0; q < FTAMD_SYS_COLS; q++) {
+                        cols[q] = l0 - q;
+                        jfirst[q] = l0 - q + m - 1;
+                    }
+                    if (!chain_record(T, c, cols, jfirst, N, 1, aoff, &x)) continue;
+                    if (nch < cap) buf[nch] = x;
+                    nch++;
+                }
+            } else {
+                const int jf = m - 1 < jtop ? m - 1 : jtop;
+                for (int k0 = 0; k0 < N; k0 += FTAMD_SYS_COLS) {
+                    for (int q = 0; q < FTAMD_SYS_COLS; q++) {
+                        cols[q] = k0 + q < N ? k0 + q : -1;
+                        jfirst[q] = jf;
+                    }
+                    if (!chain_record(T, c, cols, jfirst, 1, N, aoff, &x)) continue;
+                    if (nch < cap) buf[nch] = x;
+                    nch++;
+                }
+            }
+        }
+    }
+    *nout = nch;
     if (nch > cap) return FTAMD_ENOMEM;
     sort_chains(buf, nch);
     return FTAMD_OK;
@@ -485,6 +556,28 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
     return rc;
 }
 
+/* The chain parts of one of the tetrahedron's stages (ftamd_sys_chains_tet_fill)
+ * in a malloc'd array the caller frees, as ftamd_sys_chains. */
+int ftamd_sys_chains_tet(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0, int count,
+                         int maxpart, FtamdSysChain **out, int *nout, int *nprog) {
+    *out = NULL;
+    *nout = 0;
+    *nprog = 0;
+    int need = 0, n = 0;
+    ftamd_sys_chains_tet_fill(RP, T, part, N, L, m0, count, NULL, 0, &need);
+    FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)(need > 0 ? need : 1));
+    if (!ch) return FTAMD_ENOMEM;
+    int rc = ftamd_sys_chains_tet_fill(RP, T, part, N, L, m0, count, ch, need, &n);
+    if (rc == FTAMD_OK) rc = split_alloc(&ch, n, maxpart, nout, nprog);
+    if (rc != FTAMD_OK) {
+        free(ch);
+        *nout = 0;
+        return rc;
+    }
+    *out = ch;
+    return FTAMD_OK;
+}
+
 /* Bands and chains of a batch of transforms of one kind, direction,
  * precision and S (every plan's tables built): the bands of each distinct
  * plan once, their table offsets taken from the first plan's tables, the
@@ -564,6 +657,29 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
     *chains_out = all;
     *nchains_out = nparts;
     *nprog_out = nprog;
+    return FTAMD_OK;
+}
+
+/* Diagnostics (no device work) of a tetrahedron stage: the FP64 bands and
+ * table of RP and the chain parts ftamd_sys_chains_tet builds, as
+ * ftamd_sys_debug_host. */
+int ftamd_sys_debug_tet(const ft_rotation_plan *RP, int dir, int S, int part, int N, int L, int m0, int count,
+                        int maxpart, FtamdSysBand **bands, int *nbands, double **tab, long long *nentries,
+                        FtamdSysChain **chains, int *nchains) {
+    ft_sys_tables T;
+    memset(&T, 0, sizeof(T));
+    void *host = NULL;
+    int rc = tables_host(RP, dir, 0, S, &T, &host, nentries);
+    int nprog = 0;
+    if (rc == FTAMD_OK) rc = ftamd_sys_chains_tet(RP, &T, part, N, L, m0, count, maxpart, chains, nchains, &nprog);
+    if (rc != FTAMD_OK) {
+        free(T.bands);
+        free(host);
+        return rc;
+    }
+    *bands = T.bands;
+    *nbands = T.nbands;
+    *tab = host;
     return FTAMD_OK;
 }
 

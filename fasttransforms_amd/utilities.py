@@ -102,3 +102,19 @@ def rotations(kind, n, M=None):
             if top >= 0:
                 total += top + 1
     return total
+
+
+def tet_rotations(N, L, M):
+    """Number of rotations ft_execute_tet_hi2lo applies to an N x L x M array
+    (drivers.c:510-533; 6 algorithmic flops each): in slab m, column l < L-m
+    takes the triangle's orders l+m-1 .. 0 (N-1-j rotations each), and the
+    block rotations take, for j < m with j <= L-2, L-1-j rotations of whole
+    rows (N entries each)."""
+    total = 0
+    for m in range(M):
+        for l in range(max(L - m, 0)):
+            e = min(l + m, N - 1)
+            total += e * (N - 1) - e * (e - 1) // 2
+        jj = min(m, L - 1) if L >= 1 else 0
+        total += N * (jj * (L - 1) - jj * (jj - 1) // 2)
+    return total

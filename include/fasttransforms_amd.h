@@ -72,6 +72,13 @@ ft_rotation_plan *ft_plan_rotrectdisk(const int n, const double beta);
 void ft_kernel_rectdisk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 void ft_kernel_rectdisk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 
+/* fasttransforms.h:332-333 (rotations.c:297-321) -- the tetrahedron's block
+ * rotations of slab m: A is n x L (n = RP->n, L <= n), its columns l and l+1
+ * rotated for j = m-1 .. 0 and l = L-2-j .. 0 with RP's s(l, j), c(l, j).
+ * RP is a triangle plan. */
+void ft_kernel_tet_hi2lo(const ft_rotation_plan *RP, const int L, const int m, double *A);
+void ft_kernel_tet_lo2hi(const ft_rotation_plan *RP, const int L, const int m, double *A);
+
 /* fasttransforms.h:349-350 (drivers.c:58-92) -- spherical harmonics.
  * A is n x M (M odd, M <= 2n-1), updated in place; B is the caller's n x M
  * scratch, unused here (the GPU path needs no host scratch). */
@@ -97,6 +104,18 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
  * M <= n. */
 void ft_execute_rectdisk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
 void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:364-365 (drivers.c:510-533) -- Proriol^3 (tetrahedron):
+ * A is N x L x M (slab m = the N x L block at N*L*m), RP1 and RP2 triangle
+ * plans of size N (the reference's are rottriangle(N, alpha, beta,
+ * gamma+delta+1) and rottriangle(N, beta, gamma, delta)).  hi2lo: in every
+ * slab m, the triangle's rotations of order l+m on its column l (l < L-m)
+ * with RP1, then the block rotations of ft_kernel_tet_hi2lo with RP2; lo2hi
+ * the transposes in reverse order.  L <= N; other shapes set FT_GPU_EINVAL. */
+void ft_execute_tet_hi2lo(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, double *A, const int L,
+                          const int M);
+void ft_execute_tet_lo2hi(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, double *A, const int L,
+                          const int M);
 
 /* fasttransforms.h:371-385 -- the harmonic plan: rotation plans, the 1D
  * connection matrices P and their inverses Pinv (host FP64, n x n
@@ -214,6 +233,13 @@ enum { FT_GPU_SPH = 0, FT_GPU_TRI = 1, FT_GPU_DISK = 2, FT_GPU_SPHV = 3, FT_GPU_
  * outside any HIP-graph capture before capturing. */
 int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int precision, void *dA, int M,
                    void *stream);
+
+/* ft_execute_tet_{hi2lo,lo2hi} (direction 0, 1) on a device-resident FP64
+ * N x L x M array dA, asynchronously on a HIP stream, as ft_gpu_execute (the
+ * same rules for shared schedules and the first call).  Two launches of the
+ * band-systolic engine: the slab columns' rotations and the block rotations. */
+int ft_gpu_execute_tet(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, int direction, double *dA, int L,
+                       int M, void *stream);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP-graph-capture discipline, tight serving loops,

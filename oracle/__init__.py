@@ -54,6 +54,8 @@ def _load():
             for d in ("hi2lo", "lo2hi"):
                 k = getattr(lib, "oracle_kernel_%s_%s" % (name, d))
                 k.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+        lib.oracle_execute_tet.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
+        lib.oracle_kernel_tet.argtypes = [ctypes.c_int, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
         lib.oracle_execute_q.restype = ctypes.c_int
         lib.oracle_execute_q.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
@@ -150,6 +152,31 @@ def execute_sphv(plan, direction, A, M=None):
         raise ValueError("sphv runs on a sphere plan")
     M = A.shape[1] if M is None else M
     lib.oracle_execute_sphv(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+def _check_tet(plan1, plan2, A, L, M):
+    if plan1.kind != TRI or plan2.kind != TRI or plan1.n != plan2.n:
+        raise ValueError("the tetrahedron takes two triangle plans of one size")
+    if not (0 <= L <= plan1.n) or M < 0:
+        raise ValueError("L must be in [0, n] and M >= 0")
+    if A.dtype != np.float64 or not A.flags["F_CONTIGUOUS"] or A.size < plan1.n * L * M:
+        raise ValueError("A must be a Fortran-ordered float64 array of n*L*M entries")
+
+
+def execute_tet(plan1, plan2, direction, A, L, M):
+    """ft_execute_tet_<direction> restated (drivers.c:510-533; in place on the
+    Fortran-ordered n x L x M array A)."""
+    _check_tet(plan1, plan2, A, L, M)
+    _load().oracle_execute_tet(int(direction), _ptr(plan1.s), _ptr(plan1.c), _ptr(plan2.s), _ptr(plan2.c), plan1.n,
+                               int(L), int(M), _ptr(A))
+    return A
+
+
+def kernel_tet(plan, direction, L, m, A):
+    """ft_kernel_tet_<direction> restated (rotations.c:297-321) on the n x L slab A."""
+    _check_tet(plan, plan, A, L, 1)
+    _load().oracle_kernel_tet(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, int(L), int(m), _ptr(A))
     return A
 
 
@@ -376,6 +403,7 @@ def _load_ref():
         for name in ("sphv", "rectdisk"):
             getattr(lib, "ref_execute_" + name).argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         lib.ref_select_isa.restype = ctypes.c_int
+        lib.ref_execute_tet.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
         lib.ref_set_num_threads.argtypes = [ctypes.c_int]
         lib.ref_max_threads.restype = ctypes.c_int
         _ref = lib
@@ -420,6 +448,19 @@ def ref_execute_rectdisk(plan, direction, A, M=None):
     _check_matrix(A, plan.n)
     M = A.shape[1] if M is None else M
     lib.ref_execute_rectdisk(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+def ref_execute_tet(plan1, plan_L, direction, A, L, M):
+    """The reference's scalar ft_execute_tet_* through its compiled triangle
+    kernels: plan1 of size n for the slab columns, plan_L the triangle plan of
+    size L with the block rotations' parameters (oracle/ref_exec.c)."""
+    lib = _load_ref()
+    if plan_L.n != L:
+        raise ValueError("plan_L must have size L")
+    _check_tet(plan1, plan1, A, L, M)
+    lib.ref_execute_tet(int(direction), _ptr(plan1.s), _ptr(plan1.c), _ptr(plan_L.s), _ptr(plan_L.c), plan1.n,
+                        int(L), int(M), _ptr(A))
     return A
 
 

@@ -25,9 +25,11 @@
  *   executors    src/drivers.c:94-110 (sphere), :314-326 (triangle), :414-430 (disk):
  *                orders 0-1 (sphere, disk) and 0 (triangle) are untouched;
  *                :204-220 (vector sphere: the sphere kernels on columns 2m+1,
- *                2m+2 of orders 2..M/2-1) and :496-508 (rectangle-disk: its
+ *                2m+2 of orders 2..M/2-1), :496-508 (rectangle-disk: its
  *                kernels, rotations_default.c:73-85, which are the sphere's
- *                loop nests, on column m of orders 2..M-1).
+ *                loop nests, on column m of orders 2..M-1) and :510-533
+ *                (tetrahedron: per N x L slab m, the triangle kernel on its
+ *                columns and the block rotations of rotations.c:297-321).
  *   fills        src/ftutilities.c:67-113 (valid triangle of each layout), but
  *                seeded with a counter-based SplitMix64 instead of libc rand().
  *   FP32         the same loop nests in float (s, c rounded once from the FP64
@@ -461,6 +463,41 @@ void oracle_execute_rectdisk(int dir, const double *s, const double *c, int n, d
         double *col = A + (long)n * m;
         if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
         else oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+    }
+}
+
+/* rotations.c:297-321: the block rotations of the n x L slab A of order m,
+ * columns l and l+1 (every row k) for j = m-1 .. 0, l = L-2-j .. 0 (hi2lo),
+ * the transposes in the opposite order (lo2hi); (s, c) of the triangle plan
+ * of size n. */
+void oracle_kernel_tet(int dir, const double *s, const double *c, int n, int L, int m, double *A) {
+    if (dir == 0) {
+        for (int j = m - 1; j >= 0; j--)
+            for (int l = L - 2 - j; l >= 0; l--)
+                for (int k = 0; k < n; k++)
+                    givens(s[idx_tri(n, l, j)], c[idx_tri(n, l, j)], A + k + (long)n * l, A + k + (long)n * (l + 1));
+    } else {
+        for (int j = 0; j < m; j++)
+            for (int l = 0; l <= L - 2 - j; l++)
+                for (int k = 0; k < n; k++)
+                    givens_t(s[idx_tri(n, l, j)], c[idx_tri(n, l, j)], A + k + (long)n * l, A + k + (long)n * (l + 1));
+    }
+}
+
+/* drivers.c:510-533: for every slab m < M of the n x L x M array, hi2lo
+ * rotates its column l (l < L-m) with the triangle kernel of orders l+m .. 1
+ * of plan 1 (rotations_default.c:45-50), then runs the block rotations with
+ * plan 2; lo2hi the block rotations first, then the triangle's transposes. */
+void oracle_execute_tet(int dir, const double *s1, const double *c1, const double *s2, const double *c2, int n, int L,
+                        int M, double *A) {
+    _Pragma("omp parallel for schedule(dynamic, 1)") for (int m = 0; m < M; m++) {
+        double *slab = A + (long)n * L * m;
+        if (dir == 1) oracle_kernel_tet(1, s2, c2, n, L, m, slab);
+        for (int l = 0; l < L - m; l++) {
+            if (dir == 0) oracle_kernel_tri_hi2lo(s1, c1, n, 0, l + m, slab + (long)n * l, 1);
+            else oracle_kernel_tri_lo2hi(s1, c1, n, 0, l + m, slab + (long)n * l, 1);
+        }
+        if (dir == 0) oracle_kernel_tet(0, s2, c2, n, L, m, slab);
     }
 }
 

@@ -153,5 +153,28 @@ void ref_execute_rectdisk(int dir, double *s, double *c, int n, double *A, int M
         k(&RP, m % 2, m, A + (long)n * m, 1);
 }
 
+/* ft_execute_tet_{hi2lo,lo2hi} (drivers.c:510-533) around the reference's
+ * compiled triangle kernels (kernel_tri_*_default).  The slab columns run
+ * through them as drivers.c:514 does (plan 1, size n).  The block rotations
+ * (rotations.c:297-321, which cannot be compiled here) run as the same
+ * compiled kernel on every row k of the slab at stride n, with the table of
+ * the triangle plan of size L (s2, c2): rotations.c:297-321 applies, on each
+ * row, j = m-1 .. 0 and l = L-2-j .. 0 with s(l, j), the loop nest of
+ * rotations_default.c:45-50 for a plan of size L, whose entries are those of
+ * the size-n plan (rotations.c:143-160 does not read the size). */
+void ref_execute_tet(int dir, double *s1, double *c1, double *s2, double *c2, int n, int L, int M, double *A) {
+    ft_rotation_plan RP1 = {s1, c1, n}, RP2 = {s2, c2, L};
+    ref_kernel tri = pick(1, dir, REF_DEFAULT);
+    #pragma omp parallel for schedule(static, 1)
+    for (int m = 0; m < M; m++) {
+        double *slab = A + (long)n * L * m;
+        if (dir == 1)
+            for (int k = 0; k < n; k++) tri(&RP2, 0, m, slab + k, n);
+        for (int l = 0; l < L - m; l++) tri(&RP1, 0, l + m, slab + (long)n * l, 1);
+        if (dir == 0)
+            for (int k = 0; k < n; k++) tri(&RP2, 0, m, slab + k, n);
+    }
+}
+
 void ref_set_num_threads(int t) { omp_set_num_threads(t); }
 int ref_max_threads(void) { return omp_get_max_threads(); }

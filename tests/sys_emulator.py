@@ -58,12 +58,16 @@ def host_build(lib, plan_ptr, direction, S, M, maxpart=16, layout="std"):
 
 
 def emulate(bands, tab, chains, S, A):
-    """Runs every chain part on the n x M Fortran-ordered array A (in place),
-    in list order (the parts of a chain are listed in chain order).  A part
-    after the first of its chain (split_in >= 0) reads its whole stream from
-    A, where the part before it (split_out >= 0) stored every row it emitted:
-    what the kernel hands over through A and a progress counter."""
+    """Runs every chain part on the Fortran-ordered array A (in place), in
+    list order (the parts of a chain are listed in chain order): element e of
+    column c of a chain is A's entry aoff + c*lda + e in Fortran order, as the
+    kernel addresses it.  A part after the first of its chain (split_in >= 0)
+    reads its whole stream from A, where the part before it (split_out >= 0)
+    stored every row it emitted: what the kernel hands over through A and a
+    progress counter."""
     B = LANES * S
+    F = A.reshape(-1, order="F")
+    assert np.shares_memory(F, A)
     for ch in chains:
         cols = np.array(list(ch.col)).reshape(4, 2)
         valid = cols >= 0
@@ -82,7 +86,7 @@ def emulate(bands, tab, chains, S, A):
             for p in range(min(steps + 2, R + 1)):
                 if p < npa:
                     e = rstep * row_of(p) + rpar
-                    inp[p][valid] = A[e, cols[valid]]
+                    inp[p][valid] = F[ch.aoff + cols[valid] * ch.lda + e]
                 else:
                     inp[p] = prev[p]
             q = np.arange(B)
@@ -116,9 +120,38 @@ def emulate(bands, tab, chains, S, A):
                         nxt[P] = E[B - 1].copy()
                     else:
                         e = rstep * row_of(po) + rpar
-                        A[e, cols[valid]] = E[B - 1][valid]
+                        F[ch.aoff + cols[valid] * ch.lda + e] = E[B - 1][valid]
             prev = nxt
     return A
+
+
+def host_build_tet(lib, plan_ptr, direction, S, part, N, L, m0, count, maxpart=16):
+    """(bands, table, chain parts) of one of the tetrahedron's stages as the
+    library builds them (ftamd_sys_debug_tet): part 0 the slab columns on the
+    triangle plan of size N, part 1 the slab rows on the plan of size L."""
+    fn = lib.ftamd_sys_debug_tet
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 8 + [
+        ctypes.POINTER(ctypes.POINTER(Band)), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_longlong),
+        ctypes.POINTER(ctypes.POINTER(Chain)), ctypes.POINTER(ctypes.c_int)]
+    pb, nb = ctypes.POINTER(Band)(), ctypes.c_int()
+    pt, ne = ctypes.POINTER(ctypes.c_double)(), ctypes.c_longlong()
+    pc, nc = ctypes.POINTER(Chain)(), ctypes.c_int()
+    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, part, N, L, m0, count, maxpart, ctypes.byref(pb),
+            ctypes.byref(nb), ctypes.byref(pt), ctypes.byref(ne), ctypes.byref(pc), ctypes.byref(nc))
+    if rc != 0:
+        raise RuntimeError("ftamd_sys_debug_tet failed: %d" % rc)
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    try:
+        bands = [Band.from_buffer_copy(pb[i]) for i in range(nb.value)]
+        tab = np.ctypeslib.as_array(pt, shape=(max(ne.value, 1) * 2,)).copy().reshape(-1, 2)
+        chains = [Chain.from_buffer_copy(pc[i]) for i in range(nc.value)]
+    finally:
+        for p in (pb, pt, pc):
+            libc.free(ctypes.cast(p, ctypes.c_void_p))
+    return bands, tab, chains
 
 
 class Item(ctypes.Structure):
