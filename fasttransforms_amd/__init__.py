@@ -94,6 +94,12 @@ for _dn in ("hi2lo", "lo2hi"):
     SIGNATURES["ft_execute_tet_" + _dn] = (None, [_vp, _vp, _vp, _i, _i])
     SIGNATURES["ft_kernel_tet_" + _dn] = (None, [_vp, _i, _i, _vp])
 SIGNATURES["ft_gpu_execute_tet"] = (_i, [_vp, _vp, _i, _vp, _i, _i, _vp])
+SIGNATURES["ft_plan_rotspinsphere"] = (_vp, [_i, _i])
+SIGNATURES["ft_destroy_spin_rotation_plan"] = (None, [_vp])
+SIGNATURES["ft_gpu_execute_spinsph"] = (_i, [_vp, _i, _vp, _i, _vp])
+for _dn in ("hi2lo", "lo2hi"):
+    SIGNATURES["ft_execute_spinsph_" + _dn] = (None, [_vp, _vp, _vp, _i])
+    SIGNATURES["ft_kernel_spinsph_" + _dn] = (None, [_vp, _i, _vp, _i])
 
 for _name, (_res, _args) in SIGNATURES.items():
     _fn = getattr(_lib, _name)
@@ -161,6 +167,121 @@ class RotationPlan:
         ptr, self._ptr = getattr(self, "_ptr", None), None
         if ptr:
             self._destroy(ptr)
+
+
+class _SpinHead(ctypes.Structure):
+    # The reference's leading members {double *s1, *c1, *s2, *c2; int n, s;}.
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("s1", _dp), ("c1", _dp), ("s2", _dp), ("c2", _dp), ("n", ctypes.c_int), ("s", ctypes.c_int)]
+
+
+class SpinRotationPlan:
+    """Owner of an ``ft_spin_rotation_plan *`` (destroyed with the object)."""
+
+    def __init__(self, ptr, n, s):
+        if not ptr:
+            _check()
+            raise GPUError("plan construction failed")
+        self._ptr = ptr
+        self._destroy = _lib.ft_destroy_spin_rotation_plan
+        self.n, self.s = n, s
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def _table(self, name):
+        head = ctypes.cast(self._ptr, ctypes.POINTER(_SpinHead)).contents
+        size = self.n * self.n * (1 if name in ("s1", "c1") else abs(self.s))
+        if size == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(getattr(head, name), shape=(size,)).copy()
+
+    s1 = property(lambda self: self._table("s1"))
+    c1 = property(lambda self: self._table("c1"))
+    s2 = property(lambda self: self._table("s2"))
+    c2 = property(lambda self: self._table("c2"))
+
+    def __del__(self):
+        ptr, self._ptr = getattr(self, "_ptr", None), None
+        if ptr:
+            self._destroy(ptr)
+
+
+def plan_rotspinsphere(n, s):
+    """ft_plan_rotspinsphere: spin-weighted spherical harmonics of spin s."""
+    return SpinRotationPlan(_lib.ft_plan_rotspinsphere(int(n), int(s)), int(n), int(s))
+
+
+def _complex_pointer(A, n, M):
+    """Address of an n x M complex column-major buffer: a Fortran-ordered
+    complex128 numpy array, or a contiguous torch complex128 tensor of shape
+    (M, n) (row j = column j) or of n*M entries."""
+    if _is_torch(A):
+        import torch
+        if A.dtype != torch.complex128 or not A.is_contiguous() or A.numel() < n * M:
+            raise ValueError("expected a contiguous complex128 tensor of n*M entries")
+        if A.dim() == 2 and A.shape[1] != n:
+            raise ValueError("expected a (M, n) tensor")
+        return _vp(A.data_ptr())
+    if not isinstance(A, np.ndarray) or A.dtype != np.complex128:
+        raise TypeError("expected a complex128 numpy array")
+    if A.ndim == 2 and (A.shape[0] != n or not A.flags.f_contiguous):
+        raise ValueError("expected a Fortran-ordered (n, M) array")
+    if A.ndim != 2 and not A.flags.c_contiguous:
+        raise ValueError("expected a contiguous buffer")
+    if A.size < n * M:
+        raise ValueError("array has fewer than n*M entries")
+    return A.ctypes.data_as(_vp)
+
+
+def _execute_spin(direction):
+    fn = getattr(_lib, "ft_execute_spinsph_" + direction)
+
+    def run(plan, A, M=None):
+        M = int(A.shape[1]) if M is None and A.ndim == 2 else (A.size // plan.n if M is None else int(M))
+        fn(plan.ptr, _complex_pointer(A, plan.n, M), None, M)
+        _check()
+        return A
+
+    run.__name__ = "execute_spinsph_" + direction
+    return run
+
+
+def _kernel_spin(direction):
+    fn = getattr(_lib, "ft_kernel_spinsph_" + direction)
+
+    def run(plan, m, A, S=1):
+        span = (plan.n - 1) * int(S) + 1
+        if A.size < span:
+            raise ValueError("buffer shorter than (n-1)*S+1")
+        fn(plan.ptr, int(m), _complex_pointer(A.reshape(-1), span, 1), int(S))
+        _check()
+        return A
+
+    run.__name__ = "kernel_spinsph_" + direction
+    return run
+
+
+execute_spinsph_hi2lo = _execute_spin("hi2lo")
+execute_spinsph_lo2hi = _execute_spin("lo2hi")
+kernel_spinsph_hi2lo = _kernel_spin("hi2lo")
+kernel_spinsph_lo2hi = _kernel_spin("lo2hi")
+
+
+def gpu_execute_spinsph(plan, direction, A, M=None, stream=None):
+    """Asynchronous ft_gpu_execute_spinsph on a contiguous torch complex128
+    tensor (shape (M, n): row j is column j)."""
+    import torch
+    direction = _DIR.get(direction, direction)
+    M = (A.shape[0] if A.dim() == 2 else A.numel() // plan.n) if M is None else int(M)
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    rc = _lib.ft_gpu_execute_spinsph(plan.ptr, int(direction), _complex_pointer(A, plan.n, M), M,
+                                     _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_spinsph: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
 
 
 def plan_rotsphere(n):

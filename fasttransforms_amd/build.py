@@ -25,7 +25,7 @@ CC = os.environ.get("CC", "gcc")
 
 C_SOURCES = ["ft_plans.c", "ft_drivers.c", "ft_systolic.c", "ft_tuning.c", "ft_connection.c", "ft_harmonic.c"]
 HIP_SOURCES = ["ft_rotations.hip", "ft_k_sph.hip", "ft_k_tri.hip", "ft_k_disk.hip", "ft_k_sys.hip", "ft_noalloc.hip",
-               "ft_trmm.hip"]
+               "ft_trmm.hip", "ft_k_spin.hip"]
 
 INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
 C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wextra"]

@@ -1353,12 +1353,9 @@ static int single_schedule(ft_rotation_plan *RP, int kind, int dir, int jlo, int
  * m2-2*step, ... >= m1; lo2hi runs j = m1, m1+step, ... < m2.  Sequences whose
  * row range is empty (j beyond the plan) do nothing, as in the reference.
  */
-static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, int m2, double *A, int S) {
-    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
-    if (!RP || !A || RP->kind != kind || S < 1 || m1 < 0 || (long long)(RP->n - 1) * S > 0x7fffffffLL) {
-        ftamd_set_error(FT_GPU_EINVAL);
-        return;
-    }
+/* The sequences kernel_<kind>_<dir>(RP, m1, m2, ., S) applies: *jlo .. *jhi
+ * in steps of the geometry's (empty if *jhi < *jlo), those with rows only. */
+static void kernel_range(const ft_rotation_plan *RP, int kind, int dir, int m1, int m2, int *jlo_, int *jhi_) {
     int step = geo_step(kind), n = RP->n;
     int jlo, jhi;
     if (dir == 0) {
@@ -1373,6 +1370,29 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
     /* keep sequences with rows; j must also index an existing table column */
     int jcap = kind == FTAMD_DISK ? 2 * n - 2 : n - 1;
     while (jhi >= jlo && (geo_top(kind, n, jhi) < 0 || jhi > jcap)) jhi -= step;
+    *jlo_ = jlo;
+    *jhi_ = jhi;
+}
+
+/* One vector at device address d with stride S (caller holds RP->exec_lock). */
+static int kernel_dev(ft_rotation_plan *RP, int kind, int dir, int m1, int m2, void *d, int S) {
+    int jlo, jhi;
+    kernel_range(RP, kind, dir, m1, m2, &jlo, &jhi);
+    if (jhi < jlo) return FTAMD_OK;
+    ft_schedule *Sd = NULL;
+    int rc = single_schedule(RP, kind, dir, jlo, jhi, S, &Sd);
+    if (rc == FTAMD_OK) rc = launch_checked(Sd, 0, kind, dir, d);
+    return rc;
+}
+
+static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, int m2, double *A, int S) {
+    ft_rotation_plan *RP = (ft_rotation_plan *)RP_;
+    if (!RP || !A || RP->kind != kind || S < 1 || m1 < 0 || (long long)(RP->n - 1) * S > 0x7fffffffLL) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int n = RP->n, jlo, jhi;
+    kernel_range(RP, kind, dir, m1, m2, &jlo, &jhi);
     if (jhi < jlo) return;
     int rc = ftamd_hip_device(NULL, NULL);
     if (rc != FTAMD_OK) {
@@ -1389,9 +1409,7 @@ static void kernel_one(const ft_rotation_plan *RP_, int kind, int dir, int m1, i
         d = RP->d_stage;
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, span, NULL);
     }
-    ft_schedule *Sd = NULL;
-    if (rc == FTAMD_OK) rc = single_schedule(RP, kind, dir, jlo, jhi, S, &Sd);
-    if (rc == FTAMD_OK) rc = launch_checked(Sd, 0, kind, dir, d);
+    if (rc == FTAMD_OK) rc = kernel_dev(RP, kind, dir, m1, m2, d, S);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
     pthread_mutex_unlock(&RP->exec_lock);
@@ -1423,6 +1441,208 @@ void ft_kernel_rectdisk_hi2lo(const ft_rotation_plan *RP, const int m1, const in
 }
 void ft_kernel_rectdisk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S) {
     kernel_one(RP, FTAMD_SPH, 1, m1, m2, A, S);
+}
+
+/* ----------------------------------------------- spin-weighted sphere -- */
+
+/* The second stage's band-systolic schedule on the spin plan's sphere plan
+ * (cached there; caller holds no lock). */
+static int spin_schedule(ft_spin_rotation_plan *P, int dir, int M, ft_schedule **out) {
+    ft_rotation_plan *R = P->sph;
+    *out = NULL;
+    const int Sq = ftamd_sys_choose_S(FTAMD_SPH, R->n, 0);
+    if (Sq == 0) return FTAMD_EINVAL; /* a build without the band-systolic engine */
+    ftamd_choice choice;
+    ftamd_choose(FTAMD_SPH, 0, dir, 0, R->n, &choice);
+    const int maxpart = choice.maxpart;
+    pthread_mutex_lock(&R->lock);
+    int rc = ftamd_plan_upload(R, 0);
+    ft_schedule *S = NULL;
+    for (ft_schedule *x = rc == FTAMD_OK ? R->scheds : NULL; x; x = x->next)
+        if (x->sys && x->layout == FTAMD_LAYOUT_SPIN && x->M == M && x->dir == dir && x->precision == 0 &&
+            x->maxpart == maxpart && x->order0 == P->s) {
+            S = x;
+            break;
+        }
+    if (rc == FTAMD_OK && !S && !R->sys[dir][0]) rc = ftamd_sys_tables_build(R, dir, 0, Sq, &R->sys[dir][0]);
+    if (rc == FTAMD_OK && !S) {
+        const ft_sys_tables *T = R->sys[dir][0];
+        FtamdSysChain *ch = NULL;
+        int nch = 0, nprog = 0;
+        rc = ftamd_sys_chains_spin(R, T, P->s, M, maxpart, &ch, &nch, &nprog);
+        if (rc == FTAMD_OK) {
+            S = calloc(1, sizeof(ft_schedule));
+            if (!S) rc = FTAMD_ENOMEM;
+        }
+        if (rc == FTAMD_OK) {
+            S->sys = 1;
+            S->M = M;
+            S->layout = FTAMD_LAYOUT_SPIN;
+            S->dir = dir;
+            S->precision = 0;
+            S->maxpart = maxpart;
+            S->order0 = P->s;
+            rc = sys_upload(S, ch, nch, nprog, ftamd_sys_tables_host_bands(T), NULL, 0, NULL);
+            if (rc != FTAMD_OK) {
+                free(S);
+                S = NULL;
+            }
+        }
+        free(ch);
+        if (rc == FTAMD_OK) {
+            FtamdSysLaunch *a = &S->sys_launch;
+            a->bands = ftamd_sys_tables_dev_bands(T);
+            a->rowmask = 1; /* a lane row is one complex column: one order */
+            a->tab = ftamd_sys_tables_dev_tab(T);
+            S->L = ftamd_sys_tables_S(T);
+            S->next = R->scheds;
+            R->scheds = S;
+        }
+    }
+    pthread_mutex_unlock(&R->lock);
+    *out = S;
+    return rc;
+}
+
+/* An n x M complex matrix of the executors' layout: M odd (drivers.c:687-705
+ * touches column 2*(M/2), which is M for even M), M <= 2n-1 (orders up to
+ * n-1), and 2nM doubles indexable in 32 bits by the chain records. */
+static int spin_valid(const ft_spin_rotation_plan *P, int M) {
+    return P && P->sph && M >= 0 && (M == 0 || (M % 2 == 1 && M <= 2 * P->n - 1)) &&
+           2LL * P->n * M <= 0x7fffffffLL;
+}
+
+static int spin_upload(ft_spin_rotation_plan *P) {
+    pthread_mutex_lock(&P->lock);
+    int rc = ftamd_spin_upload(P);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+/* Both stages on the device matrix d (drivers.c:687-705 with rotations_default.c:
+ * 87-117): hi2lo the first stage on columns 1..M-1 (order 0 has none), then
+ * the sphere's sequences; lo2hi the reverse.  checked: synchronous, with the
+ * error word read. */
+static int spin_run(ft_spin_rotation_plan *P, int dir, void *d, int M, void *stream, int checked) {
+    ft_schedule *S = NULL;
+    int rc = spin_upload(P);
+    if (rc == FTAMD_OK) rc = spin_schedule(P, dir, M, &S);
+    for (int stage = 0; stage < 2 && rc == FTAMD_OK; stage++) {
+        if ((stage == 0) == (dir == 0)) {
+            rc = ftamd_hip_spin_stage1(P->d_tab2, P->n, P->s, (double *)d + 2LL * P->n, 1, M - 1, P->n, 1, 0, 0,
+                                       dir, checked ? NULL : stream);
+        } else if (checked) {
+            rc = launch_checked(S, 0, FTAMD_SPH, dir, d);
+        } else {
+            rc = launch(S, 0, FTAMD_SPH, dir, d, stream);
+            if (rc == FTAMD_OK) pending_add(S, stream);
+        }
+    }
+    if (rc == FTAMD_OK && checked) rc = ftamd_hip_sync(NULL);
+    return rc;
+}
+
+int ft_gpu_execute_spinsph(const ft_spin_rotation_plan *P_, int direction, void *dA, int M, void *stream) {
+    ft_spin_rotation_plan *P = (ft_spin_rotation_plan *)P_;
+    if (!spin_valid(P, M) || !dA || (direction != 0 && direction != 1)) return FT_GPU_EINVAL;
+    if (M == 0) return FT_GPU_OK;
+    int rc = spin_run(P, direction, dA, M, stream, 0);
+    if (rc == FTAMD_OK) t_last_engine = FT_GPU_ENGINE_SYSTOLIC;
+    return rc;
+}
+
+/* The synchronous executors: host or device A (n x M ft_complex), staged
+ * through the sphere plan's buffer under its execution lock. */
+static void execute_spin(const ft_spin_rotation_plan *P_, int dir, ft_complex *A, int M) {
+    ft_spin_rotation_plan *P = (ft_spin_rotation_plan *)P_;
+    if (!spin_valid(P, M) || !A) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    if (M == 0) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    ft_rotation_plan *R = P->sph;
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(A, &pk);
+    size_t bytes = 2 * (size_t)P->n * (size_t)M * sizeof(double);
+    pthread_mutex_lock(&R->exec_lock);
+    void *d = A;
+    if (pk != FTAMD_PTR_DEVICE) {
+        rc = stage_reserve(R, bytes);
+        d = R->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = spin_run(P, dir, d, M, NULL, 1);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, bytes, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&R->exec_lock);
+    ftamd_set_error(rc);
+}
+
+/* drivers.c:645-681 */
+void ft_execute_spinsph_hi2lo(const ft_spin_rotation_plan *P, ft_complex *A, ft_complex *B, const int M) {
+    (void)B;
+    execute_spin(P, 0, A, M);
+}
+void ft_execute_spinsph_lo2hi(const ft_spin_rotation_plan *P, ft_complex *A, ft_complex *B, const int M) {
+    (void)B;
+    execute_spin(P, 1, A, M);
+}
+
+/* rotations.c:477-483 (rotations_default.c:87-117): one complex vector of
+ * order m with stride S (complex entries).  The second stage is the sphere's
+ * kernel of orders d .. d%2 (d = |am - as|) on its real and imaginary parts,
+ * real vectors of stride 2S. */
+static void kernel_spin(const ft_spin_rotation_plan *P_, int dir, int m, ft_complex *A, int S) {
+    ft_spin_rotation_plan *P = (ft_spin_rotation_plan *)P_;
+    if (!P || !P->sph || !A || S < 1 || 2LL * (P->n - 1) * S + 1 > 0x7fffffffLL || m <= -0x40000000 ||
+        m >= 0x40000000) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    ft_rotation_plan *R = P->sph;
+    const int as = P->s < 0 ? -P->s : P->s, am = m < 0 ? -m : m;
+    const int dd = am > as ? am - as : as - am;
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(A, &pk);
+    size_t span = 2 * ((size_t)(P->n - 1) * (size_t)S + 1) * sizeof(double);
+    pthread_mutex_lock(&R->exec_lock);
+    void *d = A;
+    if (pk != FTAMD_PTR_DEVICE) {
+        rc = stage_reserve(R, span);
+        d = R->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, A, span, NULL);
+    }
+    if (rc == FTAMD_OK) rc = spin_upload(P);
+    for (int stage = 0; stage < 2 && rc == FTAMD_OK; stage++) {
+        if ((stage == 0) == (dir == 0)) {
+            rc = ftamd_hip_spin_stage1(P->d_tab2, P->n, P->s, d, 0, 1, 0, S, 1, m, dir, NULL);
+            if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+        } else {
+            for (int part = 0; part < 2 && rc == FTAMD_OK; part++)
+                rc = kernel_dev(R, FTAMD_SPH, dir, dd % 2, dd, (double *)d + part, 2 * S);
+        }
+    }
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(A, d, span, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&R->exec_lock);
+    ftamd_set_error(rc);
+}
+
+void ft_kernel_spinsph_hi2lo(const ft_spin_rotation_plan *P, const int m, ft_complex *A, const int S) {
+    kernel_spin(P, 0, m, A, S);
+}
+void ft_kernel_spinsph_lo2hi(const ft_spin_rotation_plan *P, const int m, ft_complex *A, const int S) {
+    kernel_spin(P, 1, m, A, S);
 }
 
 /* ------------------------------------------------------------ batch -- */

@@ -17,8 +17,10 @@ enum { FTAMD_SPH = 0, FTAMD_TRI = 1, FTAMD_DISK = 2 };
  * (orders 2..M/2-1, columns 2m+1 and 2m+2), the rectangle-disk (orders
  * 2..M-1, column m).  Triangle and disk matrices have one layout each. */
 enum { FTAMD_LAYOUT_STD = 0, FTAMD_LAYOUT_SPHV = 1, FTAMD_LAYOUT_RECTDISK = 2 };
-/* The tetrahedron's two stages on triangle plans: the slab columns, the slab rows. */
-enum { FTAMD_LAYOUT_TET = 3, FTAMD_LAYOUT_TETBLOCK = 4 };
+/* The tetrahedron's two stages on triangle plans: the slab columns, the slab
+ * rows; the spin-weighted sphere's second stage on a sphere plan (complex
+ * columns, real and imaginary parts in one lane row). */
+enum { FTAMD_LAYOUT_TET = 3, FTAMD_LAYOUT_TETBLOCK = 4, FTAMD_LAYOUT_SPIN = 5 };
 enum { FTAMD_OK = 0, FTAMD_EINVAL = 1, FTAMD_ENOMEM = 2, FTAMD_EDEVICE = 3, FTAMD_ETIMEOUT = 4, FTAMD_EPTR = 5,
        FTAMD_EINCOMPLETE = 6 };
 enum { FTAMD_PTR_HOST = 0, FTAMD_PTR_DEVICE = 1 };
@@ -182,6 +184,13 @@ typedef struct {
     int ctiles[4];
 } FtamdTrmmLaunch;
 int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream);
+/* The spin-weighted sphere's first stage (ft_k_spin.hip) on the complex
+ * columns c0 .. c0+count-1 of A (interleaved re, im): column c at element
+ * 2*(col_stride*(c - c0)) with rows at complex stride row_stride, of order
+ * m(c) (the executors' layout: 0, -1, 1, -2, 2, ...) or of order fixed_m for
+ * every column if use_fixed.  tab2: the plan's (s2, c2) pairs. */
+int ftamd_hip_spin_stage1(const double *tab2, int n, int s, double *A, int c0, int count, long long col_stride,
+                          int row_stride, int use_fixed, int fixed_m, int dir, void *stream);
 /* A[i + j n] *= (j == 0 ? s1 : 1) * (i == 0 ? s1 : 1) * s2, in that order,
  * for the n x M matrix A (the 2D Chebyshev normalization, drivers.c:6-24). */
 int ftamd_hip_cheb_scale(double *A, int n, int M, double s1, double s2, void *stream);

@@ -108,6 +108,27 @@ void ftamd_choose(int kind, int precision, int dir, int batch, int n, ftamd_choi
 int ftamd_tuned_lookup(int kind, int precision, int dir, int batch, int n, int *engine, int *maxpart, int *L, int *W,
                        int *wg_per_cu);
 
+/* The reference's layout (ftinternal.h:425-432) first, then device state.
+ * The sphere tables s1, c1 are those of the sphere plan (rotations.c:116-133
+ * and 443-455 evaluate one closed form), which runs the second stage on the
+ * band-systolic engine; s2, c2 go to the device interleaved for the first. */
+struct ft_spin_rotation_plan_s {
+    double *s1;
+    double *c1;
+    double *s2;
+    double *c2;
+    int n;
+    int s;
+    /* extension */
+    ft_rotation_plan *sph;   /* sphere plan of size n: the second stage's tables, schedules,
+                                staging buffer and execution lock */
+    int device;              /* HIP device of the upload, -1 before */
+    void *d_tab2;            /* (s2, c2) pairs, |s| n^2 of them (NULL if s = 0) */
+    pthread_mutex_t lock;    /* the upload */
+};
+
+int ftamd_spin_upload(ft_spin_rotation_plan *P);
+
 /* ft_systolic.c */
 int ftamd_sys_choose_S(int kind, int n, int precision);
 int ftamd_sys_tables_build(const ft_rotation_plan *RP, int dir, int precision, int S, ft_sys_tables **out);
@@ -127,6 +148,10 @@ int ftamd_sys_chains_tet_fill(const ft_rotation_plan *RP, const ft_sys_tables *T
                               int count, FtamdSysChain *buf, int cap, int *nout);
 int ftamd_sys_chains_tet(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0, int count,
                          int maxpart, FtamdSysChain **out, int *nout, int *nprog);
+int ftamd_sys_chains_spin_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int s, int M, FtamdSysChain *buf,
+                               int cap, int *nout);
+int ftamd_sys_chains_spin(const ft_rotation_plan *RP, const ft_sys_tables *T, int s, int M, int maxpart,
+                          FtamdSysChain **out, int *nout, int *nprog);
 int ftamd_sys_debug_tet(const ft_rotation_plan *RP, int dir, int S, int part, int N, int L, int m0, int count,
                         int maxpart, FtamdSysBand **bands, int *nbands, double **tab, long long *nentries,
                         FtamdSysChain **chains, int *nchains);

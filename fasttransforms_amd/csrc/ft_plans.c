@@ -101,6 +101,102 @@ ft_rotation_plan *ft_plan_rotdisk(const int n, const double alpha, const double 
     return RP;
 }
 
+/* rotations.c:443-475: s1, c1 (n x n, s1(l, m) at l + m*n) the sphere's
+ * closed form, s2, c2 (|s| x n x n, s2(l, j, m) at l + j*n + m*|s|*n) for
+ * l < n-m-j; every term an exact integer, as the reference computes them.
+ * The sphere plan of size n beside it holds the same s1, c1 values in the
+ * sphere's packed layout (the second stage runs on it). */
+ft_spin_rotation_plan *ft_plan_rotspinsphere(const int n, const int s) {
+    if (n < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    const int as = s < 0 ? -s : s;
+    ft_spin_rotation_plan *P = calloc(1, sizeof(ft_spin_rotation_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    const size_t nn = (size_t)n * n, n2 = (size_t)as * n * n;
+    P->s1 = calloc(nn, sizeof(double));
+    P->c1 = calloc(nn, sizeof(double));
+    P->s2 = calloc(n2 > 0 ? n2 : 1, sizeof(double));
+    P->c2 = calloc(n2 > 0 ? n2 : 1, sizeof(double));
+    P->n = n;
+    P->s = s;
+    P->device = -1;
+    pthread_mutex_init(&P->lock, NULL);
+    P->sph = ft_plan_rotsphere(n);
+    if (!P->s1 || !P->c1 || !P->s2 || !P->c2 || !P->sph) {
+        ft_destroy_spin_rotation_plan(P);
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            const double nums = (double)((long long)(l + 1) * (l + 2));
+            const double numc = (double)((long long)(2 * m + 2) * (2 * l + 2 * m + 5));
+            const double den = (double)((long long)(l + 2 * m + 3) * (l + 2 * m + 4));
+            P->s1[l + (size_t)m * n] = sqrt(nums / den);
+            P->c1[l + (size_t)m * n] = sqrt(numc / den);
+        }
+    for (int m = 0; m < n; m++)
+        for (int j = 0; j < as; j++)
+            for (int l = 0; l < n - m - j; l++) {
+                const double nums = (double)((long long)(l + 1) * (l + m + 1));
+                const double numc = (double)((long long)(2 * j + m + 1) * (2 * l + 2 * j + 2 * m + 3));
+                const double den = (double)((long long)(l + 2 * j + m + 2) * (l + 2 * j + 2 * m + 2));
+                const size_t i = l + (size_t)j * n + (size_t)m * as * n;
+                P->s2[i] = sqrt(nums / den);
+                P->c2[i] = sqrt(numc / den);
+            }
+    return P;
+}
+
+/* rotations.c:431-437 -- also frees the device copies and the sphere plan. */
+void ft_destroy_spin_rotation_plan(ft_spin_rotation_plan *P) {
+    if (!P) return;
+    if (P->d_tab2) {
+        ftamd_hip_sync(NULL);
+        ftamd_hip_free(P->d_tab2);
+    }
+    ft_destroy_rotation_plan(P->sph);
+    free(P->s1);
+    free(P->c1);
+    free(P->s2);
+    free(P->c2);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* Uploads the (s2, c2) pairs on first use (caller holds P->lock). */
+int ftamd_spin_upload(ft_spin_rotation_plan *P) {
+    const int as = P->s < 0 ? -P->s : P->s;
+    if (P->d_tab2 || as == 0) return FTAMD_OK;
+    int dev = -1, rc = ftamd_hip_device(&dev, NULL);
+    if (rc != FTAMD_OK) return rc;
+    if (P->device >= 0 && P->device != dev) return FTAMD_EDEVICE;
+    const size_t count = (size_t)as * P->n * P->n;
+    double *host = malloc(2 * count * sizeof(double));
+    if (!host) return FTAMD_ENOMEM;
+    for (size_t i = 0; i < count; i++) {
+        host[2 * i] = P->s2[i];
+        host[2 * i + 1] = P->c2[i];
+    }
+    void *d = NULL;
+    rc = ftamd_hip_malloc(&d, 2 * count * sizeof(double));
+    if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, host, 2 * count * sizeof(double), NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    free(host);
+    if (rc != FTAMD_OK) {
+        ftamd_hip_free(d);
+        return rc;
+    }
+    P->d_tab2 = d;
+    P->device = dev;
+    return FTAMD_OK;
+}
+
 /* rotations.c:270-287.  The sphere's table layout and kernels; each term is
  * summed left to right in double as the reference writes it, the numerator
  * of s an exact integer product. */

@@ -237,8 +237,9 @@ static void sort_chains(FtamdSysChain *c, int n) {
 
 /* The chain record of class c over up to FTAMD_SYS_COLS columns: column
  * col[q] of stride lda from aoff (-1: none) applies the class's sequences from
- * j = jfirst[q] down (hi2lo order; lo2hi up to it), the class's rows at
- * stride rmul * rstep.  0 if no column starts at a sequence of the class. */
+ * j = jfirst[q] down (hi2lo order; lo2hi up to it), the class's row e at
+ * element rmul * (rstep*e + rpar) of the column.  0 if no column starts at a
+ * sequence of the class. */
 static int chain_record(const ft_sys_tables *T, int c, const int *col, const int *jfirst, int lda, int rmul,
                         long long aoff, FtamdSysChain *out) {
     const int B = FTAMD_SYS_LANES * T->S;
@@ -271,7 +272,7 @@ static int chain_record(const ft_sys_tables *T, int c, const int *col, const int
     x.nb = b_last - b_first + 1;
     x.dir = T->dir;
     x.rstep = rmul * cl->rstep;
-    x.rpar = cl->rpar;
+    x.rpar = rmul * cl->rpar;
     x.lda = lda;
     x.aoff = aoff;
     x.split_in = x.split_out = -1;
@@ -556,6 +557,86 @@ int ftamd_sys_chains(const ft_rotation_plan *RP, const ft_sys_tables *T, int M, 
     return rc;
 }
 
+/* Whole chains of the spin-weighted sphere's second stage (rotations_default.c:
+ * 99-101, 105-107) on the n x M complex matrix A (interleaved re, im; column
+ * c of order m = 0, -1, 1, -2, 2, ... as drivers.c:687-705 lays them out) for
+ * spin s, on the sphere plan RP of size n, into buf as ftamd_sys_chains_fill
+ * does: complex column c takes the sphere's sequences of order d = |am - as|
+ * (j = d-2, d-4, ... >= d%2, capped at the class's highest); its real and
+ * imaginary parts are the two columns of a lane row (doubles 2n*c and 2n*c+1,
+ * rows at stride 2), four complex columns of one order parity per chain,
+ * highest orders first. */
+int ftamd_sys_chains_spin_fill(const ft_rotation_plan *RP, const ft_sys_tables *T, int s, int M, FtamdSysChain *buf,
+                               int cap, int *nout) {
+    const int n = RP->n, as = s < 0 ? -s : s;
+    int nch = 0;
+    for (int c = 0; c < T->ncls; c++) {
+        const sys_class *cl = &T->cls[c];
+        /* columns of the class's order parity, by d descending: d's of one
+         * column are at most max(am, as) <= M/2 + as, visited top down */
+        const int dmax = M / 2 + as;
+        int cols[FTAMD_SYS_COLS], jfirst[FTAMD_SYS_COLS], q = 0;
+        for (int d = dmax - ((dmax - cl->jpar) % 2 + 2) % 2; d >= 2; d -= 2) {
+            /* the orders with |am - as| = d: am = as + d and am = as - d, each as -am and +am */
+            int ams[2] = {as + d, as - d}, na = as - d >= 0 && d != 0 ? 2 : 1;
+            for (int a = 0; a < na; a++) {
+                const int am = ams[a];
+                for (int sg = -1; sg <= 1; sg += 2) {
+                    const int col = am == 0 ? (sg < 0 ? 0 : -1) : (sg < 0 ? 2 * am - 1 : 2 * am);
+                    if (col < 0 || col >= M) continue;
+                    const int jf = d - 2 < cl->jtop ? d - 2 : cl->jtop;
+                    cols[q] = 2 * n * col;
+                    cols[q + 1] = 2 * n * col + 1;
+                    jfirst[q] = jfirst[q + 1] = jf;
+                    q += 2;
+                    if (q == FTAMD_SYS_COLS) {
+                        FtamdSysChain x;
+                        if (chain_record(T, c, cols, jfirst, 1, 2, 0, &x)) {
+                            if (nch < cap) buf[nch] = x;
+                            nch++;
+                        }
+                        q = 0;
+                    }
+                }
+            }
+        }
+        if (q > 0) {
+            for (; q < FTAMD_SYS_COLS; q++) cols[q] = -1, jfirst[q] = 0;
+            FtamdSysChain x;
+            if (chain_record(T, c, cols, jfirst, 1, 2, 0, &x)) {
+                if (nch < cap) buf[nch] = x;
+                nch++;
+            }
+        }
+    }
+    *nout = nch;
+    if (nch > cap) return FTAMD_ENOMEM;
+    sort_chains(buf, nch);
+    return FTAMD_OK;
+}
+
+/* The chain parts of the spin-weighted sphere's second stage in a malloc'd
+ * array the caller frees, as ftamd_sys_chains. */
+int ftamd_sys_chains_spin(const ft_rotation_plan *RP, const ft_sys_tables *T, int s, int M, int maxpart,
+                          FtamdSysChain **out, int *nout, int *nprog) {
+    *out = NULL;
+    *nout = 0;
+    *nprog = 0;
+    int need = 0, k = 0;
+    ftamd_sys_chains_spin_fill(RP, T, s, M, NULL, 0, &need);
+    FtamdSysChain *ch = malloc(sizeof(FtamdSysChain) * (size_t)(need > 0 ? need : 1));
+    if (!ch) return FTAMD_ENOMEM;
+    int rc = ftamd_sys_chains_spin_fill(RP, T, s, M, ch, need, &k);
+    if (rc == FTAMD_OK) rc = split_alloc(&ch, k, maxpart, nout, nprog);
+    if (rc != FTAMD_OK) {
+        free(ch);
+        *nout = 0;
+        return rc;
+    }
+    *out = ch;
+    return FTAMD_OK;
+}
+
 /* The chain parts of one of the tetrahedron's stages (ftamd_sys_chains_tet_fill)
  * in a malloc'd array the caller frees, as ftamd_sys_chains. */
 int ftamd_sys_chains_tet(const ft_rotation_plan *RP, const ft_sys_tables *T, int part, int N, int L, int m0, int count,
@@ -657,6 +738,28 @@ int ftamd_sys_batch(const ft_rotation_plan *const *plans, ft_sys_tables *const *
     *chains_out = all;
     *nchains_out = nparts;
     *nprog_out = nprog;
+    return FTAMD_OK;
+}
+
+/* Diagnostics (no device work) of the spin-weighted sphere's second stage:
+ * the FP64 bands and table of the sphere plan RP and the chain parts
+ * ftamd_sys_chains_spin builds, as ftamd_sys_debug_host. */
+int ftamd_sys_debug_spin(const ft_rotation_plan *RP, int dir, int S, int s, int M, int maxpart, FtamdSysBand **bands,
+                         int *nbands, double **tab, long long *nentries, FtamdSysChain **chains, int *nchains) {
+    ft_sys_tables T;
+    memset(&T, 0, sizeof(T));
+    void *host = NULL;
+    int rc = tables_host(RP, dir, 0, S, &T, &host, nentries);
+    int nprog = 0;
+    if (rc == FTAMD_OK) rc = ftamd_sys_chains_spin(RP, &T, s, M, maxpart, chains, nchains, &nprog);
+    if (rc != FTAMD_OK) {
+        free(T.bands);
+        free(host);
+        return rc;
+    }
+    *bands = T.bands;
+    *nbands = T.nbands;
+    *tab = host;
     return FTAMD_OK;
 }
 

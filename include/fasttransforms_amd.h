@@ -23,6 +23,14 @@ extern "C" {
  * RP->n are the host FP64 tables; device copies follow them. */
 typedef struct ft_rotation_plan_s ft_rotation_plan;
 
+/* fasttransforms.h:11 -- a complex number as two doubles (re, im). */
+typedef double ft_complex[2];
+
+/* fasttransforms.h:336.  The first members are the reference's
+ * {double *s1, *c1, *s2, *c2; int n, s;} (ftinternal.h:425-432): the host
+ * FP64 tables in its layout; device state follows them. */
+typedef struct ft_spin_rotation_plan_s ft_spin_rotation_plan;
+
 /* fasttransforms.h:295 -- kept for link compatibility.  No computation here
  * uses host threads, so the value is recorded and otherwise ignored. */
 void ft_set_num_threads(const int n);
@@ -72,6 +80,19 @@ ft_rotation_plan *ft_plan_rotrectdisk(const int n, const double beta);
 void ft_kernel_rectdisk_hi2lo(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 void ft_kernel_rectdisk_lo2hi(const ft_rotation_plan *RP, const int m1, const int m2, double *A, const int S);
 
+/* fasttransforms.h:339, 341 (rotations.c:431-475) -- spin-weighted spherical
+ * harmonics of spin s; tables bit-identical to the reference's. */
+ft_spin_rotation_plan *ft_plan_rotspinsphere(const int n, const int s);
+void ft_destroy_spin_rotation_plan(ft_spin_rotation_plan *SRP);
+
+/* fasttransforms.h:344, 346 (rotations.c:477-483, rotations_default.c:87-117)
+ * -- one complex vector of order m (any sign) with stride S (complex
+ * entries): hi2lo the min(|m|,|s|) sequences of adjacent-row rotations
+ * (s2, c2; their transposes where m*s < 0), then the sphere's sequences of
+ * order |m|-|s| ... (s1, c1); lo2hi the transposes in reverse order. */
+void ft_kernel_spinsph_hi2lo(const ft_spin_rotation_plan *SRP, const int m, ft_complex *A, const int S);
+void ft_kernel_spinsph_lo2hi(const ft_spin_rotation_plan *SRP, const int m, ft_complex *A, const int S);
+
 /* fasttransforms.h:332-333 (rotations.c:297-321) -- the tetrahedron's block
  * rotations of slab m: A is n x L (n = RP->n, L <= n), its columns l and l+1
  * rotated for j = m-1 .. 0 and l = L-2-j .. 0 with RP's s(l, j), c(l, j).
@@ -104,6 +125,12 @@ void ft_execute_disk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, con
  * M <= n. */
 void ft_execute_rectdisk_hi2lo(const ft_rotation_plan *RP, double *A, double *B, const int M);
 void ft_execute_rectdisk_lo2hi(const ft_rotation_plan *RP, double *A, double *B, const int M);
+
+/* fasttransforms.h:367-368 (drivers.c:645-705) -- spin-weighted spherical
+ * harmonics: A is n x M complex (column 0 order 0, columns 2m-1 and 2m
+ * orders -m and m), M odd, M <= 2n-1; B is unused. */
+void ft_execute_spinsph_hi2lo(const ft_spin_rotation_plan *SRP, ft_complex *A, ft_complex *B, const int M);
+void ft_execute_spinsph_lo2hi(const ft_spin_rotation_plan *SRP, ft_complex *A, ft_complex *B, const int M);
 
 /* fasttransforms.h:364-365 (drivers.c:510-533) -- Proriol^3 (tetrahedron):
  * A is N x L x M (slab m = the N x L block at N*L*m), RP1 and RP2 triangle
@@ -240,6 +267,11 @@ int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int prec
  * band-systolic engine: the slab columns' rotations and the block rotations. */
 int ft_gpu_execute_tet(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, int direction, double *dA, int L,
                        int M, void *stream);
+
+/* ft_execute_spinsph_{hi2lo,lo2hi} (direction 0, 1) on a device-resident
+ * n x M complex array dA (interleaved FP64), asynchronously on a HIP stream,
+ * as ft_gpu_execute: the first stage's kernel and one band-systolic launch. */
+int ft_gpu_execute_spinsph(const ft_spin_rotation_plan *SRP, int direction, void *dA, int M, void *stream);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP-graph-capture discipline, tight serving loops,

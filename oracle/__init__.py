@@ -56,6 +56,11 @@ def _load():
                 k.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
         lib.oracle_execute_tet.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
         lib.oracle_kernel_tet.argtypes = [ctypes.c_int, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
+        lib.oracle_plan_spin.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+        lib.oracle_kernel_spinsph.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, _dp, ctypes.c_int]
+        lib.oracle_execute_spinsph.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                               ctypes.c_int]
         lib.oracle_execute_q.restype = ctypes.c_int
         lib.oracle_execute_q.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
@@ -152,6 +157,48 @@ def execute_sphv(plan, direction, A, M=None):
         raise ValueError("sphv runs on a sphere plan")
     M = A.shape[1] if M is None else M
     lib.oracle_execute_sphv(int(direction), _ptr(plan.s), _ptr(plan.c), plan.n, _ptr(A), int(M))
+    return A
+
+
+class SpinPlan:
+    """The spin-weighted sphere's tables as the oracle builds them
+    (rotations.c:443-475): s1, c1 (n x n) and s2, c2 (|s| x n x n), in the
+    reference's layouts."""
+
+    def __init__(self, n, s):
+        self.n, self.s = int(n), int(s)
+        a = abs(self.s)
+        self.s1, self.c1 = np.zeros(n * n), np.zeros(n * n)
+        self.s2, self.c2 = np.zeros(max(a * n * n, 1)), np.zeros(max(a * n * n, 1))
+        _load().oracle_plan_spin(self.n, self.s, _ptr(self.s1), _ptr(self.c1), _ptr(self.s2), _ptr(self.c2))
+
+    def _args(self):
+        return _ptr(self.s1), _ptr(self.c1), _ptr(self.s2), _ptr(self.c2)
+
+
+def _check_complex(A, n, M):
+    if A.dtype != np.complex128 or not A.flags["F_CONTIGUOUS"] or A.shape[0] != n or A.shape[1] < M:
+        raise ValueError("A must be a Fortran-ordered complex128 array of n rows")
+
+
+def execute_spinsph(plan, direction, A, M=None):
+    """ft_execute_spinsph_<direction> restated (drivers.c:687-705; in place
+    on the n x M complex array A)."""
+    M = A.shape[1] if M is None else int(M)
+    _check_complex(A, plan.n, M)
+    _load().oracle_execute_spinsph(int(direction), *plan._args(), plan.n, plan.s,
+                                   A.ctypes.data_as(_dp), M)
+    return A
+
+
+def kernel_spinsph(plan, direction, m, A, S=1):
+    """ft_kernel_spinsph_<direction> restated (rotations_default.c:87-117) on
+    the complex vector A with stride S."""
+    if A.dtype != np.complex128 or not (A.flags["C_CONTIGUOUS"] or A.flags["F_CONTIGUOUS"]) or \
+            A.size < (plan.n - 1) * S + 1:
+        raise ValueError("A must be a contiguous complex128 array of (n-1)*S+1 entries")
+    _load().oracle_kernel_spinsph(int(direction), *plan._args(), plan.n, plan.s, int(m), A.ctypes.data_as(_dp),
+                                  int(S))
     return A
 
 
@@ -404,6 +451,8 @@ def _load_ref():
             getattr(lib, "ref_execute_" + name).argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         lib.ref_select_isa.restype = ctypes.c_int
         lib.ref_execute_tet.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 3 + [_dp]
+        lib.ref_execute_spinsph.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                            ctypes.c_int, _dp, _dp, ctypes.c_int]
         lib.ref_set_num_threads.argtypes = [ctypes.c_int]
         lib.ref_max_threads.restype = ctypes.c_int
         _ref = lib
@@ -461,6 +510,19 @@ def ref_execute_tet(plan1, plan_L, direction, A, L, M):
     _check_tet(plan1, plan1, A, L, M)
     lib.ref_execute_tet(int(direction), _ptr(plan1.s), _ptr(plan1.c), _ptr(plan_L.s), _ptr(plan_L.c), plan1.n,
                         int(L), int(M), _ptr(A))
+    return A
+
+
+def ref_execute_spinsph(plan, direction, A, M=None, isa="default"):
+    """The reference's ft_execute_spinsph_* through its compiled spin kernels:
+    isa "default" the scalar executor, "avx_fma" the dispatched one (scalar
+    orders 0, +-1, fused twisted kernels on the rest)."""
+    lib = _load_ref()
+    M = A.shape[1] if M is None else int(M)
+    _check_complex(A, plan.n, M)
+    B = np.zeros(2 * plan.n * max(M, 1))
+    lib.ref_execute_spinsph(int(direction), 0 if isa == "default" else 3, *plan._args(), plan.n, plan.s,
+                            A.ctypes.data_as(_dp), _ptr(B), M)
     return A
 
 

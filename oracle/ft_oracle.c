@@ -29,7 +29,10 @@
  *                kernels, rotations_default.c:73-85, which are the sphere's
  *                loop nests, on column m of orders 2..M-1) and :510-533
  *                (tetrahedron: per N x L slab m, the triangle kernel on its
- *                columns and the block rotations of rotations.c:297-321).
+ *                columns and the block rotations of rotations.c:297-321);
+ *                :687-705 (spin-weighted sphere: complex columns of orders
+ *                0, -1, 1, -2, 2, ... through rotations_default.c:87-117,
+ *                the plan of rotations.c:443-475).
  *   fills        src/ftutilities.c:67-113 (valid triangle of each layout), but
  *                seeded with a counter-based SplitMix64 instead of libc rand().
  *   FP32         the same loop nests in float (s, c rounded once from the FP64
@@ -463,6 +466,89 @@ void oracle_execute_rectdisk(int dir, const double *s, const double *c, int n, d
         double *col = A + (long)n * m;
         if (dir == 0) oracle_kernel_sph_hi2lo(s, c, n, m % 2, m, col, 1);
         else oracle_kernel_sph_lo2hi(s, c, n, m % 2, m, col, 1);
+    }
+}
+
+/* rotations.c:443-475: the spin-weighted sphere's tables of spin s:
+ * s1, c1 (n x n, (l, m) at l + m*n) and s2, c2 (|s| x n x n, (l, j, m) at
+ * l + j*n + m*|s|*n), every term an exact integer. */
+void oracle_plan_spin(int n, int s, double *s1, double *c1, double *s2, double *c2) {
+    const int as = s < 0 ? -s : s;
+    memset(s1, 0, sizeof(double) * (size_t)n * n);
+    memset(c1, 0, sizeof(double) * (size_t)n * n);
+    memset(s2, 0, sizeof(double) * (size_t)as * n * n);
+    memset(c2, 0, sizeof(double) * (size_t)as * n * n);
+    for (int m = 0; m < n; m++)
+        for (int l = 0; l < n - m; l++) {
+            double nums = (double)((long)(l + 1) * (l + 2));
+            double numc = (double)((long)(2 * m + 2) * (2 * l + 2 * m + 5));
+            double den = (double)((long)(l + 2 * m + 3) * (l + 2 * m + 4));
+            s1[l + (long)m * n] = sqrt(nums / den);
+            c1[l + (long)m * n] = sqrt(numc / den);
+        }
+    for (int m = 0; m < n; m++)
+        for (int j = 0; j < as; j++)
+            for (int l = 0; l < n - m - j; l++) {
+                double nums = (double)((long)(l + 1) * (l + m + 1));
+                double numc = (double)((long)(2 * j + m + 1) * (2 * l + 2 * j + 2 * m + 3));
+                double den = (double)((long)(l + 2 * j + m + 2) * (l + 2 * j + 2 * m + 2));
+                long i = l + (long)j * n + (long)m * as * n;
+                s2[i] = sqrt(nums / den);
+                c2[i] = sqrt(numc / den);
+            }
+}
+
+/* Rotation of the complex entries X, Y: the same real rotation on the real
+ * and the imaginary parts. */
+static inline void givens_c(double S, double C, double *X, double *Y) {
+    givens(S, C, X, Y);
+    givens(S, C, X + 1, Y + 1);
+}
+static inline void givens_tc(double S, double C, double *X, double *Y) {
+    givens_t(S, C, X, Y);
+    givens_t(S, C, X + 1, Y + 1);
+}
+
+/* rotations_default.c:87-117: one complex vector (interleaved re, im; stride
+ * S complex entries) of order m, spin s. */
+void oracle_kernel_spinsph(int dir, const double *s1, const double *c1, const double *s2, const double *c2, int n,
+                           int s, int m, double *A, int S) {
+    const int as = s < 0 ? -s : s, am = m < 0 ? -m : m;
+    const int d = am > as ? am - as : as - am, nseq = am < as ? am : as;
+    const int pos = (long)m * s >= 0;
+#define OR_E(k) (A + 2 * (long)(k) * S)
+    if (dir == 0) {
+        for (int j = nseq - 1; j >= 0; j--)
+            for (int k = n - 2 - d - j; k >= 0; k--) {
+                long i = k + (long)j * n + (long)d * as * n;
+                if (pos) givens_c(s2[i], c2[i], OR_E(k), OR_E(k + 1));
+                else givens_tc(s2[i], c2[i], OR_E(k), OR_E(k + 1));
+            }
+        for (int j = d - 2; j >= (am + as) % 2; j -= 2)
+            for (int k = n - 3 - j; k >= 0; k--)
+                givens_c(s1[k + (long)j * n], c1[k + (long)j * n], OR_E(k), OR_E(k + 2));
+    } else {
+        for (int j = (am + as) % 2; j <= d - 2; j += 2)
+            for (int k = 0; k <= n - 3 - j; k++)
+                givens_tc(s1[k + (long)j * n], c1[k + (long)j * n], OR_E(k), OR_E(k + 2));
+        for (int j = 0; j < nseq; j++)
+            for (int k = 0; k <= n - 2 - d - j; k++) {
+                long i = k + (long)j * n + (long)d * as * n;
+                if (pos) givens_tc(s2[i], c2[i], OR_E(k), OR_E(k + 1));
+                else givens_c(s2[i], c2[i], OR_E(k), OR_E(k + 1));
+            }
+    }
+#undef OR_E
+}
+
+/* drivers.c:687-705: column 0 of order 0, columns 2m-1 and 2m of orders -m
+ * and m for m = 1 .. M/2 of the n x M complex matrix. */
+void oracle_execute_spinsph(int dir, const double *s1, const double *c1, const double *s2, const double *c2, int n,
+                            int s, double *A, int M) {
+    if (M > 0) oracle_kernel_spinsph(dir, s1, c1, s2, c2, n, s, 0, A, 1);
+    _Pragma("omp parallel for schedule(dynamic, 1)") for (int m = 1; m <= M / 2; m++) {
+        oracle_kernel_spinsph(dir, s1, c1, s2, c2, n, s, -m, A + 2 * (long)n * (2 * m - 1), 1);
+        oracle_kernel_spinsph(dir, s1, c1, s2, c2, n, s, m, A + 2 * (long)n * (2 * m), 1);
     }
 }
 

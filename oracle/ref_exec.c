@@ -176,5 +176,37 @@ void ref_execute_tet(int dir, double *s1, double *c1, double *s2, double *c2, in
     }
 }
 
+/* ft_execute_spinsph_{hi2lo,lo2hi} (drivers.c:645-773) around the
+ * reference's compiled spin kernels: isa REF_DEFAULT the scalar executor
+ * (drivers.c:687-705, kernel_spinsph_*_default), REF_AVX_FMA the dispatched
+ * one (drivers.c:747-773: orders 0, -1, 1 by the scalar kernel, the rest as
+ * column pairs permuted into B by permute_sph and run by
+ * kernel_spinsph_*_AVX_FMA).  B holds 2n x M doubles. */
+void ref_execute_spinsph(int dir, int isa, double *s1, double *c1, double *s2, double *c2, int n, int s, double *A,
+                         double *B, int M) {
+    ft_spin_rotation_plan SRP = {s1, c1, s2, c2, n, s};
+    ft_complex *AC = (ft_complex *)A, *BC = (ft_complex *)B;
+    void (*kd)(const ft_spin_rotation_plan *, const int, ft_complex *, const int) =
+        dir == 0 ? kernel_spinsph_hi2lo_default : kernel_spinsph_lo2hi_default;
+    if (isa == REF_DEFAULT) {
+        kd(&SRP, 0, AC, 1);
+        #pragma omp parallel for schedule(static, 1)
+        for (int m = 1; m <= M / 2; m++) {
+            kd(&SRP, -m, AC + (long)n * (2 * m - 1), 1);
+            kd(&SRP, m, AC + (long)n * (2 * m), 1);
+        }
+        return;
+    }
+    void (*kf)(const ft_spin_rotation_plan *, const int, ft_complex *, const int) =
+        dir == 0 ? kernel_spinsph_hi2lo_AVX_FMA : kernel_spinsph_lo2hi_AVX_FMA;
+    kd(&SRP, 0, AC, 1);
+    if (M > 1) kd(&SRP, -1, AC + n, 1);
+    if (M > 2) kd(&SRP, 1, AC + 2 * n, 1);
+    permute_sph(A, B, 2 * n, M, 2);
+    #pragma omp parallel for schedule(static, 1)
+    for (int m = 2; m <= M / 2; m++) kf(&SRP, m, BC + (long)n * (2 * m - 1), 2);
+    permute_t_sph(A, B, 2 * n, M, 2);
+}
+
 void ref_set_num_threads(int t) { omp_set_num_threads(t); }
 int ref_max_threads(void) { return omp_get_max_threads(); }

@@ -154,6 +154,34 @@ def host_build_tet(lib, plan_ptr, direction, S, part, N, L, m0, count, maxpart=1
     return bands, tab, chains
 
 
+def host_build_spin(lib, plan_ptr, direction, S, s, M, maxpart=16):
+    """(bands, table, chain parts) of the spin-weighted sphere's second stage
+    on a sphere plan as the library builds them (ftamd_sys_debug_spin)."""
+    fn = lib.ftamd_sys_debug_spin
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [
+        ctypes.POINTER(ctypes.POINTER(Band)), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_longlong),
+        ctypes.POINTER(ctypes.POINTER(Chain)), ctypes.POINTER(ctypes.c_int)]
+    pb, nb = ctypes.POINTER(Band)(), ctypes.c_int()
+    pt, ne = ctypes.POINTER(ctypes.c_double)(), ctypes.c_longlong()
+    pc, nc = ctypes.POINTER(Chain)(), ctypes.c_int()
+    rc = fn(ctypes.c_void_p(plan_ptr), direction, S, s, M, maxpart, ctypes.byref(pb), ctypes.byref(nb),
+            ctypes.byref(pt), ctypes.byref(ne), ctypes.byref(pc), ctypes.byref(nc))
+    if rc != 0:
+        raise RuntimeError("ftamd_sys_debug_spin failed: %d" % rc)
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    try:
+        bands = [Band.from_buffer_copy(pb[i]) for i in range(nb.value)]
+        tab = np.ctypeslib.as_array(pt, shape=(max(ne.value, 1) * 2,)).copy().reshape(-1, 2)
+        chains = [Chain.from_buffer_copy(pc[i]) for i in range(nc.value)]
+    finally:
+        for p in (pb, pt, pc):
+            libc.free(ctypes.cast(p, ctypes.c_void_p))
+    return bands, tab, chains
+
+
 class Item(ctypes.Structure):
     _fields_ = [("chain", ctypes.c_int), ("wave0", ctypes.c_int)]
 
