@@ -101,6 +101,12 @@ for _dn in ("hi2lo", "lo2hi"):
     SIGNATURES["ft_execute_spinsph_" + _dn] = (None, [_vp, _vp, _vp, _i])
     SIGNATURES["ft_kernel_spinsph_" + _dn] = (None, [_vp, _i, _vp, _i])
 
+for _t in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_sphere_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_gpu_execute_sph_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
+
 for _name, (_res, _args) in SIGNATURES.items():
     _fn = getattr(_lib, _name)
     _fn.restype = _res
@@ -281,6 +287,76 @@ def gpu_execute_spinsph(plan, direction, A, M=None, stream=None):
                                      _vp(stream.cuda_stream))
     if rc:
         raise GPUError("ft_gpu_execute_spinsph: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
+
+
+class SphereGridPlan:
+    """Owner of an ``ft_sphere_fftw_plan *`` (the grid transforms' device plan)."""
+
+    def __init__(self, ptr, N, M, synthesis, sphv):
+        if not ptr:
+            _check()
+            raise GPUError("plan construction failed")
+        self._ptr = ptr
+        self._destroy = _lib.ft_destroy_sphere_fftw_plan
+        self.N, self.M, self.synthesis, self.sphv = N, M, synthesis, sphv
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def __del__(self):
+        ptr, self._ptr = getattr(self, "_ptr", None), None
+        if ptr:
+            self._destroy(ptr)
+
+
+def _plan_grid(name, synthesis, sphv):
+    fn = getattr(_lib, "ft_plan_" + name)
+
+    def make(N, M, flags=0):
+        return SphereGridPlan(fn(int(N), int(M), int(flags)), int(N), int(M), synthesis, sphv)
+
+    make.__name__ = "plan_" + name
+    return make
+
+
+plan_sph_synthesis = _plan_grid("sph_synthesis", True, False)
+plan_sph_analysis = _plan_grid("sph_analysis", False, False)
+plan_sphv_synthesis = _plan_grid("sphv_synthesis", True, True)
+plan_sphv_analysis = _plan_grid("sphv_analysis", False, True)
+_GRID_WHICH = {"sph_synthesis": 0, "sph_analysis": 1, "sphv_synthesis": 2, "sphv_analysis": 3}
+
+
+def _execute_grid(name):
+    fn = getattr(_lib, "ft_execute_" + name)
+
+    def run(trans, plan, X):
+        N, M = plan.N, plan.M
+        fn(trans.encode() if isinstance(trans, str) else trans, plan.ptr, _pointer(X, N, M), N, M)
+        _check()
+        return X
+
+    run.__name__ = "execute_" + name
+    return run
+
+
+execute_sph_synthesis = _execute_grid("sph_synthesis")
+execute_sph_analysis = _execute_grid("sph_analysis")
+execute_sphv_synthesis = _execute_grid("sphv_synthesis")
+execute_sphv_analysis = _execute_grid("sphv_analysis")
+
+
+def gpu_execute_sph_grid(name, trans, plan, A, stream=None):
+    """Asynchronous ft_gpu_execute_sph_grid on a torch float64 tensor laid out
+    column-major (shape (M, N): row j is column j)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    rc = _lib.ft_gpu_execute_sph_grid(_GRID_WHICH[name], trans.encode(), plan.ptr, _pointer(A, plan.N, plan.M),
+                                      plan.N, plan.M, _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_sph_grid: %s" % _lib.ft_gpu_error_string(rc).decode())
     return A
 
 

@@ -23,9 +23,10 @@ ARCH = os.environ.get("FTAMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
-C_SOURCES = ["ft_plans.c", "ft_drivers.c", "ft_systolic.c", "ft_tuning.c", "ft_connection.c", "ft_harmonic.c"]
+C_SOURCES = ["ft_plans.c", "ft_drivers.c", "ft_systolic.c", "ft_tuning.c", "ft_connection.c", "ft_harmonic.c",
+             "ft_grid.c"]
 HIP_SOURCES = ["ft_rotations.hip", "ft_k_sph.hip", "ft_k_tri.hip", "ft_k_disk.hip", "ft_k_sys.hip", "ft_noalloc.hip",
-               "ft_trmm.hip", "ft_k_spin.hip"]
+               "ft_trmm.hip", "ft_k_spin.hip", "ft_grid.hip"]
 
 INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
 C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wextra"]
@@ -59,7 +60,11 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         outs = list(ex.map(_run, jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    # hipFFT (the grid transforms' FFTs): under PyTorch the process already
+    # holds torch's copy (same soname), which this library then binds to
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lhipfft", "-lpthread"])
     os.replace(tmp, LIB)
     if verbose:
         for o in outs:

@@ -1,0 +1,191 @@
+/*
+ * ft_grid.c -- the sphere's synthesis and analysis plans and executors
+ * (fftw.c:62-270, fasttransforms.h:466-493) on the GPU.
+ *
+ * A plan fixes the theta transforms of its column classes (synthesis: DCT-III
+ * on the cosine class, DST-III on the sine class; analysis: DCT-II, DST-II;
+ * the vector-field plans swap the classes) and the direction of the phi DFT
+ * (synthesis c2r, analysis r2c).  An executor fixes the order of the steps,
+ * the scalings and which rows it doubles or halves (fftw.c:126-270):
+ *
+ *   synthesis 'N' and analysis 'T' take a synthesis plan: fix-ups, theta,
+ *   scalings, phi;  synthesis 'T' and analysis 'N' take an analysis plan:
+ *   phi, scalings, theta, fix-ups (the reference's tests pair them so,
+ *   test_fftw.c:52-64).  A plan of the other kind sets FT_GPU_EINVAL.
+ *
+ * X is the reference's N x M column-major array (M odd: fftw.c:40-60 reads
+ * column 2*(M/2) for every M), host or device, overwritten in place.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ft_internal.h"
+
+struct ft_sphere_fftw_plan_s {
+    int N, M;
+    int synthesis; /* 1: DCT-III / DST-III and c2r; 0: DCT-II / DST-II and r2c */
+    int sphv;      /* the vector-field plans' column classes */
+    int device;
+    FtamdGrid *g;
+    void *d_stage; /* staging buffer of host-pointer calls */
+    size_t stage_bytes;
+    pthread_mutex_t lock; /* held for the whole of a call */
+};
+
+/* the reference's constants (ftinternal.h:48-56) */
+#define FT_SQRT2 1.4142135623730950488
+#define FT_SQRT1_2 0.70710678118654752440
+#define FT_1_4_SQRT_PI 0.141047395886939072
+#define FT_1_2_SQRT_PI 0.282094791773878143
+#define FT_4_SQRT_PI 7.089815403622064109
+#define FT_2_SQRT_PI 3.544907701811032055
+
+static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv) {
+    if (N < 1 || M < 1 || M % 2 == 0) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    int dev = -1, rc = ftamd_hip_device(&dev, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    ft_sphere_fftw_plan *P = calloc(1, sizeof(ft_sphere_fftw_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    P->N = N;
+    P->M = M;
+    P->synthesis = synthesis;
+    P->sphv = sphv;
+    P->device = dev;
+    pthread_mutex_init(&P->lock, NULL);
+    rc = ftamd_grid_create(N, M, synthesis, sphv, &P->g);
+    if (rc != FTAMD_OK) {
+        pthread_mutex_destroy(&P->lock);
+        free(P);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    return P;
+}
+
+/* fasttransforms.h:481-485 (fftw.c:108-126).  flags (FFTW's planner flags)
+ * are accepted and ignored. */
+ft_sphere_fftw_plan *ft_plan_sph_synthesis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 1, 0);
+}
+ft_sphere_fftw_plan *ft_plan_sph_analysis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 0, 0);
+}
+ft_sphere_fftw_plan *ft_plan_sphv_synthesis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 1, 1);
+}
+ft_sphere_fftw_plan *ft_plan_sphv_analysis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 0, 1);
+}
+
+/* fasttransforms.h:477 (fftw.c:66-74) */
+void ft_destroy_sphere_fftw_plan(ft_sphere_fftw_plan *P) {
+    if (!P) return;
+    ftamd_grid_destroy(P->g);
+    ftamd_hip_free(P->d_stage);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* The step order, fix-ups and scalings of executor `which` (0 sph synthesis,
+ * 1 sph analysis, 2 sphv synthesis, 3 sphv analysis) and TRANS; *need_syn:
+ * the plan kind it runs on. */
+typedef struct {
+    int need_syn, exec_sphv;
+    double f0, f1, a, b;
+} grid_step;
+
+static int grid_steps(int which, char TRANS, int N, int M, grid_step *s) {
+    const int analysis = which & 1, sphv = which >> 1;
+    if (TRANS != 'N' && TRANS != 'T') return 0;
+    const int trans = TRANS == 'T';
+    s->exec_sphv = sphv;
+    if (!analysis && !trans) { /* fftw.c:128-146: x2 row 0 (cosine class), III, x 1/(4 sqrt(pi)), col 0 x sqrt 2, c2r */
+        s->need_syn = 1, s->f0 = 2.0, s->f1 = 1.0, s->a = FT_1_4_SQRT_PI, s->b = FT_SQRT2;
+    } else if (!analysis) { /* fftw.c:148-163: r2c, x 1/(2 sqrt(pi)), col 0 x 1/sqrt 2, II, x1/2 last row (sine class) */
+        s->need_syn = 0, s->f0 = 1.0, s->f1 = 0.5, s->a = FT_1_2_SQRT_PI, s->b = FT_SQRT1_2;
+    } else if (!trans) { /* fftw.c:167-183: r2c, x 4 sqrt(pi)/(2NM), col 0 x 1/sqrt 2, II, x1/2 row 0 (cosine class) */
+        s->need_syn = 0, s->f0 = 0.5, s->f1 = 1.0, s->a = FT_4_SQRT_PI / (2.0 * N * M), s->b = FT_SQRT1_2;
+    } else { /* fftw.c:185-200: x2 last row (sine class), III, x 2 sqrt(pi)/(2NM), col 0 x sqrt 2, c2r */
+        s->need_syn = 1, s->f0 = 1.0, s->f1 = 2.0, s->a = FT_2_SQRT_PI / (2.0 * N * M), s->b = FT_SQRT2;
+    }
+    return 1;
+}
+
+static int valid_grid(const ft_sphere_fftw_plan *P, int N, int M, const grid_step *s) {
+    return P && P->N == N && P->M == M && P->synthesis == s->need_syn;
+}
+
+int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, double *dX, int N, int M,
+                            void *stream) {
+    ft_sphere_fftw_plan *P = (ft_sphere_fftw_plan *)P_;
+    grid_step s;
+    if (which < 0 || which > 3 || !grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, N, M, &s) || !dX)
+        return FT_GPU_EINVAL;
+    pthread_mutex_lock(&P->lock);
+    int rc = ftamd_grid_run(P->g, dX, s.exec_sphv, s.f0, s.f1, s.a, s.b, stream);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+static void execute_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, double *X, int N, int M) {
+    ft_sphere_fftw_plan *P = (ft_sphere_fftw_plan *)P_;
+    grid_step s;
+    if (!grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, N, M, &s) || !X) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(X, &pk);
+    const size_t bytes = (size_t)N * (size_t)M * sizeof(double);
+    pthread_mutex_lock(&P->lock);
+    void *d = X;
+    if (pk != FTAMD_PTR_DEVICE) {
+        if (P->stage_bytes < bytes) {
+            ftamd_hip_free(P->d_stage);
+            P->d_stage = NULL;
+            P->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&P->d_stage, bytes);
+            if (rc == FTAMD_OK) P->stage_bytes = bytes;
+        }
+        d = P->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = ftamd_grid_run(P->g, d, s.exec_sphv, s.f0, s.f1, s.a, s.b, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&P->lock);
+    ftamd_set_error(rc);
+}
+
+/* fasttransforms.h:488-493 (fftw.c:126-270) */
+void ft_execute_sph_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(0, TRANS, P, X, N, M);
+}
+void ft_execute_sph_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(1, TRANS, P, X, N, M);
+}
+void ft_execute_sphv_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(2, TRANS, P, X, N, M);
+}
+void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(3, TRANS, P, X, N, M);
+}

@@ -1,0 +1,214 @@
+// ft_grid.hip -- synthesis and analysis on the sphere's tensor-product grid
+// (fftw.c:62-270): per column of the n x M coefficient array, a DCT or DST
+// of length N along theta (FFTW's REDFT01/RODFT01 for synthesis plans,
+// REDFT10/RODFT10 for analysis plans), and per row a real DFT of length M
+// along phi (c2r / r2c), with the reference's scalings and the column layout
+// of its data_r2c / data_c2r (fftw.c:40-60).
+//
+// The DCTs run as complex FFTs of length N (hipFFT) between two HIP kernels,
+// by Makhoul's reordering: DCT-II(x)_k = 2 Re(e^{-i pi k/2N} FFT(v)_k) with
+// v_m = x_{2m}, v_{N-1-m} = x_{2m+1}; DCT-III(X) is the inverse map,
+// v = IFFT(e^{i pi k/2N} (X_k - i X_{N-k})) (unnormalized, X_N = 0),
+// y_{2m} = Re v_m, y_{2m+1} = Re v_{N-1-m}.  The DSTs are DCTs of the
+// reversed input with alternating output signs (DST-III) or of the
+// sign-alternated input with reversed output (DST-II).  The phi DFTs are
+// hipFFT's strided real transforms (row i is the transform at element i,
+// stride N).
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <math.h>
+#include <stdlib.h>
+
+#include "ft_hip_abi.h"
+
+struct FtamdGrid {
+    int N, M, type3, plan_sphv;
+    hipfftHandle theta, phi;
+    double2 *tw;  // e^{i pi k / 2N}, k < N
+    double2 *Z;   // N x M complex: the columns' FFTs
+    double2 *Y;   // N x (M/2+1) complex: the rows' half spectra
+};
+
+namespace ftgrid {
+
+// Columns of the cosine class: j mod 4 in {0, 3} (fftw.c:84-96: plans 1 and
+// 4 take kind[0], plans 2 and 3 kind[1]); the vector-field plans swap the kinds.
+__device__ __forceinline__ bool cos_class(int j, int sphv) {
+    const int r = j & 3;
+    return ((r == 0) || (r == 3)) != (sphv != 0);
+}
+
+// Pipeline with DCT-III first (synthesis plans): fix-ups, DCT-III / DST-III
+// of every column into Z (pre-twiddled for the inverse FFT).
+__global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw, int N,
+                     int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= N || j >= M) return;
+    const double *x = X + (long long)N * j;
+    const bool tcos = cos_class(j, plan_sphv), fcos = cos_class(j, exec_sphv);
+    // entry e of the column after the executor's fix-ups
+    auto val = [&](int e) {
+        double v = x[e];
+        if (fcos && e == 0) v *= f_row0_cos;
+        if (!fcos && e == N - 1) v *= f_last_sin;
+        return v;
+    };
+    // input of the DCT-III: the column itself (cosine class) or reversed (sine class)
+    auto in = [&](int e) { return tcos ? val(e) : val(N - 1 - e); };
+    const double re = in(k), im = k == 0 ? 0.0 : -in(N - k);
+    const double2 t = tw[k];
+    Z[(long long)N * j + k] = make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
+}
+
+// After the inverse FFT: y_n of every column, signs of the sine class, the
+// scalings a (all) and b (column 0 again), into Y as data_r2c lays it out.
+__global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
+                      double b) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (n >= N || j >= M) return;
+    const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    double y = Z[(long long)N * j + src].x;
+    if (!cos_class(j, plan_sphv) && (n & 1)) y = -y;
+    y *= a;
+    if (j == 0) y *= b;
+    double *Yd = (double *)Y;
+    if (j == 0) {
+        Y[n] = make_double2(y, 0.0);
+    } else {
+        const int q = (j + 1) >> 1;
+        Yd[2 * ((long long)N * q + n) + ((j & 1) ? 1 : 0)] = (j & 1) ? -y : y;
+    }
+}
+
+// Pipeline with the row DFT first (analysis plans): data_c2r, the scalings,
+// the sine class's alternating signs, Makhoul's reordering into Z.
+__global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int N, int M, int plan_sphv, double a,
+                     double b) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (n >= N || j >= M) return;
+    const double *Yd = (const double *)Y;
+    double x;
+    if (j == 0) {
+        x = Yd[2 * n];
+    } else {
+        const int q = (j + 1) >> 1;
+        const double v = Yd[2 * ((long long)N * q + n) + ((j & 1) ? 1 : 0)];
+        x = (j & 1) ? -v : v;
+    }
+    x *= a;
+    if (j == 0) x *= b;
+    if (!cos_class(j, plan_sphv) && (n & 1)) x = -x;
+    const int dst = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    Z[(long long)N * j + dst] = make_double2(x, 0.0);
+}
+
+// After the forward FFT: the DCT-II (cosine class) or DST-II (sine class) of
+// every column, then the executor's fix-ups, into X.
+__global__ void post2(const double2 *__restrict__ Z, double *__restrict__ X, const double2 *__restrict__ tw, int N,
+                      int M, int plan_sphv, int exec_sphv, double g_row0_cos, double g_last_sin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= N || j >= M) return;
+    const bool tcos = cos_class(j, plan_sphv), fcos = cos_class(j, exec_sphv);
+    const int src = tcos ? k : N - 1 - k;
+    const double2 V = Z[(long long)N * j + src];
+    const double2 t = tw[src];
+    double y = 2.0 * (t.x * V.x + t.y * V.y);
+    if (fcos && k == 0) y *= g_row0_cos;
+    if (!fcos && k == N - 1) y *= g_last_sin;
+    X[(long long)N * j + k] = y;
+}
+
+}  // namespace ftgrid
+
+static int fft_ok(hipfftResult r) { return r == HIPFFT_SUCCESS ? FTAMD_OK : FTAMD_EDEVICE; }
+
+extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_sphv, FtamdGrid **out) {
+    *out = NULL;
+    /* columns are the launch grid's y dimension (at most 65535) */
+    if (N < 1 || M < 1 || M % 2 == 0 || M > 65535) return FTAMD_EINVAL;
+    FtamdGrid *g = (FtamdGrid *)calloc(1, sizeof(FtamdGrid));
+    if (!g) return FTAMD_ENOMEM;
+    g->N = N;
+    g->M = M;
+    g->type3 = type3;
+    g->plan_sphv = plan_sphv;
+    g->theta = g->phi = 0;
+    int rc = FTAMD_OK;
+    const size_t zbytes = sizeof(double2) * (size_t)N * M, ybytes = sizeof(double2) * (size_t)N * (M / 2 + 1);
+    double2 *tw = (double2 *)malloc(sizeof(double2) * (size_t)N);
+    if (!tw) rc = FTAMD_ENOMEM;
+    for (int k = 0; rc == FTAMD_OK && k < N; k++) {
+        const long double th = 3.141592653589793238462643383279502884L * k / (2.0L * N);
+        tw[k] = make_double2((double)cosl(th), (double)sinl(th));
+    }
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->tw, sizeof(double2) * (size_t)N) != hipSuccess) rc = FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->Z, zbytes) != hipSuccess) rc = FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->Y, ybytes) != hipSuccess) rc = FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && hipMemcpy(g->tw, tw, sizeof(double2) * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
+        rc = FTAMD_EDEVICE;
+    free(tw);
+    // theta: M contiguous complex transforms of length N
+    int n1[1] = {N};
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, M));
+    // phi: N real transforms of length M, element i of row-transform e at i + e*N
+    int n2[1] = {M}, nc[1] = {M / 2 + 1};
+    if (rc == FTAMD_OK) {
+        if (type3)
+            rc = fft_ok(hipfftPlanMany(&g->phi, 1, n2, nc, N, 1, n2, N, 1, HIPFFT_Z2D, N));
+        else
+            rc = fft_ok(hipfftPlanMany(&g->phi, 1, n2, n2, N, 1, nc, N, 1, HIPFFT_D2Z, N));
+    }
+    if (rc != FTAMD_OK) {
+        ftamd_grid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FTAMD_OK;
+}
+
+extern "C" void ftamd_grid_destroy(FtamdGrid *g) {
+    if (!g) return;
+    hipDeviceSynchronize();
+    if (g->theta) hipfftDestroy(g->theta);
+    if (g->phi) hipfftDestroy(g->phi);
+    hipFree(g->tw);
+    hipFree(g->Z);
+    hipFree(g->Y);
+    free(g);
+}
+
+extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0, double f1, double a, double b,
+                              void *stream) {
+    if (!g || !X) return FTAMD_EINVAL;
+    const int N = g->N, M = g->M;
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 block(256), grid((N + 255) / 256, M);
+    int rc = fft_ok(hipfftSetStream(g->theta, st));
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(g->phi, st));
+    if (rc != FTAMD_OK) return rc;
+    if (g->type3) {
+        // fix-ups f0 (row 0, cosine columns) and f1 (last row, sine columns),
+        // DCT-III / DST-III, scalings, data_r2c, c2r
+        hipLaunchKernelGGL(ftgrid::pre3, grid, block, 0, st, X, g->Z, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        return fft_ok(hipfftExecZ2D(g->phi, (hipfftDoubleComplex *)g->Y, X));
+    }
+    // r2c, data_c2r, scalings, DCT-II / DST-II, fix-ups f0 and f1
+    rc = fft_ok(hipfftExecD2Z(g->phi, X, (hipfftDoubleComplex *)g->Y));
+    if (rc != FTAMD_OK) return rc;
+    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b);
+    if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+    rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));
+    if (rc != FTAMD_OK) return rc;
+    hipLaunchKernelGGL(ftgrid::post2, grid, block, 0, st, g->Z, X, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1);
+    return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+}

@@ -268,6 +268,35 @@ int ft_gpu_execute(const ft_rotation_plan *RP, int kind, int direction, int prec
 int ft_gpu_execute_tet(const ft_rotation_plan *RP1, const ft_rotation_plan *RP2, int direction, double *dA, int L,
                        int M, void *stream);
 
+/* fasttransforms.h:466-493 (fftw.c:62-270) -- synthesis (coefficients of
+ * the bivariate Fourier series sph2fourier produces -> values on the
+ * N x M grid theta_i = (i+1/2) pi / N, phi_j = 2 pi j / M) and analysis (the
+ * inverse) on the sphere, and their vector-field variants, on the GPU
+ * (hipFFT plus this library's kernels in place of FFTW).  The plan holds
+ * device work buffers (about 24 N M bytes); flags are ignored.  X is N x M
+ * column-major, host or device, M odd, overwritten in place.  Synthesis 'N'
+ * and analysis 'T' take a synthesis plan, synthesis 'T' and analysis 'N' an
+ * analysis plan (as the reference's own tests pair them); another plan, or
+ * N, M other than the plan's, sets FT_GPU_EINVAL.  Results agree with
+ * FFTW's to rounding (the FFTs sum in another order). */
+typedef struct ft_sphere_fftw_plan_s ft_sphere_fftw_plan;
+void ft_destroy_sphere_fftw_plan(ft_sphere_fftw_plan *P);
+ft_sphere_fftw_plan *ft_plan_sph_synthesis(const int N, const int M, const unsigned flags);
+ft_sphere_fftw_plan *ft_plan_sph_analysis(const int N, const int M, const unsigned flags);
+ft_sphere_fftw_plan *ft_plan_sphv_synthesis(const int N, const int M, const unsigned flags);
+ft_sphere_fftw_plan *ft_plan_sphv_analysis(const int N, const int M, const unsigned flags);
+void ft_execute_sph_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
+void ft_execute_sph_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
+void ft_execute_sphv_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
+void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
+
+/* The grid transforms on a device-resident X asynchronously on a HIP stream:
+ * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
+ * analysis, TRANS and the plan as the executors above.  Calls on one plan
+ * share its work buffers: order them on one stream. */
+int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P, double *dX, int N, int M,
+                            void *stream);
+
 /* ft_execute_spinsph_{hi2lo,lo2hi} (direction 0, 1) on a device-resident
  * n x M complex array dA (interleaved FP64), asynchronously on a HIP stream,
  * as ft_gpu_execute: the first stage's kernel and one band-systolic launch. */

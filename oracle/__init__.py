@@ -545,3 +545,91 @@ def relinf(A, B):
     d = np.max(np.abs(np.asarray(A) - np.asarray(B)))
     b = np.max(np.abs(np.asarray(B)))
     return d / b if b else d
+
+
+# ---------------------------------------------------------------- grids --
+# fftw.c:40-270 restated with scipy.fft / numpy.fft (pocketfft) in place of
+# FFTW: the r2r kinds are FFTW's definitions (REDFT01 = scipy DCT-III,
+# RODFT01 = DST-III, REDFT10 = DCT-II, RODFT10 = DST-II, all unnormalized),
+# the phi DFTs unnormalized real transforms.  Pinned in tests against the
+# definitions summed directly and against closed forms.
+
+_GRID_C = {"sqrt2": 1.4142135623730950488, "sqrt1_2": 0.70710678118654752440,
+           "1_4_sqrt_pi": 0.141047395886939072, "1_2_sqrt_pi": 0.282094791773878143,
+           "4_sqrt_pi": 7.089815403622064109, "2_sqrt_pi": 3.544907701811032055}
+
+
+def _grid_classes(M, sphv):
+    """Cosine-class columns of a plan (fftw.c:84-96: columns j mod 4 in {0, 3}
+    take kind[0], REDFT for the sphere's plans, RODFT for the vector field's)."""
+    j = np.arange(M)
+    c = (j % 4 == 0) | (j % 4 == 3)
+    return ~c if sphv else c
+
+
+def _theta(X, synthesis, plan_sphv):
+    import scipy.fft as sf
+    cos = _grid_classes(X.shape[1], plan_sphv)
+    t = 3 if synthesis else 2
+    X[:, cos] = sf.dct(X[:, cos], type=t, axis=0)
+    X[:, ~cos] = sf.dst(X[:, ~cos], type=t, axis=0)
+
+
+def _data_r2c(X):
+    """fftw.c:40-49: column 0 the DC term, columns 2j, 2j-1 (re, -im) of frequency j."""
+    N, M = X.shape
+    Y = np.zeros((N, M // 2 + 1), dtype=np.complex128)
+    Y[:, 0] = X[:, 0]
+    for j in range(1, M // 2 + 1):
+        Y[:, j] = X[:, 2 * j] - 1j * X[:, 2 * j - 1]
+    return Y
+
+
+def _data_c2r(Y, M):
+    N = Y.shape[0]
+    X = np.zeros((N, M))
+    X[:, 0] = Y[:, 0].real
+    for j in range(1, M // 2 + 1):
+        X[:, 2 * j] = Y[:, j].real
+        X[:, 2 * j - 1] = -Y[:, j].imag
+    return X
+
+
+def grid(which, trans, synthesis_plan, plan_sphv, X):
+    """ft_execute_{sph,sphv}_{synthesis,analysis}(TRANS, P, X, N, M) restated
+    (fftw.c:126-270) on a copy of the N x M array X: which is "sph_synthesis",
+    "sph_analysis", "sphv_synthesis" or "sphv_analysis"; the plan is a
+    synthesis plan (DCT-III / DST-III, c2r) or not (DCT-II / DST-II, r2c),
+    plan_sphv its column classes."""
+    X = np.array(X, dtype=np.float64, order="F")
+    N, M = X.shape
+    analysis, exec_sphv = "analysis" in which, which.startswith("sphv")
+    cos = _grid_classes(M, exec_sphv)
+    c = _GRID_C
+    pre3 = (not analysis and trans == "N") or (analysis and trans == "T")
+    if pre3 != bool(synthesis_plan):
+        raise ValueError("this executor and TRANS take the other plan")
+    if pre3:
+        if trans == "N":   # synthesis: x2 on row 0 of the cosine columns
+            X[0, cos] *= 2.0
+            a = c["1_4_sqrt_pi"]
+        else:              # analysis 'T': x2 on the last row of the sine columns
+            X[N - 1, ~cos] *= 2.0
+            a = c["2_sqrt_pi"] / (2 * N * M)
+        _theta(X, True, plan_sphv)
+        X *= a
+        X[:, 0] *= c["sqrt2"]
+        return np.fft.irfft(_data_r2c(X), n=M, axis=1) * M
+    Y = np.fft.rfft(X, axis=1)
+    X = _data_c2r(Y, M)
+    if trans == "T":       # synthesis 'T'
+        X *= c["1_2_sqrt_pi"]
+    else:                  # analysis 'N'
+        X *= c["4_sqrt_pi"] / (2 * N * M)
+    X[:, 0] *= c["sqrt1_2"]
+    _theta(X, False, plan_sphv)
+    if trans == "T":
+        X[N - 1, ~cos] *= 0.5
+    else:
+        X[0, cos] *= 0.5
+    return X

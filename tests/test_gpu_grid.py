@@ -1,0 +1,85 @@
+"""GPU synthesis and analysis on the sphere's grid (ft_execute_{sph,sphv}_
+{synthesis,analysis}, fftw.c:126-270) against the oracle (oracle.grid,
+pinned on the CPU): the FFTs sum in another order than the oracle's, so the
+bar is the relative 2-norm 2e-14 (measured ~1e-16 to 1e-15).  The whole
+spherical harmonic transform on the device -- sph2fourier, synthesis,
+analysis, fourier2sph -- returns its input within the reference's bounds
+(test_fftw.c:44-67: 4 sqrt(N) eps rel-2, 2N eps rel-inf).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPS = 2.0 ** -52
+SIZES = [(1, 1), (2, 3), (5, 9), (8, 5), (64, 127), (65, 129), (256, 511), (1024, 2047)]
+CASES = [(w, t) for w in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis") for t in ("N", "T")]
+
+
+def _say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def _plan(ft, which, trans, N, M):
+    """The plan the executor takes for TRANS: synthesis 'N' and analysis 'T'
+    a synthesis plan, the others an analysis plan (test_fftw.c:52-64)."""
+    sphv = which.startswith("sphv")
+    syn = (which.endswith("synthesis")) == (trans == "N")
+    name = ("sphv_" if sphv else "sph_") + ("synthesis" if syn else "analysis")
+    return getattr(ft, "plan_" + name)(N, M), syn, sphv
+
+
+@pytest.mark.parametrize("which,trans", CASES)
+def test_against_oracle(ft, which, trans):
+    for N, M in SIZES:
+        P, syn, sphv = _plan(ft, which, trans, N, M)
+        X = np.asfortranarray(np.random.default_rng(N * 7 + M).standard_normal((N, M)))
+        _say("grid", which, trans, "N=%d M=%d" % (N, M))
+        got = getattr(ft, "execute_" + which)(trans, P, X.copy(order="F"))
+        ref = O.grid(which, trans, syn, sphv, X)
+        assert O.rel2(got, ref) <= 2e-14, (which, trans, N, M, O.rel2(got, ref))
+
+
+def test_device_resident_and_wrong_plans(ft):
+    import torch
+    N, M = 300, 599
+    S, A = ft.plan_sph_synthesis(N, M), ft.plan_sph_analysis(N, M)
+    X = np.asfortranarray(np.random.default_rng(3).standard_normal((N, M)))
+    T = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    _say("gpu_execute_sph_grid N=%d M=%d" % (N, M))
+    ft.gpu_execute_sph_grid("sph_synthesis", "N", S, T)
+    ft.gpu_execute_sph_grid("sph_analysis", "N", A, T)
+    ft.synchronize()
+    assert O.rel2(T.cpu().numpy().T, X) <= 1e-14
+    with pytest.raises(ft.GPUError):
+        ft.execute_sph_synthesis("N", A, X.copy(order="F"))   # an analysis plan
+    with pytest.raises(ft.GPUError):
+        ft.execute_sph_analysis("N", S, X.copy(order="F"))    # a synthesis plan
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_spherical_harmonic_transform_round_trip(ft, n):
+    """test_fftw.c:44-67 on the device: sph2fourier, synthesis, analysis,
+    fourier2sph ('N'), then the same with 'T', against the reference's bounds."""
+    import torch
+    M = 2 * n - 1
+    H = ft.plan_sph2fourier(n)
+    PS, PA = ft.plan_sph_synthesis(n, M), ft.plan_sph_analysis(n, M)
+    A = O.fill("sph", n, M, 99)
+    T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    _say("SHT round trip n=%d" % n)
+    for trans, syn, ana in (("N", PS, PA), ("T", PA, PS)):
+        ft.gpu_execute_harmonic("sph2fourier", trans, H, T)
+        ft.gpu_execute_sph_grid("sph_synthesis", trans, syn, T)
+        ft.gpu_execute_sph_grid("sph_analysis", trans, ana, T)
+        ft.gpu_execute_harmonic("fourier2sph", trans, H, T)
+    ft.synchronize()
+    X = T.cpu().numpy().T
+    e2, einf = O.rel2(X, A), O.relinf(X, A)
+    _say("SHT n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
+    assert e2 < 4 * np.sqrt(n) * EPS and einf < 2 * n * EPS

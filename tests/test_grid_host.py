@@ -1,0 +1,120 @@
+"""Synthesis and analysis on the sphere's grid (fftw.c:40-270) on the CPU.
+
+The oracle (oracle.grid: scipy / numpy FFTs in place of FFTW) is pinned by:
+
+* its r2r transforms against FFTW's definitions of REDFT01, RODFT01,
+  REDFT10 and RODFT10 summed directly in long double, and its real DFTs
+  against the unnormalized DFT sums (FFTW's r2c / c2r conventions);
+* synthesis 'N' of every unit coefficient against the closed form it must
+  produce on the grid theta_n = (2n+1) pi / 2N, phi_j = 2 pi j / M:
+  a * w * {cos(l theta), sin((l+1) theta)} * {sqrt 2, 2 cos(q phi), 2 sin(q phi)}
+  for the column classes of the sphere's and the vector field's plans;
+* 'T' is the transpose of 'N' (dense operators at small sizes), for both
+  executors and both plan families;
+* analysis 'N' inverts synthesis 'N'.
+
+The library's grid plans need a device; here only their argument checks run.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+EPS = 2.0 ** -52
+
+
+def _direct_r2r(x, kind):
+    """FFTW's r2r definitions (FFTW manual, 'What FFTW Really Computes'), in long double."""
+    x = np.asarray(x, dtype=np.longdouble)
+    n = len(x)
+    k = np.arange(n, dtype=np.longdouble)
+    pi = np.longdouble(np.pi)
+    y = np.zeros(n, dtype=np.longdouble)
+    for i in range(n):
+        if kind == "REDFT01":
+            y[i] = x[0] + 2 * np.sum(x[1:] * np.cos(pi * k[1:] * (i + np.longdouble(0.5)) / n))
+        elif kind == "RODFT01":
+            y[i] = (-1) ** i * x[n - 1] + 2 * np.sum(x[:n - 1] * np.sin(pi * (k[:n - 1] + 1) * (i + np.longdouble(0.5)) / n))
+        elif kind == "REDFT10":
+            y[i] = 2 * np.sum(x * np.cos(pi * (k + np.longdouble(0.5)) * i / n))
+        else:  # RODFT10
+            y[i] = 2 * np.sum(x * np.sin(pi * (k + np.longdouble(0.5)) * (i + 1) / n))
+    return y.astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 17, 64])
+def test_r2r_kinds_are_fftw_definitions(n):
+    import scipy.fft as sf
+    x = np.random.default_rng(n).standard_normal(n)
+    for kind, got in (("REDFT01", sf.dct(x, type=3)), ("RODFT01", sf.dst(x, type=3)),
+                      ("REDFT10", sf.dct(x, type=2)), ("RODFT10", sf.dst(x, type=2))):
+        ref = _direct_r2r(x, kind)
+        assert np.max(np.abs(got - ref)) <= 64 * n * EPS * max(1.0, np.max(np.abs(ref))), kind
+
+
+@pytest.mark.parametrize("M", [1, 3, 7, 15])
+def test_real_dfts_are_unnormalized_sums(M):
+    x = np.random.default_rng(M).standard_normal(M)
+    j = np.arange(M)
+    Y = np.fft.rfft(x)
+    ref = np.array([np.sum(x * np.exp(-2j * np.pi * j * k / M)) for k in range(M // 2 + 1)])
+    assert np.max(np.abs(Y - ref)) <= 64 * M * EPS
+    back = np.fft.irfft(Y, n=M) * M
+    ref2 = np.array([Y[0].real + 2 * np.sum((Y[1:] * np.exp(2j * np.pi * np.arange(1, M // 2 + 1) * t / M)).real)
+                     for t in j])
+    assert np.max(np.abs(back - ref2)) <= 64 * M * EPS * max(1.0, np.max(np.abs(ref2)))
+
+
+@pytest.mark.parametrize("sphv", [False, True])
+@pytest.mark.parametrize("N,M", [(1, 1), (4, 7), (6, 11)])
+def test_synthesis_of_unit_coefficients(sphv, N, M):
+    which = "sphv_synthesis" if sphv else "sph_synthesis"
+    cos = O._grid_classes(M, sphv)
+    theta = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    phi = 2 * np.pi * np.arange(M) / M
+    a = 1.0 / (4.0 * np.sqrt(np.pi))
+    for l in range(N):
+        for col in range(M):
+            X = np.zeros((N, M))
+            X[l, col] = 1.0
+            G = O.grid(which, "N", True, sphv, X)
+            if cos[col]:
+                t = 2 * np.cos(l * theta)
+            else:
+                t = (2.0 if l < N - 1 else 1.0) * np.sin((l + 1) * theta)
+            if col == 0:
+                p = np.full(M, np.sqrt(2.0))
+            elif col % 2 == 0:
+                p = 2 * np.cos(col // 2 * phi)
+            else:
+                p = 2 * np.sin((col + 1) // 2 * phi)
+            assert np.max(np.abs(G - a * np.outer(t, p))) <= 32 * (N + M) * EPS, (l, col)
+
+
+def _dense(which, trans, syn_plan, sphv, N, M):
+    D = np.zeros((N * M, N * M))
+    for i in range(N * M):
+        e = np.zeros(N * M)
+        e[i] = 1.0
+        D[:, i] = O.grid(which, trans, syn_plan, sphv, e.reshape((N, M), order="F")).reshape(-1, order="F")
+    return D
+
+
+@pytest.mark.parametrize("sphv", [False, True])
+@pytest.mark.parametrize("N,M", [(3, 5), (4, 7)])
+def test_transposes_and_inverse(sphv, N, M):
+    pre = "sphv_" if sphv else "sph_"
+    SN = _dense(pre + "synthesis", "N", True, sphv, N, M)
+    ST = _dense(pre + "synthesis", "T", False, sphv, N, M)
+    AN = _dense(pre + "analysis", "N", False, sphv, N, M)
+    AT = _dense(pre + "analysis", "T", True, sphv, N, M)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
+
+
+def test_library_rejects_bad_shapes(ft):
+    with pytest.raises(ft.GPUError):
+        ft.plan_sph_synthesis(8, 14)
+    with pytest.raises(ft.GPUError):
+        ft.plan_sph_analysis(0, 15)
