@@ -124,6 +124,52 @@ def tet(ft, torch, N, direction, steps, warmup):
     return ms, 6.0 * tet_rotations(N, L, M)
 
 
+def spin_rows(ft, torch, n=1024, spins=(0, 1, 2, -2, 4)):
+    """ft_gpu_execute_spinsph on the n x (2n-1) complex array (the
+    reference's timing shape, test_drivers.c:1591-1600)."""
+    out = {}
+    M = 2 * n - 1
+    for s in spins:
+        P = ft.plan_rotspinsphere(n, s)
+        g = torch.Generator(device="cpu").manual_seed(n + s)
+        T = torch.complex(2 * torch.rand(M, n, dtype=torch.float64, generator=g) - 1,
+                          2 * torch.rand(M, n, dtype=torch.float64, generator=g) - 1).cuda()
+        stream = torch.cuda.current_stream()
+        for d in ("hi2lo", "lo2hi"):
+            ms = events(ft, torch, stream, lambda: ft.gpu_execute_spinsph(P, d, T, stream=stream), 5, 2)
+            out["spinsph_n%d_s%d_%s_ms" % (n, s, d)] = round(ms, 4)
+    return out
+
+
+def grid_rows(ft, torch, n=1024):
+    """Synthesis and analysis on the n x (2n-1) grid (ft_gpu_execute_sph_grid),
+    and the whole spherical harmonic transform on the device: sph2fourier then
+    synthesis (coefficients to grid values), analysis then fourier2sph."""
+    M = 2 * n - 1
+    PS, PA = ft.plan_sph_synthesis(n, M), ft.plan_sph_analysis(n, M)
+    H = ft.plan_sph2fourier(n)
+    g = torch.Generator(device="cpu").manual_seed(n)
+    T = (2 * torch.rand(M, n, dtype=torch.float64, generator=g) - 1).cuda()
+    stream = torch.cuda.current_stream()
+    out = {}
+    out["sph_synthesis_n%d_ms" % n] = round(events(
+        ft, torch, stream, lambda: ft.gpu_execute_sph_grid("sph_synthesis", "N", PS, T, stream=stream), 10, 2), 4)
+    out["sph_analysis_n%d_ms" % n] = round(events(
+        ft, torch, stream, lambda: ft.gpu_execute_sph_grid("sph_analysis", "N", PA, T, stream=stream), 10, 2), 4)
+
+    def sht():
+        ft.gpu_execute_harmonic("sph2fourier", "N", H, T, stream=stream)
+        ft.gpu_execute_sph_grid("sph_synthesis", "N", PS, T, stream=stream)
+
+    def isht():
+        ft.gpu_execute_sph_grid("sph_analysis", "N", PA, T, stream=stream)
+        ft.gpu_execute_harmonic("fourier2sph", "N", H, T, stream=stream)
+
+    out["sht_synthesis_n%d_ms" % n] = round(events(ft, torch, stream, sht, 10, 2), 4)
+    out["sht_analysis_n%d_ms" % n] = round(events(ft, torch, stream, isht, 10, 2), 4)
+    return out
+
+
 def tet_rows(ft, torch, sizes=(256, 512)):
     out = {}
     for N in sizes:
@@ -563,6 +609,8 @@ def main():
             "disk_hi2lo_ms": round(msd, 4), "disk_hi2lo_gflops": round(fld / (msd * 1e-3) / 1e9, 2),
             "harmonic_n%d" % n: harmonic_rows(ft, torch, n),
             "tet": tet_rows(ft, torch),
+            "spinsph": spin_rows(ft, torch),
+            "grid": grid_rows(ft, torch, n),
             "batch_sph_hi2lo_512": batch(ft, torch),
             "batch64_tri": batch64(ft, torch, "tri", (0.0, 0.0, 0.0)),
             "batch64_tri_m05": batch64(ft, torch, "tri", (-0.5, -0.5, -0.5)),

@@ -105,6 +105,11 @@ for _t in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_sphere_fftw_plan"] = (None, [_vp])
+for _t in ("tri_synthesis", "tri_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_triangle_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_gpu_execute_tri_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 SIGNATURES["ft_gpu_execute_sph_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 
 for _name, (_res, _args) in SIGNATURES.items():
@@ -357,6 +362,39 @@ def gpu_execute_sph_grid(name, trans, plan, A, stream=None):
                                       plan.N, plan.M, _vp(stream.cuda_stream))
     if rc:
         raise GPUError("ft_gpu_execute_sph_grid: %s" % _lib.ft_gpu_error_string(rc).decode())
+    return A
+
+
+class TriangleGridPlan(SphereGridPlan):
+    """Owner of an ``ft_triangle_fftw_plan *``."""
+
+    def __init__(self, ptr, N, M, synthesis):
+        SphereGridPlan.__init__(self, ptr, N, M, synthesis, False)
+        self._destroy = _lib.ft_destroy_triangle_fftw_plan
+
+
+def plan_tri_synthesis(N, M, flags=0):
+    return TriangleGridPlan(_lib.ft_plan_tri_synthesis(int(N), int(M), int(flags)), int(N), int(M), True)
+
+
+def plan_tri_analysis(N, M, flags=0):
+    return TriangleGridPlan(_lib.ft_plan_tri_analysis(int(N), int(M), int(flags)), int(N), int(M), False)
+
+
+execute_tri_synthesis = _execute_grid("tri_synthesis")
+execute_tri_analysis = _execute_grid("tri_analysis")
+
+
+def gpu_execute_tri_grid(name, trans, plan, A, stream=None):
+    """Asynchronous ft_gpu_execute_tri_grid ("tri_synthesis" or "tri_analysis")
+    on a torch float64 tensor of N*M entries."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    rc = _lib.ft_gpu_execute_tri_grid(1 if name == "tri_analysis" else 0, trans.encode(), plan.ptr,
+                                      _pointer(A, plan.N, plan.M), plan.N, plan.M, _vp(stream.cuda_stream))
+    if rc:
+        raise GPUError("ft_gpu_execute_tri_grid: %s" % _lib.ft_gpu_error_string(rc).decode())
     return A
 
 

@@ -189,3 +189,136 @@ void ft_execute_sphv_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, d
 void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
     execute_grid(3, TRANS, P, X, N, M);
 }
+
+/* ------------------------------------------------------------ triangle -- */
+
+/* fasttransforms.h:497-512 (fftw.c:275-330): the plan fixes the kinds of
+ * FFTW's r2r_2d over X as an N x M row-major buffer (synthesis REDFT01 on
+ * both axes, analysis REDFT10); the executors scale before or after it and
+ * do nothing unless N > 1 and M > 1, as the reference's do. */
+struct ft_triangle_fftw_plan_s {
+    int N, M, synthesis;
+    FtamdTriGrid *g;
+    void *d_stage;
+    size_t stage_bytes;
+    pthread_mutex_t lock;
+};
+
+static ft_triangle_fftw_plan *plan_tri_grid(int N, int M, int synthesis) {
+    if (N < 1 || M < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    ft_triangle_fftw_plan *P = calloc(1, sizeof(ft_triangle_fftw_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    P->N = N;
+    P->M = M;
+    P->synthesis = synthesis;
+    pthread_mutex_init(&P->lock, NULL);
+    rc = ftamd_trigrid_create(N, M, synthesis, &P->g);
+    if (rc != FTAMD_OK) {
+        pthread_mutex_destroy(&P->lock);
+        free(P);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    return P;
+}
+
+ft_triangle_fftw_plan *ft_plan_tri_synthesis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_tri_grid(N, M, 1);
+}
+ft_triangle_fftw_plan *ft_plan_tri_analysis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_tri_grid(N, M, 0);
+}
+
+void ft_destroy_triangle_fftw_plan(ft_triangle_fftw_plan *P) {
+    if (!P) return;
+    ftamd_trigrid_destroy(P->g);
+    ftamd_hip_free(P->d_stage);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* The scalings of executor `analysis` and TRANS (fftw.c:290-330). */
+static int tri_steps(int analysis, char TRANS, int N, int M, int *pre_fix, double *pre_mul, double *pre_div,
+                     int *post_fix, double *post_div) {
+    if (TRANS != 'N' && TRANS != 'T') return 0;
+    *pre_fix = 0, *pre_mul = 1.0, *pre_div = 0.0, *post_fix = 0, *post_div = 0.0;
+    const double nm = (double)N * (double)M;
+    if (!analysis && TRANS == 'N') *pre_fix = 1, *pre_mul = 0.25;
+    else if (!analysis) *pre_mul = 0.25;
+    else if (TRANS == 'N') *post_fix = 1, *post_div = nm;
+    else *pre_div = nm;
+    return 1;
+}
+
+int ft_gpu_execute_tri_grid(int analysis, char TRANS, const ft_triangle_fftw_plan *P_, double *dX, int N, int M,
+                            void *stream) {
+    ft_triangle_fftw_plan *P = (ft_triangle_fftw_plan *)P_;
+    int pf, qf;
+    double pm, pd, qd;
+    if ((analysis != 0 && analysis != 1) || !tri_steps(analysis, TRANS, N, M, &pf, &pm, &pd, &qf, &qd) || !P ||
+        P->N != N || P->M != M || !dX)
+        return FT_GPU_EINVAL;
+    if (N < 2 || M < 2) return FT_GPU_OK;
+    pthread_mutex_lock(&P->lock);
+    int rc = ftamd_trigrid_run(P->g, dX, pf, pm, pd, qf, qd, stream);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+static void execute_tri_grid(int analysis, char TRANS, const ft_triangle_fftw_plan *P_, double *X, int N, int M) {
+    ft_triangle_fftw_plan *P = (ft_triangle_fftw_plan *)P_;
+    int pf, qf;
+    double pm, pd, qd;
+    if (!tri_steps(analysis, TRANS, N, M, &pf, &pm, &pd, &qf, &qd) || !P || P->N != N || P->M != M || !X) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    if (N < 2 || M < 2) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(X, &pk);
+    const size_t bytes = (size_t)N * (size_t)M * sizeof(double);
+    pthread_mutex_lock(&P->lock);
+    void *d = X;
+    if (pk != FTAMD_PTR_DEVICE) {
+        if (P->stage_bytes < bytes) {
+            ftamd_hip_free(P->d_stage);
+            P->d_stage = NULL;
+            P->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&P->d_stage, bytes);
+            if (rc == FTAMD_OK) P->stage_bytes = bytes;
+        }
+        d = P->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = ftamd_trigrid_run(P->g, d, pf, pm, pd, qf, qd, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&P->lock);
+    ftamd_set_error(rc);
+}
+
+/* fasttransforms.h:509-512 (fftw.c:290-330) */
+void ft_execute_tri_synthesis(const char TRANS, const ft_triangle_fftw_plan *P, double *X, const int N, const int M) {
+    execute_tri_grid(0, TRANS, P, X, N, M);
+}
+void ft_execute_tri_analysis(const char TRANS, const ft_triangle_fftw_plan *P, double *X, const int N, const int M) {
+    execute_tri_grid(1, TRANS, P, X, N, M);
+}

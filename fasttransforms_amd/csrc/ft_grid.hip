@@ -123,6 +123,84 @@ __global__ void post2(const double2 *__restrict__ Z, double *__restrict__ X, con
     X[(long long)N * j + k] = y;
 }
 
+// ---- the triangle's two-dimensional DCTs (fftw.c:280-330): FFTW's r2r_2d
+// over the buffer as an N x M row-major array, so lines of length M at
+// stride 1 (N of them) and lines of length N at stride M (M of them).
+// Element t of line b is X[t*ts + b*bs]; the scalings act on the buffer
+// index e as the reference writes them (e < N: "the first column", e % N ==
+// 0: "the first row" of its column-major view).
+
+struct AxisScale {
+    int pre_fix;     // x2 where e < N and again where e % N == 0 (before the transform)
+    double pre_mul;  // then x pre_mul
+    double pre_div;  // then / pre_div (0: none)
+    int post_fix;    // x0.5 where e < N and again where e % N == 0 (after the transform)
+    double post_div; // then / post_div (0: none)
+};
+
+__device__ __forceinline__ double scale_pre(double v, long long e, int Nc, const AxisScale &sc) {
+    if (sc.pre_fix) {
+        if (e < Nc) v *= 2.0;
+        if (e % Nc == 0) v *= 2.0;
+    }
+    v *= sc.pre_mul;
+    if (sc.pre_div != 0.0) v /= sc.pre_div;
+    return v;
+}
+
+__device__ __forceinline__ double scale_post(double v, long long e, int Nc, const AxisScale &sc) {
+    if (sc.post_fix) {
+        if (e < Nc) v *= 0.5;
+        if (e % Nc == 0) v *= 0.5;
+    }
+    if (sc.post_div != 0.0) v /= sc.post_div;
+    return v;
+}
+
+__global__ void axis_pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw,
+                          int len, int batch, long long ts, long long bs, int Nc, AxisScale sc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (k >= len || b >= batch) return;
+    auto val = [&](int t) {
+        const long long e = t * ts + b * bs;
+        return scale_pre(X[e], e, Nc, sc);
+    };
+    const double re = val(k), im = k == 0 ? 0.0 : -val(len - k);
+    const double2 t = tw[k];
+    Z[(long long)len * b + k] = make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
+}
+
+__global__ void axis_post3(const double2 *__restrict__ Z, double *__restrict__ X, int len, int batch, long long ts,
+                           long long bs, int Nc, AxisScale sc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (n >= len || b >= batch) return;
+    const int src = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
+    const long long e = n * ts + b * bs;
+    X[e] = scale_post(Z[(long long)len * b + src].x, e, Nc, sc);
+}
+
+__global__ void axis_pre2(const double *__restrict__ X, double2 *__restrict__ Z, int len, int batch, long long ts,
+                          long long bs, int Nc, AxisScale sc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (n >= len || b >= batch) return;
+    const long long e = n * ts + b * bs;
+    const int dst = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
+    Z[(long long)len * b + dst] = make_double2(scale_pre(X[e], e, Nc, sc), 0.0);
+}
+
+__global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X, const double2 *__restrict__ tw,
+                           int len, int batch, long long ts, long long bs, int Nc, AxisScale sc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (k >= len || b >= batch) return;
+    const double2 V = Z[(long long)len * b + k], t = tw[k];
+    const long long e = k * ts + b * bs;
+    X[e] = scale_post(2.0 * (t.x * V.x + t.y * V.y), e, Nc, sc);
+}
+
 }  // namespace ftgrid
 
 static int fft_ok(hipfftResult r) { return r == HIPFFT_SUCCESS ? FTAMD_OK : FTAMD_EDEVICE; }
@@ -211,4 +289,97 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     if (rc != FTAMD_OK) return rc;
     hipLaunchKernelGGL(ftgrid::post2, grid, block, 0, st, g->Z, X, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1);
     return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+}
+
+// ---- triangle grids
+
+struct FtamdTriGrid {
+    int N, M, type3;
+    hipfftHandle ax[2];  // [0] length M, N lines; [1] length N, M lines
+    double2 *tw[2];      // e^{i pi k / 2L} for the two lengths L = M, N
+    double2 *Z;          // N x M complex
+};
+
+static int twiddles(int L, double2 **out) {
+    double2 *h = (double2 *)malloc(sizeof(double2) * (size_t)L);
+    if (!h) return FTAMD_ENOMEM;
+    for (int k = 0; k < L; k++) {
+        const long double th = 3.141592653589793238462643383279502884L * k / (2.0L * L);
+        h[k] = make_double2((double)cosl(th), (double)sinl(th));
+    }
+    int rc = hipMalloc((void **)out, sizeof(double2) * (size_t)L) == hipSuccess ? FTAMD_OK : FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && hipMemcpy(*out, h, sizeof(double2) * (size_t)L, hipMemcpyHostToDevice) != hipSuccess)
+        rc = FTAMD_EDEVICE;
+    free(h);
+    return rc;
+}
+
+extern "C" int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out) {
+    *out = NULL;
+    /* lines are the launch grid's y dimension (at most 65535) */
+    if (N < 1 || M < 1 || N > 65535 || M > 65535) return FTAMD_EINVAL;
+    FtamdTriGrid *g = (FtamdTriGrid *)calloc(1, sizeof(FtamdTriGrid));
+    if (!g) return FTAMD_ENOMEM;
+    g->N = N;
+    g->M = M;
+    g->type3 = type3;
+    int rc = twiddles(M, &g->tw[0]);
+    if (rc == FTAMD_OK) rc = twiddles(N, &g->tw[1]);
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->Z, sizeof(double2) * (size_t)N * M) != hipSuccess) rc = FTAMD_ENOMEM;
+    int lm[1] = {M}, ln[1] = {N};
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->ax[0], 1, lm, lm, 1, M, lm, 1, M, HIPFFT_Z2Z, N));
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->ax[1], 1, ln, ln, 1, N, ln, 1, N, HIPFFT_Z2Z, M));
+    if (rc != FTAMD_OK) {
+        ftamd_trigrid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FTAMD_OK;
+}
+
+extern "C" void ftamd_trigrid_destroy(FtamdTriGrid *g) {
+    if (!g) return;
+    hipDeviceSynchronize();
+    for (int i = 0; i < 2; i++) {
+        if (g->ax[i]) hipfftDestroy(g->ax[i]);
+        hipFree(g->tw[i]);
+    }
+    hipFree(g->Z);
+    free(g);
+}
+
+extern "C" int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
+                                 double post_div, void *stream) {
+    if (!g || !X) return FTAMD_EINVAL;
+    const int N = g->N, M = g->M;
+    const hipStream_t st = (hipStream_t)stream;
+    const ftgrid::AxisScale none = {0, 1.0, 0.0, 0, 0.0};
+    for (int pass = 0; pass < 2; pass++) {
+        // pass 0: lines of length M at stride 1 (N lines, M apart); pass 1: lines of length N at stride M
+        const int len = pass == 0 ? M : N, batch = pass == 0 ? N : M;
+        const long long ts = pass == 0 ? 1 : M, bs = pass == 0 ? M : 1;
+        ftgrid::AxisScale sc = none;
+        if (pass == 0) sc.pre_fix = pre_fix, sc.pre_mul = pre_mul, sc.pre_div = pre_div;
+        if (pass == 1) sc.post_fix = post_fix, sc.post_div = post_div;
+        const dim3 block(256), grid((len + 255) / 256, batch);
+        int rc = fft_ok(hipfftSetStream(g->ax[pass], st));
+        if (rc != FTAMD_OK) return rc;
+        if (g->type3) {
+            hipLaunchKernelGGL(ftgrid::axis_pre3, grid, block, 0, st, X, g->Z, g->tw[pass], len, batch, ts, bs, N, sc);
+            if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+            rc = fft_ok(hipfftExecZ2Z(g->ax[pass], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
+                                      HIPFFT_BACKWARD));
+            if (rc != FTAMD_OK) return rc;
+            hipLaunchKernelGGL(ftgrid::axis_post3, grid, block, 0, st, g->Z, X, len, batch, ts, bs, N, sc);
+        } else {
+            hipLaunchKernelGGL(ftgrid::axis_pre2, grid, block, 0, st, X, g->Z, len, batch, ts, bs, N, sc);
+            if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+            rc = fft_ok(hipfftExecZ2Z(g->ax[pass], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
+                                      HIPFFT_FORWARD));
+            if (rc != FTAMD_OK) return rc;
+            hipLaunchKernelGGL(ftgrid::axis_post2, grid, block, 0, st, g->Z, X, g->tw[pass], len, batch, ts, bs, N, sc);
+        }
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+    }
+    return FTAMD_OK;
 }

@@ -198,6 +198,18 @@ int ftamd_grid_create(int N, int M, int type3, int plan_sphv, FtamdGrid **out);
 void ftamd_grid_destroy(FtamdGrid *g);
 int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0, double f1, double a, double b, void *stream);
 
+/* The triangle's two-dimensional DCTs (ft_grid.hip): DCT-III (type3) or
+ * DCT-II along both axes of the buffer as FFTW's r2r_2d(N, M) reads it (an
+ * N x M row-major array).  Before the first axis: if pre_fix, x2 where the
+ * buffer index e < N and again where e % N == 0, then x pre_mul and
+ * / pre_div (0: none); after the second axis: if post_fix, x0.5 at the same
+ * places, then / post_div (0: none). */
+typedef struct FtamdTriGrid FtamdTriGrid;
+int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out);
+void ftamd_trigrid_destroy(FtamdTriGrid *g);
+int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
+                      double post_div, void *stream);
+
 /* The spin-weighted sphere's first stage (ft_k_spin.hip) on the complex
  * columns c0 .. c0+count-1 of A (interleaved re, im): column c at element
  * 2*(col_stride*(c - c0)) with rows at complex stride row_stride, of order

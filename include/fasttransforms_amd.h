@@ -290,6 +290,24 @@ void ft_execute_sph_analysis(const char TRANS, const ft_sphere_fftw_plan *P, dou
 void ft_execute_sphv_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
 void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M);
 
+/* fasttransforms.h:497-512 (fftw.c:275-330) -- synthesis and analysis on the
+ * triangle's grid: FFTW's two-dimensional REDFT01 (synthesis plans) or
+ * REDFT10 (analysis plans) over X read, as the reference's r2r_2d(N, M) reads
+ * it, as an N x M row-major buffer (for N = M, as the reference's tests use,
+ * the column-major array's columns and rows), with the executors' scalings;
+ * nothing happens unless N > 1 and M > 1.  Any plan of size N x M runs with
+ * either executor and TRANS (the plan fixes the kinds). */
+typedef struct ft_triangle_fftw_plan_s ft_triangle_fftw_plan;
+void ft_destroy_triangle_fftw_plan(ft_triangle_fftw_plan *P);
+ft_triangle_fftw_plan *ft_plan_tri_synthesis(const int N, const int M, const unsigned flags);
+ft_triangle_fftw_plan *ft_plan_tri_analysis(const int N, const int M, const unsigned flags);
+void ft_execute_tri_synthesis(const char TRANS, const ft_triangle_fftw_plan *P, double *X, const int N, const int M);
+void ft_execute_tri_analysis(const char TRANS, const ft_triangle_fftw_plan *P, double *X, const int N, const int M);
+/* The triangle's executors on a device-resident X on a HIP stream
+ * (analysis 0: synthesis, 1: analysis). */
+int ft_gpu_execute_tri_grid(int analysis, char TRANS, const ft_triangle_fftw_plan *P, double *dX, int N, int M,
+                            void *stream);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
  * analysis, TRANS and the plan as the executors above.  Calls on one plan

@@ -633,3 +633,37 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
     else:
         X[0, cos] *= 0.5
     return X
+
+
+def tri_grid(analysis, trans, synthesis_plan, X):
+    """ft_execute_tri_{synthesis,analysis}(TRANS, P, X, N, M) restated
+    (fftw.c:280-330) on a copy of the N x M column-major array X: FFTW's
+    r2r_2d(N, M) reads the buffer as an N x M row-major array; the plan's
+    kinds are REDFT01 on both axes (synthesis plan) or REDFT10."""
+    import scipy.fft as sf
+    X = np.array(X, dtype=np.float64, order="F")
+    N, M = X.shape
+    if N < 2 or M < 2:
+        return X
+    buf = X.reshape(-1, order="F").copy()
+    e = np.arange(N * M)
+    first = (e < N).astype(float) + 1.0   # 2 where e < N
+    zero = (e % N == 0).astype(float) + 1.0
+
+    def r2r(b):
+        B = b.reshape((N, M))             # row-major view
+        t = 3 if synthesis_plan else 2
+        return sf.dct(sf.dct(B, type=t, axis=1), type=t, axis=0).reshape(-1)
+
+    nm = float(N) * float(M)
+    if not analysis and trans == "N":
+        buf = buf * first * zero * 0.25
+        buf = r2r(buf)
+    elif not analysis:
+        buf = r2r(buf * 0.25)
+    elif trans == "N":
+        buf = r2r(buf)
+        buf = buf / first / zero / nm
+    else:
+        buf = r2r(buf / nm)
+    return buf.reshape((N, M), order="F")

@@ -83,3 +83,59 @@ def test_spherical_harmonic_transform_round_trip(ft, n):
     e2, einf = O.rel2(X, A), O.relinf(X, A)
     _say("SHT n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
     assert e2 < 4 * np.sqrt(n) * EPS and einf < 2 * n * EPS
+
+
+@pytest.mark.parametrize("analysis", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_tri_against_oracle(ft, analysis, trans):
+    for N, M in [(1, 4), (2, 2), (5, 5), (5, 8), (64, 64), (65, 65), (300, 300), (1024, 1024)]:
+        for syn_plan in (True, False):
+            P = (ft.plan_tri_synthesis if syn_plan else ft.plan_tri_analysis)(N, M)
+            X = np.asfortranarray(np.random.default_rng(N + 3 * M).standard_normal((N, M)))
+            name = "tri_analysis" if analysis else "tri_synthesis"
+            _say("grid", name, trans, "synthesis plan" if syn_plan else "analysis plan", "N=%d M=%d" % (N, M))
+            got = getattr(ft, "execute_" + name)(trans, P, X.copy(order="F"))
+            ref = O.tri_grid(analysis, trans, syn_plan, X)
+            assert O.rel2(got, ref) <= 2e-14, (name, trans, syn_plan, N, M, O.rel2(got, ref))
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_triangle_transform_round_trip(ft, n):
+    """test_fftw.c:164-196 on the device: tri2cheb, synthesis, analysis,
+    cheb2tri with 'N' and with 'T'.  At n = 64 (the reference's test size)
+    both chains in a row against its bounds (32 sqrt(N) eps rel-2, 8N eps
+    rel-inf).  At n = 1024 the chains' conditioning sets the error: the
+    oracle's own chains (same input, connection products in long double)
+    return it within 1.9e-14 ('N') and 2.9e-13 ('T') rel-2; here the products
+    are FP64, as the reference's BLAS products are (measured 4.6e-14 and
+    1.1e-12), so each chain is checked on its own against 1e-13 and 4e-12."""
+    import torch
+    H = ft.plan_tri2cheb(n, 0.0, 0.0, 0.0)
+    QS, QA = ft.plan_tri_synthesis(n, n), ft.plan_tri_analysis(n, n)
+    A = O.fill("tri", n, n, 41)
+
+    def chain(T, trans):
+        syn, ana = (QS, QA) if trans == "N" else (QA, QS)
+        ft.gpu_execute_harmonic("tri2cheb", trans, H, T)
+        ft.gpu_execute_tri_grid("tri_synthesis", trans, syn, T)
+        ft.gpu_execute_tri_grid("tri_analysis", trans, ana, T)
+        ft.gpu_execute_harmonic("cheb2tri", trans, H, T)
+
+    _say("triangle transform round trip n=%d" % n)
+    if n <= 64:
+        T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+        chain(T, "N")
+        chain(T, "T")
+        ft.synchronize()
+        X = T.cpu().numpy().T
+        e2, einf = O.rel2(X, A), O.relinf(X, A)
+        _say("triangle n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
+        assert e2 < 32 * np.sqrt(n) * EPS and einf < 8 * n * EPS
+        return
+    for trans, bound in (("N", 1e-13), ("T", 4e-12)):
+        T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+        chain(T, trans)
+        ft.synchronize()
+        X = T.cpu().numpy().T
+        _say("triangle n=%d %s chain rel2 %.2e relinf %.2e" % (n, trans, O.rel2(X, A), O.relinf(X, A)))
+        assert O.rel2(X, A) < bound

@@ -118,3 +118,42 @@ def test_library_rejects_bad_shapes(ft):
         ft.plan_sph_synthesis(8, 14)
     with pytest.raises(ft.GPUError):
         ft.plan_sph_analysis(0, 15)
+
+
+@pytest.mark.parametrize("N", [2, 3, 6])
+def test_tri_synthesis_of_unit_coefficients(N):
+    """fftw.c:290-300 with N = M: a unit coefficient at (l, c) gives
+    cos(l theta_n) cos(c theta_m) on the grid theta_n = (2n+1) pi / 2N."""
+    th = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    for l in range(N):
+        for c in range(N):
+            X = np.zeros((N, N))
+            X[l, c] = 1.0
+            G = O.tri_grid(False, "N", True, X)
+            assert np.max(np.abs(G - np.outer(np.cos(l * th), np.cos(c * th)))) <= 32 * N * EPS, (l, c)
+
+
+@pytest.mark.parametrize("N,M", [(2, 2), (3, 3), (4, 4), (7, 7)])
+def test_tri_transposes_and_inverse(N, M):
+    """For N = M (the reference's tests' shape).  For N != M the reference's
+    scalings (on the column-major array's first column and row) and FFTW's
+    r2r_2d (over the buffer as an N x M row-major array) address different
+    elements, so these identities do not hold there; the library reproduces
+    that behaviour as the oracle restates it."""
+    def dense(analysis, trans, syn):
+        D = np.zeros((N * M, N * M))
+        for i in range(N * M):
+            e = np.zeros(N * M)
+            e[i] = 1.0
+            D[:, i] = O.tri_grid(analysis, trans, syn, e.reshape((N, M), order="F")).reshape(-1, order="F")
+        return D
+    SN, ST = dense(False, "N", True), dense(False, "T", False)
+    AN, AT = dense(True, "N", False), dense(True, "T", True)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
+
+
+def test_tri_small_sizes_are_untouched():
+    X = np.random.default_rng(1).standard_normal((1, 5))
+    assert np.array_equal(O.tri_grid(False, "N", True, X), X)
