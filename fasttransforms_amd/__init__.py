@@ -105,6 +105,10 @@ for _t in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_sphere_fftw_plan"] = (None, [_vp])
+for _t in ("disk_synthesis", "disk_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_disk_fftw_plan"] = (None, [_vp])
 for _t in ("tri_synthesis", "tri_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
@@ -330,7 +334,10 @@ plan_sph_synthesis = _plan_grid("sph_synthesis", True, False)
 plan_sph_analysis = _plan_grid("sph_analysis", False, False)
 plan_sphv_synthesis = _plan_grid("sphv_synthesis", True, True)
 plan_sphv_analysis = _plan_grid("sphv_analysis", False, True)
-_GRID_WHICH = {"sph_synthesis": 0, "sph_analysis": 1, "sphv_synthesis": 2, "sphv_analysis": 3}
+plan_disk_synthesis = _plan_grid("disk_synthesis", True, False)
+plan_disk_analysis = _plan_grid("disk_analysis", False, False)
+_GRID_WHICH = {"sph_synthesis": 0, "sph_analysis": 1, "sphv_synthesis": 2, "sphv_analysis": 3,
+               "disk_synthesis": 4, "disk_analysis": 5}
 
 
 def _execute_grid(name):
@@ -350,6 +357,8 @@ execute_sph_synthesis = _execute_grid("sph_synthesis")
 execute_sph_analysis = _execute_grid("sph_analysis")
 execute_sphv_synthesis = _execute_grid("sphv_synthesis")
 execute_sphv_analysis = _execute_grid("sphv_analysis")
+execute_disk_synthesis = _execute_grid("disk_synthesis")
+execute_disk_analysis = _execute_grid("disk_analysis")
 
 
 def gpu_execute_sph_grid(name, trans, plan, A, stream=None):

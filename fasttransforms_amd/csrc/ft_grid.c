@@ -25,7 +25,8 @@
 struct ft_sphere_fftw_plan_s {
     int N, M;
     int synthesis; /* 1: DCT-III / DST-III and c2r; 0: DCT-II / DST-II and r2c */
-    int sphv;      /* the vector-field plans' column classes */
+    int sphv;      /* 0 the sphere's, 1 the vector field's column classes, 2 the disk's
+                      (DCT-IV, REDFT11, on the sine class in both plan kinds) */
     int device;
     FtamdGrid *g;
     void *d_stage; /* staging buffer of host-pointer calls */
@@ -42,6 +43,7 @@ struct ft_sphere_fftw_plan_s {
 #define FT_2_SQRT_PI 3.544907701811032055
 
 static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv) {
+    /* the disk's plans are the same object (ft_disk_fftw_plan is this type) */
     if (N < 1 || M < 1 || M % 2 == 0) {
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
@@ -101,15 +103,17 @@ void ft_destroy_sphere_fftw_plan(ft_sphere_fftw_plan *P) {
 }
 
 /* The step order, fix-ups and scalings of executor `which` (0 sph synthesis,
- * 1 sph analysis, 2 sphv synthesis, 3 sphv analysis) and TRANS; *need_syn:
- * the plan kind it runs on. */
+ * 1 sph analysis, 2 sphv synthesis, 3 sphv analysis, 4 disk synthesis, 5
+ * disk analysis) and TRANS; *need_syn: the plan kind it runs on.  The disk's
+ * executors (fftw.c:460-520) are the sphere's without the last-row fix-ups
+ * of the sine class. */
 typedef struct {
     int need_syn, exec_sphv;
     double f0, f1, a, b;
 } grid_step;
 
 static int grid_steps(int which, char TRANS, int N, int M, grid_step *s) {
-    const int analysis = which & 1, sphv = which >> 1;
+    const int analysis = which & 1, sphv = (which >> 1) == 1, disk = (which >> 1) == 2;
     if (TRANS != 'N' && TRANS != 'T') return 0;
     const int trans = TRANS == 'T';
     s->exec_sphv = sphv;
@@ -122,18 +126,21 @@ static int grid_steps(int which, char TRANS, int N, int M, grid_step *s) {
     } else { /* fftw.c:185-200: x2 last row (sine class), III, x 2 sqrt(pi)/(2NM), col 0 x sqrt 2, c2r */
         s->need_syn = 1, s->f0 = 1.0, s->f1 = 2.0, s->a = FT_2_SQRT_PI / (2.0 * N * M), s->b = FT_SQRT2;
     }
+    if (disk) s->f1 = 1.0;
     return 1;
 }
 
-static int valid_grid(const ft_sphere_fftw_plan *P, int N, int M, const grid_step *s) {
-    return P && P->N == N && P->M == M && P->synthesis == s->need_syn;
+/* The executor's geometry must be the plan's (a disk plan's sine class is
+ * another transform) and its kind the one the executor takes. */
+static int valid_grid(const ft_sphere_fftw_plan *P, int which, int N, int M, const grid_step *s) {
+    return P && P->N == N && P->M == M && P->synthesis == s->need_syn && ((which >> 1) == 2) == (P->sphv == 2);
 }
 
 int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, double *dX, int N, int M,
                             void *stream) {
     ft_sphere_fftw_plan *P = (ft_sphere_fftw_plan *)P_;
     grid_step s;
-    if (which < 0 || which > 3 || !grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, N, M, &s) || !dX)
+    if (which < 0 || which > 5 || !grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, which, N, M, &s) || !dX)
         return FT_GPU_EINVAL;
     pthread_mutex_lock(&P->lock);
     int rc = ftamd_grid_run(P->g, dX, s.exec_sphv, s.f0, s.f1, s.a, s.b, stream);
@@ -144,7 +151,7 @@ int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_
 static void execute_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, double *X, int N, int M) {
     ft_sphere_fftw_plan *P = (ft_sphere_fftw_plan *)P_;
     grid_step s;
-    if (!grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, N, M, &s) || !X) {
+    if (!grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, which, N, M, &s) || !X) {
         ftamd_set_error(FT_GPU_EINVAL);
         return;
     }
@@ -188,6 +195,25 @@ void ft_execute_sphv_synthesis(const char TRANS, const ft_sphere_fftw_plan *P, d
 }
 void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, double *X, const int N, const int M) {
     execute_grid(3, TRANS, P, X, N, M);
+}
+
+/* fasttransforms.h:542-556 (fftw.c:400-520): the disk's plans, REDFT01 or
+ * REDFT10 on columns 0 and 3 mod 4, REDFT11 on columns 1 and 2 mod 4, the
+ * phi DFT as the sphere's. */
+ft_disk_fftw_plan *ft_plan_disk_synthesis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 1, 2);
+}
+ft_disk_fftw_plan *ft_plan_disk_analysis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 0, 2);
+}
+void ft_destroy_disk_fftw_plan(ft_disk_fftw_plan *P) { ft_destroy_sphere_fftw_plan(P); }
+void ft_execute_disk_synthesis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(4, TRANS, P, X, N, M);
+}
+void ft_execute_disk_analysis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M) {
+    execute_grid(5, TRANS, P, X, N, M);
 }
 
 /* ------------------------------------------------------------ triangle -- */

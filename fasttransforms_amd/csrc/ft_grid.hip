@@ -24,9 +24,13 @@
 
 struct FtamdGrid {
     int N, M, type3, plan_sphv;
-    hipfftHandle theta, phi;
+    int disk;     // the disk's plans: DCT-IV (REDFT11) on the sine class
+    int nsin;     // sine-class columns (disk)
+    hipfftHandle theta, phi, iv;
     double2 *tw;  // e^{i pi k / 2N}, k < N
+    double2 *tw4; // e^{i pi (2k+1) / 4N}, k < N (disk)
     double2 *Z;   // N x M complex: the columns' FFTs
+    double2 *Z4;  // 2N x nsin complex: the disk's sine-class columns, zero-padded (disk)
     double2 *Y;   // N x (M/2+1) complex: the rows' half spectra
 };
 
@@ -40,12 +44,14 @@ __device__ __forceinline__ bool cos_class(int j, int sphv) {
 }
 
 // Pipeline with DCT-III first (synthesis plans): fix-ups, DCT-III / DST-III
-// of every column into Z (pre-twiddled for the inverse FFT).
+// of every column into Z (pre-twiddled for the inverse FFT).  skip_sin: the
+// sine class is the disk's DCT-IV, run by the iv kernels.
 __global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw, int N,
-                     int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin) {
+                     int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin, int skip_sin) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (k >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
     const double *x = X + (long long)N * j;
     const bool tcos = cos_class(j, plan_sphv), fcos = cos_class(j, exec_sphv);
     // entry e of the column after the executor's fix-ups
@@ -65,10 +71,11 @@ __global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, cons
 // After the inverse FFT: y_n of every column, signs of the sine class, the
 // scalings a (all) and b (column 0 again), into Y as data_r2c lays it out.
 __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
-                      double b) {
+                      double b, int skip_sin) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (n >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
     const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
     double y = Z[(long long)N * j + src].x;
     if (!cos_class(j, plan_sphv) && (n & 1)) y = -y;
@@ -86,10 +93,11 @@ __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, in
 // Pipeline with the row DFT first (analysis plans): data_c2r, the scalings,
 // the sine class's alternating signs, Makhoul's reordering into Z.
 __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int N, int M, int plan_sphv, double a,
-                     double b) {
+                     double b, int skip_sin) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (n >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
     const double *Yd = (const double *)Y;
     double x;
     if (j == 0) {
@@ -109,10 +117,11 @@ __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int
 // After the forward FFT: the DCT-II (cosine class) or DST-II (sine class) of
 // every column, then the executor's fix-ups, into X.
 __global__ void post2(const double2 *__restrict__ Z, double *__restrict__ X, const double2 *__restrict__ tw, int N,
-                      int M, int plan_sphv, int exec_sphv, double g_row0_cos, double g_last_sin) {
+                      int M, int plan_sphv, int exec_sphv, double g_row0_cos, double g_last_sin, int skip_sin) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (k >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
     const bool tcos = cos_class(j, plan_sphv), fcos = cos_class(j, exec_sphv);
     const int src = tcos ? k : N - 1 - k;
     const double2 V = Z[(long long)N * j + src];
@@ -121,6 +130,72 @@ __global__ void post2(const double2 *__restrict__ Z, double *__restrict__ X, con
     if (fcos && k == 0) y *= g_row0_cos;
     if (!fcos && k == N - 1) y *= g_last_sin;
     X[(long long)N * j + k] = y;
+}
+
+// ---- the disk's DCT-IV (REDFT11) on its sine class (fftw.c:450-458):
+// y_k = 2 sum_n x_n cos(pi (2n+1)(2k+1) / 4N) = 2 Re(e^{-i pi (2k+1)/4N} U_k),
+// U the 2N-point DFT of u_n = x_n e^{-i pi n / 2N} (n < N, zero above).
+// Sine-class column s (of nsin) is column 4(s/2) + 1 + s%2.
+
+__device__ __forceinline__ int sin_col(int s) { return 4 * (s >> 1) + 1 + (s & 1); }
+
+// u from the columns of X (synthesis plans; the executor's last-row fix-up f1)
+__global__ void pre4_x(const double *__restrict__ X, double2 *__restrict__ Z4, const double2 *__restrict__ tw, int N,
+                       int nsin, double f_last_sin) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (n >= 2 * N || s >= nsin) return;
+    double2 u = make_double2(0.0, 0.0);
+    if (n < N) {
+        double x = X[(long long)N * sin_col(s) + n];
+        if (n == N - 1) x *= f_last_sin;
+        const double2 t = tw[n];
+        u = make_double2(x * t.x, -x * t.y);
+    }
+    Z4[(long long)2 * N * s + n] = u;
+}
+
+// u from the half spectra Y after the phi DFT (analysis plans): data_c2r and the scaling a
+__global__ void pre4_y(const double2 *__restrict__ Y, double2 *__restrict__ Z4, const double2 *__restrict__ tw, int N,
+                       int nsin, double a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (n >= 2 * N || s >= nsin) return;
+    double2 u = make_double2(0.0, 0.0);
+    if (n < N) {
+        const int j = sin_col(s), q = (j + 1) >> 1;
+        const double v = ((const double *)Y)[2 * ((long long)N * q + n) + ((j & 1) ? 1 : 0)];
+        double x = (j & 1) ? -v : v;
+        x *= a;
+        const double2 t = tw[n];
+        u = make_double2(x * t.x, -x * t.y);
+    }
+    Z4[(long long)2 * N * s + n] = u;
+}
+
+// y into Y (synthesis plans): the scaling a, data_r2c
+__global__ void post4_y(const double2 *__restrict__ Z4, double2 *__restrict__ Y, const double2 *__restrict__ tw4,
+                        int N, int nsin, double a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= N || s >= nsin) return;
+    const double2 U = Z4[(long long)2 * N * s + k], t = tw4[k];
+    double y = 2.0 * (t.x * U.x + t.y * U.y);
+    y *= a;
+    const int j = sin_col(s), q = (j + 1) >> 1;
+    ((double *)Y)[2 * ((long long)N * q + k) + ((j & 1) ? 1 : 0)] = (j & 1) ? -y : y;
+}
+
+// y into X (analysis plans), the executor's last-row fix-up g1
+__global__ void post4_x(const double2 *__restrict__ Z4, double *__restrict__ X, const double2 *__restrict__ tw4, int N,
+                        int nsin, double g_last_sin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= N || s >= nsin) return;
+    const double2 U = Z4[(long long)2 * N * s + k], t = tw4[k];
+    double y = 2.0 * (t.x * U.x + t.y * U.y);
+    if (k == N - 1) y *= g_last_sin;
+    X[(long long)N * sin_col(s) + k] = y;
 }
 
 // ---- the triangle's two-dimensional DCTs (fftw.c:280-330): FFTW's r2r_2d
@@ -205,17 +280,19 @@ __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X
 
 static int fft_ok(hipfftResult r) { return r == HIPFFT_SUCCESS ? FTAMD_OK : FTAMD_EDEVICE; }
 
-extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_sphv, FtamdGrid **out) {
+extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, FtamdGrid **out) {
     *out = NULL;
     /* columns are the launch grid's y dimension (at most 65535) */
-    if (N < 1 || M < 1 || M % 2 == 0 || M > 65535) return FTAMD_EINVAL;
+    if (N < 1 || M < 1 || M % 2 == 0 || M > 65535 || plan_mode < 0 || plan_mode > 2) return FTAMD_EINVAL;
     FtamdGrid *g = (FtamdGrid *)calloc(1, sizeof(FtamdGrid));
     if (!g) return FTAMD_ENOMEM;
     g->N = N;
     g->M = M;
     g->type3 = type3;
-    g->plan_sphv = plan_sphv;
-    g->theta = g->phi = 0;
+    g->plan_sphv = plan_mode == 1;
+    g->disk = plan_mode == 2;
+    g->nsin = g->disk ? (M + 2) / 4 + (M + 1) / 4 : 0;  /* columns 1 and 2 mod 4 (fftw.c:427-431) */
+    g->theta = g->phi = g->iv = 0;
     int rc = FTAMD_OK;
     const size_t zbytes = sizeof(double2) * (size_t)N * M, ybytes = sizeof(double2) * (size_t)N * (M / 2 + 1);
     double2 *tw = (double2 *)malloc(sizeof(double2) * (size_t)N);
@@ -230,6 +307,24 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_sphv, FtamdGr
     if (rc == FTAMD_OK && hipMemcpy(g->tw, tw, sizeof(double2) * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
         rc = FTAMD_EDEVICE;
     free(tw);
+    if (rc == FTAMD_OK && g->disk && g->nsin > 0) {
+        double2 *h = (double2 *)malloc(sizeof(double2) * (size_t)N);
+        if (!h) rc = FTAMD_ENOMEM;
+        for (int k = 0; rc == FTAMD_OK && k < N; k++) {
+            const long double th = 3.141592653589793238462643383279502884L * (2 * k + 1) / (4.0L * N);
+            h[k] = make_double2((double)cosl(th), (double)sinl(th));
+        }
+        if (rc == FTAMD_OK && hipMalloc((void **)&g->tw4, sizeof(double2) * (size_t)N) != hipSuccess) rc = FTAMD_ENOMEM;
+        if (rc == FTAMD_OK && hipMemcpy(g->tw4, h, sizeof(double2) * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
+            rc = FTAMD_EDEVICE;
+        free(h);
+        if (rc == FTAMD_OK && hipMalloc((void **)&g->Z4, sizeof(double2) * 2 * (size_t)N * g->nsin) != hipSuccess)
+            rc = FTAMD_ENOMEM;
+        int n4[1] = {2 * N};
+        if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->iv, 1, n4, n4, 1, 2 * N, n4, 1, 2 * N, HIPFFT_Z2Z, g->nsin));
+    }
+    /* the sine-class slots of Z stay zero in the disk's plans (never written) */
+    if (rc == FTAMD_OK && g->disk && hipMemset(g->Z, 0, zbytes) != hipSuccess) rc = FTAMD_EDEVICE;
     // theta: M contiguous complex transforms of length N
     int n1[1] = {N};
     if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, M));
@@ -254,8 +349,11 @@ extern "C" void ftamd_grid_destroy(FtamdGrid *g) {
     hipDeviceSynchronize();
     if (g->theta) hipfftDestroy(g->theta);
     if (g->phi) hipfftDestroy(g->phi);
+    if (g->iv) hipfftDestroy(g->iv);
     hipFree(g->tw);
+    hipFree(g->tw4);
     hipFree(g->Z);
+    hipFree(g->Z4);
     hipFree(g->Y);
     free(g);
 }
@@ -268,26 +366,51 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     const dim3 block(256), grid((N + 255) / 256, M);
     int rc = fft_ok(hipfftSetStream(g->theta, st));
     if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(g->phi, st));
+    if (rc == FTAMD_OK && g->iv) rc = fft_ok(hipfftSetStream(g->iv, st));
     if (rc != FTAMD_OK) return rc;
+    const int iv = g->disk && g->nsin > 0;
+    const dim3 grid4((2 * N + 255) / 256, g->nsin > 0 ? g->nsin : 1), grid4o((N + 255) / 256, g->nsin > 0 ? g->nsin : 1);
+    // the disk's sine class: DCT-IV of the columns (synthesis plans, before the phi DFT)
+    auto iv_from_x = [&]() -> int {
+        hipLaunchKernelGGL(ftgrid::pre4_x, grid4, block, 0, st, X, g->Z4, g->tw, N, g->nsin, f1);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        int r = fft_ok(hipfftExecZ2Z(g->iv, (hipfftDoubleComplex *)g->Z4, (hipfftDoubleComplex *)g->Z4, HIPFFT_FORWARD));
+        if (r != FTAMD_OK) return r;
+        hipLaunchKernelGGL(ftgrid::post4_y, grid4o, block, 0, st, g->Z4, g->Y, g->tw4, N, g->nsin, a);
+        return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+    };
+    // ... and from the half spectra (analysis plans, after the phi DFT)
+    auto iv_from_y = [&]() -> int {
+        hipLaunchKernelGGL(ftgrid::pre4_y, grid4, block, 0, st, g->Y, g->Z4, g->tw, N, g->nsin, a);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        int r = fft_ok(hipfftExecZ2Z(g->iv, (hipfftDoubleComplex *)g->Z4, (hipfftDoubleComplex *)g->Z4, HIPFFT_FORWARD));
+        if (r != FTAMD_OK) return r;
+        hipLaunchKernelGGL(ftgrid::post4_x, grid4o, block, 0, st, g->Z4, X, g->tw4, N, g->nsin, f1);
+        return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+    };
     if (g->type3) {
         // fix-ups f0 (row 0, cosine columns) and f1 (last row, sine columns),
-        // DCT-III / DST-III, scalings, data_r2c, c2r
-        hipLaunchKernelGGL(ftgrid::pre3, grid, block, 0, st, X, g->Z, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1);
+        // DCT-III / DST-III (the disk: DCT-IV on the sine class), scalings,
+        // data_r2c, c2r
+        hipLaunchKernelGGL(ftgrid::pre3, grid, block, 0, st, X, g->Z, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1, iv);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
         if (rc != FTAMD_OK) return rc;
-        hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b);
+        hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, iv);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        if (iv && (rc = iv_from_x()) != FTAMD_OK) return rc;
         return fft_ok(hipfftExecZ2D(g->phi, (hipfftDoubleComplex *)g->Y, X));
     }
-    // r2c, data_c2r, scalings, DCT-II / DST-II, fix-ups f0 and f1
+    // r2c, data_c2r, scalings, DCT-II / DST-II (the disk: DCT-IV on the sine
+    // class), fix-ups f0 and f1
     rc = fft_ok(hipfftExecD2Z(g->phi, X, (hipfftDoubleComplex *)g->Y));
     if (rc != FTAMD_OK) return rc;
-    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b);
+    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv);
     if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+    if (iv && (rc = iv_from_y()) != FTAMD_OK) return rc;
     rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));
     if (rc != FTAMD_OK) return rc;
-    hipLaunchKernelGGL(ftgrid::post2, grid, block, 0, st, g->Z, X, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1);
+    hipLaunchKernelGGL(ftgrid::post2, grid, block, 0, st, g->Z, X, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1, iv);
     return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }
 

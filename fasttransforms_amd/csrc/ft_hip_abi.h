@@ -187,14 +187,15 @@ int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream);
 /* Synthesis / analysis on the sphere's grid (ft_grid.hip): hipFFT plans and
  * work buffers for N x M (M odd).  type3: a synthesis plan (DCT-III / DST-III
  * along theta, c2r along phi), else an analysis plan (DCT-II / DST-II, r2c);
- * plan_sphv: the vector-field plans' column classes.  ftamd_grid_run: with a
+ * plan_mode: 0 the sphere's column classes, 1 the vector field's, 2 the
+ * disk's (DCT-IV on the sine class in both kinds).  ftamd_grid_run: with a
  * synthesis plan, fix-ups (f0 on row 0 of the executor's cosine columns, f1
  * on the last row of its sine columns), the theta transforms, scalings (a on
  * every entry, b on column 0 again) and the phi transform; with an analysis
  * plan the phi transform, the scalings, the theta transforms and the
  * fix-ups.  X (device, N x M) is overwritten in place. */
 typedef struct FtamdGrid FtamdGrid;
-int ftamd_grid_create(int N, int M, int type3, int plan_sphv, FtamdGrid **out);
+int ftamd_grid_create(int N, int M, int type3, int plan_mode, FtamdGrid **out);
 void ftamd_grid_destroy(FtamdGrid *g);
 int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0, double f1, double a, double b, void *stream);
 

@@ -308,10 +308,25 @@ void ft_execute_tri_analysis(const char TRANS, const ft_triangle_fftw_plan *P, d
 int ft_gpu_execute_tri_grid(int analysis, char TRANS, const ft_triangle_fftw_plan *P, double *dX, int N, int M,
                             void *stream);
 
+/* fasttransforms.h:542-556 (fftw.c:400-520) -- synthesis and analysis on
+ * the disk's grid (coefficients of the Chebyshev x Fourier series disk2cxf
+ * produces): REDFT01 / REDFT10 on columns 0 and 3 mod 4, REDFT11 on columns
+ * 1 and 2 mod 4, the phi DFT as on the sphere; the same rules for plans,
+ * TRANS and M as the sphere's executors.  ft_disk_fftw_plan is the sphere's
+ * plan type; a disk executor takes a disk plan only, and a sphere executor
+ * a sphere plan only. */
+typedef ft_sphere_fftw_plan ft_disk_fftw_plan;
+void ft_destroy_disk_fftw_plan(ft_disk_fftw_plan *P);
+ft_disk_fftw_plan *ft_plan_disk_synthesis(const int N, const int M, const unsigned flags);
+ft_disk_fftw_plan *ft_plan_disk_analysis(const int N, const int M, const unsigned flags);
+void ft_execute_disk_synthesis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M);
+void ft_execute_disk_analysis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
- * analysis, TRANS and the plan as the executors above.  Calls on one plan
- * share its work buffers: order them on one stream. */
+ * analysis, 4 disk synthesis, 5 disk analysis, TRANS and the plan as the
+ * executors above.  Calls on one plan share its work buffers: order them on
+ * one stream. */
 int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P, double *dX, int N, int M,
                             void *stream);
 

@@ -567,12 +567,15 @@ def _grid_classes(M, sphv):
     return ~c if sphv else c
 
 
-def _theta(X, synthesis, plan_sphv):
+def _theta(X, synthesis, plan_sphv, disk=False):
     import scipy.fft as sf
     cos = _grid_classes(X.shape[1], plan_sphv)
     t = 3 if synthesis else 2
     X[:, cos] = sf.dct(X[:, cos], type=t, axis=0)
-    X[:, ~cos] = sf.dst(X[:, ~cos], type=t, axis=0)
+    if disk:   # fftw.c:450-458: REDFT11 (scipy's DCT-IV) in both plan kinds
+        X[:, ~cos] = sf.dct(X[:, ~cos], type=4, axis=0)
+    else:
+        X[:, ~cos] = sf.dst(X[:, ~cos], type=t, axis=0)
 
 
 def _data_r2c(X):
@@ -596,14 +599,17 @@ def _data_c2r(Y, M):
 
 
 def grid(which, trans, synthesis_plan, plan_sphv, X):
-    """ft_execute_{sph,sphv}_{synthesis,analysis}(TRANS, P, X, N, M) restated
-    (fftw.c:126-270) on a copy of the N x M array X: which is "sph_synthesis",
-    "sph_analysis", "sphv_synthesis" or "sphv_analysis"; the plan is a
-    synthesis plan (DCT-III / DST-III, c2r) or not (DCT-II / DST-II, r2c),
-    plan_sphv its column classes."""
+    """ft_execute_{sph,sphv,disk}_{synthesis,analysis}(TRANS, P, X, N, M)
+    restated (fftw.c:126-270, 460-520) on a copy of the N x M array X: which
+    is "sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis",
+    "disk_synthesis" or "disk_analysis"; the plan is a synthesis plan
+    (DCT-III / DST-III, c2r) or not (DCT-II / DST-II, r2c), plan_sphv its
+    column classes (the disk's plans: the sphere's classes, DCT-IV on the
+    sine class)."""
     X = np.array(X, dtype=np.float64, order="F")
     N, M = X.shape
     analysis, exec_sphv = "analysis" in which, which.startswith("sphv")
+    disk = which.startswith("disk")   # fftw.c:460-520: no last-row fix-ups
     cos = _grid_classes(M, exec_sphv)
     c = _GRID_C
     pre3 = (not analysis and trans == "N") or (analysis and trans == "T")
@@ -614,9 +620,10 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
             X[0, cos] *= 2.0
             a = c["1_4_sqrt_pi"]
         else:              # analysis 'T': x2 on the last row of the sine columns
-            X[N - 1, ~cos] *= 2.0
+            if not disk:
+                X[N - 1, ~cos] *= 2.0
             a = c["2_sqrt_pi"] / (2 * N * M)
-        _theta(X, True, plan_sphv)
+        _theta(X, True, plan_sphv, disk)
         X *= a
         X[:, 0] *= c["sqrt2"]
         return np.fft.irfft(_data_r2c(X), n=M, axis=1) * M
@@ -627,9 +634,10 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
     else:                  # analysis 'N'
         X *= c["4_sqrt_pi"] / (2 * N * M)
     X[:, 0] *= c["sqrt1_2"]
-    _theta(X, False, plan_sphv)
+    _theta(X, False, plan_sphv, disk)
     if trans == "T":
-        X[N - 1, ~cos] *= 0.5
+        if not disk:
+            X[N - 1, ~cos] *= 0.5
     else:
         X[0, cos] *= 0.5
     return X

@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 
 EPS = 2.0 ** -52
 SIZES = [(1, 1), (2, 3), (5, 9), (8, 5), (64, 127), (65, 129), (256, 511), (1024, 2047)]
-CASES = [(w, t) for w in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis") for t in ("N", "T")]
+CASES = [(w, t) for w in ("sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis", "disk_synthesis",
+                          "disk_analysis") for t in ("N", "T")]
 
 
 def _say(*args):
@@ -28,10 +29,10 @@ def _say(*args):
 def _plan(ft, which, trans, N, M):
     """The plan the executor takes for TRANS: synthesis 'N' and analysis 'T'
     a synthesis plan, the others an analysis plan (test_fftw.c:52-64)."""
-    sphv = which.startswith("sphv")
+    geo = which.split("_")[0]
     syn = (which.endswith("synthesis")) == (trans == "N")
-    name = ("sphv_" if sphv else "sph_") + ("synthesis" if syn else "analysis")
-    return getattr(ft, "plan_" + name)(N, M), syn, sphv
+    name = geo + "_" + ("synthesis" if syn else "analysis")
+    return getattr(ft, "plan_" + name)(N, M), syn, geo == "sphv"
 
 
 @pytest.mark.parametrize("which,trans", CASES)
@@ -139,3 +140,27 @@ def test_triangle_transform_round_trip(ft, n):
         X = T.cpu().numpy().T
         _say("triangle n=%d %s chain rel2 %.2e relinf %.2e" % (n, trans, O.rel2(X, A), O.relinf(X, A)))
         assert O.rel2(X, A) < bound
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_disk_transform_round_trip(ft, n):
+    """test_fftw.c:226-260 on the device: disk2cxf, synthesis, analysis,
+    cxf2disk with 'N', then with 'T', against the reference's bounds
+    (8 sqrt(N) eps rel-2, 4N eps rel-inf)."""
+    import torch
+    M = 4 * n - 3
+    H = ft.plan_disk2cxf(n, 0.0, 0.0)
+    RS, RA = ft.plan_disk_synthesis(n, M), ft.plan_disk_analysis(n, M)
+    A = O.fill("disk", n, M, 43)
+    T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    _say("disk transform round trip n=%d" % n)
+    for trans, syn, ana in (("N", RS, RA), ("T", RA, RS)):
+        ft.gpu_execute_harmonic("disk2cxf", trans, H, T)
+        ft.gpu_execute_sph_grid("disk_synthesis", trans, syn, T)
+        ft.gpu_execute_sph_grid("disk_analysis", trans, ana, T)
+        ft.gpu_execute_harmonic("cxf2disk", trans, H, T)
+    ft.synchronize()
+    X = T.cpu().numpy().T
+    e2, einf = O.rel2(X, A), O.relinf(X, A)
+    _say("disk n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
+    assert e2 < 8 * np.sqrt(n) * EPS and einf < 4 * n * EPS

@@ -37,6 +37,8 @@ def _direct_r2r(x, kind):
             y[i] = (-1) ** i * x[n - 1] + 2 * np.sum(x[:n - 1] * np.sin(pi * (k[:n - 1] + 1) * (i + np.longdouble(0.5)) / n))
         elif kind == "REDFT10":
             y[i] = 2 * np.sum(x * np.cos(pi * (k + np.longdouble(0.5)) * i / n))
+        elif kind == "REDFT11":
+            y[i] = 2 * np.sum(x * np.cos(pi * (k + np.longdouble(0.5)) * (i + np.longdouble(0.5)) / n))
         else:  # RODFT10
             y[i] = 2 * np.sum(x * np.sin(pi * (k + np.longdouble(0.5)) * (i + 1) / n))
     return y.astype(np.float64)
@@ -47,7 +49,8 @@ def test_r2r_kinds_are_fftw_definitions(n):
     import scipy.fft as sf
     x = np.random.default_rng(n).standard_normal(n)
     for kind, got in (("REDFT01", sf.dct(x, type=3)), ("RODFT01", sf.dst(x, type=3)),
-                      ("REDFT10", sf.dct(x, type=2)), ("RODFT10", sf.dst(x, type=2))):
+                      ("REDFT10", sf.dct(x, type=2)), ("RODFT10", sf.dst(x, type=2)),
+                      ("REDFT11", sf.dct(x, type=4))):
         ref = _direct_r2r(x, kind)
         assert np.max(np.abs(got - ref)) <= 64 * n * EPS * max(1.0, np.max(np.abs(ref))), kind
 
@@ -157,3 +160,37 @@ def test_tri_transposes_and_inverse(N, M):
 def test_tri_small_sizes_are_untouched():
     X = np.random.default_rng(1).standard_normal((1, 5))
     assert np.array_equal(O.tri_grid(False, "N", True, X), X)
+
+
+@pytest.mark.parametrize("N,M", [(1, 1), (4, 9), (5, 13)])
+def test_disk_synthesis_of_unit_coefficients(N, M):
+    """fftw.c:460-478: the disk's cosine class as the sphere's, its sine class
+    through REDFT11: a unit coefficient at row l gives 2 cos((l + 1/2) theta)."""
+    cos = O._grid_classes(M, False)
+    theta = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    phi = 2 * np.pi * np.arange(M) / M
+    a = 1.0 / (4.0 * np.sqrt(np.pi))
+    for l in range(N):
+        for col in range(M):
+            X = np.zeros((N, M))
+            X[l, col] = 1.0
+            G = O.grid("disk_synthesis", "N", True, False, X)
+            t = 2 * np.cos(l * theta) if cos[col] else 2 * np.cos((l + 0.5) * theta)
+            if col == 0:
+                p = np.full(M, np.sqrt(2.0))
+            elif col % 2 == 0:
+                p = 2 * np.cos(col // 2 * phi)
+            else:
+                p = 2 * np.sin((col + 1) // 2 * phi)
+            assert np.max(np.abs(G - a * np.outer(t, p))) <= 32 * (N + M) * EPS, (l, col)
+
+
+@pytest.mark.parametrize("N,M", [(3, 5), (4, 9)])
+def test_disk_transposes_and_inverse(N, M):
+    SN = _dense("disk_synthesis", "N", True, False, N, M)
+    ST = _dense("disk_synthesis", "T", False, False, N, M)
+    AN = _dense("disk_analysis", "N", False, False, N, M)
+    AT = _dense("disk_analysis", "T", True, False, N, M)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
