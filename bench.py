@@ -58,7 +58,9 @@ def parse():
     p.add_argument("--engine", choices=("auto", "pipeline", "systolic"), default="auto",
                    help="engine of the headline launches (auto: the library's own choice)")
     p.add_argument("--case", help="time one case only, KIND:N:DIR[:fp32][:batch] (for profiling runs; KIND one of "
-                                  "sph, sphv, tri, disk, rectdisk, annulus, or tet on the N x N x N array), "
+                                  "sph, sphv, tri, disk, rectdisk, annulus, or tet on the N x N x N array); "
+                                  "grid:WHICH:N[:TRANS] for the grid transforms (WHICH e.g. sph_synthesis), "
+                                  "spinsph:N:DIR:S for the spin-weighted sphere, "
                                   "e.g. tri:1024:hi2lo or sph:1024:hi2lo:batch (the 512-transform batch)")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run a plain 64 MiB device copy (untimed), to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -525,6 +527,29 @@ def run_case(ft, torch, spec, steps, warmup):
     parts = spec.split(":")
     if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv", "rectdisk2cheb", "cheb2rectdisk"):
         ms = harmonic(ft, torch, int(parts[1]), parts[0], parts[2] if len(parts) > 2 else "N", steps, warmup)
+        print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
+        return
+    if parts[0] == "grid":
+        # grid:WHICH:N[:TRANS], WHICH one of sph_synthesis, sph_analysis, disk_synthesis, ...
+        which, nn = parts[1], int(parts[2])
+        trans = parts[3] if len(parts) > 3 else "N"
+        geo = which.split("_")[0]
+        M = 4 * nn - 3 if geo == "disk" else 2 * nn - 1
+        syn = which.endswith("synthesis") == (trans == "N")
+        P = getattr(ft, "plan_%s_%s" % (geo, "synthesis" if syn else "analysis"))(nn, M)
+        T = torch.rand(M, nn, dtype=torch.float64, device="cuda")
+        stream = torch.cuda.current_stream()
+        ms = events(ft, torch, stream, lambda: ft.gpu_execute_sph_grid(which, trans, P, T, stream=stream), steps, warmup)
+        print(json.dumps({"case": spec, "ms": round(ms, 5), "GB/s": round(16.0 * nn * M / (ms * 1e-3) / 1e9, 1)}),
+              flush=True)
+        return
+    if parts[0] == "spinsph":
+        # spinsph:N:DIR:S
+        nn, d, sp = int(parts[1]), parts[2], int(parts[3]) if len(parts) > 3 else 0
+        P = ft.plan_rotspinsphere(nn, sp)
+        T = torch.rand(2 * nn - 1, nn, dtype=torch.complex128, device="cuda")
+        stream = torch.cuda.current_stream()
+        ms = events(ft, torch, stream, lambda: ft.gpu_execute_spinsph(P, d, T, stream=stream), steps, warmup)
         print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
         return
     kind, nn, d = parts[0], int(parts[1]), parts[2]

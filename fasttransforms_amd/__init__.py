@@ -113,6 +113,11 @@ for _t in ("tri_synthesis", "tri_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_triangle_fftw_plan"] = (None, [_vp])
+for _t in ("tet_synthesis", "tet_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i, _i])
+SIGNATURES["ft_destroy_tetrahedron_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_gpu_execute_tet_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _i, _vp])
 SIGNATURES["ft_gpu_execute_tri_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 SIGNATURES["ft_gpu_execute_sph_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 
@@ -405,6 +410,60 @@ def gpu_execute_tri_grid(name, trans, plan, A, stream=None):
     if rc:
         raise GPUError("ft_gpu_execute_tri_grid: %s" % _lib.ft_gpu_error_string(rc).decode())
     return A
+
+
+class TetrahedronGridPlan:
+    """Owner of an ``ft_tetrahedron_fftw_plan *``."""
+
+    def __init__(self, ptr, N, L, M, synthesis):
+        if not ptr:
+            _check()
+            raise GPUError("plan construction failed")
+        self._ptr = ptr
+        self._destroy = _lib.ft_destroy_tetrahedron_fftw_plan
+        self.N, self.L, self.M, self.synthesis = N, L, M, synthesis
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def __del__(self):
+        ptr, self._ptr = getattr(self, "_ptr", None), None
+        if ptr:
+            self._destroy(ptr)
+
+
+def plan_tet_synthesis(N, L, M, flags=0):
+    return TetrahedronGridPlan(_lib.ft_plan_tet_synthesis(int(N), int(L), int(M), int(flags)), int(N), int(L), int(M),
+                               True)
+
+
+def plan_tet_analysis(N, L, M, flags=0):
+    return TetrahedronGridPlan(_lib.ft_plan_tet_analysis(int(N), int(L), int(M), int(flags)), int(N), int(L), int(M),
+                               False)
+
+
+def _execute_tet_grid(name):
+    fn = getattr(_lib, "ft_execute_" + name)
+
+    def run(trans, plan, X):
+        N, L, M = plan.N, plan.L, plan.M
+        fn(trans.encode(), plan.ptr, _tet_pointer_raw(X, N * L * M), N, L, M)
+        _check()
+        return X
+
+    run.__name__ = "execute_" + name
+    return run
+
+
+def _tet_pointer_raw(X, size):
+    if not isinstance(X, np.ndarray) or X.dtype != np.float64 or not X.flags.f_contiguous or X.size < size:
+        raise ValueError("expected a Fortran-ordered float64 array of N*L*M entries")
+    return X.ctypes.data_as(_vp)
+
+
+execute_tet_synthesis = _execute_tet_grid("tet_synthesis")
+execute_tet_analysis = _execute_tet_grid("tet_analysis")
 
 
 def plan_rotsphere(n):

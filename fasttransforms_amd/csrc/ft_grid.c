@@ -348,3 +348,144 @@ void ft_execute_tri_synthesis(const char TRANS, const ft_triangle_fftw_plan *P, 
 void ft_execute_tri_analysis(const char TRANS, const ft_triangle_fftw_plan *P, double *X, const int N, const int M) {
     execute_tri_grid(1, TRANS, P, X, N, M);
 }
+
+/* --------------------------------------------------------- tetrahedron -- */
+
+/* fasttransforms.h:514-524 (fftw.c:333-395): FFTW's r2r_3d(N, L, M) over X as
+ * a row-major N x L x M buffer (synthesis REDFT01 on every axis, analysis
+ * REDFT10) with the executors' face scalings on the column-major view
+ * (x2 / x0.5 where i, j or k is 0); nothing unless N, L and M exceed 1. */
+struct ft_tetrahedron_fftw_plan_s {
+    int N, L, M, synthesis;
+    FtamdTriGrid *g;
+    void *d_stage;
+    size_t stage_bytes;
+    pthread_mutex_t lock;
+};
+
+static ft_tetrahedron_fftw_plan *plan_tet_grid(int N, int L, int M, int synthesis) {
+    if (N < 1 || L < 1 || M < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    ft_tetrahedron_fftw_plan *P = calloc(1, sizeof(ft_tetrahedron_fftw_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    P->N = N;
+    P->L = L;
+    P->M = M;
+    P->synthesis = synthesis;
+    pthread_mutex_init(&P->lock, NULL);
+    const int dims[3] = {N, L, M};
+    rc = ftamd_boxgrid_create(3, dims, synthesis, &P->g);
+    if (rc != FTAMD_OK) {
+        pthread_mutex_destroy(&P->lock);
+        free(P);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    return P;
+}
+
+ft_tetrahedron_fftw_plan *ft_plan_tet_synthesis(const int N, const int L, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_tet_grid(N, L, M, 1);
+}
+ft_tetrahedron_fftw_plan *ft_plan_tet_analysis(const int N, const int L, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_tet_grid(N, L, M, 0);
+}
+
+void ft_destroy_tetrahedron_fftw_plan(ft_tetrahedron_fftw_plan *P) {
+    if (!P) return;
+    ftamd_trigrid_destroy(P->g);
+    ftamd_hip_free(P->d_stage);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* The scalings of the tetrahedron's executors (fftw.c:349-395). */
+static int tet_steps(int analysis, char TRANS, int N, int L, int M, int *pf, double *pm, double *pd, int *qf, double *qm,
+                     double *qd) {
+    if (TRANS != 'N' && TRANS != 'T') return 0;
+    *pf = 0, *pm = 1.0, *pd = 0.0, *qf = 0, *qm = 1.0, *qd = 0.0;
+    const double nlm = (double)N * (double)L * (double)M;
+    if (!analysis && TRANS == 'N') *pf = 1, *pm = 0.125;
+    else if (!analysis) *qm = 0.125;
+    else if (TRANS == 'N') *qf = 1, *qd = nlm;
+    else *pd = nlm;
+    return 1;
+}
+
+static int run_tet_grid(ft_tetrahedron_fftw_plan *P, int analysis, char TRANS, double *d, void *stream) {
+    int pf, qf;
+    double pm, pd, qm, qd;
+    tet_steps(analysis, TRANS, P->N, P->L, P->M, &pf, &pm, &pd, &qf, &qm, &qd);
+    return ftamd_boxgrid_run(P->g, d, pf, pm, pd, qf, qm, qd, stream);
+}
+
+int ft_gpu_execute_tet_grid(int analysis, char TRANS, const ft_tetrahedron_fftw_plan *P_, double *dX, int N, int L,
+                            int M, void *stream) {
+    ft_tetrahedron_fftw_plan *P = (ft_tetrahedron_fftw_plan *)P_;
+    if ((analysis != 0 && analysis != 1) || (TRANS != 'N' && TRANS != 'T') || !P || P->N != N || P->L != L ||
+        P->M != M || !dX)
+        return FT_GPU_EINVAL;
+    if (N < 2 || L < 2 || M < 2) return FT_GPU_OK;
+    pthread_mutex_lock(&P->lock);
+    int rc = run_tet_grid(P, analysis, TRANS, dX, stream);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+static void execute_tet_grid(int analysis, char TRANS, const ft_tetrahedron_fftw_plan *P_, double *X, int N, int L,
+                             int M) {
+    ft_tetrahedron_fftw_plan *P = (ft_tetrahedron_fftw_plan *)P_;
+    if ((TRANS != 'N' && TRANS != 'T') || !P || P->N != N || P->L != L || P->M != M || !X) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    if (N < 2 || L < 2 || M < 2) return;
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(X, &pk);
+    const size_t bytes = (size_t)N * L * M * sizeof(double);
+    pthread_mutex_lock(&P->lock);
+    void *d = X;
+    if (pk != FTAMD_PTR_DEVICE) {
+        if (P->stage_bytes < bytes) {
+            ftamd_hip_free(P->d_stage);
+            P->d_stage = NULL;
+            P->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&P->d_stage, bytes);
+            if (rc == FTAMD_OK) P->stage_bytes = bytes;
+        }
+        d = P->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = run_tet_grid(P, analysis, TRANS, d, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&P->lock);
+    ftamd_set_error(rc);
+}
+
+/* fasttransforms.h:526-527 (fftw.c:349-395) */
+void ft_execute_tet_synthesis(const char TRANS, const ft_tetrahedron_fftw_plan *P, double *X, const int N, const int L,
+                              const int M) {
+    execute_tet_grid(0, TRANS, P, X, N, L, M);
+}
+void ft_execute_tet_analysis(const char TRANS, const ft_tetrahedron_fftw_plan *P, double *X, const int N, const int L,
+                             const int M) {
+    execute_tet_grid(1, TRANS, P, X, N, L, M);
+}

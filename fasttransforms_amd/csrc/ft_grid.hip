@@ -198,82 +198,93 @@ __global__ void post4_x(const double2 *__restrict__ Z4, double *__restrict__ X, 
     X[(long long)N * sin_col(s) + k] = y;
 }
 
-// ---- the triangle's two-dimensional DCTs (fftw.c:280-330): FFTW's r2r_2d
-// over the buffer as an N x M row-major array, so lines of length M at
-// stride 1 (N of them) and lines of length N at stride M (M of them).
-// Element t of line b is X[t*ts + b*bs]; the scalings act on the buffer
-// index e as the reference writes them (e < N: "the first column", e % N ==
-// 0: "the first row" of its column-major view).
+// ---- the triangle's and the tetrahedron's DCTs (fftw.c:280-400): FFTW's
+// r2r_2d(N, M) / r2r_3d(N, L, M) over the buffer as a row-major array, one
+// pass per axis.  Line b along an axis of length len and stride st (the
+// product of the later axes' lengths) holds the elements
+// e = (b / st) * len * st + b % st + t * st, t < len.  The executors' face
+// scalings act where a coordinate of e in the reference's column-major view
+// (e = i + N (j + L k)) is 0.
 
 struct AxisScale {
-    int pre_fix;     // x2 where e < N and again where e % N == 0 (before the transform)
+    int pre_fix;     // x2 per zero coordinate (before the transform)
     double pre_mul;  // then x pre_mul
     double pre_div;  // then / pre_div (0: none)
-    int post_fix;    // x0.5 where e < N and again where e % N == 0 (after the transform)
+    int post_fix;    // x0.5 per zero coordinate (after the transform)
+    double post_mul; // then x post_mul
     double post_div; // then / post_div (0: none)
+    int rank;        // 2 (triangle) or 3 (tetrahedron)
+    int cN, cL;      // the column-major view's first two sizes (cL unused for rank 2)
 };
 
-__device__ __forceinline__ double scale_pre(double v, long long e, int Nc, const AxisScale &sc) {
-    if (sc.pre_fix) {
-        if (e < Nc) v *= 2.0;
-        if (e % Nc == 0) v *= 2.0;
-    }
+__device__ __forceinline__ int zero_coords(long long e, const AxisScale &sc) {
+    if (sc.rank == 2) return (e % sc.cN == 0) + (e < sc.cN);
+    const long long i = e % sc.cN, jk = e / sc.cN;
+    return (i == 0) + (jk % sc.cL == 0) + (jk < sc.cL);
+}
+
+__device__ __forceinline__ double scale_pre(double v, long long e, const AxisScale &sc) {
+    if (sc.pre_fix)
+        for (int z = zero_coords(e, sc); z > 0; --z) v *= 2.0;
     v *= sc.pre_mul;
     if (sc.pre_div != 0.0) v /= sc.pre_div;
     return v;
 }
 
-__device__ __forceinline__ double scale_post(double v, long long e, int Nc, const AxisScale &sc) {
-    if (sc.post_fix) {
-        if (e < Nc) v *= 0.5;
-        if (e % Nc == 0) v *= 0.5;
-    }
+__device__ __forceinline__ double scale_post(double v, long long e, const AxisScale &sc) {
+    if (sc.post_fix)
+        for (int z = zero_coords(e, sc); z > 0; --z) v *= 0.5;
+    v *= sc.post_mul;
     if (sc.post_div != 0.0) v /= sc.post_div;
     return v;
 }
 
+__device__ __forceinline__ long long line_elem(int b, int t, int len, long long st) {
+    return (b / st) * len * st + b % st + t * st;
+}
+
 __global__ void axis_pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw,
-                          int len, int batch, long long ts, long long bs, int Nc, AxisScale sc) {
+                          int len, int batch, long long st, AxisScale sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = blockIdx.y + blockIdx.z * 65535;
     if (k >= len || b >= batch) return;
     auto val = [&](int t) {
-        const long long e = t * ts + b * bs;
-        return scale_pre(X[e], e, Nc, sc);
+        const long long e = line_elem(b, t, len, st);
+        return scale_pre(X[e], e, sc);
     };
     const double re = val(k), im = k == 0 ? 0.0 : -val(len - k);
     const double2 t = tw[k];
     Z[(long long)len * b + k] = make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
 }
 
-__global__ void axis_post3(const double2 *__restrict__ Z, double *__restrict__ X, int len, int batch, long long ts,
-                           long long bs, int Nc, AxisScale sc) {
+__global__ void axis_post3(const double2 *__restrict__ Z, double *__restrict__ X, int len, int batch, long long st,
+                           AxisScale sc) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = blockIdx.y + blockIdx.z * 65535;
     if (n >= len || b >= batch) return;
     const int src = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
-    const long long e = n * ts + b * bs;
-    X[e] = scale_post(Z[(long long)len * b + src].x, e, Nc, sc);
+    const long long e = line_elem(b, n, len, st);
+    X[e] = scale_post(Z[(long long)len * b + src].x, e, sc);
 }
 
-__global__ void axis_pre2(const double *__restrict__ X, double2 *__restrict__ Z, int len, int batch, long long ts,
-                          long long bs, int Nc, AxisScale sc) {
+__global__ void axis_pre2(const double *__restrict__ X, double2 *__restrict__ Z, int len, int batch, long long st,
+                          AxisScale sc) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = blockIdx.y + blockIdx.z * 65535;
     if (n >= len || b >= batch) return;
-    const long long e = n * ts + b * bs;
+    const long long e = line_elem(b, n, len, st);
     const int dst = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
-    Z[(long long)len * b + dst] = make_double2(scale_pre(X[e], e, Nc, sc), 0.0);
+    Z[(long long)len * b + dst] = make_double2(scale_pre(X[e], e, sc), 0.0);
 }
 
 __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X, const double2 *__restrict__ tw,
-                           int len, int batch, long long ts, long long bs, int Nc, AxisScale sc) {
+                           int len, int batch, long long st, AxisScale sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = blockIdx.y + blockIdx.z * 65535;
     if (k >= len || b >= batch) return;
     const double2 V = Z[(long long)len * b + k], t = tw[k];
-    const long long e = k * ts + b * bs;
-    X[e] = scale_post(2.0 * (t.x * V.x + t.y * V.y), e, Nc, sc);
+    const long long e = line_elem(b, k, len, st);
+    X[e] = scale_post(2.0 * (t.x * V.x + t.y * V.y), e, sc);
 }
 
 }  // namespace ftgrid
@@ -414,13 +425,16 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }
 
-// ---- triangle grids
+// ---- triangle and tetrahedron grids
 
 struct FtamdTriGrid {
-    int N, M, type3;
-    hipfftHandle ax[2];  // [0] length M, N lines; [1] length N, M lines
-    double2 *tw[2];      // e^{i pi k / 2L} for the two lengths L = M, N
-    double2 *Z;          // N x M complex
+    int rank, type3;
+    int dims[3];         // FFTW's order: (N, M) or (N, L, M)
+    int cN, cL;          // the column-major view's first sizes
+    long long total;
+    hipfftHandle ax[3];  // one batched FFT per axis
+    double2 *tw[3];      // e^{i pi k / 2 len} per axis
+    double2 *Z;          // total complex
 };
 
 static int twiddles(int L, double2 **out) {
@@ -437,21 +451,32 @@ static int twiddles(int L, double2 **out) {
     return rc;
 }
 
-extern "C" int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out) {
+extern "C" int ftamd_boxgrid_create(int rank, const int *dims, int type3, FtamdTriGrid **out) {
     *out = NULL;
-    /* lines are the launch grid's y dimension (at most 65535) */
-    if (N < 1 || M < 1 || N > 65535 || M > 65535) return FTAMD_EINVAL;
+    if (rank != 2 && rank != 3) return FTAMD_EINVAL;
+    long long total = 1;
+    for (int a = 0; a < rank; a++) {
+        if (dims[a] < 1) return FTAMD_EINVAL;
+        total *= dims[a];
+    }
+    /* lines are the launch grid's y and z dimensions: at most 65535^2 per axis */
+    if (total > 0x7fffffffLL) return FTAMD_EINVAL;
     FtamdTriGrid *g = (FtamdTriGrid *)calloc(1, sizeof(FtamdTriGrid));
     if (!g) return FTAMD_ENOMEM;
-    g->N = N;
-    g->M = M;
+    g->rank = rank;
     g->type3 = type3;
-    int rc = twiddles(M, &g->tw[0]);
-    if (rc == FTAMD_OK) rc = twiddles(N, &g->tw[1]);
-    if (rc == FTAMD_OK && hipMalloc((void **)&g->Z, sizeof(double2) * (size_t)N * M) != hipSuccess) rc = FTAMD_ENOMEM;
-    int lm[1] = {M}, ln[1] = {N};
-    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->ax[0], 1, lm, lm, 1, M, lm, 1, M, HIPFFT_Z2Z, N));
-    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->ax[1], 1, ln, ln, 1, N, ln, 1, N, HIPFFT_Z2Z, M));
+    for (int a = 0; a < rank; a++) g->dims[a] = dims[a];
+    g->cN = dims[0];
+    g->cL = rank == 3 ? dims[1] : 1;
+    g->total = total;
+    int rc = hipMalloc((void **)&g->Z, sizeof(double2) * (size_t)total) == hipSuccess ? FTAMD_OK : FTAMD_ENOMEM;
+    for (int a = 0; rc == FTAMD_OK && a < rank; a++) {
+        const int len = dims[a];
+        const int batch = (int)(total / len);
+        rc = twiddles(len, &g->tw[a]);
+        int l1[1] = {len};
+        if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->ax[a], 1, l1, l1, 1, len, l1, 1, len, HIPFFT_Z2Z, batch));
+    }
     if (rc != FTAMD_OK) {
         ftamd_trigrid_destroy(g);
         return rc;
@@ -460,49 +485,61 @@ extern "C" int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out)
     return FTAMD_OK;
 }
 
+extern "C" int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out) {
+    const int dims[2] = {N, M};
+    return ftamd_boxgrid_create(2, dims, type3, out);
+}
+
 extern "C" void ftamd_trigrid_destroy(FtamdTriGrid *g) {
     if (!g) return;
     hipDeviceSynchronize();
-    for (int i = 0; i < 2; i++) {
-        if (g->ax[i]) hipfftDestroy(g->ax[i]);
-        hipFree(g->tw[i]);
+    for (int a = 0; a < 3; a++) {
+        if (g->ax[a]) hipfftDestroy(g->ax[a]);
+        hipFree(g->tw[a]);
     }
     hipFree(g->Z);
     free(g);
 }
 
-extern "C" int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
-                                 double post_div, void *stream) {
+extern "C" int ftamd_boxgrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
+                                 double post_mul, double post_div, void *stream) {
     if (!g || !X) return FTAMD_EINVAL;
-    const int N = g->N, M = g->M;
     const hipStream_t st = (hipStream_t)stream;
-    const ftgrid::AxisScale none = {0, 1.0, 0.0, 0, 0.0};
-    for (int pass = 0; pass < 2; pass++) {
-        // pass 0: lines of length M at stride 1 (N lines, M apart); pass 1: lines of length N at stride M
-        const int len = pass == 0 ? M : N, batch = pass == 0 ? N : M;
-        const long long ts = pass == 0 ? 1 : M, bs = pass == 0 ? M : 1;
+    ftgrid::AxisScale none = {0, 1.0, 0.0, 0, 1.0, 0.0, g->rank, g->cN, g->cL};
+    for (int a = g->rank - 1; a >= 0; a--) {
+        // the last axis first: contiguous lines
+        const int first = a == g->rank - 1, last = a == 0;
+        const int len = g->dims[a];
+        const int batch = (int)(g->total / len);
+        long long stride = 1;
+        for (int b = a + 1; b < g->rank; b++) stride *= g->dims[b];
         ftgrid::AxisScale sc = none;
-        if (pass == 0) sc.pre_fix = pre_fix, sc.pre_mul = pre_mul, sc.pre_div = pre_div;
-        if (pass == 1) sc.post_fix = post_fix, sc.post_div = post_div;
-        const dim3 block(256), grid((len + 255) / 256, batch);
-        int rc = fft_ok(hipfftSetStream(g->ax[pass], st));
+        if (first) sc.pre_fix = pre_fix, sc.pre_mul = pre_mul, sc.pre_div = pre_div;
+        if (last) sc.post_fix = post_fix, sc.post_mul = post_mul, sc.post_div = post_div;
+        const dim3 block(256), grid((len + 255) / 256, batch < 65535 ? batch : 65535, (batch + 65534) / 65535);
+        int rc = fft_ok(hipfftSetStream(g->ax[a], st));
         if (rc != FTAMD_OK) return rc;
         if (g->type3) {
-            hipLaunchKernelGGL(ftgrid::axis_pre3, grid, block, 0, st, X, g->Z, g->tw[pass], len, batch, ts, bs, N, sc);
+            hipLaunchKernelGGL(ftgrid::axis_pre3, grid, block, 0, st, X, g->Z, g->tw[a], len, batch, stride, sc);
             if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
-            rc = fft_ok(hipfftExecZ2Z(g->ax[pass], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
+            rc = fft_ok(hipfftExecZ2Z(g->ax[a], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
                                       HIPFFT_BACKWARD));
             if (rc != FTAMD_OK) return rc;
-            hipLaunchKernelGGL(ftgrid::axis_post3, grid, block, 0, st, g->Z, X, len, batch, ts, bs, N, sc);
+            hipLaunchKernelGGL(ftgrid::axis_post3, grid, block, 0, st, g->Z, X, len, batch, stride, sc);
         } else {
-            hipLaunchKernelGGL(ftgrid::axis_pre2, grid, block, 0, st, X, g->Z, len, batch, ts, bs, N, sc);
+            hipLaunchKernelGGL(ftgrid::axis_pre2, grid, block, 0, st, X, g->Z, len, batch, stride, sc);
             if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
-            rc = fft_ok(hipfftExecZ2Z(g->ax[pass], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
+            rc = fft_ok(hipfftExecZ2Z(g->ax[a], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z,
                                       HIPFFT_FORWARD));
             if (rc != FTAMD_OK) return rc;
-            hipLaunchKernelGGL(ftgrid::axis_post2, grid, block, 0, st, g->Z, X, g->tw[pass], len, batch, ts, bs, N, sc);
+            hipLaunchKernelGGL(ftgrid::axis_post2, grid, block, 0, st, g->Z, X, g->tw[a], len, batch, stride, sc);
         }
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
     }
     return FTAMD_OK;
+}
+
+extern "C" int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
+                                 double post_div, void *stream) {
+    return ftamd_boxgrid_run(g, X, pre_fix, pre_mul, pre_div, post_fix, 1.0, post_div, stream);
 }

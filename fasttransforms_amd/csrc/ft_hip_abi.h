@@ -210,6 +210,12 @@ int ftamd_trigrid_create(int N, int M, int type3, FtamdTriGrid **out);
 void ftamd_trigrid_destroy(FtamdTriGrid *g);
 int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
                       double post_div, void *stream);
+/* The same for FFTW's r2r_3d(N, L, M) (dims in FFTW's order, rank 2 or 3);
+ * the face scalings act per zero coordinate of the column-major view
+ * (e = i + N (j + L k)); post_mul scales after the transform. */
+int ftamd_boxgrid_create(int rank, const int *dims, int type3, FtamdTriGrid **out);
+int ftamd_boxgrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
+                      double post_mul, double post_div, void *stream);
 
 /* The spin-weighted sphere's first stage (ft_k_spin.hip) on the complex
  * columns c0 .. c0+count-1 of A (interleaved re, im): column c at element

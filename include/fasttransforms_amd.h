@@ -322,6 +322,24 @@ ft_disk_fftw_plan *ft_plan_disk_analysis(const int N, const int M, const unsigne
 void ft_execute_disk_synthesis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M);
 void ft_execute_disk_analysis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M);
 
+/* fasttransforms.h:514-527 (fftw.c:333-395) -- synthesis and analysis on the
+ * tetrahedron's grid: FFTW's three-dimensional REDFT01 (synthesis plans) or
+ * REDFT10 (analysis plans) over X read, as the reference's r2r_3d(N, L, M)
+ * reads it, as a row-major N x L x M buffer, with the executors' scalings on
+ * the faces of its column-major view; nothing unless N, L and M exceed 1. */
+typedef struct ft_tetrahedron_fftw_plan_s ft_tetrahedron_fftw_plan;
+void ft_destroy_tetrahedron_fftw_plan(ft_tetrahedron_fftw_plan *P);
+ft_tetrahedron_fftw_plan *ft_plan_tet_synthesis(const int N, const int L, const int M, const unsigned flags);
+ft_tetrahedron_fftw_plan *ft_plan_tet_analysis(const int N, const int L, const int M, const unsigned flags);
+void ft_execute_tet_synthesis(const char TRANS, const ft_tetrahedron_fftw_plan *P, double *X, const int N, const int L,
+                              const int M);
+void ft_execute_tet_analysis(const char TRANS, const ft_tetrahedron_fftw_plan *P, double *X, const int N, const int L,
+                             const int M);
+/* The tetrahedron's executors on a device-resident X on a HIP stream
+ * (analysis 0: synthesis, 1: analysis). */
+int ft_gpu_execute_tet_grid(int analysis, char TRANS, const ft_tetrahedron_fftw_plan *P, double *dX, int N, int L,
+                            int M, void *stream);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
  * analysis, 4 disk synthesis, 5 disk analysis, TRANS and the plan as the

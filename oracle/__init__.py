@@ -675,3 +675,37 @@ def tri_grid(analysis, trans, synthesis_plan, X):
     else:
         buf = r2r(buf / nm)
     return buf.reshape((N, M), order="F")
+
+
+def tet_grid(analysis, trans, synthesis_plan, X):
+    """ft_execute_tet_{synthesis,analysis}(TRANS, P, X, N, L, M) restated
+    (fftw.c:333-395) on a copy of the N x L x M column-major array X: FFTW's
+    r2r_3d(N, L, M) reads the buffer as a row-major N x L x M array; the face
+    scalings act where a coordinate of the column-major view is 0."""
+    import scipy.fft as sf
+    X = np.array(X, dtype=np.float64, order="F")
+    N, L, M = X.shape
+    if N < 2 or L < 2 or M < 2:
+        return X
+    buf = X.reshape(-1, order="F").copy()
+    e = np.arange(N * L * M)
+    i, j, k = e % N, (e // N) % L, e // (N * L)
+    faces = (i == 0).astype(int) + (j == 0).astype(int) + (k == 0).astype(int)
+
+    def r2r(b):
+        B = b.reshape((N, L, M))          # row-major view
+        t = 3 if synthesis_plan else 2
+        for ax in (2, 1, 0):
+            B = sf.dct(B, type=t, axis=ax)
+        return B.reshape(-1)
+
+    nlm = float(N) * float(L) * float(M)
+    if not analysis and trans == "N":
+        buf = r2r(buf * 2.0 ** faces * 0.125)
+    elif not analysis:
+        buf = r2r(buf) * 0.125
+    elif trans == "N":
+        buf = r2r(buf) * 0.5 ** faces / nlm
+    else:
+        buf = r2r(buf / nlm)
+    return buf.reshape((N, L, M), order="F")

@@ -164,3 +164,17 @@ def test_disk_transform_round_trip(ft, n):
     e2, einf = O.rel2(X, A), O.relinf(X, A)
     _say("disk n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
     assert e2 < 8 * np.sqrt(n) * EPS and einf < 4 * n * EPS
+
+
+@pytest.mark.parametrize("analysis", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_tet_against_oracle(ft, analysis, trans):
+    for N, L, M in [(1, 3, 3), (2, 2, 2), (5, 5, 5), (5, 3, 4), (64, 64, 64), (128, 128, 128)]:
+        for syn_plan in (True, False):
+            P = (ft.plan_tet_synthesis if syn_plan else ft.plan_tet_analysis)(N, L, M)
+            X = np.asfortranarray(np.random.default_rng(N + 7 * L + M).standard_normal((N, L, M)))
+            name = "tet_analysis" if analysis else "tet_synthesis"
+            _say("grid", name, trans, "synthesis plan" if syn_plan else "analysis plan", "N=%d L=%d M=%d" % (N, L, M))
+            got = getattr(ft, "execute_" + name)(trans, P, X.copy(order="F"))
+            ref = O.tet_grid(analysis, trans, syn_plan, X)
+            assert O.rel2(got, ref) <= 2e-14, (name, trans, syn_plan, N, L, M, O.rel2(got, ref))

@@ -194,3 +194,37 @@ def test_disk_transposes_and_inverse(N, M):
     assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_tet_synthesis_of_unit_coefficients(N):
+    """fftw.c:349-362 with N = L = M: a unit coefficient at (l, m, p) gives
+    cos(l theta_i) cos(m theta_j) cos(p theta_k) on the grid."""
+    th = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    for l in range(N):
+        for m in range(N):
+            for p in range(N):
+                X = np.zeros((N, N, N), order="F")
+                X[l, m, p] = 1.0
+                G = O.tet_grid(False, "N", True, X)
+                ref = np.einsum("i,j,k->ijk", np.cos(l * th), np.cos(m * th), np.cos(p * th))
+                assert np.max(np.abs(G - ref)) <= 64 * N * EPS, (l, m, p)
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_tet_transposes_and_inverse(N):
+    L = M = N
+    size = N * L * M
+
+    def dense(analysis, trans, syn):
+        D = np.zeros((size, size))
+        for c in range(size):
+            e = np.zeros(size)
+            e[c] = 1.0
+            D[:, c] = O.tet_grid(analysis, trans, syn, e.reshape((N, L, M), order="F")).reshape(-1, order="F")
+        return D
+    SN, ST = dense(False, "N", True), dense(False, "T", False)
+    AN, AT = dense(True, "N", False), dense(True, "T", True)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * size
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * size
+    assert np.max(np.abs(AN @ SN - np.eye(size))) <= 64 * EPS * size
