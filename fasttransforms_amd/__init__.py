@@ -109,6 +109,11 @@ for _t in ("disk_synthesis", "disk_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_disk_fftw_plan"] = (None, [_vp])
+for _t in ("annulus_synthesis", "annulus_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, _d, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_annulus_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_get_rho_annulus_fftw_plan"] = (_d, [_vp])
 for _t in ("tri_synthesis", "tri_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
@@ -342,7 +347,17 @@ plan_sphv_analysis = _plan_grid("sphv_analysis", False, True)
 plan_disk_synthesis = _plan_grid("disk_synthesis", True, False)
 plan_disk_analysis = _plan_grid("disk_analysis", False, False)
 _GRID_WHICH = {"sph_synthesis": 0, "sph_analysis": 1, "sphv_synthesis": 2, "sphv_analysis": 3,
-               "disk_synthesis": 4, "disk_analysis": 5}
+               "disk_synthesis": 4, "disk_analysis": 5, "annulus_synthesis": 6, "annulus_analysis": 7}
+
+
+def plan_annulus_synthesis(N, M, rho, flags=0):
+    return SphereGridPlan(_lib.ft_plan_annulus_synthesis(int(N), int(M), float(rho), int(flags)), int(N), int(M),
+                          True, False)
+
+
+def plan_annulus_analysis(N, M, rho, flags=0):
+    return SphereGridPlan(_lib.ft_plan_annulus_analysis(int(N), int(M), float(rho), int(flags)), int(N), int(M),
+                          False, False)
 
 
 def _execute_grid(name):
@@ -364,6 +379,8 @@ execute_sphv_synthesis = _execute_grid("sphv_synthesis")
 execute_sphv_analysis = _execute_grid("sphv_analysis")
 execute_disk_synthesis = _execute_grid("disk_synthesis")
 execute_disk_analysis = _execute_grid("disk_analysis")
+execute_annulus_synthesis = _execute_grid("annulus_synthesis")
+execute_annulus_analysis = _execute_grid("annulus_analysis")
 
 
 def gpu_execute_sph_grid(name, trans, plan, A, stream=None):

@@ -22,11 +22,17 @@
 
 #include "ft_internal.h"
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
 struct ft_sphere_fftw_plan_s {
     int N, M;
     int synthesis; /* 1: DCT-III / DST-III and c2r; 0: DCT-II / DST-II and r2c */
     int sphv;      /* 0 the sphere's, 1 the vector field's column classes, 2 the disk's
-                      (DCT-IV, REDFT11, on the sine class in both plan kinds) */
+                      (DCT-IV, REDFT11, on the sine class in both plan kinds), 3 the
+                      annulus's (one class, radial weights) */
+    double rho;    /* the annulus's inner radius */
     int device;
     FtamdGrid *g;
     void *d_stage; /* staging buffer of host-pointer calls */
@@ -42,9 +48,9 @@ struct ft_sphere_fftw_plan_s {
 #define FT_4_SQRT_PI 7.089815403622064109
 #define FT_2_SQRT_PI 3.544907701811032055
 
-static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv) {
-    /* the disk's plans are the same object (ft_disk_fftw_plan is this type) */
-    if (N < 1 || M < 1 || M % 2 == 0) {
+static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv, double rho) {
+    /* the disk's and the annulus's plans are the same object */
+    if (N < 1 || M < 1 || M % 2 == 0 || (sphv == 3 && !(rho >= 0.0 && rho < 1.0))) {
         ftamd_set_error(FT_GPU_EINVAL);
         return NULL;
     }
@@ -62,9 +68,27 @@ static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv) {
     P->M = M;
     P->synthesis = synthesis;
     P->sphv = sphv;
+    P->rho = rho;
     P->device = dev;
     pthread_mutex_init(&P->lock, NULL);
-    rc = ftamd_grid_create(N, M, synthesis, sphv, &P->g);
+    double *w = NULL;
+    if (sphv == 3) {
+        /* fftw.c:540-548: the radial weights at theta = (2n+1) pi / 2N */
+        w = malloc(sizeof(double) * (size_t)N);
+        if (!w) {
+            pthread_mutex_destroy(&P->lock);
+            free(P);
+            ftamd_set_error(FT_GPU_ENOMEM);
+            return NULL;
+        }
+        for (int n = 0; n < N; n++) {
+            const double t = (N - n - 0.5) * M_PI / (2.0 * N);
+            const double ct = sin(t), st = cos(t);
+            w[n] = sqrt(2.0 * (ct * ct + rho * rho * st * st) / (1.0 - rho * rho));
+        }
+    }
+    rc = ftamd_grid_create(N, M, synthesis, sphv, w, &P->g);
+    free(w);
     if (rc != FTAMD_OK) {
         pthread_mutex_destroy(&P->lock);
         free(P);
@@ -78,19 +102,19 @@ static ft_sphere_fftw_plan *plan_grid(int N, int M, int synthesis, int sphv) {
  * are accepted and ignored. */
 ft_sphere_fftw_plan *ft_plan_sph_synthesis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 1, 0);
+    return plan_grid(N, M, 1, 0, 0.0);
 }
 ft_sphere_fftw_plan *ft_plan_sph_analysis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 0, 0);
+    return plan_grid(N, M, 0, 0, 0.0);
 }
 ft_sphere_fftw_plan *ft_plan_sphv_synthesis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 1, 1);
+    return plan_grid(N, M, 1, 1, 0.0);
 }
 ft_sphere_fftw_plan *ft_plan_sphv_analysis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 0, 1);
+    return plan_grid(N, M, 0, 1, 0.0);
 }
 
 /* fasttransforms.h:477 (fftw.c:66-74) */
@@ -108,15 +132,17 @@ void ft_destroy_sphere_fftw_plan(ft_sphere_fftw_plan *P) {
  * executors (fftw.c:460-520) are the sphere's without the last-row fix-ups
  * of the sine class. */
 typedef struct {
-    int need_syn, exec_sphv;
+    int need_syn, exec_sphv, wmode;
     double f0, f1, a, b;
 } grid_step;
 
 static int grid_steps(int which, char TRANS, int N, int M, grid_step *s) {
-    const int analysis = which & 1, sphv = (which >> 1) == 1, disk = (which >> 1) == 2;
+    const int analysis = which & 1, sphv = (which >> 1) == 1, disk = (which >> 1) == 2, ann = (which >> 1) == 3;
     if (TRANS != 'N' && TRANS != 'T') return 0;
     const int trans = TRANS == 'T';
-    s->exec_sphv = sphv;
+    s->exec_sphv = ann ? 3 : sphv;
+    /* the annulus's weights (fftw.c:583-655): multiplied by synthesis, divided by analysis */
+    s->wmode = ann ? (analysis ? 2 : 1) : 0;
     if (!analysis && !trans) { /* fftw.c:128-146: x2 row 0 (cosine class), III, x 1/(4 sqrt(pi)), col 0 x sqrt 2, c2r */
         s->need_syn = 1, s->f0 = 2.0, s->f1 = 1.0, s->a = FT_1_4_SQRT_PI, s->b = FT_SQRT2;
     } else if (!analysis) { /* fftw.c:148-163: r2c, x 1/(2 sqrt(pi)), col 0 x 1/sqrt 2, II, x1/2 last row (sine class) */
@@ -126,24 +152,26 @@ static int grid_steps(int which, char TRANS, int N, int M, grid_step *s) {
     } else { /* fftw.c:185-200: x2 last row (sine class), III, x 2 sqrt(pi)/(2NM), col 0 x sqrt 2, c2r */
         s->need_syn = 1, s->f0 = 1.0, s->f1 = 2.0, s->a = FT_2_SQRT_PI / (2.0 * N * M), s->b = FT_SQRT2;
     }
-    if (disk) s->f1 = 1.0;
+    if (disk || ann) s->f1 = 1.0;
     return 1;
 }
 
 /* The executor's geometry must be the plan's (a disk plan's sine class is
  * another transform) and its kind the one the executor takes. */
 static int valid_grid(const ft_sphere_fftw_plan *P, int which, int N, int M, const grid_step *s) {
-    return P && P->N == N && P->M == M && P->synthesis == s->need_syn && ((which >> 1) == 2) == (P->sphv == 2);
+    const int geo = which >> 1;
+    return P && P->N == N && P->M == M && P->synthesis == s->need_syn && (geo == 2) == (P->sphv == 2) &&
+           (geo == 3) == (P->sphv == 3);
 }
 
 int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, double *dX, int N, int M,
                             void *stream) {
     ft_sphere_fftw_plan *P = (ft_sphere_fftw_plan *)P_;
     grid_step s;
-    if (which < 0 || which > 5 || !grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, which, N, M, &s) || !dX)
+    if (which < 0 || which > 7 || !grid_steps(which, TRANS, N, M, &s) || !valid_grid(P, which, N, M, &s) || !dX)
         return FT_GPU_EINVAL;
     pthread_mutex_lock(&P->lock);
-    int rc = ftamd_grid_run(P->g, dX, s.exec_sphv, s.f0, s.f1, s.a, s.b, stream);
+    int rc = ftamd_grid_run(P->g, dX, s.exec_sphv, s.f0, s.f1, s.a, s.b, s.wmode, stream);
     pthread_mutex_unlock(&P->lock);
     return rc;
 }
@@ -176,7 +204,7 @@ static void execute_grid(int which, char TRANS, const ft_sphere_fftw_plan *P_, d
         d = P->d_stage;
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
     }
-    if (rc == FTAMD_OK) rc = ftamd_grid_run(P->g, d, s.exec_sphv, s.f0, s.f1, s.a, s.b, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_grid_run(P->g, d, s.exec_sphv, s.f0, s.f1, s.a, s.b, s.wmode, NULL);
     if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     pthread_mutex_unlock(&P->lock);
@@ -202,11 +230,11 @@ void ft_execute_sphv_analysis(const char TRANS, const ft_sphere_fftw_plan *P, do
  * phi DFT as the sphere's. */
 ft_disk_fftw_plan *ft_plan_disk_synthesis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 1, 2);
+    return plan_grid(N, M, 1, 2, 0.0);
 }
 ft_disk_fftw_plan *ft_plan_disk_analysis(const int N, const int M, const unsigned flags) {
     (void)flags;
-    return plan_grid(N, M, 0, 2);
+    return plan_grid(N, M, 0, 2, 0.0);
 }
 void ft_destroy_disk_fftw_plan(ft_disk_fftw_plan *P) { ft_destroy_sphere_fftw_plan(P); }
 void ft_execute_disk_synthesis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M) {
@@ -214,6 +242,28 @@ void ft_execute_disk_synthesis(const char TRANS, const ft_disk_fftw_plan *P, dou
 }
 void ft_execute_disk_analysis(const char TRANS, const ft_disk_fftw_plan *P, double *X, const int N, const int M) {
     execute_grid(5, TRANS, P, X, N, M);
+}
+
+/* fasttransforms.h:558-575 (fftw.c:530-655): the annulus's plans, REDFT01 or
+ * REDFT10 on every column, the radial weights on columns 1 and 2 mod 4, the
+ * phi DFT as the sphere's; rho in [0, 1). */
+ft_annulus_fftw_plan *ft_plan_annulus_synthesis(const int N, const int M, const double rho, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 1, 3, rho);
+}
+ft_annulus_fftw_plan *ft_plan_annulus_analysis(const int N, const int M, const double rho, const unsigned flags) {
+    (void)flags;
+    return plan_grid(N, M, 0, 3, rho);
+}
+double ft_get_rho_annulus_fftw_plan(ft_annulus_fftw_plan *P) { return P ? P->rho : 0.0; }
+void ft_destroy_annulus_fftw_plan(ft_annulus_fftw_plan *P) { ft_destroy_sphere_fftw_plan(P); }
+void ft_execute_annulus_synthesis(const char TRANS, const ft_annulus_fftw_plan *P, double *X, const int N,
+                                  const int M) {
+    execute_grid(6, TRANS, P, X, N, M);
+}
+void ft_execute_annulus_analysis(const char TRANS, const ft_annulus_fftw_plan *P, double *X, const int N,
+                                 const int M) {
+    execute_grid(7, TRANS, P, X, N, M);
 }
 
 /* ------------------------------------------------------------ triangle -- */

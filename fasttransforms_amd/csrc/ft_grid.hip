@@ -23,7 +23,8 @@
 #include "ft_hip_abi.h"
 
 struct FtamdGrid {
-    int N, M, type3, plan_sphv;
+    int N, M, type3;
+    int plan_sphv;  // column classes of the plan: 0 the sphere's, 1 the vector field's, 3 all cosine (annulus)
     int disk;     // the disk's plans: DCT-IV (REDFT11) on the sine class
     int nsin;     // sine-class columns (disk)
     hipfftHandle theta, phi, iv;
@@ -32,15 +33,26 @@ struct FtamdGrid {
     double2 *Z;   // N x M complex: the columns' FFTs
     double2 *Z4;  // 2N x nsin complex: the disk's sine-class columns, zero-padded (disk)
     double2 *Y;   // N x (M/2+1) complex: the rows' half spectra
+    double *w;    // the annulus's radial weights (N), NULL otherwise
 };
 
 namespace ftgrid {
 
 // Columns of the cosine class: j mod 4 in {0, 3} (fftw.c:84-96: plans 1 and
-// 4 take kind[0], plans 2 and 3 kind[1]); the vector-field plans swap the kinds.
-__device__ __forceinline__ bool cos_class(int j, int sphv) {
+// 4 take kind[0], plans 2 and 3 kind[1]); the vector-field plans (mode 1)
+// swap the kinds; the annulus's (mode 3, fftw.c:560-575) have one kind.
+__device__ __forceinline__ bool cos_class(int j, int mode) {
+    if (mode == 3) return true;
     const int r = j & 3;
-    return ((r == 0) || (r == 3)) != (sphv != 0);
+    return ((r == 0) || (r == 3)) != (mode == 1);
+}
+
+// The annulus's radial weights on columns 1 and 2 mod 4 (fftw.c:588-592):
+// wmode 1 multiplies, 2 divides, 0 leaves the entry.
+__device__ __forceinline__ double weigh(double v, int j, int n, const double *w, int wmode) {
+    const int r = j & 3;
+    if (wmode == 0 || (r != 1 && r != 2)) return v;
+    return wmode == 1 ? v * w[n] : v / w[n];
 }
 
 // Pipeline with DCT-III first (synthesis plans): fix-ups, DCT-III / DST-III
@@ -71,7 +83,7 @@ __global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, cons
 // After the inverse FFT: y_n of every column, signs of the sine class, the
 // scalings a (all) and b (column 0 again), into Y as data_r2c lays it out.
 __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
-                      double b, int skip_sin) {
+                      double b, int skip_sin, const double *__restrict__ w, int wmode) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (n >= N || j >= M) return;
@@ -79,6 +91,7 @@ __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, in
     const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
     double y = Z[(long long)N * j + src].x;
     if (!cos_class(j, plan_sphv) && (n & 1)) y = -y;
+    y = weigh(y, j, n, w, wmode);
     y *= a;
     if (j == 0) y *= b;
     double *Yd = (double *)Y;
@@ -93,7 +106,7 @@ __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, in
 // Pipeline with the row DFT first (analysis plans): data_c2r, the scalings,
 // the sine class's alternating signs, Makhoul's reordering into Z.
 __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int N, int M, int plan_sphv, double a,
-                     double b, int skip_sin) {
+                     double b, int skip_sin, const double *__restrict__ w, int wmode) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (n >= N || j >= M) return;
@@ -109,6 +122,7 @@ __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int
     }
     x *= a;
     if (j == 0) x *= b;
+    x = weigh(x, j, n, w, wmode);
     if (!cos_class(j, plan_sphv) && (n & 1)) x = -x;
     const int dst = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
     Z[(long long)N * j + dst] = make_double2(x, 0.0);
@@ -291,16 +305,17 @@ __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X
 
 static int fft_ok(hipfftResult r) { return r == HIPFFT_SUCCESS ? FTAMD_OK : FTAMD_EDEVICE; }
 
-extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, FtamdGrid **out) {
+extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const double *w, FtamdGrid **out) {
     *out = NULL;
     /* columns are the launch grid's y dimension (at most 65535) */
-    if (N < 1 || M < 1 || M % 2 == 0 || M > 65535 || plan_mode < 0 || plan_mode > 2) return FTAMD_EINVAL;
+    if (N < 1 || M < 1 || M % 2 == 0 || M > 65535 || plan_mode < 0 || plan_mode > 3 || (plan_mode == 3 && !w))
+        return FTAMD_EINVAL;
     FtamdGrid *g = (FtamdGrid *)calloc(1, sizeof(FtamdGrid));
     if (!g) return FTAMD_ENOMEM;
     g->N = N;
     g->M = M;
     g->type3 = type3;
-    g->plan_sphv = plan_mode == 1;
+    g->plan_sphv = plan_mode == 1 ? 1 : plan_mode == 3 ? 3 : 0;
     g->disk = plan_mode == 2;
     g->nsin = g->disk ? (M + 2) / 4 + (M + 1) / 4 : 0;  /* columns 1 and 2 mod 4 (fftw.c:427-431) */
     g->theta = g->phi = g->iv = 0;
@@ -334,6 +349,10 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, FtamdGr
         int n4[1] = {2 * N};
         if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->iv, 1, n4, n4, 1, 2 * N, n4, 1, 2 * N, HIPFFT_Z2Z, g->nsin));
     }
+    if (rc == FTAMD_OK && plan_mode == 3) {
+        if (hipMalloc((void **)&g->w, sizeof(double) * (size_t)N) != hipSuccess) rc = FTAMD_ENOMEM;
+        else if (hipMemcpy(g->w, w, sizeof(double) * (size_t)N, hipMemcpyHostToDevice) != hipSuccess) rc = FTAMD_EDEVICE;
+    }
     /* the sine-class slots of Z stay zero in the disk's plans (never written) */
     if (rc == FTAMD_OK && g->disk && hipMemset(g->Z, 0, zbytes) != hipSuccess) rc = FTAMD_EDEVICE;
     // theta: M contiguous complex transforms of length N
@@ -365,12 +384,13 @@ extern "C" void ftamd_grid_destroy(FtamdGrid *g) {
     hipFree(g->tw4);
     hipFree(g->Z);
     hipFree(g->Z4);
+    hipFree(g->w);
     hipFree(g->Y);
     free(g);
 }
 
 extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0, double f1, double a, double b,
-                              void *stream) {
+                              int wmode, void *stream) {
     if (!g || !X) return FTAMD_EINVAL;
     const int N = g->N, M = g->M;
     const hipStream_t st = (hipStream_t)stream;
@@ -407,7 +427,8 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
         if (rc != FTAMD_OK) return rc;
-        hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, iv);
+        hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, iv, g->w,
+                           g->w ? wmode : 0);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         if (iv && (rc = iv_from_x()) != FTAMD_OK) return rc;
         return fft_ok(hipfftExecZ2D(g->phi, (hipfftDoubleComplex *)g->Y, X));
@@ -416,7 +437,8 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     // class), fix-ups f0 and f1
     rc = fft_ok(hipfftExecD2Z(g->phi, X, (hipfftDoubleComplex *)g->Y));
     if (rc != FTAMD_OK) return rc;
-    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv);
+    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv, g->w,
+                       g->w ? wmode : 0);
     if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
     if (iv && (rc = iv_from_y()) != FTAMD_OK) return rc;
     rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));

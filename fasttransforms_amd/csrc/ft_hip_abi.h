@@ -188,16 +188,21 @@ int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream);
  * work buffers for N x M (M odd).  type3: a synthesis plan (DCT-III / DST-III
  * along theta, c2r along phi), else an analysis plan (DCT-II / DST-II, r2c);
  * plan_mode: 0 the sphere's column classes, 1 the vector field's, 2 the
- * disk's (DCT-IV on the sine class in both kinds).  ftamd_grid_run: with a
+ * disk's (DCT-IV on the sine class in both kinds), 3 the annulus's (one
+ * class; w: its N radial weights, applied on columns 1 and 2 mod 4 after the
+ * theta transform of a synthesis plan or before that of an analysis plan,
+ * multiplied (wmode 1) or divided (wmode 2)); exec_mode the executor's
+ * classes for the fix-ups, in the same coding.  ftamd_grid_run: with a
  * synthesis plan, fix-ups (f0 on row 0 of the executor's cosine columns, f1
  * on the last row of its sine columns), the theta transforms, scalings (a on
  * every entry, b on column 0 again) and the phi transform; with an analysis
  * plan the phi transform, the scalings, the theta transforms and the
  * fix-ups.  X (device, N x M) is overwritten in place. */
 typedef struct FtamdGrid FtamdGrid;
-int ftamd_grid_create(int N, int M, int type3, int plan_mode, FtamdGrid **out);
+int ftamd_grid_create(int N, int M, int type3, int plan_mode, const double *w, FtamdGrid **out);
 void ftamd_grid_destroy(FtamdGrid *g);
-int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0, double f1, double a, double b, void *stream);
+int ftamd_grid_run(FtamdGrid *g, double *X, int exec_mode, double f0, double f1, double a, double b, int wmode,
+                   void *stream);
 
 /* The triangle's two-dimensional DCTs (ft_grid.hip): DCT-III (type3) or
  * DCT-II along both axes of the buffer as FFTW's r2r_2d(N, M) reads it (an

@@ -340,10 +340,26 @@ void ft_execute_tet_analysis(const char TRANS, const ft_tetrahedron_fftw_plan *P
 int ft_gpu_execute_tet_grid(int analysis, char TRANS, const ft_tetrahedron_fftw_plan *P, double *dX, int N, int L,
                             int M, void *stream);
 
+/* fasttransforms.h:558-575 (fftw.c:530-655) -- synthesis and analysis on the
+ * annulus's grid (inner radius rho in [0, 1)): REDFT01 / REDFT10 on every
+ * column, the radial weights sqrt(2 (sin^2 t + rho^2 cos^2 t) / (1 - rho^2)),
+ * t = (N - n - 1/2) pi / 2N, on columns 1 and 2 mod 4 (multiplied by
+ * synthesis, divided by analysis), the phi DFT as on the sphere.
+ * ft_annulus_fftw_plan is the sphere's plan type; an annulus executor takes
+ * an annulus plan only. */
+typedef ft_sphere_fftw_plan ft_annulus_fftw_plan;
+void ft_destroy_annulus_fftw_plan(ft_annulus_fftw_plan *P);
+double ft_get_rho_annulus_fftw_plan(ft_annulus_fftw_plan *P);
+ft_annulus_fftw_plan *ft_plan_annulus_synthesis(const int N, const int M, const double rho, const unsigned flags);
+ft_annulus_fftw_plan *ft_plan_annulus_analysis(const int N, const int M, const double rho, const unsigned flags);
+void ft_execute_annulus_synthesis(const char TRANS, const ft_annulus_fftw_plan *P, double *X, const int N,
+                                  const int M);
+void ft_execute_annulus_analysis(const char TRANS, const ft_annulus_fftw_plan *P, double *X, const int N, const int M);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
- * analysis, 4 disk synthesis, 5 disk analysis, TRANS and the plan as the
- * executors above.  Calls on one plan share its work buffers: order them on
+ * analysis, 4 disk synthesis, 5 disk analysis, 6 annulus synthesis, 7
+ * annulus analysis, TRANS and the plan as the executors above.  Calls on one plan share its work buffers: order them on
  * one stream. */
 int ft_gpu_execute_sph_grid(int which, char TRANS, const ft_sphere_fftw_plan *P, double *dX, int N, int M,
                             void *stream);

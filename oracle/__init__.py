@@ -561,10 +561,23 @@ _GRID_C = {"sqrt2": 1.4142135623730950488, "sqrt1_2": 0.70710678118654752440,
 
 def _grid_classes(M, sphv):
     """Cosine-class columns of a plan (fftw.c:84-96: columns j mod 4 in {0, 3}
-    take kind[0], REDFT for the sphere's plans, RODFT for the vector field's)."""
+    take kind[0], REDFT for the sphere's plans, RODFT for the vector field's;
+    sphv == "annulus": every column, fftw.c:560-575)."""
     j = np.arange(M)
+    if sphv == "annulus":
+        return np.ones(M, dtype=bool)
     c = (j % 4 == 0) | (j % 4 == 3)
     return ~c if sphv else c
+
+
+def annulus_weights(N, rho):
+    """fftw.c:540-548."""
+    w = np.empty(N)
+    for n in range(N):
+        t = (N - n - 0.5) * np.pi / (2.0 * N)
+        ct, st = np.sin(t), np.cos(t)
+        w[n] = np.sqrt(2.0 * (ct * ct + rho * rho * st * st) / (1.0 - rho * rho))
+    return w
 
 
 def _theta(X, synthesis, plan_sphv, disk=False):
@@ -598,7 +611,7 @@ def _data_c2r(Y, M):
     return X
 
 
-def grid(which, trans, synthesis_plan, plan_sphv, X):
+def grid(which, trans, synthesis_plan, plan_sphv, X, rho=None):
     """ft_execute_{sph,sphv,disk}_{synthesis,analysis}(TRANS, P, X, N, M)
     restated (fftw.c:126-270, 460-520) on a copy of the N x M array X: which
     is "sph_synthesis", "sph_analysis", "sphv_synthesis", "sphv_analysis",
@@ -610,6 +623,11 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
     N, M = X.shape
     analysis, exec_sphv = "analysis" in which, which.startswith("sphv")
     disk = which.startswith("disk")   # fftw.c:460-520: no last-row fix-ups
+    ann = which.startswith("annulus")  # fftw.c:583-655: one class, radial weights
+    if ann:
+        plan_sphv = exec_sphv = "annulus"
+        wcols = (np.arange(M) % 4 == 1) | (np.arange(M) % 4 == 2)
+        w = annulus_weights(N, rho)[:, None]
     cos = _grid_classes(M, exec_sphv)
     c = _GRID_C
     pre3 = (not analysis and trans == "N") or (analysis and trans == "T")
@@ -624,6 +642,8 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
                 X[N - 1, ~cos] *= 2.0
             a = c["2_sqrt_pi"] / (2 * N * M)
         _theta(X, True, plan_sphv, disk)
+        if ann:
+            X[:, wcols] = X[:, wcols] * w if trans == "N" else X[:, wcols] / w
         X *= a
         X[:, 0] *= c["sqrt2"]
         return np.fft.irfft(_data_r2c(X), n=M, axis=1) * M
@@ -634,6 +654,8 @@ def grid(which, trans, synthesis_plan, plan_sphv, X):
     else:                  # analysis 'N'
         X *= c["4_sqrt_pi"] / (2 * N * M)
     X[:, 0] *= c["sqrt1_2"]
+    if ann:
+        X[:, wcols] = X[:, wcols] * w if trans == "T" else X[:, wcols] / w
     _theta(X, False, plan_sphv, disk)
     if trans == "T":
         if not disk:

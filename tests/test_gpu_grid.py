@@ -178,3 +178,16 @@ def test_tet_against_oracle(ft, analysis, trans):
             got = getattr(ft, "execute_" + name)(trans, P, X.copy(order="F"))
             ref = O.tet_grid(analysis, trans, syn_plan, X)
             assert O.rel2(got, ref) <= 2e-14, (name, trans, syn_plan, N, L, M, O.rel2(got, ref))
+
+
+@pytest.mark.parametrize("which,trans", [(w, t) for w in ("annulus_synthesis", "annulus_analysis") for t in ("N", "T")])
+def test_annulus_against_oracle(ft, which, trans):
+    for N, M in [(1, 1), (5, 9), (64, 253), (256, 1021), (1024, 4093)]:
+        for rho in (0.0, float(np.sqrt(1.0 / 3.0))):
+            syn = which.endswith("synthesis") == (trans == "N")
+            P = (ft.plan_annulus_synthesis if syn else ft.plan_annulus_analysis)(N, M, rho)
+            X = np.asfortranarray(np.random.default_rng(N + M).standard_normal((N, M)))
+            _say("grid", which, trans, "rho=%.3f N=%d M=%d" % (rho, N, M))
+            got = getattr(ft, "execute_" + which)(trans, P, X.copy(order="F"))
+            ref = O.grid(which, trans, syn, False, X, rho=rho)
+            assert O.rel2(got, ref) <= 2e-14, (which, trans, rho, N, M, O.rel2(got, ref))

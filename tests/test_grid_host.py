@@ -228,3 +228,49 @@ def test_tet_transposes_and_inverse(N):
     assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * size
     assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * size
     assert np.max(np.abs(AN @ SN - np.eye(size))) <= 64 * EPS * size
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5])
+def test_annulus_synthesis_of_unit_coefficients(rho):
+    """fftw.c:583-600: every column through REDFT01, the radial weights on
+    columns 1 and 2 mod 4: a unit coefficient at row l gives
+    2 cos(l theta) w (those columns) or 2 cos(l theta)."""
+    N, M = 5, 9
+    theta = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    phi = 2 * np.pi * np.arange(M) / M
+    a = 1.0 / (4.0 * np.sqrt(np.pi))
+    w = O.annulus_weights(N, rho)
+    # the weights are sqrt(2/(1-rho^2)) r at the grid's radii, (2r^2-1-rho^2)/(1-rho^2) = cos(theta)
+    # (fftw.c:535-538)
+    r = np.sqrt((1 + rho ** 2 + (1 - rho ** 2) * np.cos(theta)) / 2)
+    assert np.max(np.abs(w - np.sqrt(2 / (1 - rho ** 2)) * r)) <= 8 * EPS
+    for l in range(N):
+        for col in range(M):
+            X = np.zeros((N, M))
+            X[l, col] = 1.0
+            G = O.grid("annulus_synthesis", "N", True, False, X, rho=rho)
+            t = 2 * np.cos(l * theta) * (w if col % 4 in (1, 2) else 1.0)
+            if col == 0:
+                p = np.full(M, np.sqrt(2.0))
+            elif col % 2 == 0:
+                p = 2 * np.cos(col // 2 * phi)
+            else:
+                p = 2 * np.sin((col + 1) // 2 * phi)
+            assert np.max(np.abs(G - a * np.outer(t, p))) <= 64 * (N + M) * EPS, (l, col)
+
+
+def test_annulus_transposes_and_inverse():
+    N, M, rho = 4, 7, 0.3
+
+    def dense(which, trans, syn):
+        D = np.zeros((N * M, N * M))
+        for i in range(N * M):
+            e = np.zeros(N * M)
+            e[i] = 1.0
+            D[:, i] = O.grid(which, trans, syn, False, e.reshape((N, M), order="F"), rho=rho).reshape(-1, order="F")
+        return D
+    SN, ST = dense("annulus_synthesis", "N", True), dense("annulus_synthesis", "T", False)
+    AN, AT = dense("annulus_analysis", "N", False), dense("annulus_analysis", "T", True)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
