@@ -118,6 +118,11 @@ for _t in ("tri_synthesis", "tri_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_triangle_fftw_plan"] = (None, [_vp])
+for _t in ("rectdisk_synthesis", "rectdisk_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_rectdisk_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_gpu_execute_rectdisk_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 for _t in ("tet_synthesis", "tet_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i, _i])
@@ -412,6 +417,24 @@ def plan_tri_analysis(N, M, flags=0):
     return TriangleGridPlan(_lib.ft_plan_tri_analysis(int(N), int(M), int(flags)), int(N), int(M), False)
 
 
+class RectdiskGridPlan(SphereGridPlan):
+    """Owner of an ``ft_rectdisk_fftw_plan *``."""
+
+    def __init__(self, ptr, N, M, synthesis):
+        SphereGridPlan.__init__(self, ptr, N, M, synthesis, False)
+        self._destroy = _lib.ft_destroy_rectdisk_fftw_plan
+
+
+def plan_rectdisk_synthesis(N, M, flags=0):
+    return RectdiskGridPlan(_lib.ft_plan_rectdisk_synthesis(int(N), int(M), int(flags)), int(N), int(M), True)
+
+
+def plan_rectdisk_analysis(N, M, flags=0):
+    return RectdiskGridPlan(_lib.ft_plan_rectdisk_analysis(int(N), int(M), int(flags)), int(N), int(M), False)
+
+
+execute_rectdisk_synthesis = _execute_grid("rectdisk_synthesis")
+execute_rectdisk_analysis = _execute_grid("rectdisk_analysis")
 execute_tri_synthesis = _execute_grid("tri_synthesis")
 execute_tri_analysis = _execute_grid("tri_analysis")
 

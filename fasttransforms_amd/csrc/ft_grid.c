@@ -539,3 +539,144 @@ void ft_execute_tet_analysis(const char TRANS, const ft_tetrahedron_fftw_plan *P
                              const int M) {
     execute_tet_grid(1, TRANS, P, X, N, L, M);
 }
+
+/* ------------------------------------------------------ rectangle-disk -- */
+
+/* fasttransforms.h:577-590 (fftw.c:657-740): the plan fixes the kinds (x:
+ * REDFT01 on the even columns, RODFT01 on the odd ones, y: REDFT01 along the
+ * rows for synthesis plans; REDFT10 / RODFT10 / REDFT10 for analysis plans);
+ * the executors fix the order of the two passes and the scalings. */
+struct ft_rectdisk_fftw_plan_s {
+    int N, M, synthesis;
+    FtamdRectGrid *g;
+    void *d_stage;
+    size_t stage_bytes;
+    pthread_mutex_t lock;
+};
+
+static ft_rectdisk_fftw_plan *plan_rect_grid(int N, int M, int synthesis) {
+    if (N < 1 || M < 1) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    ft_rectdisk_fftw_plan *P = calloc(1, sizeof(ft_rectdisk_fftw_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    P->N = N;
+    P->M = M;
+    P->synthesis = synthesis;
+    pthread_mutex_init(&P->lock, NULL);
+    rc = ftamd_rectgrid_create(N, M, synthesis, &P->g);
+    if (rc != FTAMD_OK) {
+        pthread_mutex_destroy(&P->lock);
+        free(P);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    return P;
+}
+
+ft_rectdisk_fftw_plan *ft_plan_rectdisk_synthesis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_rect_grid(N, M, 1);
+}
+ft_rectdisk_fftw_plan *ft_plan_rectdisk_analysis(const int N, const int M, const unsigned flags) {
+    (void)flags;
+    return plan_rect_grid(N, M, 0);
+}
+
+void ft_destroy_rectdisk_fftw_plan(ft_rectdisk_fftw_plan *P) {
+    if (!P) return;
+    ftamd_rectgrid_destroy(P->g);
+    ftamd_hip_free(P->d_stage);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* The two passes of the executors (fftw.c:690-740).  Masks: 1 row 0 of the
+ * even columns, 2 the last row of the odd columns, 3 column 0. */
+static int run_rect_grid(ft_rectdisk_fftw_plan *P, int analysis, char TRANS, double *d, void *stream) {
+    FtamdRectGrid *g = P->g;
+    const double nm = (double)P->N * (double)P->M;
+    int rc;
+    if (!analysis && TRANS == 'N') {
+        /* x2 row 0 of the even columns, x, x2 column 0, x0.25, y */
+        rc = ftamd_rectgrid_pass(g, d, 0, 1, 2.0, 3, 2.0, 0.25, 0.0, stream);
+        if (rc == FTAMD_OK) rc = ftamd_rectgrid_pass(g, d, 1, 0, 1.0, 0, 1.0, 1.0, 0.0, stream);
+    } else if (!analysis) {
+        /* y, x0.25, x, x0.5 the last row of the odd columns */
+        rc = ftamd_rectgrid_pass(g, d, 1, 0, 1.0, 0, 1.0, 0.25, 0.0, stream);
+        if (rc == FTAMD_OK) rc = ftamd_rectgrid_pass(g, d, 0, 0, 1.0, 2, 0.5, 1.0, 0.0, stream);
+    } else if (TRANS == 'N') {
+        /* y, / NM, x0.5 column 0, x, x0.5 row 0 of the even columns */
+        rc = ftamd_rectgrid_pass(g, d, 1, 0, 1.0, 3, 0.5, 1.0, nm, stream);
+        if (rc == FTAMD_OK) rc = ftamd_rectgrid_pass(g, d, 0, 0, 1.0, 1, 0.5, 1.0, 0.0, stream);
+    } else {
+        /* x2 the last row of the odd columns, x, y, / NM */
+        rc = ftamd_rectgrid_pass(g, d, 0, 2, 2.0, 0, 1.0, 1.0, 0.0, stream);
+        if (rc == FTAMD_OK) rc = ftamd_rectgrid_pass(g, d, 1, 0, 1.0, 0, 1.0, 1.0, nm, stream);
+    }
+    return rc;
+}
+
+int ft_gpu_execute_rectdisk_grid(int analysis, char TRANS, const ft_rectdisk_fftw_plan *P_, double *dX, int N, int M,
+                                 void *stream) {
+    ft_rectdisk_fftw_plan *P = (ft_rectdisk_fftw_plan *)P_;
+    if ((analysis != 0 && analysis != 1) || (TRANS != 'N' && TRANS != 'T') || !P || P->N != N || P->M != M || !dX)
+        return FT_GPU_EINVAL;
+    pthread_mutex_lock(&P->lock);
+    int rc = run_rect_grid(P, analysis, TRANS, dX, stream);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+static void execute_rect_grid(int analysis, char TRANS, const ft_rectdisk_fftw_plan *P_, double *X, int N, int M) {
+    ft_rectdisk_fftw_plan *P = (ft_rectdisk_fftw_plan *)P_;
+    if ((TRANS != 'N' && TRANS != 'T') || !P || P->N != N || P->M != M || !X) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(X, &pk);
+    const size_t bytes = (size_t)N * M * sizeof(double);
+    pthread_mutex_lock(&P->lock);
+    void *d = X;
+    if (pk != FTAMD_PTR_DEVICE) {
+        if (P->stage_bytes < bytes) {
+            ftamd_hip_free(P->d_stage);
+            P->d_stage = NULL;
+            P->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&P->d_stage, bytes);
+            if (rc == FTAMD_OK) P->stage_bytes = bytes;
+        }
+        d = P->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = run_rect_grid(P, analysis, TRANS, d, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&P->lock);
+    ftamd_set_error(rc);
+}
+
+/* fasttransforms.h:589-590 (fftw.c:690-740) */
+void ft_execute_rectdisk_synthesis(const char TRANS, const ft_rectdisk_fftw_plan *P, double *X, const int N,
+                                   const int M) {
+    execute_rect_grid(0, TRANS, P, X, N, M);
+}
+void ft_execute_rectdisk_analysis(const char TRANS, const ft_rectdisk_fftw_plan *P, double *X, const int N,
+                                  const int M) {
+    execute_rect_grid(1, TRANS, P, X, N, M);
+}

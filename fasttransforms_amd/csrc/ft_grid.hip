@@ -229,7 +229,21 @@ struct AxisScale {
     double post_div; // then / post_div (0: none)
     int rank;        // 2 (triangle) or 3 (tetrahedron)
     int cN, cL;      // the column-major view's first two sizes (cL unused for rank 2)
+    // the rectangle-disk's (fftw.c:657-740): a factor on one set of entries of
+    // the N x M column-major array before (pre_mask) and after (post_mask)
+    // the transform -- 1: row 0 of the even columns, 2: the last row of the
+    // odd columns, 3: column 0 -- and DSTs on the odd lines (sin_odd)
+    int pre_mask, post_mask;
+    double pre_mask_f, post_mask_f;
+    int sin_odd;
 };
+
+__device__ __forceinline__ double masked(double v, long long e, int mask, double f, int cN) {
+    if (mask == 0) return v;
+    const long long i = e % cN, j = e / cN;
+    const bool on = mask == 1 ? (i == 0 && (j & 1) == 0) : mask == 2 ? (i == cN - 1 && (j & 1)) : j == 0;
+    return on ? v * f : v;
+}
 
 __device__ __forceinline__ int zero_coords(long long e, const AxisScale &sc) {
     if (sc.rank == 2) return (e % sc.cN == 0) + (e < sc.cN);
@@ -238,6 +252,7 @@ __device__ __forceinline__ int zero_coords(long long e, const AxisScale &sc) {
 }
 
 __device__ __forceinline__ double scale_pre(double v, long long e, const AxisScale &sc) {
+    v = masked(v, e, sc.pre_mask, sc.pre_mask_f, sc.cN);
     if (sc.pre_fix)
         for (int z = zero_coords(e, sc); z > 0; --z) v *= 2.0;
     v *= sc.pre_mul;
@@ -246,6 +261,7 @@ __device__ __forceinline__ double scale_pre(double v, long long e, const AxisSca
 }
 
 __device__ __forceinline__ double scale_post(double v, long long e, const AxisScale &sc) {
+    v = masked(v, e, sc.post_mask, sc.post_mask_f, sc.cN);
     if (sc.post_fix)
         for (int z = zero_coords(e, sc); z > 0; --z) v *= 0.5;
     v *= sc.post_mul;
@@ -262,7 +278,9 @@ __global__ void axis_pre3(const double *__restrict__ X, double2 *__restrict__ Z,
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y + blockIdx.z * 65535;
     if (k >= len || b >= batch) return;
+    const bool sn = sc.sin_odd && (b & 1);
     auto val = [&](int t) {
+        if (sn) t = len - 1 - t;  // DST-III: the DCT-III of the reversed line
         const long long e = line_elem(b, t, len, st);
         return scale_pre(X[e], e, sc);
     };
@@ -278,7 +296,9 @@ __global__ void axis_post3(const double2 *__restrict__ Z, double *__restrict__ X
     if (n >= len || b >= batch) return;
     const int src = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
     const long long e = line_elem(b, n, len, st);
-    X[e] = scale_post(Z[(long long)len * b + src].x, e, sc);
+    double y = Z[(long long)len * b + src].x;
+    if (sc.sin_odd && (b & 1) && (n & 1)) y = -y;  // DST-III: alternating signs
+    X[e] = scale_post(y, e, sc);
 }
 
 __global__ void axis_pre2(const double *__restrict__ X, double2 *__restrict__ Z, int len, int batch, long long st,
@@ -288,7 +308,9 @@ __global__ void axis_pre2(const double *__restrict__ X, double2 *__restrict__ Z,
     if (n >= len || b >= batch) return;
     const long long e = line_elem(b, n, len, st);
     const int dst = (n & 1) ? len - 1 - (n >> 1) : (n >> 1);
-    Z[(long long)len * b + dst] = make_double2(scale_pre(X[e], e, sc), 0.0);
+    double x = scale_pre(X[e], e, sc);
+    if (sc.sin_odd && (b & 1) && (n & 1)) x = -x;  // DST-II: the DCT-II of the sign-alternated line
+    Z[(long long)len * b + dst] = make_double2(x, 0.0);
 }
 
 __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X, const double2 *__restrict__ tw,
@@ -296,7 +318,8 @@ __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y + blockIdx.z * 65535;
     if (k >= len || b >= batch) return;
-    const double2 V = Z[(long long)len * b + k], t = tw[k];
+    const int src = sc.sin_odd && (b & 1) ? len - 1 - k : k;  // DST-II: reversed output
+    const double2 V = Z[(long long)len * b + src], t = tw[src];
     const long long e = line_elem(b, k, len, st);
     X[e] = scale_post(2.0 * (t.x * V.x + t.y * V.y), e, sc);
 }
@@ -527,7 +550,7 @@ extern "C" int ftamd_boxgrid_run(FtamdTriGrid *g, double *X, int pre_fix, double
                                  double post_mul, double post_div, void *stream) {
     if (!g || !X) return FTAMD_EINVAL;
     const hipStream_t st = (hipStream_t)stream;
-    ftgrid::AxisScale none = {0, 1.0, 0.0, 0, 1.0, 0.0, g->rank, g->cN, g->cL};
+    ftgrid::AxisScale none = {0, 1.0, 0.0, 0, 1.0, 0.0, g->rank, g->cN, g->cL, 0, 0, 1.0, 1.0, 0};
     for (int a = g->rank - 1; a >= 0; a--) {
         // the last axis first: contiguous lines
         const int first = a == g->rank - 1, last = a == 0;
@@ -564,4 +587,81 @@ extern "C" int ftamd_boxgrid_run(FtamdTriGrid *g, double *X, int pre_fix, double
 extern "C" int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
                                  double post_div, void *stream) {
     return ftamd_boxgrid_run(g, X, pre_fix, pre_mul, pre_div, post_fix, 1.0, post_div, stream);
+}
+
+// ---- rectangle-disk grids (fftw.c:657-740): the column transforms (x:
+// DCT on the even columns, DST on the odd ones) and the row transforms (y:
+// DCT along rows), each a pass of the axis kernels over the N x M
+// column-major array, in the executor's order with its scalings.
+
+struct FtamdRectGrid {
+    int N, M, type3;
+    hipfftHandle ax[2];  // [0] columns: length N, M lines; [1] rows: length M, N lines
+    double2 *tw[2];
+    double2 *Z;
+};
+
+extern "C" int ftamd_rectgrid_create(int N, int M, int type3, FtamdRectGrid **out) {
+    *out = NULL;
+    if (N < 1 || M < 1 || (long long)N * M > 0x7fffffffLL) return FTAMD_EINVAL;
+    FtamdRectGrid *g = (FtamdRectGrid *)calloc(1, sizeof(FtamdRectGrid));
+    if (!g) return FTAMD_ENOMEM;
+    g->N = N;
+    g->M = M;
+    g->type3 = type3;
+    int rc = hipMalloc((void **)&g->Z, sizeof(double2) * (size_t)N * M) == hipSuccess ? FTAMD_OK : FTAMD_ENOMEM;
+    const int lens[2] = {N, M}, batches[2] = {M, N};
+    for (int a = 0; rc == FTAMD_OK && a < 2; a++) {
+        rc = twiddles(lens[a], &g->tw[a]);
+        int l1[1] = {lens[a]};
+        if (rc == FTAMD_OK)
+            rc = fft_ok(hipfftPlanMany(&g->ax[a], 1, l1, l1, 1, lens[a], l1, 1, lens[a], HIPFFT_Z2Z, batches[a]));
+    }
+    if (rc != FTAMD_OK) {
+        ftamd_rectgrid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FTAMD_OK;
+}
+
+extern "C" void ftamd_rectgrid_destroy(FtamdRectGrid *g) {
+    if (!g) return;
+    hipDeviceSynchronize();
+    for (int a = 0; a < 2; a++) {
+        if (g->ax[a]) hipfftDestroy(g->ax[a]);
+        hipFree(g->tw[a]);
+    }
+    hipFree(g->Z);
+    free(g);
+}
+
+// One pass: axis 0 the columns (DST on the odd ones), 1 the rows; the masks
+// and factors before and after it.
+extern "C" int ftamd_rectgrid_pass(FtamdRectGrid *g, double *X, int axis, int pre_mask, double pre_f, int post_mask,
+                                   double post_f, double post_mul, double post_div, void *stream) {
+    if (!g || !X || (axis != 0 && axis != 1)) return FTAMD_EINVAL;
+    const hipStream_t st = (hipStream_t)stream;
+    const int N = g->N, M = g->M;
+    const int len = axis == 0 ? N : M, batch = axis == 0 ? M : N;
+    const long long stride = axis == 0 ? 1 : N;
+    ftgrid::AxisScale sc = {0, 1.0, 0.0, 0, post_mul, post_div, 2, N, 1, pre_mask, post_mask, pre_f, post_f, axis == 0};
+    // line b of the rows: e = (b / N) * M * N + b % N + t * N = b + t * N (b < N)
+    const dim3 block(256), grid((len + 255) / 256, batch < 65535 ? batch : 65535, (batch + 65534) / 65535);
+    int rc = fft_ok(hipfftSetStream(g->ax[axis], st));
+    if (rc != FTAMD_OK) return rc;
+    if (g->type3) {
+        hipLaunchKernelGGL(ftgrid::axis_pre3, grid, block, 0, st, X, g->Z, g->tw[axis], len, batch, stride, sc);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        rc = fft_ok(hipfftExecZ2Z(g->ax[axis], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::axis_post3, grid, block, 0, st, g->Z, X, len, batch, stride, sc);
+    } else {
+        hipLaunchKernelGGL(ftgrid::axis_pre2, grid, block, 0, st, X, g->Z, len, batch, stride, sc);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        rc = fft_ok(hipfftExecZ2Z(g->ax[axis], (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::axis_post2, grid, block, 0, st, g->Z, X, g->tw[axis], len, batch, stride, sc);
+    }
+    return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }

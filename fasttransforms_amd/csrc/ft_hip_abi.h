@@ -219,6 +219,17 @@ int ftamd_trigrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, d
  * the face scalings act per zero coordinate of the column-major view
  * (e = i + N (j + L k)); post_mul scales after the transform. */
 int ftamd_boxgrid_create(int rank, const int *dims, int type3, FtamdTriGrid **out);
+/* The rectangle-disk's grid (ft_grid.hip): passes of DCT-III (type3) or
+ * DCT-II along the columns (axis 0; DST on the odd columns) or the rows
+ * (axis 1) of the N x M column-major array, with a factor on a set of entries
+ * before (pre_mask, pre_f) and after (post_mask, post_f) it -- 1: row 0 of
+ * the even columns, 2: the last row of the odd columns, 3: column 0 -- then
+ * x post_mul and / post_div (0: none). */
+typedef struct FtamdRectGrid FtamdRectGrid;
+int ftamd_rectgrid_create(int N, int M, int type3, FtamdRectGrid **out);
+void ftamd_rectgrid_destroy(FtamdRectGrid *g);
+int ftamd_rectgrid_pass(FtamdRectGrid *g, double *X, int axis, int pre_mask, double pre_f, int post_mask, double post_f,
+                        double post_mul, double post_div, void *stream);
 int ftamd_boxgrid_run(FtamdTriGrid *g, double *X, int pre_fix, double pre_mul, double pre_div, int post_fix,
                       double post_mul, double post_div, void *stream);
 

@@ -356,6 +356,24 @@ void ft_execute_annulus_synthesis(const char TRANS, const ft_annulus_fftw_plan *
                                   const int M);
 void ft_execute_annulus_analysis(const char TRANS, const ft_annulus_fftw_plan *P, double *X, const int N, const int M);
 
+/* fasttransforms.h:577-590 (fftw.c:657-740) -- synthesis and analysis on the
+ * rectangle-disk's grid: along the columns REDFT01 (even columns) and
+ * RODFT01 (odd columns), along the rows REDFT01 (synthesis plans; REDFT10,
+ * RODFT10, REDFT10 for analysis plans), in the executors' order with their
+ * scalings.  Any plan of size N x M runs with either executor and TRANS. */
+typedef struct ft_rectdisk_fftw_plan_s ft_rectdisk_fftw_plan;
+void ft_destroy_rectdisk_fftw_plan(ft_rectdisk_fftw_plan *P);
+ft_rectdisk_fftw_plan *ft_plan_rectdisk_synthesis(const int N, const int M, const unsigned flags);
+ft_rectdisk_fftw_plan *ft_plan_rectdisk_analysis(const int N, const int M, const unsigned flags);
+void ft_execute_rectdisk_synthesis(const char TRANS, const ft_rectdisk_fftw_plan *P, double *X, const int N,
+                                   const int M);
+void ft_execute_rectdisk_analysis(const char TRANS, const ft_rectdisk_fftw_plan *P, double *X, const int N,
+                                  const int M);
+/* The rectangle-disk's executors on a device-resident X on a HIP stream
+ * (analysis 0: synthesis, 1: analysis). */
+int ft_gpu_execute_rectdisk_grid(int analysis, char TRANS, const ft_rectdisk_fftw_plan *P, double *dX, int N, int M,
+                                 void *stream);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
  * analysis, 4 disk synthesis, 5 disk analysis, 6 annulus synthesis, 7

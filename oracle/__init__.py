@@ -731,3 +731,46 @@ def tet_grid(analysis, trans, synthesis_plan, X):
     else:
         buf = r2r(buf / nlm)
     return buf.reshape((N, L, M), order="F")
+
+
+def rect_grid(analysis, trans, synthesis_plan, X):
+    """ft_execute_rectdisk_{synthesis,analysis}(TRANS, P, X, N, M) restated
+    (fftw.c:657-740) on a copy of the N x M column-major array X: x, the
+    column transforms (REDFT on the even columns, RODFT on the odd ones), and
+    y, REDFT along the rows, of kind 01 (synthesis plan) or 10."""
+    import scipy.fft as sf
+    X = np.array(X, dtype=np.float64, order="F")
+    N, M = X.shape
+    t = 3 if synthesis_plan else 2
+    nm = float(N) * float(M)
+
+    def x(A):
+        A[:, 0::2] = sf.dct(A[:, 0::2], type=t, axis=0)
+        A[:, 1::2] = sf.dst(A[:, 1::2], type=t, axis=0)
+
+    def y(A):
+        A[:, :] = sf.dct(A, type=t, axis=1)
+
+    if not analysis and trans == "N":
+        X[0, 0::2] *= 2.0
+        x(X)
+        X[:, 0] *= 2.0
+        X *= 0.25
+        y(X)
+    elif not analysis:
+        y(X)
+        X *= 0.25
+        x(X)
+        X[N - 1, 1::2] *= 0.5
+    elif trans == "N":
+        y(X)
+        X /= nm
+        X[:, 0] *= 0.5
+        x(X)
+        X[0, 0::2] *= 0.5
+    else:
+        X[N - 1, 1::2] *= 2.0
+        x(X)
+        y(X)
+        X /= nm
+    return X

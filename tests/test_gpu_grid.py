@@ -191,3 +191,40 @@ def test_annulus_against_oracle(ft, which, trans):
             got = getattr(ft, "execute_" + which)(trans, P, X.copy(order="F"))
             ref = O.grid(which, trans, syn, False, X, rho=rho)
             assert O.rel2(got, ref) <= 2e-14, (which, trans, rho, N, M, O.rel2(got, ref))
+
+
+@pytest.mark.parametrize("analysis", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_rectdisk_against_oracle(ft, analysis, trans):
+    for N, M in [(1, 1), (3, 4), (5, 5), (64, 64), (300, 300), (1024, 1024)]:
+        for syn_plan in (True, False):
+            P = (ft.plan_rectdisk_synthesis if syn_plan else ft.plan_rectdisk_analysis)(N, M)
+            X = np.asfortranarray(np.random.default_rng(N + 5 * M).standard_normal((N, M)))
+            name = "rectdisk_analysis" if analysis else "rectdisk_synthesis"
+            _say("grid", name, trans, "synthesis plan" if syn_plan else "analysis plan", "N=%d M=%d" % (N, M))
+            got = getattr(ft, "execute_" + name)(trans, P, X.copy(order="F"))
+            ref = O.rect_grid(analysis, trans, syn_plan, X)
+            assert O.rel2(got, ref) <= 2e-14, (name, trans, syn_plan, N, M, O.rel2(got, ref))
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_rectdisk_transform_round_trip(ft, n):
+    """test_fftw.c:352-380 through the host-pointer executors: rectdisk2cheb,
+    synthesis, analysis, cheb2rectdisk ('N', then 'T'), against the
+    reference's bounds (8 sqrt(N) eps rel-2, 4N eps rel-inf)."""
+    H = ft.plan_rectdisk2cheb(n, 0.0)
+    SS, SA = ft.plan_rectdisk_synthesis(n, n), ft.plan_rectdisk_analysis(n, n)
+    # rectdiskrand (ftutilities.c:123-130): U(-1, 1) where i + j < M
+    A = np.random.default_rng(47).uniform(-1, 1, (n, n))
+    A[np.add.outer(np.arange(n), np.arange(n)) >= n] = 0.0
+    A = np.asfortranarray(A)
+    X = A.copy(order="F")
+    _say("rectdisk transform round trip n=%d" % n)
+    for trans, syn, ana in (("N", SS, SA), ("T", SA, SS)):
+        ft.execute_rectdisk2cheb(trans, H, X)
+        ft.execute_rectdisk_synthesis(trans, syn, X)
+        ft.execute_rectdisk_analysis(trans, ana, X)
+        ft.execute_cheb2rectdisk(trans, H, X)
+    e2, einf = O.rel2(X, A), O.relinf(X, A)
+    _say("rectdisk n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
+    assert e2 < 8 * np.sqrt(n) * EPS and einf < 4 * n * EPS

@@ -274,3 +274,40 @@ def test_annulus_transposes_and_inverse():
     assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
+
+
+@pytest.mark.parametrize("N,M", [(3, 4), (4, 5), (1, 3)])
+def test_rectdisk_synthesis_of_unit_coefficients(N, M):
+    """fftw.c:690-700: a unit coefficient at (l, c) gives
+    {cos(l theta), sin((l+1) theta) (1/2 at l = N-1)} x cos(c phi) with
+    theta_n = (2n+1) pi / 2N, phi_m = (2m+1) pi / 2M (even c: cosines, odd c:
+    sines along the columns)."""
+    th = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    ph = (2 * np.arange(M) + 1) * np.pi / (2 * M)
+    for l in range(N):
+        for c in range(M):
+            X = np.zeros((N, M))
+            X[l, c] = 1.0
+            G = O.rect_grid(False, "N", True, X)
+            if c % 2 == 0:
+                t = np.cos(l * th)
+            else:
+                t = (0.5 if l == N - 1 else 1.0) * np.sin((l + 1) * th)
+            ref = np.outer(t, np.cos(c * ph))
+            assert np.max(np.abs(G - ref)) <= 32 * (N + M) * EPS, (l, c)
+
+
+@pytest.mark.parametrize("N,M", [(3, 4), (4, 5)])
+def test_rectdisk_transposes_and_inverse(N, M):
+    def dense(analysis, trans, syn):
+        D = np.zeros((N * M, N * M))
+        for i in range(N * M):
+            e = np.zeros(N * M)
+            e[i] = 1.0
+            D[:, i] = O.rect_grid(analysis, trans, syn, e.reshape((N, M), order="F")).reshape(-1, order="F")
+        return D
+    SN, ST = dense(False, "N", True), dense(False, "T", False)
+    AN, AT = dense(True, "N", False), dense(True, "T", True)
+    assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
