@@ -118,6 +118,12 @@ for _t in ("tri_synthesis", "tri_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 SIGNATURES["ft_destroy_triangle_fftw_plan"] = (None, [_vp])
+for _t in ("spinsph_synthesis", "spinsph_analysis"):
+    SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, _i, ctypes.c_uint])
+    SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
+SIGNATURES["ft_destroy_spinsphere_fftw_plan"] = (None, [_vp])
+SIGNATURES["ft_get_spin_spinsphere_fftw_plan"] = (_i, [_vp])
+SIGNATURES["ft_gpu_execute_spinsph_grid"] = (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp])
 for _t in ("rectdisk_synthesis", "rectdisk_analysis"):
     SIGNATURES["ft_plan_" + _t] = (_vp, [_i, _i, ctypes.c_uint])
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
@@ -433,6 +439,40 @@ def plan_rectdisk_analysis(N, M, flags=0):
     return RectdiskGridPlan(_lib.ft_plan_rectdisk_analysis(int(N), int(M), int(flags)), int(N), int(M), False)
 
 
+class SpinSphereGridPlan(SphereGridPlan):
+    """Owner of an ``ft_spinsphere_fftw_plan *``."""
+
+    def __init__(self, ptr, N, M, S, synthesis):
+        SphereGridPlan.__init__(self, ptr, N, M, synthesis, S % 2 != 0)
+        self._destroy = _lib.ft_destroy_spinsphere_fftw_plan
+        self.S = S
+
+
+def plan_spinsph_synthesis(N, M, S, flags=0):
+    return SpinSphereGridPlan(_lib.ft_plan_spinsph_synthesis(int(N), int(M), int(S), int(flags)), int(N), int(M),
+                              int(S), True)
+
+
+def plan_spinsph_analysis(N, M, S, flags=0):
+    return SpinSphereGridPlan(_lib.ft_plan_spinsph_analysis(int(N), int(M), int(S), int(flags)), int(N), int(M),
+                              int(S), False)
+
+
+def _execute_spin_grid(name):
+    fn = getattr(_lib, "ft_execute_" + name)
+
+    def run(trans, plan, X):
+        N, M = plan.N, plan.M
+        fn(trans.encode(), plan.ptr, _complex_pointer(X, N, M), N, M)
+        _check()
+        return X
+
+    run.__name__ = "execute_" + name
+    return run
+
+
+execute_spinsph_synthesis = _execute_spin_grid("spinsph_synthesis")
+execute_spinsph_analysis = _execute_spin_grid("spinsph_analysis")
 execute_rectdisk_synthesis = _execute_grid("rectdisk_synthesis")
 execute_rectdisk_analysis = _execute_grid("rectdisk_analysis")
 execute_tri_synthesis = _execute_grid("tri_synthesis")

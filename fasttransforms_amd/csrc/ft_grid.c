@@ -680,3 +680,141 @@ void ft_execute_rectdisk_analysis(const char TRANS, const ft_rectdisk_fftw_plan 
                                   const int M) {
     execute_rect_grid(1, TRANS, P, X, N, M);
 }
+
+/* -------------------------------------------------- spin-weighted sphere -- */
+
+/* fasttransforms.h:592-617 (fftw.c:742-937): complex N x M arrays; the plan
+ * fixes the theta kinds (the sphere's column classes for even spin S, the
+ * vector field's for odd) and the row DFT's sign (synthesis backward,
+ * analysis forward); synthesis 'N' and analysis 'C' take a synthesis plan,
+ * synthesis 'C' and analysis 'N' an analysis plan (test_fftw.c pairs them
+ * so); another pairing sets FT_GPU_EINVAL. */
+struct ft_spinsphere_fftw_plan_s {
+    int N, M, S, synthesis;
+    FtamdSpinGrid *g;
+    void *d_stage;
+    size_t stage_bytes;
+    pthread_mutex_t lock;
+};
+
+#define FT_1_2_SQRT_2PI 0.199471140200716338
+#define FT_2_SQRT_2PI 5.013256549262001005
+
+static ft_spinsphere_fftw_plan *plan_spin_grid(int N, int M, int S, int synthesis) {
+    if (N < 1 || M < 1 || M % 2 == 0) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    ft_spinsphere_fftw_plan *P = calloc(1, sizeof(ft_spinsphere_fftw_plan));
+    if (!P) {
+        ftamd_set_error(FT_GPU_ENOMEM);
+        return NULL;
+    }
+    P->N = N;
+    P->M = M;
+    P->S = S;
+    P->synthesis = synthesis;
+    pthread_mutex_init(&P->lock, NULL);
+    rc = ftamd_spingrid_create(N, M, synthesis, S % 2 != 0, &P->g);
+    if (rc != FTAMD_OK) {
+        pthread_mutex_destroy(&P->lock);
+        free(P);
+        ftamd_set_error(rc);
+        return NULL;
+    }
+    return P;
+}
+
+ft_spinsphere_fftw_plan *ft_plan_spinsph_synthesis(const int N, const int M, const int S, const unsigned flags) {
+    (void)flags;
+    return plan_spin_grid(N, M, S, 1);
+}
+ft_spinsphere_fftw_plan *ft_plan_spinsph_analysis(const int N, const int M, const int S, const unsigned flags) {
+    (void)flags;
+    return plan_spin_grid(N, M, S, 0);
+}
+int ft_get_spin_spinsphere_fftw_plan(const ft_spinsphere_fftw_plan *P) { return P ? P->S : 0; }
+
+void ft_destroy_spinsphere_fftw_plan(ft_spinsphere_fftw_plan *P) {
+    if (!P) return;
+    ftamd_spingrid_destroy(P->g);
+    ftamd_hip_free(P->d_stage);
+    pthread_mutex_destroy(&P->lock);
+    free(P);
+}
+
+/* The fix-ups and scaling of the executors (fftw.c:800-937). */
+static int spin_steps(const ft_spinsphere_fftw_plan *P, int analysis, char TRANS, int N, int M, double *f0, double *f1,
+                      double *a) {
+    if (!P || P->N != N || P->M != M || (TRANS != 'N' && TRANS != 'C')) return 0;
+    const int need_syn = analysis == (TRANS == 'C');
+    if (P->synthesis != need_syn) return 0;
+    if (!analysis && TRANS == 'N') *f0 = 2.0, *f1 = 1.0, *a = FT_1_2_SQRT_2PI;
+    else if (!analysis) *f0 = 1.0, *f1 = 0.5, *a = FT_1_2_SQRT_2PI;
+    else if (TRANS == 'N') *f0 = 0.5, *f1 = 1.0, *a = FT_2_SQRT_2PI / (2.0 * N * M);
+    else *f0 = 1.0, *f1 = 2.0, *a = FT_2_SQRT_2PI / (2.0 * N * M);
+    return 1;
+}
+
+int ft_gpu_execute_spinsph_grid(int analysis, char TRANS, const ft_spinsphere_fftw_plan *P_, void *dX, int N, int M,
+                                void *stream) {
+    ft_spinsphere_fftw_plan *P = (ft_spinsphere_fftw_plan *)P_;
+    double f0, f1, a;
+    if ((analysis != 0 && analysis != 1) || !spin_steps(P, analysis, TRANS, N, M, &f0, &f1, &a) || !dX)
+        return FT_GPU_EINVAL;
+    pthread_mutex_lock(&P->lock);
+    int rc = ftamd_spingrid_run(P->g, dX, f0, f1, a, stream);
+    pthread_mutex_unlock(&P->lock);
+    return rc;
+}
+
+static void execute_spin_grid(int analysis, char TRANS, const ft_spinsphere_fftw_plan *P_, ft_complex *X, int N,
+                              int M) {
+    ft_spinsphere_fftw_plan *P = (ft_spinsphere_fftw_plan *)P_;
+    double f0, f1, a;
+    if (!spin_steps(P, analysis, TRANS, N, M, &f0, &f1, &a) || !X) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return;
+    }
+    int rc = ftamd_hip_device(NULL, NULL);
+    if (rc != FTAMD_OK) {
+        ftamd_set_error(rc);
+        return;
+    }
+    int pk = FTAMD_PTR_HOST;
+    ftamd_hip_pointer_kind(X, &pk);
+    const size_t bytes = 2 * (size_t)N * M * sizeof(double);
+    pthread_mutex_lock(&P->lock);
+    void *d = X;
+    if (pk != FTAMD_PTR_DEVICE) {
+        if (P->stage_bytes < bytes) {
+            ftamd_hip_free(P->d_stage);
+            P->d_stage = NULL;
+            P->stage_bytes = 0;
+            rc = ftamd_hip_malloc(&P->d_stage, bytes);
+            if (rc == FTAMD_OK) P->stage_bytes = bytes;
+        }
+        d = P->d_stage;
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d, X, bytes, NULL);
+    }
+    if (rc == FTAMD_OK) rc = ftamd_spingrid_run(P->g, d, f0, f1, a, NULL);
+    if (rc == FTAMD_OK && pk != FTAMD_PTR_DEVICE) rc = ftamd_hip_d2h(X, d, bytes, NULL);
+    if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
+    pthread_mutex_unlock(&P->lock);
+    ftamd_set_error(rc);
+}
+
+/* fasttransforms.h:614-617 (fftw.c:800-937) */
+void ft_execute_spinsph_synthesis(const char TRANS, const ft_spinsphere_fftw_plan *P, ft_complex *X, const int N,
+                                  const int M) {
+    execute_spin_grid(0, TRANS, P, X, N, M);
+}
+void ft_execute_spinsph_analysis(const char TRANS, const ft_spinsphere_fftw_plan *P, ft_complex *X, const int N,
+                                 const int M) {
+    execute_spin_grid(1, TRANS, P, X, N, M);
+}

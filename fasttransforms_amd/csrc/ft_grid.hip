@@ -665,3 +665,149 @@ extern "C" int ftamd_rectgrid_pass(FtamdRectGrid *g, double *X, int axis, int pr
     }
     return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }
+
+// ---- spin-weighted sphere grids (fftw.c:742-937): the sphere's theta
+// transforms on the real and the imaginary part of every complex column
+// (the column classes of the sphere for even spin, of the vector field for
+// odd), the scaling a, the column permutation colswap (fftw.c:6-21: column 0,
+// then 2j to j and 2j-1 to M-j) and a complex DFT of length M along the rows
+// (backward for synthesis plans, forward for analysis plans).  Line 2j + p of
+// Z is part p (0 re, 1 im) of column j.
+
+namespace ftgrid {
+
+__device__ __forceinline__ int swap_dst(int j, int M) { return j == 0 ? 0 : ((j & 1) ? M - ((j + 1) >> 1) : (j >> 1)); }
+
+// synthesis plans: fix-ups, DCT-III / DST-III of both parts into Z
+__global__ void cpre3(const double2 *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw, int N,
+                      int M, int cls, double f0, double f1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = blockIdx.y;  // 2j + p
+    if (k >= N || line >= 2 * M) return;
+    const int j = line >> 1, p = line & 1;
+    const double2 *x = X + (long long)N * j;
+    const bool c = cos_class(j, cls);
+    auto val = [&](int e) {
+        double v = p ? x[e].y : x[e].x;
+        if (c && e == 0) v *= f0;
+        if (!c && e == N - 1) v *= f1;
+        return v;
+    };
+    auto in = [&](int e) { return c ? val(e) : val(N - 1 - e); };
+    const double re = in(k), im = k == 0 ? 0.0 : -in(N - k);
+    const double2 t = tw[k];
+    Z[(long long)N * line + k] = make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
+}
+
+// after the inverse FFT: y, the sine class's signs, x a, into Yc at the colswap position
+__global__ void cpost3(const double2 *__restrict__ Z, double2 *__restrict__ Yc, int N, int M, int cls, double a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = blockIdx.y;
+    if (n >= N || line >= 2 * M) return;
+    const int j = line >> 1, p = line & 1;
+    const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    double y = Z[(long long)N * line + src].x;
+    if (!cos_class(j, cls) && (n & 1)) y = -y;
+    y *= a;
+    ((double *)Yc)[2 * ((long long)N * swap_dst(j, M) + n) + p] = y;
+}
+
+// analysis plans, after the row DFT: colswap_t, x a, the sine class's signs, Makhoul's reordering
+__global__ void cpre2(const double2 *__restrict__ Yc, double2 *__restrict__ Z, int N, int M, int cls, double a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = blockIdx.y;
+    if (n >= N || line >= 2 * M) return;
+    const int j = line >> 1, p = line & 1;
+    double x = ((const double *)Yc)[2 * ((long long)N * swap_dst(j, M) + n) + p];
+    x *= a;
+    if (!cos_class(j, cls) && (n & 1)) x = -x;
+    const int dst = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    Z[(long long)N * line + dst] = make_double2(x, 0.0);
+}
+
+// after the forward FFT: DCT-II / DST-II of both parts, the fix-ups, into X
+__global__ void cpost2(const double2 *__restrict__ Z, double2 *__restrict__ X, const double2 *__restrict__ tw, int N,
+                       int M, int cls, double g0, double g1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = blockIdx.y;
+    if (k >= N || line >= 2 * M) return;
+    const int j = line >> 1, p = line & 1;
+    const bool c = cos_class(j, cls);
+    const int src = c ? k : N - 1 - k;
+    const double2 V = Z[(long long)N * line + src], t = tw[src];
+    double y = 2.0 * (t.x * V.x + t.y * V.y);
+    if (c && k == 0) y *= g0;
+    if (!c && k == N - 1) y *= g1;
+    ((double *)X)[2 * ((long long)N * j + k) + p] = y;
+}
+
+}  // namespace ftgrid
+
+struct FtamdSpinGrid {
+    int N, M, type3, cls;
+    hipfftHandle theta, phi;
+    double2 *tw, *Z, *Yc;
+};
+
+extern "C" int ftamd_spingrid_create(int N, int M, int type3, int odd_spin, FtamdSpinGrid **out) {
+    *out = NULL;
+    /* lines (two per column) are the launch grid's y dimension */
+    if (N < 1 || M < 1 || M % 2 == 0 || 2 * M > 65535) return FTAMD_EINVAL;
+    FtamdSpinGrid *g = (FtamdSpinGrid *)calloc(1, sizeof(FtamdSpinGrid));
+    if (!g) return FTAMD_ENOMEM;
+    g->N = N;
+    g->M = M;
+    g->type3 = type3;
+    g->cls = odd_spin ? 1 : 0;
+    int rc = twiddles(N, &g->tw);
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->Z, sizeof(double2) * 2 * (size_t)N * M) != hipSuccess) rc = FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && hipMalloc((void **)&g->Yc, sizeof(double2) * (size_t)N * M) != hipSuccess) rc = FTAMD_ENOMEM;
+    int n1[1] = {N}, n2[1] = {M};
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, 2 * M));
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->phi, 1, n2, n2, N, 1, n2, N, 1, HIPFFT_Z2Z, N));
+    if (rc != FTAMD_OK) {
+        ftamd_spingrid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FTAMD_OK;
+}
+
+extern "C" void ftamd_spingrid_destroy(FtamdSpinGrid *g) {
+    if (!g) return;
+    hipDeviceSynchronize();
+    if (g->theta) hipfftDestroy(g->theta);
+    if (g->phi) hipfftDestroy(g->phi);
+    hipFree(g->tw);
+    hipFree(g->Z);
+    hipFree(g->Yc);
+    free(g);
+}
+
+extern "C" int ftamd_spingrid_run(FtamdSpinGrid *g, double *X, double f0, double f1, double a, void *stream) {
+    if (!g || !X) return FTAMD_EINVAL;
+    const int N = g->N, M = g->M;
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 block(256), grid((N + 255) / 256, 2 * M);
+    double2 *Xc = (double2 *)X;
+    int rc = fft_ok(hipfftSetStream(g->theta, st));
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(g->phi, st));
+    if (rc != FTAMD_OK) return rc;
+    if (g->type3) {
+        hipLaunchKernelGGL(ftgrid::cpre3, grid, block, 0, st, Xc, g->Z, g->tw, N, M, g->cls, f0, f1);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::cpost3, grid, block, 0, st, g->Z, g->Yc, N, M, g->cls, a);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        return fft_ok(hipfftExecZ2Z(g->phi, (hipfftDoubleComplex *)g->Yc, (hipfftDoubleComplex *)Xc, HIPFFT_BACKWARD));
+    }
+    rc = fft_ok(hipfftExecZ2Z(g->phi, (hipfftDoubleComplex *)Xc, (hipfftDoubleComplex *)g->Yc, HIPFFT_FORWARD));
+    if (rc != FTAMD_OK) return rc;
+    hipLaunchKernelGGL(ftgrid::cpre2, grid, block, 0, st, g->Yc, g->Z, N, M, g->cls, a);
+    if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+    rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));
+    if (rc != FTAMD_OK) return rc;
+    hipLaunchKernelGGL(ftgrid::cpost2, grid, block, 0, st, g->Z, Xc, g->tw, N, M, g->cls, f0, f1);
+    return hipGetLastError() == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+}

@@ -225,6 +225,16 @@ int ftamd_boxgrid_create(int rank, const int *dims, int type3, FtamdTriGrid **ou
  * before (pre_mask, pre_f) and after (post_mask, post_f) it -- 1: row 0 of
  * the even columns, 2: the last row of the odd columns, 3: column 0 -- then
  * x post_mul and / post_div (0: none). */
+/* The spin-weighted sphere's grid (ft_grid.hip): complex N x M (M odd);
+ * type3: a synthesis plan (DCT-III / DST-III, backward row DFT), else an
+ * analysis plan; odd_spin: the vector field's column classes.  A run with a
+ * synthesis plan: fix-ups (f0 on row 0 of the cosine columns, f1 on the last
+ * row of the sine columns), theta, x a, colswap, row DFT; with an analysis
+ * plan: row DFT, colswap's inverse, x a, theta, fix-ups. */
+typedef struct FtamdSpinGrid FtamdSpinGrid;
+int ftamd_spingrid_create(int N, int M, int type3, int odd_spin, FtamdSpinGrid **out);
+void ftamd_spingrid_destroy(FtamdSpinGrid *g);
+int ftamd_spingrid_run(FtamdSpinGrid *g, double *X, double f0, double f1, double a, void *stream);
 typedef struct FtamdRectGrid FtamdRectGrid;
 int ftamd_rectgrid_create(int N, int M, int type3, FtamdRectGrid **out);
 void ftamd_rectgrid_destroy(FtamdRectGrid *g);

@@ -374,6 +374,27 @@ void ft_execute_rectdisk_analysis(const char TRANS, const ft_rectdisk_fftw_plan 
 int ft_gpu_execute_rectdisk_grid(int analysis, char TRANS, const ft_rectdisk_fftw_plan *P, double *dX, int N, int M,
                                  void *stream);
 
+/* fasttransforms.h:592-617 (fftw.c:742-937) -- synthesis and analysis of
+ * spin-weighted functions on the sphere's grid: X is N x M complex (M odd);
+ * the theta transforms of the sphere (even spin S) or of the vector field
+ * (odd S) on the real and imaginary parts, the column permutation of the
+ * reference's colswap and a complex DFT along the rows (synthesis backward,
+ * analysis forward).  TRANS is 'N' or 'C'; synthesis 'N' and analysis 'C'
+ * take a synthesis plan, synthesis 'C' and analysis 'N' an analysis plan. */
+typedef struct ft_spinsphere_fftw_plan_s ft_spinsphere_fftw_plan;
+void ft_destroy_spinsphere_fftw_plan(ft_spinsphere_fftw_plan *P);
+int ft_get_spin_spinsphere_fftw_plan(const ft_spinsphere_fftw_plan *P);
+ft_spinsphere_fftw_plan *ft_plan_spinsph_synthesis(const int N, const int M, const int S, const unsigned flags);
+ft_spinsphere_fftw_plan *ft_plan_spinsph_analysis(const int N, const int M, const int S, const unsigned flags);
+void ft_execute_spinsph_synthesis(const char TRANS, const ft_spinsphere_fftw_plan *P, ft_complex *X, const int N,
+                                  const int M);
+void ft_execute_spinsph_analysis(const char TRANS, const ft_spinsphere_fftw_plan *P, ft_complex *X, const int N,
+                                 const int M);
+/* The spin-weighted sphere's grid executors on a device-resident complex X
+ * on a HIP stream (analysis 0: synthesis, 1: analysis). */
+int ft_gpu_execute_spinsph_grid(int analysis, char TRANS, const ft_spinsphere_fftw_plan *P, void *dX, int N, int M,
+                                void *stream);
+
 /* The grid transforms on a device-resident X asynchronously on a HIP stream:
  * which = 0 sph synthesis, 1 sph analysis, 2 sphv synthesis, 3 sphv
  * analysis, 4 disk synthesis, 5 disk analysis, 6 annulus synthesis, 7

@@ -774,3 +774,49 @@ def rect_grid(analysis, trans, synthesis_plan, X):
         y(X)
         X /= nm
     return X
+
+
+def spin_grid(analysis, trans, synthesis_plan, S, X):
+    """ft_execute_spinsph_{synthesis,analysis}(TRANS, P, X, N, M) restated
+    (fftw.c:742-937) on a copy of the N x M complex array X: the theta
+    transforms of the sphere's (even S) or the vector field's (odd S) column
+    classes on the real and imaginary parts, colswap (fftw.c:6-21), a complex
+    DFT along the rows (synthesis plans backward, analysis plans forward)."""
+    X = np.array(X, dtype=np.complex128, order="F")
+    N, M = X.shape
+    sphv = S % 2 != 0
+    cos = _grid_classes(M, sphv)
+    perm = np.zeros(M, dtype=int)          # X column j goes to Y column perm[j]
+    for j in range(1, M):
+        perm[j] = j // 2 if j % 2 == 0 else M - (j + 1) // 2
+
+    def theta(A, syn):
+        re, im = np.array(A.real, order="F"), np.array(A.imag, order="F")
+        _theta(re, syn, sphv)
+        _theta(im, syn, sphv)
+        return re + 1j * im
+
+    s12, s2 = 0.199471140200716338, 5.013256549262001005
+    pre3 = (not analysis and trans == "N") or (analysis and trans == "C")
+    if pre3 != bool(synthesis_plan):
+        raise ValueError("this executor and TRANS take the other plan")
+    if pre3:
+        if trans == "N":
+            X[0, cos] *= 2.0
+            a = s12
+        else:
+            X[N - 1, ~cos] *= 2.0
+            a = s2 / (2 * N * M)
+        X = theta(X, True) * a
+        Y = np.zeros_like(X)
+        Y[:, perm] = X
+        return np.fft.ifft(Y, axis=1) * M
+    Y = np.fft.fft(X, axis=1)
+    X = Y[:, perm]
+    X = X * (s12 if trans == "C" else s2 / (2 * N * M))
+    X = theta(X, False)
+    if trans == "C":
+        X[N - 1, ~cos] *= 0.5
+    else:
+        X[0, cos] *= 0.5
+    return X

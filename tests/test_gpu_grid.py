@@ -228,3 +228,47 @@ def test_rectdisk_transform_round_trip(ft, n):
     e2, einf = O.rel2(X, A), O.relinf(X, A)
     _say("rectdisk n=%d round trip rel2 %.2e relinf %.2e" % (n, e2, einf))
     assert e2 < 8 * np.sqrt(n) * EPS and einf < 4 * n * EPS
+
+
+@pytest.mark.parametrize("analysis", [False, True])
+@pytest.mark.parametrize("trans", ["N", "C"])
+def test_spinsph_grid_against_oracle(ft, analysis, trans):
+    for N, M in [(1, 1), (4, 7), (65, 129), (256, 511), (1024, 2047)]:
+        for S in (0, 1, -2):
+            syn = (not analysis) == (trans == "N")
+            P = (ft.plan_spinsph_synthesis if syn else ft.plan_spinsph_analysis)(N, M, S)
+            g = np.random.default_rng(N + M + S + 9)
+            X = np.asfortranarray(g.standard_normal((N, M)) + 1j * g.standard_normal((N, M)))
+            name = "spinsph_analysis" if analysis else "spinsph_synthesis"
+            _say("grid", name, trans, "S=%d N=%d M=%d" % (S, N, M))
+            got = getattr(ft, "execute_" + name)(trans, P, X.copy(order="F"))
+            ref = O.spin_grid(analysis, trans, syn, S, X)
+            assert O.rel2(got, ref) <= 2e-14, (name, trans, S, N, M, O.rel2(got, ref))
+
+
+@pytest.mark.parametrize("S", [0, 1, -2])
+def test_spin_transform_round_trip_1024(ft, S):
+    """The spin-weighted rotations and the grid on the device at n = 1024:
+    hi2lo, synthesis, analysis, lo2hi returns the input within the sphere's
+    bounds (4 sqrt(N) eps rel-2, 2N eps rel-inf)."""
+    import torch
+    n = 1024
+    M = 2 * n - 1
+    R = ft.plan_rotspinsphere(n, S)
+    PS, PA = ft.plan_spinsph_synthesis(n, M, S), ft.plan_spinsph_analysis(n, M, S)
+    g = np.random.default_rng(5 + S)
+    A = np.asfortranarray(g.uniform(-1, 1, (n, M)) + 1j * g.uniform(-1, 1, (n, M)))
+    T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    _say("spin transform round trip S=%d" % S)
+    ft.gpu_execute_spinsph(R, "hi2lo", T)
+    ft.synchronize()
+    X = T.cpu().numpy().T.copy(order="F")
+    ft.execute_spinsph_synthesis("N", PS, X)
+    ft.execute_spinsph_analysis("N", PA, X)
+    T = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    ft.gpu_execute_spinsph(R, "lo2hi", T)
+    ft.synchronize()
+    X = T.cpu().numpy().T
+    e2, einf = O.rel2(X, A), O.relinf(X, A)
+    _say("spin S=%d n=1024 round trip rel2 %.2e relinf %.2e" % (S, e2, einf))
+    assert e2 < 4 * np.sqrt(n) * EPS and einf < 2 * n * EPS

@@ -311,3 +311,43 @@ def test_rectdisk_transposes_and_inverse(N, M):
     assert np.max(np.abs(ST - SN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AT - AN.T)) <= 64 * EPS * N * M
     assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
+
+
+@pytest.mark.parametrize("S", [0, 1, -2])
+def test_spin_synthesis_of_unit_coefficients(S):
+    """fftw.c:800-828: a unit coefficient in row l of column 2q (2q-1) gives
+    a w {cos(l theta), sin((l+1) theta)} e^{+i q phi} (e^{-i q phi}) with
+    a = 1 / (2 sqrt(2 pi)), the sphere's column classes for even S and the
+    vector field's for odd S."""
+    N, M = 4, 7
+    cos = O._grid_classes(M, S % 2 != 0)
+    theta = (2 * np.arange(N) + 1) * np.pi / (2 * N)
+    phi = 2 * np.pi * np.arange(M) / M
+    a = 1.0 / (2.0 * np.sqrt(2.0 * np.pi))
+    for l in range(N):
+        for col in range(M):
+            X = np.zeros((N, M), dtype=np.complex128)
+            X[l, col] = 1.0
+            G = O.spin_grid(False, "N", True, S, X)
+            t = 2 * np.cos(l * theta) if cos[col] else (2.0 if l < N - 1 else 1.0) * np.sin((l + 1) * theta)
+            q = (col + 1) // 2
+            p = np.exp((1j if col % 2 == 0 else -1j) * q * phi)
+            assert np.max(np.abs(G - a * np.outer(t, p))) <= 64 * (N + M) * EPS, (l, col)
+
+
+@pytest.mark.parametrize("S", [0, 3])
+def test_spin_adjoints_and_inverse(S):
+    N, M = 3, 5
+
+    def dense(analysis, trans, syn):
+        D = np.zeros((N * M, N * M), dtype=np.complex128)
+        for i in range(N * M):
+            e = np.zeros(N * M, dtype=np.complex128)
+            e[i] = 1.0
+            D[:, i] = O.spin_grid(analysis, trans, syn, S, e.reshape((N, M), order="F")).reshape(-1, order="F")
+        return D
+    SN, SC = dense(False, "N", True), dense(False, "C", False)
+    AN, AC = dense(True, "N", False), dense(True, "C", True)
+    assert np.max(np.abs(SC - SN.conj().T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AC - AN.conj().T)) <= 64 * EPS * N * M
+    assert np.max(np.abs(AN @ SN - np.eye(N * M))) <= 64 * EPS * N * M
