@@ -500,6 +500,8 @@ def harmonic(ft, torch, n, transform, trans, steps, warmup):
         H, kind = ft.plan_disk2cxf(n, 0.0, 0.0), "disk"
     elif transform in ("rectdisk2cheb", "cheb2rectdisk"):
         H, kind = ft.plan_rectdisk2cheb(n, 0.0), "tri"
+    elif transform in ("ann2cxf", "cxf2ann"):  # the reference test's annulus (test_drivers.c:21)
+        H, kind = ft.plan_ann2cxf(n, 0.0, 0.0, 0.0, float(np.sqrt(1.0 / 3.0))), "disk"
     else:
         H, kind = ft.plan_sph2fourier(n), "sph"
     M = {"sph": 2 * n - 1, "tri": n, "disk": 4 * n - 3}[kind]
@@ -512,7 +514,8 @@ def harmonic(ft, torch, n, transform, trans, steps, warmup):
 def harmonic_rows(ft, torch, n):
     """The harmonic transforms at n against the reference's published times."""
     out = {}
-    for transform in ("sph2fourier", "fourier2sph", "tri2cheb", "cheb2tri", "disk2cxf", "cxf2disk"):
+    for transform in ("sph2fourier", "fourier2sph", "tri2cheb", "cheb2tri", "disk2cxf", "cxf2disk", "ann2cxf",
+                      "cxf2ann"):
         ms = harmonic(ft, torch, n, transform, "N", 10, 2)
         out["%s_ms" % transform] = round(ms, 4)
         pub = PUBLISHED_S[transform].get(n)
@@ -525,7 +528,8 @@ def harmonic_rows(ft, torch, n):
 def run_case(ft, torch, spec, steps, warmup):
     """--case: one configuration only, for profiling runs."""
     parts = spec.split(":")
-    if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv", "rectdisk2cheb", "cheb2rectdisk"):
+    if parts[0] in PUBLISHED_S or parts[0] in ("sphv2fourier", "fourier2sphv", "rectdisk2cheb", "cheb2rectdisk",
+                                               "ann2cxf", "cxf2ann"):
         ms = harmonic(ft, torch, int(parts[1]), parts[0], parts[2] if len(parts) > 2 else "N", steps, warmup)
         print(json.dumps({"case": spec, "ms": round(ms, 5)}), flush=True)
         return

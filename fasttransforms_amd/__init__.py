@@ -78,11 +78,12 @@ SIGNATURES = {
     "ft_plan_tri2cheb": (_vp, [_i, _d, _d, _d]),
     "ft_plan_disk2cxf": (_vp, [_i, _d, _d]),
     "ft_plan_rectdisk2cheb": (_vp, [_i, _d]),
+    "ft_plan_ann2cxf": (_vp, [_i, _d, _d, _d, _d]),
     "ft_destroy_harmonic_plan": (None, [_vp]),
     "ft_gpu_execute_harmonic": (_i, [_i, ctypes.c_char, _vp, _vp, _i, _i, _vp]),
 }
 for _t in ("sph2fourier", "fourier2sph", "sphv2fourier", "fourier2sphv", "tri2cheb", "cheb2tri", "disk2cxf",
-           "cxf2disk", "rectdisk2cheb", "cheb2rectdisk"):
+           "cxf2disk", "rectdisk2cheb", "cheb2rectdisk", "ann2cxf", "cxf2ann"):
     SIGNATURES["ft_execute_" + _t] = (None, [ctypes.c_char, _vp, _vp, _i, _i])
 for _dn in ("hi2lo", "lo2hi"):
     for _k in ("sph", "tri", "disk", "rectdisk"):
@@ -653,10 +654,11 @@ class _HarmonicHead(ctypes.Structure):
 
 
 (SPH2FOURIER, FOURIER2SPH, SPHV2FOURIER, FOURIER2SPHV, DISK2CXF, CXF2DISK, TRI2CHEB, CHEB2TRI, RECTDISK2CHEB,
- CHEB2RECTDISK) = range(10)
+ CHEB2RECTDISK, ANN2CXF, CXF2ANN) = range(12)
 _HARMONIC = {"sph2fourier": SPH2FOURIER, "fourier2sph": FOURIER2SPH, "sphv2fourier": SPHV2FOURIER,
              "fourier2sphv": FOURIER2SPHV, "disk2cxf": DISK2CXF, "cxf2disk": CXF2DISK, "tri2cheb": TRI2CHEB,
-             "cheb2tri": CHEB2TRI, "rectdisk2cheb": RECTDISK2CHEB, "cheb2rectdisk": CHEB2RECTDISK}
+             "cheb2tri": CHEB2TRI, "rectdisk2cheb": RECTDISK2CHEB, "cheb2rectdisk": CHEB2RECTDISK,
+             "ann2cxf": ANN2CXF, "cxf2ann": CXF2ANN}
 
 
 class HarmonicPlan:
@@ -685,10 +687,26 @@ class HarmonicPlan:
         return np.ctypeslib.as_array(self._head().Pinv[k], shape=(self.n * self.n,)).copy().reshape(self.n, self.n,
                                                                                                     order="F")
 
+    def R(self):
+        """The annulus plan's bidiagonal factor MP[1] as a dense upper
+        bidiagonal n x n matrix (a copy)."""
+        head = self._head()
+        if head.NMP != 2:
+            raise ValueError("not an annulus plan")
+        mp = ctypes.cast(head.MP, ctypes.POINTER(ctypes.POINTER(_ModifiedHead)))[1].contents
+        v = np.ctypeslib.as_array(mp.R, shape=(2 * self.n,)).copy()
+        return np.diag(v[:self.n]) + np.diag(v[self.n:2 * self.n - 1], 1)
+
     def __del__(self):
         ptr, self._ptr = getattr(self, "_ptr", None), None
         if ptr:
             self._destroy(ptr)
+
+
+class _ModifiedHead(ctypes.Structure):
+    # the library's modified plan: n, nu, nv, bandwidth, {diagonal[n], superdiagonal[n-1]}
+    _fields_ = [("n", ctypes.c_int), ("nu", ctypes.c_int), ("nv", ctypes.c_int), ("b", ctypes.c_int),
+                ("R", ctypes.POINTER(ctypes.c_double))]
 
 
 def plan_sph2fourier(n):
@@ -705,6 +723,11 @@ def plan_disk2cxf(n, alpha, beta):
 
 def plan_rectdisk2cheb(n, beta):
     return HarmonicPlan(_lib.ft_plan_rectdisk2cheb(int(n), float(beta)), int(n))
+
+
+def plan_ann2cxf(n, alpha, beta, gamma, rho):
+    """ft_plan_ann2cxf: annulus polynomials of inner radius rho (gamma = 0)."""
+    return HarmonicPlan(_lib.ft_plan_ann2cxf(int(n), float(alpha), float(beta), float(gamma), float(rho)), int(n))
 
 
 def _harmonic(name):
@@ -732,6 +755,8 @@ execute_disk2cxf = _harmonic("disk2cxf")
 execute_cxf2disk = _harmonic("cxf2disk")
 execute_rectdisk2cheb = _harmonic("rectdisk2cheb")
 execute_cheb2rectdisk = _harmonic("cheb2rectdisk")
+execute_ann2cxf = _harmonic("ann2cxf")
+execute_cxf2ann = _harmonic("cxf2ann")
 
 
 def gpu_execute_harmonic(transform, trans, plan, A, M=None, stream=None):

@@ -12,7 +12,9 @@
  * Pinv with lo2hi after ('N') or hi2lo before ('T').  tri2cheb: hi2lo, P[0]
  * from the left on every column, P[1]^T from the right on every row, the 2D
  * Chebyshev normalization; cheb2tri runs the inverses in reverse order
- * (drivers.c:921-957).
+ * (drivers.c:921-957).  ann2cxf: the disk's pattern with the annulus's
+ * rotations, a bidiagonal solve (or product) on the columns j = 1, 2 mod 4
+ * and one matrix P[0] on every column (drivers.c:1023-1121).
  */
 #include <math.h>
 #include <pthread.h>
@@ -21,10 +23,19 @@
 
 #include "ft_internal.h"
 
+/* A modified plan of the annulus (drivers.c:1027-1046): the identity (b 0)
+ * or the upper bidiagonal R (b 1), R[0 .. n-1] its diagonal and R[n ..
+ * 2n-2] its superdiagonal.  nu, nv as the reference's members. */
+struct ft_modified_plan_s {
+    int n, nu, nv, b;
+    double *R;
+};
+
 struct ftamd_harmonic_dev {
     pthread_mutex_t lock; /* held for a whole call: guards the buffers below */
     int device;           /* HIP device of the uploads, -1 before */
     double *dP[3], *dPinv[3];
+    double *dR; /* MP[1]'s bidiagonal factor (the annulus's plans) */
     void *dY;             /* product output, n x M */
     size_t y_bytes;
     void *dA;             /* staging of host matrices */
@@ -40,6 +51,7 @@ void ft_destroy_harmonic_plan(ft_harmonic_plan *P) {
             ftamd_hip_free(d->dP[k]);
             ftamd_hip_free(d->dPinv[k]);
         }
+        ftamd_hip_free(d->dR);
         ftamd_hip_free(d->dY);
         ftamd_hip_free(d->dA);
         pthread_mutex_destroy(&d->lock);
@@ -53,6 +65,11 @@ void ft_destroy_harmonic_plan(ft_harmonic_plan *P) {
     }
     free(P->P);
     free(P->Pinv);
+    for (int i = 0; P->MP && i < P->NMP; i++) {
+        if (P->MP[i]) free(P->MP[i]->R);
+        free(P->MP[i]);
+    }
+    free(P->MP);
     free(P);
 }
 
@@ -175,6 +192,91 @@ ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double
     return plan_done(P);
 }
 
+/* drivers.c:1023-1071 with banded_source.c:484-499: MP[1] holds the upper
+ * Cholesky factor of t I + J, J the normalized Jacobi(beta, alpha)
+ * multiplication operator (banded_source.c:1797-1820), in the reference's
+ * order: the subdiagonal entry divided by the pivot before its square root,
+ * the next pivot less that quotient times the superdiagonal entry, the
+ * superdiagonal entry divided by the square root. */
+static ft_modified_plan *annulus_factor(int n, double alpha, double beta, double rho) {
+    ft_modified_plan *F = calloc(1, sizeof(ft_modified_plan));
+    double *R = malloc(sizeof(double) * (size_t)(2 * n));
+    if (!F || !R) {
+        free(F);
+        free(R);
+        return NULL;
+    }
+    F->n = n;
+    F->nu = 1;
+    F->b = 1;
+    F->R = R;
+    const double a = beta, b = alpha; /* the operator's (alpha, beta) are the plan's (beta, alpha) */
+    const double t = 1.0 + 2.0 * rho * rho / (1.0 - rho * rho);
+    double *dg = R, *up = R + n;
+    double p = t + (b - a) / (a + b + 2); /* pivot of row 0 */
+    for (int j = 0; j < n; j++) {
+        if (j + 1 < n) {
+            /* entries (j+1, j) and (j, j+1) of J; equal formulas but for j = 0 */
+            const double lo = j == 0 ? 2 * sqrt(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)))
+                                     : 2 * sqrt(((j + 1) * (j + a + 1) * (j + b + 1) * (j + a + b + 1)) /
+                                                ((2 * j + a + b + 1) * (2 * j + a + b + 2) * (2 * j + a + b + 2) *
+                                                 (2 * j + a + b + 3)));
+            const int k = j + 1;
+            const double upj = k == 1 ? 2 * sqrt(((a + 1) * (b + 1)) / ((a + b + 2) * (a + b + 2) * (a + b + 3)))
+                                      : 2 * sqrt((k * (k + a) * (k + b) * (k + a + b)) /
+                                                 ((2 * k + a + b - 1) * (2 * k + a + b) * (2 * k + a + b) *
+                                                  (2 * k + a + b + 1)));
+            const double diag = ((b - a) * (a + b)) / ((2 * k + a + b) * (2 * k + a + b + 2));
+            const double mult = lo / p;
+            dg[j] = sqrt(p);
+            up[j] = upj / dg[j];
+            p = (t + diag) - mult * upj;
+        } else {
+            dg[j] = sqrt(p);
+        }
+    }
+    return F;
+}
+
+/* drivers.c:1023-1071: gamma = 0 only. */
+ft_harmonic_plan *ft_plan_ann2cxf(const int n, const double alpha, const double beta, const double gamma,
+                                  const double rho) {
+    if (n < 1 || gamma != 0.0 || !(rho >= 0.0 && rho < 1.0)) {
+        ftamd_set_error(FT_GPU_EINVAL);
+        return NULL;
+    }
+    ft_harmonic_plan *P = plan_alloc(1);
+    if (!P) return plan_done(NULL);
+    P->RP[0] = ft_plan_rotannulus(n, alpha, beta, gamma, rho);
+    P->P[0] = ftamd_plan_jac2cheb(1, 0, n, beta, alpha);
+    P->Pinv[0] = ftamd_plan_cheb2jac(0, 1, n, beta, alpha);
+    P->MP = calloc(2, sizeof(ft_modified_plan *));
+    if (P->MP) {
+        P->NMP = 2;
+        P->MP[0] = calloc(1, sizeof(ft_modified_plan));
+        if (P->MP[0]) P->MP[0]->n = n; /* the identity: R NULL, b 0 */
+        P->MP[1] = annulus_factor(n, alpha, beta, rho);
+    }
+    if (!P->MP || !P->MP[0] || !P->MP[1]) {
+        ft_destroy_harmonic_plan(P);
+        return plan_done(NULL);
+    }
+    if (P->P[0] && P->Pinv[0]) {
+        const double cst = sqrt(2.0) * pow(2.0 / (1.0 - rho * rho), 0.5 * (alpha + beta + gamma + 1.0));
+        const double cstinv = 1.0 / cst;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i <= j; i++) {
+                P->P[0][i + (size_t)j * n] *= cst;
+                P->Pinv[0][i + (size_t)j * n] *= cstinv;
+            }
+    }
+    P->alpha = alpha;
+    P->beta = beta;
+    P->gamma = gamma;
+    P->rho = rho;
+    return plan_done(P);
+}
+
 /* Device copies of the connection matrices (caller holds the lock). */
 static int upload(ft_harmonic_plan *P) {
     struct ftamd_harmonic_dev *d = P->dev;
@@ -188,6 +290,11 @@ static int upload(ft_harmonic_plan *P) {
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d->dP[k], P->P[k], bytes, NULL);
         if (rc == FTAMD_OK) rc = ftamd_hip_malloc((void **)&d->dPinv[k], bytes);
         if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d->dPinv[k], P->Pinv[k], bytes, NULL);
+    }
+    if (rc == FTAMD_OK && P->NMP == 2) {
+        const size_t rbytes = sizeof(double) * (size_t)(2 * n);
+        rc = ftamd_hip_malloc((void **)&d->dR, rbytes);
+        if (rc == FTAMD_OK) rc = ftamd_hip_h2d(d->dR, P->MP[1]->R, rbytes, NULL);
     }
     if (rc == FTAMD_OK) rc = ftamd_hip_sync(NULL);
     if (rc == FTAMD_OK) d->device = dev;
@@ -213,7 +320,9 @@ static int rotation_kind(int transform) {
     case FT_GPU_SPHV2FOURIER:
     case FT_GPU_FOURIER2SPHV: return FT_GPU_SPHV;
     case FT_GPU_DISK2CXF:
-    case FT_GPU_CXF2DISK: return FT_GPU_DISK;
+    case FT_GPU_CXF2DISK:
+    case FT_GPU_ANN2CXF:
+    case FT_GPU_CXF2ANN: return FT_GPU_DISK;
     case FT_GPU_TRI2CHEB:
     case FT_GPU_CHEB2TRI: return FT_GPU_TRI;
     case FT_GPU_RECTDISK2CHEB:
@@ -227,7 +336,10 @@ static int valid(int transform, char TRANS, const ft_harmonic_plan *P, int N, in
     if (!P || !P->dev || !P->RP || !P->RP[0] || rk < 0 || (TRANS != 'N' && TRANS != 'T')) return 0;
     /* the plan's rotation tables must be the transform's geometry */
     const int geo = rk == FT_GPU_TRI ? FTAMD_TRI : rk == FT_GPU_DISK ? FTAMD_DISK : FTAMD_SPH;
-    const int np = rk == FT_GPU_RECTDISK ? 3 : 2; /* matrix pairs the transform uses */
+    const int ann = transform == FT_GPU_ANN2CXF || transform == FT_GPU_CXF2ANN;
+    const int np = rk == FT_GPU_RECTDISK ? 3 : ann ? 1 : 2; /* matrix pairs the transform uses */
+    /* the annulus's transforms need its plan (rotations and factor), the disk's a disk plan */
+    if (ann != (P->RP[0]->family == FTAMD_FAMILY_ANNULUS) || (ann && (P->NMP != 2 || !P->MP))) return 0;
     return P->RP[0]->kind == geo && P->NP >= np && N == P->RP[0]->n && M >= 1 && ftamd_valid_M(rk, N, M);
 }
 
@@ -357,6 +469,28 @@ static int run_rectdisk(ft_harmonic_plan *P, int inverse, char TRANS, void *dA, 
     return rc;
 }
 
+/* The annulus's transforms (drivers.c:1073-1121): ann2cxf 'N' is hi2lo,
+ * R^{-1} on the columns j = 1, 2 mod 4 (the identity MP[0] on j = 0, 3 is
+ * skipped), P[0] on every column; 'T' is P[0]^T, R^{-T}, lo2hi.  cxf2ann 'N'
+ * is Pinv[0], R, lo2hi; 'T' is hi2lo, R^T, Pinv[0]^T. */
+static int run_annulus(ft_harmonic_plan *P, int inverse, char TRANS, void *dA, int M, size_t bytes, void *stream) {
+    struct ftamd_harmonic_dev *d = P->dev;
+    const int n = P->RP[0]->n, trans = TRANS == 'T';
+    const double *H = inverse ? d->dPinv[0] : d->dP[0];
+    const int rot_first = inverse ? trans : !trans;
+    int rc = FTAMD_OK;
+    if (rot_first) {
+        rc = ft_gpu_execute(P->RP[0], FT_GPU_DISK, 0, FT_GPU_FP64, dA, M, stream);
+        if (rc == FTAMD_OK) rc = ftamd_hip_bidiag(dA, n, M, d->dR, !inverse, trans, stream);
+        if (rc == FTAMD_OK) rc = tri_product(d, H, 0, trans, n, M, dA, bytes, stream);
+    } else {
+        rc = tri_product(d, H, 0, trans, n, M, dA, bytes, stream);
+        if (rc == FTAMD_OK) rc = ftamd_hip_bidiag(dA, n, M, d->dR, !inverse, trans, stream);
+        if (rc == FTAMD_OK) rc = ft_gpu_execute(P->RP[0], FT_GPU_DISK, 1, FT_GPU_FP64, dA, M, stream);
+    }
+    return rc;
+}
+
 /* The transform on device matrix dA, asynchronously on `stream` (caller
  * holds the lock and has uploaded the matrices). */
 static int run(ft_harmonic_plan *P, int transform, char TRANS, void *dA, int M, void *stream) {
@@ -368,6 +502,8 @@ static int run(ft_harmonic_plan *P, int transform, char TRANS, void *dA, int M, 
     if (rc != FTAMD_OK) return rc;
     if (rk == FT_GPU_TRI) return run_tri(P, inverse, TRANS, dA, M, bytes, stream);
     if (rk == FT_GPU_RECTDISK) return run_rectdisk(P, inverse, TRANS, dA, M, bytes, stream);
+    if (transform == FT_GPU_ANN2CXF || transform == FT_GPU_CXF2ANN)
+        return run_annulus(P, inverse, TRANS, dA, M, bytes, stream);
     /* drivers.c:833-899, 987-1021: rotations first for the forward 'N' and the inverse 'T' */
     const int rot_first = inverse ? TRANS == 'T' : TRANS == 'N';
     const int rot_dir = rot_first ? 0 : 1;
@@ -461,4 +597,12 @@ void ft_execute_disk2cxf(const char TRANS, const ft_harmonic_plan *P, double *A,
 /* drivers.c:1005-1021 */
 void ft_execute_cxf2disk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
     harmonic(FT_GPU_CXF2DISK, TRANS, P, A, N, M);
+}
+/* drivers.c:1073-1096 */
+void ft_execute_ann2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_ANN2CXF, TRANS, P, A, N, M);
+}
+/* drivers.c:1098-1121 */
+void ft_execute_cxf2ann(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M) {
+    harmonic(FT_GPU_CXF2ANN, TRANS, P, A, N, M);
 }

@@ -253,6 +253,10 @@ int ftamd_hip_spin_stage1(const double *tab2, int n, int s, double *A, int c0, i
 /* A[i + j n] *= (j == 0 ? s1 : 1) * (i == 0 ? s1 : 1) * s2, in that order,
  * for the n x M matrix A (the 2D Chebyshev normalization, drivers.c:6-24). */
 int ftamd_hip_cheb_scale(double *A, int n, int M, double s1, double s2, void *stream);
+/* On the columns j = 1, 2 mod 4 of the n x M matrix A: x <- R^{-1} x
+ * (solve, trans 0), R^{-T} x (solve, trans 1), R x or R^T x (solve 0), R the
+ * upper bidiagonal device matrix {diagonal[n], superdiagonal[n-1]}. */
+int ftamd_hip_bidiag(double *A, int n, int M, const double *R, int solve, int trans, void *stream);
 int ftamd_hip_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
 /* The allocation-free entry point's device scratch: one fixed array per

@@ -160,7 +160,74 @@ __global__ void cheb_scale_kernel(double *A, int n, int M, double s1, double s2)
     }
 }
 
+// The annulus's semi-classical <-> classical conversion on the columns j = 1,
+// 2 mod 4 of the n x M matrix A (drivers.c:1073-1121 with banded_source.c:
+// 381-422 for the bidiagonal factor): x <- R^{-1} x, R^{-T} x (SOLVE) or
+// R x, R^T x, R upper bidiagonal with diagonal d and superdiagonal u.  One
+// thread per column walks it in the reference's order (the solves are a
+// recurrence; the products read each old entry once before it is
+// overwritten), with products and differences rounded one at a time as the
+// reference's loops do.  The entries of a column sit 8 bytes apart, so a
+// wave's 64 columns reuse each cache line over 16 steps.
+template <bool SOLVE, bool TRANS>
+__global__ void __launch_bounds__(64) bidiag_kernel(double *A, int n, int count, const double *d, const double *u) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= count) return;
+    double *x = A + (4LL * (q >> 1) + 1 + (q & 1)) * n;
+    if (!TRANS) {
+        if (SOLVE) {  // i = n-1 .. 0: x_i = (x_i - u_i x_{i+1}) / d_i
+            double xn = x[n - 1] / d[n - 1];
+            x[n - 1] = xn;
+            for (int i = n - 2; i >= 0; --i) {
+                xn = (x[i] - u[i] * xn) / d[i];
+                x[i] = xn;
+            }
+        } else {  // i = 0 .. n-1: x_i = d_i x_i + u_i x_{i+1} (old x_{i+1})
+            double xi = x[0];
+            for (int i = 0; i + 1 < n; ++i) {
+                const double xn = x[i + 1];
+                x[i] = d[i] * xi + u[i] * xn;
+                xi = xn;
+            }
+            x[n - 1] = d[n - 1] * xi;
+        }
+    } else {
+        if (SOLVE) {  // i = 0 .. n-1: x_i = (x_i - u_{i-1} x_{i-1}) / d_i
+            double xp = x[0] / d[0];
+            x[0] = xp;
+            for (int i = 1; i < n; ++i) {
+                xp = (x[i] - u[i - 1] * xp) / d[i];
+                x[i] = xp;
+            }
+        } else {  // i = n-1 .. 0: x_i = u_{i-1} x_{i-1} + d_i x_i (old x_{i-1})
+            double xi = x[n - 1];
+            for (int i = n - 1; i > 0; --i) {
+                const double xp = x[i - 1];
+                x[i] = u[i - 1] * xp + d[i] * xi;
+                xi = xp;
+            }
+            x[0] = d[0] * xi;
+        }
+    }
+}
+
 }  // namespace fttrmm
+
+extern "C" int ftamd_hip_bidiag(double *A, int n, int M, const double *R, int solve, int trans, void *stream) {
+    if (n < 1 || M < 0 || !A || !R) return FTAMD_EINVAL;
+    const int count = M / 4 * 2 + (M % 4 >= 2 ? 1 : 0) + (M % 4 >= 3 ? 1 : 0);  // columns j < M, j = 1, 2 mod 4
+    if (count == 0) return FTAMD_OK;
+    const dim3 grid((count + 63) / 64), block(64);
+    hipStream_t st = (hipStream_t)stream;
+    const double *d = R, *u = R + n;
+    if (solve && trans) hipLaunchKernelGGL((fttrmm::bidiag_kernel<true, true>), grid, block, 0, st, A, n, count, d, u);
+    else if (solve) hipLaunchKernelGGL((fttrmm::bidiag_kernel<true, false>), grid, block, 0, st, A, n, count, d, u);
+    else if (trans) hipLaunchKernelGGL((fttrmm::bidiag_kernel<false, true>), grid, block, 0, st, A, n, count, d, u);
+    else hipLaunchKernelGGL((fttrmm::bidiag_kernel<false, false>), grid, block, 0, st, A, n, count, d, u);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
+}
 
 extern "C" int ftamd_hip_trmm_launch(const FtamdTrmmLaunch *a, void *stream) {
     if (a->n < 1 || a->M < 0 || a->cstride < 1 || a->ncls < 1 || a->ncls > 4 || !a->X || !a->Y) return FTAMD_EINVAL;

@@ -148,8 +148,9 @@ void ft_execute_tet_lo2hi(const ft_rotation_plan *RP1, const ft_rotation_plan *R
  * connection matrices P and their inverses Pinv (host FP64, n x n
  * column-major, upper triangular) and parameters; the reference's members
  * first, then this library's device state.  ft_modified_plan is opaque here
- * (no plan of this library builds one; MP is NULL, NMP 0), and B, the
- * reference's host scratch, is NULL: the device path needs none. */
+ * (only ft_plan_ann2cxf builds them; every other plan has MP NULL, NMP 0),
+ * and B, the reference's host scratch, is NULL: the device path needs
+ * none. */
 typedef struct ft_modified_plan_s ft_modified_plan;
 typedef struct {
     ft_rotation_plan **RP;
@@ -220,6 +221,24 @@ void ft_execute_cheb2rectdisk(const char TRANS, const ft_harmonic_plan *P, doubl
 ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double beta);
 void ft_execute_disk2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 void ft_execute_cxf2disk(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+
+/* fasttransforms.h:417-423 (drivers.c:1023-1121) -- annulus polynomials
+ * (inner radius rho, gamma = 0 only, as the reference) <-> Chebyshev x
+ * Fourier: the annulus rotation plan; MP[0] the identity (columns j = 0, 3
+ * mod 4) and MP[1] the upper bidiagonal Cholesky factor R of t I + J, J the
+ * normalized Jacobi(beta, alpha) multiplication operator and t = 1 +
+ * 2 rho^2 / (1 - rho^2) (columns j = 1, 2 mod 4); P[0] = Jacobi(beta, alpha)
+ * -> Chebyshev and Pinv[0] its inverse, scaled by sqrt(2) (2 / (1 -
+ * rho^2))^((alpha + beta + 1) / 2) and its reciprocal.  A is N x M (M odd,
+ * M <= 4N-3).  ann2cxf 'N' is hi2lo, R^{-1} on the columns j = 1, 2 mod 4,
+ * then P[0] from the left on every column; 'T' the transposes in reverse
+ * order.  cxf2ann 'N' is Pinv[0], R, then lo2hi.  gamma != 0 or rho outside
+ * [0, 1) sets FT_GPU_EINVAL (the reference warns and builds an unusable
+ * plan). */
+ft_harmonic_plan *ft_plan_ann2cxf(const int n, const double alpha, const double beta, const double gamma,
+                                  const double rho);
+void ft_execute_ann2cxf(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
+void ft_execute_cxf2ann(const char TRANS, const ft_harmonic_plan *P, double *A, const int N, const int M);
 
 /* ------------------------------------------------------------------------
  * Extensions (no counterpart in the reference).
@@ -428,7 +447,7 @@ int ft_gpu_execute_noalloc(const ft_rotation_plan *RP, int kind, int direction, 
                            void *stream);
 
 /* The harmonic transforms on a device-resident A (dA, N x M FP64) on a HIP
- * stream, asynchronously: transform FT_GPU_SPH2FOURIER .. FT_GPU_CHEB2RECTDISK,
+ * stream, asynchronously: transform FT_GPU_SPH2FOURIER .. FT_GPU_CXF2ANN,
  * TRANS 'N' or 'T', as the executors above.
  * Errors of the launches come back through ft_gpu_synchronize.  Calls on
  * one harmonic plan share its device work buffer: order them on one
@@ -443,7 +462,9 @@ enum {
     FT_GPU_TRI2CHEB = 6,
     FT_GPU_CHEB2TRI = 7,
     FT_GPU_RECTDISK2CHEB = 8,
-    FT_GPU_CHEB2RECTDISK = 9
+    FT_GPU_CHEB2RECTDISK = 9,
+    FT_GPU_ANN2CXF = 10,
+    FT_GPU_CXF2ANN = 11
 };
 int ft_gpu_execute_harmonic(int transform, char TRANS, const ft_harmonic_plan *P, void *dA, int N, int M,
                             void *stream);

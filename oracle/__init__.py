@@ -375,16 +375,97 @@ def _rectdisk_harmonic(transform, trans, plan, H, A, M):
     return A
 
 
-def harmonic(transform, trans, plan, P, Pinv, A, M=None, params=None):
+def jacobi_operator(n, a, b):
+    """(diagonal, off-diagonal) of the n x n normalized Jacobi(a, b)
+    multiplication operator (banded_source.c:1797-1820 with norm = 1): x p_k =
+    off_{k-1} p_{k-1} + diag_k p_k + off_k p_{k+1} for the orthonormal
+    Jacobi polynomials p_k of weight (1 - x)^a (1 + x)^b."""
+    k = np.arange(n, dtype=np.float64)
+    s = 2 * k + a + b
+    diag = np.empty(n)
+    diag[0] = (b - a) / (a + b + 2)
+    diag[1:] = (b - a) * (a + b) / (s[1:] * (s[1:] + 2))
+    j = k[1:]
+    sj = s[1:]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        off = 2 * np.sqrt(j * (j + a) * (j + b) * (j + a + b) / ((sj - 1) * sj * sj * (sj + 1)))
+    if n > 1:  # the first entry: j (j + a + b) / (2 j + a + b - 1) is 0/0 at a + b = -1
+        off[0] = 2 * np.sqrt((a + 1) * (b + 1) / ((a + b + 2) ** 2 * (a + b + 3)))
+    return diag, off[:n - 1]
+
+
+def annulus_factor(n, alpha, beta, rho):
+    """The upper bidiagonal Cholesky factor R of t I + J (drivers.c:1027-1046):
+    J the normalized Jacobi(beta, alpha) operator, t = 1 + 2 rho^2 / (1 -
+    rho^2); numpy's Cholesky of the dense matrix."""
+    d, e = jacobi_operator(n, beta, alpha)
+    t = 1.0 + 2.0 * rho * rho / (1.0 - rho * rho)
+    T = np.diag(d + t) + np.diag(e, 1) + np.diag(e, -1)
+    return np.triu(np.linalg.cholesky(T).T)
+
+
+def _bidiag(R, X, solve, trans):
+    """x <- R^{-1} x, R^{-T} x, R x or R^T x on every column of X for the upper
+    bidiagonal R, in the order of banded_source.c:381-422."""
+    n = R.shape[0]
+    d, u = np.diag(R).copy(), np.diag(R, 1).copy()
+    if solve and not trans:
+        for i in range(n - 1, -1, -1):
+            X[i] = (X[i] - (u[i] * X[i + 1] if i + 1 < n else 0.0)) / d[i]
+    elif solve:
+        for i in range(n):
+            X[i] = (X[i] - (u[i - 1] * X[i - 1] if i > 0 else 0.0)) / d[i]
+    elif not trans:
+        for i in range(n):
+            X[i] = d[i] * X[i] + (u[i] * X[i + 1] if i + 1 < n else 0.0)
+    else:
+        for i in range(n - 1, -1, -1):
+            X[i] = (u[i - 1] * X[i - 1] if i > 0 else 0.0) + d[i] * X[i]
+    return X
+
+
+def _annulus_harmonic(transform, trans, plan, H, R, A, M):
+    """ann2cxf / cxf2ann (drivers.c:1073-1121): the annulus's rotations, the
+    bidiagonal R on the columns j = 1, 2 mod 4 (a solve for ann2cxf, a
+    product for cxf2ann; the identity on j = 0, 3) and one matrix on every
+    column (long double product)."""
+    inverse = transform == "cxf2ann"
+    cols = np.array([j for j in range(M) if j % 4 in (1, 2)], dtype=np.int64)
+
+    def conv():
+        if cols.size:
+            A[:, cols] = _bidiag(R, A[:, cols].copy(), not inverse, trans == "T")
+
+    def prod():
+        U = np.triu(H).astype(np.longdouble)
+        A[:, :M] = ((U.T if trans == "T" else U) @ A[:, :M].astype(np.longdouble)).astype(np.float64)
+
+    rot_first = (trans == "T") if inverse else (trans == "N")
+    if rot_first:
+        execute(plan, HI2LO, A, M)
+        conv()
+        prod()
+    else:
+        prod()
+        conv()
+        execute(plan, LO2HI, A, M)
+    return A
+
+
+def harmonic(transform, trans, plan, P, Pinv, A, M=None, params=None, R=None):
     """The harmonic transforms of drivers.c:833-1021 on the n x M matrix A (in
     place): transform "sph2fourier", "fourier2sph", "sphv2fourier",
     "fourier2sphv" (plan: the sphere's rotation Plan), "disk2cxf", "cxf2disk"
     (the disk's), "tri2cheb", "cheb2tri" (the triangle's, params its
     (alpha, beta, gamma)) or "rectdisk2cheb", "cheb2rectdisk" (the
     rectangle-disk's; three matrices); trans "N" or "T"; P and Pinv the
-    connection matrices and their inverses.  The rotations are the oracle's; each
-    triangular product is evaluated in long double and rounded once."""
+    connection matrices and their inverses; "ann2cxf", "cxf2ann" (the
+    annulus's Plan, P = (P[0],), R the bidiagonal factor of annulus_factor).
+    The rotations are the oracle's; each triangular product is evaluated in
+    long double and rounded once."""
     M = A.shape[1] if M is None else M
+    if transform in ("ann2cxf", "cxf2ann"):
+        return _annulus_harmonic(transform, trans, plan, (Pinv if transform == "cxf2ann" else P)[0], R, A, M)
     if transform in ("tri2cheb", "cheb2tri"):
         return _tri_harmonic(transform, trans, plan, Pinv if transform == "cheb2tri" else P, A, M, params)
     if transform in ("rectdisk2cheb", "cheb2rectdisk"):

@@ -140,3 +140,70 @@ def test_rectdisk_transforms(ft, transform, trans, beta):
             G = getattr(ft, "execute_" + transform)(trans, H, A.copy(order="F"))
             R = O.harmonic(transform, trans, Q, P, Pinv, A.copy(order="F"))
             assert O.rel2(G, R) <= 1e-14, (transform, trans, beta, n, M, O.rel2(G, R))
+
+
+ANNULUS = [(0.0, 0.0, float(np.sqrt(1.0 / 3.0))), (0.5, 1.5, 0.3), (-0.5, -0.5, 0.0)]
+
+
+@pytest.mark.parametrize("transform", ["ann2cxf", "cxf2ann"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("params", ANNULUS)
+def test_annulus_transforms(ft, transform, trans, params):
+    """ann2cxf / cxf2ann (drivers.c:1073-1121: the annulus rotations, the
+    bidiagonal factor on the columns j = 1, 2 mod 4, P[0] on every column)
+    against the oracle composite with the plan's own matrices."""
+    alpha, beta, rho = params
+    for n in (1, 2, 5, 64, 129, 300):
+        H = ft.plan_ann2cxf(n, alpha, beta, 0.0, rho)
+        Q = O.Plan("annulus", n, alpha, beta, rho=rho)
+        P, Pinv, R = (H.P(0),), (H.Pinv(0),), H.R()
+        for M in sorted({m for m in (1, 3, 5, 37, 4 * n - 3) if m <= 4 * n - 3}):
+            A = np.asfortranarray(np.random.default_rng(n + M).uniform(-1, 1, (n, M)))
+            _say("harmonic", transform, params, trans, "n=%d M=%d" % (n, M))
+            G = getattr(ft, "execute_" + transform)(trans, H, A.copy(order="F"))
+            X = O.harmonic(transform, trans, Q, P, Pinv, A.copy(order="F"), R=R)
+            assert O.rel2(G, X) <= 1e-14, (transform, params, trans, n, M, O.rel2(G, X))
+
+
+def test_annulus_round_trips_as_the_reference_tests(ft):
+    """The reference's accuracy test (test_drivers.c:1128-1156): ann2cxf and
+    cxf2ann 'N', then 'T', on a disk-shaped random matrix with (alpha, beta)
+    = (0, 0) and rho = sqrt(1/3); rel-2 error under 12 sqrt(N) eps and rel-inf
+    under 6 N eps.  The device-resident entry point runs the same sequence;
+    then the grid round trip of test_fftw.c:289-321 (ann2cxf, annulus
+    synthesis, analysis, cxf2ann, and the transposes) under 8 sqrt(N) eps and
+    4 N eps."""
+    import torch
+    rho = float(np.sqrt(1.0 / 3.0))
+    for N in (65, 257, 1025):
+        M = 4 * N - 3
+        H = ft.plan_ann2cxf(N, 0.0, 0.0, 0.0, rho)
+        A = O.fill("disk", N, M, N)
+        X = A.copy(order="F")
+        for trans in ("N", "T"):
+            ft.execute_ann2cxf(trans, H, X)
+            ft.execute_cxf2ann(trans, H, X)
+        e2, einf = O.rel2(X, A), O.relinf(X, A)
+        _say("annulus round trip N=%d rel2 %.2e relinf %.2e" % (N, e2, einf))
+        assert e2 < 12 * np.sqrt(N) * 2.0 ** -52 and einf < 6 * N * 2.0 ** -52
+        T = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+        for trans in ("N", "T"):
+            ft.gpu_execute_harmonic("ann2cxf", trans, H, T)
+            ft.gpu_execute_harmonic("cxf2ann", trans, H, T)
+        ft.synchronize()
+        Y = T.cpu().numpy().T
+        assert np.array_equal(Y, X) or O.rel2(Y, X) < 1e-15
+        S, An = ft.plan_annulus_synthesis(N, M, rho), ft.plan_annulus_analysis(N, M, rho)
+        X = A.copy(order="F")
+        ft.execute_ann2cxf("N", H, X)
+        ft.execute_annulus_synthesis("N", S, X)
+        ft.execute_annulus_analysis("N", An, X)
+        ft.execute_cxf2ann("N", H, X)
+        ft.execute_ann2cxf("T", H, X)
+        ft.execute_annulus_synthesis("T", An, X)
+        ft.execute_annulus_analysis("T", S, X)
+        ft.execute_cxf2ann("T", H, X)
+        e2, einf = O.rel2(X, A), O.relinf(X, A)
+        _say("annulus grid round trip N=%d rel2 %.2e relinf %.2e" % (N, e2, einf))
+        assert e2 < 8 * np.sqrt(N) * 2.0 ** -52 and einf < 4 * N * 2.0 ** -52
+

@@ -289,3 +289,76 @@ def test_rectdisk2cheb_plan_and_oracle_round_trip(ft, beta):
     A = O.fill("tri", n, n, 9)
     X = O.harmonic("cheb2rectdisk", "N", Q, Pm, Pi, O.harmonic("rectdisk2cheb", "N", Q, Pm, Pi, A.copy(order="F")))
     assert O.rel2(X, A) < 1e-13
+
+
+@pytest.mark.parametrize("alpha,beta,rho", [(0.0, 0.0, math.sqrt(1.0 / 3.0)), (0.5, -0.25, 0.2), (-0.5, -0.5, 0.9),
+                                            (1.5, 0.5, 0.0)])
+def test_annulus_factor_is_the_gram_cholesky(ft, alpha, beta, rho):
+    """ft_plan_ann2cxf's MP[1] (drivers.c:1036-1046): R^T R is the Gram matrix
+    of the orthonormal Jacobi(beta, alpha) polynomials under the extra weight
+    t + x, t = (1 + rho^2) / (1 - rho^2) (the annulus's r^2 in x), computed by
+    Gauss-Jacobi quadrature with scipy's polynomials; the oracle's operator
+    has the Gauss-Jacobi nodes as eigenvalues and its Cholesky factor agrees
+    with the library's to rounding."""
+    from scipy.special import eval_jacobi, roots_jacobi
+    n = 40
+    H = ft.plan_ann2cxf(n, alpha, beta, 0.0, rho)
+    R = H.R()
+    x, w = roots_jacobi(n + 1, beta, alpha)
+    P = np.array([eval_jacobi(k, beta, alpha, x) / _jacobi_norm(k, beta, alpha) for k in range(n)])
+    t = (1 + rho * rho) / (1 - rho * rho)
+    G = (P * (w * (t + x))) @ P.T
+    assert np.max(np.abs(R.T @ R - G)) <= 1e-12 * np.max(np.abs(G))
+    d, e = O.jacobi_operator(n, beta, alpha)
+    nodes = np.sort(np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1)))
+    assert np.max(np.abs(nodes - np.sort(roots_jacobi(n, beta, alpha)[0]))) < 1e-14
+    Ro = O.annulus_factor(n, alpha, beta, rho)
+    assert np.max(np.abs(R - Ro)) <= 8 * EPS * np.max(np.abs(Ro))
+
+
+@pytest.mark.parametrize("alpha,beta,rho", [(0.0, 0.0, math.sqrt(1.0 / 3.0)), (0.5, 1.5, 0.3)])
+def test_ann2cxf_plan_and_oracle_round_trip(ft, alpha, beta, rho):
+    """The plan holds Jacobi(beta, alpha) -> Chebyshev scaled by sqrt(2) (2 /
+    (1 - rho^2))^((alpha + beta + 1) / 2) (drivers.c:1054-1063), and its
+    inverse by the reciprocal; the composite oracle (drivers.c:1073-1121)
+    inverts itself and its 'T' is the transpose of its 'N'."""
+    n = 40
+    H = ft.plan_ann2cxf(n, alpha, beta, 0.0, rho)
+    head = H._head()
+    assert head.NP == 1 and head.NMP == 2 and head.rho == rho
+    cst = math.sqrt(2.0) * (2.0 / (1.0 - rho * rho)) ** (0.5 * (alpha + beta + 1.0))
+    C = np.triu(_lib_matrix(ft, "ftamd_plan_jac2cheb", n, 1, 0, n, beta, alpha))
+    assert np.max(np.abs(H.P(0) - C * cst)) <= 2 * EPS * np.max(np.abs(C * cst))
+    assert np.max(np.abs(H.Pinv(0) @ H.P(0) - np.eye(n))) < 1e-12
+    Q, M = O.Plan("annulus", n, alpha, beta, rho=rho), 4 * n - 3
+    P, Pinv, R = (H.P(0),), (H.Pinv(0),), H.R()
+    A = O.fill("disk", n, M, 5)
+    X = O.harmonic("cxf2ann", "N", Q, P, Pinv, O.harmonic("ann2cxf", "N", Q, P, Pinv, A.copy(order="F"), R=R), R=R)
+    assert O.rel2(X, A) < 1e-13
+    rng = np.random.default_rng(4)
+    x = np.asfortranarray(rng.standard_normal((n, M)))
+    y = np.asfortranarray(rng.standard_normal((n, M)))
+    Fx = O.harmonic("ann2cxf", "N", Q, P, Pinv, x.copy(order="F"), R=R)
+    Fty = O.harmonic("ann2cxf", "T", Q, P, Pinv, y.copy(order="F"), R=R)
+    assert abs(np.sum(y * Fx) - np.sum(Fty * x)) < 1e-12 * np.sqrt(np.sum(y * y) * np.sum(x * x)) * n
+
+
+@pytest.mark.parametrize("args", [(10, 0.0, 0.0, 0.5, 0.5), (10, 0.0, 0.0, 0.0, 1.0), (10, 0.0, 0.0, 0.0, -0.1),
+                                  (0, 0.0, 0.0, 0.0, 0.5)])
+def test_ann2cxf_rejects_what_the_reference_cannot_plan(ft, args):
+    """gamma != 0 (drivers.c:1048-1051 only warns), rho outside [0, 1) and
+    n < 1 are errors."""
+    with pytest.raises(ft.GPUError):
+        ft.plan_ann2cxf(*args)
+
+
+def test_annulus_plan_mismatch_is_rejected(ft):
+    """An annulus plan does not run the disk's transforms, nor a disk plan the
+    annulus's (the reference would read members the plan lacks)."""
+    A = np.asfortranarray(np.zeros((8, 29)))
+    H = ft.plan_ann2cxf(8, 0.0, 0.0, 0.0, 0.5)
+    D = ft.plan_disk2cxf(8, 0.0, 0.0)
+    with pytest.raises(ft.GPUError):
+        ft.execute_disk2cxf("N", H, A)
+    with pytest.raises(ft.GPUError):
+        ft.execute_ann2cxf("N", D, A)
