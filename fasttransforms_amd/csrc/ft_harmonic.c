@@ -200,7 +200,7 @@ ft_harmonic_plan *ft_plan_disk2cxf(const int n, const double alpha, const double
  * superdiagonal entry divided by the square root. */
 static ft_modified_plan *annulus_factor(int n, double alpha, double beta, double rho) {
     ft_modified_plan *F = calloc(1, sizeof(ft_modified_plan));
-    double *R = malloc(sizeof(double) * (size_t)(2 * n));
+    double *R = calloc((size_t)(2 * n), sizeof(double)); /* the last entry is padding */
     if (!F || !R) {
         free(F);
         free(R);

@@ -162,53 +162,111 @@ __global__ void cheb_scale_kernel(double *A, int n, int M, double s1, double s2)
 
 // The annulus's semi-classical <-> classical conversion on the columns j = 1,
 // 2 mod 4 of the n x M matrix A (drivers.c:1073-1121 with banded_source.c:
-// 381-422 for the bidiagonal factor): x <- R^{-1} x, R^{-T} x (SOLVE) or
-// R x, R^T x, R upper bidiagonal with diagonal d and superdiagonal u.  One
-// thread per column walks it in the reference's order (the solves are a
-// recurrence; the products read each old entry once before it is
-// overwritten), with products and differences rounded one at a time as the
-// reference's loops do.  The entries of a column sit 8 bytes apart, so a
-// wave's 64 columns reuse each cache line over 16 steps.
-template <bool SOLVE, bool TRANS>
-__global__ void __launch_bounds__(64) bidiag_kernel(double *A, int n, int count, const double *d, const double *u) {
+// 381-422 for the bidiagonal factor): x <- R^{-1} x, R^{-T} x (solves) or
+// R x, R^T x (products), R upper bidiagonal with diagonal d and
+// superdiagonal u, products and differences rounded one at a time in the
+// reference's order.  Columns q = 0 .. count-1 of the two classes are
+// column 4 (q / 2) + 1 + q % 2.
+__device__ inline double *bidiag_column(double *A, int n, int q) { return A + (4LL * (q >> 1) + 1 + (q & 1)) * n; }
+
+// The solves are a recurrence: one thread per column walks it (R^{-T} from row
+// 0 up, R^{-1} from row n-1 down).  The column streams through registers in
+// chunks of CH rows (one 128-byte line) with the next chunk's loads in flight
+// during this one's arithmetic, so the division chain is the critical path.
+template <bool TRANS>
+__global__ void __launch_bounds__(64) bidiag_solve_kernel(double *A, int n, int count, const double *d,
+                                                          const double *u) {
 #pragma clang fp contract(off)
+    constexpr int CH = 16;
+    constexpr bool ASC = TRANS;
     const int q = blockIdx.x * 64 + threadIdx.x;
     if (q >= count) return;
-    double *x = A + (4LL * (q >> 1) + 1 + (q & 1)) * n;
-    if (!TRANS) {
-        if (SOLVE) {  // i = n-1 .. 0: x_i = (x_i - u_i x_{i+1}) / d_i
-            double xn = x[n - 1] / d[n - 1];
-            x[n - 1] = xn;
-            for (int i = n - 2; i >= 0; --i) {
-                xn = (x[i] - u[i] * xn) / d[i];
-                x[i] = xn;
-            }
-        } else {  // i = 0 .. n-1: x_i = d_i x_i + u_i x_{i+1} (old x_{i+1})
-            double xi = x[0];
-            for (int i = 0; i + 1 < n; ++i) {
-                const double xn = x[i + 1];
-                x[i] = d[i] * xi + u[i] * xn;
-                xi = xn;
-            }
-            x[n - 1] = d[n - 1] * xi;
+    double *x = bidiag_column(A, n, q);
+    // row i's superdiagonal entry is u[i - 1] (R^T) or u[i] (R); the walk's first row has none.
+    // u[-1] and u[n - 1] are in the caller's buffer (d[n - 1] and the padding entry).
+    auto has_u = [&](int i) { return TRANS ? i > 0 : i + 1 < n; };
+    // full chunks: rows b .. b + CH - 1 with b = c CH (ascending) or n - (c + 1) CH (descending)
+    const int nfull = n / CH;
+    auto base = [&](int c) { return ASC ? c * CH : n - (c + 1) * CH; };
+    double cx[CH], cd[CH], cu[CH], nx[CH], nd[CH], nu[CH];
+    auto load = [&](int b, double *vx, double *vd, double *vu) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            vx[k] = x[b + k];
+            vd[k] = d[b + k];
+            vu[k] = u[TRANS ? b + k - 1 : b + k];
         }
-    } else {
-        if (SOLVE) {  // i = 0 .. n-1: x_i = (x_i - u_{i-1} x_{i-1}) / d_i
-            double xp = x[0] / d[0];
-            x[0] = xp;
-            for (int i = 1; i < n; ++i) {
-                xp = (x[i] - u[i - 1] * xp) / d[i];
-                x[i] = xp;
-            }
-        } else {  // i = n-1 .. 0: x_i = u_{i-1} x_{i-1} + d_i x_i (old x_{i-1})
-            double xi = x[n - 1];
-            for (int i = n - 1; i > 0; --i) {
-                const double xp = x[i - 1];
-                x[i] = u[i - 1] * xp + d[i] * xi;
-                xi = xp;
-            }
-            x[0] = d[0] * xi;
+    };
+    if (nfull > 0) load(base(0), cx, cd, cu);
+    double carry = 0.0;  // the last new entry
+    for (int c = 0; c < nfull; ++c) {
+        const int b = base(c);
+        load(c + 1 < nfull ? base(c + 1) : b, nx, nd, nu);
+#pragma unroll
+        for (int kk = 0; kk < CH; ++kk) {
+            const int k = ASC ? kk : CH - 1 - kk;
+            const int i = b + k;
+            carry = (cx[k] - (has_u(i) ? cu[k] * carry : 0.0)) / cd[k];
+            x[i] = carry;
         }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            cx[k] = nx[k];
+            cd[k] = nd[k];
+            cu[k] = nu[k];
+        }
+    }
+    // the remaining n mod CH rows (ascending: the last ones, descending: the first ones)
+    const int rem = n - nfull * CH;
+    if (rem == 0) return;
+    const int b = ASC ? nfull * CH : 0;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const bool in = k < rem;
+        cx[k] = in ? x[b + k] : 0.0;
+        cd[k] = in ? d[b + k] : 1.0;
+        cu[k] = in && has_u(b + k) ? u[TRANS ? b + k - 1 : b + k] : 0.0;
+    }
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+        const int k = ASC ? kk : CH - 1 - kk;
+        if (k >= rem) continue;
+        const int i = b + k;
+        carry = (cx[k] - (has_u(i) ? cu[k] * carry : 0.0)) / cd[k];
+        x[i] = carry;
+    }
+}
+
+// The products have no recurrence: one wave per column, lane l on row r0 + l
+// of each 64-row segment (coalesced 512-byte accesses), its neighbour from
+// the next lane (R: x_{i+1}) or the previous one (R^T: x_{i-1}), the segment
+// edge's from memory.  Segments run in the order that reads every neighbour
+// before its own segment overwrites it: upward for R, downward for R^T.
+template <bool TRANS>
+__global__ void __launch_bounds__(256) bidiag_mul_kernel(double *A, int n, int count, const double *d,
+                                                        const double *u) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (q >= count) return;  // whole waves
+    double *x = bidiag_column(A, n, q);
+    const int nseg = (n + 63) / 64;
+    for (int s = 0; s < nseg; ++s) {
+        const int r0 = (TRANS ? nseg - 1 - s : s) * 64, i = r0 + lane;
+        const bool in = i < n;
+        const double xi = in ? x[i] : 0.0;
+        double y;
+        if (!TRANS) {  // x_i = d_i x_i + u_i x_{i+1}
+            const double edge = r0 + 64 < n ? x[r0 + 64] : 0.0;
+            const double nb = __shfl_down(xi, 1, 64);
+            const double xn = lane == 63 ? edge : nb;
+            y = in ? (i + 1 < n ? d[i] * xi + u[i] * xn : d[i] * xi) : 0.0;
+        } else {  // x_i = u_{i-1} x_{i-1} + d_i x_i
+            const double edge = r0 > 0 ? x[r0 - 1] : 0.0;
+            const double nb = __shfl_up(xi, 1, 64);
+            const double xp = lane == 0 ? edge : nb;
+            y = in ? (i > 0 ? u[i - 1] * xp + d[i] * xi : d[i] * xi) : 0.0;
+        }
+        if (in) x[i] = y;
     }
 }
 
@@ -218,13 +276,14 @@ extern "C" int ftamd_hip_bidiag(double *A, int n, int M, const double *R, int so
     if (n < 1 || M < 0 || !A || !R) return FTAMD_EINVAL;
     const int count = M / 4 * 2 + (M % 4 >= 2 ? 1 : 0) + (M % 4 >= 3 ? 1 : 0);  // columns j < M, j = 1, 2 mod 4
     if (count == 0) return FTAMD_OK;
-    const dim3 grid((count + 63) / 64), block(64);
     hipStream_t st = (hipStream_t)stream;
     const double *d = R, *u = R + n;
-    if (solve && trans) hipLaunchKernelGGL((fttrmm::bidiag_kernel<true, true>), grid, block, 0, st, A, n, count, d, u);
-    else if (solve) hipLaunchKernelGGL((fttrmm::bidiag_kernel<true, false>), grid, block, 0, st, A, n, count, d, u);
-    else if (trans) hipLaunchKernelGGL((fttrmm::bidiag_kernel<false, true>), grid, block, 0, st, A, n, count, d, u);
-    else hipLaunchKernelGGL((fttrmm::bidiag_kernel<false, false>), grid, block, 0, st, A, n, count, d, u);
+    const dim3 sgrid((count + 63) / 64), sblock(64);  // a thread per column
+    const dim3 mgrid((count + 3) / 4), mblock(256);   // a wave per column
+    if (solve && trans) hipLaunchKernelGGL(fttrmm::bidiag_solve_kernel<true>, sgrid, sblock, 0, st, A, n, count, d, u);
+    else if (solve) hipLaunchKernelGGL(fttrmm::bidiag_solve_kernel<false>, sgrid, sblock, 0, st, A, n, count, d, u);
+    else if (trans) hipLaunchKernelGGL(fttrmm::bidiag_mul_kernel<true>, mgrid, mblock, 0, st, A, n, count, d, u);
+    else hipLaunchKernelGGL(fttrmm::bidiag_mul_kernel<false>, mgrid, mblock, 0, st, A, n, count, d, u);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
 }
