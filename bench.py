@@ -518,7 +518,7 @@ def harmonic_rows(ft, torch, n):
                       "cxf2ann"):
         ms = harmonic(ft, torch, n, transform, "N", 10, 2)
         out["%s_ms" % transform] = round(ms, 4)
-        pub = PUBLISHED_S[transform].get(n)
+        pub = PUBLISHED_S.get(transform, {}).get(n)
         if pub:
             out["%s_published_ms" % transform] = round(pub * 1e3, 3)
             out["%s_vs_published" % transform] = round(pub * 1e3 / ms, 1)
