@@ -27,12 +27,14 @@ struct FtamdGrid {
     int plan_sphv;  // column classes of the plan: 0 the sphere's, 1 the vector field's, 3 all cosine (annulus)
     int disk;     // the disk's plans: DCT-IV (REDFT11) on the sine class
     int nsin;     // sine-class columns (disk)
+    int pair;     // phi DFTs of two real rows as one complex DFT (N even, not the disk)
     hipfftHandle theta, phi, iv;
+    hipfftHandle phi_r;  // the real phi DFTs of a paired plan, made on first use (X not 16-byte aligned)
     double2 *tw;  // e^{i pi k / 2N}, k < N
     double2 *tw4; // e^{i pi (2k+1) / 4N}, k < N (disk)
     double2 *Z;   // N x M complex: the columns' FFTs
     double2 *Z4;  // 2N x nsin complex: the disk's sine-class columns, zero-padded (disk)
-    double2 *Y;   // N x (M/2+1) complex: the rows' half spectra
+    double2 *Y;   // N x (M/2+1) complex: the rows' half spectra (paired: N/2 x M full spectra)
     double *w;    // the annulus's radial weights (N), NULL otherwise
 };
 
@@ -82,18 +84,24 @@ __global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, cons
 
 // After the inverse FFT: y_n of every column, signs of the sine class, the
 // scalings a (all) and b (column 0 again), into Y as data_r2c lays it out.
-__global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
-                      double b, int skip_sin, const double *__restrict__ w, int wmode) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (n >= N || j >= M) return;
-    if (skip_sin && !cos_class(j, plan_sphv)) return;
+__device__ __forceinline__ double post3_value(const double2 *__restrict__ Z, int N, int n, int j, int plan_sphv,
+                                              double a, double b, const double *__restrict__ w, int wmode) {
     const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
     double y = Z[(long long)N * j + src].x;
     if (!cos_class(j, plan_sphv) && (n & 1)) y = -y;
     y = weigh(y, j, n, w, wmode);
     y *= a;
     if (j == 0) y *= b;
+    return y;
+}
+
+__global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
+                      double b, int skip_sin, const double *__restrict__ w, int wmode) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (n >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
+    const double y = post3_value(Z, N, n, j, plan_sphv, a, b, w, wmode);
     double *Yd = (double *)Y;
     if (j == 0) {
         Y[n] = make_double2(y, 0.0);
@@ -103,8 +111,39 @@ __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, in
     }
 }
 
+// The same for rows paired into one complex DFT (N even): rows 2p and 2p+1
+// have half spectra Y0, Y1; the pair's full spectrum is Zp[q] = Y0[q] + i
+// Y1[q], Zp[M-q] = conj(Y0[q]) + i conj(Y1[q]), at Yp[p + q N/2], whose
+// inverse DFT is row 2p in its real and row 2p+1 in its imaginary parts: X
+// itself read as N/2 x M complex.  A thread per (p, q <= M/2).
+__global__ void post3_pair(const double2 *__restrict__ Z, double2 *__restrict__ Yp, int N, int M, int plan_sphv,
+                           double a, double b, const double *__restrict__ w, int wmode) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    const int P = N >> 1;
+    if (p >= P || 2 * q > M) return;
+    const int e0 = 2 * p, e1 = 2 * p + 1;
+    // real parts from column 2q, imaginary parts (data_r2c: minus the value) from column 2q - 1
+    const double r0 = post3_value(Z, N, e0, 2 * q, plan_sphv, a, b, w, wmode);
+    const double r1 = post3_value(Z, N, e1, 2 * q, plan_sphv, a, b, w, wmode);
+    const double i0 = q > 0 ? -post3_value(Z, N, e0, 2 * q - 1, plan_sphv, a, b, w, wmode) : 0.0;
+    const double i1 = q > 0 ? -post3_value(Z, N, e1, 2 * q - 1, plan_sphv, a, b, w, wmode) : 0.0;
+    Yp[p + (long long)q * P] = make_double2(r0 - i1, i0 + r1);
+    if (q > 0) Yp[p + (long long)(M - q) * P] = make_double2(r0 + i1, r1 - i0);
+}
+
 // Pipeline with the row DFT first (analysis plans): data_c2r, the scalings,
 // the sine class's alternating signs, Makhoul's reordering into Z.
+__device__ __forceinline__ void pre2_store(double x, double2 *__restrict__ Z, int N, int n, int j, int plan_sphv,
+                                           double a, double b, const double *__restrict__ w, int wmode) {
+    x *= a;
+    if (j == 0) x *= b;
+    x = weigh(x, j, n, w, wmode);
+    if (!cos_class(j, plan_sphv) && (n & 1)) x = -x;
+    const int dst = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    Z[(long long)N * j + dst] = make_double2(x, 0.0);
+}
+
 __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int N, int M, int plan_sphv, double a,
                      double b, int skip_sin, const double *__restrict__ w, int wmode) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,12 +159,30 @@ __global__ void pre2(const double2 *__restrict__ Y, double2 *__restrict__ Z, int
         const double v = Yd[2 * ((long long)N * q + n) + ((j & 1) ? 1 : 0)];
         x = (j & 1) ? -v : v;
     }
-    x *= a;
-    if (j == 0) x *= b;
-    x = weigh(x, j, n, w, wmode);
-    if (!cos_class(j, plan_sphv) && (n & 1)) x = -x;
-    const int dst = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
-    Z[(long long)N * j + dst] = make_double2(x, 0.0);
+    pre2_store(x, Z, N, n, j, plan_sphv, a, b, w, wmode);
+}
+
+// pre2 from the paired rows' spectrum Zp (the forward DFT of X read as N/2 x
+// M complex): Y0[q] = (Zp[q] + conj(Zp[M-q])) / 2, Y1[q] = (Zp[q] -
+// conj(Zp[M-q])) / 2i.
+__global__ void pre2_pair(const double2 *__restrict__ Yp, double2 *__restrict__ Z, int N, int M, int plan_sphv,
+                          double a, double b, const double *__restrict__ w, int wmode) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (n >= N || j >= M) return;
+    const int P = N >> 1, p = n >> 1, q = (j + 1) >> 1;
+    const double2 zk = Yp[p + (long long)q * P], zm = Yp[p + (long long)(q == 0 ? 0 : M - q) * P];
+    double re, im;
+    if ((n & 1) == 0) {
+        re = 0.5 * (zk.x + zm.x);
+        im = 0.5 * (zk.y - zm.y);
+    } else {
+        re = 0.5 * (zk.y + zm.y);
+        im = 0.5 * (zm.x - zk.x);
+    }
+    // data_c2r: column 2q the real part, column 2q - 1 minus the imaginary part
+    const double x = (j & 1) ? -im : re;
+    pre2_store(x, Z, N, n, j, plan_sphv, a, b, w, wmode);
 }
 
 // After the forward FFT: the DCT-II (cosine class) or DST-II (sine class) of
@@ -328,6 +385,19 @@ __global__ void axis_post2(const double2 *__restrict__ Z, double *__restrict__ X
 
 static int fft_ok(hipfftResult r) { return r == HIPFFT_SUCCESS ? FTAMD_OK : FTAMD_EDEVICE; }
 
+static int phi_plan_real(FtamdGrid *g, hipfftHandle *h) {
+    const int N = g->N, M = g->M;
+    int n2[1] = {M}, nc[1] = {M / 2 + 1};
+    if (g->type3) return fft_ok(hipfftPlanMany(h, 1, n2, nc, N, 1, n2, N, 1, HIPFFT_Z2D, N));
+    return fft_ok(hipfftPlanMany(h, 1, n2, n2, N, 1, nc, N, 1, HIPFFT_D2Z, N));
+}
+
+static int phi_plan_pair(FtamdGrid *g) {
+    const int P = g->N / 2;
+    int n2[1] = {g->M};
+    return fft_ok(hipfftPlanMany(&g->phi, 1, n2, n2, P, 1, n2, P, 1, HIPFFT_Z2Z, P));
+}
+
 extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const double *w, FtamdGrid **out) {
     *out = NULL;
     /* columns are the launch grid's y dimension (at most 65535) */
@@ -341,7 +411,8 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     g->plan_sphv = plan_mode == 1 ? 1 : plan_mode == 3 ? 3 : 0;
     g->disk = plan_mode == 2;
     g->nsin = g->disk ? (M + 2) / 4 + (M + 1) / 4 : 0;  /* columns 1 and 2 mod 4 (fftw.c:427-431) */
-    g->theta = g->phi = g->iv = 0;
+    g->pair = N % 2 == 0 && !g->disk;
+    g->theta = g->phi = g->iv = g->phi_r = 0;
     int rc = FTAMD_OK;
     const size_t zbytes = sizeof(double2) * (size_t)N * M, ybytes = sizeof(double2) * (size_t)N * (M / 2 + 1);
     double2 *tw = (double2 *)malloc(sizeof(double2) * (size_t)N);
@@ -381,14 +452,9 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     // theta: M contiguous complex transforms of length N
     int n1[1] = {N};
     if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, M));
-    // phi: N real transforms of length M, element i of row-transform e at i + e*N
-    int n2[1] = {M}, nc[1] = {M / 2 + 1};
-    if (rc == FTAMD_OK) {
-        if (type3)
-            rc = fft_ok(hipfftPlanMany(&g->phi, 1, n2, nc, N, 1, n2, N, 1, HIPFFT_Z2D, N));
-        else
-            rc = fft_ok(hipfftPlanMany(&g->phi, 1, n2, n2, N, 1, nc, N, 1, HIPFFT_D2Z, N));
-    }
+    // phi: N real transforms of length M, element i of row-transform e at e + i*N; paired, N/2
+    // complex ones, element i of pair p at p + i*N/2
+    if (rc == FTAMD_OK) rc = g->pair ? phi_plan_pair(g) : phi_plan_real(g, &g->phi);
     if (rc != FTAMD_OK) {
         ftamd_grid_destroy(g);
         return rc;
@@ -402,6 +468,7 @@ extern "C" void ftamd_grid_destroy(FtamdGrid *g) {
     hipDeviceSynchronize();
     if (g->theta) hipfftDestroy(g->theta);
     if (g->phi) hipfftDestroy(g->phi);
+    if (g->phi_r) hipfftDestroy(g->phi_r);
     if (g->iv) hipfftDestroy(g->iv);
     hipFree(g->tw);
     hipFree(g->tw4);
@@ -418,8 +485,16 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     const int N = g->N, M = g->M;
     const hipStream_t st = (hipStream_t)stream;
     const dim3 block(256), grid((N + 255) / 256, M);
+    // the paired phi DFTs read or write X as complex: 16-byte aligned X only, else the real DFTs
+    const int pair = g->pair && ((uintptr_t)X & 15) == 0;
+    if (g->pair && !pair && !g->phi_r) {
+        const int r = phi_plan_real(g, &g->phi_r);
+        if (r != FTAMD_OK) return r;
+    }
+    const hipfftHandle phi = g->pair && !pair ? g->phi_r : g->phi;
+    const dim3 gridp((N / 2 + 255) / 256, M / 2 + 1);
     int rc = fft_ok(hipfftSetStream(g->theta, st));
-    if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(g->phi, st));
+    if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(phi, st));
     if (rc == FTAMD_OK && g->iv) rc = fft_ok(hipfftSetStream(g->iv, st));
     if (rc != FTAMD_OK) return rc;
     const int iv = g->disk && g->nsin > 0;
@@ -450,18 +525,31 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
         if (rc != FTAMD_OK) return rc;
+        if (pair) {
+            hipLaunchKernelGGL(ftgrid::post3_pair, gridp, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, g->w,
+                               g->w ? wmode : 0);
+            if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+            return fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)g->Y, (hipfftDoubleComplex *)X, HIPFFT_BACKWARD));
+        }
         hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, iv, g->w,
                            g->w ? wmode : 0);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         if (iv && (rc = iv_from_x()) != FTAMD_OK) return rc;
-        return fft_ok(hipfftExecZ2D(g->phi, (hipfftDoubleComplex *)g->Y, X));
+        return fft_ok(hipfftExecZ2D(phi, (hipfftDoubleComplex *)g->Y, X));
     }
     // r2c, data_c2r, scalings, DCT-II / DST-II (the disk: DCT-IV on the sine
     // class), fix-ups f0 and f1
-    rc = fft_ok(hipfftExecD2Z(g->phi, X, (hipfftDoubleComplex *)g->Y));
-    if (rc != FTAMD_OK) return rc;
-    hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv, g->w,
-                       g->w ? wmode : 0);
+    if (pair) {
+        rc = fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)X, (hipfftDoubleComplex *)g->Y, HIPFFT_FORWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::pre2_pair, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, g->w,
+                           g->w ? wmode : 0);
+    } else {
+        rc = fft_ok(hipfftExecD2Z(phi, X, (hipfftDoubleComplex *)g->Y));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv, g->w,
+                           g->w ? wmode : 0);
+    }
     if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
     if (iv && (rc = iv_from_y()) != FTAMD_OK) return rc;
     rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_FORWARD));

@@ -63,6 +63,41 @@ def test_device_resident_and_wrong_plans(ft):
         ft.execute_sph_analysis("N", S, X.copy(order="F"))    # a synthesis plan
 
 
+@pytest.mark.parametrize("which", ["sph_synthesis", "sph_analysis", "sphv_synthesis", "annulus_analysis"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_paired_and_real_phi_dfts_agree(ft, which, trans):
+    """With N even the phi DFTs of rows 2p and 2p+1 run as one complex DFT
+    over X read as complex; a device array that is not 16-byte aligned takes
+    the real DFTs instead.  Both against the oracle."""
+    import torch
+    N, M = 64, 127
+    geo, kind = which.split("_")
+    syn = (kind == "synthesis") == (trans == "N")
+    if geo == "annulus":
+        rho = 0.5
+        P = (ft.plan_annulus_synthesis if syn else ft.plan_annulus_analysis)(N, M, rho)
+    else:
+        rho = None
+        P = getattr(ft, "plan_%s_%s" % (geo, "synthesis" if syn else "analysis"))(N, M)
+    X = np.asfortranarray(np.random.default_rng(77).standard_normal((N, M)))
+    if geo == "annulus":
+        ref = O.grid(which, trans, syn, False, X, rho=rho)
+    else:
+        ref = O.grid(which, trans, syn, geo == "sphv", X)
+    flat = np.ascontiguousarray(X.T).reshape(-1)
+    T1 = torch.from_numpy(flat.copy()).cuda().view(M, N)
+    buf = torch.zeros(N * M + 1, dtype=torch.float64, device="cuda")
+    T2 = buf[1:].view(M, N)
+    T2.copy_(T1)
+    assert T1.data_ptr() % 16 == 0 and T2.data_ptr() % 16 == 8
+    _say("grid", which, trans, "aligned and unaligned X, N=%d M=%d" % (N, M))
+    ft.gpu_execute_sph_grid(which, trans, P, T1)
+    ft.gpu_execute_sph_grid(which, trans, P, T2)
+    ft.synchronize()
+    g1, g2 = T1.cpu().numpy().T, T2.cpu().numpy().T
+    assert O.rel2(g1, ref) <= 2e-14 and O.rel2(g2, ref) <= 2e-14, (O.rel2(g1, ref), O.rel2(g2, ref))
+
+
 @pytest.mark.parametrize("n", [64, 1024])
 def test_spherical_harmonic_transform_round_trip(ft, n):
     """test_fftw.c:44-67 on the device: sph2fourier, synthesis, analysis,
