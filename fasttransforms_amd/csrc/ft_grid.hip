@@ -12,8 +12,9 @@
 // y_{2m} = Re v_m, y_{2m+1} = Re v_{N-1-m}.  The DSTs are DCTs of the
 // reversed input with alternating output signs (DST-III) or of the
 // sign-alternated input with reversed output (DST-II).  The phi DFTs are
-// hipFFT's strided real transforms (row i is the transform at element i,
-// stride N).
+// hipFFT's strided transforms (row i is the transform at element i, stride
+// N); with N even, rows 2p and 2p+1 share one complex DFT over X read as
+// N/2 x M complex.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
@@ -27,14 +28,15 @@ struct FtamdGrid {
     int plan_sphv;  // column classes of the plan: 0 the sphere's, 1 the vector field's, 3 all cosine (annulus)
     int disk;     // the disk's plans: DCT-IV (REDFT11) on the sine class
     int nsin;     // sine-class columns (disk)
-    int pair;     // phi DFTs of two real rows as one complex DFT (N even, not the disk)
+    int pair;     // phi DFTs of two real rows as one complex DFT (N even)
     hipfftHandle theta, phi, iv;
     hipfftHandle phi_r;  // the real phi DFTs of a paired plan, made on first use (X not 16-byte aligned)
     double2 *tw;  // e^{i pi k / 2N}, k < N
     double2 *tw4; // e^{i pi (2k+1) / 4N}, k < N (disk)
     double2 *Z;   // N x M complex: the columns' FFTs
     double2 *Z4;  // 2N x nsin complex: the disk's sine-class columns, zero-padded (disk)
-    double2 *Y;   // N x (M/2+1) complex: the rows' half spectra (paired: N/2 x M full spectra)
+    double2 *Y;   // N x (M/2+1) complex: the rows' half spectra (paired, not the disk: N/2 x M full spectra)
+    double2 *Yp;  // the disk's paired plans: N/2 x M full spectra of the row pairs
     double *w;    // the annulus's radial weights (N), NULL otherwise
 };
 
@@ -130,6 +132,31 @@ __global__ void post3_pair(const double2 *__restrict__ Z, double2 *__restrict__ 
     const double i1 = q > 0 ? -post3_value(Z, N, e1, 2 * q - 1, plan_sphv, a, b, w, wmode) : 0.0;
     Yp[p + (long long)q * P] = make_double2(r0 - i1, i0 + r1);
     if (q > 0) Yp[p + (long long)(M - q) * P] = make_double2(r0 + i1, r1 - i0);
+}
+
+// The disk's paired plans, whose sine class reaches Y through the DCT-IV
+// kernels: the half spectra Y of rows 2p and 2p+1 into the pair's full
+// spectrum (as post3_pair; the imaginary part of each row's mean is taken
+// as 0, as a c2r DFT takes it), and back (as pre2_pair).
+__global__ void pack_pairs(const double2 *__restrict__ Y, double2 *__restrict__ Yp, int N, int M) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    const int P = N >> 1;
+    if (p >= P || 2 * q > M) return;
+    const double2 y0 = Y[2 * p + (long long)q * N], y1 = Y[2 * p + 1 + (long long)q * N];
+    const double i0 = q > 0 ? y0.y : 0.0, i1 = q > 0 ? y1.y : 0.0;
+    Yp[p + (long long)q * P] = make_double2(y0.x - i1, i0 + y1.x);
+    if (q > 0) Yp[p + (long long)(M - q) * P] = make_double2(y0.x + i1, y1.x - i0);
+}
+
+__global__ void unpack_pairs(const double2 *__restrict__ Yp, double2 *__restrict__ Y, int N, int M) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (n >= N || 2 * q > M) return;
+    const int P = N >> 1, p = n >> 1;
+    const double2 zk = Yp[p + (long long)q * P], zm = Yp[p + (long long)(q == 0 ? 0 : M - q) * P];
+    Y[n + (long long)q * N] = (n & 1) == 0 ? make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y))
+                                           : make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
 }
 
 // Pipeline with the row DFT first (analysis plans): data_c2r, the scalings,
@@ -411,7 +438,7 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     g->plan_sphv = plan_mode == 1 ? 1 : plan_mode == 3 ? 3 : 0;
     g->disk = plan_mode == 2;
     g->nsin = g->disk ? (M + 2) / 4 + (M + 1) / 4 : 0;  /* columns 1 and 2 mod 4 (fftw.c:427-431) */
-    g->pair = N % 2 == 0 && !g->disk;
+    g->pair = N % 2 == 0;
     g->theta = g->phi = g->iv = g->phi_r = 0;
     int rc = FTAMD_OK;
     const size_t zbytes = sizeof(double2) * (size_t)N * M, ybytes = sizeof(double2) * (size_t)N * (M / 2 + 1);
@@ -424,6 +451,9 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     if (rc == FTAMD_OK && hipMalloc((void **)&g->tw, sizeof(double2) * (size_t)N) != hipSuccess) rc = FTAMD_ENOMEM;
     if (rc == FTAMD_OK && hipMalloc((void **)&g->Z, zbytes) != hipSuccess) rc = FTAMD_ENOMEM;
     if (rc == FTAMD_OK && hipMalloc((void **)&g->Y, ybytes) != hipSuccess) rc = FTAMD_ENOMEM;
+    if (rc == FTAMD_OK && g->pair && g->disk &&
+        hipMalloc((void **)&g->Yp, sizeof(double2) * (size_t)(N / 2) * M) != hipSuccess)
+        rc = FTAMD_ENOMEM;
     if (rc == FTAMD_OK && hipMemcpy(g->tw, tw, sizeof(double2) * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
         rc = FTAMD_EDEVICE;
     free(tw);
@@ -476,6 +506,7 @@ extern "C" void ftamd_grid_destroy(FtamdGrid *g) {
     hipFree(g->Z4);
     hipFree(g->w);
     hipFree(g->Y);
+    hipFree(g->Yp);
     free(g);
 }
 
@@ -525,7 +556,7 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
         if (rc != FTAMD_OK) return rc;
-        if (pair) {
+        if (pair && !g->disk) {
             hipLaunchKernelGGL(ftgrid::post3_pair, gridp, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, g->w,
                                g->w ? wmode : 0);
             if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
@@ -535,14 +566,26 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
                            g->w ? wmode : 0);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         if (iv && (rc = iv_from_x()) != FTAMD_OK) return rc;
+        if (pair) {  // the disk
+            hipLaunchKernelGGL(ftgrid::pack_pairs, gridp, block, 0, st, g->Y, g->Yp, N, M);
+            if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+            return fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)g->Yp, (hipfftDoubleComplex *)X, HIPFFT_BACKWARD));
+        }
         return fft_ok(hipfftExecZ2D(phi, (hipfftDoubleComplex *)g->Y, X));
     }
     // r2c, data_c2r, scalings, DCT-II / DST-II (the disk: DCT-IV on the sine
     // class), fix-ups f0 and f1
-    if (pair) {
+    if (pair && !g->disk) {
         rc = fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)X, (hipfftDoubleComplex *)g->Y, HIPFFT_FORWARD));
         if (rc != FTAMD_OK) return rc;
         hipLaunchKernelGGL(ftgrid::pre2_pair, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, g->w,
+                           g->w ? wmode : 0);
+    } else if (pair) {  // the disk: unpacked into half spectra for pre2 and the DCT-IV
+        rc = fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)X, (hipfftDoubleComplex *)g->Yp, HIPFFT_FORWARD));
+        if (rc != FTAMD_OK) return rc;
+        hipLaunchKernelGGL(ftgrid::unpack_pairs, dim3((N + 255) / 256, M / 2 + 1), block, 0, st, g->Yp, g->Y, N, M);
+        if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
+        hipLaunchKernelGGL(ftgrid::pre2, grid, block, 0, st, g->Y, g->Z, N, M, g->plan_sphv, a, b, iv, g->w,
                            g->w ? wmode : 0);
     } else {
         rc = fft_ok(hipfftExecD2Z(phi, X, (hipfftDoubleComplex *)g->Y));

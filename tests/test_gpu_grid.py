@@ -63,7 +63,8 @@ def test_device_resident_and_wrong_plans(ft):
         ft.execute_sph_analysis("N", S, X.copy(order="F"))    # a synthesis plan
 
 
-@pytest.mark.parametrize("which", ["sph_synthesis", "sph_analysis", "sphv_synthesis", "annulus_analysis"])
+@pytest.mark.parametrize("which", ["sph_synthesis", "sph_analysis", "sphv_synthesis", "annulus_analysis",
+                                   "disk_synthesis", "disk_analysis"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_paired_and_real_phi_dfts_agree(ft, which, trans):
     """With N even the phi DFTs of rows 2p and 2p+1 run as one complex DFT
