@@ -94,28 +94,20 @@ __device__ void sweep(double2 *col, long long rs, const double2 *__restrict__ t,
             for (int q = Q - 1; q >= 0; --q) {
                 const int p = tt - 2 * q;
                 const double2 x = q == 0 ? xb[i] : em[q - 1];  // what q-1 emitted at the previous step
-                if (p == 0) {
-                    cy[q] = x;
-                } else if (p >= 1 && p <= R) {
-                    // hi2lo: the pair (r, r+1), x row r, the carry row r+1;
-                    // lo2hi: the pair (r, r+1), the carry row r, x row r+1
-                    const int r = pair_of(tt, q);
-                    if (r <= top[q]) {
-                        const double S = sg * scb[i][q].x, C = scb[i][q].y;
-                        if (DIR == 0) {
-                            em[q] = rot_y(C, S, x, cy[q]);
-                            cy[q] = rot_x(C, S, x, cy[q]);
-                        } else {
-                            em[q] = rot_x(C, S, cy[q], x);
-                            cy[q] = rot_y(C, S, cy[q], x);
-                        }
-                    } else {
-                        em[q] = cy[q];
-                        cy[q] = x;
-                    }
-                } else if (p == R + 1) {
-                    em[q] = cy[q];
-                }
+                // hi2lo: the pair (r, r+1), x row r, the carry row r+1; lo2hi: the pair (r, r+1), the
+                // carry row r, x row r+1.  The rotation is computed at every step and kept by selects
+                // (the lanes' ranges differ; branches here would make the compiler wait on the
+                // prefetched loads): position 0 takes x as the carry, 1 .. R rotate where the pair
+                // is in the sequence's range and pass x on otherwise, R + 1 emits the carry.
+                const bool mid = p >= 1 && p <= R;
+                const bool act = mid && pair_of(tt, q) <= top[q];
+                const double S = sg * scb[i][q].x, C = scb[i][q].y;
+                const double2 e1 = DIR == 0 ? rot_y(C, S, x, cy[q]) : rot_x(C, S, cy[q], x);
+                const double2 c1 = DIR == 0 ? rot_x(C, S, x, cy[q]) : rot_y(C, S, cy[q], x);
+                const double2 en = act ? e1 : ((mid || p == R + 1) ? cy[q] : em[q]);
+                const double2 cn = act ? c1 : ((mid || p == 0) ? x : cy[q]);
+                em[q] = en;
+                cy[q] = cn;
             }
             ob[i] = em[Q - 1];
         }
