@@ -2,7 +2,9 @@
 -lfasttransforms_amd, as the reference's callers are: it builds and links
 here (every declared entry point resolves), and on the GPU it runs the
 reference's own round-trip and single-column accuracy checks
-(test_drivers.c:37-57, test_rotations.c:25-39)."""
+(test_drivers.c:37-57, test_rotations.c:25-39) and its harmonic-transform
+round trips (test_drivers.c:226-248, 700-720, 937-957, 1128-1156,
+1234-1254)."""
 import os
 import subprocess
 
