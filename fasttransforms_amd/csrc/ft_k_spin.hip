@@ -127,6 +127,68 @@ __device__ void sweep(double2 *col, long long rs, const double2 *__restrict__ t,
     }
 }
 
+// One sequence (|s| = 1, or the last of a column): the sweep without the
+// systolic bookkeeping.  The pairs are r = R-1 .. 0 (hi2lo) or 0 .. R-1
+// (lo2hi), every one of them in the sequence's range, so a step is one
+// rotation of the fresh row against the carried one, emitting a finished
+// row.  Full blocks of KB steps run without a branch, the next block's rows
+// and (s, c) loaded at clamped indices while this one is computed; the
+// remaining R mod KB steps and the last row follow.
+template <int DIR>
+__device__ void sweep1(double2 *col, long long rs, const double2 *__restrict__ tq, int R, double sg) {
+    constexpr int KB = 8;
+    // step p = 1 .. R: the fresh row and the pair, clamped into range past R
+    auto frow = [&](int p) {
+        const int pc = p > R ? R : p;
+        return DIR == 0 ? R - pc : pc;
+    };
+    auto pair = [&](int p) {
+        const int pc = p > R ? R : p;
+        return DIR == 0 ? R - pc : pc - 1;
+    };
+    auto load = [&](int p0, double2 (&x)[KB], double2 (&sc)[KB]) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            x[k] = col[(long long)frow(p0 + k) * rs];
+            sc[k] = tq[pair(p0 + k)];
+        }
+    };
+    // hi2lo: x row r, the carry row r+1, emits row r+1; lo2hi: the carry row r, x row r+1, emits row r
+    auto step = [&](double2 x, double2 sc, double2 &cy) {
+        const double S = sg * sc.x, C = sc.y;
+        double2 em;
+        if (DIR == 0) {
+            em = rot_y(C, S, x, cy);
+            cy = rot_x(C, S, x, cy);
+        } else {
+            em = rot_x(C, S, cy, x);
+            cy = rot_y(C, S, cy, x);
+        }
+        return em;
+    };
+    auto out_row = [&](int p) { return (long long)(DIR == 0 ? R - p + 1 : p - 1) * rs; };
+    double2 cy = col[(long long)(DIR == 0 ? R : 0) * rs];
+    const int nb = R / KB;
+    double2 xb[KB], sb[KB], xn[KB], sn[KB];
+    load(1, xb, sb);
+    for (int b = 0; b < nb; ++b) {
+        const int p0 = 1 + b * KB;
+        load(p0 + KB, xn, sn);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) col[out_row(p0 + k)] = step(xb[k], sb[k], cy);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            xb[k] = xn[k];
+            sb[k] = sn[k];
+        }
+    }
+    const int p0 = 1 + nb * KB, rem = R - nb * KB;
+#pragma unroll
+    for (int k = 0; k < KB; ++k)
+        if (k < rem) col[out_row(p0 + k)] = step(xb[k], sb[k], cy);
+    col[(long long)(DIR == 0 ? 0 : R) * rs] = cy;
+}
+
 template <int Q>
 __global__ void __launch_bounds__(64) stage1_kernel(const double2 *__restrict__ tab, int n, int s, double2 *A, int c0,
                                                     int count, long long col_stride, int row_stride, int use_fixed,
@@ -147,7 +209,13 @@ __global__ void __launch_bounds__(64) stage1_kernel(const double2 *__restrict__ 
     const double sg = (dir == 0) == rot ? 1.0 : -1.0;
     double2 *col = A + col_stride * i;
     const double2 *t = tab + (long long)d * as * n;
-    if (dir == 0) {
+    if (Q == 1) {  // one sequence a sweep: R = n-1-d-j0, (s, c) of sequence j0
+        if (dir == 0) {
+            for (int j0 = nrot - 1; j0 >= 0; --j0) sweep1<0>(col, row_stride, t + (long long)j0 * n, n - 1 - d - j0, sg);
+        } else {
+            for (int j0 = 0; j0 < nrot; ++j0) sweep1<1>(col, row_stride, t + (long long)j0 * n, n - 1 - d - j0, sg);
+        }
+    } else if (dir == 0) {
         // j = nseq-1 .. 0; those above nrot-1 have no pairs
         for (int j0 = nrot - 1; j0 >= 0; j0 -= Q) sweep<Q, 0>(col, row_stride, t, n, d, j0, j0 + 1 < Q ? j0 + 1 : Q, sg);
     } else {
