@@ -16,7 +16,8 @@
 // sequence's range pass through it unchanged.  Fresh rows and the (s, c) of
 // every sequence are loaded, and the last sequence's rows stored, in blocks of
 // steps, the next block's loads issued before the current block's arithmetic.
-// Columns with more than Q sequences take several sweeps.
+// Columns with more than Q sequences take several sweeps.  A lone sequence
+// (Q = 1) takes sweep1, which has none of the systolic bookkeeping.
 //
 // The transpose of the rotation (s, c) is the rotation (-s, c), bit for bit
 // in the fused forms: fma(c, x, (-s)*y) = fma(c, x, -(s*y)) and
@@ -237,18 +238,15 @@ extern "C" int ftamd_hip_spin_stage1(const double *tab2, int n, int s, double *A
     const hipStream_t st = (hipStream_t)stream;
     const double2 *tab = (const double2 *)tab2;
     double2 *a = (double2 *)A;
-    // sequences per sweep: the fewest of 1, 2, 4, 8 that hold |s| (more take several sweeps)
-    if (as <= 1)
-        hipLaunchKernelGGL(ftspin::stage1_kernel<1>, grid, dim3(block), 0, st, tab, n, s, a, c0, count, col_stride,
-                           row_stride, use_fixed, fixed_m, dir);
-    else if (as <= 2)
-        hipLaunchKernelGGL(ftspin::stage1_kernel<2>, grid, dim3(block), 0, st, tab, n, s, a, c0, count, col_stride,
-                           row_stride, use_fixed, fixed_m, dir);
-    else if (as <= 4)
+    // one sequence a sweep (sweep1), except lo2hi with |s| = 3, 4, where the four-sequence systolic
+    // sweep is faster (measured transforms at n = 1024, s = 4: lo2hi 0.72 ms with it against 0.78
+    // ms; hi2lo 0.66 ms one sequence at a time against 0.69; s = 9 hi2lo: 1.1 ms against 2.1 ms in
+    // sweeps of eight sequences)
+    if (dir == 1 && as >= 3 && as <= 4)
         hipLaunchKernelGGL(ftspin::stage1_kernel<4>, grid, dim3(block), 0, st, tab, n, s, a, c0, count, col_stride,
                            row_stride, use_fixed, fixed_m, dir);
     else
-        hipLaunchKernelGGL(ftspin::stage1_kernel<8>, grid, dim3(block), 0, st, tab, n, s, a, c0, count, col_stride,
+        hipLaunchKernelGGL(ftspin::stage1_kernel<1>, grid, dim3(block), 0, st, tab, n, s, a, c0, count, col_stride,
                            row_stride, use_fixed, fixed_m, dir);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FTAMD_OK : FTAMD_EDEVICE;
