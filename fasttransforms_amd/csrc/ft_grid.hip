@@ -29,6 +29,7 @@ struct FtamdGrid {
     int disk;     // the disk's plans: DCT-IV (REDFT11) on the sine class
     int nsin;     // sine-class columns (disk)
     int pair;     // phi DFTs of two real rows as one complex DFT (N even)
+    int tpair;    // theta FFTs of two columns as one (synthesis, not the disk): column j in part j & 1 of line j / 2
     hipfftHandle theta, phi, iv;
     hipfftHandle phi_r;  // the real phi DFTs of a paired plan, made on first use (X not 16-byte aligned)
     double2 *tw;  // e^{i pi k / 2N}, k < N
@@ -62,12 +63,9 @@ __device__ __forceinline__ double weigh(double v, int j, int n, const double *w,
 // Pipeline with DCT-III first (synthesis plans): fix-ups, DCT-III / DST-III
 // of every column into Z (pre-twiddled for the inverse FFT).  skip_sin: the
 // sine class is the disk's DCT-IV, run by the iv kernels.
-__global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw, int N,
-                     int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin, int skip_sin) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (k >= N || j >= M) return;
-    if (skip_sin && !cos_class(j, plan_sphv)) return;
+__device__ __forceinline__ double2 pre3_value(const double *__restrict__ X, const double2 *__restrict__ tw, int N,
+                                               int k, int j, int plan_sphv, int exec_sphv, double f_row0_cos,
+                                               double f_last_sin) {
     const double *x = X + (long long)N * j;
     const bool tcos = cos_class(j, plan_sphv), fcos = cos_class(j, exec_sphv);
     // entry e of the column after the executor's fix-ups
@@ -81,15 +79,38 @@ __global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, cons
     auto in = [&](int e) { return tcos ? val(e) : val(N - 1 - e); };
     const double re = in(k), im = k == 0 ? 0.0 : -in(N - k);
     const double2 t = tw[k];
-    Z[(long long)N * j + k] = make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
+    return make_double2(t.x * re - t.y * im, t.y * re + t.x * im);
+}
+
+__global__ void pre3(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw, int N,
+                     int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin, int skip_sin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= N || j >= M) return;
+    if (skip_sin && !cos_class(j, plan_sphv)) return;
+    Z[(long long)N * j + k] = pre3_value(X, tw, N, k, j, plan_sphv, exec_sphv, f_row0_cos, f_last_sin);
+}
+
+// Columns 2c and 2c+1 share one inverse FFT: their inputs Z0, Z1 (whose
+// transforms are real) go in as Z0 + i Z1, so the transform holds column 2c
+// in its real and column 2c+1 in its imaginary part.  A thread per (k, c).
+__global__ void pre3_pair(const double *__restrict__ X, double2 *__restrict__ Z, const double2 *__restrict__ tw,
+                          int N, int M, int plan_sphv, int exec_sphv, double f_row0_cos, double f_last_sin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= N || 2 * c >= M) return;
+    const double2 z0 = pre3_value(X, tw, N, k, 2 * c, plan_sphv, exec_sphv, f_row0_cos, f_last_sin);
+    const double2 z1 = 2 * c + 1 < M ? pre3_value(X, tw, N, k, 2 * c + 1, plan_sphv, exec_sphv, f_row0_cos, f_last_sin)
+                                     : make_double2(0.0, 0.0);
+    Z[(long long)N * c + k] = make_double2(z0.x - z1.y, z0.y + z1.x);
 }
 
 // After the inverse FFT: y_n of every column, signs of the sine class, the
 // scalings a (all) and b (column 0 again), into Y as data_r2c lays it out.
 __device__ __forceinline__ double post3_value(const double2 *__restrict__ Z, int N, int n, int j, int plan_sphv,
-                                              double a, double b, const double *__restrict__ w, int wmode) {
+                                              double a, double b, const double *__restrict__ w, int wmode, int tp) {
     const int src = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
-    double y = Z[(long long)N * j + src].x;
+    double y = tp ? ((const double *)Z)[2 * ((long long)N * (j >> 1) + src) + (j & 1)] : Z[(long long)N * j + src].x;
     if (!cos_class(j, plan_sphv) && (n & 1)) y = -y;
     y = weigh(y, j, n, w, wmode);
     y *= a;
@@ -98,12 +119,12 @@ __device__ __forceinline__ double post3_value(const double2 *__restrict__ Z, int
 }
 
 __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N, int M, int plan_sphv, double a,
-                      double b, int skip_sin, const double *__restrict__ w, int wmode) {
+                      double b, int skip_sin, const double *__restrict__ w, int wmode, int tp) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (n >= N || j >= M) return;
     if (skip_sin && !cos_class(j, plan_sphv)) return;
-    const double y = post3_value(Z, N, n, j, plan_sphv, a, b, w, wmode);
+    const double y = post3_value(Z, N, n, j, plan_sphv, a, b, w, wmode, tp);
     double *Yd = (double *)Y;
     if (j == 0) {
         Y[n] = make_double2(y, 0.0);
@@ -119,17 +140,17 @@ __global__ void post3(const double2 *__restrict__ Z, double2 *__restrict__ Y, in
 // inverse DFT is row 2p in its real and row 2p+1 in its imaginary parts: X
 // itself read as N/2 x M complex.  A thread per (p, q <= M/2).
 __global__ void post3_pair(const double2 *__restrict__ Z, double2 *__restrict__ Yp, int N, int M, int plan_sphv,
-                           double a, double b, const double *__restrict__ w, int wmode) {
+                           double a, double b, const double *__restrict__ w, int wmode, int tp) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     const int P = N >> 1;
     if (p >= P || 2 * q > M) return;
     const int e0 = 2 * p, e1 = 2 * p + 1;
     // real parts from column 2q, imaginary parts (data_r2c: minus the value) from column 2q - 1
-    const double r0 = post3_value(Z, N, e0, 2 * q, plan_sphv, a, b, w, wmode);
-    const double r1 = post3_value(Z, N, e1, 2 * q, plan_sphv, a, b, w, wmode);
-    const double i0 = q > 0 ? -post3_value(Z, N, e0, 2 * q - 1, plan_sphv, a, b, w, wmode) : 0.0;
-    const double i1 = q > 0 ? -post3_value(Z, N, e1, 2 * q - 1, plan_sphv, a, b, w, wmode) : 0.0;
+    const double r0 = post3_value(Z, N, e0, 2 * q, plan_sphv, a, b, w, wmode, tp);
+    const double r1 = post3_value(Z, N, e1, 2 * q, plan_sphv, a, b, w, wmode, tp);
+    const double i0 = q > 0 ? -post3_value(Z, N, e0, 2 * q - 1, plan_sphv, a, b, w, wmode, tp) : 0.0;
+    const double i1 = q > 0 ? -post3_value(Z, N, e1, 2 * q - 1, plan_sphv, a, b, w, wmode, tp) : 0.0;
     Yp[p + (long long)q * P] = make_double2(r0 - i1, i0 + r1);
     if (q > 0) Yp[p + (long long)(M - q) * P] = make_double2(r0 + i1, r1 - i0);
 }
@@ -439,6 +460,9 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     g->disk = plan_mode == 2;
     g->nsin = g->disk ? (M + 2) / 4 + (M + 1) / 4 : 0;  /* columns 1 and 2 mod 4 (fftw.c:427-431) */
     g->pair = N % 2 == 0;
+    /* the synthesis plans only: splitting a paired forward FFT (analysis) costs the post kernel a
+     * second read per entry and measured slower (n = 1024: 0.108 against 0.104 ms) */
+    g->tpair = !g->disk && type3;
     g->theta = g->phi = g->iv = g->phi_r = 0;
     int rc = FTAMD_OK;
     const size_t zbytes = sizeof(double2) * (size_t)N * M, ybytes = sizeof(double2) * (size_t)N * (M / 2 + 1);
@@ -481,7 +505,8 @@ extern "C" int ftamd_grid_create(int N, int M, int type3, int plan_mode, const d
     if (rc == FTAMD_OK && g->disk && hipMemset(g->Z, 0, zbytes) != hipSuccess) rc = FTAMD_EDEVICE;
     // theta: M contiguous complex transforms of length N
     int n1[1] = {N};
-    if (rc == FTAMD_OK) rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, M));
+    if (rc == FTAMD_OK)
+        rc = fft_ok(hipfftPlanMany(&g->theta, 1, n1, n1, 1, N, n1, 1, N, HIPFFT_Z2Z, g->tpair ? (M + 1) / 2 : M));
     // phi: N real transforms of length M, element i of row-transform e at e + i*N; paired, N/2
     // complex ones, element i of pair p at p + i*N/2
     if (rc == FTAMD_OK) rc = g->pair ? phi_plan_pair(g) : phi_plan_real(g, &g->phi);
@@ -524,6 +549,7 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
     }
     const hipfftHandle phi = g->pair && !pair ? g->phi_r : g->phi;
     const dim3 gridp((N / 2 + 255) / 256, M / 2 + 1);
+    const int tp = g->tpair;
     int rc = fft_ok(hipfftSetStream(g->theta, st));
     if (rc == FTAMD_OK) rc = fft_ok(hipfftSetStream(phi, st));
     if (rc == FTAMD_OK && g->iv) rc = fft_ok(hipfftSetStream(g->iv, st));
@@ -552,18 +578,23 @@ extern "C" int ftamd_grid_run(FtamdGrid *g, double *X, int exec_sphv, double f0,
         // fix-ups f0 (row 0, cosine columns) and f1 (last row, sine columns),
         // DCT-III / DST-III (the disk: DCT-IV on the sine class), scalings,
         // data_r2c, c2r
-        hipLaunchKernelGGL(ftgrid::pre3, grid, block, 0, st, X, g->Z, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1, iv);
+        if (tp)
+            hipLaunchKernelGGL(ftgrid::pre3_pair, dim3((N + 255) / 256, (M + 1) / 2), block, 0, st, X, g->Z, g->tw, N,
+                               M, g->plan_sphv, exec_sphv, f0, f1);
+        else
+            hipLaunchKernelGGL(ftgrid::pre3, grid, block, 0, st, X, g->Z, g->tw, N, M, g->plan_sphv, exec_sphv, f0, f1,
+                               iv);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         rc = fft_ok(hipfftExecZ2Z(g->theta, (hipfftDoubleComplex *)g->Z, (hipfftDoubleComplex *)g->Z, HIPFFT_BACKWARD));
         if (rc != FTAMD_OK) return rc;
         if (pair && !g->disk) {
             hipLaunchKernelGGL(ftgrid::post3_pair, gridp, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, g->w,
-                               g->w ? wmode : 0);
+                               g->w ? wmode : 0, tp);
             if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
             return fft_ok(hipfftExecZ2Z(phi, (hipfftDoubleComplex *)g->Y, (hipfftDoubleComplex *)X, HIPFFT_BACKWARD));
         }
         hipLaunchKernelGGL(ftgrid::post3, grid, block, 0, st, g->Z, g->Y, N, M, g->plan_sphv, a, b, iv, g->w,
-                           g->w ? wmode : 0);
+                           g->w ? wmode : 0, tp);
         if (hipGetLastError() != hipSuccess) return FTAMD_EDEVICE;
         if (iv && (rc = iv_from_x()) != FTAMD_OK) return rc;
         if (pair) {  // the disk
