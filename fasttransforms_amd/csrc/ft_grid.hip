@@ -5,8 +5,8 @@
 // along phi (c2r / r2c), with the reference's scalings and the column layout
 // of its data_r2c / data_c2r (fftw.c:40-60).
 //
-// The DCTs run as complex FFTs of length N (hipFFT) between two HIP kernels,
-// by Makhoul's reordering: DCT-II(x)_k = 2 Re(e^{-i pi k/2N} FFT(v)_k) with
+// The DCTs run as complex FFTs of length N (hipFFT; the synthesis plans put
+// two columns in one FFT) between two HIP kernels, by Makhoul's reordering: DCT-II(x)_k = 2 Re(e^{-i pi k/2N} FFT(v)_k) with
 // v_m = x_{2m}, v_{N-1-m} = x_{2m+1}; DCT-III(X) is the inverse map,
 // v = IFFT(e^{i pi k/2N} (X_k - i X_{N-k})) (unnormalized, X_N = 0),
 // y_{2m} = Re v_m, y_{2m+1} = Re v_{N-1-m}.  The DSTs are DCTs of the
