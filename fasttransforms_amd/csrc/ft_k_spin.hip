@@ -137,7 +137,7 @@ __device__ void sweep(double2 *col, long long rs, const double2 *__restrict__ t,
 // remaining R mod KB steps and the last row follow.
 template <int DIR>
 __device__ void sweep1(double2 *col, long long rs, const double2 *__restrict__ tq, int R, double sg) {
-    constexpr int KB = 8;
+    constexpr int KB = 12;
     // step p = 1 .. R: the fresh row and the pair, clamped into range past R
     auto frow = [&](int p) {
         const int pc = p > R ? R : p;
