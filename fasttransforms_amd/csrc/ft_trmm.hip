@@ -174,8 +174,8 @@ __device__ inline double *bidiag_column(double *A, int n, int q) { return A + (4
 // chunks of CH rows (one 128-byte line) with the next chunk's loads in flight
 // during this one's arithmetic, so the division chain is the critical path.
 template <bool TRANS>
-__global__ void __launch_bounds__(64) bidiag_solve_kernel(double *A, int n, int count, const double *d,
-                                                          const double *u) {
+__global__ void __launch_bounds__(64) bidiag_solve_kernel(double *__restrict__ A, int n, int count,
+                                                          const double *__restrict__ d, const double *__restrict__ u) {
 #pragma clang fp contract(off)
     constexpr int CH = 16;
     constexpr bool ASC = TRANS;
